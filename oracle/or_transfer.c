@@ -1,0 +1,551 @@
+/*
+ * Oracle (test infrastructure): geodesic integration, polarized transfer and
+ * the image loop.  Restates src/geodesics.c, src/model_geodesics.c,
+ * src/ipolarray.c and the fast-light part of src/main.c.
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern or_geom_t OR_G;
+extern or_model_t OR_M;
+
+#define CUT_SMALL_OPTICAL_DEPTH 1e-5
+#define CUT_PREVENT_NAN 1e-80
+
+/* -------------------------------------------------------------- geodesics */
+
+/* src/geodesics.c:9-64 */
+void or_push_photon(double X[4], double Kcon[4], double dl, double Xhalf[4], double Kconhalf[4])
+{
+  double lconn[4][4][4], dKcon[4], Xh[4], Kconh[4];
+  int i, j, k;
+  or_connection(X, lconn);
+  for (k = 0; k < 4; k++) dKcon[k] = 0.;
+  for (i = 0; i < 4; i++)
+    for (j = 0; j < 4; j++)
+      for (k = 0; k < 4; k++) dKcon[k] -= 0.5 * dl * lconn[k][i][j] * Kcon[i] * Kcon[j];
+  for (k = 0; k < 4; k++) Kconh[k] = Kcon[k] + dKcon[k];
+  for (i = 0; i < 4; i++) Xh[i] = X[i] + 0.5 * dl * Kcon[i];
+  for (i = 0; i < 4; i++) Xhalf[i] = Xh[i];
+  for (i = 0; i < 4; i++) Kconhalf[i] = Kconh[i];
+  or_connection(Xh, lconn);
+  for (k = 0; k < 4; k++) dKcon[k] = 0.;
+  for (i = 0; i < 4; i++)
+    for (j = 0; j < 4; j++)
+      for (k = 0; k < 4; k++) dKcon[k] -= dl * lconn[k][i][j] * Kconh[i] * Kconh[j];
+  for (k = 0; k < 4; k++) Kcon[k] += dKcon[k];
+  for (k = 0; k < 4; k++) X[k] += dl * Kconh[k];
+}
+
+/* src/model_geodesics.c:218-254 (no thin disk) */
+int or_stop_backward(const double X[4], const double Kcon[4])
+{
+  double r, th;
+  or_bl_coord(X, &r, &th);
+  if ((r > OR_G.rmax_geo && Kcon[1] < 0.) || r < (OR_G.Rh + 0.0001)) return 1;
+  return 0;
+}
+
+/* src/model_geodesics.c:267-293 (STEP_STRICT_MIN = 0) */
+double or_stepsize(const double X[4], const double Kcon[4], double eps)
+{
+  const double SS = OR_SMALL * OR_SMALL;
+  double deh = fmin(fabs(X[1] - OR_G.cstartx[1]), 0.1);
+  double dlx1 = eps * (10 * deh) / (fabs(Kcon[1]) + SS);
+  double cut = 0.02;
+  double lx2 = OR_G.cstopx[2] - OR_G.cstartx[2];
+  double dpole = fmin(fabs(X[2] / lx2), fabs((OR_G.cstopx[2] - X[2]) / lx2));
+  double d2fac = (dpole < cut) ? dpole / 3 : fmin(cut / 3 + (dpole - cut) * 10., 1);
+  double dlx2 = eps * d2fac / (fabs(Kcon[2]) + SS);
+  double dlx3 = eps / (fabs(Kcon[3]) + SS);
+  double idlx1 = 1. / (fabs(dlx1) + SS);
+  double idlx2 = 1. / (fabs(dlx2) + SS);
+  double idlx3 = 1. / (fabs(dlx3) + SS);
+  return 1. / (idlx1 + idlx2 + idlx3);
+}
+
+/* src/model_geodesics.c:38-171 */
+int or_trace(const double Xi[4], const double Kconi[4], or_traj_t *traj, double eps,
+             int step_max, const double Xcam[4], int subring_dtheta)
+{
+  double X[4], Kcon[4], Xhalf[4], Kconhalf[4];
+  for (int mu = 0; mu < 4; mu++) {
+    X[mu] = Xi[mu];
+    Xhalf[mu] = Xi[mu];
+    Kcon[mu] = Kconi[mu];
+    Kconhalf[mu] = Kconi[mu];
+    traj[0].X[mu] = Xi[mu];
+    traj[0].Kcon[mu] = Kconi[mu];
+  }
+  int nstep = 0, nturns = 0, passed_midplane_if_zero;
+  if (Xcam[2] < 0.5) passed_midplane_if_zero = 1;
+  else if (Xcam[2] > 0.5) passed_midplane_if_zero = -1;
+  else passed_midplane_if_zero = 0;
+  while ((!or_stop_backward(X, Kcon)) && (nstep < step_max - 1)) {
+    nstep++;
+    double dl = or_stepsize(X, Kcon, eps);
+    traj[nstep].dl = dl * OR_M.L_unit * OR_HPL / (OR_ME * OR_CL * OR_CL);
+    or_push_photon(X, Kcon, -dl, Xhalf, Kconhalf);
+    for (int mu = 0; mu < 4; mu++) {
+      traj[nstep].X[mu] = X[mu];
+      traj[nstep].Kcon[mu] = Kcon[mu];
+      traj[nstep].Xhalf[mu] = Xhalf[mu];
+      traj[nstep].Kconhalf[mu] = Kconhalf[mu];
+    }
+    if (passed_midplane_if_zero != 0) {
+      if (passed_midplane_if_zero > 0) {
+        if (traj[nstep].X[2] > 0.5) passed_midplane_if_zero = 0;
+      } else {
+        if (traj[nstep].X[2] < 0.5) passed_midplane_if_zero = 0;
+      }
+    } else if (nstep > 3) {
+      if (subring_dtheta == 1) {
+        double dX2a = traj[nstep - 1].X[2] - traj[nstep - 2].X[2];
+        double dX2b = traj[nstep].X[2] - traj[nstep - 1].X[2];
+        if (dX2a * dX2b < 0) nturns++;
+      } else {
+        double ra, rb, rc, ha, hb, hc;
+        or_bl_coord(traj[nstep - 2].X, &ra, &ha);
+        or_bl_coord(traj[nstep - 1].X, &rb, &hb);
+        or_bl_coord(traj[nstep - 0].X, &rc, &hc);
+        double dz_1 = rb * cos(hb) - ra * cos(ha);
+        double dz_2 = rc * cos(hc) - rb * cos(hb);
+        if (dz_1 * dz_2 < 0) nturns++;
+      }
+    }
+    traj[nstep].nturns = nturns;
+  }
+  return nstep;
+}
+
+/* src/geometry.c:190-201 */
+static void null_normalize(double K[4], double fnorm)
+{
+  double inorm = sqrt(K[1] * K[1] + K[2] * K[2] + K[3] * K[3]);
+  K[0] = fnorm;
+  K[1] *= fnorm / inorm;
+  K[2] *= fnorm / inorm;
+  K[3] *= fnorm / inorm;
+}
+
+/* src/model_geodesics.c:178-212 and src/tetrads.c:22-33 */
+void or_init_XK(long i, long j, const or_params_t *p, const or_camera_t *cam, double X[4],
+                double Kcon[4])
+{
+  double Econ[4][4], Ecov[4][4], Kt[4];
+  if (p->old_centering) or_camera_tetrad_old(cam->Xcam, Econ, Ecov);
+  else or_camera_tetrad(cam->Xcam, Econ, Ecov);
+  double dxoff = (i + 0.5 + p->xoff - 0.01) / p->nx - 0.5;
+  double dyoff = (j + 0.5 + p->yoff) / p->ny - 0.5;
+  Kt[0] = 0.;
+  Kt[1] = (dxoff * cos(cam->rotcam_rad) - dyoff * sin(cam->rotcam_rad)) * cam->fovx;
+  Kt[2] = (dxoff * sin(cam->rotcam_rad) + dyoff * cos(cam->rotcam_rad)) * cam->fovy;
+  Kt[3] = 1.;
+  null_normalize(Kt, 1.);
+  for (int l = 0; l < 4; l++)
+    Kcon[l] = Econ[0][l] * Kt[0] + Econ[1][l] * Kt[1] + Econ[2][l] * Kt[2] + Econ[3][l] * Kt[3];
+  for (int mu = 0; mu < 4; mu++) X[mu] = cam->Xcam[mu];
+}
+
+/* -------------------------------------------------------------- transfer */
+
+/* src/ipolarray.c:607-628 */
+double or_approximate_solve(double Ii, double ji, double ki, double jf, double kf, double dl,
+                            double *tau)
+{
+  double javg = (ji + jf) / 2.;
+  double kavg = (ki + kf) / 2.;
+  double dtau = dl * kavg;
+  *tau += dtau;
+  if (dtau < 1.e-3)
+    return Ii + (javg - Ii * kavg) * dl * (1. - (dtau / 2.) * (1. - dtau / 3.));
+  double efac = exp(-dtau);
+  return Ii * efac + (javg / kavg) * (1. - efac);
+}
+
+/* src/ipolarray.c:253-289 */
+void or_push_polar(const double Xm[4], const double Km[4], double complex Ni[4][4],
+                   double complex Nm[4][4], double complex Nf[4][4], double dlam)
+{
+  double dl = dlam / (OR_M.L_unit * OR_HPL / (OR_ME * OR_CL * OR_CL));
+  double lconn[4][4][4];
+  or_connection(Xm, lconn);
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) Nf[i][j] = Ni[i][j];
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      for (int k = 0; k < 4; k++)
+        for (int l = 0; l < 4; l++)
+          Nf[i][j] += -(lconn[i][k][l] * Nm[k][j] * Km[l] + lconn[j][k][l] * Nm[i][k] * Km[l]) * dl;
+}
+
+/* src/ipolarray.c:673-694 */
+static void stokes_to_tensor(double fI, double fQ, double fU, double fV, double complex f[4][4])
+{
+  f[1][1] = (fI + fQ + 0. * I);
+  f[1][2] = (fU - I * fV);
+  f[2][1] = (fU + I * fV);
+  f[2][2] = (fI - fQ + 0. * I);
+}
+static void tensor_to_stokes(double complex f[4][4], double *fI, double *fQ, double *fU, double *fV)
+{
+  *fI = creal(f[1][1] + f[2][2]) / 2;
+  *fQ = creal(f[1][1] - f[2][2]) / 2;
+  *fU = creal(f[1][2] + f[2][1]) / 2;
+  *fV = cimag(f[2][1] - f[1][2]) / 2;
+}
+
+/* src/ipolarray.c:710-748 */
+static void coord_to_tetrad_rank2(double complex Tc[4][4], double Ecov[4][4], double complex Tt[4][4])
+{
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) Tt[i][j] = 0. + I * 0.;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      for (int k = 0; k < 4; k++)
+        for (int l = 0; l < 4; l++) Tt[i][j] += Tc[k][l] * Ecov[i][k] * Ecov[j][l];
+}
+static void tetrad_to_coord_rank2(double complex Tt[4][4], double Econ[4][4], double complex Tc[4][4])
+{
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) Tc[i][j] = 0. + I * 0.;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      for (int k = 0; k < 4; k++)
+        for (int l = 0; l < 4; l++) Tc[i][j] += Tt[k][l] * Econ[k][i] * Econ[l][j];
+}
+
+/* src/ipolarray.c:332-575 */
+int or_evolve_N(const double Xf[4], const double Kconf[4], double dlam, double complex N_coord[4][4],
+                double *tauF, int zero_emission, const or_params_t *p)
+{
+  double gcov[4][4], Ucon[4], Bcon[4], Ucov[4], Bcov[4], Ecov[4][4], Econ[4][4];
+  double complex N_tetrad[4][4];
+  double c11[11];
+  double SI0, SQ0, SU0, SV0, SI1, SQ1, SU1, SV1, SI2, SQ2, SU2, SV2, SI, SQ, SU, SV;
+  int oddflag = 0;
+  or_jar(p->emission_type, 1, Xf, Kconf, c11);
+  double jI = c11[0], jQ = c11[1], jU = c11[2], jV = c11[3];
+  double aI = c11[4], aQ = c11[5], aU = c11[6], aV = c11[7];
+  double rQ = c11[8], rU = c11[9], rV = c11[10];
+  if (zero_emission) {
+    jI = 0.;
+    jQ = 0.;
+    jU = 0.;
+    jV = 0.;
+  }
+  or_fourv(Xf, Kconf, Ucon, Ucov, Bcon, Bcov);
+  double bsq = 0.;
+  for (int mu = 0; mu < 4; mu++) bsq += Bcon[mu] * Bcov[mu];
+  if (bsq <= 0.) {
+    Bcon[0] = 0.;
+    Bcon[1] = 1.;
+    Bcon[2] = 1.;
+    Bcon[3] = 1.;
+  }
+  or_gcov(Xf, gcov);
+  oddflag |= or_plasma_tetrad(Ucon, Kconf, Bcon, gcov, Econ, Ecov);
+  coord_to_tetrad_rank2(N_coord, Ecov, N_tetrad);
+  tensor_to_stokes(N_tetrad, &SI0, &SQ0, &SU0, &SV0);
+
+  double x = dlam * 0.5;
+  double rho2 = rQ * rQ + rU * rU + rV * rV;
+  if (rho2 > CUT_PREVENT_NAN) {
+    double rho = sqrt(rho2);
+    double rdS = rQ * SQ0 + rU * SU0 + rV * SV0;
+    double c = cos(rho * x), s = sin(rho * x), sh = sin(0.5 * rho * x);
+    SI1 = SI0;
+    SQ1 = SQ0 * c + 2 * rQ * rdS / rho2 * sh * sh + (rU * SV0 - rV * SU0) / rho * s;
+    SU1 = SU0 * c + 2 * rU * rdS / rho2 * sh * sh + (rV * SQ0 - rQ * SV0) / rho * s;
+    SV1 = SV0 * c + 2 * rV * rdS / rho2 * sh * sh + (rQ * SU0 - rU * SQ0) / rho * s;
+  } else {
+    SI1 = SI0;
+    SQ1 = SQ0 + (-rV * SU0 + rU * SV0) * x;
+    SU1 = SU0 + (rV * SQ0 - rQ * SV0) * x;
+    SV1 = SV0 + (-rU * SQ0 + rQ * SU0) * x;
+  }
+
+  x = dlam;
+  double aP2 = aQ * aQ + aU * aU + aV * aV;
+  if (aP2 > CUT_PREVENT_NAN) {
+    double aP = sqrt(aP2);
+    double tauP = aP * x;
+    double tauI = aI * x;
+    double ads0 = aQ * SQ1 + aU * SU1 + aV * SV1;
+    double adj = aQ * jQ + aU * jU + aV * jV;
+    double efacm = exp(-tauI + tauP);
+    double efacp = exp(-tauI - tauP);
+    double efac = exp(-tauI);
+    SI2 = efacm * (SI1 / 2. - ads0 / (2. * aP)) + efacp * (SI1 / 2. + ads0 / (2. * aP));
+    SQ2 = efacm * (-SI1 * aQ * aP + ads0 * aQ) / (2. * aP2)
+        + efacp * (SI1 * aQ * aP + ads0 * aQ) / (2. * aP2) + efac * (SQ1 - ads0 * aQ / (aP2));
+    SU2 = efacm * (-SI1 * aU * aP + ads0 * aU) / (2. * aP2)
+        + efacp * (SI1 * aU * aP + ads0 * aU) / (2. * aP2) + efac * (SU1 - ads0 * aU / (aP2));
+    SV2 = efacm * (-SI1 * aV * aP + ads0 * aV) / (2. * aP2)
+        + efacp * (SI1 * aV * aP + ads0 * aV) / (2. * aP2) + efac * (SV1 - ads0 * aV / (aP2));
+    double afacm = 1. - efacm, afac = 1. - efac, afacp = 1. - efacp;
+    if (tauI - tauP <= CUT_SMALL_OPTICAL_DEPTH) {
+      double e = tauI - tauP;
+      afacm = e * (1. - (e / 2.) * (1. - e / 3.));
+      if (tauI <= CUT_SMALL_OPTICAL_DEPTH) {
+        e = tauI;
+        afac = e * (1. - (e / 2.) * (1. - e / 3.));
+        if (tauI + tauP <= CUT_SMALL_OPTICAL_DEPTH) {
+          e = tauI + tauP;
+          afacp = e * (1. - (e / 2.) * (1. - e / 3.));
+        }
+      }
+    }
+    SQ2 += afac * (jQ / aI - aQ * adj / (aI * aP2));
+    SU2 += afac * (jU / aI - aU * adj / (aI * aP2));
+    SV2 += afac * (jV / aI - aV * adj / (aI * aP2));
+    SI2 += afacm * (aP * jI - adj) / (2. * aP * (aI - aP));
+    SQ2 += afacm * aQ * (-aP * jI + adj) / (2. * aP2 * (aI - aP));
+    SU2 += afacm * aU * (-aP * jI + adj) / (2. * aP2 * (aI - aP));
+    SV2 += afacm * aV * (-aP * jI + adj) / (2. * aP2 * (aI - aP));
+    SI2 += afacp * (aP * jI + adj) / (2. * aP * (aI + aP));
+    SQ2 += afacp * aQ * (aP * jI + adj) / (2. * aP2 * (aI + aP));
+    SU2 += afacp * aU * (aP * jI + adj) / (2. * aP2 * (aI + aP));
+    SV2 += afacp * aV * (aP * jI + adj) / (2. * aP2 * (aI + aP));
+  } else {
+    double tau_fake = 0;
+    SI2 = or_approximate_solve(SI1, jI, aI, jI, aI, x, &tau_fake);
+    SQ2 = or_approximate_solve(SQ1, jQ, aI, jQ, aI, x, &tau_fake);
+    SU2 = or_approximate_solve(SU1, jU, aI, jU, aI, x, &tau_fake);
+    SV2 = or_approximate_solve(SV1, jV, aI, jV, aI, x, &tau_fake);
+  }
+
+  x = dlam * 0.5;
+  rho2 = rQ * rQ + rU * rU + rV * rV;
+  if (rho2 > CUT_PREVENT_NAN) {
+    double rdS = rQ * SQ2 + rU * SU2 + rV * SV2;
+    double rho = sqrt(rho2);
+    double c = cos(rho * x), s = sin(rho * x), sh = sin(0.5 * rho * x);
+    SI = SI2;
+    SQ = SQ2 * c + 2 * rQ * rdS / rho2 * sh * sh + (rU * SV2 - rV * SU2) / rho * s;
+    SU = SU2 * c + 2 * rU * rdS / rho2 * sh * sh + (rV * SQ2 - rQ * SV2) / rho * s;
+    SV = SV2 * c + 2 * rV * rdS / rho2 * sh * sh + (rQ * SU2 - rU * SQ2) / rho * s;
+  } else {
+    SI = SI2;
+    SQ = SQ2 + (-rV * SU2 + rU * SV2) * x;
+    SU = SU2 + (rV * SQ2 - rQ * SV2) * x;
+    SV = SV2 + (-rU * SQ2 + rQ * SU2) * x;
+  }
+  *tauF += dlam * fabs(rV);
+  if (p->stokes_floors) {
+    if (SI < 0) {
+      SI = 0;
+      SQ = 0;
+      SU = 0;
+      SV = 0;
+    } else {
+      double pol_frac = sqrt(SQ * SQ + SU * SU + SV * SV) / SI;
+      if (pol_frac > 1.) {
+        SQ /= pol_frac;
+        SU /= pol_frac;
+        SV /= pol_frac;
+      }
+    }
+  }
+  stokes_to_tensor(SI, SQ, SU, SV, N_tetrad);
+  tetrad_to_coord_rank2(N_tetrad, Econ, N_coord);
+  if (*tauF > 1.e100 || *tauF < -1.e100 || isnan(*tauF)) oddflag |= 2;
+  if (isnan(creal(N_tetrad[0][0])) || isnan(creal(N_coord[0][0]))) oddflag |= 4;
+  if (isnan(Ucov[0])) oddflag |= 8;
+  return oddflag;
+}
+
+/* src/ipolarray.c:578-598 */
+void or_project_N(const double X[4], double complex N_coord[4][4], double *Stokes_I,
+                  double *Stokes_Q, double *Stokes_U, double *Stokes_V, double rotcam)
+{
+  double complex N_tetrad[4][4];
+  double Econ[4][4], Ecov[4][4], Q, U;
+  or_camera_tetrad(X, Econ, Ecov);
+  coord_to_tetrad_rank2(N_coord, Ecov, N_tetrad);
+  tensor_to_stokes(N_tetrad, Stokes_I, &Q, &U, Stokes_V);
+  rotcam *= -2.;
+  *Stokes_Q = Q * cos(rotcam) - U * sin(rotcam);
+  *Stokes_U = Q * sin(rotcam) + U * cos(rotcam);
+}
+
+/* src/model_radiation.c:119-123 */
+static void get_jkinv(const double X[4], const double K[4], double *j, double *k, int dist)
+{
+  double c[11];
+  or_jar(dist, 0, X, K, c);
+  *j = c[0];
+  *k = c[4];
+}
+
+/* src/ipolarray.c:59-247 (non-debug build: flags are OR-ed) */
+int or_integrate_emission(or_traj_t *traj, int nsteps, double *Intensity, double *Tau,
+                          double *tauF, double complex N_coord[4][4], const or_params_t *p)
+{
+  *tauF = 0.;
+  *Intensity = 0.;
+  *Tau = 0.;
+  double js = 0., ks;
+  int oddflag = 0;
+  get_jkinv(traj[nsteps].X, traj[nsteps].Kcon, &js, &ks, p->emission_type);
+  for (int nstep = nsteps; nstep > 0; --nstep) {
+    int sflag = 0;
+    or_traj_t ti = traj[nstep];
+    or_traj_t tf = traj[nstep - 1];
+    if (!p->only_unpolarized) {
+      double complex Nh[4][4];
+      or_push_polar(ti.X, ti.Kcon, N_coord, N_coord, Nh, 0.5 * ti.dl);
+      or_push_polar(ti.Xhalf, ti.Kconhalf, N_coord, Nh, N_coord, ti.dl);
+    }
+    if (or_radiating_region(tf.X)) {
+      int zero_emission = 0;
+      if (p->target_nturns >= 0 && ti.nturns != p->target_nturns) zero_emission = 1;
+      if (p->counterjet == 1) {
+        if (tf.X[2] < (OR_G.cstopx[2] - OR_G.cstartx[2]) / 2) zero_emission = 1;
+      } else if (p->counterjet == 2) {
+        if (tf.X[2] > (OR_G.cstopx[2] - OR_G.cstartx[2]) / 2) zero_emission = 1;
+      }
+      double ji = js, ki = ks, jf, kf;
+      get_jkinv(tf.X, tf.Kcon, &jf, &kf, p->emission_type);
+      js = jf;
+      ks = kf;
+      if (zero_emission) *Intensity = or_approximate_solve(*Intensity, 0, ki, 0, kf, ti.dl, Tau);
+      else *Intensity = or_approximate_solve(*Intensity, ji, ki, jf, kf, ti.dl, Tau);
+      if (!p->only_unpolarized)
+        sflag |= or_evolve_N(tf.X, tf.Kcon, ti.dl, N_coord, tauF, zero_emission, p);
+    }
+    oddflag |= sflag;
+  }
+  return oddflag;
+}
+
+/* -------------------------------------------------------------- image */
+
+/* src/main.c:125-174 */
+int or_setup_camera(const or_params_t *p, or_camera_t *cam)
+{
+  memset(cam, 0, sizeof(*cam));
+  cam->freq = p->freqcgs * OR_HPL / (OR_ME * OR_CL * OR_CL);
+  if (cam->freq == 0) return -1;
+  cam->rotcam_rad = p->rotcam * M_PI / 180.;
+  or_native_coord(p->rcam, p->thetacam, p->phicam, cam->Xcam);
+  cam->dsource_cm = p->dsource * OR_PC;
+  double Dsource = cam->dsource_cm;
+  double L_unit = OR_M.L_unit;
+  double fov_to_d = Dsource / L_unit / OR_MUAS_PER_RAD;
+  double fx = p->fovx_dsource, fy = p->fovy_dsource;
+  if (fx != 0.0) {
+  } else if (p->dx != 0.0) {
+    fx = p->dx / fov_to_d;
+    fy = p->dy / fov_to_d;
+  } else {
+    fx = 160.;
+    fy = 160.;
+  }
+  cam->DX = fx * fov_to_d;
+  cam->DY = fy * fov_to_d;
+  cam->fovx = cam->DX / p->rcam;
+  cam->fovy = cam->DY / p->rcam;
+  cam->scale = (cam->DX * L_unit / p->nx) * (cam->DY * L_unit / p->ny) / (Dsource * Dsource) / OR_JY;
+  return 0;
+}
+
+/* One pixel: src/main.c:822-883 (get_pixel) without trace output.
+ * out7 = Intensity, Is, Qs, Us, Vs, Tau, tauF (before the nu^3 scaling). */
+static int pixel(long i, long j, const or_params_t *p, const or_camera_t *cam, or_traj_t *traj,
+                 double out7[7], int *nstep_out, int *flag_out)
+{
+  double X[4] = {0.}, Kcon[4] = {0.};
+  double complex N[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) N[a][b] = 0.;
+  or_init_XK(i, j, p, cam, X, Kcon);
+  memset(traj, 0, sizeof(or_traj_t)); /* traj[0]; later entries are written before use */
+  for (int mu = 0; mu < 4; mu++) Kcon[mu] *= cam->freq;
+  int nstep = or_trace(X, Kcon, traj, p->eps, p->maxnstep, cam->Xcam, p->subring_dtheta);
+  *nstep_out = nstep;
+  if (nstep >= p->maxnstep - 1) return -10;
+  double Intensity, Tau, tauF, Is = 0, Qs = 0, Us = 0, Vs = 0;
+  int oddflag = or_integrate_emission(traj, nstep, &Intensity, &Tau, &tauF, N, p);
+  if (!p->only_unpolarized) or_project_N(X, N, &Is, &Qs, &Us, &Vs, cam->rotcam_rad);
+  *flag_out = oddflag;
+  out7[0] = Intensity;
+  out7[1] = Is;
+  out7[2] = Qs;
+  out7[3] = Us;
+  out7[4] = Vs;
+  out7[5] = Tau;
+  out7[6] = tauF;
+  return 0;
+}
+
+/* Pixels listed as (i, j) pairs; returns 0 or -10 (maxnstep exceeded). */
+int or_render_pixels(const or_params_t *p, const or_camera_t *cam, const int *pix, int npix,
+                     double *out7, int *nsteps, int *flags, int nthreads)
+{
+  int status = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    or_traj_t *traj = malloc(sizeof(or_traj_t) * (size_t)p->maxnstep);
+#pragma omp for schedule(dynamic, 1)
+    for (int n = 0; n < npix; n++) {
+      int rc = pixel(pix[2 * n], pix[2 * n + 1], p, cam, traj, out7 + 7 * (size_t)n, nsteps + n,
+                     flags + n);
+      if (rc) {
+#pragma omp critical
+        status = rc;
+      }
+    }
+    free(traj);
+  }
+  return status;
+}
+
+/* src/main.c:536-615 (fast light, no refinement) and save_pixel :885-910 */
+int or_render(const or_params_t *p, or_camera_t *cam, double *image, double *imageS, double *taus,
+              int *nsteps, int *flags, int nthreads)
+{
+  int rc = or_setup_camera(p, cam);
+  if (rc) return rc;
+  size_t nx = p->nx, ny = p->ny, npix = nx * ny;
+  int *pix = malloc(sizeof(int) * 2 * npix);
+  double *out7 = malloc(sizeof(double) * 7 * npix);
+  for (size_t i = 0; i < nx; i++)
+    for (size_t j = 0; j < ny; j++) {
+      pix[2 * (i * ny + j)] = (int)i;
+      pix[2 * (i * ny + j) + 1] = (int)j;
+    }
+  rc = or_render_pixels(p, cam, pix, (int)npix, out7, nsteps, flags, nthreads);
+  if (rc == 0) {
+    double f3 = pow(p->freqcgs, 3);
+    for (size_t n = 0; n < npix; n++) {
+      double *o = out7 + 7 * n;
+      image[n] = o[0] * f3;
+      taus[n] = o[5];
+      if (!p->only_unpolarized) {
+        imageS[n * 5 + 0] = o[1] * f3;
+        if (p->qu_conv == 0) {
+          imageS[n * 5 + 1] = -o[2] * f3;
+          imageS[n * 5 + 2] = -o[3] * f3;
+        } else {
+          imageS[n * 5 + 1] = o[2] * f3;
+          imageS[n * 5 + 2] = o[3] * f3;
+        }
+        imageS[n * 5 + 3] = o[4] * f3;
+        imageS[n * 5 + 4] = o[6];
+        if (isnan(imageS[n * 5])) rc = -1;
+      }
+    }
+  }
+  free(pix);
+  free(out7);
+  return rc;
+}
