@@ -1,0 +1,153 @@
+/*
+ * ipgpu: C-ABI of the MI355X (gfx950) per-pixel polarized ray pipeline.
+ *
+ * This library replaces the OpenMP pixel loop of the reference ipole
+ * (src/main.c:536-615, get_pixel src/main.c:822-883, save_pixel :885-910)
+ * with GPU kernels, behind plain C entry points: pointers, sizes and POD
+ * structs, no torch or HIP types.  One context drives one GPU and one HIP
+ * stream; calls on one context come from one host thread.
+ *
+ * Status codes: 0 on success, negative on failure (never exit()); the
+ * message is available from ipgpu_last_error().  IPGPU_ERR_MAXNSTEP is the
+ * condition on which the reference calls exit(-10) (src/main.c:841-845).
+ */
+#ifndef IPGPU_H
+#define IPGPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IPGPU_OK 0
+#define IPGPU_ERR_ARG (-1)
+#define IPGPU_ERR_HIP (-2)
+#define IPGPU_ERR_NAN (-3)
+#define IPGPU_ERR_NOMEM (-4)
+#define IPGPU_ERR_STATE (-5)
+#define IPGPU_ERR_MAXNSTEP (-10)
+
+/* coordinate systems, same codes as the reference (src/coordinates.h:450-464) */
+#define IPGPU_METRIC_MKS 0
+#define IPGPU_METRIC_FMKS 2
+#define IPGPU_METRIC_EMINKOWSKI 5
+
+/* fluid models */
+#define IPGPU_MODEL_IHARM 1   /* model/iharm/model.c: ghosted dump fields */
+#define IPGPU_MODEL_SPHERE 2  /* model/sphere/model.c: isothermal sphere */
+
+/* number of ghosted per-cell fields the iharm model reads on the path:
+ * U1 U2 U3 B1 B2 B3 (primitives) ne thetae b sigma */
+#define IPGPU_NFIELDS 10
+
+/* Coordinate globals of src/coordinates.h:25-33 as the model sets them
+ * (model/iharm/model.c:979-1065, :326). */
+typedef struct ipgpu_geom {
+  int metric;
+  int N1, N2, N3;
+  double a, hslope, poly_xt, poly_alpha, mks_smooth, poly_norm;
+  double startx[4], stopx[4], dx[4], cstartx[4], cstopx[4];
+  double Rin, Rout, Rh, rmin_geo, rmax_geo;
+} ipgpu_geom;
+
+/* Model constants (model/iharm/model.c:49-94, model/sphere/model.c:20-96). */
+typedef struct ipgpu_model {
+  int kind;
+  double L_unit;                         /* cm per GM/c^2 */
+  double th_beg, sigma_cut, sigma_cut_high;
+  double sph_R0, sph_Ne0, sph_Thetae0, sph_B0;
+} ipgpu_model;
+
+/* Units and electron model for the on-device dump derivation
+ * (model/iharm/model.c:595-608, :610-717; ELECTRONS = 2). */
+typedef struct ipgpu_units {
+  double M_unit, MBH_solar;
+  double trat_small, trat_large, beta_crit;
+  double gam, game, gamp;
+} ipgpu_units;
+
+/* One image: the Params fields get_pixel reads (src/par.h:14-73) after the
+ * camera set-up of src/main.c:126-174. */
+typedef struct ipgpu_image {
+  int nx, ny;
+  double Xcam[4];            /* native_coord(rcam, thetacam, phicam) */
+  double fovx, fovy;         /* DX / rcam, DY / rcam */
+  double freq;               /* freqcgs * HPL / (ME CL^2) */
+  double freqcgs;
+  double rotcam;             /* radians */
+  double xoff, yoff;
+  double eps;
+  int maxnstep;
+  int old_centering;
+  int only_unpolarized;
+  int qu_conv;
+  int stokes_floors;
+  int target_nturns;
+  int subring_dtheta;
+  int counterjet;
+  int emission_type;         /* 4 (Dexter thermal, default) or 1 (Pandya thermal) */
+} ipgpu_image;
+
+typedef struct ipgpu_ctx ipgpu_ctx;
+
+/* Context on GPU `device`; replaces nothing in the reference (set-up). */
+int ipgpu_create(ipgpu_ctx **ctx, int device);
+void ipgpu_destroy(ipgpu_ctx *ctx);
+const char *ipgpu_last_error(const ipgpu_ctx *ctx);
+
+/* Geometry and model constants; mirror the globals init_model leaves behind
+ * (model/iharm/model.c:295-389). */
+int ipgpu_set_geometry(ipgpu_ctx *ctx, const ipgpu_geom *geom);
+int ipgpu_set_model(ipgpu_ctx *ctx, const ipgpu_model *model);
+
+/* Upload the model's ghosted (N1+2)(N2+2)(N3+2) arrays, x3 fastest, as the
+ * iharm plugin holds them after load (data[0]->p[U1..B3], ne, thetae, b,
+ * sigma: model/iharm/model.c:104-114, utils.c:41-98).  The caller keeps
+ * ownership; this is a copy into the device layout. */
+int ipgpu_upload_fields(ipgpu_ctx *ctx, const double *const fields[IPGPU_NFIELDS],
+                        size_t ncells_ghosted);
+
+/* Upload dump primitives [N1][N2][N3][nprim] (rho u U1 U2 U3 B1 B2 B3 ...)
+ * and derive the fields on the device: b from the primitives
+ * (model/iharm/model.c:1954-2010), ghost zones (:1314-1370), n_e, Theta_e,
+ * sigma (:610-717). */
+int ipgpu_upload_prims(ipgpu_ctx *ctx, const double *prims, int nprim, const ipgpu_units *units);
+
+/* Copy the derived ghosted fields back (11 arrays: the 10 above and beta),
+ * for checking the derivation. */
+int ipgpu_download_fields(ipgpu_ctx *ctx, double *out11, size_t ncells_ghosted);
+
+/* Render one image: the fast-light pixel loop of src/main.c:536-615.
+ * Outputs use the reference layout (i*ny + j; imageS has 5 entries per
+ * pixel: I Q U V tauF), already multiplied by freqcgs^3 as save_pixel does.
+ * Any output pointer may be NULL: results then stay on the device and can
+ * be fetched with ipgpu_fetch().  nstep/flag are per pixel (nstep of the
+ * trajectory, oddflag bits of integrate_emission).  imageS is ignored with
+ * only_unpolarized. */
+int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *imageS,
+                 double *taus, int *nstep, int *flag);
+int ipgpu_fetch(ipgpu_ctx *ctx, double *image, double *imageS, double *taus, int *nstep,
+                int *flag);
+
+/* Pixel (i, j) that exceeded maxnstep in the last render (IPGPU_ERR_MAXNSTEP). */
+int ipgpu_bad_pixel(const ipgpu_ctx *ctx, int *i, int *j);
+
+/* Device time of the last render's kernels, from HIP events on the
+ * context's stream: geodesic pass, transfer pass, total. */
+int ipgpu_kernel_times(const ipgpu_ctx *ctx, double *ms_geodesic, double *ms_transfer,
+                       double *ms_total);
+
+/* The context's HIP stream (a hipStream_t), for callers that time or order
+ * their own work against it. */
+void *ipgpu_stream(ipgpu_ctx *ctx);
+
+/* Host helpers of the camera set-up: native_coord/root_find
+ * (src/coordinates.c:375-442). */
+int ipgpu_native_coord(const ipgpu_geom *geom, double r, double th_deg, double phi_deg,
+                       double X[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1092 @@
+/*
+ * Device physics of the per-pixel ray pipeline (gfx950, FP64).
+ *
+ * Everything a lane needs to trace one ray and integrate polarized transfer
+ * along it.  The formulation is built for a GPU lane, not translated from the
+ * reference loops:
+ *   - the metric of KS/MKS/FMKS/eMinkowski is evaluated once per point with
+ *     its exact X1/X2 derivatives (chain rule through r(X1), theta(X1,X2)) and
+ *     its closed-form inverse; the reference instead differences 8 metric
+ *     evaluations (DEL = 1e-7) and inverts by LU (src/geometry.c:93-144).
+ *     The two agree to the reference's own finite-difference round-off
+ *     (~1e-9 relative in Gamma).  t and phi derivatives vanish identically
+ *     for these stationary, axisymmetric metrics and are never formed.
+ *   - Gamma is never materialized: the geodesic needs Gamma^i_jk K^j K^k and
+ *     parallel transport needs M^i_k = Gamma^i_kl K^l; both are contracted
+ *     directly from dg/dX1, dg/dX2 and g^-1.
+ *   - the coherency tensor N is Hermitian and is kept as its upper triangle
+ *     (4 real diagonal + 6 complex entries, 16 doubles); the reference keeps
+ *     the full complex 4x4 (src/ipolarray.c:253-289, :710-748).
+ *   - the Stokes update in the plasma frame only touches the [1..2][1..2]
+ *     tetrad block (src/ipolarray.c:673-682); it is applied to N as
+ *     N += Econ^T dT Econ instead of a full rank-2 round trip.
+ */
+#pragma once
+
+#include <math.h>
+#include "ipgpu.h"
+#include "ipg_bessel_coeffs.h"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define IPG_HD __host__ __device__ inline
+#else
+#define IPG_HD inline
+#endif
+
+namespace ipg {
+
+/* constants of src/constants.h */
+constexpr double EE = 4.80320680e-10;
+constexpr double CL = 2.99792458e10;
+constexpr double ME = 9.1093826e-28;
+constexpr double MP = 1.67262171e-24;
+constexpr double HPL = 6.6260693e-27;
+constexpr double SMALL = 1.e-40;
+constexpr double PI = 3.14159265358979323846;
+
+/* packed index of a symmetric 4x4 (i <= j): 00 01 02 03 11 12 13 22 23 33 */
+IPG_HD constexpr int sy(int i, int j)
+{
+  return (i <= j) ? (i * 4 - (i * (i - 1)) / 2 + (j - i)) : (j * 4 - (j * (j - 1)) / 2 + (i - j));
+}
+
+IPG_HD double ipow_d(double x, int n)
+{
+  double r = 1.0, b = x;
+  unsigned u = (n < 0) ? (unsigned)(-n) : (unsigned)n;
+  while (u) {
+    if (u & 1u) r *= b;
+    b *= b;
+    u >>= 1;
+  }
+  return (n < 0) ? 1.0 / r : r;
+}
+
+/* x^alpha for FMKS's poly_alpha; integer-valued exponents (iharm3d uses 14)
+ * by squaring, which is also defined for negative x as pow() is there. */
+IPG_HD double powa(double x, double alpha)
+{
+  double ai = floor(alpha);
+  if (ai == alpha && fabs(alpha) < 64.0) return ipow_d(x, (int)ai);
+  return pow(x, alpha);
+}
+
+/* ------------------------------------------------------------------ metric */
+
+struct Metric {
+  double r, th;
+  double g[10];   /* g_{mu nu} */
+  double gc[10];  /* g^{mu nu} */
+  double d1[10];  /* d g_{mu nu} / dX1 */
+  double d2[10];  /* d g_{mu nu} / dX2 */
+  double sqrtmg;  /* sqrt(|det g|) */
+};
+
+struct Theta {
+  double th, t1, t2, t11, t12, t22;
+};
+
+/* theta(X1, X2) and derivatives for MKS/FMKS: src/coordinates.c:69-81
+ * (values) and :278-305 (first derivatives; the second derivatives follow
+ * from the same closed form). */
+IPG_HD void theta_mks(const ipgpu_geom &G, double X1, double X2, Theta &T)
+{
+  double s2p, c2p;
+  double arg = 2. * PI * X2;
+  s2p = sin(arg);
+  c2p = cos(arg);
+  double hs = G.hslope;
+  double thG = PI * X2 + ((1. - hs) / 2.) * s2p;
+  double dG = PI + (1. - hs) * PI * c2p;
+  double ddG = -2. * PI * PI * (1. - hs) * s2p;
+  if (G.metric == IPGPU_METRIC_MKS) {
+    T.th = thG;
+    T.t1 = 0.;
+    T.t2 = dG;
+    T.t11 = 0.;
+    T.t12 = 0.;
+    T.t22 = ddG;
+    return;
+  }
+  double y = 2. * X2 - 1.;
+  double al = G.poly_alpha, xt = G.poly_xt, pn = G.poly_norm;
+  double P = y / xt;
+  double Pa1 = powa(P, al - 1.);
+  double Pa = Pa1 * P;
+  double thJ = pn * y * (1. + Pa / (al + 1.)) + 0.5 * PI;
+  double dJ = 2. * pn * (1. + Pa);
+  double ddJ = 4. * pn * al * Pa1 / xt;
+  double E = exp(G.mks_smooth * (G.startx[1] - X1));
+  double ms = G.mks_smooth;
+  T.th = thG + E * (thJ - thG);
+  T.t1 = -ms * E * (thJ - thG);
+  T.t2 = dG + E * (dJ - dG);
+  T.t11 = ms * ms * E * (thJ - thG);
+  T.t12 = -ms * E * (dJ - dG);
+  T.t22 = ddG + E * (ddJ - ddG);
+}
+
+/* r, theta only (src/coordinates.c:46-97) */
+IPG_HD void bl_coord(const ipgpu_geom &G, double X1, double X2, double &r, double &th)
+{
+  r = exp(X1);
+  if (G.metric == IPGPU_METRIC_EMINKOWSKI) {
+    th = X2;
+    return;
+  }
+  double thG = PI * X2 + ((1. - G.hslope) / 2.) * sin(2. * PI * X2);
+  if (G.metric == IPGPU_METRIC_MKS) {
+    th = thG;
+    return;
+  }
+  double y = 2. * X2 - 1.;
+  double thJ = G.poly_norm * y * (1. + powa(y / G.poly_xt, G.poly_alpha) / (G.poly_alpha + 1.))
+      + 0.5 * PI;
+  th = thG + exp(G.mks_smooth * (G.startx[1] - X1)) * (thJ - thG);
+}
+
+/* Metric, inverse and X1/X2 derivatives at (X1, X2).  Same tensor as
+ * gcov_func (src/coordinates.c:175-213, gcov_ks :216-239, set_dxdX
+ * :263-348): g_MKS = J^T g_KS J with J = dx^KS/dX. */
+IPG_HD void metric(const ipgpu_geom &G, double X1, double X2, Metric &m)
+{
+  double r = exp(X1);
+  m.r = r;
+  if (G.metric == IPGPU_METRIC_EMINKOWSKI) {
+    double th = X2, s = sin(th), c = cos(th);
+    double r2 = r * r;
+    m.th = th;
+    for (int n = 0; n < 10; n++) {
+      m.g[n] = 0.;
+      m.gc[n] = 0.;
+      m.d1[n] = 0.;
+      m.d2[n] = 0.;
+    }
+    m.g[0] = -1.;
+    m.g[4] = r2;
+    m.g[7] = r2;
+    m.g[9] = r2 * s * s;
+    m.gc[0] = -1.;
+    m.gc[4] = 1. / r2;
+    m.gc[7] = 1. / r2;
+    m.gc[9] = 1. / (r2 * s * s);
+    m.d1[4] = 2. * r2;
+    m.d1[7] = 2. * r2;
+    m.d1[9] = 2. * r2 * s * s;
+    m.d2[9] = 2. * r2 * s * c;
+    m.sqrtmg = r2 * r * fabs(s);
+    return;
+  }
+  Theta T;
+  theta_mks(G, X1, X2, T);
+  m.th = T.th;
+  double a = G.a, a2 = a * a;
+  double s = sin(T.th), c = cos(T.th);
+  double s2 = s * s, s2t = 2. * s * c;
+  double Sig = r * r + a2 * c * c;
+  double Sr = 2. * r, St = -2. * a2 * c * s;
+  double iS = 1. / Sig;
+  double q = 2. * r * iS;
+  double qr = (2. - q * Sr) * iS;
+  double qt = -q * St * iS;
+  double opq = 1. + q;
+  /* KS components and (r, theta) partials */
+  double k00 = -1. + q, k01 = q, k03 = -a * s2 * q, k11 = opq, k13 = -a * s2 * opq, k22 = Sig;
+  double k33 = s2 * (Sig + a2 * s2 * opq);
+  double k03r = -a * s2 * qr, k03t = -a * (s2t * q + s2 * qt);
+  double k13r = -a * s2 * qr, k13t = -a * (s2t * opq + s2 * qt);
+  double k33r = s2 * Sr + a2 * s2 * s2 * qr;
+  double k33t = s2t * Sig + s2 * St + a2 * (2. * s2 * s2t * opq + s2 * s2 * qt);
+  /* chain rule: d/dX1 = r d/dr + t1 d/dth, d/dX2 = t2 d/dth */
+  double t1 = T.t1, t2 = T.t2;
+  double k00_1 = qr * r + qt * t1, k00_2 = qt * t2;
+  double k03_1 = k03r * r + k03t * t1, k03_2 = k03t * t2;
+  double k13_1 = k13r * r + k13t * t1, k13_2 = k13t * t2;
+  double k22_1 = Sr * r + St * t1, k22_2 = St * t2;
+  double k33_1 = k33r * r + k33t * t1, k33_2 = k33t * t2;
+  /* MKS components: J11 = r, J21 = t1, J22 = t2 */
+  m.g[0] = k00;
+  m.g[1] = r * k01;
+  m.g[2] = 0.;
+  m.g[3] = k03;
+  m.g[4] = r * r * k11 + t1 * t1 * k22;
+  m.g[5] = t1 * t2 * k22;
+  m.g[6] = r * k13;
+  m.g[7] = t2 * t2 * k22;
+  m.g[8] = 0.;
+  m.g[9] = k33;
+  /* X1 derivatives: dr/dX1 = r, dt1/dX1 = t11, dt2/dX1 = t12 */
+  m.d1[0] = k00_1;
+  m.d1[1] = r * k01 + r * k00_1;
+  m.d1[2] = 0.;
+  m.d1[3] = k03_1;
+  m.d1[4] = 2. * r * r * k11 + r * r * k00_1 + 2. * t1 * T.t11 * k22 + t1 * t1 * k22_1;
+  m.d1[5] = (T.t11 * t2 + t1 * T.t12) * k22 + t1 * t2 * k22_1;
+  m.d1[6] = r * k13 + r * k13_1;
+  m.d1[7] = 2. * t2 * T.t12 * k22 + t2 * t2 * k22_1;
+  m.d1[8] = 0.;
+  m.d1[9] = k33_1;
+  /* X2 derivatives: dr/dX2 = 0, dt1/dX2 = t12, dt2/dX2 = t22 */
+  m.d2[0] = k00_2;
+  m.d2[1] = r * k00_2;
+  m.d2[2] = 0.;
+  m.d2[3] = k03_2;
+  m.d2[4] = r * r * k00_2 + 2. * t1 * T.t12 * k22 + t1 * t1 * k22_2;
+  m.d2[5] = (T.t12 * t2 + t1 * T.t22) * k22 + t1 * t2 * k22_2;
+  m.d2[6] = r * k13_2;
+  m.d2[7] = 2. * t2 * T.t22 * k22 + t2 * t2 * k22_2;
+  m.d2[8] = 0.;
+  m.d2[9] = k33_2;
+  /* inverse: KS contravariant metric, then g^MKS = Jinv g^KS Jinv^T with
+   * Jinv11 = 1/r, Jinv21 = -t1/(r t2), Jinv22 = 1/t2 */
+  double Del = r * r - 2. * r + a2;
+  double c00 = -opq, c01 = q, c11 = Del * iS, c13 = a * iS, c22 = iS, c33 = iS / s2;
+  double ir = 1. / r, u = -t1 / (r * t2), w = 1. / t2;
+  m.gc[0] = c00;
+  m.gc[1] = ir * c01;
+  m.gc[2] = u * c01;
+  m.gc[3] = 0.;
+  m.gc[4] = ir * ir * c11;
+  m.gc[5] = ir * u * c11;
+  m.gc[6] = ir * c13;
+  m.gc[7] = u * u * c11 + w * w * c22;
+  m.gc[8] = u * c13;
+  m.gc[9] = c33;
+  m.sqrtmg = Sig * fabs(s) * r * fabs(t2);
+}
+
+IPG_HD void lower(const double g[10], const double u[4], double l[4])
+{
+  l[0] = g[0] * u[0] + g[1] * u[1] + g[2] * u[2] + g[3] * u[3];
+  l[1] = g[1] * u[0] + g[4] * u[1] + g[5] * u[2] + g[6] * u[3];
+  l[2] = g[2] * u[0] + g[5] * u[1] + g[7] * u[2] + g[8] * u[3];
+  l[3] = g[3] * u[0] + g[6] * u[1] + g[8] * u[2] + g[9] * u[3];
+}
+
+IPG_HD double dot4(const double a[4], const double b[4])
+{
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+}
+
+/* acc^i = Gamma^i_jk K^j K^k (the contraction push_photon forms,
+ * src/geodesics.c:22-26) from dg/dX1, dg/dX2. */
+IPG_HD void geodesic_accel(const Metric &m, const double K[4], double acc[4])
+{
+  double v1[4], v2[4];
+  lower(m.d1, K, v1);
+  lower(m.d2, K, v2);
+  double al[4];
+  for (int l = 0; l < 4; l++) al[l] = K[1] * v1[l] + K[2] * v2[l];
+  al[1] -= 0.5 * dot4(v1, K);
+  al[2] -= 0.5 * dot4(v2, K);
+  lower(m.gc, al, acc);
+}
+
+/* M^i_k = Gamma^i_kl K^l, the matrix parallel transport of N applies
+ * (src/ipolarray.c:281-287). */
+IPG_HD void transport_matrix(const Metric &m, const double K[4], double M[4][4])
+{
+  double v1[4], v2[4];
+  lower(m.d1, K, v1);
+  lower(m.d2, K, v2);
+  double Ml[4][4];
+  for (int l = 0; l < 4; l++)
+    for (int k = 0; k < 4; k++) {
+      double t = K[1] * m.d1[sy(l, k)] + K[2] * m.d2[sy(l, k)];
+      if (k == 1) t += v1[l];
+      if (k == 2) t += v2[l];
+      if (l == 1) t -= v1[k];
+      if (l == 2) t -= v2[k];
+      Ml[l][k] = 0.5 * t;
+    }
+  for (int i = 0; i < 4; i++)
+    for (int k = 0; k < 4; k++)
+      M[i][k] = m.gc[sy(i, 0)] * Ml[0][k] + m.gc[sy(i, 1)] * Ml[1][k] + m.gc[sy(i, 2)] * Ml[2][k]
+          + m.gc[sy(i, 3)] * Ml[3][k];
+}
+
+/* ------------------------------------------------------------- geodesics */
+
+/* src/model_geodesics.c:267-293 */
+IPG_HD double stepsize(const ipgpu_geom &G, const double X[4], const double K[4], double eps)
+{
+  const double SS = SMALL * SMALL;
+  double deh = fmin(fabs(X[1] - G.cstartx[1]), 0.1);
+  double dlx1 = eps * (10 * deh) / (fabs(K[1]) + SS);
+  double cut = 0.02;
+  double lx2 = G.cstopx[2] - G.cstartx[2];
+  double dpole = fmin(fabs(X[2] / lx2), fabs((G.cstopx[2] - X[2]) / lx2));
+  double d2fac = (dpole < cut) ? dpole / 3 : fmin(cut / 3 + (dpole - cut) * 10., 1);
+  double dlx2 = eps * d2fac / (fabs(K[2]) + SS);
+  double dlx3 = eps / (fabs(K[3]) + SS);
+  double idlx1 = 1. / (fabs(dlx1) + SS);
+  double idlx2 = 1. / (fabs(dlx2) + SS);
+  double idlx3 = 1. / (fabs(dlx3) + SS);
+  return 1. / (idlx1 + idlx2 + idlx3);
+}
+
+/* src/model_geodesics.c:218-237 */
+IPG_HD bool stop_backward(const ipgpu_geom &G, double r, const double K[4])
+{
+  return (r > G.rmax_geo && K[1] < 0.) || r < (G.Rh + 0.0001);
+}
+
+/* First half of push_photon (src/geodesics.c:15-31): from (X, K) at the
+ * start of a step of length dlp (negative when tracing backward). */
+IPG_HD void half_step(const Metric &m0, const double X[4], const double K[4], double dlp,
+                      double Xh[4], double Kh[4])
+{
+  double acc[4];
+  geodesic_accel(m0, K, acc);
+  for (int k = 0; k < 4; k++) Kh[k] = K[k] - 0.5 * dlp * acc[k];
+  for (int i = 0; i < 4; i++) Xh[i] = X[i] + 0.5 * dlp * K[i];
+}
+
+/* --------------------------------------------------------------- tetrads */
+
+IPG_HD double gdot(const double g[10], const double a[4], const double b[4])
+{
+  double bl[4];
+  lower(g, b, bl);
+  return dot4(a, bl);
+}
+
+IPG_HD void normalize(double v[4], const double g[10])
+{
+  double n = sqrt(fabs(gdot(g, v, v)));
+  for (int k = 0; k < 4; k++) v[k] /= n;
+}
+
+/* src/tetrads.c:95-115 */
+IPG_HD void project_out(double a[4], const double b[4], const double g[10])
+{
+  double bb = gdot(g, b, b);
+  double ab = gdot(g, a, b);
+  for (int k = 0; k < 4; k++) a[k] -= b[k] * ab / bb;
+}
+
+IPG_HD void set_from_trial(double E[4], int defdir, const double t[4])
+{
+  double n = fabs(t[1]) + fabs(t[2]) + fabs(t[3]);
+  for (int k = 0; k < 4; k++) E[k] = (n <= SMALL) ? (double)(k == defdir) : t[k];
+}
+
+IPG_HD double det4(const double E[4][4])
+{
+  double s0 = E[0][0] * E[1][1] - E[1][0] * E[0][1];
+  double s1 = E[0][0] * E[1][2] - E[1][0] * E[0][2];
+  double s2 = E[0][0] * E[1][3] - E[1][0] * E[0][3];
+  double s3 = E[0][1] * E[1][2] - E[1][1] * E[0][2];
+  double s4 = E[0][1] * E[1][3] - E[1][1] * E[0][3];
+  double s5 = E[0][2] * E[1][3] - E[1][2] * E[0][3];
+  double c5 = E[2][2] * E[3][3] - E[3][2] * E[2][3];
+  double c4 = E[2][1] * E[3][3] - E[3][1] * E[2][3];
+  double c3 = E[2][1] * E[3][2] - E[3][1] * E[2][2];
+  double c2 = E[2][0] * E[3][3] - E[3][0] * E[2][3];
+  double c1 = E[2][0] * E[3][2] - E[3][0] * E[2][2];
+  double c0 = E[2][0] * E[3][1] - E[3][0] * E[2][1];
+  return s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+}
+
+/* Plasma-frame tetrad, src/model_tetrads.c:35-127: e0 || U, e3 || K,
+ * e2 || B, e1 completes a right-handed set; the same Gram-Schmidt order with
+ * one re-orthogonalization pass.  Returns the oddflag bits (1: handedness
+ * product off unit by more than 1e-10).  E is Econ[a][mu]. */
+IPG_HD int plasma_tetrad(const double U[4], const double K[4], const double B[4], const double g[10],
+                         double sqrtmg, double E[4][4])
+{
+  const double ones[4] = {1., 1., 1., 1.};
+  set_from_trial(E[0], 0, U);
+  set_from_trial(E[1], 3, ones);
+  set_from_trial(E[2], 2, B);
+  set_from_trial(E[3], 3, K);
+  normalize(E[0], g);
+  project_out(E[3], E[0], g);
+  project_out(E[3], E[0], g);
+  normalize(E[3], g);
+  project_out(E[2], E[0], g);
+  project_out(E[2], E[3], g);
+  project_out(E[2], E[0], g);
+  project_out(E[2], E[3], g);
+  normalize(E[2], g);
+  project_out(E[1], E[0], g);
+  project_out(E[1], E[2], g);
+  project_out(E[1], E[3], g);
+  project_out(E[1], E[0], g);
+  project_out(E[1], E[2], g);
+  project_out(E[1], E[3], g);
+  normalize(E[1], g);
+  double dot = sqrtmg * det4(E);
+  int flag = 0;
+  if (fabs(fabs(dot) - 1.) > 1.e-10) flag |= 1;
+  if (dot < 0.)
+    for (int k = 0; k < 4; k++) E[1][k] *= -1.;
+  return flag;
+}
+
+/* Camera tetrad at the camera position, src/model_tetrads.c:143-193 and the
+ * old centering of :209-249. */
+IPG_HD int camera_tetrad(const ipgpu_geom &G, const double Xcam[4], int old_centering,
+                         double E[4][4], Metric &m)
+{
+  metric(G, Xcam[1], Xcam[2], m);
+  double U[4], K[4];
+  if (old_centering) {
+    U[0] = 1.;
+    U[1] = 0.;
+    U[2] = 0.;
+    U[3] = 0.;
+  } else {
+    for (int i = 0; i < 4; i++) U[i] = -m.gc[sy(i, 0)];
+  }
+  for (int i = 0; i < 4; i++) K[i] = m.gc[sy(i, 0)] + m.gc[sy(i, 1)];
+  const double trial[4] = {0., 0., 1., 0.};
+  return plasma_tetrad(U, K, trial, m.g, m.sqrtmg, E);
+}
+
+/* Wavevector of pixel (i, j), src/model_geodesics.c:178-212 with
+ * null_normalize (src/geometry.c:190-201) and the K *= freq of
+ * src/main.c:837. */
+IPG_HD void init_K(const ipgpu_image &P, const double E[4][4], long i, long j, double K[4])
+{
+  double dxoff = (i + 0.5 + P.xoff - 0.01) / P.nx - 0.5;
+  double dyoff = (j + 0.5 + P.yoff) / P.ny - 0.5;
+  double Kt[4];
+  Kt[0] = 0.;
+  Kt[1] = (dxoff * cos(P.rotcam) - dyoff * sin(P.rotcam)) * P.fovx;
+  Kt[2] = (dxoff * sin(P.rotcam) + dyoff * cos(P.rotcam)) * P.fovy;
+  Kt[3] = 1.;
+  double inorm = sqrt(Kt[1] * Kt[1] + Kt[2] * Kt[2] + Kt[3] * Kt[3]);
+  Kt[0] = 1.;
+  Kt[1] *= 1. / inorm;
+  Kt[2] *= 1. / inorm;
+  Kt[3] *= 1. / inorm;
+  for (int l = 0; l < 4; l++)
+    K[l] = (E[0][l] * Kt[0] + E[1][l] * Kt[1] + E[2][l] * Kt[2] + E[3][l] * Kt[3]) * P.freq;
+}
+
+/* ------------------------------------------------------------ fluid model */
+
+struct Fluid {
+  double Ucon[4], Ucov[4], Bcon[4], Bcov[4];
+  double ne, thetae, B;
+};
+
+/* Ghosted-grid trilinear weights, src/grid.c:216-283 (Xtoijk_ghost) */
+struct Cell {
+  int i, j, k;
+  double d1, d2, d3;
+};
+
+IPG_HD void locate(const ipgpu_geom &G, double X1, double X2, double X3, Cell &c)
+{
+  double phi = fmod(X3, G.cstopx[3]);
+  if (phi < 0.0) phi = G.cstopx[3] + phi;
+  double fi = (X1 - G.startx[1]) / G.dx[1] - 0.5 + 1000;
+  double fj = (X2 - G.startx[2]) / G.dx[2] - 0.5 + 1000;
+  double fk = (phi - G.startx[3]) / G.dx[3] - 0.5 + 1000;
+  /* clamp before the conversion so out-of-range or NaN positions stay in
+   * bounds; inside the range this is the reference's (int) truncation */
+  fi = fmin(fmax(fi, 0.), 1.e9);
+  fj = fmin(fmax(fj, 0.), 1.e9);
+  fk = fmin(fmax(fk, 0.), 1.e9);
+  int i = (int)fi - 1000, j = (int)fj - 1000, k = (int)fk - 1000;
+  if (i < -1) i = -1;
+  if (j < -1) j = -1;
+  if (k < -1) k = -1;
+  if (i >= G.N1) i = G.N1 - 1;
+  if (j >= G.N2) j = G.N2 - 1;
+  if (k >= G.N3) k = G.N3 - 1;
+  double d1 = (X1 - ((i + 0.5) * G.dx[1] + G.startx[1])) / G.dx[1];
+  double d2 = (X2 - ((j + 0.5) * G.dx[2] + G.startx[2])) / G.dx[2];
+  double d3 = (phi - ((k + 0.5) * G.dx[3] + G.startx[3])) / G.dx[3];
+  if (d1 < 0.) d1 = 0.;
+  if (d1 >= 1.) d1 = 1.;
+  if (d2 < 0.) d2 = 0.;
+  if (d2 >= 1.) d2 = 1.;
+  if (d3 < 0.) d3 = 0.;
+  if (d3 >= 1.) d3 = 1.;
+  c.i = i + 1;
+  c.j = j + 1;
+  c.k = k + 1;
+  c.d1 = d1;
+  c.d2 = d2;
+  c.d3 = d3;
+}
+
+IPG_HD bool in_domain(const ipgpu_geom &G, double X1, double X2)
+{
+  return !(X1 < G.startx[1] || X1 >= G.stopx[1] || X2 < G.startx[2] || X2 >= G.stopx[2]);
+}
+
+/* Cell record of the device dump: 16 doubles (one 128-byte line) holding
+ * U1 U2 U3 B1 B2 B3 ne thetae b sigma and 6 unused slots. */
+constexpr int CELL_STRIDE = 16;
+
+/* Trilinear interpolation of the 10 fields at once (src/grid.c:15-44). */
+IPG_HD void gather10(const ipgpu_geom &G, const double *dump, const Cell &c, double v[10])
+{
+  const long s3 = 1, s2 = (G.N3 + 2), s1 = (long)(G.N2 + 2) * (G.N3 + 2);
+  const long base = (long)c.i * s1 + (long)c.j * s2 + c.k * s3;
+  const double b1 = 1. - c.d1, b2 = 1. - c.d2;
+  const double w00 = b1 * b2, w01 = b1 * c.d2, w10 = c.d1 * b2, w11 = c.d1 * c.d2;
+  const double z0 = 1. - c.d3, z1 = c.d3;
+  const double *p000 = dump + CELL_STRIDE * base;
+  const double *p010 = dump + CELL_STRIDE * (base + s2);
+  const double *p100 = dump + CELL_STRIDE * (base + s1);
+  const double *p110 = dump + CELL_STRIDE * (base + s1 + s2);
+  for (int f = 0; f < 10; f++) {
+    double lo = p000[f] * w00 + p010[f] * w01 + p100[f] * w10 + p110[f] * w11;
+    double hi = p000[CELL_STRIDE + f] * w00 + p010[CELL_STRIDE + f] * w01
+        + p100[CELL_STRIDE + f] * w10 + p110[CELL_STRIDE + f] * w11;
+    v[f] = z0 * lo + z1 * hi;
+  }
+}
+
+/* model/iharm/model.c:567-575 */
+IPG_HD double sigma_smoothfac(const ipgpu_model &M, double sigma)
+{
+  double above = M.sigma_cut;
+  if (M.sigma_cut_high > 0) above = M.sigma_cut_high;
+  if (sigma < M.sigma_cut) return 1;
+  if (sigma >= above) return 0;
+  double dsig = above - M.sigma_cut;
+  return cos(PI / 2. / dsig * (sigma - M.sigma_cut));
+}
+
+IPG_HD bool radiating_region(const ipgpu_geom &G, const ipgpu_model &M, double r, double th)
+{
+  if (M.kind == IPGPU_MODEL_SPHERE) return r < M.sph_R0;       /* model/sphere/model.c:213-218 */
+  return (r > G.rmin_geo && r < G.rmax_geo && th > M.th_beg     /* model/iharm/model.c:2093-2098 */
+          && th < (PI - M.th_beg));
+}
+
+/* Fluid state at a point: get_model_fourv/ne/thetae/b in one evaluation
+ * (model/iharm/model.c:420-593; model/sphere/model.c:126-211). */
+IPG_HD void fluid(const ipgpu_geom &G, const ipgpu_model &M, const double *dump, const double X[4],
+                  const Metric &m, Fluid &F)
+{
+  if (M.kind == IPGPU_MODEL_SPHERE) {
+    double U[4] = {1., 0., 0., 0.};
+    double B[4] = {0., M.sph_B0 * cos(m.th) / m.r, -M.sph_B0 * sin(m.th) / (m.r + 1.e-8), 0.};
+    for (int i = 0; i < 4; i++) {
+      F.Ucon[i] = U[i];
+      F.Bcon[i] = B[i];
+    }
+    lower(m.g, F.Ucon, F.Ucov);
+    lower(m.g, F.Bcon, F.Bcov);
+    bool rad = m.r < M.sph_R0;
+    F.ne = rad ? M.sph_Ne0 : 0.;
+    F.thetae = rad ? M.sph_Thetae0 : 0.;
+    F.B = rad ? M.sph_B0 : 0.;
+    return;
+  }
+  if (!in_domain(G, X[1], X[2])) {
+    F.Ucov[0] = -1. / sqrt(-m.gc[0]);
+    F.Ucov[1] = 0.;
+    F.Ucov[2] = 0.;
+    F.Ucov[3] = 0.;
+    for (int i = 0; i < 4; i++) F.Ucon[i] = m.gc[sy(i, 0)] * F.Ucov[0];
+    for (int i = 0; i < 4; i++) {
+      F.Bcon[i] = 0.;
+      F.Bcov[i] = 0.;
+    }
+    F.ne = 0.;
+    F.thetae = 0.;
+    F.B = 0.;
+    return;
+  }
+  Cell c;
+  locate(G, X[1], X[2], X[3], c);
+  double v[10];
+  gather10(G, dump, c, v);
+  double VdotV = 0.;
+  for (int i = 1; i < 4; i++)
+    for (int j = 1; j < 4; j++) VdotV += m.g[sy(i, j)] * v[i - 1] * v[j - 1];
+  double Vfac = sqrt(-1. / m.gc[0] * (1. + fabs(VdotV)));
+  F.Ucon[0] = -Vfac * m.gc[0];
+  for (int i = 1; i < 4; i++) F.Ucon[i] = v[i - 1] - Vfac * m.gc[sy(0, i)];
+  lower(m.g, F.Ucon, F.Ucov);
+  F.Bcon[0] = v[3] * F.Ucov[1] + v[4] * F.Ucov[2] + v[5] * F.Ucov[3];
+  F.Bcon[1] = (v[3] + F.Ucon[1] * F.Bcon[0]) / F.Ucon[0];
+  F.Bcon[2] = (v[4] + F.Ucon[2] * F.Bcon[0]) / F.Ucon[0];
+  F.Bcon[3] = (v[5] + F.Ucon[3] * F.Bcon[0]) / F.Ucon[0];
+  lower(m.g, F.Bcon, F.Bcov);
+  double sigma = v[9];
+  F.ne = (sigma > M.sigma_cut) ? 0. : v[6] * sigma_smoothfac(M, sigma);
+  F.thetae = v[7];
+  F.B = v[8];
+}
+
+/* --------------------------------------------------------- K_n(1/Theta_e) */
+
+/* Exponentially scaled K0, K1, K2 at x: ks[n] = e^x K_n(x), and e^-x.
+ * x <= 2: power series (A&S 9.6.10-9.6.13); x > 2: Chebyshev expansion of
+ * sqrt(x) e^x K_n(x) in 4/x - 1 (ipg_bessel_coeffs.h); K2 by the upward
+ * recurrence K2 = K0 + (2/x) K1 as GSL's K_n. */
+IPG_HD void bessel_k012(double x, double ks[3], double &emx)
+{
+  emx = exp(-x);
+  double k0s, k1s;
+  if (x <= 2.0) {
+    const double EG = 0.57721566490153286061;
+    double y = 0.25 * x * x;
+    double t0 = 1.0, t1 = 0.5 * x;
+    double I0 = 1.0, I1 = t1, S0 = 0., S1 = t1 * (1.0 - 2.0 * EG);
+    double Hk = 0.0, p1 = -EG, p2 = 1.0 - EG;
+    for (int k = 1; k < 16; k++) {
+      double kk = (double)k;
+      t0 *= y / (kk * kk);
+      t1 *= y / (kk * (kk + 1.));
+      Hk += 1.0 / kk;
+      p1 += 1.0 / kk;
+      p2 += 1.0 / (kk + 1.);
+      I0 += t0;
+      I1 += t1;
+      S0 += t0 * Hk;
+      S1 += t1 * (p1 + p2);
+    }
+    double lg = log(0.5 * x);
+    double ex = exp(x);
+    k0s = (-(lg + EG) * I0 + S0) * ex;
+    k1s = (1.0 / x + lg * I1 - 0.5 * S1) * ex;
+  } else {
+    double t = 4.0 / x - 1.0, t2 = 2.0 * t;
+    double b0 = 0., b1 = 0., b2 = 0.;
+    for (int k = IPG_K0_NCHEB - 1; k >= 1; k--) {
+      b2 = b1;
+      b1 = b0;
+      b0 = t2 * b1 - b2 + ipg_k0_cheb[k];
+    }
+    double f0 = t * b0 - b1 + ipg_k0_cheb[0];
+    b0 = b1 = b2 = 0.;
+    for (int k = IPG_K1_NCHEB - 1; k >= 1; k--) {
+      b2 = b1;
+      b1 = b0;
+      b0 = t2 * b1 - b2 + ipg_k1_cheb[k];
+    }
+    double f1 = t * b0 - b1 + ipg_k1_cheb[0];
+    double isx = 1.0 / sqrt(x);
+    k0s = f0 * isx;
+    k1s = f1 * isx;
+  }
+  ks[0] = k0s;
+  ks[1] = k1s;
+  ks[2] = k0s + (2.0 / x) * k1s;
+}
+
+/* ---------------------------------------------------- transfer coefficients */
+
+struct Coef {
+  double jI, jQ, jU, jV, aI, aQ, aU, aV, rQ, rU, rV;
+};
+
+/* src/radiation.c:14-23 */
+IPG_HD double Bnu_inv(double nu, double Thetae)
+{
+  double x = HPL * nu / (ME * CL * CL * Thetae);
+  if (x < 2.e-3) return ((2. * HPL / (CL * CL)) / (x / 24. * (24. + x * (12. + x * (4. + x)))));
+  return ((2. * HPL / (CL * CL)) / (exp(x) - 1.));
+}
+
+/* src/symphony/maxwell_juettner_fits.c:334-356 with x^(1/3) shared */
+IPG_HD void dexter_IQV(double x, double &II, double &IQ, double &IV)
+{
+  double c = cbrt(x);
+  double ic = 1. / c, ic2 = ic * ic;
+  double e = exp(-1.8899 * c);
+  II = 2.5651 * (1 + 1.92 * ic + 0.9977 * ic2) * e;
+  IQ = 2.5651 * (1 + 0.93193 * ic + 0.499873 * ic2) * e;
+  IV = (1.81384 / x + 3.42319 * ic2 + 0.0292545 / sqrt(x) + 2.03773 * ic) * e;
+}
+
+/* Thermal (Maxwell-Juettner) coefficients at one point: the pol = 0 branch
+ * (Leung j_I, Kirchhoff a_I; src/model_radiation.c:226-237) and, if pol,
+ * the pol = 1 branch (Dexter or Pandya j_S, Kirchhoff a_S, Dexter rho_Q and
+ * Shcherbakov rho_V; :243-305), sharing the fluid state, the fluid-frame
+ * frequency and angle, and one K_n(1/Theta_e) evaluation.
+ * ju/ku: j_I/nu^2 and a_I nu of the unpolarized solve. */
+IPG_HD void coefficients(const Fluid &F, const double K[4], int emission_type, bool pol,
+                         double &ju, double &ku, Coef &c)
+{
+  ju = 0.;
+  ku = 0.;
+  c.jI = c.jQ = c.jU = c.jV = c.aI = c.aQ = c.aU = c.aV = c.rQ = c.rU = c.rV = 0.;
+  double Ne = F.ne;
+  if (Ne <= 0.) return;
+  /* get_fluid_nu, get_bk_angle: src/radiation.c:26-72 */
+  double nu = -(K[0] * F.Ucov[0] + K[1] * F.Ucov[1] + K[2] * F.Ucov[2] + K[3] * F.Ucov[3]) * ME
+      * CL * CL / HPL;
+  double nusq = nu * nu;
+  double theta;
+  {
+    double Bm = sqrt(fabs(F.Bcon[0] * F.Bcov[0] + F.Bcon[1] * F.Bcov[1] + F.Bcon[2] * F.Bcov[2]
+                          + F.Bcon[3] * F.Bcov[3]));
+    if (Bm == 0.) {
+      theta = PI / 2.;
+    } else {
+      double kk = fabs(K[0] * F.Ucov[0] + K[1] * F.Ucov[1] + K[2] * F.Ucov[2] + K[3] * F.Ucov[3]);
+      double mu = (K[0] * F.Bcov[0] + K[1] * F.Bcov[1] + K[2] * F.Bcov[2] + K[3] * F.Bcov[3])
+          / (kk * Bm);
+      if (fabs(mu) > 1.) mu /= fabs(mu);
+      theta = acos(mu);
+    }
+  }
+  double B = F.B, Th = F.thetae;
+  double ks[3], emx;
+  bessel_k012(1. / Th, ks, emx);
+  const double sth = sin(theta), cth = cos(theta);
+  /* Faraday fits, src/symphony/maxwell_juettner_fits.c:207-284 */
+  double omega0 = EE * B / (ME * CL);
+  double wp2 = 4. * PI * Ne * (EE * EE) / ME;
+  double xr = Th * sqrt(sqrt(2.) * sth * (1.e3 * omega0 / (2. * PI * nu)));
+  double w = 2. * PI * nu;
+  if (theta <= 0.0 || theta >= PI) {
+    if (pol) {
+      double fit;
+      if (emission_type != 1 && emx * ks[2] > 0) {
+        double sh = 0.43793091 * log(1. + 0.00185777 * pow(xr, 1.50316886));
+        fit = (emx * ks[0] - sh) / (emx * ks[2]);
+      } else {
+        fit = (ks[0] / ks[2]) * (1 - 0.11 * log(1 + 0.035 * xr));
+      }
+      c.rV = 2. * PI * nu / CL * (wp2 * omega0 / (w * w * w) * fit * cth) * nu;
+    }
+    return;
+  }
+  double Bnuinv = Bnu_inv(nu, Th);
+  /* unpolarized: Leung fit, maxwell_juettner_fits.c:368-395 */
+  {
+    double K2 = fmax(emx * ks[2], SMALL);
+    double nuc = EE * B / (2. * PI * ME * CL);
+    double nus = (2. / 9.) * nuc * Th * Th * sth;
+    double j = 0.;
+    if (!(nu > 1.e12 * nus)) {
+      double x = nu / nus;
+      double f = sqrt(x) + pow(2., 11. / 12.) * pow(x, 1. / 6.);
+      f = f * f;
+      j = (sqrt(2.) * PI * EE * EE * Ne * nus / (3. * CL * K2)) * f * exp(-cbrt(x));
+    }
+    ju = j / nusq;
+    ku = (Bnuinv > 0) ? ju / Bnuinv : 0;
+  }
+  if (!pol) return;
+  double jI, jQ, jV;
+  if (emission_type == 1) {
+    /* Pandya fits, maxwell_juettner_fits.c:27-129 */
+    double nu_c = EE * B / (2. * PI * ME * CL);
+    double nu_s = (2. / 9.) * nu_c * sth * Th * Th;
+    double X = nu / nu_s;
+    double pre = (Ne * (EE * EE) * nu_c) / CL;
+    double X13 = cbrt(X), X16 = pow(X, 1. / 6.), e13 = exp(-X13);
+    double t1 = sqrt(2.) * PI / 27. * sth;
+    double tI = sqrt(X) + pow(2., 11. / 12.) * X16;
+    jI = pre * t1 * (tI * tI) * e13;
+    double Tq = pow(Th, 24. / 25.);
+    double tq2 = (7. * Tq + 35.) / (10. * Tq + 75.);
+    double tq3 = sqrt(X) + tq2 * pow(2., 11. / 12.) * X16;
+    jQ = pre * t1 * (tq3 * tq3) * e13;  /* = -maxwell_juettner_Q */
+    double tv1 = (37. - 87. * sin(theta - 28. / 25.)) / (100. * (Th + 1.));
+    double tv2 = pow(1. + (pow(Th, 3. / 5.) / 25. + 7. / 10.) * pow(X, 9. / 25.), 5. / 3.);
+    jV = pre * tv1 * tv2 * e13;
+  } else {
+    /* Dexter fits, maxwell_juettner_fits.c:287-330 */
+    double nus = 3.0 * EE * B * sth / 4.0 / PI / ME / CL * Th * Th + 1.0;
+    double x = nu / nus;
+    double II, IQ, IV;
+    dexter_IQV(x, II, IQ, IV);
+    double pre = Ne * EE * EE * nu / 2. / sqrt(3.) / CL / Th / Th;
+    jI = pre * II;
+    jQ = pre * IQ;
+    jV = 2. * Ne * EE * EE * nu / (sth / cth) / 3. / sqrt(3.) / CL / Th / Th / Th * IV;
+  }
+  /* src/model_radiation.c:245-281 */
+  c.jI = jI / nusq;
+  c.jQ = jQ / nusq;
+  c.jU = -0. / nusq;
+  c.jV = jV / nusq;
+  double jP = sqrt(c.jQ * c.jQ + c.jU * c.jU + c.jV * c.jV);
+  if (c.jI < jP / 0.99) {
+    double pf = c.jI / jP * 0.99;
+    c.jQ *= pf;
+    c.jU *= pf;
+    c.jV *= pf;
+  }
+  if (Bnuinv > 0) {
+    c.aI = c.jI / Bnuinv;
+    c.aQ = c.jQ / Bnuinv;
+    c.aU = c.jU / Bnuinv;
+    c.aV = c.jV / Bnuinv;
+  }
+  /* rho_Q (Dexter 2016) and rho_V (Shcherbakov), dexter_fit = 0 */
+  {
+    double xe = xr + 1.e-16;
+    double extraterm = (.011 * exp(-xr / 47.2)
+                        - pow(2., (-1. / 3.)) / pow(3., (23. / 6.)) * PI * 1.e4 * pow(xe, (-8. / 3.)))
+        * (0.5 + 0.5 * tanh((log(xr) - log(120.)) / 0.1));
+    double jff = 2.011 * exp(-pow(xr, 1.035) / 4.7) - cos(xr / 2.) * exp(-pow(xr, 1.2) / 2.73)
+        - .011 * exp(-xr / 47.2) + extraterm;
+    double kr = ks[1] / ks[2];
+    double w2 = w * w;
+    double eps11m22 = jff * wp2 * (omega0 * omega0) / (w2 * w2) * (kr + 6. * Th) * (sth * sth);
+    c.rQ = (2. * PI * nu / (2. * CL) * eps11m22) * nu;
+    double fit = (ks[0] / ks[2]) * (1 - 0.11 * log(1 + 0.035 * xr));
+    double eps12 = wp2 * omega0 / (w2 * w) * fit * cth;
+    c.rV = (2. * PI * nu / CL * eps12) * nu;
+    c.rU = 0. * nu;
+  }
+}
+
+/* ------------------------------------------------------------ radiative step */
+
+/* src/ipolarray.c:607-628 */
+IPG_HD double approximate_solve(double Ii, double ji, double ki, double jf, double kf, double dl,
+                                double &tau)
+{
+  double javg = (ji + jf) / 2.;
+  double kavg = (ki + kf) / 2.;
+  double dtau = dl * kavg;
+  tau += dtau;
+  if (dtau < 1.e-3) return Ii + (javg - Ii * kavg) * dl * (1. - (dtau / 2.) * (1. - dtau / 3.));
+  double efac = exp(-dtau);
+  return Ii * efac + (javg / kavg) * (1. - efac);
+}
+
+/* Faraday rotation over x, src/ipolarray.c:391-408 */
+IPG_HD void faraday(const Coef &c, double x, double S[4])
+{
+  double rQ = c.rQ, rU = c.rU, rV = c.rV;
+  double rho2 = rQ * rQ + rU * rU + rV * rV;
+  double Q = S[1], U = S[2], V = S[3];
+  if (rho2 > 1e-80) {
+    double rho = sqrt(rho2);
+    double rdS = rQ * Q + rU * U + rV * V;
+    double cc = cos(rho * x), s = sin(rho * x), sh = sin(0.5 * rho * x);
+    S[1] = Q * cc + 2 * rQ * rdS / rho2 * sh * sh + (rU * V - rV * U) / rho * s;
+    S[2] = U * cc + 2 * rU * rdS / rho2 * sh * sh + (rV * Q - rQ * V) / rho * s;
+    S[3] = V * cc + 2 * rV * rdS / rho2 * sh * sh + (rQ * U - rU * Q) / rho * s;
+  } else {
+    S[1] = Q + (-rV * U + rU * V) * x;
+    S[2] = U + (rV * Q - rQ * V) * x;
+    S[3] = V + (-rU * Q + rQ * U) * x;
+  }
+}
+
+/* Constant-coefficient emission/absorption over x, src/ipolarray.c:411-500 */
+IPG_HD void absorb_emit(const Coef &c, double x, double S[4])
+{
+  double jI = c.jI, jQ = c.jQ, jU = c.jU, jV = c.jV, aI = c.aI, aQ = c.aQ, aU = c.aU, aV = c.aV;
+  double SI1 = S[0], SQ1 = S[1], SU1 = S[2], SV1 = S[3];
+  double aP2 = aQ * aQ + aU * aU + aV * aV;
+  if (aP2 > 1e-80) {
+    double aP = sqrt(aP2);
+    double tauP = aP * x, tauI = aI * x;
+    double ads0 = aQ * SQ1 + aU * SU1 + aV * SV1;
+    double adj = aQ * jQ + aU * jU + aV * jV;
+    double efacm = exp(-tauI + tauP), efacp = exp(-tauI - tauP), efac = exp(-tauI);
+    double SI2 = efacm * (SI1 / 2. - ads0 / (2. * aP)) + efacp * (SI1 / 2. + ads0 / (2. * aP));
+    double SQ2 = efacm * (-SI1 * aQ * aP + ads0 * aQ) / (2. * aP2)
+        + efacp * (SI1 * aQ * aP + ads0 * aQ) / (2. * aP2) + efac * (SQ1 - ads0 * aQ / (aP2));
+    double SU2 = efacm * (-SI1 * aU * aP + ads0 * aU) / (2. * aP2)
+        + efacp * (SI1 * aU * aP + ads0 * aU) / (2. * aP2) + efac * (SU1 - ads0 * aU / (aP2));
+    double SV2 = efacm * (-SI1 * aV * aP + ads0 * aV) / (2. * aP2)
+        + efacp * (SI1 * aV * aP + ads0 * aV) / (2. * aP2) + efac * (SV1 - ads0 * aV / (aP2));
+    double afacm = 1. - efacm, afac = 1. - efac, afacp = 1. - efacp;
+    if (tauI - tauP <= 1e-5) {
+      double e = tauI - tauP;
+      afacm = e * (1. - (e / 2.) * (1. - e / 3.));
+      if (tauI <= 1e-5) {
+        e = tauI;
+        afac = e * (1. - (e / 2.) * (1. - e / 3.));
+        if (tauI + tauP <= 1e-5) {
+          e = tauI + tauP;
+          afacp = e * (1. - (e / 2.) * (1. - e / 3.));
+        }
+      }
+    }
+    SQ2 += afac * (jQ / aI - aQ * adj / (aI * aP2));
+    SU2 += afac * (jU / aI - aU * adj / (aI * aP2));
+    SV2 += afac * (jV / aI - aV * adj / (aI * aP2));
+    SI2 += afacm * (aP * jI - adj) / (2. * aP * (aI - aP));
+    SQ2 += afacm * aQ * (-aP * jI + adj) / (2. * aP2 * (aI - aP));
+    SU2 += afacm * aU * (-aP * jI + adj) / (2. * aP2 * (aI - aP));
+    SV2 += afacm * aV * (-aP * jI + adj) / (2. * aP2 * (aI - aP));
+    SI2 += afacp * (aP * jI + adj) / (2. * aP * (aI + aP));
+    SQ2 += afacp * aQ * (aP * jI + adj) / (2. * aP2 * (aI + aP));
+    SU2 += afacp * aU * (aP * jI + adj) / (2. * aP2 * (aI + aP));
+    SV2 += afacp * aV * (aP * jI + adj) / (2. * aP2 * (aI + aP));
+    S[0] = SI2;
+    S[1] = SQ2;
+    S[2] = SU2;
+    S[3] = SV2;
+  } else {
+    double tf;
+    S[0] = approximate_solve(SI1, jI, aI, jI, aI, x, tf = 0.);
+    S[1] = approximate_solve(SQ1, jQ, aI, jQ, aI, x, tf = 0.);
+    S[2] = approximate_solve(SU1, jU, aI, jU, aI, x, tf = 0.);
+    S[3] = approximate_solve(SV1, jV, aI, jV, aI, x, tf = 0.);
+  }
+}
+
+/* Hermitian coherency tensor: n[0..3] real diagonal, then (re, im) of
+ * N01 N02 N03 N12 N13 N23. */
+struct Herm {
+  double d[4];
+  double re[6], im[6];
+};
+
+IPG_HD constexpr int off(int i, int j)  /* i < j */
+{
+  return (i == 0) ? (j - 1) : (i == 1) ? (j + 1) : 5;
+}
+
+IPG_HD void herm_get(const Herm &N, int i, int j, double &re, double &im)
+{
+  if (i == j) {
+    re = N.d[i];
+    im = 0.;
+  } else if (i < j) {
+    re = N.re[off(i, j)];
+    im = N.im[off(i, j)];
+  } else {
+    re = N.re[off(j, i)];
+    im = -N.im[off(j, i)];
+  }
+}
+
+IPG_HD bool herm_nonzero(const Herm &N)
+{
+  bool z = true;
+  for (int i = 0; i < 4; i++) z = z && (N.d[i] == 0.);
+  for (int i = 0; i < 6; i++) z = z && (N.re[i] == 0.) && (N.im[i] == 0.);
+  return !z;
+}
+
+/* Nout = Nin - dl (M Nm + Nm M^T): push_polar, src/ipolarray.c:253-289 */
+IPG_HD void transport(const Herm &Nin, const Herm &Nm, const double M[4][4], double dl, Herm &Nout)
+{
+  for (int i = 0; i < 4; i++)
+    for (int j = i; j < 4; j++) {
+      double sr = 0., si = 0.;
+      for (int k = 0; k < 4; k++) {
+        double ar, ai, br, bi;
+        herm_get(Nm, k, j, ar, ai);
+        herm_get(Nm, i, k, br, bi);
+        sr += M[i][k] * ar + M[j][k] * br;
+        si += M[i][k] * ai + M[j][k] * bi;
+      }
+      if (i == j) {
+        Nout.d[i] = Nin.d[i] - dl * sr;
+      } else {
+        Nout.re[off(i, j)] = Nin.re[off(i, j)] - dl * sr;
+        Nout.im[off(i, j)] = Nin.im[off(i, j)] - dl * si;
+      }
+    }
+}
+
+/* Stokes parameters of N in a tetrad with covariant rows e1 = Ecov[1],
+ * e2 = Ecov[2] (complex_coord_to_tetrad_rank2 + tensor_to_stokes,
+ * src/ipolarray.c:684-694, :710-728, restricted to the [1..2] block). */
+IPG_HD void herm_to_stokes(const Herm &N, const double e1[4], const double e2[4], double S[4])
+{
+  double t11 = 0., t22 = 0., r12 = 0., i12 = 0.;
+  for (int k = 0; k < 4; k++) {
+    t11 += N.d[k] * e1[k] * e1[k];
+    t22 += N.d[k] * e2[k] * e2[k];
+    r12 += N.d[k] * e1[k] * e2[k];
+  }
+  for (int k = 0; k < 4; k++)
+    for (int l = k + 1; l < 4; l++) {
+      double re = N.re[off(k, l)], im = N.im[off(k, l)];
+      t11 += 2. * re * e1[k] * e1[l];
+      t22 += 2. * re * e2[k] * e2[l];
+      r12 += re * (e1[k] * e2[l] + e1[l] * e2[k]);
+      i12 += im * (e1[k] * e2[l] - e1[l] * e2[k]);
+    }
+  S[0] = (t11 + t22) / 2;
+  S[1] = (t11 - t22) / 2;
+  S[2] = r12;
+  S[3] = -i12;
+}
+
+/* N += Econ^T dT Econ for a Stokes change dS in the [1..2] tetrad block
+ * (stokes_to_tensor + complex_tetrad_to_coord_rank2, src/ipolarray.c:673-682,
+ * :730-748, applied as a difference). */
+IPG_HD void herm_add_stokes(Herm &N, const double E1[4], const double E2[4], const double dS[4])
+{
+  double a = dS[0] + dS[1], b = dS[0] - dS[1], u = dS[2], v = dS[3];
+  for (int k = 0; k < 4; k++) N.d[k] += a * E1[k] * E1[k] + b * E2[k] * E2[k] + 2. * u * E1[k] * E2[k];
+  for (int k = 0; k < 4; k++)
+    for (int l = k + 1; l < 4; l++) {
+      N.re[off(k, l)] += a * E1[k] * E1[l] + b * E2[k] * E2[l] + u * (E1[k] * E2[l] + E2[k] * E1[l]);
+      N.im[off(k, l)] += v * (E2[k] * E1[l] - E1[k] * E2[l]);
+    }
+}
+
+/* One radiating polarized step, evolve_N (src/ipolarray.c:332-575):
+ * coefficients at the end point, plasma tetrad, Faraday half step,
+ * absorption/emission, Faraday half step, tauF; returns oddflag bits. */
+IPG_HD int evolve(const Fluid &Fin, const Metric &m, const double K[4], const Coef &c0, double dlam,
+                  int zero_emission, int stokes_floors, Herm &N, double &tauF)
+{
+  Coef c = c0;
+  if (zero_emission) c.jI = c.jQ = c.jU = c.jV = 0.;
+  double Bcon[4] = {Fin.Bcon[0], Fin.Bcon[1], Fin.Bcon[2], Fin.Bcon[3]};
+  double bsq = Fin.Bcon[0] * Fin.Bcov[0] + Fin.Bcon[1] * Fin.Bcov[1] + Fin.Bcon[2] * Fin.Bcov[2]
+      + Fin.Bcon[3] * Fin.Bcov[3];
+  if (bsq <= 0.) {
+    Bcon[0] = 0.;
+    Bcon[1] = 1.;
+    Bcon[2] = 1.;
+    Bcon[3] = 1.;
+  }
+  double E[4][4];
+  int flag = plasma_tetrad(Fin.Ucon, K, Bcon, m.g, m.sqrtmg, E);
+  double e1[4], e2[4];
+  lower(m.g, E[1], e1);
+  lower(m.g, E[2], e2);
+  double S0[4], S[4];
+  herm_to_stokes(N, e1, e2, S0);
+  for (int n = 0; n < 4; n++) S[n] = S0[n];
+  faraday(c, dlam * 0.5, S);
+  absorb_emit(c, dlam, S);
+  faraday(c, dlam * 0.5, S);
+  tauF += dlam * fabs(c.rV);
+  if (stokes_floors) {
+    if (S[0] < 0) {
+      S[0] = S[1] = S[2] = S[3] = 0.;
+    } else {
+      double pf = sqrt(S[1] * S[1] + S[2] * S[2] + S[3] * S[3]) / S[0];
+      if (pf > 1.) {
+        S[1] /= pf;
+        S[2] /= pf;
+        S[3] /= pf;
+      }
+    }
+  }
+  double dS[4] = {S[0] - S0[0], S[1] - S0[1], S[2] - S0[2], S[3] - S0[3]};
+  herm_add_stokes(N, E[1], E[2], dS);
+  if (tauF > 1.e100 || tauF < -1.e100 || isnan(tauF)) flag |= 2;
+  if (isnan(N.d[0]) || isnan(S[0])) flag |= 4;
+  if (isnan(Fin.Ucov[0])) flag |= 8;
+  return flag;
+}
+
+/* Camera-frame Stokes of N, project_N (src/ipolarray.c:578-598). */
+IPG_HD void project(const Herm &N, const double Ecam[4][4], const Metric &mcam, double rotcam,
+                    double S[4])
+{
+  double e1[4], e2[4];
+  lower(mcam.g, Ecam[1], e1);
+  lower(mcam.g, Ecam[2], e2);
+  double T[4];
+  herm_to_stokes(N, e1, e2, T);
+  double rc = -2. * rotcam;
+  S[0] = T[0];
+  S[1] = T[1] * cos(rc) - T[2] * sin(rc);
+  S[2] = T[1] * sin(rc) + T[2] * cos(rc);
+  S[3] = T[3];
+}
+
+}  // namespace ipg

@@ -1,0 +1,660 @@
+/*
+ * ipgpu: gfx950 kernels and the C-ABI of include/ipgpu.h.
+ *
+ * Pipeline of one image (replaces src/main.c:536-615):
+ *   k_trace     one lane per ray: camera tetrad, initial wavevector and the
+ *               backward geodesic (src/model_geodesics.c:38-171), writing one
+ *               64-byte record per step (X1 X2 X3 K0 K1 K2 K3 dl) to HBM.
+ *   k_transfer  one lane per ray: walks its records from the far end toward
+ *               the camera (src/ipolarray.c:59-247): parallel transport of N,
+ *               fluid gathers from the dump in HBM, thermal coefficients,
+ *               unpolarized and polarized steps; then project_N.
+ *   k_finalize  reference output layout and units (save_pixel,
+ *               src/main.c:885-910).
+ * Rays are numbered in 8x8 pixel tiles so that one 64-lane wavefront traces
+ * a compact patch of the image (similar path lengths, nearby dump cells).
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "ipg_physics.h"
+#include "ipg_pipeline.h"
+#include "ipgpu.h"
+
+using namespace ipg;
+
+namespace {
+
+constexpr int TILE = 8;         /* pixels per tile side */
+constexpr int WAVE = 64;
+constexpr int NSTATUS = 4;      /* [0] overflowed rays [1] first maxnstep ray+1 [2] NaN pixel+1 */
+
+__device__ __forceinline__ void ray_pixel(int ray, int nx, int ny, int &i, int &j)
+{
+  int tiles_y = (ny + TILE - 1) / TILE;
+  int t = ray / WAVE, l = ray % WAVE;
+  int ti = t / tiles_y, tj = t % tiles_y;
+  i = ti * TILE + l / TILE;
+  j = tj * TILE + l % TILE;
+}
+
+__global__ __launch_bounds__(WAVE) void k_trace(ipgpu_geom G, ipgpu_image P, int nrays, int cap,
+                                                double *__restrict__ traj, int *__restrict__ nstep_out,
+                                                int *__restrict__ nturns_buf, int *__restrict__ status)
+{
+  int ray = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ray >= nrays) return;
+  int pi, pj;
+  ray_pixel(ray, P.nx, P.ny, pi, pj);
+  if (pi >= P.nx || pj >= P.ny) {
+    nstep_out[ray] = 0;
+    return;
+  }
+  double *rec = traj + (size_t)ray * cap * REC;
+  int *nt = nturns_buf ? nturns_buf + (size_t)ray * cap : nullptr;
+  int nstep = trace_ray(G, P, pi, pj, cap, rec, nt);
+  nstep_out[ray] = nstep;
+  if (nstep >= cap) atomicAdd(&status[0], 1);
+}
+
+__global__ __launch_bounds__(WAVE) void k_transfer(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
+                                                   int nrays, int cap, const double *__restrict__ traj,
+                                                   const int *__restrict__ nstep_in,
+                                                   const int *__restrict__ nturns_buf,
+                                                   const double *__restrict__ dump,
+                                                   double *__restrict__ out7, int *__restrict__ flags,
+                                                   int *__restrict__ status)
+{
+  int ray = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ray >= nrays) return;
+  int pi, pj;
+  ray_pixel(ray, P.nx, P.ny, pi, pj);
+  if (pi >= P.nx || pj >= P.ny) return;
+  int ns = nstep_in[ray];
+  double *o = out7 + (size_t)ray * 7;
+  if (ns >= P.maxnstep - 1) {
+    atomicMin(&status[1], pi * P.ny + pj + 1);
+    for (int q = 0; q < 7; q++) o[q] = 0.;
+    flags[ray] = 0;
+    return;
+  }
+  const double *rec = traj + (size_t)ray * cap * REC;
+  const int *nt = nturns_buf ? nturns_buf + (size_t)ray * cap : nullptr;
+  double res[7];
+  flags[ray] = transfer_ray(G, Mo, P, ns, rec, nt, dump, res);
+  for (int q = 0; q < 7; q++) o[q] = res[q];
+}
+
+/* save_pixel (src/main.c:885-910): reference layout, nu^3 scaling, QU sign */
+__global__ void k_finalize(ipgpu_image P, int nrays, double f3, const double *__restrict__ out7,
+                           const int *__restrict__ flags_ray, const int *__restrict__ nstep_ray,
+                           double *__restrict__ image, double *__restrict__ imageS,
+                           double *__restrict__ taus, int *__restrict__ nstep_pix,
+                           int *__restrict__ flag_pix, int *__restrict__ status)
+{
+  int ray = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ray >= nrays) return;
+  int i, j;
+  ray_pixel(ray, P.nx, P.ny, i, j);
+  if (i >= P.nx || j >= P.ny) return;
+  size_t p = (size_t)i * P.ny + j;
+  const double *o = out7 + (size_t)ray * 7;
+  image[p] = o[0] * f3;
+  taus[p] = o[5];
+  nstep_pix[p] = nstep_ray[ray];
+  flag_pix[p] = flags_ray[ray];
+  if (!P.only_unpolarized) {
+    double qs = (P.qu_conv == 0) ? -1. : 1.;
+    imageS[p * 5 + 0] = o[1] * f3;
+    imageS[p * 5 + 1] = qs * o[2] * f3;
+    imageS[p * 5 + 2] = qs * o[3] * f3;
+    imageS[p * 5 + 3] = o[4] * f3;
+    imageS[p * 5 + 4] = o[6];
+    if (isnan(imageS[p * 5])) atomicMin(&status[2], (int)p + 1);
+  }
+}
+
+/* ------------------------------------------------------------ dump kernels */
+
+__device__ __forceinline__ size_t gcell(const ipgpu_geom &G, int i, int j, int k)
+{
+  return ((size_t)i * (G.N2 + 2) + j) * (size_t)(G.N3 + 2) + k;
+}
+
+/* ghosted SoA fields -> 16-double cell records */
+__global__ void k_pack(size_t ncell, const double *__restrict__ soa, double *__restrict__ dump)
+{
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  double *d = dump + CELL_STRIDE * c;
+  for (int f = 0; f < IPGPU_NFIELDS; f++) d[f] = soa[f * ncell + c];
+  for (int f = IPGPU_NFIELDS; f < CELL_STRIDE; f++) d[f] = 0.;
+}
+
+/* slots of the derivation: 10 rho, 11 u, 12 beta */
+constexpr int S_RHO = 10, S_UU = 11, S_BETA = 12;
+
+/* interior cells: primitives in, b from the local metric
+ * (model/iharm/model.c:1954-2010) */
+__global__ void k_derive_b(ipgpu_geom G, double B_unit, int nprim, const double *__restrict__ prims,
+                           double *__restrict__ dump)
+{
+  size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t ntot = (size_t)G.N1 * G.N2 * G.N3;
+  if (n >= ntot) return;
+  int k = (int)(n % G.N3);
+  int j = (int)((n / G.N3) % G.N2);
+  int i = (int)(n / ((size_t)G.N3 * G.N2));
+  const double *p = prims + n * nprim;
+  double *d = dump + CELL_STRIDE * gcell(G, i + 1, j + 1, k + 1);
+  Metric m;
+  double X1 = G.startx[1] + (i + 0.5) * G.dx[1];
+  double X2 = G.startx[2] + (j + 0.5) * G.dx[2];
+  metric(G, X1, X2, m);
+  double U[3] = {p[2], p[3], p[4]}, Bp[3] = {p[5], p[6], p[7]};
+  double UdotU = 0.;
+  for (int l = 1; l < 4; l++)
+    for (int mm = 1; mm < 4; mm++) UdotU += m.g[sy(l, mm)] * U[l - 1] * U[mm - 1];
+  double ufac = sqrt(-1. / m.gc[0] * (1 + fabs(UdotU)));
+  double ucon[4], ucov[4], bcon[4], bcov[4];
+  ucon[0] = -ufac * m.gc[0];
+  for (int l = 1; l < 4; l++) ucon[l] = U[l - 1] - ufac * m.gc[sy(0, l)];
+  lower(m.g, ucon, ucov);
+  double udotB = 0.;
+  for (int l = 1; l < 4; l++) udotB += ucov[l] * Bp[l - 1];
+  bcon[0] = udotB;
+  for (int l = 1; l < 4; l++) bcon[l] = (Bp[l - 1] + ucon[l] * udotB) / ucon[0];
+  lower(m.g, bcon, bcov);
+  double bsq = dot4(bcon, bcov);
+  for (int f = 0; f < 6; f++) d[f] = p[2 + f];
+  d[8] = sqrt(bsq) * B_unit;
+  d[S_RHO] = p[0];
+  d[S_UU] = p[1];
+}
+
+__device__ __forceinline__ void copy_ghost(double *dst, const double *src)
+{
+  for (int f = 0; f < 6; f++) dst[f] = src[f];
+  dst[8] = src[8];
+  dst[S_RHO] = src[S_RHO];
+  dst[S_UU] = src[S_UU];
+}
+
+/* ghost zones, model/iharm/model.c:1314-1370: radial copy */
+__global__ void k_bc_radial(ipgpu_geom G, double *dump)
+{
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.N2 * G.N3) return;
+  int j = n / G.N3 + 1, k = n % G.N3 + 1;
+  copy_ghost(dump + CELL_STRIDE * gcell(G, 0, j, k), dump + CELL_STRIDE * gcell(G, 1, j, k));
+  copy_ghost(dump + CELL_STRIDE * gcell(G, G.N1 + 1, j, k), dump + CELL_STRIDE * gcell(G, G.N1, j, k));
+}
+
+/* polar: rotation by pi (kflip) */
+__global__ void k_bc_polar(ipgpu_geom G, double *dump)
+{
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (G.N1 + 2) * G.N3) return;
+  int i = n / G.N3, k = n % G.N3 + 1;
+  const int N2 = G.N2, N3 = G.N3;
+  if (N3 % 2 == 0) {
+    int kflip = ((k - 1) + (N3 / 2)) % N3 + 1;
+    copy_ghost(dump + CELL_STRIDE * gcell(G, i, 0, k), dump + CELL_STRIDE * gcell(G, i, 1, kflip));
+    copy_ghost(dump + CELL_STRIDE * gcell(G, i, N2 + 1, k), dump + CELL_STRIDE * gcell(G, i, N2, kflip));
+  } else {
+    int k1 = (k + (N3 / 2)) % N3, k2 = (k + (N3 / 2) + 1) % N3;
+    const double *a0 = dump + CELL_STRIDE * gcell(G, i, 1, k1), *a1 = dump + CELL_STRIDE * gcell(G, i, 1, k2);
+    const double *b0 = dump + CELL_STRIDE * gcell(G, i, N2, k1), *b1 = dump + CELL_STRIDE * gcell(G, i, N2, k2);
+    double *lo = dump + CELL_STRIDE * gcell(G, i, 0, k), *hi = dump + CELL_STRIDE * gcell(G, i, N2 + 1, k);
+    const int fs[9] = {0, 1, 2, 3, 4, 5, 8, S_RHO, S_UU};
+    for (int q = 0; q < 9; q++) {
+      lo[fs[q]] = (a0[fs[q]] + a1[fs[q]]) / 2.;
+      hi[fs[q]] = (b0[fs[q]] + b1[fs[q]]) / 2.;
+    }
+  }
+}
+
+/* azimuth: periodic */
+__global__ void k_bc_phi(ipgpu_geom G, double *dump)
+{
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (G.N1 + 2) * (G.N2 + 2)) return;
+  int i = n / (G.N2 + 2), j = n % (G.N2 + 2);
+  copy_ghost(dump + CELL_STRIDE * gcell(G, i, j, 0), dump + CELL_STRIDE * gcell(G, i, j, G.N3));
+  copy_ghost(dump + CELL_STRIDE * gcell(G, i, j, G.N3 + 1), dump + CELL_STRIDE * gcell(G, i, j, 1));
+}
+
+/* n_e, Theta_e (R_high/R_low, ELECTRONS = 2), sigma, beta over all cells:
+ * model/iharm/model.c:610-717 */
+__global__ void k_derive_thermo(size_t ncell, ipgpu_units U, double RHO_unit, double B_unit,
+                                double *__restrict__ dump)
+{
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  double *d = dump + CELL_STRIDE * c;
+  double rho = d[S_RHO], uu = d[S_UU];
+  d[6] = rho * RHO_unit / (MP + ME) * 1.;
+  double bsq = d[8] / B_unit;
+  bsq = bsq * bsq;
+  double sigma_m = bsq / rho;
+  double beta_m = uu * (U.gam - 1.) / 0.5 / bsq;
+  double betasq = beta_m * beta_m / U.beta_crit / U.beta_crit;
+  double trat = U.trat_large * betasq / (1. + betasq) + U.trat_small / (1. + betasq);
+  double Tu = (MP / ME) * (U.game - 1.) * (U.gamp - 1.) / ((U.gamp - 1.) + (U.game - 1.) * trat);
+  double th = Tu * uu / rho;
+  d[7] = fmax(th, 1.e-3);
+  d[9] = fmax(sigma_m, SMALL);
+  d[S_BETA] = fmax(beta_m, SMALL);
+}
+
+__global__ void k_unpack(size_t ncell, const double *__restrict__ dump, double *__restrict__ soa11)
+{
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const double *d = dump + CELL_STRIDE * c;
+  for (int f = 0; f < 10; f++) soa11[f * ncell + c] = d[f];
+  soa11[10 * ncell + c] = d[S_BETA];
+}
+
+}  // namespace
+
+/* ------------------------------------------------------------------ C-ABI */
+
+struct ipgpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  char err[512] = {0};
+  ipgpu_geom geom{};
+  bool has_geom = false;
+  ipgpu_model model{};
+  bool has_model = false;
+  double *d_dump = nullptr;
+  size_t dump_cells = 0;
+  double *d_traj = nullptr;
+  size_t traj_doubles = 0;
+  int *d_nturns = nullptr;
+  size_t nturns_ints = 0;
+  int *d_nstep_ray = nullptr, *d_flag_ray = nullptr;
+  double *d_out7 = nullptr;
+  double *d_image = nullptr, *d_imageS = nullptr, *d_taus = nullptr;
+  int *d_nstep_pix = nullptr, *d_flag_pix = nullptr;
+  size_t cap_nstep_ray = 0, cap_flag_ray = 0, cap_out7 = 0, cap_image = 0, cap_taus = 0;
+  size_t cap_nstep_pix = 0, cap_flag_pix = 0, cap_imageS = 0;
+  int *d_status = nullptr;
+  int last_npix = 0, last_unpol = 0, have_result = 0;
+  float ms_geo = 0, ms_xfer = 0, ms_total = 0;
+  int bad_i = -1, bad_j = -1;
+  int traj_cap_steps = 8192;
+};
+
+#define HIPCHK(ctx, call)                                                                    \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      snprintf((ctx)->err, sizeof((ctx)->err), "%s failed: %s", #call, hipGetErrorString(e_)); \
+      return IPGPU_ERR_HIP;                                                                  \
+    }                                                                                        \
+  } while (0)
+
+static int set_err(ipgpu_ctx *ctx, int code, const char *msg)
+{
+  if (ctx) snprintf(ctx->err, sizeof(ctx->err), "%s", msg);
+  return code;
+}
+
+template <typename T>
+static int grow(ipgpu_ctx *ctx, T *&ptr, size_t &cap, size_t need)
+{
+  if (need <= cap && ptr) return 0;
+  if (ptr) HIPCHK(ctx, hipFree(ptr));
+  ptr = nullptr;
+  cap = 0;
+  HIPCHK(ctx, hipMalloc((void **)&ptr, need * sizeof(T)));
+  cap = need;
+  return 0;
+}
+
+extern "C" {
+
+int ipgpu_create(ipgpu_ctx **out, int device)
+{
+  if (!out) return IPGPU_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return IPGPU_ERR_HIP;
+  if (device < 0 || device >= ndev) return IPGPU_ERR_ARG;
+  ipgpu_ctx *ctx = new ipgpu_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return IPGPU_ERR_HIP;
+  }
+  for (int q = 0; q < 4; q++)
+    if (hipEventCreate(&ctx->ev[q]) != hipSuccess) {
+      delete ctx;
+      return IPGPU_ERR_HIP;
+    }
+  if (hipMalloc((void **)&ctx->d_status, NSTATUS * sizeof(int)) != hipSuccess) {
+    delete ctx;
+    return IPGPU_ERR_HIP;
+  }
+  *out = ctx;
+  return IPGPU_OK;
+}
+
+void ipgpu_destroy(ipgpu_ctx *ctx)
+{
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  void *ptrs[] = {ctx->d_dump, ctx->d_traj, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
+                  ctx->d_out7, ctx->d_image, ctx->d_imageS, ctx->d_taus, ctx->d_nstep_pix,
+                  ctx->d_flag_pix, ctx->d_status};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  for (int q = 0; q < 4; q++)
+    if (ctx->ev[q]) hipEventDestroy(ctx->ev[q]);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *ipgpu_last_error(const ipgpu_ctx *ctx) { return ctx ? ctx->err : "null context"; }
+
+void *ipgpu_stream(ipgpu_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int ipgpu_set_geometry(ipgpu_ctx *ctx, const ipgpu_geom *g)
+{
+  if (!ctx || !g) return IPGPU_ERR_ARG;
+  if (g->metric != IPGPU_METRIC_MKS && g->metric != IPGPU_METRIC_FMKS
+      && g->metric != IPGPU_METRIC_EMINKOWSKI)
+    return set_err(ctx, IPGPU_ERR_ARG, "unsupported metric (MKS, FMKS, eMinkowski)");
+  if (g->N1 <= 0 || g->N2 <= 0 || g->N3 <= 0) return set_err(ctx, IPGPU_ERR_ARG, "bad grid size");
+  ctx->geom = *g;
+  ctx->has_geom = true;
+  return IPGPU_OK;
+}
+
+int ipgpu_set_model(ipgpu_ctx *ctx, const ipgpu_model *m)
+{
+  if (!ctx || !m) return IPGPU_ERR_ARG;
+  if (m->kind != IPGPU_MODEL_IHARM && m->kind != IPGPU_MODEL_SPHERE)
+    return set_err(ctx, IPGPU_ERR_ARG, "unsupported model kind");
+  ctx->model = *m;
+  ctx->has_model = true;
+  return IPGPU_OK;
+}
+
+static size_t ghosted_cells(const ipgpu_geom &g)
+{
+  return (size_t)(g.N1 + 2) * (g.N2 + 2) * (g.N3 + 2);
+}
+
+static int ensure_dump(ipgpu_ctx *ctx)
+{
+  size_t nc = ghosted_cells(ctx->geom);
+  if (ctx->d_dump && ctx->dump_cells == nc) return 0;
+  if (ctx->d_dump) HIPCHK(ctx, hipFree(ctx->d_dump));
+  ctx->d_dump = nullptr;
+  HIPCHK(ctx, hipMalloc((void **)&ctx->d_dump, nc * CELL_STRIDE * sizeof(double)));
+  ctx->dump_cells = nc;
+  return 0;
+}
+
+int ipgpu_upload_fields(ipgpu_ctx *ctx, const double *const fields[IPGPU_NFIELDS], size_t nc)
+{
+  if (!ctx || !fields) return IPGPU_ERR_ARG;
+  if (!ctx->has_geom) return set_err(ctx, IPGPU_ERR_STATE, "geometry not set");
+  if (nc != ghosted_cells(ctx->geom)) return set_err(ctx, IPGPU_ERR_ARG, "ncells_ghosted mismatch");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int rc = ensure_dump(ctx);
+  if (rc) return rc;
+  double *soa = nullptr;
+  HIPCHK(ctx, hipMalloc((void **)&soa, nc * IPGPU_NFIELDS * sizeof(double)));
+  for (int f = 0; f < IPGPU_NFIELDS; f++) {
+    if (!fields[f]) {
+      hipFree(soa);
+      return set_err(ctx, IPGPU_ERR_ARG, "null field pointer");
+    }
+    HIPCHK(ctx, hipMemcpyAsync(soa + f * nc, fields[f], nc * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  }
+  k_pack<<<(unsigned)((nc + 255) / 256), 256, 0, ctx->stream>>>(nc, soa, ctx->d_dump);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipFree(soa));
+  return IPGPU_OK;
+}
+
+int ipgpu_upload_prims(ipgpu_ctx *ctx, const double *prims, int nprim, const ipgpu_units *u)
+{
+  if (!ctx || !prims || !u || nprim < 8) return IPGPU_ERR_ARG;
+  if (!ctx->has_geom) return set_err(ctx, IPGPU_ERR_STATE, "geometry not set");
+  const ipgpu_geom &G = ctx->geom;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int rc = ensure_dump(ctx);
+  if (rc) return rc;
+  size_t nc = ghosted_cells(G);
+  size_t nin = (size_t)G.N1 * G.N2 * G.N3;
+  double *dp = nullptr;
+  HIPCHK(ctx, hipMalloc((void **)&dp, nin * nprim * sizeof(double)));
+  HIPCHK(ctx, hipMemcpyAsync(dp, prims, nin * nprim * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  /* set_units, model/iharm/model.c:595-608 */
+  double MBH = u->MBH_solar * 1.989e33;
+  double L_unit = 6.6742e-8 * MBH / (CL * CL);
+  double RHO_unit = u->M_unit / pow(L_unit, 3);
+  double B_unit = CL * sqrt(4. * M_PI * RHO_unit);
+  k_derive_b<<<(unsigned)((nin + 127) / 128), 128, 0, ctx->stream>>>(G, B_unit, nprim, dp, ctx->d_dump);
+  int nr = G.N2 * G.N3, np = (G.N1 + 2) * G.N3, nf = (G.N1 + 2) * (G.N2 + 2);
+  k_bc_radial<<<(nr + 255) / 256, 256, 0, ctx->stream>>>(G, ctx->d_dump);
+  k_bc_polar<<<(np + 255) / 256, 256, 0, ctx->stream>>>(G, ctx->d_dump);
+  k_bc_phi<<<(nf + 255) / 256, 256, 0, ctx->stream>>>(G, ctx->d_dump);
+  k_derive_thermo<<<(unsigned)((nc + 255) / 256), 256, 0, ctx->stream>>>(nc, *u, RHO_unit, B_unit, ctx->d_dump);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipFree(dp));
+  return IPGPU_OK;
+}
+
+int ipgpu_download_fields(ipgpu_ctx *ctx, double *out11, size_t nc)
+{
+  if (!ctx || !out11) return IPGPU_ERR_ARG;
+  if (!ctx->d_dump || nc != ctx->dump_cells) return set_err(ctx, IPGPU_ERR_STATE, "no dump on device");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  double *soa = nullptr;
+  HIPCHK(ctx, hipMalloc((void **)&soa, nc * 11 * sizeof(double)));
+  k_unpack<<<(unsigned)((nc + 255) / 256), 256, 0, ctx->stream>>>(nc, ctx->d_dump, soa);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipMemcpyAsync(out11, soa, nc * 11 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipFree(soa));
+  return IPGPU_OK;
+}
+
+static int run_trace(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int cap, bool turns)
+{
+  size_t need = (size_t)nrays * cap * REC;
+  int rc = grow(ctx, ctx->d_traj, ctx->traj_doubles, need);
+  if (rc) return rc;
+  if (turns) {
+    rc = grow(ctx, ctx->d_nturns, ctx->nturns_ints, (size_t)nrays * cap);
+    if (rc) return rc;
+  }
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
+  k_trace<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, P, nrays, cap, ctx->d_traj,
+                                                   ctx->d_nstep_ray, turns ? ctx->d_nturns : nullptr,
+                                                   ctx->d_status);
+  HIPCHK(ctx, hipGetLastError());
+  return 0;
+}
+
+int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *imageS,
+                 double *taus, int *nstep, int *flag)
+{
+  if (!ctx || !img) return IPGPU_ERR_ARG;
+  if (!ctx->has_geom || !ctx->has_model) return set_err(ctx, IPGPU_ERR_STATE, "geometry/model not set");
+  if (ctx->model.kind == IPGPU_MODEL_IHARM && !ctx->d_dump)
+    return set_err(ctx, IPGPU_ERR_STATE, "no fluid dump uploaded");
+  const ipgpu_image &P = *img;
+  if (P.nx <= 0 || P.ny <= 0 || P.maxnstep < 2) return set_err(ctx, IPGPU_ERR_ARG, "bad image size or maxnstep");
+  if (P.emission_type != 4 && P.emission_type != 1)
+    return set_err(ctx, IPGPU_ERR_ARG, "emission_type must be 4 (Dexter thermal) or 1 (Pandya thermal)");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int tiles = ((P.nx + TILE - 1) / TILE) * ((P.ny + TILE - 1) / TILE);
+  int nrays = tiles * WAVE;
+  size_t npix = (size_t)P.nx * P.ny;
+  int rc;
+  if ((rc = grow(ctx, ctx->d_nstep_ray, ctx->cap_nstep_ray, (size_t)nrays))) return rc;
+  if ((rc = grow(ctx, ctx->d_flag_ray, ctx->cap_flag_ray, (size_t)nrays))) return rc;
+  if ((rc = grow(ctx, ctx->d_out7, ctx->cap_out7, (size_t)nrays * 7))) return rc;
+  if ((rc = grow(ctx, ctx->d_image, ctx->cap_image, npix))) return rc;
+  if ((rc = grow(ctx, ctx->d_taus, ctx->cap_taus, npix))) return rc;
+  if ((rc = grow(ctx, ctx->d_nstep_pix, ctx->cap_nstep_pix, npix))) return rc;
+  if ((rc = grow(ctx, ctx->d_flag_pix, ctx->cap_flag_pix, npix))) return rc;
+  if ((rc = grow(ctx, ctx->d_imageS, ctx->cap_imageS, npix * 5))) return rc;
+
+  bool turns = P.target_nturns >= 0;
+  int cap = ctx->traj_cap_steps < P.maxnstep ? ctx->traj_cap_steps : P.maxnstep;
+  HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+  if ((rc = run_trace(ctx, P, nrays, cap, turns))) return rc;
+  int status[NSTATUS];
+  HIPCHK(ctx, hipMemcpyAsync(status, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (status[0] > 0) {
+    /* some rays outgrew the per-ray record capacity: retrace with room for
+     * the longest one (bounded by maxnstep) */
+    std::vector<int> ns(nrays);
+    HIPCHK(ctx, hipMemcpy(ns.data(), ctx->d_nstep_ray, nrays * sizeof(int), hipMemcpyDeviceToHost));
+    int mx = 0;
+    for (int v : ns) mx = v > mx ? v : mx;
+    cap = mx + 1 < P.maxnstep ? mx + 1 : P.maxnstep;
+    ctx->traj_cap_steps = cap;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    if ((rc = run_trace(ctx, P, nrays, cap, turns))) return rc;
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+  int init[NSTATUS] = {0, 0x7fffffff, 0x7fffffff, 0};
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_status, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
+  k_transfer<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, ctx->d_traj,
+                                                      ctx->d_nstep_ray, turns ? ctx->d_nturns : nullptr,
+                                                      ctx->d_dump, ctx->d_out7, ctx->d_flag_ray,
+                                                      ctx->d_status);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+  double f3 = pow(P.freqcgs, 3);
+  k_finalize<<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(P, nrays, f3, ctx->d_out7, ctx->d_flag_ray,
+                                                           ctx->d_nstep_ray, ctx->d_image, ctx->d_imageS,
+                                                           ctx->d_taus, ctx->d_nstep_pix, ctx->d_flag_pix,
+                                                           ctx->d_status);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(status, ctx->d_status, sizeof(status), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  hipEventElapsedTime(&ctx->ms_geo, ctx->ev[0], ctx->ev[1]);
+  hipEventElapsedTime(&ctx->ms_xfer, ctx->ev[1], ctx->ev[2]);
+  hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[3]);
+  ctx->last_npix = (int)npix;
+  ctx->last_unpol = P.only_unpolarized;
+  ctx->have_result = 1;
+  if (status[1] != 0x7fffffff) {
+    int p = status[1] - 1;
+    ctx->bad_i = p / P.ny;
+    ctx->bad_j = p % P.ny;
+    snprintf(ctx->err, sizeof(ctx->err), "MaxNStep exceeded in pixel %d %d", ctx->bad_i, ctx->bad_j);
+    return IPGPU_ERR_MAXNSTEP;
+  }
+  if (status[2] != 0x7fffffff) {
+    int p = status[2] - 1;
+    ctx->bad_i = p / P.ny;
+    ctx->bad_j = p % P.ny;
+    snprintf(ctx->err, sizeof(ctx->err), "NaN in image at pixel %d %d", ctx->bad_i, ctx->bad_j);
+    return IPGPU_ERR_NAN;
+  }
+  return ipgpu_fetch(ctx, image, imageS, taus, nstep, flag);
+}
+
+int ipgpu_fetch(ipgpu_ctx *ctx, double *image, double *imageS, double *taus, int *nstep, int *flag)
+{
+  if (!ctx) return IPGPU_ERR_ARG;
+  if (!ctx->have_result) return set_err(ctx, IPGPU_ERR_STATE, "no render result");
+  size_t npix = ctx->last_npix;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (image) HIPCHK(ctx, hipMemcpyAsync(image, ctx->d_image, npix * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (imageS && !ctx->last_unpol)
+    HIPCHK(ctx, hipMemcpyAsync(imageS, ctx->d_imageS, npix * 5 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (taus) HIPCHK(ctx, hipMemcpyAsync(taus, ctx->d_taus, npix * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (nstep) HIPCHK(ctx, hipMemcpyAsync(nstep, ctx->d_nstep_pix, npix * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  if (flag) HIPCHK(ctx, hipMemcpyAsync(flag, ctx->d_flag_pix, npix * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return IPGPU_OK;
+}
+
+int ipgpu_bad_pixel(const ipgpu_ctx *ctx, int *i, int *j)
+{
+  if (!ctx || !i || !j) return IPGPU_ERR_ARG;
+  *i = ctx->bad_i;
+  *j = ctx->bad_j;
+  return IPGPU_OK;
+}
+
+int ipgpu_kernel_times(const ipgpu_ctx *ctx, double *g, double *x, double *t)
+{
+  if (!ctx) return IPGPU_ERR_ARG;
+  if (g) *g = ctx->ms_geo;
+  if (x) *x = ctx->ms_xfer;
+  if (t) *t = ctx->ms_total;
+  return IPGPU_OK;
+}
+
+/* native_coord + root_find, src/coordinates.c:375-442 */
+int ipgpu_native_coord(const ipgpu_geom *G, double r, double thd, double phid, double X[4])
+{
+  if (!G || !X) return IPGPU_ERR_ARG;
+  if (G->metric == IPGPU_METRIC_EMINKOWSKI) {
+    X[0] = 1;
+    X[1] = log(r);
+    X[2] = thd / 180 * M_PI;
+    X[3] = phid / 180 * M_PI;
+    return IPGPU_OK;
+  }
+  double th = thd / 180. * M_PI;
+  double x1 = log(r), xa, xb, xc;
+  if (th < M_PI / 2.) {
+    xa = G->cstartx[2];
+    xb = (G->cstopx[2] - G->cstartx[2]) / 2 + SMALL;
+  } else {
+    xa = (G->cstopx[2] - G->cstartx[2]) / 2 - SMALL;
+    xb = G->cstopx[2];
+  }
+  auto theta = [&](double x2) {
+    double rr, t;
+    bl_coord(*G, x1, x2, rr, t);
+    return t;
+  };
+  double tol = 1.e-9, tha = theta(xa), thb = theta(xb), thc;
+  X[0] = 0.0;
+  X[1] = x1;
+  X[3] = phid / 180. * M_PI;
+  if (fabs(tha - th) < tol) {
+    X[2] = xa;
+    return IPGPU_OK;
+  } else if (fabs(thb - th) < tol) {
+    X[2] = xb;
+    return IPGPU_OK;
+  }
+  xc = xa;
+  for (int i = 0; i < 1000; i++) {
+    xc = 0.5 * (xa + xb);
+    thc = theta(xc);
+    if ((thc - th) * (thb - th) < 0.) xa = xc;
+    else xb = xc;
+    if (fabs(theta(xc) - th) < tol) break;
+  }
+  X[2] = xc;
+  return IPGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,311 @@
+"""ctypes binding of libipgpu.so (include/ipgpu.h) and the host-side set-up
+the reference's main.c does before its pixel loop.
+
+The C-ABI is the product boundary; this module only marshals arguments.  It
+raises if the HIP library is not built: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "lib", "libipgpu.so")
+
+D4 = C.c_double * 4
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+
+METRIC_MKS, METRIC_FMKS, METRIC_EMINKOWSKI = 0, 2, 5
+MODEL_IHARM, MODEL_SPHERE = 1, 2
+NFIELDS = 10
+ERR_MAXNSTEP = -10
+
+# constants of src/constants.h
+CL, ME, MP, HPL, GNEWT = 2.99792458e10, 9.1093826e-28, 1.67262171e-24, 6.6260693e-27, 6.6742e-8
+MSUN, PC, JY, MUAS_PER_RAD, SIGMA_THOMSON = 1.989e33, 3.085678e18, 1.e-23, 2.06265e11, 0.665245873e-24
+
+
+class Geom(C.Structure):
+    _fields_ = [("metric", C.c_int), ("N1", C.c_int), ("N2", C.c_int), ("N3", C.c_int),
+                ("a", C.c_double), ("hslope", C.c_double), ("poly_xt", C.c_double),
+                ("poly_alpha", C.c_double), ("mks_smooth", C.c_double), ("poly_norm", C.c_double),
+                ("startx", D4), ("stopx", D4), ("dx", D4), ("cstartx", D4), ("cstopx", D4),
+                ("Rin", C.c_double), ("Rout", C.c_double), ("Rh", C.c_double),
+                ("rmin_geo", C.c_double), ("rmax_geo", C.c_double)]
+
+
+class Model(C.Structure):
+    _fields_ = [("kind", C.c_int), ("L_unit", C.c_double), ("th_beg", C.c_double),
+                ("sigma_cut", C.c_double), ("sigma_cut_high", C.c_double), ("sph_R0", C.c_double),
+                ("sph_Ne0", C.c_double), ("sph_Thetae0", C.c_double), ("sph_B0", C.c_double)]
+
+
+class Units(C.Structure):
+    _fields_ = [("M_unit", C.c_double), ("MBH_solar", C.c_double), ("trat_small", C.c_double),
+                ("trat_large", C.c_double), ("beta_crit", C.c_double), ("gam", C.c_double),
+                ("game", C.c_double), ("gamp", C.c_double)]
+
+
+class Image(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("Xcam", D4), ("fovx", C.c_double),
+                ("fovy", C.c_double), ("freq", C.c_double), ("freqcgs", C.c_double),
+                ("rotcam", C.c_double), ("xoff", C.c_double), ("yoff", C.c_double),
+                ("eps", C.c_double), ("maxnstep", C.c_int), ("old_centering", C.c_int),
+                ("only_unpolarized", C.c_int), ("qu_conv", C.c_int), ("stokes_floors", C.c_int),
+                ("target_nturns", C.c_int), ("subring_dtheta", C.c_int), ("counterjet", C.c_int),
+                ("emission_type", C.c_int)]
+
+
+class IpgpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("ipgpu error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("ipole_amd/lib/libipgpu.so is not built: run "
+                               "`python -c 'import __graft_entry__ as g; g.build()'` or make -C ipole_amd")
+        L = C.CDLL(LIBPATH)
+        vp = C.c_void_p
+        L.ipgpu_create.argtypes = [C.POINTER(vp), C.c_int]
+        L.ipgpu_destroy.argtypes = [vp]
+        L.ipgpu_destroy.restype = None
+        L.ipgpu_last_error.argtypes = [vp]
+        L.ipgpu_last_error.restype = C.c_char_p
+        L.ipgpu_set_geometry.argtypes = [vp, C.POINTER(Geom)]
+        L.ipgpu_set_model.argtypes = [vp, C.POINTER(Model)]
+        L.ipgpu_upload_fields.argtypes = [vp, C.POINTER(dp), C.c_size_t]
+        L.ipgpu_upload_prims.argtypes = [vp, dp, C.c_int, C.POINTER(Units)]
+        L.ipgpu_download_fields.argtypes = [vp, dp, C.c_size_t]
+        L.ipgpu_render.argtypes = [vp, C.POINTER(Image), dp, dp, dp, ip, ip]
+        L.ipgpu_fetch.argtypes = [vp, dp, dp, dp, ip, ip]
+        L.ipgpu_bad_pixel.argtypes = [vp, ip, ip]
+        L.ipgpu_kernel_times.argtypes = [vp, dp, dp, dp]
+        L.ipgpu_stream.argtypes = [vp]
+        L.ipgpu_stream.restype = vp
+        L.ipgpu_native_coord.argtypes = [C.POINTER(Geom), C.c_double, C.c_double, C.c_double, dp]
+        _lib = L
+    return _lib
+
+
+def P(a):
+    return a.ctypes.data_as(dp)
+
+
+def IP(a):
+    return a.ctypes.data_as(ip)
+
+
+# ------------------------------------------------------------------ set-up
+
+def iharm_geom(h, rmax_geo=100., rmin_geo=1.):
+    """Coordinate globals of an iharm dump header, as init_iharm_grid sets them
+    (model/iharm/model.c:979-1065) and init_model's horizon (:326)."""
+    g = Geom()
+    g.metric = METRIC_FMKS if h["metric"] in ("FMKS", "MMKS") else METRIC_MKS
+    g.N1, g.N2, g.N3 = h["N1"], h["N2"], h["N3"]
+    g.a, g.hslope = h["a"], h["hslope"]
+    if g.metric == METRIC_FMKS:
+        g.poly_xt, g.poly_alpha, g.mks_smooth = h["poly_xt"], h["poly_alpha"], h["mks_smooth"]
+        g.poly_norm = 0.5 * np.pi * 1. / (1. + 1. / (g.poly_alpha + 1.) * 1. / g.poly_xt ** g.poly_alpha)
+    g.Rin, g.Rout = h["Rin"], h["Rout"]
+    for mu in range(1, 4):
+        g.startx[mu] = h["startx"][mu]
+        g.dx[mu] = h["dx"][mu]
+    g.rmax_geo = min(rmax_geo, g.Rout)
+    g.rmin_geo = max(rmin_geo, g.Rin)
+    g.stopx[0] = 1.
+    g.stopx[1] = g.startx[1] + g.N1 * g.dx[1]
+    g.stopx[2] = g.startx[2] + g.N2 * g.dx[2]
+    g.stopx[3] = g.startx[3] + g.N3 * g.dx[3]
+    g.cstopx[1] = np.log(g.Rout)
+    g.cstopx[2] = 1.0
+    g.cstopx[3] = 2 * np.pi
+    g.Rh = 1 + np.sqrt(1. - g.a * g.a)
+    return g
+
+
+def iharm_model(MBH=6.2e9, sigma_cut=1., sigma_cut_high=-1., th_beg=0.0174):
+    m = Model()
+    m.kind = MODEL_IHARM
+    m.L_unit = GNEWT * MBH * MSUN / (CL * CL)
+    m.th_beg, m.sigma_cut, m.sigma_cut_high = th_beg, sigma_cut, sigma_cut_high
+    return m
+
+
+def iharm_units(M_unit, MBH=6.2e9, trat_small=1., trat_large=40., beta_crit=1., gam=13. / 9):
+    u = Units()
+    u.M_unit, u.MBH_solar = M_unit, MBH
+    u.trat_small, u.trat_large, u.beta_crit = trat_small, trat_large, beta_crit
+    u.gam, u.game, u.gamp = gam, 1.333333333333333, 1.666666666666667
+    return u
+
+
+def sphere_setup(R0=100., tau0=1.e-5, Thetae0=10., beta0=20., gam=13. / 9, tp_over_te=3., MBH=4.1e6):
+    """Isothermal sphere of model/sphere/model.c:49-117 (eMinkowski)."""
+    m = Model()
+    m.kind = MODEL_SPHERE
+    m.L_unit = GNEWT * MBH * MSUN / (CL * CL)
+    m.sph_R0 = R0
+    m.sph_Ne0 = tau0 / SIGMA_THOMSON / R0 / m.L_unit
+    theta_unit = MP / ME * (gam - 1.) / (1. + tp_over_te)
+    m.sph_B0 = CL * np.sqrt(8 * np.pi * (gam - 1.) * (MP + ME) / beta0) * np.sqrt(m.sph_Ne0 * Thetae0) / np.sqrt(theta_unit)
+    m.sph_Thetae0 = Thetae0
+    g = Geom()
+    g.metric = METRIC_EMINKOWSKI
+    g.N1, g.N2, g.N3 = 300, 100, 1
+    g.dx[0], g.dx[1], g.dx[2], g.dx[3] = 0.1, R0 / 300, np.pi / 100, 2 * np.pi
+    g.stopx[0] = 1.
+    for mu in (1, 2, 3):
+        g.stopx[mu] = g.startx[mu] + (g.N1, g.N2, g.N3)[mu - 1] * g.dx[mu]
+    for mu in range(4):
+        g.cstartx[mu] = g.startx[mu]
+        g.cstopx[mu] = g.stopx[mu]
+    g.rmin_geo = 0.
+    g.rmax_geo = max(100., R0)
+    return g, m
+
+
+def native_coord(geom, r, th_deg, phi_deg):
+    X = np.zeros(4)
+    rc = lib().ipgpu_native_coord(C.byref(geom), r, th_deg, phi_deg, P(X))
+    if rc:
+        raise IpgpuError(rc, "native_coord")
+    return X
+
+
+def make_image(geom, L_unit, nx=160, ny=160, rcam=1000., thetacam=90., phicam=0., rotcam=0.,
+               fovx_dsource=0., fovy_dsource=0., dx=0., dy=0., dsource=16.9e6, freqcgs=230e9,
+               eps=0.01, maxnstep=50000, old_centering=0, only_unpolarized=0, qu_conv=0,
+               stokes_floors=0, target_nturns=-1, subring_dtheta=0, counterjet=0, emission_type=4,
+               xoff=0., yoff=0.):
+    """Camera set-up of src/main.c:125-174.  Returns (Image, scale, info)."""
+    im = Image()
+    im.nx, im.ny = nx, ny
+    X = native_coord(geom, rcam, thetacam, phicam)
+    for mu in range(4):
+        im.Xcam[mu] = X[mu]
+    im.freq = freqcgs * HPL / (ME * CL * CL)
+    im.freqcgs = freqcgs
+    im.rotcam = rotcam * np.pi / 180.
+    Dsource = dsource * PC
+    fov_to_d = Dsource / L_unit / MUAS_PER_RAD
+    if fovx_dsource != 0.0:
+        fx, fy = fovx_dsource, fovy_dsource
+    elif dx != 0.0:
+        fx, fy = dx / fov_to_d, dy / fov_to_d
+    else:
+        fx = fy = 160.
+    DX, DY = fx * fov_to_d, fy * fov_to_d
+    im.fovx, im.fovy = DX / rcam, DY / rcam
+    scale = (DX * L_unit / nx) * (DY * L_unit / ny) / (Dsource * Dsource) / JY
+    im.xoff, im.yoff = xoff, yoff
+    im.eps, im.maxnstep = eps, maxnstep
+    im.old_centering, im.only_unpolarized, im.qu_conv = old_centering, only_unpolarized, qu_conv
+    im.stokes_floors, im.target_nturns, im.subring_dtheta = stokes_floors, target_nturns, subring_dtheta
+    im.counterjet, im.emission_type = counterjet, emission_type
+    return im, scale, dict(DX=DX, DY=DY, fovx_dsource=fx, fovy_dsource=fy, Dsource=Dsource)
+
+
+# ------------------------------------------------------------------ context
+
+class Context:
+    """One GPU context (ipgpu_create .. ipgpu_destroy)."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        self.h = C.c_void_p()
+        rc = self.L.ipgpu_create(C.byref(self.h), device)
+        if rc:
+            raise IpgpuError(rc, "ipgpu_create(device=%d)" % device)
+
+    def _chk(self, rc):
+        if rc:
+            raise IpgpuError(rc, self.L.ipgpu_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.L.ipgpu_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_geometry(self, g):
+        self.geom = g
+        self._chk(self.L.ipgpu_set_geometry(self.h, C.byref(g)))
+
+    def set_model(self, m):
+        self.model = m
+        self._chk(self.L.ipgpu_set_model(self.h, C.byref(m)))
+
+    def upload_fields(self, fields10):
+        """fields10: array-like of 10 ghosted (N1+2, N2+2, N3+2) float64 arrays."""
+        arrs = [np.ascontiguousarray(f, dtype=np.float64) for f in fields10]
+        ptrs = (dp * NFIELDS)(*[P(a) for a in arrs])
+        self._chk(self.L.ipgpu_upload_fields(self.h, ptrs, arrs[0].size))
+
+    def upload_prims(self, prims, units):
+        pr = np.ascontiguousarray(prims, dtype=np.float64)
+        self._chk(self.L.ipgpu_upload_prims(self.h, P(pr), pr.shape[-1], C.byref(units)))
+
+    def download_fields(self):
+        g = self.geom
+        nc = (g.N1 + 2) * (g.N2 + 2) * (g.N3 + 2)
+        out = np.zeros(11 * nc)
+        self._chk(self.L.ipgpu_download_fields(self.h, P(out), nc))
+        return out.reshape(11, g.N1 + 2, g.N2 + 2, g.N3 + 2)
+
+    def render(self, image, fetch=True):
+        """Renders one image.  With fetch, returns dict of host arrays in the
+        reference layout; otherwise results stay on the device."""
+        if not fetch:
+            rc = self.L.ipgpu_render(self.h, C.byref(image), None, None, None, None, None)
+            self._chk(rc)
+            return None
+        n = image.nx * image.ny
+        out = dict(image=np.zeros(n), imageS=np.zeros(5 * n), taus=np.zeros(n),
+                   nstep=np.zeros(n, np.int32), flag=np.zeros(n, np.int32))
+        rc = self.L.ipgpu_render(self.h, C.byref(image), P(out["image"]), P(out["imageS"]),
+                                 P(out["taus"]), IP(out["nstep"]), IP(out["flag"]))
+        if rc == ERR_MAXNSTEP:
+            i, j = C.c_int(), C.c_int()
+            self.L.ipgpu_bad_pixel(self.h, C.byref(i), C.byref(j))
+            raise IpgpuError(rc, "MaxNStep exceeded in pixel %d %d" % (i.value, j.value))
+        self._chk(rc)
+        nx, ny = image.nx, image.ny
+        return dict(image=out["image"].reshape(nx, ny), imageS=out["imageS"].reshape(nx, ny, 5),
+                    taus=out["taus"].reshape(nx, ny), nstep=out["nstep"].reshape(nx, ny),
+                    flag=out["flag"].reshape(nx, ny))
+
+    def kernel_times(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self._chk(self.L.ipgpu_kernel_times(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def stream(self):
+        return self.L.ipgpu_stream(self.h)
+
+
+def image_stats(res, scale, freqcgs, dsource_pc, only_unpolarized=False):
+    """The summary print_image_stats computes (src/main.c:1001-1042)."""
+    img, S = res["image"], res["imageS"]
+    Ftot_unpol = float(np.sum(img * scale))
+    out = dict(Ftot_unpol=Ftot_unpol)
+    if not only_unpolarized:
+        Ftot = float(np.sum(S[..., 0] * scale))
+        Q, U, V = (float(np.sum(S[..., k] * scale)) for k in (1, 2, 3))
+        out.update(Ftot=Ftot, Qtot=Q, Utot=U, Vtot=V,
+                   LP=100. * np.sqrt(Q * Q + U * U) / Ftot, CP=100. * V / Ftot,
+                   nuLnu=4. * np.pi * Ftot * (dsource_pc * PC) ** 2 * JY * freqcgs)
+    return out

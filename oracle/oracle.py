@@ -356,3 +356,57 @@ def ref_geom(model, keys):
     g.Rin, g.Rout, g.Rh, g.rmin_geo, g.rmax_geo = o[27:32]
     g.N1, g.N2, g.N3 = int(o[32]), int(o[33]), int(o[34])
     return g, o[35]
+
+
+def iharm_geom(h, rmax_geo=100., rmin_geo=1.):
+    """Geometry globals as model/iharm/model.c:979-1065 and :326 set them."""
+    g = Geom()
+    g.metric = METRIC_FMKS if h["metric"] in ("FMKS", "MMKS") else METRIC_MKS
+    g.a, g.hslope = h["a"], h["hslope"]
+    if g.metric == METRIC_FMKS:
+        g.poly_xt, g.poly_alpha, g.mks_smooth = h["poly_xt"], h["poly_alpha"], h["mks_smooth"]
+        g.poly_norm = 0.5 * np.pi * 1. / (1. + 1. / (g.poly_alpha + 1.) * 1. / g.poly_xt ** g.poly_alpha)
+    g.Rin, g.Rout = h["Rin"], h["Rout"]
+    g.N1, g.N2, g.N3 = h["N1"], h["N2"], h["N3"]
+    for mu in range(1, 4):
+        g.startx[mu] = h["startx"][mu]
+        g.dx[mu] = h["dx"][mu]
+    g.rmax_geo = min(rmax_geo, g.Rout)
+    g.rmin_geo = max(rmin_geo, g.Rin)
+    g.stopx[0] = 1.
+    g.stopx[1] = g.startx[1] + g.N1 * g.dx[1]
+    g.stopx[2] = g.startx[2] + g.N2 * g.dx[2]
+    g.stopx[3] = g.startx[3] + g.N3 * g.dx[3]
+    g.cstopx[1] = np.log(g.Rout)
+    g.cstopx[2] = 1.0
+    g.cstopx[3] = 2 * np.pi
+    g.Rh = 1 + np.sqrt(1. - g.a * g.a)
+    return g
+
+
+def iharm_units(M_unit, MBH=6.2e9, trat_small=1., trat_large=40., beta_crit=1., gam=13. / 9):
+    u = Units()
+    u.M_unit, u.MBH_solar = M_unit, MBH
+    u.trat_small, u.trat_large, u.beta_crit = trat_small, trat_large, beta_crit
+    u.gam, u.game, u.gamp = gam, 1.333333333333333, 1.666666666666667
+    return u
+
+
+def iharm_oracle(h, prims, M_unit, MBH=6.2e9, trat_small=1., trat_large=40., beta_crit=1.,
+                 sigma_cut=1., th_beg=0.0174, rmax_geo=100.):
+    """Oracle set up for an iharm dump: derives the ghosted fields as
+    model/iharm/model.c does and returns (Oracle, fields11)."""
+    g = iharm_geom(h, rmax_geo=rmax_geo)
+    L = lib()
+    L.or_set_geom(C.byref(g))
+    u = iharm_units(M_unit, MBH, trat_small, trat_large, beta_crit, h["gam"])
+    nc = (g.N1 + 2) * (g.N2 + 2) * (g.N3 + 2)
+    out = np.zeros(11 * nc)
+    pr = f64(prims)
+    L.or_iharm_derive(P(pr), pr.shape[-1], C.byref(u), P(out))
+    m = Model()
+    m.kind = MODEL_IHARM
+    m.L_unit = GNEWT * MBH * MSUN / (CL * CL)
+    m.th_beg, m.sigma_cut, m.sigma_cut_high = th_beg, sigma_cut, -1.
+    fields = out[:10 * nc].copy()
+    return Oracle(g, m, fields), out.reshape(11, g.N1 + 2, g.N2 + 2, g.N3 + 2)

@@ -1,0 +1,112 @@
+/*
+ * CPU build of the GPU library's per-ray code (ipole_amd/csrc/ipg_pipeline.h)
+ * for checking the kernel arithmetic against the oracle in a container
+ * without a GPU.  Test infrastructure only: it is not reachable from the
+ * library's C-ABI and is never used as a fallback.
+ */
+#include <cstring>
+#include <vector>
+
+#include "ipg_pipeline.h"
+
+using namespace ipg;
+
+extern "C" {
+
+/* metric + derivatives + inverse at (X1, X2): 42 doubles (r th g gc d1 d2 sqrtmg) */
+void emu_metric(const ipgpu_geom *G, double X1, double X2, double *out)
+{
+  Metric m;
+  metric(*G, X1, X2, m);
+  out[0] = m.r;
+  out[1] = m.th;
+  memcpy(out + 2, m.g, 10 * sizeof(double));
+  memcpy(out + 12, m.gc, 10 * sizeof(double));
+  memcpy(out + 22, m.d1, 10 * sizeof(double));
+  memcpy(out + 32, m.d2, 10 * sizeof(double));
+  out[42] = m.sqrtmg;
+}
+
+void emu_accel(const ipgpu_geom *G, const double *X, const double *K, double *acc, double *M16)
+{
+  Metric m;
+  metric(*G, X[1], X[2], m);
+  geodesic_accel(m, K, acc);
+  double M[4][4];
+  transport_matrix(m, K, M);
+  memcpy(M16, M, 16 * sizeof(double));
+}
+
+void emu_bessel(double x, double *k3, double *emx)
+{
+  bessel_k012(x, k3, *emx);
+}
+
+/* fluid state + coefficients at X, K: Ucon Ucov Bcon Bcov ne thetae B (19),
+ * ju ku (2), pol coefficients (11) */
+void emu_point(const ipgpu_geom *G, const ipgpu_model *M, const double *dump, int emission_type,
+               const double *X, const double *K, double *out)
+{
+  Metric m;
+  metric(*G, X[1], X[2], m);
+  Fluid F;
+  fluid(*G, *M, dump, X, m, F);
+  memcpy(out, F.Ucon, 4 * sizeof(double));
+  memcpy(out + 4, F.Ucov, 4 * sizeof(double));
+  memcpy(out + 8, F.Bcon, 4 * sizeof(double));
+  memcpy(out + 12, F.Bcov, 4 * sizeof(double));
+  out[16] = F.ne;
+  out[17] = F.thetae;
+  out[18] = F.B;
+  Coef c;
+  coefficients(F, K, emission_type, true, out[19], out[20], c);
+  double *o = out + 21;
+  o[0] = c.jI; o[1] = c.jQ; o[2] = c.jU; o[3] = c.jV;
+  o[4] = c.aI; o[5] = c.aQ; o[6] = c.aU; o[7] = c.aV;
+  o[8] = c.rQ; o[9] = c.rU; o[10] = c.rV;
+}
+
+int emu_plasma_tetrad(const ipgpu_geom *G, const double *X, const double *U, const double *K,
+                      const double *B, double *E16)
+{
+  Metric m;
+  metric(*G, X[1], X[2], m);
+  double E[4][4];
+  int fl = plasma_tetrad(U, K, B, m.g, m.sqrtmg, E);
+  memcpy(E16, E, 16 * sizeof(double));
+  return fl;
+}
+
+/* One image, per pixel (i*ny + j): out7, nstep, flag.  cap = maxnstep. */
+int emu_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, const double *dump,
+               const int *pix, int npix, double *out7, int *nstep, int *flag)
+{
+  int status = 0;
+#pragma omp parallel
+  {
+    std::vector<double> rec((size_t)P->maxnstep * REC);
+    std::vector<int> nt(P->target_nturns >= 0 ? P->maxnstep : 1);
+#pragma omp for schedule(dynamic, 1)
+    for (int n = 0; n < npix; n++) {
+      int i = pix[2 * n], j = pix[2 * n + 1];
+      int *ntp = P->target_nturns >= 0 ? nt.data() : nullptr;
+      int ns = trace_ray(*G, *P, i, j, P->maxnstep, rec.data(), ntp);
+      nstep[n] = ns;
+      if (ns >= P->maxnstep - 1) {
+#pragma omp critical
+        status = -10;
+        continue;
+      }
+      flag[n] = transfer_ray(*G, *M, *P, ns, rec.data(), ntp, dump, out7 + 7 * (size_t)n);
+    }
+  }
+  return status;
+}
+
+/* Trajectory of one pixel: nstep+1 records of REC doubles. */
+int emu_trace(const ipgpu_geom *G, const ipgpu_image *P, int i, int j, double *rec)
+{
+  return trace_ray(*G, *P, i, j, P->maxnstep, rec, nullptr);
+}
+
+}

@@ -1,0 +1,88 @@
+"""ctypes binding of the CPU build of the GPU library's per-ray code
+(tests/kernel_emu/libkernel_emu.so).  Test infrastructure only."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from ipole_amd import ipgpu as G
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libkernel_emu.so")
+dp, ip = G.dp, G.ip
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+        L = C.CDLL(LIB)
+        L.emu_metric.argtypes = [C.POINTER(G.Geom), C.c_double, C.c_double, dp]
+        L.emu_accel.argtypes = [C.POINTER(G.Geom), dp, dp, dp, dp]
+        L.emu_bessel.argtypes = [C.c_double, dp, dp]
+        L.emu_point.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), dp, C.c_int, dp, dp, dp]
+        L.emu_plasma_tetrad.argtypes = [C.POINTER(G.Geom), dp, dp, dp, dp, dp]
+        L.emu_render.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), dp, ip,
+                                 C.c_int, dp, ip, ip]
+        L.emu_trace.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Image), C.c_int, C.c_int, dp]
+        _lib = L
+    return _lib
+
+
+def f64(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def pack_dump(fields10):
+    """Ghosted SoA fields (10, N1+2, N2+2, N3+2) -> 16-double cell records."""
+    f = np.asarray(fields10)[:10]
+    nc = f[0].size
+    d = np.zeros((nc, 16))
+    d[:, :10] = f.reshape(10, nc).T
+    return d.ravel()
+
+
+def metric(geom, X1, X2):
+    o = np.zeros(43)
+    lib().emu_metric(C.byref(geom), X1, X2, G.P(o))
+    return o
+
+
+def accel(geom, X, K):
+    a, M = np.zeros(4), np.zeros(16)
+    lib().emu_accel(C.byref(geom), G.P(f64(X)), G.P(f64(K)), G.P(a), G.P(M))
+    return a, M.reshape(4, 4)
+
+
+def bessel(x):
+    k, e = np.zeros(3), C.c_double()
+    lib().emu_bessel(x, G.P(k), C.byref(e))
+    return k, e.value
+
+
+def point(geom, model, dump, X, K, emission_type=4):
+    o = np.zeros(32)
+    d = dump if dump is not None else np.zeros(16)
+    lib().emu_point(C.byref(geom), C.byref(model), G.P(d), emission_type, G.P(f64(X)), G.P(f64(K)), G.P(o))
+    return o
+
+
+def render(geom, model, image, dump, pix=None):
+    if pix is None:
+        pix = np.array([(i, j) for i in range(image.nx) for j in range(image.ny)], np.int32)
+    pix = np.ascontiguousarray(pix, dtype=np.int32)
+    n = len(pix)
+    out7 = np.zeros((n, 7))
+    ns, fl = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    d = dump if dump is not None else np.zeros(16)
+    rc = lib().emu_render(C.byref(geom), C.byref(model), C.byref(image), G.P(d), G.IP(pix), n,
+                          G.P(out7), G.IP(ns), G.IP(fl))
+    return rc, out7, ns, fl
+
+
+def trace(geom, image, i, j):
+    rec = np.zeros(image.maxnstep * 8)
+    ns = lib().emu_trace(C.byref(geom), C.byref(image), i, j, G.P(rec))
+    return ns, rec[: 8 * (ns + 1)].reshape(ns + 1, 8)
