@@ -1,0 +1,146 @@
+"""Benchmark: polarized pixels/s of the per-pixel ray pipeline on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): a seeded
+synthetic iharm-format FMKS dump of 288x128x128 cells (a = 0.9375), resident
+in HBM after on-device derivation, imaged at 160x160 pixels, thetacam 163,
+230 GHz, R_high = 40, M_unit = 3e26, FP64.  One step = one full image
+(geodesic pass + polarized transfer pass + output layout) with the inputs
+already on the device; outputs stay on the device.
+
+Prints one JSON line (see DESIGN.md for the field definitions).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "polarized pixels/s on 160^2 FMKS dump: batched/mixed vs plain FP64, dFtot<=1e-4"
+PEAK_FP64 = 78.6  # TFLOP/s, MI355X FP64 vector
+C1 = dict(N=(288, 128, 128), a=0.9375, seed=1, M_unit=3e26, nx=160, ny=160, thetacam=163.,
+          freqcgs=230e9, trat_large=40.)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(h, prims, cfg, sample_stride, nthreads):
+    """The oracle (C + OpenMP restatement of the reference) on the host cores,
+    on a bounded, evenly strided sample of the same image's pixels."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    t0 = time.time()
+    orc, _ = O.iharm_oracle(h, prims, M_unit=cfg["M_unit"], trat_large=cfg["trat_large"])
+    log("cpu baseline: derivation %.1fs" % (time.time() - t0))
+    p = O.default_params(nx=cfg["nx"], ny=cfg["ny"], thetacam=cfg["thetacam"], freqcgs=cfg["freqcgs"])
+    cam = orc.camera(p)
+    pix = np.array([(i, j) for i in range(0, cfg["nx"], sample_stride)
+                    for j in range(0, cfg["ny"], sample_stride)], np.int32)
+    t0 = time.time()
+    rc, out7, ns, fl = orc.render_pixels(p, cam, pix, nthreads=nthreads)
+    dt = time.time() - t0
+    assert rc == 0
+    log("cpu baseline: %d pixels in %.1fs on %d threads" % (len(pix), dt, nthreads))
+    return dict(value=len(pix) / dt, unit="pixels/s", cores=nthreads, kind="port",
+                sample="%d of %d pixels (every %dth row and column) of the same 160x160 image, "
+                       "oracle C+OpenMP, %d threads" % (len(pix), cfg["nx"] * cfg["ny"], sample_stride,
+                                                       nthreads))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-stride", type=int, default=10)
+    ap.add_argument("--small", action="store_true", help="small dump for quick checks")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this path is data-parallel over images, and no "
+              "multi-GPU figure is produced (got --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from ipole_amd import ipgpu as G
+    from ipole_amd import synth
+
+    cfg = dict(C1)
+    if args.small:
+        cfg.update(N=(96, 64, 64), nx=64, ny=64)
+    t0 = time.time()
+    h, prims = synth.make_dump(*cfg["N"], a=cfg["a"], seed=cfg["seed"])
+    log("dump generated in %.1fs" % (time.time() - t0))
+    ctx = G.Context(0)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    ctx.set_geometry(g)
+    ctx.set_model(m)
+    ctx.upload_prims(prims, G.iharm_units(cfg["M_unit"], trat_large=cfg["trat_large"]))
+    im, scale, _ = G.make_image(g, m.L_unit, nx=cfg["nx"], ny=cfg["ny"], thetacam=cfg["thetacam"],
+                                freqcgs=cfg["freqcgs"])
+    npix = cfg["nx"] * cfg["ny"]
+
+    for w in range(args.warmup):
+        ctx.render(im, fetch=False)
+        log("warmup %d done: %.2f ms" % (w, ctx.kernel_times()[2]))
+    ms_geo, ms_xfer, ms_tot = [], [], []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ctx.render(im, fetch=False)   # synchronizes on the library's stream before returning
+        a, b, c = ctx.kernel_times()
+        ms_geo.append(a)
+        ms_xfer.append(b)
+        ms_tot.append(c)
+    wall = time.perf_counter() - t0
+    n_geo, n_rad = ctx.step_counts()
+    res = ctx.render(im)
+    st = G.image_stats(res, scale, cfg["freqcgs"], 16.9e6)
+    ctx.close()
+    ms_step = 1e3 * wall / args.steps
+    log("per step: wall %.3f ms, device %.3f ms (geodesic %.3f, transfer %.3f)" %
+        (ms_step, np.mean(ms_tot), np.mean(ms_geo), np.mean(ms_xfer)))
+    log("image: Ftot %.6g Jy, unpol %.6g Jy, LP %.3f%%, CP %.3f%%; geodesic steps %d, radiating %d" %
+        (st["Ftot"], st["Ftot_unpol"], st["LP"], st["CP"], n_geo, n_rad))
+
+    fl = json.load(open(os.path.join(ROOT, "ipole_amd", "flops_per_step.json")))
+    t_geo, t_xfer = np.mean(ms_geo) * 1e-3, np.mean(ms_xfer) * 1e-3
+    ach_geo = fl["F_geo"] * n_geo / t_geo / 1e12
+    ach_xfer = fl["F_pol"] * n_rad / t_xfer / 1e12
+    if t_xfer >= t_geo:
+        dom, ach = "k_transfer", ach_xfer
+    else:
+        dom, ach = "k_trace", ach_geo
+    roof = dict(bound="valu", achieved=round(ach, 4), peak=PEAK_FP64, unit="TFLOP/s",
+                frac=round(ach / PEAK_FP64, 5), traffic=None, kernel=dom,
+                note="FP64 vector ALU bound (no dense contraction on this path); algorithmic FLOP "
+                     "= F_geo*geodesic_steps (k_trace) or F_pol*radiating_steps (k_transfer), "
+                     "ipole_amd/flops_per_step.json; k_trace %.4f TFLOP/s, k_transfer %.4f TFLOP/s"
+                     % (ach_geo, ach_xfer))
+    cpu = None
+    if not args.no_cpu:
+        nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        cpu = cpu_baseline(h, prims, cfg, args.cpu_stride, nthreads)
+    value = npix / (np.mean(ms_tot) * 1e-3)
+    out = dict(metric=METRIC, value=round(value, 1), unit="pixels/s", n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=round(ms_step, 4), higher_is_better=True,
+               scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
+               config=dict(workload="plain FP64 GPU path: synthetic FMKS dump %dx%dx%d a=%g seed %d, "
+                                    "%dx%d px, thetacam %g, %g GHz, R_high %g, M_unit %g" %
+                                    (cfg["N"] + (cfg["a"], cfg["seed"], cfg["nx"], cfg["ny"],
+                                                 cfg["thetacam"], cfg["freqcgs"] / 1e9,
+                                                 cfg["trat_large"], cfg["M_unit"])),
+                           pixels=npix, Ftot_Jy=round(st["Ftot"], 9), geodesic_steps=n_geo,
+                           radiating_steps=n_rad, ms_device=round(float(np.mean(ms_tot)), 4),
+                           ms_geodesic=round(float(np.mean(ms_geo)), 4),
+                           ms_transfer=round(float(np.mean(ms_xfer)), 4)),
+               roofline=roof, cpu_baseline=cpu)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

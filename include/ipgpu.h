@@ -138,6 +138,11 @@ int ipgpu_bad_pixel(const ipgpu_ctx *ctx, int *i, int *j);
 int ipgpu_kernel_times(const ipgpu_ctx *ctx, double *ms_geodesic, double *ms_transfer,
                        double *ms_total);
 
+/* Work counts of the last render: geodesic steps summed over rays (the
+ * nstep of trace_geodesic) and radiating steps of the transfer pass (steps
+ * with radiating_region true, src/ipolarray.c:122). */
+int ipgpu_step_counts(const ipgpu_ctx *ctx, long long *geodesic_steps, long long *radiating_steps);
+
 /* The context's HIP stream (a hipStream_t), for callers that time or order
  * their own work against it. */
 void *ipgpu_stream(ipgpu_ctx *ctx);

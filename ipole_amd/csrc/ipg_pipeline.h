@@ -123,8 +123,10 @@ IPG_HD int trace_ray(const ipgpu_geom &G, const ipgpu_image &P, int pi, int pj, 
 /* out7 = Intensity, Is, Qs, Us, Vs, Tau, tauF (before the nu^3 scaling of
  * save_pixel); returns the oddflag bits. */
 IPG_HD int transfer_ray(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int ns,
-                        const double *rec, const int *nt, const double *dump, double out7[7])
+                        const double *rec, const int *nt, const double *dump, double out7[7],
+                        int &nrad)
 {
+  nrad = 0;
   const bool pol = !P.only_unpolarized;
   const double conv = Mo.L_unit * HPL / (ME * CL * CL);
   const double half_x2 = (G.cstopx[2] - G.cstartx[2]) / 2;
@@ -177,6 +179,7 @@ IPG_HD int transfer_ray(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_
       bl_coord(G, tf.X[1], tf.X[2], r, th);
     }
     if (radiating_region(G, Mo, r, th)) {
+      nrad++;
       if (!have_mf) {
         metric(G, tf.X[1], tf.X[2], m);
         have_mf = true;

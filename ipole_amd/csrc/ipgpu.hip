@@ -44,7 +44,8 @@ __device__ __forceinline__ void ray_pixel(int ray, int nx, int ny, int &i, int &
 
 __global__ __launch_bounds__(WAVE) void k_trace(ipgpu_geom G, ipgpu_image P, int nrays, int cap,
                                                 double *__restrict__ traj, int *__restrict__ nstep_out,
-                                                int *__restrict__ nturns_buf, int *__restrict__ status)
+                                                int *__restrict__ nturns_buf, int *__restrict__ status,
+                                                unsigned long long *__restrict__ counts)
 {
   int ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= nrays) return;
@@ -59,6 +60,7 @@ __global__ __launch_bounds__(WAVE) void k_trace(ipgpu_geom G, ipgpu_image P, int
   int nstep = trace_ray(G, P, pi, pj, cap, rec, nt);
   nstep_out[ray] = nstep;
   if (nstep >= cap) atomicAdd(&status[0], 1);
+  atomicAdd(&counts[0], (unsigned long long)nstep);
 }
 
 __global__ __launch_bounds__(WAVE) void k_transfer(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
@@ -67,7 +69,8 @@ __global__ __launch_bounds__(WAVE) void k_transfer(ipgpu_geom G, ipgpu_model Mo,
                                                    const int *__restrict__ nturns_buf,
                                                    const double *__restrict__ dump,
                                                    double *__restrict__ out7, int *__restrict__ flags,
-                                                   int *__restrict__ status)
+                                                   int *__restrict__ status,
+                                                   unsigned long long *__restrict__ counts)
 {
   int ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= nrays) return;
@@ -85,8 +88,10 @@ __global__ __launch_bounds__(WAVE) void k_transfer(ipgpu_geom G, ipgpu_model Mo,
   const double *rec = traj + (size_t)ray * cap * REC;
   const int *nt = nturns_buf ? nturns_buf + (size_t)ray * cap : nullptr;
   double res[7];
-  flags[ray] = transfer_ray(G, Mo, P, ns, rec, nt, dump, res);
+  int nrad;
+  flags[ray] = transfer_ray(G, Mo, P, ns, rec, nt, dump, res, nrad);
   for (int q = 0; q < 7; q++) o[q] = res[q];
+  atomicAdd(&counts[1], (unsigned long long)nrad);
 }
 
 /* save_pixel (src/main.c:885-910): reference layout, nu^3 scaling, QU sign */
@@ -286,6 +291,8 @@ struct ipgpu_ctx {
   size_t cap_nstep_ray = 0, cap_flag_ray = 0, cap_out7 = 0, cap_image = 0, cap_taus = 0;
   size_t cap_nstep_pix = 0, cap_flag_pix = 0, cap_imageS = 0;
   int *d_status = nullptr;
+  unsigned long long *d_counts = nullptr;
+  unsigned long long counts[2] = {0, 0};
   int last_npix = 0, last_unpol = 0, have_result = 0;
   float ms_geo = 0, ms_xfer = 0, ms_total = 0;
   int bad_i = -1, bad_j = -1;
@@ -339,7 +346,8 @@ int ipgpu_create(ipgpu_ctx **out, int device)
       delete ctx;
       return IPGPU_ERR_HIP;
     }
-  if (hipMalloc((void **)&ctx->d_status, NSTATUS * sizeof(int)) != hipSuccess) {
+  if (hipMalloc((void **)&ctx->d_status, NSTATUS * sizeof(int)) != hipSuccess
+      || hipMalloc((void **)&ctx->d_counts, 2 * sizeof(unsigned long long)) != hipSuccess) {
     delete ctx;
     return IPGPU_ERR_HIP;
   }
@@ -354,7 +362,7 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   void *ptrs[] = {ctx->d_dump, ctx->d_traj, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
                   ctx->d_out7, ctx->d_image, ctx->d_imageS, ctx->d_taus, ctx->d_nstep_pix,
-                  ctx->d_flag_pix, ctx->d_status};
+                  ctx->d_flag_pix, ctx->d_status, ctx->d_counts};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int q = 0; q < 4; q++)
@@ -484,9 +492,10 @@ static int run_trace(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int cap, b
     if (rc) return rc;
   }
   HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_counts, 0, 2 * sizeof(unsigned long long), ctx->stream));
   k_trace<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, P, nrays, cap, ctx->d_traj,
                                                    ctx->d_nstep_ray, turns ? ctx->d_nturns : nullptr,
-                                                   ctx->d_status);
+                                                   ctx->d_status, ctx->d_counts);
   HIPCHK(ctx, hipGetLastError());
   return 0;
 }
@@ -541,7 +550,7 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
   k_transfer<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, ctx->d_traj,
                                                       ctx->d_nstep_ray, turns ? ctx->d_nturns : nullptr,
                                                       ctx->d_dump, ctx->d_out7, ctx->d_flag_ray,
-                                                      ctx->d_status);
+                                                      ctx->d_status, ctx->d_counts);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
   double f3 = pow(P.freqcgs, 3);
@@ -552,6 +561,7 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(status, ctx->d_status, sizeof(status), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->counts, ctx->d_counts, sizeof(ctx->counts), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   hipEventElapsedTime(&ctx->ms_geo, ctx->ev[0], ctx->ev[1]);
   hipEventElapsedTime(&ctx->ms_xfer, ctx->ev[1], ctx->ev[2]);
@@ -589,6 +599,14 @@ int ipgpu_fetch(ipgpu_ctx *ctx, double *image, double *imageS, double *taus, int
   if (nstep) HIPCHK(ctx, hipMemcpyAsync(nstep, ctx->d_nstep_pix, npix * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   if (flag) HIPCHK(ctx, hipMemcpyAsync(flag, ctx->d_flag_pix, npix * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return IPGPU_OK;
+}
+
+int ipgpu_step_counts(const ipgpu_ctx *ctx, long long *geodesic_steps, long long *radiating_steps)
+{
+  if (!ctx) return IPGPU_ERR_ARG;
+  if (geodesic_steps) *geodesic_steps = (long long)ctx->counts[0];
+  if (radiating_steps) *radiating_steps = (long long)ctx->counts[1];
   return IPGPU_OK;
 }
 

@@ -1,0 +1,84 @@
+"""ctypes binding of the host HDF5 I/O (lib/libipole_h5.so, host/h5io.c):
+iharm-format dumps and the reference's image file layout."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "lib", "libipole_h5.so")
+dp = C.POINTER(C.c_double)
+D4 = C.c_double * 4
+
+
+class DumpHeader(C.Structure):
+    _fields_ = [("metric", C.c_char * 32), ("N1", C.c_int), ("N2", C.c_int), ("N3", C.c_int),
+                ("n_prim", C.c_int), ("gam", C.c_double), ("a", C.c_double), ("hslope", C.c_double),
+                ("Rin", C.c_double), ("Rout", C.c_double), ("poly_xt", C.c_double),
+                ("poly_alpha", C.c_double), ("mks_smooth", C.c_double), ("startx", D4), ("dx", D4),
+                ("t", C.c_double), ("dump_cadence", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("ipole_amd/lib/libipole_h5.so is not built (make -C ipole_amd)")
+        L = C.CDLL(LIBPATH)
+        L.ipole_h5_write_dump.argtypes = [C.c_char_p, C.POINTER(DumpHeader), dp]
+        L.ipole_h5_read_dump_header.argtypes = [C.c_char_p, C.POINTER(DumpHeader)]
+        L.ipole_h5_read_dump_prims.argtypes = [C.c_char_p, C.POINTER(DumpHeader), dp]
+        L.ipole_h5_read_image.argtypes = [C.c_char_p, C.c_int, C.c_int, dp, dp, dp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def header_struct(h):
+    s = DumpHeader()
+    s.metric = h["metric"].encode()
+    s.N1, s.N2, s.N3, s.n_prim = h["N1"], h["N2"], h["N3"], h.get("n_prim", 8)
+    for k in ("gam", "a", "hslope", "Rin", "Rout", "t", "dump_cadence"):
+        setattr(s, k, h[k])
+    for k in ("poly_xt", "poly_alpha", "mks_smooth"):
+        setattr(s, k, h.get(k, 0.))
+    for mu in range(4):
+        s.startx[mu] = h["startx"][mu]
+        s.dx[mu] = h["dx"][mu]
+    return s
+
+
+def write_dump(path, h, prims):
+    pr = np.ascontiguousarray(prims, dtype=np.float64)
+    s = header_struct(h)
+    s.n_prim = pr.shape[-1]
+    if lib().ipole_h5_write_dump(path.encode(), C.byref(s), pr.ctypes.data_as(dp)):
+        raise IOError("could not write dump %s" % path)
+
+
+def read_dump(path):
+    s = DumpHeader()
+    if lib().ipole_h5_read_dump_header(path.encode(), C.byref(s)):
+        raise IOError("could not read dump header %s" % path)
+    h = dict(metric=s.metric.decode(), N1=s.N1, N2=s.N2, N3=s.N3, n_prim=s.n_prim, gam=s.gam, a=s.a,
+             hslope=s.hslope, Rin=s.Rin, Rout=s.Rout, poly_xt=s.poly_xt, poly_alpha=s.poly_alpha,
+             mks_smooth=s.mks_smooth, startx=list(s.startx), dx=list(s.dx), t=s.t,
+             dump_cadence=s.dump_cadence)
+    prims = np.zeros((s.N1, s.N2, s.N3, s.n_prim))
+    if lib().ipole_h5_read_dump_prims(path.encode(), C.byref(s), prims.ctypes.data_as(dp)):
+        raise IOError("could not read prims %s" % path)
+    return h, prims
+
+
+def read_image(path, nx, ny, polarized=True):
+    image, taus = np.zeros((nx, ny)), np.zeros((nx, ny))
+    imageS = np.zeros((nx, ny, 5)) if polarized else None
+    Ft, Fu = C.c_double(0.), C.c_double(0.)
+    rc = lib().ipole_h5_read_image(path.encode(), nx, ny, image.ctypes.data_as(dp),
+                                   imageS.ctypes.data_as(dp) if polarized else None,
+                                   taus.ctypes.data_as(dp), C.byref(Ft), C.byref(Fu))
+    if rc:
+        raise IOError("could not read image %s" % path)
+    return dict(image=image, imageS=imageS, taus=taus, Ftot=Ft.value, Ftot_unpol=Fu.value)

@@ -1,0 +1,304 @@
+/*
+ * HDF5 I/O of the host: iharm-format dumps (read and write) and the image
+ * file layout of the reference (src/io.c:140-273, model/iharm/model.c:
+ * 1254-1301).  Plain C over libhdf5; no GPU code.
+ */
+#include "h5io.h"
+
+#include <hdf5.h>
+#include <stdio.h>
+#include <string.h>
+
+static int w_scalar(hid_t loc, const char *name, hid_t type, const void *v)
+{
+  hid_t sp = H5Screate(H5S_SCALAR);
+  hid_t ds = H5Dcreate2(loc, name, type, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+  if (ds < 0) {
+    H5Sclose(sp);
+    return -1;
+  }
+  herr_t e = H5Dwrite(ds, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, v);
+  H5Dclose(ds);
+  H5Sclose(sp);
+  return e < 0 ? -1 : 0;
+}
+
+static int w_str(hid_t loc, const char *name, const char *s, size_t len)
+{
+  hid_t t = H5Tcopy(H5T_C_S1);
+  H5Tset_size(t, len);
+  char buf[256] = {0};
+  strncpy(buf, s, sizeof(buf) - 1);
+  int rc = w_scalar(loc, name, t, buf);
+  H5Tclose(t);
+  return rc;
+}
+
+static int w_d(hid_t loc, const char *name, double v) { return w_scalar(loc, name, H5T_IEEE_F64LE, &v); }
+static int w_i(hid_t loc, const char *name, int v) { return w_scalar(loc, name, H5T_STD_I32LE, &v); }
+
+static int w_array(hid_t loc, const char *name, int rank, const hsize_t *dims, hid_t type,
+                   const void *data)
+{
+  hid_t sp = H5Screate_simple(rank, dims, NULL);
+  hid_t ds = H5Dcreate2(loc, name, type, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+  if (ds < 0) {
+    H5Sclose(sp);
+    return -1;
+  }
+  herr_t e = H5Dwrite(ds, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, data);
+  H5Dclose(ds);
+  H5Sclose(sp);
+  return e < 0 ? -1 : 0;
+}
+
+static int r_scalar(hid_t loc, const char *name, hid_t type, void *v)
+{
+  if (H5Lexists(loc, name, H5P_DEFAULT) <= 0) return -1;
+  hid_t ds = H5Dopen2(loc, name, H5P_DEFAULT);
+  if (ds < 0) return -1;
+  herr_t e = H5Dread(ds, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, v);
+  H5Dclose(ds);
+  return e < 0 ? -1 : 0;
+}
+
+/* Dump keys as model/iharm/model.c:860-1051 and :1895-1942 read them. */
+int ipole_h5_write_dump(const char *path, const ipole_dump_header *h, const double *prims)
+{
+  hid_t f = H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+  if (f < 0) return -1;
+  int rc = 0;
+  hid_t hd = H5Gcreate2(f, "header", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+  rc |= w_str(hd, "version", "ipole_amd-synthetic", 20);
+  rc |= w_str(hd, "metric", h->metric, 20);
+  rc |= w_i(hd, "n1", h->N1);
+  rc |= w_i(hd, "n2", h->N2);
+  rc |= w_i(hd, "n3", h->N3);
+  rc |= w_i(hd, "n_prim", h->n_prim);
+  rc |= w_d(hd, "gam", h->gam);
+  rc |= w_i(hd, "has_electrons", 0);
+  hid_t geom = H5Gcreate2(hd, "geom", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+  rc |= w_d(geom, "startx1", h->startx[1]);
+  rc |= w_d(geom, "startx2", h->startx[2]);
+  rc |= w_d(geom, "startx3", h->startx[3]);
+  rc |= w_d(geom, "dx1", h->dx[1]);
+  rc |= w_d(geom, "dx2", h->dx[2]);
+  rc |= w_d(geom, "dx3", h->dx[3]);
+  int fmks = (strcmp(h->metric, "FMKS") == 0 || strcmp(h->metric, "MMKS") == 0);
+  hid_t mk = H5Gcreate2(geom, fmks ? "mmks" : "mks", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+  rc |= w_d(mk, "a", h->a);
+  rc |= w_d(mk, "hslope", h->hslope);
+  rc |= w_d(mk, "Rin", h->Rin);
+  rc |= w_d(mk, "Rout", h->Rout);
+  if (fmks) {
+    rc |= w_d(mk, "poly_xt", h->poly_xt);
+    rc |= w_d(mk, "poly_alpha", h->poly_alpha);
+    rc |= w_d(mk, "mks_smooth", h->mks_smooth);
+  }
+  H5Gclose(mk);
+  H5Gclose(geom);
+  H5Gclose(hd);
+  rc |= w_d(f, "dump_cadence", h->dump_cadence);
+  rc |= w_d(f, "t", h->t);
+  hsize_t dims[4] = {(hsize_t)h->N1, (hsize_t)h->N2, (hsize_t)h->N3, (hsize_t)h->n_prim};
+  rc |= w_array(f, "prims", 4, dims, H5T_IEEE_F64LE, prims);
+  H5Fclose(f);
+  return rc ? -1 : 0;
+}
+
+int ipole_h5_read_dump_header(const char *path, ipole_dump_header *h)
+{
+  memset(h, 0, sizeof(*h));
+  hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+  if (f < 0) return -1;
+  int rc = 0;
+  hid_t s20 = H5Tcopy(H5T_C_S1);
+  H5Tset_size(s20, 20);
+  char metric[21] = {0};
+  rc |= r_scalar(f, "/header/metric", s20, metric);
+  H5Tclose(s20);
+  strncpy(h->metric, metric, sizeof(h->metric) - 1);
+  rc |= r_scalar(f, "/header/n1", H5T_NATIVE_INT, &h->N1);
+  rc |= r_scalar(f, "/header/n2", H5T_NATIVE_INT, &h->N2);
+  rc |= r_scalar(f, "/header/n3", H5T_NATIVE_INT, &h->N3);
+  rc |= r_scalar(f, "/header/n_prim", H5T_NATIVE_INT, &h->n_prim);
+  rc |= r_scalar(f, "/header/gam", H5T_NATIVE_DOUBLE, &h->gam);
+  rc |= r_scalar(f, "/header/geom/startx1", H5T_NATIVE_DOUBLE, &h->startx[1]);
+  rc |= r_scalar(f, "/header/geom/startx2", H5T_NATIVE_DOUBLE, &h->startx[2]);
+  rc |= r_scalar(f, "/header/geom/startx3", H5T_NATIVE_DOUBLE, &h->startx[3]);
+  rc |= r_scalar(f, "/header/geom/dx1", H5T_NATIVE_DOUBLE, &h->dx[1]);
+  rc |= r_scalar(f, "/header/geom/dx2", H5T_NATIVE_DOUBLE, &h->dx[2]);
+  rc |= r_scalar(f, "/header/geom/dx3", H5T_NATIVE_DOUBLE, &h->dx[3]);
+  int fmks = (strncmp(h->metric, "FMKS", 19) == 0 || strncmp(h->metric, "MMKS", 19) == 0);
+  const char *g = fmks ? "/header/geom/mmks/" : "/header/geom/mks/";
+  char key[128];
+#define RD(name, dst)                                                     \
+  snprintf(key, sizeof(key), "%s%s", g, name);                            \
+  rc |= r_scalar(f, key, H5T_NATIVE_DOUBLE, dst);
+  RD("a", &h->a);
+  RD("hslope", &h->hslope);
+  snprintf(key, sizeof(key), "%sRin", g);
+  if (H5Lexists(f, key, H5P_DEFAULT) > 0) {
+    RD("Rin", &h->Rin);
+    RD("Rout", &h->Rout);
+  } else {
+    RD("r_in", &h->Rin);
+    RD("r_out", &h->Rout);
+  }
+  if (fmks) {
+    RD("poly_xt", &h->poly_xt);
+    RD("poly_alpha", &h->poly_alpha);
+    RD("mks_smooth", &h->mks_smooth);
+  }
+#undef RD
+  r_scalar(f, "/dump_cadence", H5T_NATIVE_DOUBLE, &h->dump_cadence);
+  r_scalar(f, "/t", H5T_NATIVE_DOUBLE, &h->t);
+  H5Fclose(f);
+  return rc ? -1 : 0;
+}
+
+int ipole_h5_read_dump_prims(const char *path, const ipole_dump_header *h, double *prims)
+{
+  hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+  if (f < 0) return -1;
+  hid_t ds = H5Dopen2(f, "/prims", H5P_DEFAULT);
+  if (ds < 0) {
+    H5Fclose(f);
+    return -1;
+  }
+  herr_t e = H5Dread(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, prims);
+  H5Dclose(ds);
+  H5Fclose(f);
+  (void)h;
+  return e < 0 ? -1 : 0;
+}
+
+/* Image file, src/io.c:229-273 (dump) with the header of :140-191 and the
+ * iharm model header of model/iharm/model.c:1254-1301. */
+int ipole_h5_write_image(const char *path, const ipole_image_meta *mt, const double *image,
+                         const double *imageS, const double *taus)
+{
+  hid_t f = H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+  if (f < 0) return -1;
+  int rc = 0;
+  /* githash attribute on / */
+  {
+    hid_t t = H5Tcopy(H5T_C_S1);
+    H5Tset_size(t, 20);
+    char buf[20] = {0};
+    strncpy(buf, mt->githash, sizeof(buf) - 1);
+    hid_t sp = H5Screate(H5S_SCALAR);
+    hid_t at = H5Acreate2(f, "githash", t, sp, H5P_DEFAULT, H5P_DEFAULT);
+    if (at >= 0) {
+      H5Awrite(at, t, buf);
+      H5Aclose(at);
+    }
+    H5Sclose(sp);
+    H5Tclose(t);
+  }
+  hid_t hd = H5Gcreate2(f, "header", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+  rc |= w_str(hd, "version", mt->version, 20);
+  rc |= w_str(hd, "githash", mt->githash, 20);
+  rc |= w_d(hd, "freqcgs", mt->freqcgs);
+  rc |= w_d(hd, "scale", mt->scale);
+  rc |= w_d(hd, "dsource", mt->dsource);
+  rc |= w_str(hd, "evpa_0", mt->qu_conv == 0 ? "N" : "W", 2);
+  hid_t cam = H5Gcreate2(hd, "camera", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+  rc |= w_i(cam, "nx", mt->nx);
+  rc |= w_i(cam, "ny", mt->ny);
+  rc |= w_d(cam, "dx", mt->dx);
+  rc |= w_d(cam, "dy", mt->dy);
+  rc |= w_d(cam, "fovx_dsource", mt->fovx_dsource);
+  rc |= w_d(cam, "fovy_dsource", mt->fovy_dsource);
+  rc |= w_d(cam, "rcam", mt->rcam);
+  rc |= w_d(cam, "thetacam", mt->thetacam);
+  rc |= w_d(cam, "phicam", mt->phicam);
+  rc |= w_d(cam, "rotcam", mt->rotcam);
+  rc |= w_d(cam, "fovx", mt->fovx);
+  rc |= w_d(cam, "fovy", mt->fovy);
+  rc |= w_d(cam, "xoff", mt->xoff);
+  rc |= w_d(cam, "yoff", mt->yoff);
+  hsize_t d4 = 4;
+  rc |= w_array(cam, "x", 1, &d4, H5T_IEEE_F64LE, mt->Xcam);
+  H5Gclose(cam);
+  if (mt->model_kind == 1) {
+    /* model/iharm/model.c:1254-1301 (the /fluid_header copy is added by the
+     * host when it has the dump file) */
+    rc |= w_d(hd, "t", mt->t);
+    rc |= w_d(hd, "sigma_cut", mt->sigma_cut);
+    hid_t el = H5Gcreate2(hd, "electrons", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    rc |= w_d(el, "rlow", mt->trat_small);
+    rc |= w_d(el, "rhigh", mt->trat_large);
+    rc |= w_d(el, "beta_crit", mt->beta_crit);
+    rc |= w_i(el, "type", 2);
+    H5Gclose(el);
+    rc |= w_i(hd, "field_config", mt->reverse_field);
+    hid_t un = H5Gcreate2(hd, "units", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    rc |= w_d(un, "L_unit", mt->L_unit);
+    rc |= w_d(un, "M_unit", mt->M_unit);
+    rc |= w_d(un, "T_unit", mt->T_unit);
+    rc |= w_d(un, "Thetae_unit", mt->Thetae_unit);
+    H5Gclose(un);
+    rc |= w_d(f, "Mdot", mt->Mdot);
+    rc |= w_d(f, "MdotEdd", mt->MdotEdd);
+    rc |= w_d(f, "Ladv", mt->Ladv);
+  } else {
+    hid_t un = H5Gcreate2(hd, "units", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    rc |= w_d(un, "L_unit", mt->L_unit);
+    H5Gclose(un);
+  }
+  H5Gclose(hd);
+  double Ftot_unpol = 0., Ftot = 0.;
+  size_t nx = mt->nx, ny = mt->ny;
+  for (size_t i = 0; i < nx; ++i)
+    for (size_t j = 0; j < ny; ++j) {
+      Ftot_unpol += image[i * ny + j] * mt->scale;
+      if (!mt->only_unpolarized) Ftot += imageS[(i * ny + j) * 5 + 0] * mt->scale;
+    }
+  rc |= w_d(f, "Ftot_unpol", Ftot_unpol);
+  if (!mt->only_unpolarized) rc |= w_d(f, "Ftot", Ftot);
+  double nuLnu = 4. * 3.14159265358979323846 * Ftot * mt->dsource * mt->dsource * 1.e-23 * mt->freqcgs;
+  rc |= w_d(f, "nuLnu", nuLnu);
+  nuLnu = 4. * 3.14159265358979323846 * Ftot_unpol * mt->dsource * mt->dsource * 1.e-23 * mt->freqcgs;
+  rc |= w_d(f, "nuLnu_unpol", nuLnu);
+  hsize_t d2[2] = {nx, ny}, d3[3] = {nx, ny, 5};
+  rc |= w_array(f, "unpol", 2, d2, H5T_IEEE_F64LE, image);
+  rc |= w_array(f, "tau", 2, d2, H5T_IEEE_F64LE, taus);
+  if (!mt->only_unpolarized) rc |= w_array(f, "pol", 3, d3, H5T_IEEE_F64LE, imageS);
+  H5Fclose(f);
+  return rc ? -1 : 0;
+}
+
+/* Copy /header of the dump into /fluid_header of the image file
+ * (model/iharm/model.c:1259-1260, via an object copy). */
+int ipole_h5_copy_fluid_header(const char *dump_path, const char *image_path)
+{
+  hid_t src = H5Fopen(dump_path, H5F_ACC_RDONLY, H5P_DEFAULT);
+  if (src < 0) return -1;
+  hid_t dst = H5Fopen(image_path, H5F_ACC_RDWR, H5P_DEFAULT);
+  if (dst < 0) {
+    H5Fclose(src);
+    return -1;
+  }
+  herr_t e = H5Ocopy(src, "/header", dst, "/fluid_header", H5P_DEFAULT, H5P_DEFAULT);
+  H5Fclose(dst);
+  H5Fclose(src);
+  return e < 0 ? -1 : 0;
+}
+
+int ipole_h5_read_image(const char *path, int nx, int ny, double *image, double *imageS,
+                        double *taus, double *Ftot, double *Ftot_unpol)
+{
+  hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+  if (f < 0) return -1;
+  int rc = 0;
+  (void)nx;
+  (void)ny;
+  if (image) rc |= r_scalar(f, "/unpol", H5T_NATIVE_DOUBLE, image);
+  if (taus) rc |= r_scalar(f, "/tau", H5T_NATIVE_DOUBLE, taus);
+  if (imageS) rc |= r_scalar(f, "/pol", H5T_NATIVE_DOUBLE, imageS);
+  if (Ftot) r_scalar(f, "/Ftot", H5T_NATIVE_DOUBLE, Ftot);
+  if (Ftot_unpol) rc |= r_scalar(f, "/Ftot_unpol", H5T_NATIVE_DOUBLE, Ftot_unpol);
+  H5Fclose(f);
+  return rc ? -1 : 0;
+}

@@ -88,6 +88,7 @@ def lib():
         L.ipgpu_fetch.argtypes = [vp, dp, dp, dp, ip, ip]
         L.ipgpu_bad_pixel.argtypes = [vp, ip, ip]
         L.ipgpu_kernel_times.argtypes = [vp, dp, dp, dp]
+        L.ipgpu_step_counts.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.ipgpu_stream.argtypes = [vp]
         L.ipgpu_stream.restype = vp
         L.ipgpu_native_coord.argtypes = [C.POINTER(Geom), C.c_double, C.c_double, C.c_double, dp]
@@ -292,6 +293,11 @@ class Context:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         self._chk(self.L.ipgpu_kernel_times(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    def step_counts(self):
+        a, b = C.c_longlong(), C.c_longlong()
+        self._chk(self.L.ipgpu_step_counts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def stream(self):
         return self.L.ipgpu_stream(self.h)

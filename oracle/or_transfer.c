@@ -220,39 +220,15 @@ static void tetrad_to_coord_rank2(double complex Tt[4][4], double Econ[4][4], do
         for (int l = 0; l < 4; l++) Tc[i][j] += Tt[k][l] * Econ[k][i] * Econ[l][j];
 }
 
-/* src/ipolarray.c:332-575 */
-int or_evolve_N(const double Xf[4], const double Kconf[4], double dlam, double complex N_coord[4][4],
-                double *tauF, int zero_emission, const or_params_t *p)
+/* The Stokes update of evolve_N in the plasma frame, src/ipolarray.c:390-520:
+ * Faraday half step, absorption/emission, Faraday half step.  S = I Q U V. */
+void or_stokes_step(const double c11[11], double dlam, double S[4])
 {
-  double gcov[4][4], Ucon[4], Bcon[4], Ucov[4], Bcov[4], Ecov[4][4], Econ[4][4];
-  double complex N_tetrad[4][4];
-  double c11[11];
-  double SI0, SQ0, SU0, SV0, SI1, SQ1, SU1, SV1, SI2, SQ2, SU2, SV2, SI, SQ, SU, SV;
-  int oddflag = 0;
-  or_jar(p->emission_type, 1, Xf, Kconf, c11);
   double jI = c11[0], jQ = c11[1], jU = c11[2], jV = c11[3];
   double aI = c11[4], aQ = c11[5], aU = c11[6], aV = c11[7];
   double rQ = c11[8], rU = c11[9], rV = c11[10];
-  if (zero_emission) {
-    jI = 0.;
-    jQ = 0.;
-    jU = 0.;
-    jV = 0.;
-  }
-  or_fourv(Xf, Kconf, Ucon, Ucov, Bcon, Bcov);
-  double bsq = 0.;
-  for (int mu = 0; mu < 4; mu++) bsq += Bcon[mu] * Bcov[mu];
-  if (bsq <= 0.) {
-    Bcon[0] = 0.;
-    Bcon[1] = 1.;
-    Bcon[2] = 1.;
-    Bcon[3] = 1.;
-  }
-  or_gcov(Xf, gcov);
-  oddflag |= or_plasma_tetrad(Ucon, Kconf, Bcon, gcov, Econ, Ecov);
-  coord_to_tetrad_rank2(N_coord, Ecov, N_tetrad);
-  tensor_to_stokes(N_tetrad, &SI0, &SQ0, &SU0, &SV0);
-
+  double SI0 = S[0], SQ0 = S[1], SU0 = S[2], SV0 = S[3];
+  double SI1, SQ1, SU1, SV1, SI2, SQ2, SU2, SV2, SI, SQ, SU, SV;
   double x = dlam * 0.5;
   double rho2 = rQ * rQ + rU * rU + rV * rV;
   if (rho2 > CUT_PREVENT_NAN) {
@@ -336,6 +312,52 @@ int or_evolve_N(const double Xf[4], const double Kconf[4], double dlam, double c
     SU = SU2 + (rV * SQ2 - rQ * SV2) * x;
     SV = SV2 + (-rU * SQ2 + rQ * SU2) * x;
   }
+  S[0] = SI;
+  S[1] = SQ;
+  S[2] = SU;
+  S[3] = SV;
+}
+
+/* src/ipolarray.c:332-575 */
+int or_evolve_N(const double Xf[4], const double Kconf[4], double dlam, double complex N_coord[4][4],
+                double *tauF, int zero_emission, const or_params_t *p)
+{
+  double gcov[4][4], Ucon[4], Bcon[4], Ucov[4], Bcov[4], Ecov[4][4], Econ[4][4];
+  double complex N_tetrad[4][4];
+  double c11[11];
+  double SI0, SQ0, SU0, SV0, SI, SQ, SU, SV;
+  int oddflag = 0;
+  or_jar(p->emission_type, 1, Xf, Kconf, c11);
+  double jI = c11[0], jQ = c11[1], jU = c11[2], jV = c11[3];
+  double aI = c11[4], aQ = c11[5], aU = c11[6], aV = c11[7];
+  double rQ = c11[8], rU = c11[9], rV = c11[10];
+  if (zero_emission) {
+    jI = 0.;
+    jQ = 0.;
+    jU = 0.;
+    jV = 0.;
+  }
+  or_fourv(Xf, Kconf, Ucon, Ucov, Bcon, Bcov);
+  double bsq = 0.;
+  for (int mu = 0; mu < 4; mu++) bsq += Bcon[mu] * Bcov[mu];
+  if (bsq <= 0.) {
+    Bcon[0] = 0.;
+    Bcon[1] = 1.;
+    Bcon[2] = 1.;
+    Bcon[3] = 1.;
+  }
+  or_gcov(Xf, gcov);
+  oddflag |= or_plasma_tetrad(Ucon, Kconf, Bcon, gcov, Econ, Ecov);
+  coord_to_tetrad_rank2(N_coord, Ecov, N_tetrad);
+  tensor_to_stokes(N_tetrad, &SI0, &SQ0, &SU0, &SV0);
+
+  double S4[4] = {SI0, SQ0, SU0, SV0};
+  double c11s[11] = {jI, jQ, jU, jV, aI, aQ, aU, aV, rQ, rU, rV};
+  or_stokes_step(c11s, dlam, S4);
+  SI = S4[0];
+  SQ = S4[1];
+  SU = S4[2];
+  SV = S4[3];
   *tauF += dlam * fabs(rV);
   if (p->stokes_floors) {
     if (SI < 0) {

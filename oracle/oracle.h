@@ -167,6 +167,7 @@ double or_approximate_solve(double Ii, double ji, double ki, double jf, double k
                             double dl, double *tau);
 void or_push_polar(const double Xm[4], const double Km[4], double complex Ni[4][4],
                    double complex Nm[4][4], double complex Nf[4][4], double dlam);
+void or_stokes_step(const double c11[11], double dlam, double S[4]);
 int or_evolve_N(const double Xf[4], const double Kf[4], double dlam,
                 double complex N[4][4], double *tauF, int zero_emission,
                 const or_params_t *p);

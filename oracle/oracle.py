@@ -133,6 +133,7 @@ def lib():
         L.or_approximate_solve.argtypes = [C.c_double] * 6 + [dp]
         L.or_approximate_solve.restype = C.c_double
         L.or_push_polar.argtypes = [dp, dp, dp, dp, dp, C.c_double]
+        L.or_stokes_step.argtypes = [dp, C.c_double, dp]
         L.or_evolve_N.argtypes = [dp, dp, C.c_double, dp, dp, C.c_int, C.POINTER(Params)]
         L.or_evolve_N.restype = C.c_int
         L.or_project_N.argtypes = [dp, dp, dp, dp, dp, dp, C.c_double]
@@ -284,6 +285,12 @@ class Oracle:
         rc = self.L.or_render_pixels(C.byref(params), C.byref(cam), IP(pix), n, P(out7), IP(nsteps),
                                      IP(flags), nthreads)
         return rc, out7, nsteps, flags
+
+
+def stokes_step(c11, dlam, S):
+    S = np.array(S, dtype=np.float64)
+    lib().or_stokes_step(P(f64(c11)), dlam, P(S))
+    return S
 
 
 def bessel_kn(n, x):

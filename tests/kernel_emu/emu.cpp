@@ -97,7 +97,8 @@ int emu_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, 
         status = -10;
         continue;
       }
-      flag[n] = transfer_ray(*G, *M, *P, ns, rec.data(), ntp, dump, out7 + 7 * (size_t)n);
+      int nrad;
+      flag[n] = transfer_ray(*G, *M, *P, ns, rec.data(), ntp, dump, out7 + 7 * (size_t)n, nrad);
     }
   }
   return status;

@@ -1,0 +1,129 @@
+"""The GPU library's per-ray code compiled for the CPU (tests/kernel_emu)
+against the oracle: checks the kernel arithmetic (exact-derivative
+connection, Hermitian coherency tensor, block Stokes update, Chebyshev K_n)
+in a container without a GPU.  The same bars as the GPU parity tests."""
+import numpy as np
+import pytest
+from scipy import special
+
+import emu
+import oracle as O
+from helpers import small_dump
+from ipole_amd import ipgpu as G
+
+
+def test_metric_derivatives_vs_oracle_fd():
+    """dg/dX1, dg/dX2 against the oracle's central differences (DEL = 1e-7,
+    as src/geometry.c:91), g and g^-1 against the oracle's metric and LU."""
+    h, _ = small_dump(N=(8, 8, 8))
+    g = G.iharm_geom(h)
+    go = O.iharm_geom(h)
+    L = O.lib()
+    L.or_set_geom(O.C.byref(go))
+    orc = O.Oracle.__new__(O.Oracle)
+    orc.L = L
+    rng = np.random.default_rng(0)
+    idx = [(0, 0), (0, 1), (0, 2), (0, 3), (1, 1), (1, 2), (1, 3), (2, 2), (2, 3), (3, 3)]
+    for _ in range(300):
+        X1, X2 = rng.uniform(0.1, 6.8), rng.uniform(0.02, 0.98)
+        e = emu.metric(g, X1, X2)
+        gc_ = orc.gcov([0, X1, X2, 0])
+        gcon = orc.gcon(gc_)
+        for n, (i, j) in enumerate(idx):
+            scale = np.sqrt(abs(gc_[i, i] * gc_[j, j])) + 1e-300
+            assert abs(e[2 + n] - gc_[i, j]) < 1e-14 * scale
+            sc2 = np.sqrt(abs(gcon[i, i] * gcon[j, j]))
+            assert abs(e[12 + n] - gcon[i, j]) < 1e-12 * sc2
+        for k, off in ((1, 22), (2, 32)):
+            hh = 1e-4
+            G4 = []
+            for s in (2, 1, -1, -2):
+                Xs = np.array([0, X1, X2, 0.])
+                Xs[k] += s * hh
+                G4.append(orc.gcov(Xs))
+            fd = (-G4[0] + 8 * G4[1] - 8 * G4[2] + G4[3]) / (12 * hh)   # O(h^4) stencil
+            for n, (i, j) in enumerate(idx):
+                scale = max(np.sqrt(abs(gc_[i, i] * gc_[j, j])), abs(fd[i, j]))
+                assert abs(e[off + n] - fd[i, j]) < 1e-9 * scale, (k, i, j)
+
+
+def test_connection_contractions_vs_oracle():
+    """Gamma^i_jk K^j K^k and Gamma^i_kl K^l from the exact derivatives
+    against the oracle's finite-difference connection."""
+    h, _ = small_dump(N=(8, 8, 8))
+    g = G.iharm_geom(h)
+    orc = O.Oracle(O.iharm_geom(h), O.Model())
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        X = np.array([0., rng.uniform(0.1, 6.8), rng.uniform(0.02, 0.98), rng.uniform(0, 6)])
+        K = rng.normal(size=4)
+        a, M = emu.accel(g, X, K)
+        conn = orc.connection(X)
+        a_ref = np.einsum("ijk,j,k->i", conn, K, K)
+        M_ref = np.einsum("ikl,l->ik", conn, K)
+        sa = np.max(np.abs(np.einsum("ijk,j,k->ijk", np.abs(conn), np.abs(K), np.abs(K)).sum(axis=(1, 2))))
+        assert np.max(np.abs(a - a_ref)) < 1e-7 * sa
+        sm = np.max(np.abs(conn).sum(axis=2) * np.abs(K).max())
+        assert np.max(np.abs(M - M_ref)) < 1e-7 * sm
+
+
+def test_device_bessel_vs_scipy():
+    for x in np.concatenate([10 ** np.linspace(-3, 2.85, 300), [1.9999999, 2.0, 2.0000001]]):
+        ks, emx = emu.bessel(x)
+        for n in (0, 1, 2):
+            ref = special.kve(n, x)
+            assert abs(ks[n] / ref - 1) < 3e-14, (n, x)
+        assert abs(emx / np.exp(-x) - 1) < 1e-15
+
+
+def image_errors(ref, out7, f3, scale, unpol=False):
+    I_ref = ref["image"].ravel()
+    I_got = out7[:, 0] * f3
+    e_un = np.abs(I_got - I_ref) / np.maximum(I_ref, 1e-6 * I_ref.max())
+    F_rel = abs(I_got.sum() / I_ref.sum() - 1)
+    if unpol:
+        return F_rel, {"unpol": e_un}
+    S = ref["imageS"].reshape(-1, 5)
+    den = np.maximum(np.abs(S[:, 0]), 1e-6 * np.abs(S[:, 0]).max())
+    errs = {"unpol": e_un}
+    for k, n in enumerate("IQUV"):
+        sign = -1. if n in "QU" else 1.   # qu_conv 0: save_pixel negates Q, U
+        errs[n] = np.abs(out7[:, k + 1] * f3 * sign - S[:, k]) / den
+    Ft = abs((out7[:, 1] * f3).sum() / S[:, 0].sum() - 1)
+    return max(F_rel, Ft), errs
+
+
+@pytest.mark.parametrize("unpol", [0, 1])
+def test_emu_sphere(unpol):
+    go, mo = O.sphere_model(R0=50., tau0=2.0143e-5, Thetae0=10., beta0=30.)
+    orc = O.Oracle(go, mo)
+    p = O.default_params(nx=24, ny=24, thetacam=60., dx=100., dy=100., freqcgs=230e9, only_unpolarized=unpol)
+    ref = orc.render(p)
+    gs, ms = G.sphere_setup(R0=50., tau0=2.0143e-5, Thetae0=10., beta0=30.)
+    im, scale, _ = G.make_image(gs, ms.L_unit, nx=24, ny=24, thetacam=60., dx=100., dy=100.,
+                                freqcgs=230e9, only_unpolarized=unpol)
+    rc, out7, ns, fl = emu.render(gs, ms, im, None)
+    assert rc == 0 and np.array_equal(ns, ref["nstep"].ravel())
+    F, errs = image_errors(ref, out7, 230e9 ** 3, scale, unpol=bool(unpol))
+    assert F < 1e-6
+    for k, e in errs.items():
+        assert e.max() < 1e-6, (k, e.max())
+
+
+@pytest.mark.parametrize("thetacam,etype,unpol", [(163., 4, 0), (60., 1, 0), (90., 4, 1)])
+def test_emu_iharm(thetacam, etype, unpol):
+    h, prims = small_dump()
+    orc, fields = O.iharm_oracle(h, prims, M_unit=3e26)
+    p = O.default_params(nx=20, ny=20, thetacam=thetacam, freqcgs=230e9, emission_type=etype,
+                         only_unpolarized=unpol)
+    ref = orc.render(p)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    im, scale, _ = G.make_image(g, m.L_unit, nx=20, ny=20, thetacam=thetacam, freqcgs=230e9,
+                                emission_type=etype, only_unpolarized=unpol)
+    rc, out7, ns, fl = emu.render(g, m, im, emu.pack_dump(fields))
+    assert rc == 0 and np.array_equal(ns, ref["nstep"].ravel())
+    F, errs = image_errors(ref, out7, 230e9 ** 3, scale, unpol=bool(unpol))
+    assert F < 1e-6
+    for k, e in errs.items():
+        assert e.max() < 1e-5, (k, e.max())
+        assert np.mean(e > 1e-6) < 1e-3, (k, np.mean(e > 1e-6))
