@@ -94,9 +94,17 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
-            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
-        L = C.CDLL(LIB)
+        _lib = load(LIB)
+    return _lib
+
+
+def load(path):
+    """Loads and declares one build of the oracle library (each build keeps
+    its own global state)."""
+    if True:
+        if not os.path.exists(path):
+            raise RuntimeError("%s is not built (run make -C oracle)" % path)
+        L = C.CDLL(path)
         L.or_set_geom.argtypes = [C.POINTER(Geom)]
         L.or_get_geom.argtypes = [C.POINTER(Geom)]
         L.or_set_model.argtypes = [C.POINTER(Model)]
@@ -144,8 +152,7 @@ def lib():
         L.or_render_pixels.argtypes = [C.POINTER(Params), C.POINTER(Camera), ip, C.c_int, dp, ip,
                                        ip, C.c_int]
         L.or_render_pixels.restype = C.c_int
-        _lib = L
-    return _lib
+        return L
 
 
 def P(a):
@@ -164,8 +171,8 @@ class Oracle:
     """Holds the global geometry/model state of the oracle and keeps the
     field arrays alive."""
 
-    def __init__(self, geom, model, fields=None):
-        self.L = lib()
+    def __init__(self, geom, model, fields=None, L=None):
+        self.L = L or lib()
         self.geom = geom
         self.model = model
         self.fields = None if fields is None else f64(fields)
@@ -400,11 +407,11 @@ def iharm_units(M_unit, MBH=6.2e9, trat_small=1., trat_large=40., beta_crit=1., 
 
 
 def iharm_oracle(h, prims, M_unit, MBH=6.2e9, trat_small=1., trat_large=40., beta_crit=1.,
-                 sigma_cut=1., th_beg=0.0174, rmax_geo=100.):
+                 sigma_cut=1., th_beg=0.0174, rmax_geo=100., L=None):
     """Oracle set up for an iharm dump: derives the ghosted fields as
     model/iharm/model.c does and returns (Oracle, fields11)."""
     g = iharm_geom(h, rmax_geo=rmax_geo)
-    L = lib()
+    L = L or lib()
     L.or_set_geom(C.byref(g))
     u = iharm_units(M_unit, MBH, trat_small, trat_large, beta_crit, h["gam"])
     nc = (g.N1 + 2) * (g.N2 + 2) * (g.N3 + 2)
@@ -416,4 +423,4 @@ def iharm_oracle(h, prims, M_unit, MBH=6.2e9, trat_small=1., trat_large=40., bet
     m.L_unit = GNEWT * MBH * MSUN / (CL * CL)
     m.th_beg, m.sigma_cut, m.sigma_cut_high = th_beg, sigma_cut, -1.
     fields = out[:10 * nc].copy()
-    return Oracle(g, m, fields), out.reshape(11, g.N1 + 2, g.N2 + 2, g.N3 + 2)
+    return Oracle(g, m, fields, L=L), out.reshape(11, g.N1 + 2, g.N2 + 2, g.N3 + 2)

