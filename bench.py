@@ -88,7 +88,7 @@ def main():
     for w in range(args.warmup):
         ctx.render(im, fetch=False)
         log("warmup %d done: %.2f ms" % (w, ctx.kernel_times()[2]))
-    ms_geo, ms_xfer, ms_tot = [], [], []
+    ms_geo, ms_xfer, ms_tot, ms_stage = [], [], [], []
     t0 = time.perf_counter()
     for s in range(args.steps):
         ctx.render(im, fetch=False)   # synchronizes on the library's stream before returning
@@ -96,14 +96,16 @@ def main():
         ms_geo.append(a)
         ms_xfer.append(b)
         ms_tot.append(c)
+        ms_stage.append(ctx.stage_times())
     wall = time.perf_counter() - t0
     n_geo, n_rad = ctx.step_counts()
     res = ctx.render(im)
     st = G.image_stats(res, scale, cfg["freqcgs"], 16.9e6)
     ctx.close()
     ms_step = 1e3 * wall / args.steps
-    log("per step: wall %.3f ms, device %.3f ms (geodesic %.3f, transfer %.3f)" %
-        (ms_step, np.mean(ms_tot), np.mean(ms_geo), np.mean(ms_xfer)))
+    st_ms = np.mean(np.array(ms_stage), axis=0)
+    log("per step: wall %.3f ms, device %.3f ms (k_trace %.3f, k_prep %.3f, k_chain %.3f, k_finalize %.3f)" %
+        ((ms_step, np.mean(ms_tot)) + tuple(st_ms)))
     log("image: Ftot %.6g Jy, unpol %.6g Jy, LP %.3f%%, CP %.3f%%; geodesic steps %d, radiating %d" %
         (st["Ftot"], st["Ftot_unpol"], st["LP"], st["CP"], n_geo, n_rad))
 

@@ -138,6 +138,13 @@ int ipgpu_bad_pixel(const ipgpu_ctx *ctx, int *i, int *j);
 int ipgpu_kernel_times(const ipgpu_ctx *ctx, double *ms_geodesic, double *ms_transfer,
                        double *ms_total);
 
+/* Per-stage device time of the last render (HIP events on the context's
+ * stream), ms[0..3] = geodesic pass (k_trace), transfer step packets (k_scan
+ * + k_prep), transfer recurrence (k_chain), output layout (k_finalize).
+ * Measurement only; the reference prints one wallclock figure
+ * (src/main.c:1038-1040). */
+int ipgpu_stage_times(const ipgpu_ctx *ctx, double ms[4]);
+
 /* Work counts of the last render: geodesic steps summed over rays (the
  * nstep of trace_geodesic) and radiating steps of the transfer pass (steps
  * with radiating_region true, src/ipolarray.c:122). */

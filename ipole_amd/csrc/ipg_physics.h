@@ -853,45 +853,54 @@ IPG_HD double approximate_solve(double Ii, double ji, double ki, double jf, doub
   return Ii * efac + (javg / kavg) * (1. - efac);
 }
 
-/* Faraday rotation over x, src/ipolarray.c:391-408 */
-IPG_HD void faraday(const Coef &c, double x, double S[4])
+/* The plasma-frame Stokes step of evolve_N (Faraday half step, constant-
+ * coefficient absorption/emission, Faraday half step; src/ipolarray.c:
+ * 391-520) as the affine map S -> T S + cv it is.  T and cv depend only on
+ * the coefficients and x, so they are formed wherever the coefficients are
+ * (once per step, off the state recurrence); composing the three factors
+ * reorders the reference's roundoff only.  The branches on rho^2, aP^2
+ * (1e-80), the small-optical-depth series (1e-5) and approximate_solve's
+ * 1e-3 switch are the reference's. */
+IPG_HD void faraday_matrix(const Coef &c, double x, double R[3][3])
 {
-  double rQ = c.rQ, rU = c.rU, rV = c.rV;
-  double rho2 = rQ * rQ + rU * rU + rV * rV;
-  double Q = S[1], U = S[2], V = S[3];
+  const double r[3] = {c.rQ, c.rU, c.rV};
+  const double rho2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+  /* [r]x S = r cross S on (Q, U, V) */
+  const double X[3][3] = {{0., -r[2], r[1]}, {r[2], 0., -r[0]}, {-r[1], r[0], 0.}};
   if (rho2 > 1e-80) {
     double rho = sqrt(rho2);
-    double rdS = rQ * Q + rU * U + rV * V;
-    double cc = cos(rho * x), s = sin(rho * x), sh = sin(0.5 * rho * x);
-    S[1] = Q * cc + 2 * rQ * rdS / rho2 * sh * sh + (rU * V - rV * U) / rho * s;
-    S[2] = U * cc + 2 * rU * rdS / rho2 * sh * sh + (rV * Q - rQ * V) / rho * s;
-    S[3] = V * cc + 2 * rV * rdS / rho2 * sh * sh + (rQ * U - rU * Q) / rho * s;
+    double s, cc;
+    sincos(rho * x, &s, &cc);
+    double sh = sin(0.5 * rho * x);
+    double w = 2. * sh * sh / rho2, v = s / rho;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) R[i][j] = (i == j ? cc : 0.) + w * r[i] * r[j] + v * X[i][j];
   } else {
-    S[1] = Q + (-rV * U + rU * V) * x;
-    S[2] = U + (rV * Q - rQ * V) * x;
-    S[3] = V + (-rU * Q + rQ * U) * x;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) R[i][j] = (i == j ? 1. : 0.) + x * X[i][j];
   }
 }
 
-/* Constant-coefficient emission/absorption over x, src/ipolarray.c:411-500 */
-IPG_HD void absorb_emit(const Coef &c, double x, double S[4])
+IPG_HD void absorb_emit_affine(const Coef &c, double x, double A[4][4], double b[4])
 {
-  double jI = c.jI, jQ = c.jQ, jU = c.jU, jV = c.jV, aI = c.aI, aQ = c.aQ, aU = c.aU, aV = c.aV;
-  double SI1 = S[0], SQ1 = S[1], SU1 = S[2], SV1 = S[3];
-  double aP2 = aQ * aQ + aU * aU + aV * aV;
+  const double j[4] = {c.jI, c.jQ, c.jU, c.jV};
+  const double a[3] = {c.aQ, c.aU, c.aV};
+  const double aI = c.aI;
+  const double aP2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+  for (int p = 0; p < 4; p++)
+    for (int q = 0; q < 4; q++) A[p][q] = 0.;
   if (aP2 > 1e-80) {
     double aP = sqrt(aP2);
     double tauP = aP * x, tauI = aI * x;
-    double ads0 = aQ * SQ1 + aU * SU1 + aV * SV1;
-    double adj = aQ * jQ + aU * jU + aV * jV;
     double efacm = exp(-tauI + tauP), efacp = exp(-tauI - tauP), efac = exp(-tauI);
-    double SI2 = efacm * (SI1 / 2. - ads0 / (2. * aP)) + efacp * (SI1 / 2. + ads0 / (2. * aP));
-    double SQ2 = efacm * (-SI1 * aQ * aP + ads0 * aQ) / (2. * aP2)
-        + efacp * (SI1 * aQ * aP + ads0 * aQ) / (2. * aP2) + efac * (SQ1 - ads0 * aQ / (aP2));
-    double SU2 = efacm * (-SI1 * aU * aP + ads0 * aU) / (2. * aP2)
-        + efacp * (SI1 * aU * aP + ads0 * aU) / (2. * aP2) + efac * (SU1 - ads0 * aU / (aP2));
-    double SV2 = efacm * (-SI1 * aV * aP + ads0 * aV) / (2. * aP2)
-        + efacp * (SI1 * aV * aP + ads0 * aV) / (2. * aP2) + efac * (SV1 - ads0 * aV / (aP2));
+    double ch = 0.5 * (efacm + efacp), shp = (efacp - efacm) / (2. * aP);
+    double iaP2 = 1. / aP2;
+    A[0][0] = ch;
+    for (int k = 0; k < 3; k++) {
+      A[0][k + 1] = shp * a[k];
+      A[k + 1][0] = shp * a[k];
+      for (int l = 0; l < 3; l++) A[k + 1][l + 1] = (k == l ? efac : 0.) + a[k] * a[l] * (ch - efac) * iaP2;
+    }
     double afacm = 1. - efacm, afac = 1. - efac, afacp = 1. - efacp;
     if (tauI - tauP <= 1e-5) {
       double e = tauI - tauP;
@@ -905,27 +914,54 @@ IPG_HD void absorb_emit(const Coef &c, double x, double S[4])
         }
       }
     }
-    SQ2 += afac * (jQ / aI - aQ * adj / (aI * aP2));
-    SU2 += afac * (jU / aI - aU * adj / (aI * aP2));
-    SV2 += afac * (jV / aI - aV * adj / (aI * aP2));
-    SI2 += afacm * (aP * jI - adj) / (2. * aP * (aI - aP));
-    SQ2 += afacm * aQ * (-aP * jI + adj) / (2. * aP2 * (aI - aP));
-    SU2 += afacm * aU * (-aP * jI + adj) / (2. * aP2 * (aI - aP));
-    SV2 += afacm * aV * (-aP * jI + adj) / (2. * aP2 * (aI - aP));
-    SI2 += afacp * (aP * jI + adj) / (2. * aP * (aI + aP));
-    SQ2 += afacp * aQ * (aP * jI + adj) / (2. * aP2 * (aI + aP));
-    SU2 += afacp * aU * (aP * jI + adj) / (2. * aP2 * (aI + aP));
-    SV2 += afacp * aV * (aP * jI + adj) / (2. * aP2 * (aI + aP));
-    S[0] = SI2;
-    S[1] = SQ2;
-    S[2] = SU2;
-    S[3] = SV2;
+    double adj = a[0] * j[1] + a[1] * j[2] + a[2] * j[3];
+    double m = afacm * (aP * j[0] - adj) / (2. * aP * (aI - aP));
+    double p = afacp * (aP * j[0] + adj) / (2. * aP * (aI + aP));
+    b[0] = m + p;
+    for (int k = 0; k < 3; k++)
+      b[k + 1] = afac * (j[k + 1] / aI - a[k] * adj / (aI * aP2)) + (p - m) * a[k] / aP;
   } else {
-    double tf;
-    S[0] = approximate_solve(SI1, jI, aI, jI, aI, x, tf = 0.);
-    S[1] = approximate_solve(SQ1, jQ, aI, jQ, aI, x, tf = 0.);
-    S[2] = approximate_solve(SU1, jU, aI, jU, aI, x, tf = 0.);
-    S[3] = approximate_solve(SV1, jV, aI, jV, aI, x, tf = 0.);
+    double dtau = x * aI;
+    if (dtau < 1.e-3) {
+      double g = x * (1. - (dtau / 2.) * (1. - dtau / 3.));
+      for (int k = 0; k < 4; k++) {
+        A[k][k] = 1. - aI * g;
+        b[k] = j[k] * g;
+      }
+    } else {
+      double efac = exp(-dtau);
+      for (int k = 0; k < 4; k++) {
+        A[k][k] = efac;
+        b[k] = (j[k] / aI) * (1. - efac);
+      }
+    }
+  }
+}
+
+IPG_HD void stokes_affine(const Coef &c, double x, double T[4][4], double cv[4])
+{
+  double R3[3][3], A[4][4], b[4];
+  faraday_matrix(c, 0.5 * x, R3);
+  absorb_emit_affine(c, x, A, b);
+  double R[4][4];
+  for (int i = 0; i < 4; i++)
+    for (int k = 0; k < 4; k++) R[i][k] = (i == 0 || k == 0) ? (i == k ? 1. : 0.) : R3[i - 1][k - 1];
+  double RA[4][4];
+  for (int i = 0; i < 4; i++)
+    for (int k = 0; k < 4; k++) {
+      double s = 0.;
+      for (int l = 0; l < 4; l++) s += R[i][l] * A[l][k];
+      RA[i][k] = s;
+    }
+  for (int i = 0; i < 4; i++) {
+    for (int k = 0; k < 4; k++) {
+      double s = 0.;
+      for (int l = 0; l < 4; l++) s += RA[i][l] * R[l][k];
+      T[i][k] = s;
+    }
+    double s = 0.;
+    for (int l = 0; l < 4; l++) s += R[i][l] * b[l];
+    cv[i] = s;
   }
 }
 
@@ -1022,55 +1058,6 @@ IPG_HD void herm_add_stokes(Herm &N, const double E1[4], const double E2[4], con
       N.re[off(k, l)] += a * E1[k] * E1[l] + b * E2[k] * E2[l] + u * (E1[k] * E2[l] + E2[k] * E1[l]);
       N.im[off(k, l)] += v * (E2[k] * E1[l] - E1[k] * E2[l]);
     }
-}
-
-/* One radiating polarized step, evolve_N (src/ipolarray.c:332-575):
- * coefficients at the end point, plasma tetrad, Faraday half step,
- * absorption/emission, Faraday half step, tauF; returns oddflag bits. */
-IPG_HD int evolve(const Fluid &Fin, const Metric &m, const double K[4], const Coef &c0, double dlam,
-                  int zero_emission, int stokes_floors, Herm &N, double &tauF)
-{
-  Coef c = c0;
-  if (zero_emission) c.jI = c.jQ = c.jU = c.jV = 0.;
-  double Bcon[4] = {Fin.Bcon[0], Fin.Bcon[1], Fin.Bcon[2], Fin.Bcon[3]};
-  double bsq = Fin.Bcon[0] * Fin.Bcov[0] + Fin.Bcon[1] * Fin.Bcov[1] + Fin.Bcon[2] * Fin.Bcov[2]
-      + Fin.Bcon[3] * Fin.Bcov[3];
-  if (bsq <= 0.) {
-    Bcon[0] = 0.;
-    Bcon[1] = 1.;
-    Bcon[2] = 1.;
-    Bcon[3] = 1.;
-  }
-  double E[4][4];
-  int flag = plasma_tetrad(Fin.Ucon, K, Bcon, m.g, m.sqrtmg, E);
-  double e1[4], e2[4];
-  lower(m.g, E[1], e1);
-  lower(m.g, E[2], e2);
-  double S0[4], S[4];
-  herm_to_stokes(N, e1, e2, S0);
-  for (int n = 0; n < 4; n++) S[n] = S0[n];
-  faraday(c, dlam * 0.5, S);
-  absorb_emit(c, dlam, S);
-  faraday(c, dlam * 0.5, S);
-  tauF += dlam * fabs(c.rV);
-  if (stokes_floors) {
-    if (S[0] < 0) {
-      S[0] = S[1] = S[2] = S[3] = 0.;
-    } else {
-      double pf = sqrt(S[1] * S[1] + S[2] * S[2] + S[3] * S[3]) / S[0];
-      if (pf > 1.) {
-        S[1] /= pf;
-        S[2] /= pf;
-        S[3] /= pf;
-      }
-    }
-  }
-  double dS[4] = {S[0] - S0[0], S[1] - S0[1], S[2] - S0[2], S[3] - S0[3]};
-  herm_add_stokes(N, E[1], E[2], dS);
-  if (tauF > 1.e100 || tauF < -1.e100 || isnan(tauF)) flag |= 2;
-  if (isnan(N.d[0]) || isnan(S[0])) flag |= 4;
-  if (isnan(Fin.Ucov[0])) flag |= 8;
-  return flag;
 }
 
 /* Camera-frame Stokes of N, project_N (src/ipolarray.c:578-598). */

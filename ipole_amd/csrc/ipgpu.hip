@@ -5,10 +5,14 @@
  *   k_trace     one lane per ray: camera tetrad, initial wavevector and the
  *               backward geodesic (src/model_geodesics.c:38-171), writing one
  *               64-byte record per step (X1 X2 X3 K0 K1 K2 K3 dl) to HBM.
- *   k_transfer  one lane per ray: walks its records from the far end toward
- *               the camera (src/ipolarray.c:59-247): parallel transport of N,
- *               fluid gathers from the dump in HBM, thermal coefficients,
- *               unpolarized and polarized steps; then project_N.
+ *   k_scan      packet offsets: prefix sum of the per-ray step counts.
+ *   k_prep      one lane per transfer step of every ray (src/ipolarray.c:
+ *               59-247 minus the state): connection contractions for the
+ *               parallel transport of N, fluid gathers from the dump in HBM,
+ *               thermal coefficients and the plasma tetrad, written as one
+ *               step packet; fully parallel over all steps of the image.
+ *   k_chain     one lane per ray: the serial recurrence of N and I over its
+ *               packets from the far end toward the camera, then project_N.
  *   k_finalize  reference output layout and units (save_pixel,
  *               src/main.c:885-910).
  * Rays are numbered in 8x8 pixel tiles so that one 64-lane wavefront traces
@@ -63,14 +67,72 @@ __global__ __launch_bounds__(WAVE) void k_trace(ipgpu_geom G, ipgpu_image P, int
   atomicAdd(&counts[0], (unsigned long long)nstep);
 }
 
-__global__ __launch_bounds__(WAVE) void k_transfer(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
-                                                   int nrays, int cap, const double *__restrict__ traj,
-                                                   const int *__restrict__ nstep_in,
-                                                   const int *__restrict__ nturns_buf,
-                                                   const double *__restrict__ dump,
-                                                   double *__restrict__ out7, int *__restrict__ flags,
-                                                   int *__restrict__ status,
-                                                   unsigned long long *__restrict__ counts)
+/* Exclusive prefix sum of the per-ray step counts (one workgroup): packet
+ * offsets for k_prep / k_chain.  off has nrays + 1 entries. */
+constexpr int SCAN_THREADS = 1024;
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan(int nrays, const int *__restrict__ nstep,
+                                                       long long *__restrict__ off)
+{
+  __shared__ long long part[SCAN_THREADS];
+  const int t = threadIdx.x;
+  const int chunk = (nrays + SCAN_THREADS - 1) / SCAN_THREADS;
+  const int lo = t * chunk, hi = min(lo + chunk, nrays);
+  long long s = 0;
+  for (int r = lo; r < hi; r++) s += nstep[r];
+  part[t] = s;
+  __syncthreads();
+  for (int d = 1; d < SCAN_THREADS; d <<= 1) {
+    long long v = (t >= d) ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  long long base = part[t] - s;
+  for (int r = lo; r < hi; r++) {
+    off[r] = base;
+    base += nstep[r];
+  }
+  if (t == SCAN_THREADS - 1) off[nrays] = part[t];
+}
+
+/* One lane per transfer step of every ray: the state-independent part of
+ * integrate_emission (transport matrices, fluid, coefficients, tetrad). */
+__global__ __launch_bounds__(256) void k_prep(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P, int nrays,
+                                              int cap, int stride, long long total,
+                                              const long long *__restrict__ off,
+                                              const int *__restrict__ nstep_in,
+                                              const double *__restrict__ traj,
+                                              const int *__restrict__ nturns_buf,
+                                              const double *__restrict__ dump, double *__restrict__ pkt)
+{
+  long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  /* ray = last r with off[r] <= g */
+  int lo = 0, hi = nrays;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (off[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  const int ray = lo;
+  const int ns = nstep_in[ray];
+  if (ns >= P.maxnstep - 1) return;
+  const int q = (int)(g - off[ray]);
+  const double *rec = traj + (size_t)ray * cap * REC;
+  const int *nt = nturns_buf ? nturns_buf + (size_t)ray * cap : nullptr;
+  prep_step(G, Mo, P, ns - q, rec, nt, dump, pkt + (size_t)g * stride, stride == PK);
+}
+
+/* One lane per ray: the recurrence of N and I over the ray's packets. */
+__global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P, int nrays,
+                                                int cap, int stride, const long long *__restrict__ off,
+                                                const int *__restrict__ nstep_in,
+                                                const double *__restrict__ traj,
+                                                const double *__restrict__ dump,
+                                                const double *__restrict__ pkt,
+                                                double *__restrict__ out7, int *__restrict__ flags,
+                                                int *__restrict__ status,
+                                                unsigned long long *__restrict__ counts)
 {
   int ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= nrays) return;
@@ -86,10 +148,11 @@ __global__ __launch_bounds__(WAVE) void k_transfer(ipgpu_geom G, ipgpu_model Mo,
     return;
   }
   const double *rec = traj + (size_t)ray * cap * REC;
-  const int *nt = nturns_buf ? nturns_buf + (size_t)ray * cap : nullptr;
+  double js, ks;
+  far_end_coefficients(G, Mo, P, ns, rec, dump, js, ks);
   double res[7];
   int nrad;
-  flags[ray] = transfer_ray(G, Mo, P, ns, rec, nt, dump, res, nrad);
+  flags[ray] = chain_ray(G, P, ns, pkt + (size_t)off[ray] * stride, stride, js, ks, res, nrad);
   for (int q = 0; q < 7; q++) o[q] = res[q];
   atomicAdd(&counts[1], (unsigned long long)nrad);
 }
@@ -272,7 +335,7 @@ __global__ void k_unpack(size_t ncell, const double *__restrict__ dump, double *
 struct ipgpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   char err[512] = {0};
   ipgpu_geom geom{};
   bool has_geom = false;
@@ -284,6 +347,10 @@ struct ipgpu_ctx {
   size_t traj_doubles = 0;
   int *d_nturns = nullptr;
   size_t nturns_ints = 0;
+  double *d_pkt = nullptr;
+  size_t pkt_doubles = 0;
+  long long *d_off = nullptr;
+  size_t cap_off = 0;
   int *d_nstep_ray = nullptr, *d_flag_ray = nullptr;
   double *d_out7 = nullptr;
   double *d_image = nullptr, *d_imageS = nullptr, *d_taus = nullptr;
@@ -294,7 +361,7 @@ struct ipgpu_ctx {
   unsigned long long *d_counts = nullptr;
   unsigned long long counts[2] = {0, 0};
   int last_npix = 0, last_unpol = 0, have_result = 0;
-  float ms_geo = 0, ms_xfer = 0, ms_total = 0;
+  float ms_geo = 0, ms_xfer = 0, ms_total = 0, ms_stage[4] = {0, 0, 0, 0};
   int bad_i = -1, bad_j = -1;
   int traj_cap_steps = 8192;
 };
@@ -341,7 +408,7 @@ int ipgpu_create(ipgpu_ctx **out, int device)
     delete ctx;
     return IPGPU_ERR_HIP;
   }
-  for (int q = 0; q < 4; q++)
+  for (int q = 0; q < 5; q++)
     if (hipEventCreate(&ctx->ev[q]) != hipSuccess) {
       delete ctx;
       return IPGPU_ERR_HIP;
@@ -360,12 +427,12 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
   if (!ctx) return;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
-  void *ptrs[] = {ctx->d_dump, ctx->d_traj, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
+  void *ptrs[] = {ctx->d_dump, ctx->d_traj, ctx->d_pkt, ctx->d_off, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
                   ctx->d_out7, ctx->d_image, ctx->d_imageS, ctx->d_taus, ctx->d_nstep_pix,
                   ctx->d_flag_pix, ctx->d_status, ctx->d_counts};
   for (void *p : ptrs)
     if (p) hipFree(p);
-  for (int q = 0; q < 4; q++)
+  for (int q = 0; q < 5; q++)
     if (ctx->ev[q]) hipEventDestroy(ctx->ev[q]);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -530,7 +597,9 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
   HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
   if ((rc = run_trace(ctx, P, nrays, cap, turns))) return rc;
   int status[NSTATUS];
+  unsigned long long total = 0;
   HIPCHK(ctx, hipMemcpyAsync(status, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(&total, ctx->d_counts, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   if (status[0] > 0) {
     /* some rays outgrew the per-ray record capacity: retrace with room for
@@ -543,29 +612,44 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
     ctx->traj_cap_steps = cap;
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     if ((rc = run_trace(ctx, P, nrays, cap, turns))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(&total, ctx->d_counts, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+  const int stride = P.only_unpolarized ? PK_UNPOL : PK;
+  if ((rc = grow(ctx, ctx->d_pkt, ctx->pkt_doubles, (size_t)(total > 0 ? total : 1) * stride))) return rc;
+  if ((rc = grow(ctx, ctx->d_off, ctx->cap_off, (size_t)nrays + 1))) return rc;
   int init[NSTATUS] = {0, 0x7fffffff, 0x7fffffff, 0};
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_status, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
-  k_transfer<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, ctx->d_traj,
-                                                      ctx->d_nstep_ray, turns ? ctx->d_nturns : nullptr,
-                                                      ctx->d_dump, ctx->d_out7, ctx->d_flag_ray,
-                                                      ctx->d_status, ctx->d_counts);
+  k_scan<<<1, SCAN_THREADS, 0, ctx->stream>>>(nrays, ctx->d_nstep_ray, ctx->d_off);
   HIPCHK(ctx, hipGetLastError());
+  if (total > 0) {
+    unsigned nb = (unsigned)((total + 255) / 256);
+    k_prep<<<nb, 256, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, (long long)total,
+                                        ctx->d_off, ctx->d_nstep_ray, ctx->d_traj,
+                                        turns ? ctx->d_nturns : nullptr, ctx->d_dump, ctx->d_pkt);
+    HIPCHK(ctx, hipGetLastError());
+  }
   HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+  k_chain<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, ctx->d_off,
+                                                   ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, ctx->d_pkt,
+                                                   ctx->d_out7, ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
   double f3 = pow(P.freqcgs, 3);
   k_finalize<<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(P, nrays, f3, ctx->d_out7, ctx->d_flag_ray,
                                                            ctx->d_nstep_ray, ctx->d_image, ctx->d_imageS,
                                                            ctx->d_taus, ctx->d_nstep_pix, ctx->d_flag_pix,
                                                            ctx->d_status);
   HIPCHK(ctx, hipGetLastError());
-  HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(status, ctx->d_status, sizeof(status), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(ctx->counts, ctx->d_counts, sizeof(ctx->counts), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  hipEventElapsedTime(&ctx->ms_geo, ctx->ev[0], ctx->ev[1]);
-  hipEventElapsedTime(&ctx->ms_xfer, ctx->ev[1], ctx->ev[2]);
-  hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[3]);
+  for (int q = 0; q < 4; q++) hipEventElapsedTime(&ctx->ms_stage[q], ctx->ev[q], ctx->ev[q + 1]);
+  ctx->ms_geo = ctx->ms_stage[0];
+  ctx->ms_xfer = ctx->ms_stage[1] + ctx->ms_stage[2];
+  hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[4]);
   ctx->last_npix = (int)npix;
   ctx->last_unpol = P.only_unpolarized;
   ctx->have_result = 1;
@@ -624,6 +708,13 @@ int ipgpu_kernel_times(const ipgpu_ctx *ctx, double *g, double *x, double *t)
   if (g) *g = ctx->ms_geo;
   if (x) *x = ctx->ms_xfer;
   if (t) *t = ctx->ms_total;
+  return IPGPU_OK;
+}
+
+int ipgpu_stage_times(const ipgpu_ctx *ctx, double ms[4])
+{
+  if (!ctx || !ms) return IPGPU_ERR_ARG;
+  for (int q = 0; q < 4; q++) ms[q] = ctx->ms_stage[q];
   return IPGPU_OK;
 }
 

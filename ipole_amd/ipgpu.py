@@ -88,6 +88,7 @@ def lib():
         L.ipgpu_fetch.argtypes = [vp, dp, dp, dp, ip, ip]
         L.ipgpu_bad_pixel.argtypes = [vp, ip, ip]
         L.ipgpu_kernel_times.argtypes = [vp, dp, dp, dp]
+        L.ipgpu_stage_times.argtypes = [vp, dp]
         L.ipgpu_step_counts.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.ipgpu_stream.argtypes = [vp]
         L.ipgpu_stream.restype = vp
@@ -293,6 +294,12 @@ class Context:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         self._chk(self.L.ipgpu_kernel_times(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    def stage_times(self):
+        """ms of k_trace, k_scan + k_prep, k_chain, k_finalize in the last render."""
+        ms = (C.c_double * 4)()
+        self._chk(self.L.ipgpu_stage_times(self.h, ms))
+        return tuple(ms)
 
     def step_counts(self):
         a, b = C.c_longlong(), C.c_longlong()

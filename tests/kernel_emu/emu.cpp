@@ -66,6 +66,19 @@ void emu_point(const ipgpu_geom *G, const ipgpu_model *M, const double *dump, in
   o[8] = c.rQ; o[9] = c.rU; o[10] = c.rV;
 }
 
+/* The affine form of the plasma-frame Stokes step: S_out = T S_in + cv. */
+void emu_stokes_affine(const double *c11, double x, const double *S_in, double *S_out)
+{
+  Coef c;
+  c.jI = c11[0]; c.jQ = c11[1]; c.jU = c11[2]; c.jV = c11[3];
+  c.aI = c11[4]; c.aQ = c11[5]; c.aU = c11[6]; c.aV = c11[7];
+  c.rQ = c11[8]; c.rU = c11[9]; c.rV = c11[10];
+  double T[4][4], cv[4];
+  stokes_affine(c, x, T, cv);
+  for (int i = 0; i < 4; i++)
+    S_out[i] = cv[i] + T[i][0] * S_in[0] + T[i][1] * S_in[1] + T[i][2] * S_in[2] + T[i][3] * S_in[3];
+}
+
 int emu_plasma_tetrad(const ipgpu_geom *G, const double *X, const double *U, const double *K,
                       const double *B, double *E16)
 {
@@ -86,6 +99,9 @@ int emu_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, 
   {
     std::vector<double> rec((size_t)P->maxnstep * REC);
     std::vector<int> nt(P->target_nturns >= 0 ? P->maxnstep : 1);
+    const bool pol = !P->only_unpolarized;
+    const int stride = pol ? PK : PK_UNPOL;
+    std::vector<double> pkt;
 #pragma omp for schedule(dynamic, 1)
     for (int n = 0; n < npix; n++) {
       int i = pix[2 * n], j = pix[2 * n + 1];
@@ -97,8 +113,14 @@ int emu_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, 
         status = -10;
         continue;
       }
+      /* the same per-step packets and per-ray chain as the GPU kernels */
+      pkt.assign((size_t)(ns > 0 ? ns : 1) * stride, 0.);
+      for (int q = 0; q < ns; q++)
+        prep_step(*G, *M, *P, ns - q, rec.data(), ntp, dump, pkt.data() + (size_t)q * stride, pol);
+      double js, ks;
+      far_end_coefficients(*G, *M, *P, ns, rec.data(), dump, js, ks);
       int nrad;
-      flag[n] = transfer_ray(*G, *M, *P, ns, rec.data(), ntp, dump, out7 + 7 * (size_t)n, nrad);
+      flag[n] = chain_ray(*G, *P, ns, pkt.data(), stride, js, ks, out7 + 7 * (size_t)n, nrad);
     }
   }
   return status;

@@ -23,6 +23,7 @@ def lib():
         L.emu_accel.argtypes = [C.POINTER(G.Geom), dp, dp, dp, dp]
         L.emu_bessel.argtypes = [C.c_double, dp, dp]
         L.emu_point.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), dp, C.c_int, dp, dp, dp]
+        L.emu_stokes_affine.argtypes = [dp, C.c_double, dp, dp]
         L.emu_plasma_tetrad.argtypes = [C.POINTER(G.Geom), dp, dp, dp, dp, dp]
         L.emu_render.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), dp, ip,
                                  C.c_int, dp, ip, ip]
@@ -67,6 +68,12 @@ def point(geom, model, dump, X, K, emission_type=4):
     d = dump if dump is not None else np.zeros(16)
     lib().emu_point(C.byref(geom), C.byref(model), G.P(d), emission_type, G.P(f64(X)), G.P(f64(K)), G.P(o))
     return o
+
+
+def stokes_affine(c11, x, S):
+    out = np.zeros(4)
+    lib().emu_stokes_affine(G.P(f64(c11)), x, G.P(f64(S)), G.P(out))
+    return out
 
 
 def render(geom, model, image, dump, pix=None):

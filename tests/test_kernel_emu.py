@@ -76,6 +76,36 @@ def test_device_bessel_vs_scipy():
         assert abs(emx / np.exp(-x) - 1) < 1e-15
 
 
+def test_stokes_affine_vs_oracle_step():
+    """The kernel applies evolve_N's plasma-frame step as one affine map
+    S -> T S + cv formed from the coefficients; against the oracle's
+    sequential Faraday / absorption-emission / Faraday step over every
+    branch (rho^2 and aP^2 above and below 1e-80, optical depths either side
+    of the 1e-5 series switch and of approximate_solve's 1e-3 switch)."""
+    rng = np.random.default_rng(11)
+    n = 0
+    for rot in (0., 1e-45, 1.):
+        for apol in (0., 1e-45, 0.5):
+            for x in (1e-9, 3e-6, 2e-4, 0.05, 1.5, 30.):
+                for _ in range(6):
+                    c = np.zeros(11)
+                    c[4] = rng.uniform(0.3, 2.)
+                    a = rng.normal(size=3)
+                    c[5:8] = a / np.linalg.norm(a) * apol * c[4] * rng.uniform(0.1, 0.95)
+                    c[0] = rng.uniform(0.1, 1.)
+                    jp = rng.normal(size=3)
+                    c[1:4] = jp / np.linalg.norm(jp) * 0.6 * c[0]
+                    c[8:11] = rng.normal(size=3) * rot
+                    S0 = rng.normal(size=4)
+                    S0[0] = abs(S0[0]) + 2.
+                    ref = O.stokes_step(c, x, S0)
+                    got = emu.stokes_affine(c, x, S0)
+                    scale = max(np.abs(ref).max(), 1e-300)
+                    assert np.abs(got - ref).max() <= 1e-13 * scale, (rot, apol, x, got, ref)
+                    n += 1
+    assert n == 3 * 3 * 6 * 6
+
+
 def image_errors(ref, out7, f3, scale, unpol=False):
     I_ref = ref["image"].ravel()
     I_got = out7[:, 0] * f3
