@@ -41,16 +41,38 @@ def cpu_baseline(h, prims, cfg, sample_stride, nthreads):
     p = O.default_params(nx=cfg["nx"], ny=cfg["ny"], thetacam=cfg["thetacam"], freqcgs=cfg["freqcgs"])
     cam = orc.camera(p)
     pix = np.array([(i, j) for i in range(0, cfg["nx"], sample_stride)
-                    for j in range(0, cfg["ny"], sample_stride)], np.int32)
+                    for j in range(cfg["ny"])], np.int32)
     t0 = time.time()
     rc, out7, ns, fl = orc.render_pixels(p, cam, pix, nthreads=nthreads)
     dt = time.time() - t0
     assert rc == 0
     log("cpu baseline: %d pixels in %.1fs on %d threads" % (len(pix), dt, nthreads))
     return dict(value=len(pix) / dt, unit="pixels/s", cores=nthreads, kind="port",
-                sample="%d of %d pixels (every %dth row and column) of the same 160x160 image, "
+                sample="%d of %d pixels (every %d-th pixel row, all columns) of the same 160x160 image, "
                        "oracle C+OpenMP, %d threads" % (len(pix), cfg["nx"] * cfg["ny"], sample_stride,
                                                        nthreads))
+
+
+def roofline(stage_ms, n_geo):
+    """Roofline of the dominant kernel (longest stage): algorithmic FLOP per
+    launch = per-unit count (op-counting build, ipole_amd/flops_per_step.json)
+    x units (geodesic steps = transfer steps per image); traffic = measured
+    HBM bytes per launch (ipole_amd/hbm_traffic.json)."""
+    fl = json.load(open(os.path.join(ROOT, "ipole_amd", "flops_per_step.json")))["pol"]
+    tr = json.load(open(os.path.join(ROOT, "ipole_amd", "hbm_traffic.json")))["bytes_per_launch"]
+    per_unit = dict(k_trace=fl["F_trace_per_step"], k_prep=fl["F_prep_per_step"],
+                    k_chain=fl["F_chain_per_step"])
+    t = dict(k_trace=stage_ms[0], k_prep=stage_ms[1], k_chain=stage_ms[2])
+    ach = {k: per_unit[k] * n_geo / (t[k] * 1e-3) / 1e12 for k in t}
+    dom = max(t, key=t.get)
+    return dict(bound="mfma", achieved=round(ach[dom], 4), peak=PEAK_FP64, unit="TFLOP/s",
+                frac=round(ach[dom] / PEAK_FP64, 5), traffic=tr.get(dom), kernel=dom,
+                note="FP64 compute roof (MI355X FP64 peak 78.6 TFLOP/s, which the FP64 vector ALU "
+                     "also reaches; these kernels use the vector ALU, no matrix instructions). "
+                     "FLOP per unit from the op-counting build (%s: %.0f per step); per-kernel "
+                     "TFLOP/s: %s; traffic = HBM bytes per launch from FETCH_SIZE/WRITE_SIZE "
+                     "counters scaled by a plain-copy calibration"
+                     % (dom, per_unit[dom], ", ".join("%s %.3f" % (k, v) for k, v in ach.items())))
 
 
 def main():
@@ -59,13 +81,28 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-stride", type=int, default=10)
+    ap.add_argument("--cpu-stride", type=int, default=2,
+                    help="CPU baseline sample: every n-th pixel row (all columns)")
     ap.add_argument("--small", action="store_true", help="small dump for quick checks")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run one plain device-to-device copy of MIB MiB (the reference "
+                         "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this path is data-parallel over images, and no "
               "multi-GPU figure is produced (got --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied %d MiB" % args.copy_calibration)
+        return
 
     from ipole_amd import ipgpu as G
     from ipole_amd import synth
@@ -101,6 +138,11 @@ def main():
     n_geo, n_rad = ctx.step_counts()
     res = ctx.render(im)
     st = G.image_stats(res, scale, cfg["freqcgs"], 16.9e6)
+    ns_img = res["nstep"]
+    tiles = ns_img[:ns_img.shape[0] // 8 * 8, :ns_img.shape[1] // 8 * 8].reshape(
+        ns_img.shape[0] // 8, 8, ns_img.shape[1] // 8, 8).max(axis=(1, 3))
+    log("nstep: max %d, mean %.1f, p99 %d; sum over 8x8 tiles of the tile max %d" %
+        (ns_img.max(), ns_img.mean(), np.percentile(ns_img, 99), tiles.sum()))
     ctx.close()
     ms_step = 1e3 * wall / args.steps
     st_ms = np.mean(np.array(ms_stage), axis=0)
@@ -109,20 +151,7 @@ def main():
     log("image: Ftot %.6g Jy, unpol %.6g Jy, LP %.3f%%, CP %.3f%%; geodesic steps %d, radiating %d" %
         (st["Ftot"], st["Ftot_unpol"], st["LP"], st["CP"], n_geo, n_rad))
 
-    fl = json.load(open(os.path.join(ROOT, "ipole_amd", "flops_per_step.json")))
-    t_geo, t_xfer = np.mean(ms_geo) * 1e-3, np.mean(ms_xfer) * 1e-3
-    ach_geo = fl["F_geo"] * n_geo / t_geo / 1e12
-    ach_xfer = fl["F_pol"] * n_rad / t_xfer / 1e12
-    if t_xfer >= t_geo:
-        dom, ach = "k_transfer", ach_xfer
-    else:
-        dom, ach = "k_trace", ach_geo
-    roof = dict(bound="valu", achieved=round(ach, 4), peak=PEAK_FP64, unit="TFLOP/s",
-                frac=round(ach / PEAK_FP64, 5), traffic=None, kernel=dom,
-                note="FP64 vector ALU bound (no dense contraction on this path); algorithmic FLOP "
-                     "= F_geo*geodesic_steps (k_trace) or F_pol*radiating_steps (k_transfer), "
-                     "ipole_amd/flops_per_step.json; k_trace %.4f TFLOP/s, k_transfer %.4f TFLOP/s"
-                     % (ach_geo, ach_xfer))
+    roof = roofline(np.mean(np.array(ms_stage), axis=0), n_geo)
     cpu = None
     if not args.no_cpu:
         nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()

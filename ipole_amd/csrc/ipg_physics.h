@@ -26,6 +26,7 @@
 #include <math.h>
 #include "ipgpu.h"
 #include "ipg_bessel_coeffs.h"
+#include "ipg_math.h"
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -93,9 +94,7 @@ struct Theta {
 IPG_HD void theta_mks(const ipgpu_geom &G, double X1, double X2, Theta &T)
 {
   double s2p, c2p;
-  double arg = 2. * PI * X2;
-  s2p = sin(arg);
-  c2p = cos(arg);
+  sincospi_(2. * X2, s2p, c2p);
   double hs = G.hslope;
   double thG = PI * X2 + ((1. - hs) / 2.) * s2p;
   double dG = PI + (1. - hs) * PI * c2p;
@@ -111,13 +110,14 @@ IPG_HD void theta_mks(const ipgpu_geom &G, double X1, double X2, Theta &T)
   }
   double y = 2. * X2 - 1.;
   double al = G.poly_alpha, xt = G.poly_xt, pn = G.poly_norm;
-  double P = y / xt;
+  double ixt = rcp(xt);
+  double P = y * ixt;
   double Pa1 = powa(P, al - 1.);
   double Pa = Pa1 * P;
-  double thJ = pn * y * (1. + Pa / (al + 1.)) + 0.5 * PI;
+  double thJ = pn * y * (1. + Pa * rcp(al + 1.)) + 0.5 * PI;
   double dJ = 2. * pn * (1. + Pa);
-  double ddJ = 4. * pn * al * Pa1 / xt;
-  double E = exp(G.mks_smooth * (G.startx[1] - X1));
+  double ddJ = 4. * pn * al * Pa1 * ixt;
+  double E = exp_(G.mks_smooth * (G.startx[1] - X1));
   double ms = G.mks_smooth;
   T.th = thG + E * (thJ - thG);
   T.t1 = -ms * E * (thJ - thG);
@@ -130,20 +130,22 @@ IPG_HD void theta_mks(const ipgpu_geom &G, double X1, double X2, Theta &T)
 /* r, theta only (src/coordinates.c:46-97) */
 IPG_HD void bl_coord(const ipgpu_geom &G, double X1, double X2, double &r, double &th)
 {
-  r = exp(X1);
+  r = exp_(X1);
   if (G.metric == IPGPU_METRIC_EMINKOWSKI) {
     th = X2;
     return;
   }
-  double thG = PI * X2 + ((1. - G.hslope) / 2.) * sin(2. * PI * X2);
+  double s2p, c2p;
+  sincospi_(2. * X2, s2p, c2p);
+  double thG = PI * X2 + ((1. - G.hslope) / 2.) * s2p;
   if (G.metric == IPGPU_METRIC_MKS) {
     th = thG;
     return;
   }
   double y = 2. * X2 - 1.;
-  double thJ = G.poly_norm * y * (1. + powa(y / G.poly_xt, G.poly_alpha) / (G.poly_alpha + 1.))
+  double thJ = G.poly_norm * y * (1. + powa(y * rcp(G.poly_xt), G.poly_alpha) * rcp(G.poly_alpha + 1.))
       + 0.5 * PI;
-  th = thG + exp(G.mks_smooth * (G.startx[1] - X1)) * (thJ - thG);
+  th = thG + exp_(G.mks_smooth * (G.startx[1] - X1)) * (thJ - thG);
 }
 
 /* Metric, inverse and X1/X2 derivatives at (X1, X2).  Same tensor as
@@ -151,10 +153,11 @@ IPG_HD void bl_coord(const ipgpu_geom &G, double X1, double X2, double &r, doubl
  * :263-348): g_MKS = J^T g_KS J with J = dx^KS/dX. */
 IPG_HD void metric(const ipgpu_geom &G, double X1, double X2, Metric &m)
 {
-  double r = exp(X1);
+  double r = exp_(X1);
   m.r = r;
   if (G.metric == IPGPU_METRIC_EMINKOWSKI) {
-    double th = X2, s = sin(th), c = cos(th);
+    double th = X2, s, c;
+    sincos_(th, s, c);
     double r2 = r * r;
     m.th = th;
     for (int n = 0; n < 10; n++) {
@@ -168,9 +171,10 @@ IPG_HD void metric(const ipgpu_geom &G, double X1, double X2, Metric &m)
     m.g[7] = r2;
     m.g[9] = r2 * s * s;
     m.gc[0] = -1.;
-    m.gc[4] = 1. / r2;
-    m.gc[7] = 1. / r2;
-    m.gc[9] = 1. / (r2 * s * s);
+    double ir2 = rcp(r2);
+    m.gc[4] = ir2;
+    m.gc[7] = ir2;
+    m.gc[9] = rcp(r2 * s * s);
     m.d1[4] = 2. * r2;
     m.d1[7] = 2. * r2;
     m.d1[9] = 2. * r2 * s * s;
@@ -182,11 +186,12 @@ IPG_HD void metric(const ipgpu_geom &G, double X1, double X2, Metric &m)
   theta_mks(G, X1, X2, T);
   m.th = T.th;
   double a = G.a, a2 = a * a;
-  double s = sin(T.th), c = cos(T.th);
+  double s, c;
+  sincos_(T.th, s, c);
   double s2 = s * s, s2t = 2. * s * c;
   double Sig = r * r + a2 * c * c;
   double Sr = 2. * r, St = -2. * a2 * c * s;
-  double iS = 1. / Sig;
+  double iS = rcp(Sig);
   double q = 2. * r * iS;
   double qr = (2. - q * Sr) * iS;
   double qt = -q * St * iS;
@@ -241,8 +246,8 @@ IPG_HD void metric(const ipgpu_geom &G, double X1, double X2, Metric &m)
   /* inverse: KS contravariant metric, then g^MKS = Jinv g^KS Jinv^T with
    * Jinv11 = 1/r, Jinv21 = -t1/(r t2), Jinv22 = 1/t2 */
   double Del = r * r - 2. * r + a2;
-  double c00 = -opq, c01 = q, c11 = Del * iS, c13 = a * iS, c22 = iS, c33 = iS / s2;
-  double ir = 1. / r, u = -t1 / (r * t2), w = 1. / t2;
+  double c00 = -opq, c01 = q, c11 = Del * iS, c13 = a * iS, c22 = iS, c33 = iS * rcp(s2);
+  double ir = rcp(r), w = rcp(t2), u = -t1 * ir * w;
   m.gc[0] = c00;
   m.gc[1] = ir * c01;
   m.gc[2] = u * c01;
@@ -313,17 +318,17 @@ IPG_HD double stepsize(const ipgpu_geom &G, const double X[4], const double K[4]
 {
   const double SS = SMALL * SMALL;
   double deh = fmin(fabs(X[1] - G.cstartx[1]), 0.1);
-  double dlx1 = eps * (10 * deh) / (fabs(K[1]) + SS);
+  double dlx1 = eps * (10 * deh) * rcp(fabs(K[1]) + SS);
   double cut = 0.02;
-  double lx2 = G.cstopx[2] - G.cstartx[2];
-  double dpole = fmin(fabs(X[2] / lx2), fabs((G.cstopx[2] - X[2]) / lx2));
-  double d2fac = (dpole < cut) ? dpole / 3 : fmin(cut / 3 + (dpole - cut) * 10., 1);
-  double dlx2 = eps * d2fac / (fabs(K[2]) + SS);
-  double dlx3 = eps / (fabs(K[3]) + SS);
-  double idlx1 = 1. / (fabs(dlx1) + SS);
-  double idlx2 = 1. / (fabs(dlx2) + SS);
-  double idlx3 = 1. / (fabs(dlx3) + SS);
-  return 1. / (idlx1 + idlx2 + idlx3);
+  double ilx2 = rcp(G.cstopx[2] - G.cstartx[2]);
+  double dpole = fmin(fabs(X[2] * ilx2), fabs((G.cstopx[2] - X[2]) * ilx2));
+  double d2fac = (dpole < cut) ? dpole * (1. / 3.) : fmin(cut / 3 + (dpole - cut) * 10., 1);
+  double dlx2 = eps * d2fac * rcp(fabs(K[2]) + SS);
+  double dlx3 = eps * rcp(fabs(K[3]) + SS);
+  double idlx1 = rcp(fabs(dlx1) + SS);
+  double idlx2 = rcp(fabs(dlx2) + SS);
+  double idlx3 = rcp(fabs(dlx3) + SS);
+  return rcp(idlx1 + idlx2 + idlx3);
 }
 
 /* src/model_geodesics.c:218-237 */

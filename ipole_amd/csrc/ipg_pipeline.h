@@ -71,8 +71,8 @@ IPG_HD int trace_ray(const ipgpu_geom &G, const ipgpu_image &P, int pi, int pj, 
 {
   double E[4][4];
   Metric m;
-  camera_tetrad(G, P.Xcam, P.old_centering, E, m);
   double X[4] = {P.Xcam[0], P.Xcam[1], P.Xcam[2], P.Xcam[3]};
+  camera_tetrad(G, X, P.old_centering, E, m);
   double K[4];
   init_K(P, E, pi, pj, K);
   store_rec(rec, X, K, 0.);
@@ -362,7 +362,8 @@ IPG_HD int chain_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const do
   if (pol) {
     double E[4][4];
     Metric m;
-    camera_tetrad(G, P.Xcam, 0, E, m);
+    const double Xc[4] = {P.Xcam[0], P.Xcam[1], P.Xcam[2], P.Xcam[3]};
+    camera_tetrad(G, Xc, 0, E, m);
     project(N, E, m, P.rotcam, S);
   }
   out7[0] = I;

@@ -1,0 +1,88 @@
+"""Counts the algorithmic FLOPs per unit of each pipeline kernel with the
+op-counting CPU build (tests/kernel_emu/opcount.cpp) on the 160x160 FMKS
+workload (synthetic 288x128x128 dump, a = 0.9375, seed 1, thetacam 163,
+230 GHz, R_high 40, M_unit 3e26), over an evenly strided pixel sample, and
+writes ipole_amd/flops_per_step.json, which bench.py reads for the
+roofline's achieved FLOP/s.
+
+    python tests/kernel_emu/count_flops.py [--stride 8]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, HERE)
+
+from ipole_amd import ipgpu as G  # noqa: E402
+from ipole_amd import synth  # noqa: E402
+import emu  # noqa: E402
+import oracle as O  # noqa: E402
+
+
+def count(geom, model, image, dump, pix):
+    subprocess.run(["make", "-s", "-C", HERE, "libopcount.so"], check=True)
+    L = C.CDLL(os.path.join(HERE, "libopcount.so"))
+    L.opcount_render.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), G.dp, G.ip,
+                                 C.c_int, G.dp]
+    out = np.zeros(9)
+    pix = np.ascontiguousarray(pix, np.int32)
+    rc = L.opcount_render(C.byref(geom), C.byref(model), C.byref(image),
+                          G.P(dump) if dump is not None else None, G.IP(pix), len(pix), G.P(out))
+    assert rc == 0
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--stride", type=int, default=8)
+    args = ap.parse_args()
+    t0 = time.time()
+    h, prims = synth.make_dump(288, 128, 128, a=0.9375, seed=1)
+    orc, fields = O.iharm_oracle(h, prims, M_unit=3e26, trat_large=40.)
+    dump = emu.pack_dump(fields)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    pix = np.array([(i, j) for i in range(0, 160, args.stride) for j in range(0, 160, args.stride)])
+    res = {}
+    for unpol in (0, 1):
+        im, _, _ = G.make_image(g, m.L_unit, nx=160, ny=160, thetacam=163., freqcgs=230e9,
+                                only_unpolarized=unpol)
+        o = count(g, m, im, dump, pix)
+        n_geo, n_step, n_rad = o[0], o[1], o[2]
+        res["unpol" if unpol else "pol"] = dict(
+            geodesic_steps=int(n_geo), transfer_steps=int(n_step), radiating_steps=int(n_rad), rays=int(o[8]),
+            F_trace_per_step=o[3] / n_geo,
+            F_prep_per_radiating_step=o[4] / n_rad,
+            F_prep_per_nonradiating_step=o[5] / max(n_step - n_rad, 1),
+            F_prep_per_step=(o[4] + o[5]) / n_step,
+            F_chain_per_step=(o[6] + o[7]) / n_step,
+        )
+        print(("unpol" if unpol else "pol"), json.dumps(res["unpol" if unpol else "pol"]), flush=True)
+    out = dict(
+        source="op-counting CPU build of the kernel code (tests/kernel_emu/opcount.cpp) over %d pixels "
+               "(every %dth row and column) of the 160x160 FMKS workload; +,-,*,/,sqrt = 1, fma = 2, "
+               "each transcendental (exp, log, sin, cos, pow, acos, cbrt, tanh) = 20"
+               % (len(pix), args.stride),
+        command="python tests/kernel_emu/count_flops.py --stride %d" % args.stride,
+        units=dict(F_trace_per_step="FLOP per geodesic step of k_trace (stepsize + midpoint push)",
+                   F_prep_per_step="FLOP per transfer step of k_prep (mix of radiating and not)",
+                   F_chain_per_step="FLOP per transfer step of k_chain, including its per-ray "
+                                    "far-end coefficients and projection"),
+        **res)
+    path = os.path.join(ROOT, "ipole_amd", "flops_per_step.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote %s in %.0fs" % (path, time.time() - t0))
+
+
+if __name__ == "__main__":
+    main()

@@ -66,6 +66,20 @@ void emu_point(const ipgpu_geom *G, const ipgpu_model *M, const double *dump, in
   o[8] = c.rQ; o[9] = c.rU; o[10] = c.rV;
 }
 
+/* The pipeline's elementary functions: kind 0 exp_, 1 sincos_, 2 sincospi_,
+ * 3 rcp; out holds one value (two for the sincos forms). */
+void emu_math(int kind, const double *x, int n, double *out)
+{
+  for (int i = 0; i < n; i++) {
+    double *o = out + 2 * i;
+    o[1] = 0.;
+    if (kind == 0) o[0] = exp_(x[i]);
+    else if (kind == 1) sincos_(x[i], o[0], o[1]);
+    else if (kind == 2) sincospi_(x[i], o[0], o[1]);
+    else o[0] = rcp(x[i]);
+  }
+}
+
 /* The affine form of the plasma-frame Stokes step: S_out = T S_in + cv. */
 void emu_stokes_affine(const double *c11, double x, const double *S_in, double *S_out)
 {

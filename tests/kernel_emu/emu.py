@@ -23,6 +23,7 @@ def lib():
         L.emu_accel.argtypes = [C.POINTER(G.Geom), dp, dp, dp, dp]
         L.emu_bessel.argtypes = [C.c_double, dp, dp]
         L.emu_point.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), dp, C.c_int, dp, dp, dp]
+        L.emu_math.argtypes = [C.c_int, dp, C.c_int, dp]
         L.emu_stokes_affine.argtypes = [dp, C.c_double, dp, dp]
         L.emu_plasma_tetrad.argtypes = [C.POINTER(G.Geom), dp, dp, dp, dp, dp]
         L.emu_render.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), dp, ip,
@@ -68,6 +69,13 @@ def point(geom, model, dump, X, K, emission_type=4):
     d = dump if dump is not None else np.zeros(16)
     lib().emu_point(C.byref(geom), C.byref(model), G.P(d), emission_type, G.P(f64(X)), G.P(f64(K)), G.P(o))
     return o
+
+
+def math(kind, x):
+    x = f64(np.atleast_1d(x))
+    out = np.zeros((len(x), 2))
+    lib().emu_math(kind, G.P(x), len(x), G.P(out))
+    return out
 
 
 def stokes_affine(c11, x, S):
