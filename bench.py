@@ -75,6 +75,73 @@ def roofline(stage_ms, n_geo):
                      % (dom, per_unit[dom], ", ".join("%s %.3f" % (k, v) for k, v in ach.items())))
 
 
+def c2_jobs(seed=0, n=64):
+    """64 jobs on the 288x128x128 dump: thetacam in {17, 50, 90, 130, 163},
+    {86, 230, 345} GHz, M_unit log-uniform over 1e24-1e26, image sizes from
+    {64, 96, 128, 192, 256}^2, drawn from a seeded generator."""
+    rng = np.random.default_rng(seed)
+    th = rng.choice([17., 50., 90., 130., 163.], n)
+    fr = rng.choice([86e9, 230e9, 345e9], n)
+    mu = 10 ** rng.uniform(24., 26., n)
+    sz = rng.choice([64, 96, 128, 192, 256], n)
+    return [dict(thetacam=float(t), freqcgs=float(f), M_unit=float(m), n=int(z))
+            for t, f, m, z in zip(th, fr, mu, sz)]
+
+
+def bench_batch(args):
+    """Batched render of the 64-job set in one call against the same jobs
+    rendered one by one with the plain path (dump re-derived per M_unit)."""
+    from ipole_amd import ipgpu as G
+    from ipole_amd import synth
+    cfg = dict(C1)
+    h, prims = synth.make_dump(*cfg["N"], a=cfg["a"], seed=cfg["seed"])
+    jobs = c2_jobs()
+    if args.small:
+        jobs = jobs[:8]
+    M_ref = 1e25
+    ctx = G.Context(0)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    ctx.set_geometry(g)
+    ctx.set_model(m)
+    ims = []
+    for jb in jobs:
+        im, _, _ = G.make_image(g, m.L_unit, nx=jb["n"], ny=jb["n"], thetacam=jb["thetacam"],
+                                freqcgs=jb["freqcgs"])
+        ims.append(im)
+    npix = sum(jb["n"] ** 2 for jb in jobs)
+    # plain path, one job at a time
+    plain_ms, Fplain = [], []
+    for jb, im in zip(jobs, ims):
+        ctx.upload_prims(prims, G.iharm_units(jb["M_unit"], trat_large=cfg["trat_large"]))
+        r = ctx.render(im)
+        plain_ms.append(ctx.kernel_times()[2])
+        Fplain.append(r["imageS"][..., 0].sum())
+    log("plain: %d jobs, %d pixels, %.1f ms device" % (len(jobs), npix, sum(plain_ms)))
+    ctx.upload_prims(prims, G.iharm_units(M_ref, trat_large=cfg["trat_large"]))
+    batch = [(im, jb["M_unit"] / M_ref) for jb, im in zip(jobs, ims)]
+    for w in range(args.warmup):
+        ctx.render_batch(batch, fetch=False)
+    t_dev = []
+    for s in range(args.steps):
+        ctx.render_batch(batch, fetch=False)
+        t_dev.append(ctx.kernel_times())
+    res = ctx.render_batch(batch)
+    ctx.close()
+    dF = max(abs(r["imageS"][..., 0].sum() / f - 1) for r, f in zip(res, Fplain))
+    ms = float(np.mean([t[2] for t in t_dev]))
+    log("batch: %.1f ms device (geodesic %.1f, transfer %.1f); max |dFtot|/Ftot vs plain %.2e" %
+        (ms, np.mean([t[0] for t in t_dev]), np.mean([t[1] for t in t_dev]), dF))
+    out = dict(metric=METRIC, value=round(npix / (ms * 1e-3), 1), unit="pixels/s", n_gpus=1,
+               steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 3), higher_is_better=True,
+               scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
+               config=dict(workload="batched FP64: 64 jobs on the 288x128x128 FMKS dump (thetacam "
+                                    "{17,50,90,130,163}, {86,230,345} GHz, M_unit 1e24-1e26, 64^2-256^2)",
+                           pixels=npix, plain_ms_device=round(sum(plain_ms), 3),
+                           plain_pixels_per_s=round(npix / (sum(plain_ms) * 1e-3), 1),
+                           speedup_vs_plain=round(sum(plain_ms) / ms, 3), max_dFtot_vs_plain=dF))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +154,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain device-to-device copy of MIB MiB (the reference "
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--workload", choices=["c1", "batch"], default="c1",
+                    help="c1: one 160x160 image (default); batch: the 64-job batch against the "
+                         "same jobs rendered one by one")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this path is data-parallel over images, and no "
@@ -103,6 +173,9 @@ def main():
         torch.cuda.synchronize()
         log("copied %d MiB" % args.copy_calibration)
         return
+
+    if args.workload == "batch":
+        return bench_batch(args)
 
     from ipole_amd import ipgpu as G
     from ipole_amd import synth

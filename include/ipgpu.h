@@ -130,6 +130,37 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
 int ipgpu_fetch(ipgpu_ctx *ctx, double *image, double *imageS, double *taus, int *nstep,
                 int *flag);
 
+/* One job of a batched render: an image (camera, size, frequency and
+ * options as for ipgpu_render) of the uploaded dump, with n_e scaled by
+ * m_unit_scale and B by its square root, i.e. the dump's M_unit times
+ * m_unit_scale (1 for the uploaded one).  Output pointers (host memory, any
+ * may be NULL) receive the job's image in the layout of ipgpu_render. */
+typedef struct ipgpu_job {
+  ipgpu_image image;
+  double m_unit_scale;
+  double *image_out, *imageS_out, *taus_out;
+  int *nstep_out, *flag_out;
+} ipgpu_job;
+
+/* Batched render: many cameras, frequencies and M_unit values of one dump
+ * in one pass (the reference runs one ipole process per image; the
+ * fast-light pixel loop of src/main.c:536-615 per job).  Jobs with the same
+ * camera, image size and integration options share one geodesic trace at
+ * the frequency of the first of them (the geodesic in X does not depend on
+ * the frequency).  Geodesics are traced twice: once to count the steps,
+ * then, longest first, into exact slices of one compact record buffer; the
+ * transfer of every (job, ray) pair then runs in one fused launch, longest
+ * rays first.  Results agree with per-job ipgpu_render to
+ * the roundoff of the rescaling.  On IPGPU_ERR_MAXNSTEP the job and pixel
+ * are reported by ipgpu_bad_job.  Measurement: ipgpu_kernel_times gives the
+ * batch's geodesic, transfer and total device time. */
+int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs);
+int ipgpu_bad_job(const ipgpu_ctx *ctx, int *job, int *i, int *j);
+
+/* Device memory a batch may use for geodesic records per pass (default
+ * 160 GiB); rays are processed in as many passes as this requires. */
+int ipgpu_set_batch_memory(ipgpu_ctx *ctx, size_t traj_bytes);
+
 /* Pixel (i, j) that exceeded maxnstep in the last render (IPGPU_ERR_MAXNSTEP). */
 int ipgpu_bad_pixel(const ipgpu_ctx *ctx, int *i, int *j);
 

@@ -35,6 +35,10 @@
 
 namespace ipg {
 
+#if !defined(IPG_OPCOUNT)
+typedef double ipg_f64; /* plain double for compile-time constant tables */
+#endif
+
 #if defined(IPG_OPCOUNT)
 /* The op-counting CPU build prices each of these as the operation it
  * stands for (one division, one transcendental). */
@@ -42,6 +46,9 @@ inline double rcp(double x) { return 1.0 / x; }
 inline double exp_(double x) { return exp(x); }
 inline void sincos_(double x, double &s, double &c) { sincos(x, &s, &c); }
 inline void sincospi_(double y, double &s, double &c) { sincos(3.141592653589793 * y, &s, &c); }
+inline double log_(double x) { return log(x); }
+inline double pow_(double a, double b) { return pow(a, b); }
+inline double cbrt_(double x) { return cbrt(x); }
 #else
 
 IPG_MATH_HD double rcp(double x)
@@ -52,6 +59,8 @@ IPG_MATH_HD double rcp(double x)
   y = fma(y, e, y);
   e = fma(-x, y, 1.0);
   y = fma(y, e, y);
+  /* x = 0: the hardware value (inf) without the refinement, which would
+   * turn it into NaN.  Callers never pass infinities. */
   return (x == 0.) ? __builtin_amdgcn_rcp(x) : y;
 #else
   return 1.0 / x;
@@ -154,6 +163,56 @@ IPG_MATH_HD void sincospi_(double y, double &s, double &c)
   double sr, cr;
   sincos_reduced(r, sr, cr);
   quadrant((int)k, sr, cr, s, c);
+}
+
+constexpr double SQRT_HALF = 0.7071067811865476;
+
+/* log x: x = 2^e m, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with
+ * s = (m - 1)/(m + 1), |s| <= 0.1716, series to s^23 (truncation < 1e-18
+ * relative); outside (1e-300, 1e300) and for 0, negatives, inf, NaN the
+ * library routine. */
+IPG_MATH_HD double log_(double x)
+{
+  if (!(x >= 1.e-300 && x <= 1.e300)) return log(x);
+  int e;
+  double m = frexp(x, &e);
+  if (m < SQRT_HALF) {
+    m *= 2.;
+    e -= 1;
+  }
+  double s = (m - 1.) * rcp(m + 1.);
+  double z = s * s;
+  double p = 1.0 / 23.0;
+  p = fma(p, z, 1.0 / 21.0);
+  p = fma(p, z, 1.0 / 19.0);
+  p = fma(p, z, 1.0 / 17.0);
+  p = fma(p, z, 1.0 / 15.0);
+  p = fma(p, z, 1.0 / 13.0);
+  p = fma(p, z, 1.0 / 11.0);
+  p = fma(p, z, 1.0 / 9.0);
+  p = fma(p, z, 1.0 / 7.0);
+  p = fma(p, z, 1.0 / 5.0);
+  p = fma(p, z, 1.0 / 3.0);
+  double at2 = 2. * fma(s * z, p, s);
+  double de = (double)e;
+  return fma(de, LN2_HI, fma(de, LN2_LO, at2));
+}
+
+/* a^b for a > 0 as e^(b log a); a = 0 gives 0 for b > 0 as pow does. */
+IPG_MATH_HD double pow_(double a, double b)
+{
+  if (!(a > 0.)) return pow(a, b);
+  return exp_(b * log_(a));
+}
+
+/* cube root of x > 0 (library routine otherwise) */
+IPG_MATH_HD double cbrt_(double x)
+{
+  if (!(x > 0.)) return cbrt(x);
+  double y = exp_(log_(x) * (1.0 / 3.0));
+  /* one Newton step on y^3 = x restores the last bits lost to the
+   * log/exp round trip */
+  return y - (y * y * y - x) * rcp(3. * y * y);
 }
 
 #endif

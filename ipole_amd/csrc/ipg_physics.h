@@ -31,8 +31,11 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define IPG_HD __host__ __device__ inline
+/* for the per-step bodies whose outputs must stay in registers */
+#define IPG_HD_INLINE __host__ __device__ inline __attribute__((always_inline))
 #else
 #define IPG_HD inline
+#define IPG_HD_INLINE inline
 #endif
 
 namespace ipg {
@@ -359,8 +362,8 @@ IPG_HD double gdot(const double g[10], const double a[4], const double b[4])
 
 IPG_HD void normalize(double v[4], const double g[10])
 {
-  double n = sqrt(fabs(gdot(g, v, v)));
-  for (int k = 0; k < 4; k++) v[k] /= n;
+  double in = rcp(sqrt(fabs(gdot(g, v, v))));
+  for (int k = 0; k < 4; k++) v[k] *= in;
 }
 
 /* src/tetrads.c:95-115 */
@@ -368,7 +371,8 @@ IPG_HD void project_out(double a[4], const double b[4], const double g[10])
 {
   double bb = gdot(g, b, b);
   double ab = gdot(g, a, b);
-  for (int k = 0; k < 4; k++) a[k] -= b[k] * ab / bb;
+  double f = ab * rcp(bb);
+  for (int k = 0; k < 4; k++) a[k] -= b[k] * f;
 }
 
 IPG_HD void set_from_trial(double E[4], int defdir, const double t[4])
@@ -488,9 +492,10 @@ IPG_HD void locate(const ipgpu_geom &G, double X1, double X2, double X3, Cell &c
 {
   double phi = fmod(X3, G.cstopx[3]);
   if (phi < 0.0) phi = G.cstopx[3] + phi;
-  double fi = (X1 - G.startx[1]) / G.dx[1] - 0.5 + 1000;
-  double fj = (X2 - G.startx[2]) / G.dx[2] - 0.5 + 1000;
-  double fk = (phi - G.startx[3]) / G.dx[3] - 0.5 + 1000;
+  const double idx1 = rcp(G.dx[1]), idx2 = rcp(G.dx[2]), idx3 = rcp(G.dx[3]);
+  double fi = (X1 - G.startx[1]) * idx1 - 0.5 + 1000;
+  double fj = (X2 - G.startx[2]) * idx2 - 0.5 + 1000;
+  double fk = (phi - G.startx[3]) * idx3 - 0.5 + 1000;
   /* clamp before the conversion so out-of-range or NaN positions stay in
    * bounds; inside the range this is the reference's (int) truncation */
   fi = fmin(fmax(fi, 0.), 1.e9);
@@ -503,9 +508,9 @@ IPG_HD void locate(const ipgpu_geom &G, double X1, double X2, double X3, Cell &c
   if (i >= G.N1) i = G.N1 - 1;
   if (j >= G.N2) j = G.N2 - 1;
   if (k >= G.N3) k = G.N3 - 1;
-  double d1 = (X1 - ((i + 0.5) * G.dx[1] + G.startx[1])) / G.dx[1];
-  double d2 = (X2 - ((j + 0.5) * G.dx[2] + G.startx[2])) / G.dx[2];
-  double d3 = (phi - ((k + 0.5) * G.dx[3] + G.startx[3])) / G.dx[3];
+  double d1 = (X1 - ((i + 0.5) * G.dx[1] + G.startx[1])) * idx1;
+  double d2 = (X2 - ((j + 0.5) * G.dx[2] + G.startx[2])) * idx2;
+  double d3 = (phi - ((k + 0.5) * G.dx[3] + G.startx[3])) * idx3;
   if (d1 < 0.) d1 = 0.;
   if (d1 >= 1.) d1 = 1.;
   if (d2 < 0.) d2 = 0.;
@@ -609,14 +614,15 @@ IPG_HD void fluid(const ipgpu_geom &G, const ipgpu_model &M, const double *dump,
   double VdotV = 0.;
   for (int i = 1; i < 4; i++)
     for (int j = 1; j < 4; j++) VdotV += m.g[sy(i, j)] * v[i - 1] * v[j - 1];
-  double Vfac = sqrt(-1. / m.gc[0] * (1. + fabs(VdotV)));
+  double Vfac = sqrt(-rcp(m.gc[0]) * (1. + fabs(VdotV)));
   F.Ucon[0] = -Vfac * m.gc[0];
   for (int i = 1; i < 4; i++) F.Ucon[i] = v[i - 1] - Vfac * m.gc[sy(0, i)];
   lower(m.g, F.Ucon, F.Ucov);
   F.Bcon[0] = v[3] * F.Ucov[1] + v[4] * F.Ucov[2] + v[5] * F.Ucov[3];
-  F.Bcon[1] = (v[3] + F.Ucon[1] * F.Bcon[0]) / F.Ucon[0];
-  F.Bcon[2] = (v[4] + F.Ucon[2] * F.Bcon[0]) / F.Ucon[0];
-  F.Bcon[3] = (v[5] + F.Ucon[3] * F.Bcon[0]) / F.Ucon[0];
+  const double iu0 = rcp(F.Ucon[0]);
+  F.Bcon[1] = (v[3] + F.Ucon[1] * F.Bcon[0]) * iu0;
+  F.Bcon[2] = (v[4] + F.Ucon[2] * F.Bcon[0]) * iu0;
+  F.Bcon[3] = (v[5] + F.Ucon[3] * F.Bcon[0]) * iu0;
   lower(m.g, F.Bcon, F.Bcov);
   double sigma = v[9];
   F.ne = (sigma > M.sigma_cut) ? 0. : v[6] * sigma_smoothfac(M, sigma);
@@ -630,9 +636,24 @@ IPG_HD void fluid(const ipgpu_geom &G, const ipgpu_model &M, const double *dump,
  * x <= 2: power series (A&S 9.6.10-9.6.13); x > 2: Chebyshev expansion of
  * sqrt(x) e^x K_n(x) in 4/x - 1 (ipg_bessel_coeffs.h); K2 by the upward
  * recurrence K2 = K0 + (2/x) K1 as GSL's K_n. */
+/* 1/k^2, 1/(k(k+1)), 1/k for the series terms k = 1..15 (index k) */
+struct BesselSeriesTables {
+  ipg_f64 ik2[17], ikk1[17], ik[17];
+  constexpr BesselSeriesTables() : ik2(), ikk1(), ik()
+  {
+    for (int k = 1; k < 17; k++) {
+      ik2[k] = 1.0 / (ipg_f64)(k * k);
+      ikk1[k] = 1.0 / (ipg_f64)(k * (k + 1));
+      ik[k] = 1.0 / (ipg_f64)k;
+    }
+  }
+};
+
 IPG_HD void bessel_k012(double x, double ks[3], double &emx)
 {
-  emx = exp(-x);
+  constexpr BesselSeriesTables T;
+  emx = exp_(-x);
+  const double ix = rcp(x);
   double k0s, k1s;
   if (x <= 2.0) {
     const double EG = 0.57721566490153286061;
@@ -640,24 +661,24 @@ IPG_HD void bessel_k012(double x, double ks[3], double &emx)
     double t0 = 1.0, t1 = 0.5 * x;
     double I0 = 1.0, I1 = t1, S0 = 0., S1 = t1 * (1.0 - 2.0 * EG);
     double Hk = 0.0, p1 = -EG, p2 = 1.0 - EG;
+#pragma unroll
     for (int k = 1; k < 16; k++) {
-      double kk = (double)k;
-      t0 *= y / (kk * kk);
-      t1 *= y / (kk * (kk + 1.));
-      Hk += 1.0 / kk;
-      p1 += 1.0 / kk;
-      p2 += 1.0 / (kk + 1.);
+      t0 *= y * T.ik2[k];
+      t1 *= y * T.ikk1[k];
+      Hk += T.ik[k];
+      p1 += T.ik[k];
+      p2 += T.ik[k + 1];
       I0 += t0;
       I1 += t1;
       S0 += t0 * Hk;
       S1 += t1 * (p1 + p2);
     }
-    double lg = log(0.5 * x);
-    double ex = exp(x);
+    double lg = log_(0.5 * x);
+    double ex = rcp(emx);
     k0s = (-(lg + EG) * I0 + S0) * ex;
-    k1s = (1.0 / x + lg * I1 - 0.5 * S1) * ex;
+    k1s = (ix + lg * I1 - 0.5 * S1) * ex;
   } else {
-    double t = 4.0 / x - 1.0, t2 = 2.0 * t;
+    double t = 4.0 * ix - 1.0, t2 = 2.0 * t;
     double b0 = 0., b1 = 0., b2 = 0.;
     for (int k = IPG_K0_NCHEB - 1; k >= 1; k--) {
       b2 = b1;
@@ -672,13 +693,13 @@ IPG_HD void bessel_k012(double x, double ks[3], double &emx)
       b0 = t2 * b1 - b2 + ipg_k1_cheb[k];
     }
     double f1 = t * b0 - b1 + ipg_k1_cheb[0];
-    double isx = 1.0 / sqrt(x);
+    double isx = rcp(sqrt(x));
     k0s = f0 * isx;
     k1s = f1 * isx;
   }
   ks[0] = k0s;
   ks[1] = k1s;
-  ks[2] = k0s + (2.0 / x) * k1s;
+  ks[2] = k0s + (2.0 * ix) * k1s;
 }
 
 /* ---------------------------------------------------- transfer coefficients */
@@ -690,27 +711,38 @@ struct Coef {
 /* src/radiation.c:14-23 */
 IPG_HD double Bnu_inv(double nu, double Thetae)
 {
-  double x = HPL * nu / (ME * CL * CL * Thetae);
-  if (x < 2.e-3) return ((2. * HPL / (CL * CL)) / (x / 24. * (24. + x * (12. + x * (4. + x)))));
-  return ((2. * HPL / (CL * CL)) / (exp(x) - 1.));
+  double x = (HPL / (ME * CL * CL)) * nu * rcp(Thetae);
+  if (x < 2.e-3) return ((2. * HPL / (CL * CL)) * rcp(x * (1. / 24.) * (24. + x * (12. + x * (4. + x)))));
+  return ((2. * HPL / (CL * CL)) * rcp(exp_(x) - 1.));
 }
 
 /* src/symphony/maxwell_juettner_fits.c:334-356 with x^(1/3) shared */
 IPG_HD void dexter_IQV(double x, double &II, double &IQ, double &IV)
 {
-  double c = cbrt(x);
-  double ic = 1. / c, ic2 = ic * ic;
-  double e = exp(-1.8899 * c);
+  double c = cbrt_(x);
+  double ic = rcp(c), ic2 = ic * ic;
+  double e = exp_(-1.8899 * c);
   II = 2.5651 * (1 + 1.92 * ic + 0.9977 * ic2) * e;
   IQ = 2.5651 * (1 + 0.93193 * ic + 0.499873 * ic2) * e;
-  IV = (1.81384 / x + 3.42319 * ic2 + 0.0292545 / sqrt(x) + 2.03773 * ic) * e;
+  IV = (1.81384 * rcp(x) + 3.42319 * ic2 + 0.0292545 * rcp(sqrt(x)) + 2.03773 * ic) * e;
 }
+
+/* constants of the fits, evaluated once */
+constexpr double SQRT2 = 1.4142135623730951;
+constexpr double SQRT3 = 1.7320508075688772;
+constexpr double TWO_11_12 = 1.887748625363387;          /* 2^(11/12) */
+constexpr double C_EXTRA = 0.011767707320949835;         /* 2^(-1/3) / 3^(23/6) */
+constexpr double LOG120 = 4.787491742782046;
+constexpr double COS_28_25 = 0.43568244627671215, SIN_28_25 = 0.900100442176505;
+constexpr double COS_PI_2 = 6.123233995736766e-17;       /* cos of the double nearest pi/2 */
 
 /* Thermal (Maxwell-Juettner) coefficients at one point: the pol = 0 branch
  * (Leung j_I, Kirchhoff a_I; src/model_radiation.c:226-237) and, if pol,
  * the pol = 1 branch (Dexter or Pandya j_S, Kirchhoff a_S, Dexter rho_Q and
  * Shcherbakov rho_V; :243-305), sharing the fluid state, the fluid-frame
- * frequency and angle, and one K_n(1/Theta_e) evaluation.
+ * frequency and angle, and one K_n(1/Theta_e) evaluation.  sin and cos of
+ * the field angle come from mu = cos(theta) directly (theta = acos(mu) in
+ * [0, pi]); the fits' powers share their logarithms.
  * ju/ku: j_I/nu^2 and a_I nu of the unpolarized solve. */
 IPG_HD void coefficients(const Fluid &F, const double K[4], int emission_type, bool pol,
                          double &ju, double &ku, Coef &c)
@@ -721,124 +753,142 @@ IPG_HD void coefficients(const Fluid &F, const double K[4], int emission_type, b
   double Ne = F.ne;
   if (Ne <= 0.) return;
   /* get_fluid_nu, get_bk_angle: src/radiation.c:26-72 */
-  double nu = -(K[0] * F.Ucov[0] + K[1] * F.Ucov[1] + K[2] * F.Ucov[2] + K[3] * F.Ucov[3]) * ME
-      * CL * CL / HPL;
+  double kdotu = K[0] * F.Ucov[0] + K[1] * F.Ucov[1] + K[2] * F.Ucov[2] + K[3] * F.Ucov[3];
+  double nu = -kdotu * (ME * CL * CL / HPL);
+  double inu = rcp(nu);
   double nusq = nu * nu;
-  double theta;
+  double sth, cth;
+  bool edge = false;
   {
     double Bm = sqrt(fabs(F.Bcon[0] * F.Bcov[0] + F.Bcon[1] * F.Bcov[1] + F.Bcon[2] * F.Bcov[2]
                           + F.Bcon[3] * F.Bcov[3]));
     if (Bm == 0.) {
-      theta = PI / 2.;
+      sth = 1.;
+      cth = COS_PI_2;
     } else {
-      double kk = fabs(K[0] * F.Ucov[0] + K[1] * F.Ucov[1] + K[2] * F.Ucov[2] + K[3] * F.Ucov[3]);
       double mu = (K[0] * F.Bcov[0] + K[1] * F.Bcov[1] + K[2] * F.Bcov[2] + K[3] * F.Bcov[3])
-          / (kk * Bm);
-      if (fabs(mu) > 1.) mu /= fabs(mu);
-      theta = acos(mu);
+          * rcp(fabs(kdotu) * Bm);
+      if (fabs(mu) > 1.) mu = (mu > 0.) ? 1. : -1.;
+      edge = (mu == 1. || mu == -1.);   /* theta = 0 or pi */
+      sth = sqrt((1. - mu) * (1. + mu));
+      cth = mu;
     }
   }
   double B = F.B, Th = F.thetae;
+  double iTh = rcp(Th);
   double ks[3], emx;
-  bessel_k012(1. / Th, ks, emx);
-  const double sth = sin(theta), cth = cos(theta);
+  bessel_k012(iTh, ks, emx);
   /* Faraday fits, src/symphony/maxwell_juettner_fits.c:207-284 */
-  double omega0 = EE * B / (ME * CL);
-  double wp2 = 4. * PI * Ne * (EE * EE) / ME;
-  double xr = Th * sqrt(sqrt(2.) * sth * (1.e3 * omega0 / (2. * PI * nu)));
+  double omega0 = (EE / (ME * CL)) * B;
+  double wp2 = (4. * PI * EE * EE / ME) * Ne;
+  double xr = Th * sqrt(SQRT2 * sth * (1.e3 / (2. * PI)) * omega0 * inu);
   double w = 2. * PI * nu;
-  if (theta <= 0.0 || theta >= PI) {
+  double iw = rcp(w);
+  double iks2 = rcp(ks[2]);
+  if (edge) {
     if (pol) {
       double fit;
       if (emission_type != 1 && emx * ks[2] > 0) {
-        double sh = 0.43793091 * log(1. + 0.00185777 * pow(xr, 1.50316886));
-        fit = (emx * ks[0] - sh) / (emx * ks[2]);
+        double sh = 0.43793091 * log_(1. + 0.00185777 * pow_(xr, 1.50316886));
+        fit = (emx * ks[0] - sh) * rcp(emx * ks[2]);
       } else {
-        fit = (ks[0] / ks[2]) * (1 - 0.11 * log(1 + 0.035 * xr));
+        fit = (ks[0] * iks2) * (1 - 0.11 * log_(1 + 0.035 * xr));
       }
-      c.rV = 2. * PI * nu / CL * (wp2 * omega0 / (w * w * w) * fit * cth) * nu;
+      c.rV = 2. * PI * nu / CL * (wp2 * omega0 * (iw * iw * iw) * fit * cth) * nu;
     }
     return;
   }
   double Bnuinv = Bnu_inv(nu, Th);
+  double iBnu = (Bnuinv > 0) ? rcp(Bnuinv) : 0.;
   /* unpolarized: Leung fit, maxwell_juettner_fits.c:368-395 */
   {
     double K2 = fmax(emx * ks[2], SMALL);
-    double nuc = EE * B / (2. * PI * ME * CL);
+    double nuc = (EE / (2. * PI * ME * CL)) * B;
     double nus = (2. / 9.) * nuc * Th * Th * sth;
     double j = 0.;
     if (!(nu > 1.e12 * nus)) {
-      double x = nu / nus;
-      double f = sqrt(x) + pow(2., 11. / 12.) * pow(x, 1. / 6.);
+      double x = nu * rcp(nus);
+      double c3 = cbrt_(x);
+      double f = sqrt(x) + TWO_11_12 * sqrt(c3);   /* x^(1/6) = sqrt(x^(1/3)) */
       f = f * f;
-      j = (sqrt(2.) * PI * EE * EE * Ne * nus / (3. * CL * K2)) * f * exp(-cbrt(x));
+      j = (SQRT2 * PI * EE * EE / (3. * CL)) * Ne * nus * rcp(K2) * f * exp_(-c3);
     }
-    ju = j / nusq;
-    ku = (Bnuinv > 0) ? ju / Bnuinv : 0;
+    ju = j * rcp(nusq);
+    ku = ju * iBnu;
   }
   if (!pol) return;
   double jI, jQ, jV;
   if (emission_type == 1) {
     /* Pandya fits, maxwell_juettner_fits.c:27-129 */
-    double nu_c = EE * B / (2. * PI * ME * CL);
+    double nu_c = (EE / (2. * PI * ME * CL)) * B;
     double nu_s = (2. / 9.) * nu_c * sth * Th * Th;
-    double X = nu / nu_s;
-    double pre = (Ne * (EE * EE) * nu_c) / CL;
-    double X13 = cbrt(X), X16 = pow(X, 1. / 6.), e13 = exp(-X13);
-    double t1 = sqrt(2.) * PI / 27. * sth;
-    double tI = sqrt(X) + pow(2., 11. / 12.) * X16;
+    double X = nu * rcp(nu_s);
+    double pre = (EE * EE / CL) * Ne * nu_c;
+    double LX = log_(X);
+    double X13 = exp_(LX * (1. / 3.)), X16 = sqrt(X13), e13 = exp_(-X13);
+    double t1 = SQRT2 * PI / 27. * sth;
+    double sX = sqrt(X);
+    double tI = sX + TWO_11_12 * X16;
     jI = pre * t1 * (tI * tI) * e13;
-    double Tq = pow(Th, 24. / 25.);
-    double tq2 = (7. * Tq + 35.) / (10. * Tq + 75.);
-    double tq3 = sqrt(X) + tq2 * pow(2., 11. / 12.) * X16;
+    double LT = log_(Th);
+    double Tq = exp_(LT * (24. / 25.));
+    double tq2 = (7. * Tq + 35.) * rcp(10. * Tq + 75.);
+    double tq3 = sX + tq2 * TWO_11_12 * X16;
     jQ = pre * t1 * (tq3 * tq3) * e13;  /* = -maxwell_juettner_Q */
-    double tv1 = (37. - 87. * sin(theta - 28. / 25.)) / (100. * (Th + 1.));
-    double tv2 = pow(1. + (pow(Th, 3. / 5.) / 25. + 7. / 10.) * pow(X, 9. / 25.), 5. / 3.);
+    double sin_t = sth * COS_28_25 - cth * SIN_28_25;   /* sin(theta - 28/25) */
+    double tv1 = (37. - 87. * sin_t) * rcp(100. * (Th + 1.));
+    double tv2 = pow_(1. + (exp_(LT * (3. / 5.)) * (1. / 25.) + 7. / 10.) * exp_(LX * (9. / 25.)), 5. / 3.);
     jV = pre * tv1 * tv2 * e13;
   } else {
     /* Dexter fits, maxwell_juettner_fits.c:287-330 */
-    double nus = 3.0 * EE * B * sth / 4.0 / PI / ME / CL * Th * Th + 1.0;
-    double x = nu / nus;
+    double nus = (3.0 * EE / (4.0 * PI * ME * CL)) * B * sth * Th * Th + 1.0;
+    double x = nu * rcp(nus);
     double II, IQ, IV;
     dexter_IQV(x, II, IQ, IV);
-    double pre = Ne * EE * EE * nu / 2. / sqrt(3.) / CL / Th / Th;
+    double iTh2 = iTh * iTh;
+    double pre = (EE * EE / (2. * SQRT3 * CL)) * Ne * nu * iTh2;
     jI = pre * II;
     jQ = pre * IQ;
-    jV = 2. * Ne * EE * EE * nu / (sth / cth) / 3. / sqrt(3.) / CL / Th / Th / Th * IV;
+    jV = (2. * EE * EE / (3. * SQRT3 * CL)) * Ne * nu * (cth * rcp(sth)) * iTh2 * iTh * IV;
   }
   /* src/model_radiation.c:245-281 */
-  c.jI = jI / nusq;
-  c.jQ = jQ / nusq;
-  c.jU = -0. / nusq;
-  c.jV = jV / nusq;
+  double inu2 = rcp(nusq);
+  c.jI = jI * inu2;
+  c.jQ = jQ * inu2;
+  c.jU = -0. * inu2;
+  c.jV = jV * inu2;
   double jP = sqrt(c.jQ * c.jQ + c.jU * c.jU + c.jV * c.jV);
-  if (c.jI < jP / 0.99) {
-    double pf = c.jI / jP * 0.99;
+  if (c.jI < jP * (1. / 0.99)) {
+    double pf = c.jI * rcp(jP) * 0.99;
     c.jQ *= pf;
     c.jU *= pf;
     c.jV *= pf;
   }
-  if (Bnuinv > 0) {
-    c.aI = c.jI / Bnuinv;
-    c.aQ = c.jQ / Bnuinv;
-    c.aU = c.jU / Bnuinv;
-    c.aV = c.jV / Bnuinv;
-  }
+  c.aI = c.jI * iBnu;
+  c.aQ = c.jQ * iBnu;
+  c.aU = c.jU * iBnu;
+  c.aV = c.jV * iBnu;
   /* rho_Q (Dexter 2016) and rho_V (Shcherbakov), dexter_fit = 0 */
   {
     double xe = xr + 1.e-16;
-    double extraterm = (.011 * exp(-xr / 47.2)
-                        - pow(2., (-1. / 3.)) / pow(3., (23. / 6.)) * PI * 1.e4 * pow(xe, (-8. / 3.)))
-        * (0.5 + 0.5 * tanh((log(xr) - log(120.)) / 0.1));
-    double jff = 2.011 * exp(-pow(xr, 1.035) / 4.7) - cos(xr / 2.) * exp(-pow(xr, 1.2) / 2.73)
-        - .011 * exp(-xr / 47.2) + extraterm;
-    double kr = ks[1] / ks[2];
-    double w2 = w * w;
-    double eps11m22 = jff * wp2 * (omega0 * omega0) / (w2 * w2) * (kr + 6. * Th) * (sth * sth);
-    c.rQ = (2. * PI * nu / (2. * CL) * eps11m22) * nu;
-    double fit = (ks[0] / ks[2]) * (1 - 0.11 * log(1 + 0.035 * xr));
-    double eps12 = wp2 * omega0 / (w2 * w) * fit * cth;
-    c.rV = (2. * PI * nu / CL * eps12) * nu;
+    double Lr = log_(xr);
+    double Le = (xe == xr) ? Lr : log_(xe);
+    double e472 = exp_(-xr * (1. / 47.2));
+    /* 0.5 + 0.5 tanh(z) = 1 / (1 + e^(-2z)) */
+    double mz2 = -2. * (Lr - LOG120) * (1. / 0.1);
+    double logistic = (mz2 > 700.) ? 0. : rcp(1. + exp_(mz2));
+    double extraterm = (.011 * e472 - C_EXTRA * PI * 1.e4 * exp_(Le * (-8. / 3.))) * logistic;
+    double sh2, ch2;
+    sincos_(0.5 * xr, sh2, ch2);
+    double jff = 2.011 * exp_(-exp_(1.035 * Lr) * (1. / 4.7)) - ch2 * exp_(-exp_(1.2 * Lr) * (1. / 2.73))
+        - .011 * e472 + extraterm;
+    double kr = ks[1] * iks2;
+    double iw2 = iw * iw;
+    double eps11m22 = jff * wp2 * (omega0 * omega0) * (iw2 * iw2) * (kr + 6. * Th) * (sth * sth);
+    c.rQ = (PI / CL) * nu * eps11m22 * nu;
+    double fit = (ks[0] * iks2) * (1 - 0.11 * log_(1 + 0.035 * xr));
+    double eps12 = wp2 * omega0 * (iw2 * iw) * fit * cth;
+    c.rV = (2. * PI / CL) * nu * eps12 * nu;
     c.rU = 0. * nu;
   }
 }
@@ -853,9 +903,9 @@ IPG_HD double approximate_solve(double Ii, double ji, double ki, double jf, doub
   double kavg = (ki + kf) / 2.;
   double dtau = dl * kavg;
   tau += dtau;
-  if (dtau < 1.e-3) return Ii + (javg - Ii * kavg) * dl * (1. - (dtau / 2.) * (1. - dtau / 3.));
-  double efac = exp(-dtau);
-  return Ii * efac + (javg / kavg) * (1. - efac);
+  if (dtau < 1.e-3) return Ii + (javg - Ii * kavg) * dl * (1. - (dtau * 0.5) * (1. - dtau * (1. / 3.)));
+  double efac = exp_(-dtau);
+  return Ii * efac + (javg * rcp(kavg)) * (1. - efac);
 }
 
 /* The plasma-frame Stokes step of evolve_N (Faraday half step, constant-
@@ -874,10 +924,13 @@ IPG_HD void faraday_matrix(const Coef &c, double x, double R[3][3])
   const double X[3][3] = {{0., -r[2], r[1]}, {r[2], 0., -r[0]}, {-r[1], r[0], 0.}};
   if (rho2 > 1e-80) {
     double rho = sqrt(rho2);
-    double s, cc;
-    sincos(rho * x, &s, &cc);
-    double sh = sin(0.5 * rho * x);
-    double w = 2. * sh * sh / rho2, v = s / rho;
+    double irho = rcp(rho);
+    /* half angle: sh = sin(rho x / 2), then sin and cos of rho x by the
+     * double-angle identities */
+    double sh, ch;
+    sincos_(0.5 * rho * x, sh, ch);
+    double s = 2. * sh * ch, cc = 1. - 2. * sh * sh;
+    double w = 2. * sh * sh * (irho * irho), v = s * irho;
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++) R[i][j] = (i == j ? cc : 0.) + w * r[i] * r[j] + v * X[i][j];
   } else {
@@ -896,10 +949,11 @@ IPG_HD void absorb_emit_affine(const Coef &c, double x, double A[4][4], double b
     for (int q = 0; q < 4; q++) A[p][q] = 0.;
   if (aP2 > 1e-80) {
     double aP = sqrt(aP2);
+    double iaP = rcp(aP), iaI = rcp(aI);
     double tauP = aP * x, tauI = aI * x;
-    double efacm = exp(-tauI + tauP), efacp = exp(-tauI - tauP), efac = exp(-tauI);
-    double ch = 0.5 * (efacm + efacp), shp = (efacp - efacm) / (2. * aP);
-    double iaP2 = 1. / aP2;
+    double efacm = exp_(-tauI + tauP), efacp = exp_(-tauI - tauP), efac = exp_(-tauI);
+    double ch = 0.5 * (efacm + efacp), shp = (efacp - efacm) * (0.5 * iaP);
+    double iaP2 = iaP * iaP;
     A[0][0] = ch;
     for (int k = 0; k < 3; k++) {
       A[0][k + 1] = shp * a[k];
@@ -920,24 +974,25 @@ IPG_HD void absorb_emit_affine(const Coef &c, double x, double A[4][4], double b
       }
     }
     double adj = a[0] * j[1] + a[1] * j[2] + a[2] * j[3];
-    double m = afacm * (aP * j[0] - adj) / (2. * aP * (aI - aP));
-    double p = afacp * (aP * j[0] + adj) / (2. * aP * (aI + aP));
+    double m = afacm * (aP * j[0] - adj) * (0.5 * iaP) * rcp(aI - aP);
+    double p = afacp * (aP * j[0] + adj) * (0.5 * iaP) * rcp(aI + aP);
     b[0] = m + p;
     for (int k = 0; k < 3; k++)
-      b[k + 1] = afac * (j[k + 1] / aI - a[k] * adj / (aI * aP2)) + (p - m) * a[k] / aP;
+      b[k + 1] = afac * (j[k + 1] * iaI - a[k] * adj * (iaI * iaP2)) + (p - m) * a[k] * iaP;
   } else {
     double dtau = x * aI;
     if (dtau < 1.e-3) {
-      double g = x * (1. - (dtau / 2.) * (1. - dtau / 3.));
+      double g = x * (1. - (dtau * 0.5) * (1. - dtau * (1. / 3.)));
       for (int k = 0; k < 4; k++) {
         A[k][k] = 1. - aI * g;
         b[k] = j[k] * g;
       }
     } else {
-      double efac = exp(-dtau);
+      double efac = exp_(-dtau);
+      double iaI = rcp(aI);
       for (int k = 0; k < 4; k++) {
         A[k][k] = efac;
-        b[k] = (j[k] / aI) * (1. - efac);
+        b[k] = (j[k] * iaI) * (1. - efac);
       }
     }
   }

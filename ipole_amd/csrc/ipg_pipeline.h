@@ -64,7 +64,7 @@ IPG_HD void load_rec(const double *rec, int n, Rec &r)
 }
 
 /* Returns nstep.  Records beyond cap are not written (the caller retraces
- * with a larger cap).  nt (may be null) receives the subring count per
+ * with a larger cap; cap = 0 only counts the steps).  nt (may be null) receives the subring count per
  * step when P.target_nturns >= 0. */
 IPG_HD int trace_ray(const ipgpu_geom &G, const ipgpu_image &P, int pi, int pj, int cap, double *rec,
                      int *nt)
@@ -75,8 +75,8 @@ IPG_HD int trace_ray(const ipgpu_geom &G, const ipgpu_image &P, int pi, int pj, 
   camera_tetrad(G, X, P.old_centering, E, m);
   double K[4];
   init_K(P, E, pi, pj, K);
-  store_rec(rec, X, K, 0.);
-  if (nt) nt[0] = 0;
+  if (cap > 0) store_rec(rec, X, K, 0.);
+  if (nt && cap > 0) nt[0] = 0;
   int passed = (P.Xcam[2] < 0.5) ? 1 : ((P.Xcam[2] > 0.5) ? -1 : 0);
   int nturns = 0;
   double za = 0., zb = 0., x2a = 0., x2b = 0.;
@@ -142,14 +142,40 @@ constexpr int PK_META = 0, PK_DL = 1, PK_JU = 2, PK_KU = 3, PK_HDL = 4, PK_FDL =
               PK_T = 8, PK_CV = 24, PK_E1 = 28, PK_E2 = 32, PK_e1 = 36, PK_e2 = 40, PK_MH = 44,
               PK_MF = 60;
 
+/* Rescaling of a stored trajectory and of the dump for another job of a
+ * batch.  The geodesic in X does not depend on the frequency: K scales with
+ * it and the affine step dl inversely (stepsize, src/model_geodesics.c:
+ * 267-293, is homogeneous of degree -1 in K), so one trace serves every
+ * frequency of a camera.  n_e scales with M_unit and B with its square
+ * root (set_units, model/iharm/model.c:595-608); Theta_e, sigma and beta do
+ * not depend on it.  {1, 1, 1, 1} leaves everything bit for bit unchanged. */
+struct Scale {
+  double kf, ikf, ne, b;
+};
+IPG_HD Scale unit_scale()
+{
+  Scale s;
+  s.kf = s.ikf = s.ne = s.b = 1.;
+  return s;
+}
+
+IPG_HD void scale_rec(Rec &r, const Scale &sc)
+{
+  for (int k = 0; k < 4; k++) r.K[k] *= sc.kf;
+  r.dl *= sc.ikf;
+}
+
 /* Packet of transfer step n (from trajectory point n to n-1) of
  * integrate_emission, src/ipolarray.c:78-180.  nt may be null. */
-IPG_HD void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int n,
-                      const double *rec, const int *nt, const double *dump, double *pk, bool pol)
+IPG_HD_INLINE void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int n,
+                      const double *rec, const int *nt, const double *dump, double *pk, bool pol,
+                      const Scale &sc)
 {
   Rec ti, tf;
   load_rec(rec, n, ti);
   load_rec(rec, n - 1, tf);
+  scale_rec(ti, sc);
+  scale_rec(tf, sc);
   const double conv = Mo.L_unit * HPL / (ME * CL * CL);
   const double dl_cgs = ti.dl * conv;
   Metric m;
@@ -186,6 +212,8 @@ IPG_HD void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_im
     if (zero_emission) meta |= 2;
     Fluid F;
     fluid(G, Mo, dump, tf.X, m, F);
+    F.ne *= sc.ne;
+    F.B *= sc.b;
     Coef c;
     double ju, ku;
     coefficients(F, tf.K, P.emission_type, pol, ju, ku, c);
@@ -231,14 +259,18 @@ IPG_HD void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_im
 /* Unpolarized coefficients at the far end of the trajectory (the cached
  * js, ks of integrate_emission, src/ipolarray.c:74). */
 IPG_HD void far_end_coefficients(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P,
-                                 int ns, const double *rec, const double *dump, double &js, double &ks)
+                                 int ns, const double *rec, const double *dump, double &js, double &ks,
+                                 const Scale &sc)
 {
   Rec t;
   load_rec(rec, ns, t);
+  scale_rec(t, sc);
   Metric m;
   metric(G, t.X[1], t.X[2], m);
   Fluid F;
   fluid(G, Mo, dump, t.X, m, F);
+  F.ne *= sc.ne;
+  F.B *= sc.b;
   Coef c;
   coefficients(F, t.K, P.emission_type, false, js, ks, c);
 }
@@ -296,84 +328,219 @@ IPG_HD int stokes_apply(Herm &N, const double *pk, int stokes_floors, double &ta
   return flag;
 }
 
-/* The state recurrence of integrate_emission over the packets of one ray
- * (packet 0 is step ns, packet ns-1 is step 1), then project_N.
- * out7 = Intensity, Is, Qs, Us, Vs, Tau, tauF (before the nu^3 scaling of
- * save_pixel); returns the oddflag bits; nrad counts radiating steps. */
-IPG_HD int chain_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const double *pkt, int stride,
-                     double js, double ks, double out7[7], int &nrad)
-{
-  const bool pol = stride == PK;
+/* State of the transfer recurrence of one ray. */
+struct Chain {
   Herm N;
-  for (int q = 0; q < 4; q++) N.d[q] = 0.;
-  for (int q = 0; q < 6; q++) N.re[q] = N.im[q] = 0.;
-  double I = 0., tau = 0., tauF = 0.;
-  int flag = 0;
-  nrad = 0;
-  if (pol) {
-    bool Nnz = false;
-    double Mi[4][4];
-    Pkt cur, nxt;
-    if (ns > 0) load_pkt(pkt, PK, cur);
-    for (int q = 0; q < ns; q++) {
-      if (q + 1 < ns) load_pkt(pkt + (size_t)(q + 1) * PK, PK, nxt);
-      const double *pk = cur.v;
-      const int meta = (int)pk[PK_META];
-      if (Nnz) {
-        /* push_polar twice, src/ipolarray.c:85-89 */
-        double Mh[4][4];
-        for (int i = 0; i < 4; i++)
-          for (int k = 0; k < 4; k++) Mh[i][k] = pk[PK_MH + 4 * i + k];
-        Herm Nh;
-        transport(N, N, Mi, pk[PK_HDL], Nh);
-        transport(N, Nh, Mh, pk[PK_FDL], N);
-      }
-      if (meta & 1) {
-        nrad++;
-        double jf = pk[PK_JU], kf = pk[PK_KU];
-        double ji = (meta & 2) ? 0. : js, ki = ks;
-        js = jf;
-        ks = kf;
-        I = approximate_solve(I, ji, ki, (meta & 2) ? 0. : jf, kf, pk[PK_DL], tau);
-        flag |= stokes_apply(N, pk, P.stokes_floors, tauF);
-        flag |= meta >> 2;
-        Nnz = herm_nonzero(N);
-      }
-      if (Nnz)
-        for (int i = 0; i < 4; i++)
-          for (int k = 0; k < 4; k++) Mi[i][k] = pk[PK_MF + 4 * i + k];
-      cur = nxt;
-    }
-  } else {
-    for (int q = 0; q < ns; q++) {
-      const double *pk = pkt + (size_t)q * stride;
-      const int meta = (int)pk[PK_META];
-      if (meta & 1) {
-        nrad++;
-        double jf = pk[PK_JU], kf = pk[PK_KU];
-        double ji = (meta & 2) ? 0. : js, ki = ks;
-        js = jf;
-        ks = kf;
-        I = approximate_solve(I, ji, ki, (meta & 2) ? 0. : jf, kf, pk[PK_DL], tau);
-      }
+  double Mi[4][4];
+  double I, tau, tauF, js, ks;
+  int flag, nrad;
+  bool Nnz;
+};
+
+IPG_HD void chain_init(Chain &st, double js, double ks)
+{
+  for (int q = 0; q < 4; q++) st.N.d[q] = 0.;
+  for (int q = 0; q < 6; q++) st.N.re[q] = st.N.im[q] = 0.;
+  st.I = st.tau = st.tauF = 0.;
+  st.js = js;
+  st.ks = ks;
+  st.flag = st.nrad = 0;
+  st.Nnz = false;
+}
+
+/* One step of integrate_emission's loop body (src/ipolarray.c:78-180) on
+ * the state, from the step's packet. */
+IPG_HD_INLINE void chain_step(Chain &st, const double *pk, bool pol, int stokes_floors)
+{
+  const int meta = (int)pk[PK_META];
+  if (pol && st.Nnz) {
+    /* push_polar twice, src/ipolarray.c:85-89 */
+    double Mh[4][4];
+    for (int i = 0; i < 4; i++)
+      for (int k = 0; k < 4; k++) Mh[i][k] = pk[PK_MH + 4 * i + k];
+    Herm Nh;
+    transport(st.N, st.N, st.Mi, pk[PK_HDL], Nh);
+    transport(st.N, Nh, Mh, pk[PK_FDL], st.N);
+  }
+  if (meta & 1) {
+    st.nrad++;
+    double jf = pk[PK_JU], kf = pk[PK_KU];
+    double ji = (meta & 2) ? 0. : st.js, ki = st.ks;
+    st.js = jf;
+    st.ks = kf;
+    st.I = approximate_solve(st.I, ji, ki, (meta & 2) ? 0. : jf, kf, pk[PK_DL], st.tau);
+    if (pol) {
+      st.flag |= stokes_apply(st.N, pk, stokes_floors, st.tauF);
+      st.flag |= meta >> 2;
+      st.Nnz = herm_nonzero(st.N);
     }
   }
+  if (pol && st.Nnz)
+    for (int i = 0; i < 4; i++)
+      for (int k = 0; k < 4; k++) st.Mi[i][k] = pk[PK_MF + 4 * i + k];
+}
+
+/* project_N (src/ipolarray.c:578-598) and the per-ray outputs:
+ * out7 = Intensity, Is, Qs, Us, Vs, Tau, tauF (before the nu^3 scaling of
+ * save_pixel); returns the oddflag bits. */
+IPG_HD int chain_finish(const ipgpu_geom &G, const ipgpu_image &P, const Chain &st, double out7[7])
+{
   double S[4] = {0., 0., 0., 0.};
-  if (pol) {
+  if (!P.only_unpolarized) {
     double E[4][4];
     Metric m;
     const double Xc[4] = {P.Xcam[0], P.Xcam[1], P.Xcam[2], P.Xcam[3]};
     camera_tetrad(G, Xc, 0, E, m);
-    project(N, E, m, P.rotcam, S);
+    project(st.N, E, m, P.rotcam, S);
   }
-  out7[0] = I;
+  out7[0] = st.I;
   out7[1] = S[0];
   out7[2] = S[1];
   out7[3] = S[2];
   out7[4] = S[3];
-  out7[5] = tau;
-  out7[6] = tauF;
-  return flag;
+  out7[5] = st.tau;
+  out7[6] = st.tauF;
+  return st.flag;
+}
+
+/* The state recurrence over the packets of one ray held in memory (packet
+ * 0 is step ns, packet ns-1 is step 1), then project_N.  On the device the
+ * next packet is loaded while the current step computes. */
+IPG_HD int chain_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const double *pkt, int stride,
+                     double js, double ks, double out7[7], int &nrad)
+{
+  const bool pol = stride == PK;
+  Chain st;
+  chain_init(st, js, ks);
+  if (pol) {
+    Pkt cur, nxt;
+    if (ns > 0) load_pkt(pkt, PK, cur);
+    for (int q = 0; q < ns; q++) {
+      if (q + 1 < ns) load_pkt(pkt + (size_t)(q + 1) * PK, PK, nxt);
+      chain_step(st, cur.v, true, P.stokes_floors);
+      cur = nxt;
+    }
+  } else {
+    for (int q = 0; q < ns; q++) chain_step(st, pkt + (size_t)q * stride, false, P.stokes_floors);
+  }
+  nrad = st.nrad;
+  return chain_finish(G, P, st, out7);
+}
+
+/* One transfer step fused: the work of prep_step and chain_step in the
+ * order that keeps the fewest values live, with nothing written to memory.
+ * The same operations on the same values as the split path; the half-step
+ * metric and the end-point transport matrix are only formed while N is
+ * nonzero (transport of N = 0 is the identity). */
+IPG_HD_INLINE void fused_step(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int n,
+                              const double *rec, const int *nt, const double *dump, const Scale &sc,
+                              bool pol, Chain &st)
+{
+  Rec ti, tf;
+  load_rec(rec, n, ti);
+  load_rec(rec, n - 1, tf);
+  scale_rec(ti, sc);
+  scale_rec(tf, sc);
+  const double conv = Mo.L_unit * HPL / (ME * CL * CL);
+  const double dl_cgs = ti.dl * conv;
+  Metric m;
+  metric(G, tf.X[1], tf.X[2], m);
+  if (pol && st.Nnz) {
+    /* push_polar twice, src/ipolarray.c:85-89 */
+    double Mh[4][4];
+    {
+      double Xh[4], Kh[4];
+      half_step(m, tf.X, tf.K, -ti.dl, Xh, Kh);
+      Metric mh;
+      metric(G, Xh[1], Xh[2], mh);
+      transport_matrix(mh, Kh, Mh);
+    }
+    Herm Nh;
+    transport(st.N, st.N, st.Mi, (0.5 * dl_cgs) / conv, Nh);
+    transport(st.N, Nh, Mh, dl_cgs / conv, st.N);
+  }
+  if (radiating_region(G, Mo, m.r, m.th)) {
+    st.nrad++;
+    bool zero_emission = false;
+    if (P.target_nturns >= 0 && nt && nt[n] != P.target_nturns) zero_emission = true;
+    const double half_x2 = (G.cstopx[2] - G.cstartx[2]) / 2;
+    if (P.counterjet == 1) {
+      if (tf.X[2] < half_x2) zero_emission = true;
+    } else if (P.counterjet == 2) {
+      if (tf.X[2] > half_x2) zero_emission = true;
+    }
+    Fluid F;
+    fluid(G, Mo, dump, tf.X, m, F);
+    F.ne *= sc.ne;
+    F.B *= sc.b;
+    Coef c;
+    double jf, kf;
+    coefficients(F, tf.K, P.emission_type, pol, jf, kf, c);
+    double ji = zero_emission ? 0. : st.js, ki = st.ks;
+    st.js = jf;
+    st.ks = kf;
+    st.I = approximate_solve(st.I, ji, ki, zero_emission ? 0. : jf, kf, dl_cgs, st.tau);
+    if (pol) {
+      if (zero_emission) c.jI = c.jQ = c.jU = c.jV = 0.;
+      /* evolve_N, src/ipolarray.c:332-575 */
+      double Bcon[4] = {F.Bcon[0], F.Bcon[1], F.Bcon[2], F.Bcon[3]};
+      double bsq = F.Bcon[0] * F.Bcov[0] + F.Bcon[1] * F.Bcov[1] + F.Bcon[2] * F.Bcov[2]
+          + F.Bcon[3] * F.Bcov[3];
+      if (bsq <= 0.) {
+        Bcon[0] = 0.;
+        Bcon[1] = 1.;
+        Bcon[2] = 1.;
+        Bcon[3] = 1.;
+      }
+      double E[4][4];
+      int flag = plasma_tetrad(F.Ucon, tf.K, Bcon, m.g, m.sqrtmg, E);
+      if (isnan(F.Ucov[0])) flag |= 8;
+      double e1[4], e2[4];
+      lower(m.g, E[1], e1);
+      lower(m.g, E[2], e2);
+      double T[4][4], cv[4];
+      stokes_affine(c, dl_cgs, T, cv);
+      double S0[4], S[4];
+      herm_to_stokes(st.N, e1, e2, S0);
+      for (int i = 0; i < 4; i++) S[i] = cv[i] + T[i][0] * S0[0] + T[i][1] * S0[1] + T[i][2] * S0[2] + T[i][3] * S0[3];
+      st.tauF += dl_cgs * fabs(c.rV);
+      if (P.stokes_floors) {
+        if (S[0] < 0) {
+          S[0] = S[1] = S[2] = S[3] = 0.;
+        } else {
+          double pf = sqrt(S[1] * S[1] + S[2] * S[2] + S[3] * S[3]) / S[0];
+          if (pf > 1.) {
+            S[1] /= pf;
+            S[2] /= pf;
+            S[3] /= pf;
+          }
+        }
+      }
+      double dS[4] = {S[0] - S0[0], S[1] - S0[1], S[2] - S0[2], S[3] - S0[3]};
+      herm_add_stokes(st.N, E[1], E[2], dS);
+      if (st.tauF > 1.e100 || st.tauF < -1.e100 || isnan(st.tauF)) flag |= 2;
+      if (isnan(st.N.d[0]) || isnan(S[0])) flag |= 4;
+      st.flag |= flag;
+      st.Nnz = herm_nonzero(st.N);
+    }
+  }
+  if (pol && st.Nnz) transport_matrix(m, tf.K, st.Mi);
+}
+
+/* Whole transfer of one ray, step by step in registers (no packet memory):
+ * for launches with enough rays to fill the chip. */
+IPG_HD int transfer_fused(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int ns,
+                          const double *rec, const int *nt, const double *dump, const Scale &sc,
+                          double out7[7], int &nrad)
+{
+  const bool pol = !P.only_unpolarized;
+  double js, ks;
+  far_end_coefficients(G, Mo, P, ns, rec, dump, js, ks, sc);
+  Chain st;
+  chain_init(st, js, ks);
+  for (int q = 0; q < ns; q++) fused_step(G, Mo, P, ns - q, rec, nt, dump, sc, pol, st);
+  nrad = st.nrad;
+  return chain_finish(G, P, st, out7);
 }
 
 }  // namespace ipg

@@ -22,6 +22,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -95,32 +96,39 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(int nrays, const int *__r
   if (t == SCAN_THREADS - 1) off[nrays] = part[t];
 }
 
+/* Ray of the first step of each block of PREP_BLOCK packets: one thread per
+ * ray writes the blocks that start inside its steps. */
+constexpr int PREP_BLOCK = 256;
+__global__ void k_blockmap(int nrays, const long long *__restrict__ off, int *__restrict__ blk_ray)
+{
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  long long b0 = (off[r] + PREP_BLOCK - 1) / PREP_BLOCK, b1 = (off[r + 1] + PREP_BLOCK - 1) / PREP_BLOCK;
+  for (long long b = b0; b < b1; b++) blk_ray[b] = r;
+}
+
 /* One lane per transfer step of every ray: the state-independent part of
  * integrate_emission (transport matrices, fluid, coefficients, tetrad). */
-__global__ __launch_bounds__(256) void k_prep(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P, int nrays,
-                                              int cap, int stride, long long total,
-                                              const long long *__restrict__ off,
-                                              const int *__restrict__ nstep_in,
-                                              const double *__restrict__ traj,
-                                              const int *__restrict__ nturns_buf,
-                                              const double *__restrict__ dump, double *__restrict__ pkt)
+__global__ __launch_bounds__(PREP_BLOCK) void k_prep(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P, int nrays,
+                                                     int cap, int stride, long long total,
+                                                     const long long *__restrict__ off,
+                                                     const int *__restrict__ blk_ray,
+                                                     const int *__restrict__ nstep_in,
+                                                     const double *__restrict__ traj,
+                                                     const int *__restrict__ nturns_buf,
+                                                     const double *__restrict__ dump, double *__restrict__ pkt)
 {
-  long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long g = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
   if (g >= total) return;
-  /* ray = last r with off[r] <= g */
-  int lo = 0, hi = nrays;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (off[mid] <= g) lo = mid;
-    else hi = mid;
-  }
-  const int ray = lo;
+  /* ray = last r with off[r] <= g, from the block's first ray forward */
+  int ray = blk_ray[blockIdx.x];
+  while (ray + 1 < nrays && off[ray + 1] <= g) ray++;
   const int ns = nstep_in[ray];
   if (ns >= P.maxnstep - 1) return;
   const int q = (int)(g - off[ray]);
   const double *rec = traj + (size_t)ray * cap * REC;
   const int *nt = nturns_buf ? nturns_buf + (size_t)ray * cap : nullptr;
-  prep_step(G, Mo, P, ns - q, rec, nt, dump, pkt + (size_t)g * stride, stride == PK);
+  prep_step(G, Mo, P, ns - q, rec, nt, dump, pkt + (size_t)g * stride, stride == PK, unit_scale());
 }
 
 /* One lane per ray: the recurrence of N and I over the ray's packets. */
@@ -149,7 +157,7 @@ __global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ip
   }
   const double *rec = traj + (size_t)ray * cap * REC;
   double js, ks;
-  far_end_coefficients(G, Mo, P, ns, rec, dump, js, ks);
+  far_end_coefficients(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
   double res[7];
   int nrad;
   flags[ray] = chain_ray(G, P, ns, pkt + (size_t)off[ray] * stride, stride, js, ks, res, nrad);
@@ -328,6 +336,136 @@ __global__ void k_unpack(size_t ncell, const double *__restrict__ dump, double *
   soa11[10 * ncell + c] = d[S_BETA];
 }
 
+/* ------------------------------------------------------------- batches */
+
+/* A trace group: one camera/image geometry traced once for all its jobs. */
+struct BGroup {
+  ipgpu_image P;      /* image parameters at the trace frequency */
+  long long ray_base; /* global index of the group's first ray */
+  int nrays;          /* tiles * 64 */
+  int pad;
+};
+
+/* A job of a batch on the device. */
+struct BJob {
+  ipgpu_image P;      /* the job's own image parameters */
+  Scale sc;           /* relative to its group's trace and the uploaded dump */
+  long long pix_off;  /* first pixel of the job in the batch output arrays */
+  int group;
+  int pad;
+};
+
+__device__ __forceinline__ int group_of(const BGroup *__restrict__ groups, int ngroups, long long r)
+{
+  int lo = 0, hi = ngroups;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (groups[mid].ray_base <= r) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+/* Geodesic step counts of all rays of all groups (nothing stored). */
+__global__ __launch_bounds__(WAVE) void k_trace_count(ipgpu_geom G, const BGroup *__restrict__ groups,
+                                                      int ngroups, long long nr, int *__restrict__ nstep_out)
+{
+  long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nr) return;
+  const int g = group_of(groups, ngroups, r);
+  const ipgpu_image P = groups[g].P;
+  int pi, pj;
+  ray_pixel((int)(r - groups[g].ray_base), P.nx, P.ny, pi, pj);
+  if (pi >= P.nx || pj >= P.ny) {
+    nstep_out[r] = 0;
+    return;
+  }
+  nstep_out[r] = trace_ray(G, P, pi, pj, 0, nullptr, nullptr);
+}
+
+/* The same geodesics again, longest first, each written to its exact slice
+ * of the compact record buffer (off: first record of the ray). */
+__global__ __launch_bounds__(WAVE) void k_trace_store(ipgpu_geom G, const BGroup *__restrict__ groups,
+                                                      int ngroups, const int *__restrict__ order, int nr,
+                                                      const long long *__restrict__ off,
+                                                      const int *__restrict__ nstep_in, double *__restrict__ traj,
+                                                      int *__restrict__ nturns_buf, int *__restrict__ status,
+                                                      unsigned long long *__restrict__ counts)
+{
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nr) return;
+  const int r = order[t];
+  const int g = group_of(groups, ngroups, r);
+  const ipgpu_image P = groups[g].P;
+  int pi, pj;
+  ray_pixel((int)(r - groups[g].ray_base), P.nx, P.ny, pi, pj);
+  const int ns = nstep_in[r];
+  double *rec = traj + (size_t)off[r] * REC;
+  int *nt = (nturns_buf && P.target_nturns >= 0) ? nturns_buf + off[r] : nullptr;
+  int nstep = trace_ray(G, P, pi, pj, ns + 1, rec, nt);
+  if (nstep != ns) atomicAdd(&status[0], 1);   /* the two passes are the same arithmetic */
+  atomicAdd(&counts[0], (unsigned long long)nstep);
+}
+
+/* One lane per (job, ray) work item: the fused transfer of the ray for the
+ * job, then the job's pixel in the reference layout (save_pixel,
+ * src/main.c:885-910).  bstatus[0]: min of (job << 32 | pixel) + 1 over
+ * rays past maxnstep; bstatus[1]: the same over NaN pixels. */
+__global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model Mo, const BJob *__restrict__ jobs,
+                                                     const BGroup *__restrict__ groups,
+                                                     const int2 *__restrict__ items, int nitems,
+                                                     const long long *__restrict__ off,
+                                                     const double *__restrict__ traj,
+                                                     const int *__restrict__ nstep_in,
+                                                     const int *__restrict__ nturns_buf,
+                                                     const double *__restrict__ dump, double *__restrict__ image,
+                                                     double *__restrict__ imageS, double *__restrict__ taus,
+                                                     int *__restrict__ nstep_pix, int *__restrict__ flag_pix,
+                                                     unsigned long long *__restrict__ bstatus,
+                                                     unsigned long long *__restrict__ counts)
+{
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nitems) return;
+  const int2 it = items[w];
+  const int jb = it.x, r = it.y;
+  const BJob J = jobs[jb];
+  const ipgpu_image &P = J.P;
+  int pi, pj;
+  ray_pixel((int)(r - groups[J.group].ray_base), P.nx, P.ny, pi, pj);
+  const long long pix = (long long)pi * P.ny + pj;
+  const size_t p = (size_t)(J.pix_off + pix);
+  const int ns = nstep_in[r];
+  nstep_pix[p] = ns;
+  if (ns >= P.maxnstep - 1) {
+    atomicMin(&bstatus[0], ((unsigned long long)jb << 32 | (unsigned long long)pix) + 1ull);
+    image[p] = 0.;
+    taus[p] = 0.;
+    flag_pix[p] = 0;
+    if (!P.only_unpolarized)
+      for (int q = 0; q < 5; q++) imageS[p * 5 + q] = 0.;
+    return;
+  }
+  const double *rec = traj + (size_t)off[r] * REC;
+  const int *nt = (nturns_buf && P.target_nturns >= 0) ? nturns_buf + off[r] : nullptr;
+  double o[7];
+  int nrad;
+  int flag = transfer_fused(G, Mo, P, ns, rec, nt, dump, J.sc, o, nrad);
+  const double f3 = P.freqcgs * P.freqcgs * P.freqcgs;
+  image[p] = o[0] * f3;
+  taus[p] = o[5];
+  flag_pix[p] = flag;
+  if (!P.only_unpolarized) {
+    double qs = (P.qu_conv == 0) ? -1. : 1.;
+    imageS[p * 5 + 0] = o[1] * f3;
+    imageS[p * 5 + 1] = qs * o[2] * f3;
+    imageS[p * 5 + 2] = qs * o[3] * f3;
+    imageS[p * 5 + 3] = o[4] * f3;
+    imageS[p * 5 + 4] = o[6];
+    if (isnan(o[1])) atomicMin(&bstatus[1], ((unsigned long long)jb << 32 | (unsigned long long)pix) + 1ull);
+  }
+  atomicAdd(&counts[1], (unsigned long long)nrad);
+}
+
 }  // namespace
 
 /* ------------------------------------------------------------------ C-ABI */
@@ -351,6 +489,8 @@ struct ipgpu_ctx {
   size_t pkt_doubles = 0;
   long long *d_off = nullptr;
   size_t cap_off = 0;
+  int *d_blk = nullptr;
+  size_t cap_blk = 0;
   int *d_nstep_ray = nullptr, *d_flag_ray = nullptr;
   double *d_out7 = nullptr;
   double *d_image = nullptr, *d_imageS = nullptr, *d_taus = nullptr;
@@ -362,8 +502,20 @@ struct ipgpu_ctx {
   unsigned long long counts[2] = {0, 0};
   int last_npix = 0, last_unpol = 0, have_result = 0;
   float ms_geo = 0, ms_xfer = 0, ms_total = 0, ms_stage[4] = {0, 0, 0, 0};
-  int bad_i = -1, bad_j = -1;
+  int bad_i = -1, bad_j = -1, bad_job = -1;
   int traj_cap_steps = 8192;
+  /* batches */
+  BGroup *d_groups = nullptr;
+  size_t cap_groups = 0;
+  BJob *d_jobs = nullptr;
+  size_t cap_jobs = 0;
+  int2 *d_items = nullptr;
+  size_t cap_items = 0;
+  double *d_bimage = nullptr, *d_bimageS = nullptr, *d_btaus = nullptr;
+  int *d_bnstep = nullptr, *d_bflag = nullptr;
+  size_t cap_bimage = 0, cap_bimageS = 0, cap_btaus = 0, cap_bnstep = 0, cap_bflag = 0;
+  unsigned long long *d_bstatus = nullptr;
+  size_t batch_traj_bytes = (size_t)160 << 30; /* geodesic records per pass of a batch */
 };
 
 #define HIPCHK(ctx, call)                                                                    \
@@ -427,9 +579,11 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
   if (!ctx) return;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
-  void *ptrs[] = {ctx->d_dump, ctx->d_traj, ctx->d_pkt, ctx->d_off, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
+  void *ptrs[] = {ctx->d_dump, ctx->d_traj, ctx->d_pkt, ctx->d_off, ctx->d_blk, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
                   ctx->d_out7, ctx->d_image, ctx->d_imageS, ctx->d_taus, ctx->d_nstep_pix,
-                  ctx->d_flag_pix, ctx->d_status, ctx->d_counts};
+                  ctx->d_flag_pix, ctx->d_status, ctx->d_counts,
+                  ctx->d_groups, ctx->d_jobs, ctx->d_items, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
+                  ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int q = 0; q < 5; q++)
@@ -624,10 +778,13 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
   k_scan<<<1, SCAN_THREADS, 0, ctx->stream>>>(nrays, ctx->d_nstep_ray, ctx->d_off);
   HIPCHK(ctx, hipGetLastError());
   if (total > 0) {
-    unsigned nb = (unsigned)((total + 255) / 256);
-    k_prep<<<nb, 256, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, (long long)total,
-                                        ctx->d_off, ctx->d_nstep_ray, ctx->d_traj,
-                                        turns ? ctx->d_nturns : nullptr, ctx->d_dump, ctx->d_pkt);
+    unsigned nb = (unsigned)((total + PREP_BLOCK - 1) / PREP_BLOCK);
+    if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)nb))) return rc;
+    k_blockmap<<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(nrays, ctx->d_off, ctx->d_blk);
+    HIPCHK(ctx, hipGetLastError());
+    k_prep<<<nb, PREP_BLOCK, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, (long long)total,
+                                               ctx->d_off, ctx->d_blk, ctx->d_nstep_ray, ctx->d_traj,
+                                               turns ? ctx->d_nturns : nullptr, ctx->d_dump, ctx->d_pkt);
     HIPCHK(ctx, hipGetLastError());
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
@@ -683,6 +840,249 @@ int ipgpu_fetch(ipgpu_ctx *ctx, double *image, double *imageS, double *taus, int
   if (nstep) HIPCHK(ctx, hipMemcpyAsync(nstep, ctx->d_nstep_pix, npix * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   if (flag) HIPCHK(ctx, hipMemcpyAsync(flag, ctx->d_flag_pix, npix * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return IPGPU_OK;
+}
+
+/* ------------------------------------------------------------ batched render */
+
+static bool same_trace(const ipgpu_image &a, const ipgpu_image &b)
+{
+  for (int k = 0; k < 4; k++)
+    if (a.Xcam[k] != b.Xcam[k]) return false;
+  return a.nx == b.nx && a.ny == b.ny && a.fovx == b.fovx && a.fovy == b.fovy && a.rotcam == b.rotcam
+      && a.xoff == b.xoff && a.yoff == b.yoff && a.eps == b.eps && a.maxnstep == b.maxnstep
+      && a.old_centering == b.old_centering && a.target_nturns == b.target_nturns
+      && a.subring_dtheta == b.subring_dtheta && a.freq > 0. && b.freq > 0.;
+}
+
+static int group_of_host(const std::vector<BGroup> &groups, long long r)
+{
+  int lo = 0, hi = (int)groups.size();
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (groups[mid].ray_base <= r) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+/* host copy of ray_pixel */
+static void host_ray_pixel(int ray, int nx, int ny, int &i, int &j)
+{
+  int tiles_y = (ny + TILE - 1) / TILE;
+  int t = ray / WAVE, l = ray % WAVE;
+  i = (t / tiles_y) * TILE + l / TILE;
+  j = (t % tiles_y) * TILE + l % TILE;
+}
+
+int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
+{
+  if (!ctx || njobs <= 0 || !jobs) return IPGPU_ERR_ARG;
+  if (!ctx->has_geom || !ctx->has_model) return set_err(ctx, IPGPU_ERR_STATE, "geometry/model not set");
+  if (ctx->model.kind == IPGPU_MODEL_IHARM && !ctx->d_dump)
+    return set_err(ctx, IPGPU_ERR_STATE, "no fluid dump uploaded");
+  for (int j = 0; j < njobs; j++) {
+    const ipgpu_image &P = jobs[j].image;
+    if (P.nx <= 0 || P.ny <= 0 || P.maxnstep < 2 || !(P.freq > 0.) || !(jobs[j].m_unit_scale > 0.))
+      return set_err(ctx, IPGPU_ERR_ARG, "bad job: image size, maxnstep, frequency or m_unit_scale");
+    if (P.emission_type != 4 && P.emission_type != 1)
+      return set_err(ctx, IPGPU_ERR_ARG, "emission_type must be 4 or 1");
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int rc;
+  /* trace groups */
+  std::vector<BGroup> groups;
+  std::vector<BJob> bj(njobs);
+  std::vector<std::vector<int>> members;
+  long long pix_total = 0;
+  bool turns = false;
+  for (int j = 0; j < njobs; j++) {
+    const ipgpu_image &P = jobs[j].image;
+    int g = -1;
+    for (size_t q = 0; q < groups.size(); q++)
+      if (same_trace(groups[q].P, P)) {
+        g = (int)q;
+        break;
+      }
+    if (g < 0) {
+      BGroup gr{};
+      gr.P = P;
+      gr.nrays = ((P.nx + TILE - 1) / TILE) * ((P.ny + TILE - 1) / TILE) * WAVE;
+      gr.ray_base = groups.empty() ? 0 : groups.back().ray_base + groups.back().nrays;
+      groups.push_back(gr);
+      members.emplace_back();
+      g = (int)groups.size() - 1;
+    }
+    members[g].push_back(j);
+    BJob &b = bj[j];
+    b.P = P;
+    b.group = g;
+    b.pix_off = pix_total;
+    pix_total += (long long)P.nx * P.ny;
+    double kf = P.freq / groups[g].P.freq;
+    b.sc.kf = kf;
+    b.sc.ikf = 1. / kf;
+    b.sc.ne = jobs[j].m_unit_scale;
+    b.sc.b = sqrt(jobs[j].m_unit_scale);
+    if (P.target_nturns >= 0) turns = true;
+  }
+  const long long rays_total = groups.back().ray_base + groups.back().nrays;
+  const int ngroups = (int)groups.size();
+  if ((rc = grow(ctx, ctx->d_groups, ctx->cap_groups, groups.size()))) return rc;
+  if ((rc = grow(ctx, ctx->d_jobs, ctx->cap_jobs, (size_t)njobs))) return rc;
+  if ((rc = grow(ctx, ctx->d_bimage, ctx->cap_bimage, (size_t)pix_total))) return rc;
+  if ((rc = grow(ctx, ctx->d_bimageS, ctx->cap_bimageS, (size_t)pix_total * 5))) return rc;
+  if ((rc = grow(ctx, ctx->d_btaus, ctx->cap_btaus, (size_t)pix_total))) return rc;
+  if ((rc = grow(ctx, ctx->d_bnstep, ctx->cap_bnstep, (size_t)pix_total))) return rc;
+  if ((rc = grow(ctx, ctx->d_bflag, ctx->cap_bflag, (size_t)pix_total))) return rc;
+  if (!ctx->d_bstatus) HIPCHK(ctx, hipMalloc((void **)&ctx->d_bstatus, 2 * sizeof(unsigned long long)));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_groups, groups.data(), groups.size() * sizeof(BGroup), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_jobs, bj.data(), bj.size() * sizeof(BJob), hipMemcpyHostToDevice, ctx->stream));
+  const unsigned long long bst_init[2] = {~0ull, ~0ull};
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_bstatus, bst_init, sizeof(bst_init), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_counts, 0, 2 * sizeof(unsigned long long), ctx->stream));
+
+  float ms_geo = 0.f, ms_xfer = 0.f, t = 0.f;
+  HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+  /* pass 1: step counts of every ray */
+  if ((rc = grow(ctx, ctx->d_nstep_ray, ctx->cap_nstep_ray, (size_t)rays_total))) return rc;
+  if ((rc = grow(ctx, ctx->d_off, ctx->cap_off, (size_t)rays_total))) return rc;
+  HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+  k_trace_count<<<(unsigned)((rays_total + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>(
+      ctx->geom, ctx->d_groups, ngroups, rays_total, ctx->d_nstep_ray);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+  std::vector<int> ns_host(rays_total);
+  HIPCHK(ctx, hipMemcpyAsync(ns_host.data(), ctx->d_nstep_ray, rays_total * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  hipEventElapsedTime(&t, ctx->ev[1], ctx->ev[2]);
+  ms_geo += t;
+  /* rays with a pixel, longest first (counting sort) */
+  std::vector<char> valid(rays_total, 0);
+  int mxs = 0;
+  for (int g = 0; g < ngroups; g++)
+    for (int lr = 0; lr < groups[g].nrays; lr++) {
+      int pi, pj;
+      host_ray_pixel(lr, groups[g].P.nx, groups[g].P.ny, pi, pj);
+      if (pi < groups[g].P.nx && pj < groups[g].P.ny) {
+        long long r = groups[g].ray_base + lr;
+        valid[r] = 1;
+        mxs = ns_host[r] > mxs ? ns_host[r] : mxs;
+      }
+    }
+  std::vector<long long> bucket(mxs + 2, 0);
+  for (long long r = 0; r < rays_total; r++)
+    if (valid[r]) bucket[mxs - ns_host[r] + 1]++;
+  for (int k = 1; k <= mxs + 1; k++) bucket[k] += bucket[k - 1];
+  const long long nvalid = bucket[mxs + 1];
+  std::vector<int> order(nvalid);
+  for (long long r = 0; r < rays_total; r++)
+    if (valid[r]) order[bucket[mxs - ns_host[r]]++] = (int)r;
+  std::vector<long long> off_host(rays_total, 0);
+  const long long budget = (long long)(ctx->batch_traj_bytes / (REC * sizeof(double)));
+  std::vector<int2> items;
+  for (long long a = 0; a < nvalid;) {
+    /* the longest prefix of the remaining rays whose records fit */
+    long long b = a, recs = 0;
+    while (b < nvalid && (b == a || recs + ns_host[order[b]] + 1 <= budget)) {
+      off_host[order[b]] = recs;
+      recs += ns_host[order[b]] + 1;
+      b++;
+    }
+    const int nr = (int)(b - a);
+    if ((rc = grow(ctx, ctx->d_traj, ctx->traj_doubles, (size_t)recs * REC))) return rc;
+    if (turns && (rc = grow(ctx, ctx->d_nturns, ctx->nturns_ints, (size_t)recs))) return rc;
+    if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)nr))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_off, off_host.data(), rays_total * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_blk, order.data() + a, nr * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    k_trace_store<<<(nr + WAVE - 1) / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->d_groups, ngroups, ctx->d_blk, nr,
+                                                                   ctx->d_off, ctx->d_nstep_ray, ctx->d_traj,
+                                                                   turns ? ctx->d_nturns : nullptr, ctx->d_status,
+                                                                   ctx->d_counts);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    /* work items (job, ray) of these rays, in the rays' (longest first) order */
+    items.clear();
+    for (long long q = a; q < b; q++) {
+      int r = order[q];
+      int g = group_of_host(groups, r);
+      for (int jb : members[g]) items.push_back(make_int2(jb, r));
+    }
+    const int nitems = (int)items.size();
+    if ((rc = grow(ctx, ctx->d_items, ctx->cap_items, (size_t)(nitems > 0 ? nitems : 1)))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, items.data(), nitems * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    if (nitems > 0) {
+      k_xfer_batch<<<(nitems + WAVE - 1) / WAVE, WAVE, 0, ctx->stream>>>(
+          ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items, nitems, ctx->d_off, ctx->d_traj,
+          ctx->d_nstep_ray, turns ? ctx->d_nturns : nullptr, ctx->d_dump, ctx->d_bimage, ctx->d_bimageS,
+          ctx->d_btaus, ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts);
+      HIPCHK(ctx, hipGetLastError());
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+    int mism = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&mism, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (mism) return set_err(ctx, IPGPU_ERR_HIP, "geodesic step counts differ between the two trace passes");
+    hipEventElapsedTime(&t, ctx->ev[1], ctx->ev[2]);
+    ms_geo += t;
+    hipEventElapsedTime(&t, ctx->ev[4], ctx->ev[3]);
+    ms_xfer += t;
+    a = b;
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+  unsigned long long bst[2];
+  HIPCHK(ctx, hipMemcpyAsync(bst, ctx->d_bstatus, sizeof(bst), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->counts, ctx->d_counts, sizeof(ctx->counts), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[3]);
+  ctx->ms_geo = ms_geo;
+  ctx->ms_xfer = ms_xfer;
+  ctx->ms_stage[0] = ms_geo;
+  ctx->ms_stage[1] = 0.f;
+  ctx->ms_stage[2] = ms_xfer;
+  ctx->ms_stage[3] = 0.f;
+  for (int q = 0; q < 2; q++)
+    if (bst[q] != ~0ull) {
+      unsigned long long key = bst[q] - 1;
+      ctx->bad_job = (int)(key >> 32);
+      int pix = (int)(key & 0xffffffffull);
+      const ipgpu_image &P = jobs[ctx->bad_job].image;
+      ctx->bad_i = pix / P.ny;
+      ctx->bad_j = pix % P.ny;
+      snprintf(ctx->err, sizeof(ctx->err), "%s in job %d pixel %d %d",
+               q == 0 ? "MaxNStep exceeded" : "NaN in image", ctx->bad_job, ctx->bad_i, ctx->bad_j);
+      return q == 0 ? IPGPU_ERR_MAXNSTEP : IPGPU_ERR_NAN;
+    }
+  for (int j = 0; j < njobs; j++) {
+    const ipgpu_job &J = jobs[j];
+    size_t n = (size_t)J.image.nx * J.image.ny, o = (size_t)bj[j].pix_off;
+    if (J.image_out) HIPCHK(ctx, hipMemcpyAsync(J.image_out, ctx->d_bimage + o, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (J.imageS_out && !J.image.only_unpolarized)
+      HIPCHK(ctx, hipMemcpyAsync(J.imageS_out, ctx->d_bimageS + 5 * o, 5 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (J.taus_out) HIPCHK(ctx, hipMemcpyAsync(J.taus_out, ctx->d_btaus + o, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (J.nstep_out) HIPCHK(ctx, hipMemcpyAsync(J.nstep_out, ctx->d_bnstep + o, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (J.flag_out) HIPCHK(ctx, hipMemcpyAsync(J.flag_out, ctx->d_bflag + o, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return IPGPU_OK;
+}
+
+int ipgpu_set_batch_memory(ipgpu_ctx *ctx, size_t traj_bytes)
+{
+  if (!ctx || traj_bytes < ((size_t)1 << 20)) return IPGPU_ERR_ARG;
+  ctx->batch_traj_bytes = traj_bytes;
+  return IPGPU_OK;
+}
+
+int ipgpu_bad_job(const ipgpu_ctx *ctx, int *job, int *i, int *j)
+{
+  if (!ctx || !job || !i || !j) return IPGPU_ERR_ARG;
+  *job = ctx->bad_job;
+  *i = ctx->bad_i;
+  *j = ctx->bad_j;
   return IPGPU_OK;
 }
 

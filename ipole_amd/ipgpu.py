@@ -57,6 +57,12 @@ class Image(C.Structure):
                 ("emission_type", C.c_int)]
 
 
+class Job(C.Structure):
+    _fields_ = [("image", Image), ("m_unit_scale", C.c_double), ("image_out", C.POINTER(C.c_double)),
+                ("imageS_out", C.POINTER(C.c_double)), ("taus_out", C.POINTER(C.c_double)),
+                ("nstep_out", C.POINTER(C.c_int)), ("flag_out", C.POINTER(C.c_int))]
+
+
 class IpgpuError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("ipgpu error %d: %s" % (code, msg))
@@ -87,6 +93,9 @@ def lib():
         L.ipgpu_render.argtypes = [vp, C.POINTER(Image), dp, dp, dp, ip, ip]
         L.ipgpu_fetch.argtypes = [vp, dp, dp, dp, ip, ip]
         L.ipgpu_bad_pixel.argtypes = [vp, ip, ip]
+        L.ipgpu_render_batch.argtypes = [vp, C.c_int, C.POINTER(Job)]
+        L.ipgpu_bad_job.argtypes = [vp, ip, ip, ip]
+        L.ipgpu_set_batch_memory.argtypes = [vp, C.c_size_t]
         L.ipgpu_kernel_times.argtypes = [vp, dp, dp, dp]
         L.ipgpu_stage_times.argtypes = [vp, dp]
         L.ipgpu_step_counts.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
@@ -289,6 +298,41 @@ class Context:
         return dict(image=out["image"].reshape(nx, ny), imageS=out["imageS"].reshape(nx, ny, 5),
                     taus=out["taus"].reshape(nx, ny), nstep=out["nstep"].reshape(nx, ny),
                     flag=out["flag"].reshape(nx, ny))
+
+    def render_batch(self, jobs, fetch=True):
+        """Batched render of [(Image, m_unit_scale), ...]; returns one dict per
+        job as render() does (or None without fetch)."""
+        n = len(jobs)
+        arr = (Job * n)()
+        outs = []
+        for k, (image, mscale) in enumerate(jobs):
+            arr[k].image = image
+            arr[k].m_unit_scale = mscale
+            if fetch:
+                npx = image.nx * image.ny
+                o = dict(image=np.zeros(npx), imageS=np.zeros(5 * npx), taus=np.zeros(npx),
+                         nstep=np.zeros(npx, np.int32), flag=np.zeros(npx, np.int32))
+                arr[k].image_out, arr[k].imageS_out, arr[k].taus_out = P(o["image"]), P(o["imageS"]), P(o["taus"])
+                arr[k].nstep_out, arr[k].flag_out = IP(o["nstep"]), IP(o["flag"])
+                outs.append(o)
+        rc = self.L.ipgpu_render_batch(self.h, n, arr)
+        if rc == ERR_MAXNSTEP:
+            jb, i, j = C.c_int(), C.c_int(), C.c_int()
+            self.L.ipgpu_bad_job(self.h, C.byref(jb), C.byref(i), C.byref(j))
+            raise IpgpuError(rc, "MaxNStep exceeded in job %d pixel %d %d" % (jb.value, i.value, j.value))
+        self._chk(rc)
+        if not fetch:
+            return None
+        res = []
+        for (image, _), o in zip(jobs, outs):
+            nx, ny = image.nx, image.ny
+            res.append(dict(image=o["image"].reshape(nx, ny), imageS=o["imageS"].reshape(nx, ny, 5),
+                            taus=o["taus"].reshape(nx, ny), nstep=o["nstep"].reshape(nx, ny),
+                            flag=o["flag"].reshape(nx, ny)))
+        return res
+
+    def set_batch_memory(self, nbytes):
+        self._chk(self.L.ipgpu_set_batch_memory(self.h, int(nbytes)))
 
     def kernel_times(self):
         a, b, c = C.c_double(), C.c_double(), C.c_double()

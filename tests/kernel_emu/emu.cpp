@@ -67,7 +67,8 @@ void emu_point(const ipgpu_geom *G, const ipgpu_model *M, const double *dump, in
 }
 
 /* The pipeline's elementary functions: kind 0 exp_, 1 sincos_, 2 sincospi_,
- * 3 rcp; out holds one value (two for the sincos forms). */
+ * 3 rcp, 4 log_, 5 cbrt_, 6 pow_(x, -8/3); out holds one value (two for the
+ * sincos forms). */
 void emu_math(int kind, const double *x, int n, double *out)
 {
   for (int i = 0; i < n; i++) {
@@ -76,7 +77,10 @@ void emu_math(int kind, const double *x, int n, double *out)
     if (kind == 0) o[0] = exp_(x[i]);
     else if (kind == 1) sincos_(x[i], o[0], o[1]);
     else if (kind == 2) sincospi_(x[i], o[0], o[1]);
-    else o[0] = rcp(x[i]);
+    else if (kind == 3) o[0] = rcp(x[i]);
+    else if (kind == 4) o[0] = log_(x[i]);
+    else if (kind == 5) o[0] = cbrt_(x[i]);
+    else o[0] = pow_(x[i], -8. / 3.);
   }
 }
 
@@ -130,11 +134,47 @@ int emu_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, 
       /* the same per-step packets and per-ray chain as the GPU kernels */
       pkt.assign((size_t)(ns > 0 ? ns : 1) * stride, 0.);
       for (int q = 0; q < ns; q++)
-        prep_step(*G, *M, *P, ns - q, rec.data(), ntp, dump, pkt.data() + (size_t)q * stride, pol);
+        prep_step(*G, *M, *P, ns - q, rec.data(), ntp, dump, pkt.data() + (size_t)q * stride, pol, unit_scale());
       double js, ks;
-      far_end_coefficients(*G, *M, *P, ns, rec.data(), dump, js, ks);
+      far_end_coefficients(*G, *M, *P, ns, rec.data(), dump, js, ks, unit_scale());
       int nrad;
       flag[n] = chain_ray(*G, *P, ns, pkt.data(), stride, js, ks, out7 + 7 * (size_t)n, nrad);
+    }
+  }
+  return status;
+}
+
+/* The batch path on the CPU: geodesics traced with the image parameters
+ * Ptrace, transfer of each ray fused (no packet memory) for the job Pjob
+ * (whose frequency may differ: K and dl rescaled) with n_e scaled by ne_s
+ * and B by its square root. */
+int emu_render_scaled(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *Ptrace,
+                      const ipgpu_image *Pjob, const double *dump, const int *pix, int npix, double ne_s,
+                      double *out7, int *nstep, int *flag)
+{
+  int status = 0;
+  Scale sc;
+  sc.kf = Pjob->freq / Ptrace->freq;
+  sc.ikf = 1. / sc.kf;
+  sc.ne = ne_s;
+  sc.b = sqrt(ne_s);
+#pragma omp parallel
+  {
+    std::vector<double> rec((size_t)Ptrace->maxnstep * REC);
+    std::vector<int> nt(Ptrace->target_nturns >= 0 ? Ptrace->maxnstep : 1);
+#pragma omp for schedule(dynamic, 1)
+    for (int n = 0; n < npix; n++) {
+      int i = pix[2 * n], j = pix[2 * n + 1];
+      int *ntp = Ptrace->target_nturns >= 0 ? nt.data() : nullptr;
+      int ns = trace_ray(*G, *Ptrace, i, j, Ptrace->maxnstep, rec.data(), ntp);
+      nstep[n] = ns;
+      if (ns >= Pjob->maxnstep - 1) {
+#pragma omp critical
+        status = -10;
+        continue;
+      }
+      int nrad;
+      flag[n] = transfer_fused(*G, *M, *Pjob, ns, rec.data(), ntp, dump, sc, out7 + 7 * (size_t)n, nrad);
     }
   }
   return status;

@@ -28,6 +28,8 @@ def lib():
         L.emu_plasma_tetrad.argtypes = [C.POINTER(G.Geom), dp, dp, dp, dp, dp]
         L.emu_render.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), dp, ip,
                                  C.c_int, dp, ip, ip]
+        L.emu_render_scaled.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image),
+                                        C.POINTER(G.Image), dp, ip, C.c_int, C.c_double, dp, ip, ip]
         L.emu_trace.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Image), C.c_int, C.c_int, dp]
         _lib = L
     return _lib
@@ -94,6 +96,19 @@ def render(geom, model, image, dump, pix=None):
     d = dump if dump is not None else np.zeros(16)
     rc = lib().emu_render(C.byref(geom), C.byref(model), C.byref(image), G.P(d), G.IP(pix), n,
                           G.P(out7), G.IP(ns), G.IP(fl))
+    return rc, out7, ns, fl
+
+
+def render_scaled(geom, model, image_trace, image_job, dump, ne_scale=1.):
+    """Batch-path arithmetic on the CPU (see emu_render_scaled)."""
+    nx, ny = image_job.nx, image_job.ny
+    pix = np.array([(i, j) for i in range(nx) for j in range(ny)], np.int32)
+    n = len(pix)
+    out7, ns, fl = np.zeros((n, 7)), np.zeros(n, np.int32), np.zeros(n, np.int32)
+    d = f64(dump) if dump is not None else None
+    rc = lib().emu_render_scaled(C.byref(geom), C.byref(model), C.byref(image_trace), C.byref(image_job),
+                                 G.P(d) if d is not None else None, G.IP(pix), n, ne_scale, G.P(out7),
+                                 G.IP(ns), G.IP(fl))
     return rc, out7, ns, fl
 
 

@@ -76,6 +76,7 @@ inline void sincos(Cd x, Cd *s, Cd *c)
 }
 
 #define IPG_OPCOUNT 1
+typedef double ipg_f64;
 #define double Cd
 #include "ipg_pipeline.h"
 #undef double
@@ -110,7 +111,7 @@ int opcount_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image 
     pkt.assign((size_t)(ns > 0 ? ns : 1) * stride, Cd(0.));
     for (int q = 0; q < ns; q++) {
       Ops a = g_ops;
-      prep_step(*G, *M, *P, ns - q, rec.data(), ntp, d, pkt.data() + (size_t)q * stride, pol);
+      prep_step(*G, *M, *P, ns - q, rec.data(), ntp, d, pkt.data() + (size_t)q * stride, pol, unit_scale());
       bool rad = ((int)pkt[(size_t)q * stride + PK_META]) & 1;
       double f = g_ops.flops() - a.flops();
       if (rad) {
@@ -123,7 +124,7 @@ int opcount_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image 
     out[1] += ns;
     Ops b = g_ops;
     Cd js, ks;
-    far_end_coefficients(*G, *M, *P, ns, rec.data(), d, js, ks);
+    far_end_coefficients(*G, *M, *P, ns, rec.data(), d, js, ks, unit_scale());
     out[7] += g_ops.flops() - b.flops();
     Ops c = g_ops;
     Cd o7[7];

@@ -1,0 +1,95 @@
+"""Batched render (ipgpu_render_batch) against the oracle: jobs with
+different cameras, image sizes, frequencies, M_unit and polarization modes
+on one dump, in one call, at the plain path's parity bar.  Jobs that share a
+camera reuse one geodesic trace at another frequency (K and dl rescaled);
+M_unit enters as n_e and B scale factors.  Also: several passes over the
+rays when the trajectory memory is small, and the maxnstep error."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import gpu_image_kwargs, small_dump, stokes_errors
+from ipole_amd import ipgpu as G
+from test_gpu_parity import check, spread
+
+pytestmark = pytest.mark.gpu
+
+M_UNIT = 3e26
+JOBS = [  # thetacam, nx, ny, freq, M_unit scale, unpol, emission_type
+    (163., 24, 24, 230e9, 1.0, 0, 4),
+    (163., 24, 24, 86e9, 1.0, 0, 4),
+    (163., 24, 24, 345e9, 0.3, 0, 4),
+    (60., 16, 24, 230e9, 2.0, 0, 4),
+    (60., 16, 24, 230e9, 1.0, 1, 4),
+    (90., 20, 20, 86e9, 0.5, 0, 1),
+]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def setup():
+    h, prims = small_dump()
+    c = G.Context(0)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    c.set_geometry(g)
+    c.set_model(m)
+    c.upload_prims(prims, G.iharm_units(M_UNIT))
+    yield h, prims, c, g, m
+    c.close()
+
+
+def oracle_ref(h, prims, th, nx, ny, f, s, unpol, et):
+    orc, _ = O.iharm_oracle(h, prims, M_unit=M_UNIT * s)
+    p = O.default_params(nx=nx, ny=ny, thetacam=th, freqcgs=f, only_unpolarized=unpol, emission_type=et)
+    return orc.render(p), p, (None if unpol else spread(h, prims, M_UNIT * s, p))
+
+
+@pytest.mark.parametrize("traj_mib", [48 * 1024, 2])
+def test_batch_vs_oracle(setup, traj_mib):
+    h, prims, c, g, m = setup
+    say("case batch of %d jobs, trajectory memory %d MiB" % (len(JOBS), traj_mib))
+    c.set_batch_memory(traj_mib << 20)
+    jobs, scales, refs, floors = [], [], [], []
+    for (th, nx, ny, f, s, unpol, et) in JOBS:
+        ref, p, fl = oracle_ref(h, prims, th, nx, ny, f, s, unpol, et)
+        im, scale, _ = G.make_image(g, m.L_unit, **gpu_image_kwargs(p))
+        jobs.append((im, s))
+        scales.append(scale)
+        refs.append(ref)
+        floors.append(fl)
+    got = c.render_batch(jobs)
+    c.set_batch_memory(48 << 30)
+    for k, (ref, out, scale, fl) in enumerate(zip(refs, got, scales, floors)):
+        say("  job %d %s vs oracle, floor %s" % (k, JOBS[k], fl))
+        check(ref, out, scale, unpol=bool(JOBS[k][5]), floor=fl)
+    # against the plain path on the same job (dump derived at the job's
+    # M_unit): the same kernels' arithmetic up to the rescaling roundoff
+    for k, ((im, s), out) in enumerate(zip(jobs, got)):
+        c.upload_prims(prims, G.iharm_units(M_UNIT * s))
+        plain = c.render(im)
+        assert np.array_equal(plain["nstep"], out["nstep"])
+        e = stokes_errors(plain["imageS"], plain["image"], out["imageS"], out["image"])
+        F = abs(out["image"].sum() / plain["image"].sum() - 1)
+        say("  job %d vs plain: dF %.2e %s" % (k, F, e))
+        assert F < 1e-9
+        for name, (mx, frac) in e.items():
+            if JOBS[k][5] and name != "unpol":
+                continue
+            assert mx < 1e-6, (k, name, mx)
+    c.upload_prims(prims, G.iharm_units(M_UNIT))
+
+
+def test_batch_maxnstep(setup):
+    h, prims, c, g, m = setup
+    say("case batch maxnstep")
+    ok, _, _ = G.make_image(g, m.L_unit, nx=8, ny=8, thetacam=163., freqcgs=230e9)
+    bad, _, _ = G.make_image(g, m.L_unit, nx=8, ny=8, thetacam=60., freqcgs=230e9, maxnstep=40)
+    with pytest.raises(G.IpgpuError) as e:
+        c.render_batch([(ok, 1.), (bad, 1.)])
+    assert e.value.code == G.ERR_MAXNSTEP and "job 1" in str(e.value)

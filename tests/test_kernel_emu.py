@@ -159,6 +159,38 @@ def test_emu_iharm(thetacam, etype, unpol):
         assert np.mean(e > 1e-6) < 1e-3, (k, np.mean(e > 1e-6))
 
 
+def test_pipeline_log_pow_cbrt_vs_mpmath():
+    """log_, cbrt_ and pow_ of ipg_math.h (the thermal fits' powers and
+    logarithms) against mpmath: log within 4 ulp of max(|log x|, ln 2 / 2^10)
+    (absolute near x = 1, where any log has only absolute accuracy in its
+    argument's rounding), cbrt within 2 ulp, x^(-8/3) = e^(b log x) within
+    (4 + 2 |b log x|) ulp: the rounding of b log x, |b log x| ulp absolute,
+    carries into e^(b log x) as that much relative error (up to ~600 here;
+    the fits use |b log x| of a few tens)."""
+    import mpmath as mp
+    mp.mp.dps = 50
+    rng = np.random.default_rng(9)
+    xs = np.concatenate([10 ** rng.uniform(-299, 299, 2000), rng.uniform(0.5, 2., 2000),
+                         [1., 1. + 2.2e-16, 1. - 1.1e-16, 2., 0.5, np.sqrt(0.5), 1e-300, 1e300]])
+    got = emu.math(4, xs)[:, 0]
+    for x, g in zip(xs, got):
+        ref = float(mp.log(mp.mpf(x)))
+        assert abs(g - ref) <= 4 * 2.2e-16 * max(abs(ref), np.log(2) / 1024), (x, g, ref)
+    xs2 = 10 ** rng.uniform(-200, 200, 2000)
+    got = emu.math(5, xs2)[:, 0]
+    for x, g in zip(xs2, got):
+        ref = float(mp.cbrt(mp.mpf(x)))
+        assert abs(g / ref - 1) <= 2 * 2.2e-16, (x, g, ref)
+    xs3 = 10 ** rng.uniform(-100, 100, 2000)
+    got = emu.math(6, xs3)[:, 0]
+    for x, g in zip(xs3, got):
+        ref = float(mp.power(mp.mpf(x), mp.mpf(-8) / 3))
+        if ref == 0. or np.isinf(ref):
+            continue
+        assert abs(g / ref - 1) <= (4 + 2 * abs(8. / 3. * np.log(x))) * 2.2e-16, (x, g, ref)
+    assert emu.math(4, [0.])[0, 0] == -np.inf and np.isnan(emu.math(4, [-1.])[0, 0])
+
+
 def test_pipeline_math_vs_mpmath():
     """exp_, sincos_, sincospi_ of ipg_math.h over and beyond the argument
     ranges the loops use (X1 up to ~7, smoothing exponents, theta, 2 pi X2,
@@ -189,3 +221,30 @@ def test_pipeline_math_vs_mpmath():
         rs, rc = float(mp.sinpi(mp.mpf(y))), float(mp.cospi(mp.mpf(y)))
         assert abs(s - rs) <= 4 * 2.2e-16 * max(abs(rs), 2. ** -30), (y, s, rs)
         assert abs(c - rc) <= 4 * 2.2e-16 * max(abs(rc), 2. ** -30), (y, c, rc)
+
+
+@pytest.mark.parametrize("unpol", [0, 1])
+def test_emu_batch_rescaling(unpol):
+    """The batch path's arithmetic: geodesics traced at 86 GHz reused for a
+    230 GHz job (K and dl rescaled), fused transfer, n_e and B rescaled for
+    half the M_unit of the uploaded dump; against the oracle rendering that
+    job directly (traced at 230 GHz, dump derived at M_unit / 2).  The
+    rescaled geodesic differs from a direct one by roundoff only: nstep
+    matches per pixel and the image meets the plain path's bar."""
+    h, prims = small_dump()
+    M = 3e26
+    _, fields = O.iharm_oracle(h, prims, M_unit=M)
+    orc2, _ = O.iharm_oracle(h, prims, M_unit=M * 0.5)
+    p = O.default_params(nx=20, ny=20, thetacam=163., freqcgs=230e9, only_unpolarized=unpol)
+    ref = orc2.render(p)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    im_t, _, _ = G.make_image(g, m.L_unit, nx=20, ny=20, thetacam=163., freqcgs=86e9, only_unpolarized=unpol)
+    im_j, scale, _ = G.make_image(g, m.L_unit, nx=20, ny=20, thetacam=163., freqcgs=230e9,
+                                  only_unpolarized=unpol)
+    rc, out7, ns, fl = emu.render_scaled(g, m, im_t, im_j, emu.pack_dump(fields), ne_scale=0.5)
+    assert rc == 0 and np.array_equal(ns, ref["nstep"].ravel())
+    F, errs = image_errors(ref, out7, 230e9 ** 3, scale, unpol=bool(unpol))
+    assert F < 1e-6
+    for k, e in errs.items():
+        assert e.max() < 1e-5, (k, e.max())
+        assert np.mean(e > 1e-6) < 1e-3, (k, np.mean(e > 1e-6))
