@@ -87,6 +87,9 @@ typedef struct ipgpu_image {
   int subring_dtheta;
   int counterjet;
   int emission_type;         /* 4 (Dexter thermal, default) or 1 (Pandya thermal) */
+  int transfer_fp32;         /* 1: transfer coefficients and Stokes-step exponentials in
+                                FP32 (geodesics, connection, tetrads, N in FP64); no
+                                counterpart in the reference, which is FP64 throughout */
 } ipgpu_image;
 
 typedef struct ipgpu_ctx ipgpu_ctx;

@@ -893,6 +893,209 @@ IPG_HD void coefficients(const Fluid &F, const double K[4], int emission_type, b
   }
 }
 
+/* ------------------------------------------- coefficients, FP32 shape functions */
+
+/* The mixed-precision variant of coefficients(): the same formulas, with
+ * the dimensionless shape functions of the fits (K_n(1/Theta_e), the Dexter
+ * and Leung spectra, the Faraday fits and Bnu's exponential) evaluated in
+ * FP32, and the fluid-frame frequency, the field angle and the dimensional
+ * prefactors (whose magnitudes, ~1e-35 for j/nu^2, lie outside FP32's
+ * range) in FP64. */
+IPG_HD void bessel_k012_f(float x, float ks[3], float &emx)
+{
+  emx = expf(-x);
+  float k0s, k1s;
+  if (x <= 2.0f) {
+    const float EG = 0.5772156649f;
+    float y = 0.25f * x * x;
+    float t0 = 1.0f, t1 = 0.5f * x;
+    float I0 = 1.0f, I1 = t1, S0 = 0.f, S1 = t1 * (1.0f - 2.0f * EG);
+    float Hk = 0.0f, p1 = -EG, p2 = 1.0f - EG;
+#pragma unroll
+    for (int k = 1; k < 10; k++) {
+      const float ik = 1.0f / (float)k, ik1 = 1.0f / (float)(k + 1);
+      t0 *= y * (ik * ik);
+      t1 *= y * (ik * ik1);
+      Hk += ik;
+      p1 += ik;
+      p2 += ik1;
+      I0 += t0;
+      I1 += t1;
+      S0 += t0 * Hk;
+      S1 += t1 * (p1 + p2);
+    }
+    float lg = logf(0.5f * x);
+    float ex = expf(x);
+    k0s = (-(lg + EG) * I0 + S0) * ex;
+    k1s = (1.0f / x + lg * I1 - 0.5f * S1) * ex;
+  } else {
+    float t = 4.0f / x - 1.0f, t2 = 2.0f * t;
+    float b0 = 0.f, b1 = 0.f, b2 = 0.f;
+    for (int k = 11; k >= 1; k--) {
+      b2 = b1;
+      b1 = b0;
+      b0 = t2 * b1 - b2 + (float)ipg_k0_cheb[k];
+    }
+    float f0 = t * b0 - b1 + (float)ipg_k0_cheb[0];
+    b0 = b1 = b2 = 0.f;
+    for (int k = 11; k >= 1; k--) {
+      b2 = b1;
+      b1 = b0;
+      b0 = t2 * b1 - b2 + (float)ipg_k1_cheb[k];
+    }
+    float f1 = t * b0 - b1 + (float)ipg_k1_cheb[0];
+    float isx = 1.0f / sqrtf(x);
+    k0s = f0 * isx;
+    k1s = f1 * isx;
+  }
+  ks[0] = k0s;
+  ks[1] = k1s;
+  ks[2] = k0s + (2.0f / x) * k1s;
+}
+
+IPG_HD void coefficients_f32(const Fluid &F, const double K[4], int emission_type, bool pol, double &ju,
+                             double &ku, Coef &c)
+{
+  ju = 0.;
+  ku = 0.;
+  c.jI = c.jQ = c.jU = c.jV = c.aI = c.aQ = c.aU = c.aV = c.rQ = c.rU = c.rV = 0.;
+  double Ne = F.ne;
+  if (Ne <= 0.) return;
+  double kdotu = K[0] * F.Ucov[0] + K[1] * F.Ucov[1] + K[2] * F.Ucov[2] + K[3] * F.Ucov[3];
+  double nu = -kdotu * (ME * CL * CL / HPL);
+  double nusq = nu * nu;
+  double sth, cth;
+  bool edge = false;
+  {
+    double Bm = sqrt(fabs(F.Bcon[0] * F.Bcov[0] + F.Bcon[1] * F.Bcov[1] + F.Bcon[2] * F.Bcov[2]
+                          + F.Bcon[3] * F.Bcov[3]));
+    if (Bm == 0.) {
+      sth = 1.;
+      cth = COS_PI_2;
+    } else {
+      double mu = (K[0] * F.Bcov[0] + K[1] * F.Bcov[1] + K[2] * F.Bcov[2] + K[3] * F.Bcov[3])
+          * rcp(fabs(kdotu) * Bm);
+      if (fabs(mu) > 1.) mu = (mu > 0.) ? 1. : -1.;
+      edge = (mu == 1. || mu == -1.);
+      sth = sqrt((1. - mu) * (1. + mu));
+      cth = mu;
+    }
+  }
+  const double B = F.B, Th = F.thetae;
+  const float Thf = (float)Th, sthf = (float)sth, cthf = (float)cth;
+  float ks[3], emx;
+  bessel_k012_f(1.0f / Thf, ks, emx);
+  double omega0 = (EE / (ME * CL)) * B;
+  double wp2 = (4. * PI * EE * EE / ME) * Ne;
+  const float xr = (float)(Th * sqrt(SQRT2 * sth * (1.e3 / (2. * PI)) * omega0 / nu));
+  double w = 2. * PI * nu;
+  if (edge) {
+    if (pol) {
+      float fit;
+      if (emission_type != 1 && emx * ks[2] > 0) {
+        float sh = 0.43793091f * logf(1.f + 0.00185777f * exp2f(1.50316886f * log2f(xr)));
+        fit = (emx * ks[0] - sh) / (emx * ks[2]);
+      } else {
+        fit = (ks[0] / ks[2]) * (1.f - 0.11f * logf(1.f + 0.035f * xr));
+      }
+      c.rV = 2. * PI * nu / CL * (wp2 * omega0 / (w * w * w) * (double)fit * cth) * nu;
+    }
+    return;
+  }
+  /* Bnu_inv (src/radiation.c:14-23): the prefactor in FP64, e^x - 1 in FP32 */
+  double Bnuinv;
+  {
+    const float x = (float)((HPL / (ME * CL * CL)) * nu / Th);
+    if (x < 2.e-3f) Bnuinv = (2. * HPL / (CL * CL)) / (double)(x / 24.f * (24.f + x * (12.f + x * (4.f + x))));
+    else Bnuinv = (2. * HPL / (CL * CL)) / (double)expm1f(x);
+  }
+  double iBnu = (Bnuinv > 0) ? 1. / Bnuinv : 0.;
+  {
+    double K2 = fmax((double)emx * (double)ks[2], SMALL);
+    double nuc = (EE / (2. * PI * ME * CL)) * B;
+    double nus = (2. / 9.) * nuc * Th * Th * sth;
+    double j = 0.;
+    if (!(nu > 1.e12 * nus)) {
+      const float x = (float)(nu / nus);
+      const float c3 = cbrtf(x);
+      float f = sqrtf(x) + 1.887748625f * sqrtf(c3);
+      f = f * f;
+      j = (SQRT2 * PI * EE * EE / (3. * CL)) * Ne * nus / K2 * (double)(f * expf(-c3));
+    }
+    ju = j / nusq;
+    ku = ju * iBnu;
+  }
+  if (!pol) return;
+  double jI, jQ, jV;
+  if (emission_type == 1) {
+    double nu_c = (EE / (2. * PI * ME * CL)) * B;
+    double nu_s = (2. / 9.) * nu_c * sth * Th * Th;
+    const float X = (float)(nu / nu_s);
+    double pre = (EE * EE / CL) * Ne * nu_c;
+    const float LX = log2f(X), LT = log2f(Thf);
+    const float X13 = exp2f(LX * (1.f / 3.f)), X16 = sqrtf(X13), e13 = expf(-X13);
+    const float t1 = 1.414213562f * 3.141592654f / 27.f * sthf;
+    const float sX = sqrtf(X);
+    const float tI = sX + 1.887748625f * X16;
+    jI = pre * (double)(t1 * (tI * tI) * e13);
+    const float Tq = exp2f(LT * (24.f / 25.f));
+    const float tq2 = (7.f * Tq + 35.f) / (10.f * Tq + 75.f);
+    const float tq3 = sX + tq2 * 1.887748625f * X16;
+    jQ = pre * (double)(t1 * (tq3 * tq3) * e13);
+    const float sin_t = sthf * 0.4356824463f - cthf * 0.9001004422f;
+    const float tv1 = (37.f - 87.f * sin_t) / (100.f * (Thf + 1.f));
+    const float tv2 = exp2f((5.f / 3.f) * log2f(1.f + (exp2f(LT * 0.6f) * 0.04f + 0.7f) * exp2f(LX * 0.36f)));
+    jV = pre * (double)(tv1 * tv2 * e13);
+  } else {
+    double nus = (3.0 * EE / (4.0 * PI * ME * CL)) * B * sth * Th * Th + 1.0;
+    const float x = (float)(nu / nus);
+    const float cb = cbrtf(x), ic = 1.f / cb, ic2 = ic * ic;
+    const float e = expf(-1.8899f * cb);
+    const float II = 2.5651f * (1.f + 1.92f * ic + 0.9977f * ic2) * e;
+    const float IQ = 2.5651f * (1.f + 0.93193f * ic + 0.499873f * ic2) * e;
+    const float IV = (1.81384f / x + 3.42319f * ic2 + 0.0292545f / sqrtf(x) + 2.03773f * ic) * e;
+    double iTh2 = 1. / (Th * Th);
+    double pre = (EE * EE / (2. * SQRT3 * CL)) * Ne * nu * iTh2;
+    jI = pre * (double)II;
+    jQ = pre * (double)IQ;
+    jV = (2. * EE * EE / (3. * SQRT3 * CL)) * Ne * nu * (cth / sth) * iTh2 / Th * (double)IV;
+  }
+  double inu2 = 1. / nusq;
+  c.jI = jI * inu2;
+  c.jQ = jQ * inu2;
+  c.jU = -0. * inu2;
+  c.jV = jV * inu2;
+  double jP = sqrt(c.jQ * c.jQ + c.jU * c.jU + c.jV * c.jV);
+  if (c.jI < jP * (1. / 0.99)) {
+    double pf = c.jI / jP * 0.99;
+    c.jQ *= pf;
+    c.jU *= pf;
+    c.jV *= pf;
+  }
+  c.aI = c.jI * iBnu;
+  c.aQ = c.jQ * iBnu;
+  c.aU = c.jU * iBnu;
+  c.aV = c.jV * iBnu;
+  {
+    const float Lr = logf(xr);
+    const float Le = logf(xr + 1.e-16f);
+    const float e472 = expf(-xr * (1.f / 47.2f));
+    const float mz2 = -2.f * (Lr - 4.787491743f) * 10.f;
+    const float logistic = (mz2 > 80.f) ? 0.f : 1.f / (1.f + expf(mz2));
+    const float extraterm = (.011f * e472 - 0.01176770732f * 3.141592654f * 1.e4f * expf(Le * (-8.f / 3.f))) * logistic;
+    const float jff = 2.011f * expf(-expf(1.035f * Lr) * (1.f / 4.7f)) - cosf(0.5f * xr) * expf(-expf(1.2f * Lr) * (1.f / 2.73f))
+        - .011f * e472 + extraterm;
+    const double kr = (double)(ks[1] / ks[2]);
+    double w2 = w * w;
+    double eps11m22 = (double)jff * wp2 * (omega0 * omega0) / (w2 * w2) * (kr + 6. * Th) * (sth * sth);
+    c.rQ = (PI / CL) * nu * eps11m22 * nu;
+    const double fit = (double)((ks[0] / ks[2]) * (1.f - 0.11f * logf(1.f + 0.035f * xr)));
+    double eps12 = wp2 * omega0 / (w2 * w) * fit * cth;
+    c.rV = (2. * PI / CL) * nu * eps12 * nu;
+    c.rU = 0. * nu;
+  }
+}
+
 /* ------------------------------------------------------------ radiative step */
 
 /* src/ipolarray.c:607-628 */
@@ -1003,6 +1206,90 @@ IPG_HD void stokes_affine(const Coef &c, double x, double T[4][4], double cv[4])
   double R3[3][3], A[4][4], b[4];
   faraday_matrix(c, 0.5 * x, R3);
   absorb_emit_affine(c, x, A, b);
+  double R[4][4];
+  for (int i = 0; i < 4; i++)
+    for (int k = 0; k < 4; k++) R[i][k] = (i == 0 || k == 0) ? (i == k ? 1. : 0.) : R3[i - 1][k - 1];
+  double RA[4][4];
+  for (int i = 0; i < 4; i++)
+    for (int k = 0; k < 4; k++) {
+      double s = 0.;
+      for (int l = 0; l < 4; l++) s += R[i][l] * A[l][k];
+      RA[i][k] = s;
+    }
+  for (int i = 0; i < 4; i++) {
+    for (int k = 0; k < 4; k++) {
+      double s = 0.;
+      for (int l = 0; l < 4; l++) s += RA[i][l] * R[l][k];
+      T[i][k] = s;
+    }
+    double s = 0.;
+    for (int l = 0; l < 4; l++) s += R[i][l] * b[l];
+    cv[i] = s;
+  }
+}
+
+/* stokes_affine with its exponentials and the rotation's sine and cosine
+ * in FP32 (the matrices themselves, and everything they act on, in FP64). */
+IPG_HD void stokes_affine_f32(const Coef &c, double x, double T[4][4], double cv[4])
+{
+  double R3[3][3];
+  {
+    const double r[3] = {c.rQ, c.rU, c.rV};
+    const double rho2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    const double X[3][3] = {{0., -r[2], r[1]}, {r[2], 0., -r[0]}, {-r[1], r[0], 0.}};
+    const double hx = 0.5 * x;
+    if (rho2 > 1e-80) {
+      double rho = sqrt(rho2);
+      float shf, chf;
+      sincosf((float)(0.5 * rho * hx), &shf, &chf);
+      double sh = shf, ch = chf;
+      double s = 2. * sh * ch, cc = 1. - 2. * sh * sh;
+      double w = 2. * sh * sh / rho2, v = s / rho;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) R3[i][j] = (i == j ? cc : 0.) + w * r[i] * r[j] + v * X[i][j];
+    } else {
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) R3[i][j] = (i == j ? 1. : 0.) + hx * X[i][j];
+    }
+  }
+  double A[4][4], b[4];
+  {
+    const double j[4] = {c.jI, c.jQ, c.jU, c.jV};
+    const double a[3] = {c.aQ, c.aU, c.aV};
+    const double aI = c.aI;
+    const double aP2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    for (int p = 0; p < 4; p++)
+      for (int q = 0; q < 4; q++) A[p][q] = 0.;
+    if (aP2 > 1e-80) {
+      double aP = sqrt(aP2);
+      double tauP = aP * x, tauI = aI * x;
+      /* e^(-tauI -+ tauP) - 1 and e^(-tauI) - 1 in FP32 (expm1, so that thin
+       * steps keep their relative accuracy) */
+      float emm = expm1f((float)(-tauI + tauP)), emp = expm1f((float)(-tauI - tauP)), em = expm1f((float)(-tauI));
+      double efacm = 1. + (double)emm, efacp = 1. + (double)emp, efac = 1. + (double)em;
+      double ch = 0.5 * (efacm + efacp), shp = (efacp - efacm) / (2. * aP);
+      double iaP2 = 1. / aP2;
+      A[0][0] = ch;
+      for (int k = 0; k < 3; k++) {
+        A[0][k + 1] = shp * a[k];
+        A[k + 1][0] = shp * a[k];
+        for (int l = 0; l < 3; l++) A[k + 1][l + 1] = (k == l ? efac : 0.) + a[k] * a[l] * (ch - efac) * iaP2;
+      }
+      double afacm = -(double)emm, afac = -(double)em, afacp = -(double)emp;
+      double adj = a[0] * j[1] + a[1] * j[2] + a[2] * j[3];
+      double m = afacm * (aP * j[0] - adj) / (2. * aP * (aI - aP));
+      double p = afacp * (aP * j[0] + adj) / (2. * aP * (aI + aP));
+      b[0] = m + p;
+      for (int k = 0; k < 3; k++) b[k + 1] = afac * (j[k + 1] / aI - a[k] * adj / (aI * aP2)) + (p - m) * a[k] / aP;
+    } else {
+      double dtau = x * aI;
+      double em = (double)expm1f((float)(-dtau));
+      for (int k = 0; k < 4; k++) {
+        A[k][k] = 1. + em;
+        b[k] = (dtau < 1.e-3) ? j[k] * x * (1. - (dtau * 0.5) * (1. - dtau * (1. / 3.))) : (j[k] / aI) * (-em);
+      }
+    }
+  }
   double R[4][4];
   for (int i = 0; i < 4; i++)
     for (int k = 0; k < 4; k++) R[i][k] = (i == 0 || k == 0) ? (i == k ? 1. : 0.) : R3[i - 1][k - 1];

@@ -167,6 +167,7 @@ IPG_HD void scale_rec(Rec &r, const Scale &sc)
 
 /* Packet of transfer step n (from trajectory point n to n-1) of
  * integrate_emission, src/ipolarray.c:78-180.  nt may be null. */
+template <bool F32 = false>
 IPG_HD_INLINE void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int n,
                       const double *rec, const int *nt, const double *dump, double *pk, bool pol,
                       const Scale &sc)
@@ -216,13 +217,15 @@ IPG_HD_INLINE void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const i
     F.B *= sc.b;
     Coef c;
     double ju, ku;
-    coefficients(F, tf.K, P.emission_type, pol, ju, ku, c);
+    if (F32) coefficients_f32(F, tf.K, P.emission_type, pol, ju, ku, c);
+    else coefficients(F, tf.K, P.emission_type, pol, ju, ku, c);
     pk[PK_JU] = ju;
     pk[PK_KU] = ku;
     if (pol) {
       if (zero_emission) c.jI = c.jQ = c.jU = c.jV = 0.;
       double T[4][4], cv[4];
-      stokes_affine(c, dl_cgs, T, cv);
+      if (F32) stokes_affine_f32(c, dl_cgs, T, cv);
+      else stokes_affine(c, dl_cgs, T, cv);
       for (int i = 0; i < 4; i++) {
         for (int k = 0; k < 4; k++) pk[PK_T + 4 * i + k] = T[i][k];
         pk[PK_CV + i] = cv[i];
@@ -258,6 +261,7 @@ IPG_HD_INLINE void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const i
 
 /* Unpolarized coefficients at the far end of the trajectory (the cached
  * js, ks of integrate_emission, src/ipolarray.c:74). */
+template <bool F32 = false>
 IPG_HD void far_end_coefficients(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P,
                                  int ns, const double *rec, const double *dump, double &js, double &ks,
                                  const Scale &sc)
@@ -272,7 +276,8 @@ IPG_HD void far_end_coefficients(const ipgpu_geom &G, const ipgpu_model &Mo, con
   F.ne *= sc.ne;
   F.B *= sc.b;
   Coef c;
-  coefficients(F, t.K, P.emission_type, false, js, ks, c);
+  if (F32) coefficients_f32(F, t.K, P.emission_type, false, js, ks, c);
+  else coefficients(F, t.K, P.emission_type, false, js, ks, c);
 }
 
 /* One packet held in registers; on the device it is read with 16-byte
@@ -432,6 +437,7 @@ IPG_HD int chain_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const do
  * The same operations on the same values as the split path; the half-step
  * metric and the end-point transport matrix are only formed while N is
  * nonzero (transport of N = 0 is the identity). */
+template <bool F32 = false>
 IPG_HD_INLINE void fused_step(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int n,
                               const double *rec, const int *nt, const double *dump, const Scale &sc,
                               bool pol, Chain &st)
@@ -475,7 +481,8 @@ IPG_HD_INLINE void fused_step(const ipgpu_geom &G, const ipgpu_model &Mo, const 
     F.B *= sc.b;
     Coef c;
     double jf, kf;
-    coefficients(F, tf.K, P.emission_type, pol, jf, kf, c);
+    if (F32) coefficients_f32(F, tf.K, P.emission_type, pol, jf, kf, c);
+    else coefficients(F, tf.K, P.emission_type, pol, jf, kf, c);
     double ji = zero_emission ? 0. : st.js, ki = st.ks;
     st.js = jf;
     st.ks = kf;
@@ -499,7 +506,8 @@ IPG_HD_INLINE void fused_step(const ipgpu_geom &G, const ipgpu_model &Mo, const 
       lower(m.g, E[1], e1);
       lower(m.g, E[2], e2);
       double T[4][4], cv[4];
-      stokes_affine(c, dl_cgs, T, cv);
+      if (F32) stokes_affine_f32(c, dl_cgs, T, cv);
+      else stokes_affine(c, dl_cgs, T, cv);
       double S0[4], S[4];
       herm_to_stokes(st.N, e1, e2, S0);
       for (int i = 0; i < 4; i++) S[i] = cv[i] + T[i][0] * S0[0] + T[i][1] * S0[1] + T[i][2] * S0[2] + T[i][3] * S0[3];
@@ -529,16 +537,17 @@ IPG_HD_INLINE void fused_step(const ipgpu_geom &G, const ipgpu_model &Mo, const 
 
 /* Whole transfer of one ray, step by step in registers (no packet memory):
  * for launches with enough rays to fill the chip. */
+template <bool F32 = false>
 IPG_HD int transfer_fused(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int ns,
                           const double *rec, const int *nt, const double *dump, const Scale &sc,
                           double out7[7], int &nrad)
 {
   const bool pol = !P.only_unpolarized;
   double js, ks;
-  far_end_coefficients(G, Mo, P, ns, rec, dump, js, ks, sc);
+  far_end_coefficients<F32>(G, Mo, P, ns, rec, dump, js, ks, sc);
   Chain st;
   chain_init(st, js, ks);
-  for (int q = 0; q < ns; q++) fused_step(G, Mo, P, ns - q, rec, nt, dump, sc, pol, st);
+  for (int q = 0; q < ns; q++) fused_step<F32>(G, Mo, P, ns - q, rec, nt, dump, sc, pol, st);
   nrad = st.nrad;
   return chain_finish(G, P, st, out7);
 }

@@ -109,6 +109,7 @@ __global__ void k_blockmap(int nrays, const long long *__restrict__ off, int *__
 
 /* One lane per transfer step of every ray: the state-independent part of
  * integrate_emission (transport matrices, fluid, coefficients, tetrad). */
+template <bool F32>
 __global__ __launch_bounds__(PREP_BLOCK) void k_prep(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P, int nrays,
                                                      int cap, int stride, long long total,
                                                      const long long *__restrict__ off,
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep(ipgpu_geom G, ipgpu_model M
   const int q = (int)(g - off[ray]);
   const double *rec = traj + (size_t)ray * cap * REC;
   const int *nt = nturns_buf ? nturns_buf + (size_t)ray * cap : nullptr;
-  prep_step(G, Mo, P, ns - q, rec, nt, dump, pkt + (size_t)g * stride, stride == PK, unit_scale());
+  prep_step<F32>(G, Mo, P, ns - q, rec, nt, dump, pkt + (size_t)g * stride, stride == PK, unit_scale());
 }
 
 /* One lane per ray: the recurrence of N and I over the ray's packets. */
@@ -157,7 +158,8 @@ __global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ip
   }
   const double *rec = traj + (size_t)ray * cap * REC;
   double js, ks;
-  far_end_coefficients(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
+  if (P.transfer_fp32) far_end_coefficients<true>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
+  else far_end_coefficients<false>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
   double res[7];
   int nrad;
   flags[ray] = chain_ray(G, P, ns, pkt + (size_t)off[ray] * stride, stride, js, ks, res, nrad);
@@ -411,6 +413,7 @@ __global__ __launch_bounds__(WAVE) void k_trace_store(ipgpu_geom G, const BGroup
  * job, then the job's pixel in the reference layout (save_pixel,
  * src/main.c:885-910).  bstatus[0]: min of (job << 32 | pixel) + 1 over
  * rays past maxnstep; bstatus[1]: the same over NaN pixels. */
+template <bool F32>
 __global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model Mo, const BJob *__restrict__ jobs,
                                                      const BGroup *__restrict__ groups,
                                                      const int2 *__restrict__ items, int nitems,
@@ -449,7 +452,7 @@ __global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model M
   const int *nt = (nturns_buf && P.target_nturns >= 0) ? nturns_buf + off[r] : nullptr;
   double o[7];
   int nrad;
-  int flag = transfer_fused(G, Mo, P, ns, rec, nt, dump, J.sc, o, nrad);
+  int flag = transfer_fused<F32>(G, Mo, P, ns, rec, nt, dump, J.sc, o, nrad);
   const double f3 = P.freqcgs * P.freqcgs * P.freqcgs;
   image[p] = o[0] * f3;
   taus[p] = o[5];
@@ -782,7 +785,8 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
     if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)nb))) return rc;
     k_blockmap<<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(nrays, ctx->d_off, ctx->d_blk);
     HIPCHK(ctx, hipGetLastError());
-    k_prep<<<nb, PREP_BLOCK, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, (long long)total,
+    auto kp = P.transfer_fp32 ? k_prep<true> : k_prep<false>;
+    kp<<<nb, PREP_BLOCK, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, (long long)total,
                                                ctx->d_off, ctx->d_blk, ctx->d_nstep_ray, ctx->d_traj,
                                                turns ? ctx->d_nturns : nullptr, ctx->d_dump, ctx->d_pkt);
     HIPCHK(ctx, hipGetLastError());
@@ -887,6 +891,8 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
       return set_err(ctx, IPGPU_ERR_ARG, "bad job: image size, maxnstep, frequency or m_unit_scale");
     if (P.emission_type != 4 && P.emission_type != 1)
       return set_err(ctx, IPGPU_ERR_ARG, "emission_type must be 4 or 1");
+    if ((P.transfer_fp32 != 0) != (jobs[0].image.transfer_fp32 != 0))
+      return set_err(ctx, IPGPU_ERR_ARG, "all jobs of a batch must use the same transfer precision");
   }
   HIPCHK(ctx, hipSetDevice(ctx->device));
   int rc;
@@ -1015,7 +1021,8 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, items.data(), nitems * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
     if (nitems > 0) {
-      k_xfer_batch<<<(nitems + WAVE - 1) / WAVE, WAVE, 0, ctx->stream>>>(
+      auto kx = jobs[0].image.transfer_fp32 ? k_xfer_batch<true> : k_xfer_batch<false>;
+      kx<<<(nitems + WAVE - 1) / WAVE, WAVE, 0, ctx->stream>>>(
           ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items, nitems, ctx->d_off, ctx->d_traj,
           ctx->d_nstep_ray, turns ? ctx->d_nturns : nullptr, ctx->d_dump, ctx->d_bimage, ctx->d_bimageS,
           ctx->d_btaus, ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts);

@@ -48,6 +48,7 @@ typedef struct {
   int reverse_field;
   double sph_R0, sph_tau0, sph_Thetae0, sph_beta0, sph_gam, sph_tp_over_te, sph_MBH;
   int have_MBH;
+  int transfer_fp32; /* GPU library option, no reference counterpart */
 } params_t;
 
 static void defaults(params_t *p)
@@ -110,6 +111,7 @@ static void try_set(const char *w, const char *v, params_t *p)
   set_i(w, v, "quench_output", &p->quench_output);
   set_i(w, v, "only_unpolarized", &p->only_unpolarized);
   set_i(w, v, "emission_type", &p->emission_type);
+  set_i(w, v, "transfer_fp32", &p->transfer_fp32);
   set_i(w, v, "stokes_floors", &p->stokes_floors);
   set_i(w, v, "counterjet", &p->isolate_counterjet);
   set_i(w, v, "old_centering", &p->old_centering);
@@ -416,6 +418,7 @@ int main(int argc, char *argv[])
   im.subring_dtheta = par.subring_dtheta;
   im.counterjet = par.isolate_counterjet;
   im.emission_type = par.emission_type;
+  im.transfer_fp32 = par.transfer_fp32;
 
   size_t npix = (size_t)par.nx * par.ny;
   double *image = calloc(npix, sizeof(double)), *taus = calloc(npix, sizeof(double));

@@ -54,7 +54,7 @@ class Image(C.Structure):
                 ("eps", C.c_double), ("maxnstep", C.c_int), ("old_centering", C.c_int),
                 ("only_unpolarized", C.c_int), ("qu_conv", C.c_int), ("stokes_floors", C.c_int),
                 ("target_nturns", C.c_int), ("subring_dtheta", C.c_int), ("counterjet", C.c_int),
-                ("emission_type", C.c_int)]
+                ("emission_type", C.c_int), ("transfer_fp32", C.c_int)]
 
 
 class Job(C.Structure):
@@ -196,6 +196,7 @@ def make_image(geom, L_unit, nx=160, ny=160, rcam=1000., thetacam=90., phicam=0.
                fovx_dsource=0., fovy_dsource=0., dx=0., dy=0., dsource=16.9e6, freqcgs=230e9,
                eps=0.01, maxnstep=50000, old_centering=0, only_unpolarized=0, qu_conv=0,
                stokes_floors=0, target_nturns=-1, subring_dtheta=0, counterjet=0, emission_type=4,
+               transfer_fp32=0,
                xoff=0., yoff=0.):
     """Camera set-up of src/main.c:125-174.  Returns (Image, scale, info)."""
     im = Image()
@@ -222,6 +223,7 @@ def make_image(geom, L_unit, nx=160, ny=160, rcam=1000., thetacam=90., phicam=0.
     im.old_centering, im.only_unpolarized, im.qu_conv = old_centering, only_unpolarized, qu_conv
     im.stokes_floors, im.target_nturns, im.subring_dtheta = stokes_floors, target_nturns, subring_dtheta
     im.counterjet, im.emission_type = counterjet, emission_type
+    im.transfer_fp32 = transfer_fp32
     return im, scale, dict(DX=DX, DY=DY, fovx_dsource=fx, fovy_dsource=fy, Dsource=Dsource)
 
 

@@ -133,10 +133,16 @@ int emu_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, 
       }
       /* the same per-step packets and per-ray chain as the GPU kernels */
       pkt.assign((size_t)(ns > 0 ? ns : 1) * stride, 0.);
-      for (int q = 0; q < ns; q++)
-        prep_step(*G, *M, *P, ns - q, rec.data(), ntp, dump, pkt.data() + (size_t)q * stride, pol, unit_scale());
       double js, ks;
-      far_end_coefficients(*G, *M, *P, ns, rec.data(), dump, js, ks, unit_scale());
+      if (P->transfer_fp32) {
+        for (int q = 0; q < ns; q++)
+          prep_step<true>(*G, *M, *P, ns - q, rec.data(), ntp, dump, pkt.data() + (size_t)q * stride, pol, unit_scale());
+        far_end_coefficients<true>(*G, *M, *P, ns, rec.data(), dump, js, ks, unit_scale());
+      } else {
+        for (int q = 0; q < ns; q++)
+          prep_step<false>(*G, *M, *P, ns - q, rec.data(), ntp, dump, pkt.data() + (size_t)q * stride, pol, unit_scale());
+        far_end_coefficients<false>(*G, *M, *P, ns, rec.data(), dump, js, ks, unit_scale());
+      }
       int nrad;
       flag[n] = chain_ray(*G, *P, ns, pkt.data(), stride, js, ks, out7 + 7 * (size_t)n, nrad);
     }
@@ -174,7 +180,9 @@ int emu_render_scaled(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_ima
         continue;
       }
       int nrad;
-      flag[n] = transfer_fused(*G, *M, *Pjob, ns, rec.data(), ntp, dump, sc, out7 + 7 * (size_t)n, nrad);
+      flag[n] = Pjob->transfer_fp32
+          ? transfer_fused<true>(*G, *M, *Pjob, ns, rec.data(), ntp, dump, sc, out7 + 7 * (size_t)n, nrad)
+          : transfer_fused<false>(*G, *M, *Pjob, ns, rec.data(), ntp, dump, sc, out7 + 7 * (size_t)n, nrad);
     }
   }
   return status;

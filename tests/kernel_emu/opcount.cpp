@@ -32,6 +32,7 @@ struct Cd {
   constexpr Cd() : v(0.) {}
   constexpr Cd(double x) : v(x) {}
   explicit operator int() const { return (int)v; }
+  explicit operator float() const { return (float)v; }
   explicit operator long long() const { return (long long)v; }
   Cd &operator+=(Cd o) { g_ops.add++; v += o.v; return *this; }
   Cd &operator-=(Cd o) { g_ops.add++; v -= o.v; return *this; }

@@ -248,3 +248,37 @@ def test_emu_batch_rescaling(unpol):
     for k, e in errs.items():
         assert e.max() < 1e-5, (k, e.max())
         assert np.mean(e > 1e-6) < 1e-3, (k, np.mean(e > 1e-6))
+
+
+def image_summary(I, Q, U, V):
+    """Ftot-proportional sum, net LP and CP fractions and EVPA (rad) of an
+    image, as print_image_stats forms them (src/main.c:1001-1042)."""
+    sI, sQ, sU, sV = I.sum(), Q.sum(), U.sum(), V.sum()
+    return sI, np.hypot(sQ, sU) / sI, sV / sI, 0.5 * np.arctan2(sU, sQ)
+
+
+@pytest.mark.parametrize("thetacam,etype", [(163., 4), (60., 1)])
+def test_emu_fp32_transfer(thetacam, etype):
+    """FP32 transfer coefficients and Stokes-step exponentials (FP64
+    geodesics, connection, tetrads and N), split and fused forms, against
+    the FP64 oracle: Ftot, LP, CP and EVPA well inside the 1e-4 bar of the
+    mixed-precision mode (measured ~1e-7 here)."""
+    h, prims = small_dump()
+    orc, fields = O.iharm_oracle(h, prims, M_unit=3e26)
+    p = O.default_params(nx=20, ny=20, thetacam=thetacam, freqcgs=230e9, emission_type=etype)
+    ref = orc.render(p)
+    S = ref["imageS"].reshape(-1, 5)
+    rI, rLP, rCP, rEV = image_summary(S[:, 0], S[:, 1], S[:, 2], S[:, 3])
+    g, m = G.iharm_geom(h), G.iharm_model()
+    im, scale, _ = G.make_image(g, m.L_unit, nx=20, ny=20, thetacam=thetacam, freqcgs=230e9,
+                                emission_type=etype, transfer_fp32=1)
+    d = emu.pack_dump(fields)
+    for rc, out7, ns, fl in (emu.render(g, m, im, d), emu.render_scaled(g, m, im, im, d)):
+        assert rc == 0 and np.array_equal(ns, ref["nstep"].ravel())
+        f3 = 230e9 ** 3
+        gI, gLP, gCP, gEV = image_summary(out7[:, 1] * f3, -out7[:, 2] * f3, -out7[:, 3] * f3, out7[:, 4] * f3)
+        errs = dict(Ftot=abs(gI / rI - 1), LP=abs(gLP - rLP), CP=abs(gCP - rCP), EVPA=abs(gEV - rEV),
+                    Funpol=abs((out7[:, 0] * f3).sum() / ref["image"].sum() - 1))
+        print(errs)
+        assert errs["Ftot"] < 1e-5 and errs["Funpol"] < 1e-5
+        assert errs["LP"] < 1e-5 and errs["CP"] < 1e-5 and errs["EVPA"] < 1e-5
