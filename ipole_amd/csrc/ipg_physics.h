@@ -572,11 +572,42 @@ IPG_HD bool radiating_region(const ipgpu_geom &G, const ipgpu_model &M, double r
           && th < (PI - M.th_beg));
 }
 
+/* One slot of the 8 cell records around a point, trilinear (src/grid.c:15-44). */
+IPG_HD double gather_slot(const ipgpu_geom &G, const double *dump, const Cell &c, int f)
+{
+  const long s2 = (G.N3 + 2), s1 = (long)(G.N2 + 2) * (G.N3 + 2);
+  const long base = (long)c.i * s1 + (long)c.j * s2 + c.k;
+  const double b1 = 1. - c.d1, b2 = 1. - c.d2;
+  const double *p000 = dump + CELL_STRIDE * base;
+  const double *p010 = dump + CELL_STRIDE * (base + s2);
+  const double *p100 = dump + CELL_STRIDE * (base + s1);
+  const double *p110 = dump + CELL_STRIDE * (base + s1 + s2);
+  double lo = p000[f] * b1 * b2 + p010[f] * b1 * c.d2 + p100[f] * c.d1 * b2 + p110[f] * c.d1 * c.d2;
+  double hi = p000[CELL_STRIDE + f] * b1 * b2 + p010[CELL_STRIDE + f] * b1 * c.d2
+      + p100[CELL_STRIDE + f] * c.d1 * b2 + p110[CELL_STRIDE + f] * c.d1 * c.d2;
+  return (1. - c.d3) * lo + c.d3 * hi;
+}
+
+/* slots of a cell record beyond the 10 fields (filled by the derivation) */
+constexpr int SLOT_RHO = 10, SLOT_U = 11, SLOT_BETA = 12, SLOT_DTH_RHIGH = 13, SLOT_DTH_RLOW = 14;
+
+/* What the adjoint needs besides the fluid state: the interpolation cell
+ * (ghosted linear index of its lower corner, -1 outside the grid) and
+ * weights, the sigma smoothing factor, and dTheta_e/dR_high, dR_low at the
+ * point. */
+struct FluidExt {
+  double cell, d1, d2, d3, smooth, dth_rh, dth_rl;
+};
+
 /* Fluid state at a point: get_model_fourv/ne/thetae/b in one evaluation
  * (model/iharm/model.c:420-593; model/sphere/model.c:126-211). */
 IPG_HD void fluid(const ipgpu_geom &G, const ipgpu_model &M, const double *dump, const double X[4],
-                  const Metric &m, Fluid &F)
+                  const Metric &m, Fluid &F, FluidExt *ext = nullptr)
 {
+  if (ext) {
+    ext->cell = -1.;
+    ext->d1 = ext->d2 = ext->d3 = ext->smooth = ext->dth_rh = ext->dth_rl = 0.;
+  }
   if (M.kind == IPGPU_MODEL_SPHERE) {
     double U[4] = {1., 0., 0., 0.};
     double B[4] = {0., M.sph_B0 * cos(m.th) / m.r, -M.sph_B0 * sin(m.th) / (m.r + 1.e-8), 0.};
@@ -625,9 +656,19 @@ IPG_HD void fluid(const ipgpu_geom &G, const ipgpu_model &M, const double *dump,
   F.Bcon[3] = (v[5] + F.Ucon[3] * F.Bcon[0]) * iu0;
   lower(m.g, F.Bcon, F.Bcov);
   double sigma = v[9];
-  F.ne = (sigma > M.sigma_cut) ? 0. : v[6] * sigma_smoothfac(M, sigma);
+  const double smooth = (sigma > M.sigma_cut) ? 0. : sigma_smoothfac(M, sigma);
+  F.ne = (sigma > M.sigma_cut) ? 0. : v[6] * smooth;
   F.thetae = v[7];
   F.B = v[8];
+  if (ext) {
+    ext->cell = (double)(((long)c.i * (G.N2 + 2) + c.j) * (long)(G.N3 + 2) + c.k);
+    ext->d1 = c.d1;
+    ext->d2 = c.d2;
+    ext->d3 = c.d3;
+    ext->smooth = smooth;
+    ext->dth_rh = gather_slot(G, dump, c, SLOT_DTH_RHIGH);
+    ext->dth_rl = gather_slot(G, dump, c, SLOT_DTH_RLOW);
+  }
 }
 
 /* --------------------------------------------------------- K_n(1/Theta_e) */
@@ -744,37 +785,49 @@ constexpr double COS_PI_2 = 6.123233995736766e-17;       /* cos of the double ne
  * the field angle come from mu = cos(theta) directly (theta = acos(mu) in
  * [0, pi]); the fits' powers share their logarithms.
  * ju/ku: j_I/nu^2 and a_I nu of the unpolarized solve. */
-IPG_HD void coefficients(const Fluid &F, const double K[4], int emission_type, bool pol,
-                         double &ju, double &ku, Coef &c)
+/* Fluid-frame frequency and field angle (get_fluid_nu, get_bk_angle,
+ * src/radiation.c:26-72): nu, sin and cos of theta, and whether theta is 0
+ * or pi. */
+struct Geo {
+  double nu, sth, cth;
+  bool edge;
+};
+
+IPG_HD Geo coef_geometry(const Fluid &F, const double K[4])
+{
+  Geo q;
+  double kdotu = K[0] * F.Ucov[0] + K[1] * F.Ucov[1] + K[2] * F.Ucov[2] + K[3] * F.Ucov[3];
+  q.nu = -kdotu * (ME * CL * CL / HPL);
+  q.edge = false;
+  double Bm = sqrt(fabs(F.Bcon[0] * F.Bcov[0] + F.Bcon[1] * F.Bcov[1] + F.Bcon[2] * F.Bcov[2]
+                        + F.Bcon[3] * F.Bcov[3]));
+  if (Bm == 0.) {
+    q.sth = 1.;
+    q.cth = COS_PI_2;
+  } else {
+    double mu = (K[0] * F.Bcov[0] + K[1] * F.Bcov[1] + K[2] * F.Bcov[2] + K[3] * F.Bcov[3])
+        * rcp(fabs(kdotu) * Bm);
+    if (fabs(mu) > 1.) mu = (mu > 0.) ? 1. : -1.;
+    q.edge = (mu == 1. || mu == -1.);   /* theta = 0 or pi */
+    q.sth = sqrt((1. - mu) * (1. + mu));
+    q.cth = mu;
+  }
+  return q;
+}
+
+/* The fits proper, from the fluid-frame frequency and angle and the local
+ * n_e, Theta_e and B. */
+IPG_HD void coefficients_core(const Geo &q, double Ne, double Th, double B, int emission_type, bool pol,
+                              double &ju, double &ku, Coef &c)
 {
   ju = 0.;
   ku = 0.;
   c.jI = c.jQ = c.jU = c.jV = c.aI = c.aQ = c.aU = c.aV = c.rQ = c.rU = c.rV = 0.;
-  double Ne = F.ne;
   if (Ne <= 0.) return;
-  /* get_fluid_nu, get_bk_angle: src/radiation.c:26-72 */
-  double kdotu = K[0] * F.Ucov[0] + K[1] * F.Ucov[1] + K[2] * F.Ucov[2] + K[3] * F.Ucov[3];
-  double nu = -kdotu * (ME * CL * CL / HPL);
+  const double nu = q.nu, sth = q.sth, cth = q.cth;
+  const bool edge = q.edge;
   double inu = rcp(nu);
   double nusq = nu * nu;
-  double sth, cth;
-  bool edge = false;
-  {
-    double Bm = sqrt(fabs(F.Bcon[0] * F.Bcov[0] + F.Bcon[1] * F.Bcov[1] + F.Bcon[2] * F.Bcov[2]
-                          + F.Bcon[3] * F.Bcov[3]));
-    if (Bm == 0.) {
-      sth = 1.;
-      cth = COS_PI_2;
-    } else {
-      double mu = (K[0] * F.Bcov[0] + K[1] * F.Bcov[1] + K[2] * F.Bcov[2] + K[3] * F.Bcov[3])
-          * rcp(fabs(kdotu) * Bm);
-      if (fabs(mu) > 1.) mu = (mu > 0.) ? 1. : -1.;
-      edge = (mu == 1. || mu == -1.);   /* theta = 0 or pi */
-      sth = sqrt((1. - mu) * (1. + mu));
-      cth = mu;
-    }
-  }
-  double B = F.B, Th = F.thetae;
   double iTh = rcp(Th);
   double ks[3], emx;
   bessel_k012(iTh, ks, emx);
@@ -891,6 +944,17 @@ IPG_HD void coefficients(const Fluid &F, const double K[4], int emission_type, b
     c.rV = (2. * PI / CL) * nu * eps12 * nu;
     c.rU = 0. * nu;
   }
+}
+
+/* Coefficients at a point from the fluid state and the wavevector. */
+IPG_HD void coefficients(const Fluid &F, const double K[4], int emission_type, bool pol, double &ju,
+                         double &ku, Coef &c)
+{
+  if (F.ne <= 0.) {
+    coefficients_core(Geo{0., 1., 0., false}, 0., 1., 0., emission_type, pol, ju, ku, c);
+    return;
+  }
+  coefficients_core(coef_geometry(F, K), F.ne, F.thetae, F.B, emission_type, pol, ju, ku, c);
 }
 
 /* ------------------------------------------- coefficients, FP32 shape functions */

@@ -167,11 +167,22 @@ IPG_HD void scale_rec(Rec &r, const Scale &sc)
 
 /* Packet of transfer step n (from trajectory point n to n-1) of
  * integrate_emission, src/ipolarray.c:78-180.  nt may be null. */
+/* Adjoint record of one step (AX doubles), written by prep_step when asked:
+ * the inputs of the coefficients (fluid-frame frequency and angle, n_e,
+ * Theta_e, B) and of their dependence on the dump (interpolation cell and
+ * weights, sigma smoothing, dTheta_e/dR_high, dR_low).  ax[4] = 0 where
+ * the step does not radiate or n_e = 0. */
+constexpr int AX = 16;
+constexpr int AX_NU = 0, AX_STH = 1, AX_CTH = 2, AX_EDGE = 3, AX_NE = 4, AX_TH = 5, AX_B = 6, AX_CELL = 7,
+              AX_D1 = 8, AX_D2 = 9, AX_D3 = 10, AX_SMOOTH = 11, AX_DRH = 12, AX_DRL = 13, AX_ZERO = 14;
+
 template <bool F32 = false>
 IPG_HD_INLINE void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int n,
                       const double *rec, const int *nt, const double *dump, double *pk, bool pol,
-                      const Scale &sc)
+                      const Scale &sc, double *ax = nullptr)
 {
+  if (ax)
+    for (int q = 0; q < AX; q++) ax[q] = 0.;
   Rec ti, tf;
   load_rec(rec, n, ti);
   load_rec(rec, n - 1, tf);
@@ -212,9 +223,28 @@ IPG_HD_INLINE void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const i
     }
     if (zero_emission) meta |= 2;
     Fluid F;
-    fluid(G, Mo, dump, tf.X, m, F);
+    FluidExt ext;
+    fluid(G, Mo, dump, tf.X, m, F, ax ? &ext : nullptr);
     F.ne *= sc.ne;
     F.B *= sc.b;
+    if (ax && F.ne > 0.) {
+      Geo q = coef_geometry(F, tf.K);
+      ax[AX_NU] = q.nu;
+      ax[AX_STH] = q.sth;
+      ax[AX_CTH] = q.cth;
+      ax[AX_EDGE] = q.edge ? 1. : 0.;
+      ax[AX_NE] = F.ne;
+      ax[AX_TH] = F.thetae;
+      ax[AX_B] = F.B;
+      ax[AX_CELL] = ext.cell;
+      ax[AX_D1] = ext.d1;
+      ax[AX_D2] = ext.d2;
+      ax[AX_D3] = ext.d3;
+      ax[AX_SMOOTH] = ext.smooth;
+      ax[AX_DRH] = ext.dth_rh;
+      ax[AX_DRL] = ext.dth_rl;
+      ax[AX_ZERO] = zero_emission ? 1. : 0.;
+    }
     Coef c;
     double ju, ku;
     if (F32) coefficients_f32(F, tf.K, P.emission_type, pol, ju, ku, c);
@@ -551,5 +581,218 @@ IPG_HD int transfer_fused(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgp
   nrad = st.nrad;
   return chain_finish(G, P, st, out7);
 }
+
+#if !defined(IPG_OPCOUNT)
+/* ------------------------------------------------------------------ adjoint
+ *
+ * Reverse mode through the state recurrence of one ray, for the gradient of
+ * L = sum over pixels and I, Q, U, V of w_s (S_s - target_s)^2 (S in the
+ * reference's output units and sign convention) with respect to M_unit,
+ * R_high, R_low and a per-cell scale of n_e.  The adjoint state is a
+ * Hermitian Lambda with dL = Re tr(Lambda^dagger dN); the transposes of the
+ * step's linear maps are:
+ *   transport  N -> N - dl (M N + N M^T):        Lambda -> Lambda - dl (M^T Lambda + Lambda M)
+ *   Stokes     S = K_e(N) (herm_to_stokes, rows e1, e2):  K_e^T(s) = herm_add_stokes(e1, e2, s / 2)
+ *   update     N += L_E(dS) (herm_add_stokes, E1, E2):    L_E^T(Lambda) = 2 herm_to_stokes(Lambda, E1, E2)
+ * The step's dependence on n_e, Theta_e and B (through the fits and the
+ * exact Stokes step) is taken by central differences of the coefficient
+ * map at the step (relative step 1e-5).  States are checkpointed every CK
+ * steps on the forward pass and recomputed segment by segment.  The Stokes
+ * floors, a non-differentiable clamp that is off by default, are not
+ * differentiated. */
+constexpr int CK = 32;
+
+/* M N + N M^T (no scaling) */
+IPG_HD void herm_LM(const Herm &N, const double M[4][4], bool transposed, Herm &out)
+{
+  double Mt[4][4];
+  for (int i = 0; i < 4; i++)
+    for (int k = 0; k < 4; k++) Mt[i][k] = transposed ? M[k][i] : M[i][k];
+  Herm zero;
+  for (int q = 0; q < 4; q++) zero.d[q] = 0.;
+  for (int q = 0; q < 6; q++) zero.re[q] = zero.im[q] = 0.;
+  transport(zero, N, Mt, -1., out);
+}
+
+IPG_HD void herm_axpy(Herm &y, double a, const Herm &x)
+{
+  for (int q = 0; q < 4; q++) y.d[q] += a * x.d[q];
+  for (int q = 0; q < 6; q++) {
+    y.re[q] += a * x.re[q];
+    y.im[q] += a * x.im[q];
+  }
+}
+
+/* dF/d(ln n_e), dF/dTheta_e, dF/d(ln B) of F = T(c) S0 + cv(c) at a step. */
+IPG_HD void step_param_derivs(const double *ax, int etype, double dl, const double S0[4], double dF[3][4])
+{
+  Geo q;
+  q.nu = ax[AX_NU];
+  q.sth = ax[AX_STH];
+  q.cth = ax[AX_CTH];
+  q.edge = ax[AX_EDGE] != 0.;
+  const double Ne = ax[AX_NE], Th = ax[AX_TH], B = ax[AX_B];
+  const bool zero = ax[AX_ZERO] != 0.;
+  const double eps = 1.e-5;
+  for (int p = 0; p < 3; p++) {
+    double Fv[2][4];
+    for (int sgn = 0; sgn < 2; sgn++) {
+      const double f = sgn ? (1. - eps) : (1. + eps);
+      double ne = Ne, th = Th, b = B;
+      if (p == 0) ne *= f;
+      else if (p == 1) th *= f;
+      else b *= f;
+      double ju, ku;
+      Coef c;
+      coefficients_core(q, ne, th, b, etype, true, ju, ku, c);
+      if (zero) c.jI = c.jQ = c.jU = c.jV = 0.;
+      double T[4][4], cv[4];
+      stokes_affine(c, dl, T, cv);
+      for (int i = 0; i < 4; i++) Fv[sgn][i] = cv[i] + T[i][0] * S0[0] + T[i][1] * S0[1] + T[i][2] * S0[2] + T[i][3] * S0[3];
+    }
+    /* ln(1+eps) - ln(1-eps) for the logarithmic directions, 2 eps Theta_e for Theta_e */
+    const double den = (p == 1) ? 2. * eps * Th : (log(1. + eps) - log(1. - eps));
+    for (int i = 0; i < 4; i++) dF[p][i] = (Fv[0][i] - Fv[1][i]) / den;
+  }
+}
+
+IPG_HD void atomic_add_f64(double *p, double v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  atomicAdd(p, v);
+#else
+#pragma omp atomic
+  *p += v;
+#endif
+}
+
+/* Loss and gradient of one ray.  pkt/aux: its packets and adjoint records
+ * (packet 0 is step ns); tgt: the pixel's target I, Q, U, V (output units);
+ * w: component weights; ckpt: room for ceil(ns / CK) states; seg: room for
+ * CK states; gcell (ghosted cells) receives dL/d(n_e scale) of each cell.
+ * out: loss, dL/dln(M_unit), dL/dR_high, dL/dR_low; Simg: the pixel's
+ * I, Q, U, V. */
+IPG_HD void adjoint_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const double *pkt, const double *aux,
+                        const double tgt[4], const double w[4], const double *dump, Herm *ckpt, Herm *seg,
+                        double *gcell, double out[4], double Simg[4])
+{
+  out[0] = out[1] = out[2] = out[3] = 0.;
+  /* forward, with checkpoints */
+  Chain st;
+  chain_init(st, 0., 0.);
+  for (int q = 0; q < ns; q++) {
+    if (q % CK == 0) ckpt[q / CK] = st.N;
+    chain_step(st, pkt + (size_t)q * PK, true, 0);
+  }
+  double E[4][4];
+  Metric mc;
+  const double Xc[4] = {P.Xcam[0], P.Xcam[1], P.Xcam[2], P.Xcam[3]};
+  camera_tetrad(G, Xc, 0, E, mc);
+  double Sc[4];
+  project(st.N, E, mc, P.rotcam, Sc);
+  const double f3 = P.freqcgs * P.freqcgs * P.freqcgs;
+  const double qs = (P.qu_conv == 0) ? -1. : 1.;
+  const double sgn[4] = {1., qs, qs, 1.};
+  double sig[4];
+  for (int i = 0; i < 4; i++) {
+    Simg[i] = f3 * sgn[i] * Sc[i];
+    double r = Simg[i] - tgt[i];
+    out[0] += w[i] * r * r;
+    sig[i] = 2. * w[i] * r * f3 * sgn[i];
+  }
+  /* project_N^T: the rotation by -2 rotcam, then K_e^T with the camera rows */
+  const double rc = -2. * P.rotcam, cr = cos(rc), sr = sin(rc);
+  double tau[4] = {0.5 * sig[0], 0.5 * (sig[1] * cr + sig[2] * sr), 0.5 * (-sig[1] * sr + sig[2] * cr), 0.5 * sig[3]};
+  double e1c[4], e2c[4];
+  lower(mc.g, E[1], e1c);
+  lower(mc.g, E[2], e2c);
+  Herm L;
+  for (int q = 0; q < 4; q++) L.d[q] = 0.;
+  for (int q = 0; q < 6; q++) L.re[q] = L.im[q] = 0.;
+  herm_add_stokes(L, e1c, e2c, tau);
+  /* backward, segment by segment */
+  const long s2 = G.N3 + 2, s1 = (long)(G.N2 + 2) * (G.N3 + 2);
+  const int nseg = (ns + CK - 1) / CK;
+  for (int k = nseg - 1; k >= 0; k--) {
+    const int q0 = k * CK, q1 = (q0 + CK < ns) ? q0 + CK : ns;
+    Chain rs;
+    chain_init(rs, 0., 0.);
+    rs.N = ckpt[k];
+    rs.Nnz = herm_nonzero(rs.N);
+    if (q0 > 0)
+      for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) rs.Mi[i][j] = pkt[(size_t)(q0 - 1) * PK + PK_MF + 4 * i + j];
+    for (int q = q0; q < q1; q++) {
+      seg[q - q0] = rs.N;
+      chain_step(rs, pkt + (size_t)q * PK, true, 0);
+    }
+    for (int q = q1 - 1; q >= q0; q--) {
+      const double *pk = pkt + (size_t)q * PK;
+      const double *ax = aux + (size_t)q * AX;
+      const Herm &Nb = seg[q - q0];
+      const bool nnz = herm_nonzero(Nb);
+      double Mi[4][4], Mh[4][4];
+      if (nnz) {
+        for (int i = 0; i < 4; i++)
+          for (int j = 0; j < 4; j++) {
+            Mi[i][j] = pkt[(size_t)(q - 1) * PK + PK_MF + 4 * i + j];
+            Mh[i][j] = pk[PK_MH + 4 * i + j];
+          }
+      }
+      const int meta = (int)pk[PK_META];
+      if (meta & 1) {
+        Herm Nt = Nb;
+        if (nnz) {
+          Herm Nh;
+          transport(Nb, Nb, Mi, pk[PK_HDL], Nh);
+          transport(Nb, Nh, Mh, pk[PK_FDL], Nt);
+        }
+        double mu[4], S0[4];
+        herm_to_stokes(L, pk + PK_E1, pk + PK_E2, mu);
+        for (int i = 0; i < 4; i++) mu[i] *= 2.;
+        herm_to_stokes(Nt, pk + PK_e1, pk + PK_e2, S0);
+        if (ax[AX_NE] > 0.) {
+          double dF[3][4];
+          step_param_derivs(ax, P.emission_type, pk[PK_DL], S0, dF);
+          double g[3];
+          for (int p = 0; p < 3; p++) g[p] = mu[0] * dF[p][0] + mu[1] * dF[p][1] + mu[2] * dF[p][2] + mu[3] * dF[p][3];
+          out[1] += g[0] + 0.5 * g[2];
+          out[2] += g[1] * ax[AX_DRH];
+          out[3] += g[1] * ax[AX_DRL];
+          if (gcell && ax[AX_CELL] >= 0.) {
+            const long base = (long)ax[AX_CELL];
+            const double d[3] = {ax[AX_D1], ax[AX_D2], ax[AX_D3]};
+            const double f = g[0] * ax[AX_SMOOTH] / ax[AX_NE];
+            for (int c = 0; c < 8; c++) {
+              const int di = (c >> 2) & 1, dj = (c >> 1) & 1, dk = c & 1;
+              const long idx = base + di * s1 + dj * s2 + dk;
+              const double wgt = (di ? d[0] : 1. - d[0]) * (dj ? d[1] : 1. - d[1]) * (dk ? d[2] : 1. - d[2]);
+              const double ne_c = dump[CELL_STRIDE * idx + 6];
+              if (wgt * ne_c != 0.) atomic_add_f64(gcell + idx, f * wgt * ne_c);
+            }
+          }
+        }
+        /* Lambda += K_e^T((T - I)^T mu) */
+        double v[4];
+        for (int i = 0; i < 4; i++) {
+          double acc = -mu[i];
+          for (int r = 0; r < 4; r++) acc += pk[PK_T + 4 * r + i] * mu[r];
+          v[i] = 0.5 * acc;
+        }
+        herm_add_stokes(L, pk + PK_e1, pk + PK_e2, v);
+      }
+      if (nnz) {
+        /* transport^T: Lambda - f G + f h L_{Mi^T}(G), G = L_{Mh^T}(Lambda) */
+        Herm Gm, Hm;
+        herm_LM(L, Mh, true, Gm);
+        herm_LM(Gm, Mi, true, Hm);
+        const double f = pk[PK_FDL], h = pk[PK_HDL];
+        herm_axpy(L, -f, Gm);
+        herm_axpy(L, f * h, Hm);
+      }
+    }
+  }
+}
+#endif
 
 }  // namespace ipg

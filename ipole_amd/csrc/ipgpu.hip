@@ -327,6 +327,11 @@ __global__ void k_derive_thermo(size_t ncell, ipgpu_units U, double RHO_unit, do
   d[7] = fmax(th, 1.e-3);
   d[9] = fmax(sigma_m, SMALL);
   d[S_BETA] = fmax(beta_m, SMALL);
+  /* dTheta_e/dR_high and dR_low of this electron model (zero where the
+   * 1e-3 floor holds), for gradients of images with respect to them */
+  double dth_dtrat = (th > 1.e-3) ? -th * (U.game - 1.) / ((U.gamp - 1.) + (U.game - 1.) * trat) : 0.;
+  d[SLOT_DTH_RHIGH] = dth_dtrat * betasq / (1. + betasq);
+  d[SLOT_DTH_RLOW] = dth_dtrat / (1. + betasq);
 }
 
 __global__ void k_unpack(size_t ncell, const double *__restrict__ dump, double *__restrict__ soa11)

@@ -188,6 +188,50 @@ int emu_render_scaled(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_ima
   return status;
 }
 
+/* Loss and gradients of the L2 image loss against target (npix x 5, the
+ * reference layout) over all pixels of the image: out[0..3] = loss,
+ * dL/dln(M_unit), dL/dR_high, dL/dR_low; gcell (ghosted cells, zeroed by
+ * the caller) receives dL/d(n_e scale) per cell; Simg (npix x 4). */
+int emu_adjoint(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, const double *dump,
+                const double *target, const double *w, double *out, double *gcell, double *Simg)
+{
+  const int npix = P->nx * P->ny;
+  int status = 0;
+  double tot[4] = {0., 0., 0., 0.};
+#pragma omp parallel
+  {
+    std::vector<double> rec((size_t)P->maxnstep * REC);
+    std::vector<double> pkt, aux;
+    std::vector<Herm> ckpt, seg(CK);
+    double loc[4] = {0., 0., 0., 0.};
+#pragma omp for schedule(dynamic, 1)
+    for (int n = 0; n < npix; n++) {
+      int i = n / P->ny, j = n % P->ny;
+      int ns = trace_ray(*G, *P, i, j, P->maxnstep, rec.data(), nullptr);
+      if (ns >= P->maxnstep - 1) {
+#pragma omp critical
+        status = -10;
+        continue;
+      }
+      pkt.assign((size_t)(ns > 0 ? ns : 1) * PK, 0.);
+      aux.assign((size_t)(ns > 0 ? ns : 1) * AX, 0.);
+      for (int q = 0; q < ns; q++)
+        prep_step<false>(*G, *M, *P, ns - q, rec.data(), nullptr, dump, pkt.data() + (size_t)q * PK, true,
+                         unit_scale(), aux.data() + (size_t)q * AX);
+      ckpt.resize(ns / CK + 1);
+      double o[4];
+      const double tg[4] = {target[5 * n], target[5 * n + 1], target[5 * n + 2], target[5 * n + 3]};
+      adjoint_ray(*G, *P, ns, pkt.data(), aux.data(), tg, w, dump, ckpt.data(), seg.data(), gcell, o,
+                  Simg + 4 * (size_t)n);
+      for (int q = 0; q < 4; q++) loc[q] += o[q];
+    }
+#pragma omp critical
+    for (int q = 0; q < 4; q++) tot[q] += loc[q];
+  }
+  for (int q = 0; q < 4; q++) out[q] = tot[q];
+  return status;
+}
+
 /* Trajectory of one pixel: nstep+1 records of REC doubles. */
 int emu_trace(const ipgpu_geom *G, const ipgpu_image *P, int i, int j, double *rec)
 {

@@ -30,6 +30,8 @@ def lib():
                                  C.c_int, dp, ip, ip]
         L.emu_render_scaled.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image),
                                         C.POINTER(G.Image), dp, ip, C.c_int, C.c_double, dp, ip, ip]
+        L.emu_adjoint.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), dp, dp, dp, dp,
+                                  dp, dp]
         L.emu_trace.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Image), C.c_int, C.c_int, dp]
         _lib = L
     return _lib
@@ -110,6 +112,33 @@ def render_scaled(geom, model, image_trace, image_job, dump, ne_scale=1.):
                                  G.P(d) if d is not None else None, G.IP(pix), n, ne_scale, G.P(out7),
                                  G.IP(ns), G.IP(fl))
     return rc, out7, ns, fl
+
+
+def pack_dump_adjoint(fields11, units):
+    """16-slot cell records with slots 13, 14 = dTheta_e/dR_high, dR_low of
+    the R_high/R_low electron model (model/iharm/model.c:610-717), from the
+    derived Theta_e and beta (fields11[7], fields11[10])."""
+    d = pack_dump(fields11[:10]).reshape(-1, 16)
+    th = np.asarray(fields11[7]).ravel()
+    beta = np.asarray(fields11[10]).ravel()
+    betasq = beta ** 2 / units.beta_crit ** 2
+    trat = units.trat_large * betasq / (1. + betasq) + units.trat_small / (1. + betasq)
+    dth = np.where(th > 1e-3, -th * (units.game - 1.) / ((units.gamp - 1.) + (units.game - 1.) * trat), 0.)
+    d[:, 13] = dth * betasq / (1. + betasq)
+    d[:, 14] = dth / (1. + betasq)
+    return d.ravel()
+
+
+def adjoint(geom, model, image, dump, target, w=(1., 1., 1., 1.)):
+    """(loss, dL/dlnM, dL/dR_high, dL/dR_low), per-cell dL/d(ne scale) over
+    the ghosted cells, and the image's I Q U V."""
+    n = image.nx * image.ny
+    nc = (geom.N1 + 2) * (geom.N2 + 2) * (geom.N3 + 2)
+    out, gcell, S = np.zeros(4), np.zeros(nc), np.zeros((n, 4))
+    rc = lib().emu_adjoint(C.byref(geom), C.byref(model), C.byref(image), G.P(f64(dump)),
+                           G.P(f64(target)), G.P(f64(w)), G.P(out), G.P(gcell), G.P(S))
+    assert rc == 0
+    return out, gcell, S
 
 
 def trace(geom, image, i, j):

@@ -282,3 +282,50 @@ def test_emu_fp32_transfer(thetacam, etype):
         print(errs)
         assert errs["Ftot"] < 1e-5 and errs["Funpol"] < 1e-5
         assert errs["LP"] < 1e-5 and errs["CP"] < 1e-5 and errs["EVPA"] < 1e-5
+
+
+def test_emu_adjoint_vs_finite_differences():
+    """Reverse-mode gradient of the L2 Stokes-image loss (the GPU kernel's
+    code on the CPU) against central finite differences of the same loss:
+    d/dln(M_unit) and d/dR_high, d/dR_low through a re-derivation of the dump,
+    and d/d(n_e scale) of single cells through their n_e.  The loss is
+    strongly curved in ln M_unit (optical depth), so the differences use
+    h = 1e-5, whose truncation error is ~4e-6 here (it falls as h^2: 4e-4 at
+    h = 1e-4); bar 1e-4 relative (the adjoint's gradient-check bar is 1e-3)."""
+    h, prims = small_dump(N=(32, 24, 24), seed=5)
+    M, Rh, Rl = 3e26, 40., 1.
+    g, m = G.iharm_geom(h), G.iharm_model()
+    im, _, _ = G.make_image(g, m.L_unit, nx=12, ny=12, thetacam=60., freqcgs=230e9)
+
+    def dump_at(Mu, rh, rl):
+        _, f11 = O.iharm_oracle(h, prims, M_unit=Mu, trat_large=rh, trat_small=rl)
+        return emu.pack_dump_adjoint(f11, G.iharm_units(Mu, trat_large=rh, trat_small=rl))
+
+    _, _, St = emu.adjoint(g, m, im, dump_at(1.3 * M, Rh, Rl), np.zeros(5 * 144))
+    target = np.zeros((144, 5))
+    target[:, :4] = St
+    w = np.array([1., 3., 3., 5.])
+    d0 = dump_at(M, Rh, Rl)
+    out, gcell, S = emu.adjoint(g, m, im, d0, target.ravel(), w)
+    scale = out[0]
+    assert scale > 0
+
+    def loss(d):
+        return emu.adjoint(g, m, im, d, target.ravel(), w)[0][0]
+
+    hM = 1e-5
+    fd_M = (loss(dump_at(M * np.exp(hM), Rh, Rl)) - loss(dump_at(M * np.exp(-hM), Rh, Rl))) / (2 * hM)
+    fd_Rh = (loss(dump_at(M, Rh * (1 + hM), Rl)) - loss(dump_at(M, Rh * (1 - hM), Rl))) / (2 * hM * Rh)
+    fd_Rl = (loss(dump_at(M, Rh, Rl * (1 + hM))) - loss(dump_at(M, Rh, Rl * (1 - hM)))) / (2 * hM * Rl)
+    print("lnM", out[1], fd_M, "Rh", out[2], fd_Rh, "Rl", out[3], fd_Rl)
+    assert abs(out[1] / fd_M - 1) < 1e-4
+    assert abs(out[2] / fd_Rh - 1) < 1e-4
+    assert abs(out[3] / fd_Rl - 1) < 1e-4
+    top = np.argsort(-np.abs(gcell))[:4]
+    for c in top:
+        dp, dm = d0.copy(), d0.copy()
+        dp[16 * c + 6] *= 1 + hM
+        dm[16 * c + 6] *= 1 - hM
+        fd = (loss(dp) - loss(dm)) / (2 * hM)
+        print("cell", c, gcell[c], fd)
+        assert abs(gcell[c] / fd - 1) < 1e-4
