@@ -164,6 +164,31 @@ int ipgpu_bad_job(const ipgpu_ctx *ctx, int *job, int *i, int *j);
  * 160 GiB); rays are processed in as many passes as this requires. */
 int ipgpu_set_batch_memory(ipgpu_ctx *ctx, size_t traj_bytes);
 
+/* Gradient of the image loss
+ *   L = sum over pixels of sum_s w_s (S_s - target_s)^2,  s = I, Q, U, V,
+ * (S in the layout and units of ipgpu_render's imageS) with respect to the
+ * dump's M_unit, R_high (trat_large), R_low (trat_small) and a per-cell
+ * scale of n_e, by reverse mode through the polarized transfer (one
+ * hand-written adjoint kernel: forward recurrence with checkpoints every 32
+ * steps, recomputed per segment on the way back).  No counterpart in the
+ * reference.  Needs a dump from ipgpu_upload_prims (the electron model's
+ * R_high/R_low dependence is taken from it).
+ *   target:  nx*ny*5 doubles, ipgpu_render's imageS layout (tauF ignored)
+ *   weights: w_I, w_Q, w_U, w_V (NULL: all 1)
+ *   out4:    L, dL/dM_unit, dL/dR_high, dL/dR_low
+ *   grad_density: N1*N2*N3 doubles (x3 fastest), dL/ds_c of n_e -> s_c n_e
+ *            at s = 1 for every interior cell c (ghost-zone copies folded
+ *            back), or NULL
+ *   imageS:  the image the loss was taken on (NULL: not returned)
+ * The Stokes floors (stokes_floors = 1) are not differentiated. */
+int ipgpu_adjoint(ipgpu_ctx *ctx, const ipgpu_image *img, const double *target, const double *weights,
+                  double out4[4], double *grad_density, double *imageS);
+
+/* Multiply n_e of every interior cell by scale[N1*N2*N3] (x3 fastest) and
+ * refresh its ghost copies: the per-cell density scale whose gradient
+ * ipgpu_adjoint returns.  No counterpart in the reference. */
+int ipgpu_scale_density(ipgpu_ctx *ctx, const double *scale);
+
 /* Pixel (i, j) that exceeded maxnstep in the last render (IPGPU_ERR_MAXNSTEP). */
 int ipgpu_bad_pixel(const ipgpu_ctx *ctx, int *i, int *j);
 

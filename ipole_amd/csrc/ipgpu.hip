@@ -47,6 +47,11 @@ __device__ __forceinline__ void ray_pixel(int ray, int nx, int ny, int &i, int &
   j = tj * TILE + l % TILE;
 }
 
+/* the loss's component weights, by value */
+struct AdjW {
+  double w[4];
+};
+
 __global__ __launch_bounds__(WAVE) void k_trace(ipgpu_geom G, ipgpu_image P, int nrays, int cap,
                                                 double *__restrict__ traj, int *__restrict__ nstep_out,
                                                 int *__restrict__ nturns_buf, int *__restrict__ status,
@@ -117,7 +122,8 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep(ipgpu_geom G, ipgpu_model M
                                                      const int *__restrict__ nstep_in,
                                                      const double *__restrict__ traj,
                                                      const int *__restrict__ nturns_buf,
-                                                     const double *__restrict__ dump, double *__restrict__ pkt)
+                                                     const double *__restrict__ dump, double *__restrict__ pkt,
+                                                     double *__restrict__ aux)
 {
   long long g = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
   if (g >= total) return;
@@ -129,7 +135,8 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep(ipgpu_geom G, ipgpu_model M
   const int q = (int)(g - off[ray]);
   const double *rec = traj + (size_t)ray * cap * REC;
   const int *nt = nturns_buf ? nturns_buf + (size_t)ray * cap : nullptr;
-  prep_step<F32>(G, Mo, P, ns - q, rec, nt, dump, pkt + (size_t)g * stride, stride == PK, unit_scale());
+  prep_step<F32>(G, Mo, P, ns - q, rec, nt, dump, pkt + (size_t)g * stride, stride == PK, unit_scale(),
+                 aux ? aux + (size_t)g * AX : nullptr);
 }
 
 /* One lane per ray: the recurrence of N and I over the ray's packets. */
@@ -474,6 +481,80 @@ __global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model M
   atomicAdd(&counts[1], (unsigned long long)nrad);
 }
 
+/* ------------------------------------------------------------- adjoint */
+
+/* One lane per ray: loss and gradient of its pixel (adjoint_ray), per-ray
+ * sums to out4 (loss, dL/dln M_unit, dL/dR_high, dL/dR_low), the pixel's
+ * I Q U V to imageS. */
+__global__ __launch_bounds__(WAVE) void k_adjoint(ipgpu_geom G, ipgpu_image P, int nrays,
+                                                  const long long *__restrict__ off,
+                                                  const int *__restrict__ nstep_in,
+                                                  const double *__restrict__ pkt, const double *__restrict__ aux,
+                                                  const double *__restrict__ target, AdjW w,
+                                                  const double *__restrict__ dump, Herm *__restrict__ ckpt,
+                                                  Herm *__restrict__ segbuf, double *__restrict__ gcell,
+                                                  double *__restrict__ out4, double *__restrict__ imageS,
+                                                  int *__restrict__ status)
+{
+  int ray = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ray >= nrays) return;
+  int pi, pj;
+  ray_pixel(ray, P.nx, P.ny, pi, pj);
+  double *o = out4 + (size_t)ray * 4;
+  o[0] = o[1] = o[2] = o[3] = 0.;
+  if (pi >= P.nx || pj >= P.ny) return;
+  const int ns = nstep_in[ray];
+  const size_t p = (size_t)pi * P.ny + pj;
+  if (ns >= P.maxnstep - 1) {
+    atomicMin(&status[1], (int)p + 1);
+    return;
+  }
+  const double tg[4] = {target[5 * p], target[5 * p + 1], target[5 * p + 2], target[5 * p + 3]};
+  double res[4], S[4];
+  adjoint_ray(G, P, ns, pkt + (size_t)off[ray] * PK, aux + (size_t)off[ray] * AX, tg, w.w, dump,
+              ckpt + off[ray] / CK + ray, segbuf + (size_t)ray * CK, gcell, res, S);
+  for (int q = 0; q < 4; q++) {
+    o[q] = res[q];
+    imageS[5 * p + q] = S[q];
+  }
+  imageS[5 * p + 4] = 0.;
+}
+
+/* n_e of interior cell (i, j, k) times scale[i][j][k] */
+__global__ void k_scale_ne(ipgpu_geom G, const double *__restrict__ scale, double *__restrict__ dump)
+{
+  size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (size_t)G.N1 * G.N2 * G.N3) return;
+  int k = (int)(n % G.N3), j = (int)((n / G.N3) % G.N2), i = (int)(n / ((size_t)G.N3 * G.N2));
+  dump[CELL_STRIDE * gcell(G, i + 1, j + 1, k + 1) + 6] *= scale[n];
+}
+
+/* the ghost-zone rules of the derivation for even N3, for n_e alone (n_e
+ * is linear in rho, so a ghost's n_e follows its source's as rho does) */
+__global__ void k_bc_ne(ipgpu_geom G, int stage, double *dump)
+{
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N1 = G.N1, N2 = G.N2, N3 = G.N3;
+  auto NE = [&](int i, int j, int k) -> double & { return dump[CELL_STRIDE * gcell(G, i, j, k) + 6]; };
+  if (stage == 0) {
+    if (n >= N2 * N3) return;
+    int j = n / N3 + 1, k = n % N3 + 1;
+    NE(0, j, k) = NE(1, j, k);
+    NE(N1 + 1, j, k) = NE(N1, j, k);
+  } else if (stage == 1) {
+    if (n >= (N1 + 2) * N3) return;
+    int i = n / N3, k = n % N3 + 1;
+    int kflip = ((k - 1) + (N3 / 2)) % N3 + 1;   /* even N3 (checked by the caller) */
+    NE(i, 0, k) = NE(i, 1, kflip);
+    NE(i, N2 + 1, k) = NE(i, N2, kflip);
+  } else {
+    if (n >= (N1 + 2) * (N2 + 2)) return;
+    int i = n / (N2 + 2), j = n % (N2 + 2);
+    NE(i, j, 0) = NE(i, j, N3);
+    NE(i, j, N3 + 1) = NE(i, j, 1);
+  }
+}
+
 }  // namespace
 
 /* ------------------------------------------------------------------ C-ABI */
@@ -495,6 +576,8 @@ struct ipgpu_ctx {
   size_t nturns_ints = 0;
   double *d_pkt = nullptr;
   size_t pkt_doubles = 0;
+  double *d_aux = nullptr;
+  size_t aux_doubles = 0;
   long long *d_off = nullptr;
   size_t cap_off = 0;
   int *d_blk = nullptr;
@@ -524,6 +607,13 @@ struct ipgpu_ctx {
   size_t cap_bimage = 0, cap_bimageS = 0, cap_btaus = 0, cap_bnstep = 0, cap_bflag = 0;
   unsigned long long *d_bstatus = nullptr;
   size_t batch_traj_bytes = (size_t)160 << 30; /* geodesic records per pass of a batch */
+  /* adjoint */
+  ipgpu_units units{};
+  bool has_units = false;
+  Herm *d_ckpt = nullptr, *d_seg = nullptr;
+  size_t cap_ckpt = 0, cap_seg = 0;
+  double *d_gcell = nullptr, *d_out4 = nullptr, *d_target = nullptr, *d_scale = nullptr;
+  size_t cap_gcell = 0, cap_out4 = 0, cap_target = 0, cap_scale = 0;
 };
 
 #define HIPCHK(ctx, call)                                                                    \
@@ -587,11 +677,12 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
   if (!ctx) return;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
-  void *ptrs[] = {ctx->d_dump, ctx->d_traj, ctx->d_pkt, ctx->d_off, ctx->d_blk, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
+  void *ptrs[] = {ctx->d_dump, ctx->d_traj, ctx->d_pkt, ctx->d_aux, ctx->d_off, ctx->d_blk, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
                   ctx->d_out7, ctx->d_image, ctx->d_imageS, ctx->d_taus, ctx->d_nstep_pix,
                   ctx->d_flag_pix, ctx->d_status, ctx->d_counts,
                   ctx->d_groups, ctx->d_jobs, ctx->d_items, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
-                  ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus};
+                  ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_ckpt, ctx->d_seg, ctx->d_gcell,
+                  ctx->d_out4, ctx->d_target, ctx->d_scale};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int q = 0; q < 5; q++)
@@ -660,6 +751,7 @@ int ipgpu_upload_fields(ipgpu_ctx *ctx, const double *const fields[IPGPU_NFIELDS
     HIPCHK(ctx, hipMemcpyAsync(soa + f * nc, fields[f], nc * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   }
   k_pack<<<(unsigned)((nc + 255) / 256), 256, 0, ctx->stream>>>(nc, soa, ctx->d_dump);
+  ctx->has_units = false; /* the electron model behind these fields is not known */
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   HIPCHK(ctx, hipFree(soa));
@@ -691,6 +783,8 @@ int ipgpu_upload_prims(ipgpu_ctx *ctx, const double *prims, int nprim, const ipg
   k_bc_phi<<<(nf + 255) / 256, 256, 0, ctx->stream>>>(G, ctx->d_dump);
   k_derive_thermo<<<(unsigned)((nc + 255) / 256), 256, 0, ctx->stream>>>(nc, *u, RHO_unit, B_unit, ctx->d_dump);
   HIPCHK(ctx, hipGetLastError());
+  ctx->units = *u;
+  ctx->has_units = true;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   HIPCHK(ctx, hipFree(dp));
   return IPGPU_OK;
@@ -729,6 +823,61 @@ static int run_trace(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int cap, b
   return 0;
 }
 
+/* Geodesics (retraced once with room for the longest ray if the record
+ * capacity was short), packet offsets and the step packets of one image;
+ * with aux, also the adjoint records.  Records events ev[0..2]. */
+static int prepare_packets(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, bool aux, int &cap,
+                           unsigned long long &total, int &stride)
+{
+  int rc;
+  bool turns = P.target_nturns >= 0;
+  cap = ctx->traj_cap_steps < P.maxnstep ? ctx->traj_cap_steps : P.maxnstep;
+  HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+  if ((rc = run_trace(ctx, P, nrays, cap, turns))) return rc;
+  int status[NSTATUS];
+  total = 0;
+  HIPCHK(ctx, hipMemcpyAsync(status, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(&total, ctx->d_counts, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (status[0] > 0) {
+    /* some rays outgrew the per-ray record capacity: retrace with room for
+     * the longest one (bounded by maxnstep) */
+    std::vector<int> ns(nrays);
+    HIPCHK(ctx, hipMemcpy(ns.data(), ctx->d_nstep_ray, nrays * sizeof(int), hipMemcpyDeviceToHost));
+    int mx = 0;
+    for (int v : ns) mx = v > mx ? v : mx;
+    cap = mx + 1 < P.maxnstep ? mx + 1 : P.maxnstep;
+    ctx->traj_cap_steps = cap;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    if ((rc = run_trace(ctx, P, nrays, cap, turns))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(&total, ctx->d_counts, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+  stride = P.only_unpolarized ? PK_UNPOL : PK;
+  if (aux && (rc = grow(ctx, ctx->d_aux, ctx->aux_doubles, (size_t)(total > 0 ? total : 1) * AX))) return rc;
+  if ((rc = grow(ctx, ctx->d_pkt, ctx->pkt_doubles, (size_t)(total > 0 ? total : 1) * stride))) return rc;
+  if ((rc = grow(ctx, ctx->d_off, ctx->cap_off, (size_t)nrays + 1))) return rc;
+  static const int init[NSTATUS] = {0, 0x7fffffff, 0x7fffffff, 0};
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_status, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
+  k_scan<<<1, SCAN_THREADS, 0, ctx->stream>>>(nrays, ctx->d_nstep_ray, ctx->d_off);
+  HIPCHK(ctx, hipGetLastError());
+  if (total > 0) {
+    unsigned nb = (unsigned)((total + PREP_BLOCK - 1) / PREP_BLOCK);
+    if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)nb))) return rc;
+    k_blockmap<<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(nrays, ctx->d_off, ctx->d_blk);
+    HIPCHK(ctx, hipGetLastError());
+    auto kp = P.transfer_fp32 ? k_prep<true> : k_prep<false>;
+    kp<<<nb, PREP_BLOCK, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, (long long)total,
+                                               ctx->d_off, ctx->d_blk, ctx->d_nstep_ray, ctx->d_traj,
+                                               turns ? ctx->d_nturns : nullptr, ctx->d_dump, ctx->d_pkt,
+                                               aux ? ctx->d_aux : nullptr);
+    HIPCHK(ctx, hipGetLastError());
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+  return 0;
+}
+
 int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *imageS,
                  double *taus, int *nstep, int *flag)
 {
@@ -754,49 +903,9 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
   if ((rc = grow(ctx, ctx->d_flag_pix, ctx->cap_flag_pix, npix))) return rc;
   if ((rc = grow(ctx, ctx->d_imageS, ctx->cap_imageS, npix * 5))) return rc;
 
-  bool turns = P.target_nturns >= 0;
-  int cap = ctx->traj_cap_steps < P.maxnstep ? ctx->traj_cap_steps : P.maxnstep;
-  HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-  if ((rc = run_trace(ctx, P, nrays, cap, turns))) return rc;
-  int status[NSTATUS];
-  unsigned long long total = 0;
-  HIPCHK(ctx, hipMemcpyAsync(status, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(&total, ctx->d_counts, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  if (status[0] > 0) {
-    /* some rays outgrew the per-ray record capacity: retrace with room for
-     * the longest one (bounded by maxnstep) */
-    std::vector<int> ns(nrays);
-    HIPCHK(ctx, hipMemcpy(ns.data(), ctx->d_nstep_ray, nrays * sizeof(int), hipMemcpyDeviceToHost));
-    int mx = 0;
-    for (int v : ns) mx = v > mx ? v : mx;
-    cap = mx + 1 < P.maxnstep ? mx + 1 : P.maxnstep;
-    ctx->traj_cap_steps = cap;
-    HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    if ((rc = run_trace(ctx, P, nrays, cap, turns))) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(&total, ctx->d_counts, sizeof(total), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  }
-  HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-  const int stride = P.only_unpolarized ? PK_UNPOL : PK;
-  if ((rc = grow(ctx, ctx->d_pkt, ctx->pkt_doubles, (size_t)(total > 0 ? total : 1) * stride))) return rc;
-  if ((rc = grow(ctx, ctx->d_off, ctx->cap_off, (size_t)nrays + 1))) return rc;
-  int init[NSTATUS] = {0, 0x7fffffff, 0x7fffffff, 0};
-  HIPCHK(ctx, hipMemcpyAsync(ctx->d_status, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
-  k_scan<<<1, SCAN_THREADS, 0, ctx->stream>>>(nrays, ctx->d_nstep_ray, ctx->d_off);
-  HIPCHK(ctx, hipGetLastError());
-  if (total > 0) {
-    unsigned nb = (unsigned)((total + PREP_BLOCK - 1) / PREP_BLOCK);
-    if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)nb))) return rc;
-    k_blockmap<<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(nrays, ctx->d_off, ctx->d_blk);
-    HIPCHK(ctx, hipGetLastError());
-    auto kp = P.transfer_fp32 ? k_prep<true> : k_prep<false>;
-    kp<<<nb, PREP_BLOCK, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, (long long)total,
-                                               ctx->d_off, ctx->d_blk, ctx->d_nstep_ray, ctx->d_traj,
-                                               turns ? ctx->d_nturns : nullptr, ctx->d_dump, ctx->d_pkt);
-    HIPCHK(ctx, hipGetLastError());
-  }
-  HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+  int cap, stride;
+  unsigned long long total;
+  if ((rc = prepare_packets(ctx, P, nrays, false, cap, total, stride))) return rc;
   k_chain<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, ctx->d_off,
                                                    ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, ctx->d_pkt,
                                                    ctx->d_out7, ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
@@ -809,6 +918,7 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
                                                            ctx->d_status);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+  int status[NSTATUS];
   HIPCHK(ctx, hipMemcpyAsync(status, ctx->d_status, sizeof(status), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(ctx->counts, ctx->d_counts, sizeof(ctx->counts), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -949,7 +1059,7 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
   if (!ctx->d_bstatus) HIPCHK(ctx, hipMalloc((void **)&ctx->d_bstatus, 2 * sizeof(unsigned long long)));
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_groups, groups.data(), groups.size() * sizeof(BGroup), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_jobs, bj.data(), bj.size() * sizeof(BJob), hipMemcpyHostToDevice, ctx->stream));
-  const unsigned long long bst_init[2] = {~0ull, ~0ull};
+  static const unsigned long long bst_init[2] = {~0ull, ~0ull};
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_bstatus, bst_init, sizeof(bst_init), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipMemsetAsync(ctx->d_counts, 0, 2 * sizeof(unsigned long long), ctx->stream));
 
@@ -1095,6 +1205,130 @@ int ipgpu_bad_job(const ipgpu_ctx *ctx, int *job, int *i, int *j)
   *job = ctx->bad_job;
   *i = ctx->bad_i;
   *j = ctx->bad_j;
+  return IPGPU_OK;
+}
+
+/* ------------------------------------------------------------------ adjoint */
+
+int ipgpu_scale_density(ipgpu_ctx *ctx, const double *scale)
+{
+  if (!ctx || !scale) return IPGPU_ERR_ARG;
+  if (!ctx->d_dump || ctx->model.kind != IPGPU_MODEL_IHARM) return set_err(ctx, IPGPU_ERR_STATE, "no iharm dump");
+  if (ctx->geom.N3 % 2) return set_err(ctx, IPGPU_ERR_ARG, "per-cell density scale needs an even N3");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const ipgpu_geom &G = ctx->geom;
+  const size_t nin = (size_t)G.N1 * G.N2 * G.N3;
+  int rc;
+  if ((rc = grow(ctx, ctx->d_scale, ctx->cap_scale, nin))) return rc;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_scale, scale, nin * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  k_scale_ne<<<(unsigned)((nin + 255) / 256), 256, 0, ctx->stream>>>(G, ctx->d_scale, ctx->d_dump);
+  int nr = G.N2 * G.N3, np = (G.N1 + 2) * G.N3, nf = (G.N1 + 2) * (G.N2 + 2);
+  k_bc_ne<<<(nr + 255) / 256, 256, 0, ctx->stream>>>(G, 0, ctx->d_dump);
+  k_bc_ne<<<(np + 255) / 256, 256, 0, ctx->stream>>>(G, 1, ctx->d_dump);
+  k_bc_ne<<<(nf + 255) / 256, 256, 0, ctx->stream>>>(G, 2, ctx->d_dump);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return IPGPU_OK;
+}
+
+/* Fold the gradient of ghost cells back onto the interior cells they copy
+ * (the transpose of the ghost-zone rules, applied in reverse order:
+ * azimuth, polar, radial), then extract the interior.  Even N3 (the polar
+ * rule is then a permutation, kflip). */
+static void fold_ghosts(const ipgpu_geom &G, std::vector<double> &g, double *interior)
+{
+  const int N1 = G.N1, N2 = G.N2, N3 = G.N3;
+  auto at = [&](int i, int j, int k) -> double & { return g[((size_t)i * (N2 + 2) + j) * (N3 + 2) + k]; };
+  for (int i = 0; i < N1 + 2; i++)
+    for (int j = 0; j < N2 + 2; j++) {
+      at(i, j, N3) += at(i, j, 0);
+      at(i, j, 1) += at(i, j, N3 + 1);
+      at(i, j, 0) = at(i, j, N3 + 1) = 0.;
+    }
+  for (int i = 0; i < N1 + 2; i++)
+    for (int k = 1; k <= N3; k++) {
+      int kflip = ((k - 1) + (N3 / 2)) % N3 + 1;
+      at(i, 1, kflip) += at(i, 0, k);
+      at(i, N2, kflip) += at(i, N2 + 1, k);
+      at(i, 0, k) = at(i, N2 + 1, k) = 0.;
+    }
+  for (int j = 1; j <= N2; j++)
+    for (int k = 1; k <= N3; k++) {
+      at(1, j, k) += at(0, j, k);
+      at(N1, j, k) += at(N1 + 1, j, k);
+    }
+  for (int i = 0; i < N1; i++)
+    for (int j = 0; j < N2; j++)
+      for (int k = 0; k < N3; k++) interior[((size_t)i * N2 + j) * N3 + k] = at(i + 1, j + 1, k + 1);
+}
+
+int ipgpu_adjoint(ipgpu_ctx *ctx, const ipgpu_image *img, const double *target, const double *weights,
+                  double out4[4], double *grad_density, double *imageS)
+{
+  if (!ctx || !img || !target || !out4) return IPGPU_ERR_ARG;
+  if (!ctx->has_geom || !ctx->has_model || !ctx->d_dump || ctx->model.kind != IPGPU_MODEL_IHARM)
+    return set_err(ctx, IPGPU_ERR_STATE, "the adjoint needs an iharm dump");
+  if (!ctx->has_units) return set_err(ctx, IPGPU_ERR_STATE, "the adjoint needs a dump from ipgpu_upload_prims");
+  const ipgpu_image &P = *img;
+  if (P.nx <= 0 || P.ny <= 0 || P.maxnstep < 2) return set_err(ctx, IPGPU_ERR_ARG, "bad image size or maxnstep");
+  if (P.only_unpolarized || P.transfer_fp32 || P.target_nturns >= 0 || P.counterjet)
+    return set_err(ctx, IPGPU_ERR_ARG, "the adjoint covers the polarized FP64 path without subring or counterjet selection");
+  if (P.emission_type != 4 && P.emission_type != 1) return set_err(ctx, IPGPU_ERR_ARG, "emission_type must be 4 or 1");
+  if (grad_density && ctx->geom.N3 % 2) return set_err(ctx, IPGPU_ERR_ARG, "per-cell density gradient needs an even N3");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int tiles = ((P.nx + TILE - 1) / TILE) * ((P.ny + TILE - 1) / TILE);
+  const int nrays = tiles * WAVE;
+  const size_t npix = (size_t)P.nx * P.ny;
+  const ipgpu_geom &G = ctx->geom;
+  const size_t nc = (size_t)(G.N1 + 2) * (G.N2 + 2) * (G.N3 + 2);
+  int rc;
+  if ((rc = grow(ctx, ctx->d_nstep_ray, ctx->cap_nstep_ray, (size_t)nrays))) return rc;
+  if ((rc = grow(ctx, ctx->d_imageS, ctx->cap_imageS, npix * 5))) return rc;
+  if ((rc = grow(ctx, ctx->d_target, ctx->cap_target, npix * 5))) return rc;
+  if ((rc = grow(ctx, ctx->d_out4, ctx->cap_out4, (size_t)nrays * 4))) return rc;
+  if ((rc = grow(ctx, ctx->d_gcell, ctx->cap_gcell, nc))) return rc;
+  if ((rc = grow(ctx, ctx->d_seg, ctx->cap_seg, (size_t)nrays * CK))) return rc;
+  int cap, stride;
+  unsigned long long total;
+  if ((rc = prepare_packets(ctx, P, nrays, true, cap, total, stride))) return rc;
+  if ((rc = grow(ctx, ctx->d_ckpt, ctx->cap_ckpt, (size_t)(total / CK + nrays + 1)))) return rc;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_target, target, npix * 5 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_gcell, 0, nc * sizeof(double), ctx->stream));
+  AdjW w;
+  for (int q = 0; q < 4; q++) w.w[q] = weights ? weights[q] : 1.;
+  k_adjoint<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(G, P, nrays, ctx->d_off, ctx->d_nstep_ray, ctx->d_pkt, ctx->d_aux,
+                                                     ctx->d_target, w, ctx->d_dump, ctx->d_ckpt, ctx->d_seg,
+                                                     ctx->d_gcell, ctx->d_out4, ctx->d_imageS, ctx->d_status);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+  std::vector<double> o4((size_t)nrays * 4);
+  int status[NSTATUS];
+  HIPCHK(ctx, hipMemcpyAsync(o4.data(), ctx->d_out4, o4.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(status, ctx->d_status, sizeof(status), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[3]);
+  hipEventElapsedTime(&ctx->ms_geo, ctx->ev[0], ctx->ev[1]);
+  hipEventElapsedTime(&ctx->ms_xfer, ctx->ev[1], ctx->ev[3]);
+  if (status[1] != 0x7fffffff) {
+    int p = status[1] - 1;
+    ctx->bad_i = p / P.ny;
+    ctx->bad_j = p % P.ny;
+    snprintf(ctx->err, sizeof(ctx->err), "MaxNStep exceeded in pixel %d %d", ctx->bad_i, ctx->bad_j);
+    return IPGPU_ERR_MAXNSTEP;
+  }
+  double acc[4] = {0., 0., 0., 0.};
+  for (int r = 0; r < nrays; r++)
+    for (int q = 0; q < 4; q++) acc[q] += o4[(size_t)r * 4 + q];
+  out4[0] = acc[0];
+  out4[1] = acc[1] / ctx->units.M_unit;
+  out4[2] = acc[2];
+  out4[3] = acc[3];
+  if (grad_density) {
+    std::vector<double> g(nc);
+    HIPCHK(ctx, hipMemcpy(g.data(), ctx->d_gcell, nc * sizeof(double), hipMemcpyDeviceToHost));
+    fold_ghosts(G, g, grad_density);
+  }
+  if (imageS) HIPCHK(ctx, hipMemcpy(imageS, ctx->d_imageS, npix * 5 * sizeof(double), hipMemcpyDeviceToHost));
   return IPGPU_OK;
 }
 

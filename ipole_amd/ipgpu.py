@@ -96,6 +96,8 @@ def lib():
         L.ipgpu_render_batch.argtypes = [vp, C.c_int, C.POINTER(Job)]
         L.ipgpu_bad_job.argtypes = [vp, ip, ip, ip]
         L.ipgpu_set_batch_memory.argtypes = [vp, C.c_size_t]
+        L.ipgpu_adjoint.argtypes = [vp, C.POINTER(Image), dp, dp, dp, dp, dp]
+        L.ipgpu_scale_density.argtypes = [vp, dp]
         L.ipgpu_kernel_times.argtypes = [vp, dp, dp, dp]
         L.ipgpu_stage_times.argtypes = [vp, dp]
         L.ipgpu_step_counts.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
@@ -332,6 +334,32 @@ class Context:
                             taus=o["taus"].reshape(nx, ny), nstep=o["nstep"].reshape(nx, ny),
                             flag=o["flag"].reshape(nx, ny)))
         return res
+
+    def adjoint(self, image, target_imageS, weights=None, density=True):
+        """Loss L = sum w_s (S_s - target_s)^2 over pixels and I Q U V, and its
+        gradient: dict(loss, dM_unit, dR_high, dR_low, density (N1, N2, N3)
+        dL/d(n_e scale) per interior cell or None, imageS)."""
+        n = image.nx * image.ny
+        tgt = np.ascontiguousarray(target_imageS, dtype=np.float64).reshape(-1)
+        assert tgt.size == 5 * n
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        out4 = np.zeros(4)
+        g = self.geom
+        dens = np.zeros(g.N1 * g.N2 * g.N3) if density else None
+        S = np.zeros(5 * n)
+        rc = self.L.ipgpu_adjoint(self.h, C.byref(image), P(tgt), P(w) if w is not None else None, P(out4),
+                                  P(dens) if dens is not None else None, P(S))
+        if rc == ERR_MAXNSTEP:
+            raise IpgpuError(rc, "MaxNStep exceeded")
+        self._chk(rc)
+        return dict(loss=out4[0], dM_unit=out4[1], dR_high=out4[2], dR_low=out4[3],
+                    density=dens.reshape(g.N1, g.N2, g.N3) if dens is not None else None,
+                    imageS=S.reshape(image.nx, image.ny, 5))
+
+    def scale_density(self, scale):
+        """n_e of each interior cell times scale (N1, N2, N3)."""
+        s = np.ascontiguousarray(scale, dtype=np.float64).reshape(-1)
+        self._chk(self.L.ipgpu_scale_density(self.h, P(s)))
 
     def set_batch_memory(self, nbytes):
         self._chk(self.L.ipgpu_set_batch_memory(self.h, int(nbytes)))
