@@ -158,6 +158,111 @@ def batch_measure(args, h=None, prims=None):
                 mixed_speedup_vs_plain=round(sum(plain_ms) / ms32, 3), mixed_max_dFtot_vs_plain=dF32)
 
 
+def _pol_summary(S):
+    sI, sQ, sU, sV = (float(S[..., k].sum()) for k in range(4))
+    return sI, np.hypot(sQ, sU) / sI, sV / sI, 0.5 * np.arctan2(sU, sQ)
+
+
+def precision_measure(args, oracle_stride=10):
+    """The mixed-precision transfer on the 384x192x192 dump (a = 0.5, seed
+    1), 400x400 px, thetacam 17, 230 GHz: image Ftot, LP, CP and EVPA of the
+    FP32 transfer against the FP64 path, both against the oracle on an
+    evenly strided pixel sample, and the device time of each."""
+    from ipole_amd import ipgpu as G
+    from ipole_amd import synth
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    t0 = time.time()
+    h, prims = synth.make_dump(384, 192, 192, a=0.5, seed=1)
+    log("precision: dump generated in %.1fs" % (time.time() - t0))
+    ctx = G.Context(0)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    ctx.set_geometry(g)
+    ctx.set_model(m)
+    ctx.upload_prims(prims, G.iharm_units(3e26, trat_large=40.))
+    res, ms = {}, {}
+    for fp32 in (0, 1):
+        im, scale, _ = G.make_image(g, m.L_unit, nx=400, ny=400, thetacam=17., freqcgs=230e9, transfer_fp32=fp32)
+        ctx.render(im, fetch=False)
+        t = []
+        for _ in range(2):
+            ctx.render(im, fetch=False)
+            t.append(ctx.kernel_times()[2])
+        res[fp32] = ctx.render(im)
+        ms[fp32] = float(np.mean(t))
+    ctx.close()
+    a, b = _pol_summary(res[0]["imageS"]), _pol_summary(res[1]["imageS"])
+    err = dict(Ftot=abs(b[0] / a[0] - 1), LP=abs(b[1] - a[1]), CP=abs(b[2] - a[2]), EVPA_rad=abs(b[3] - a[3]))
+    # oracle on a pixel sample
+    orc, _ = O.iharm_oracle(h, prims, M_unit=3e26, trat_large=40.)
+    p = O.default_params(nx=400, ny=400, thetacam=17., freqcgs=230e9)
+    cam = orc.camera(p)
+    pix = np.array([(i, j) for i in range(0, 400, oracle_stride) for j in range(0, 400, oracle_stride)], np.int32)
+    nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    t0 = time.time()
+    rc, out7, ns, fl = orc.render_pixels(p, cam, pix, nthreads=nthreads)
+    log("precision: oracle %d pixels in %.1fs" % (len(pix), time.time() - t0))
+    f3 = 230e9 ** 3
+    ref = np.column_stack([out7[:, 1], -out7[:, 2], -out7[:, 3], out7[:, 4]]) * f3
+    den = np.maximum(np.abs(ref[:, 0]), 1e-6 * np.abs(ref[:, 0]).max())
+    px = {}
+    for fp32 in (0, 1):
+        S = res[fp32]["imageS"][pix[:, 0], pix[:, 1], :4]
+        px[fp32] = float((np.abs(S - ref) / den[:, None]).max())
+    return dict(workload="384x192x192 FMKS dump a=0.5 seed 1, 400x400 px, thetacam 17, 230 GHz",
+                fp64_ms_device=round(ms[0], 3), mixed_ms_device=round(ms[1], 3),
+                mixed_speedup=round(ms[0] / ms[1], 3), mixed_vs_fp64=err,
+                oracle_sample_pixels=int(len(pix)), max_pixel_err_vs_oracle_fp64=px[0],
+                max_pixel_err_vs_oracle_mixed=px[1])
+
+
+def adjoint_measure(args):
+    """The adjoint on the 96x64x64 dump (the 160x160 workload's generator),
+    128x128 px, thetacam 163, 230 GHz: loss against a target rendered at
+    perturbed parameters (M_unit x1.2, R_high x0.9, n_e scaled by a seeded
+    lognormal field of 5%), device time of the adjoint against one forward
+    render, and its dL/dM_unit against a central finite difference."""
+    from ipole_amd import ipgpu as G
+    from ipole_amd import synth
+    h, prims = synth.make_dump(96, 64, 64, a=0.9375, seed=1)
+    ctx = G.Context(0)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    ctx.set_geometry(g)
+    ctx.set_model(m)
+    im, _, _ = G.make_image(g, m.L_unit, nx=128, ny=128, thetacam=163., freqcgs=230e9)
+    M, RH = 3e26, 40.
+    rng = np.random.default_rng(3)
+    ctx.upload_prims(prims, G.iharm_units(1.2 * M, trat_large=0.9 * RH))
+    ctx.scale_density(np.exp(0.05 * rng.normal(size=(96, 64, 64))))
+    target = ctx.render(im)["imageS"]
+    ctx.upload_prims(prims, G.iharm_units(M, trat_large=RH))
+    ctx.render(im, fetch=False)
+    t_fwd = []
+    for _ in range(2):
+        ctx.render(im, fetch=False)
+        t_fwd.append(ctx.kernel_times()[2])
+    r = ctx.adjoint(im, target)
+    t_adj = []
+    for _ in range(2):
+        r = ctx.adjoint(im, target)
+        t_adj.append(ctx.kernel_times()[2])
+
+    def loss(Mu):
+        ctx.upload_prims(prims, G.iharm_units(Mu, trat_large=RH))
+        S = ctx.render(im)["imageS"]
+        return float(np.sum((S[..., :4] - target[..., :4]) ** 2))
+    hh = 1e-5
+    fd = (loss(M * (1 + hh)) - loss(M * (1 - hh))) / (2 * hh * M)
+    ctx.close()
+    return dict(workload="96x64x64 FMKS dump seed 1, 128x128 px, thetacam 163, 230 GHz; target at M_unit x1.2, "
+                         "R_high x0.9, 5% lognormal n_e scale",
+                forward_ms_device=round(float(np.mean(t_fwd)), 3), adjoint_ms_device=round(float(np.mean(t_adj)), 3),
+                adjoint_over_forward=round(float(np.mean(t_adj) / np.mean(t_fwd)), 3), loss=r["loss"],
+                dL_dM_unit=r["dM_unit"], dL_dM_unit_fd=fd, rel_err_vs_fd=abs(r["dM_unit"] / fd - 1),
+                dL_dR_high=r["dR_high"], dL_dR_low=r["dR_low"],
+                density_grad_abs_max=float(np.abs(r["density"]).max()))
+
+
 def bench_batch(args):
     """--workload batch: the batched figures as the bench line's value
     (FP64 batched pixels/s)."""
@@ -176,13 +281,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (64-job) figures")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the mixed-precision (400x400) and adjoint (128x128) figures")
     ap.add_argument("--cpu-stride", type=int, default=2,
                     help="CPU baseline sample: every n-th pixel row (all columns)")
     ap.add_argument("--small", action="store_true", help="small dump for quick checks")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain device-to-device copy of MIB MiB (the reference "
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
-    ap.add_argument("--workload", choices=["c1", "batch"], default="c1",
+    ap.add_argument("--workload", choices=["c1", "batch", "extras"], default="c1",
                     help="c1: one 160x160 image (default); batch: the 64-job batch against the "
                          "same jobs rendered one by one")
     args = ap.parse_args()
@@ -204,6 +311,9 @@ def main():
 
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "extras":
+        print(json.dumps(dict(precision=precision_measure(args), adjoint=adjoint_measure(args))), flush=True)
+        return
 
     from ipole_amd import ipgpu as G
     from ipole_amd import synth
@@ -253,9 +363,12 @@ def main():
         (st["Ftot"], st["Ftot_unpol"], st["LP"], st["CP"], n_geo, n_rad))
 
     roof = roofline(np.mean(np.array(ms_stage), axis=0), n_geo)
-    batched = None
+    batched = precision = adjoint = None
     if not args.no_batch and not args.small:
         batched = batch_measure(args, h, prims)
+    if not args.no_extras and not args.small:
+        precision = precision_measure(args)
+        adjoint = adjoint_measure(args)
     cpu = None
     if not args.no_cpu:
         nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
@@ -273,7 +386,8 @@ def main():
                            radiating_steps=n_rad, ms_device=round(float(np.mean(ms_tot)), 4),
                            ms_geodesic=round(float(np.mean(ms_geo)), 4),
                            ms_transfer=round(float(np.mean(ms_xfer)), 4)),
-               roofline=roof, cpu_baseline=cpu, batched=batched)
+               roofline=roof, cpu_baseline=cpu, batched=batched, mixed_precision=precision,
+               adjoint=adjoint)
     print(json.dumps(out), flush=True)
 
 
