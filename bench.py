@@ -53,20 +53,27 @@ def cpu_baseline(h, prims, cfg, sample_stride, nthreads):
                                                        nthreads))
 
 
-def roofline(stage_ms, n_geo):
+def roofline(stage_ms, n_geo, two_phase):
     """Roofline of the dominant kernel (longest stage): algorithmic FLOP per
     launch = per-unit count (op-counting build, ipole_amd/flops_per_step.json)
     x units (geodesic steps = transfer steps per image); traffic = measured
-    HBM bytes per launch (ipole_amd/hbm_traffic.json)."""
+    HBM bytes per launch (ipole_amd/hbm_traffic.json).  In the two-phase
+    render the geodesic pass is two launches (k_trace_budget, then
+    k_trace_resume on the paused rays), timed together; the transfer kernels
+    run beside the second one and are not timed one by one."""
     fl = json.load(open(os.path.join(ROOT, "ipole_amd", "flops_per_step.json")))["pol"]
     tr = json.load(open(os.path.join(ROOT, "ipole_amd", "hbm_traffic.json")))["bytes_per_launch"]
     per_unit = dict(k_trace=fl["F_trace_per_step"], k_prep=fl["F_prep_per_step"],
                     k_chain=fl["F_chain_per_step"])
-    t = dict(k_trace=stage_ms[0], k_prep=stage_ms[1], k_chain=stage_ms[2])
+    if two_phase:
+        t = dict(k_trace=stage_ms[0])
+    else:
+        t = dict(k_trace=stage_ms[0], k_prep=stage_ms[1], k_chain=stage_ms[2])
     ach = {k: per_unit[k] * n_geo / (t[k] * 1e-3) / 1e12 for k in t}
     dom = max(t, key=t.get)
+    name = "k_trace_budget+k_trace_resume" if two_phase and dom == "k_trace" else dom
     return dict(bound="mfma", achieved=round(ach[dom], 4), peak=PEAK_FP64, unit="TFLOP/s",
-                frac=round(ach[dom] / PEAK_FP64, 5), traffic=tr.get(dom), kernel=dom,
+                frac=round(ach[dom] / PEAK_FP64, 5), traffic=tr.get(dom), kernel=name,
                 note="FP64 compute roof (MI355X FP64 peak 78.6 TFLOP/s, which the FP64 vector ALU "
                      "also reaches; these kernels use the vector ALU, no matrix instructions). "
                      "FLOP per unit from the op-counting build (%s: %.0f per step); per-kernel "
@@ -344,9 +351,18 @@ def main():
         ms_geo.append(a)
         ms_xfer.append(b)
         ms_tot.append(c)
-        ms_stage.append(ctx.stage_times())
+        st_, two_phase = ctx.stage_times_mode()
+        ms_stage.append(st_)
     wall = time.perf_counter() - t0
     n_geo, n_rad = ctx.step_counts()
+    # the one-pass render of the same image, for the log
+    ctx.set_tuning(trace_budget=0)
+    one = []
+    for s in range(3):
+        ctx.render(im, fetch=False)
+        one.append(ctx.kernel_times()[2])
+    ctx.set_tuning(trace_budget=4096)
+    log("one-pass render of the same image: %.3f ms" % np.mean(one[1:]))
     res = ctx.render(im)
     st = G.image_stats(res, scale, cfg["freqcgs"], 16.9e6)
     ns_img = res["nstep"]
@@ -354,15 +370,24 @@ def main():
         ns_img.shape[0] // 8, 8, ns_img.shape[1] // 8, 8).max(axis=(1, 3))
     log("nstep: max %d, mean %.1f, p99 %d; sum over 8x8 tiles of the tile max %d" %
         (ns_img.max(), ns_img.mean(), np.percentile(ns_img, 99), tiles.sum()))
+    for T in (1024, 2048, 4096):
+        sel = ns_img > T
+        log("  rays > %d steps: %d, their steps %d (%.1f%%)" %
+            (T, sel.sum(), ns_img[sel].sum(), 100. * ns_img[sel].sum() / ns_img.sum()))
     ctx.close()
     ms_step = 1e3 * wall / args.steps
     st_ms = np.mean(np.array(ms_stage), axis=0)
-    log("per step: wall %.3f ms, device %.3f ms (k_trace %.3f, k_prep %.3f, k_chain %.3f, k_finalize %.3f)" %
-        ((ms_step, np.mean(ms_tot)) + tuple(st_ms)))
+    if two_phase:
+        log("per step: wall %.3f ms, device %.3f ms (two-phase: trace launches %.3f, finished rays' "
+            "transfer on the second stream %.3f, tail after the resumed trace %.3f, k_finalize %.3f)" %
+            ((ms_step, np.mean(ms_tot)) + tuple(st_ms)))
+    else:
+        log("per step: wall %.3f ms, device %.3f ms (k_trace %.3f, k_prep %.3f, k_chain %.3f, k_finalize %.3f)" %
+            ((ms_step, np.mean(ms_tot)) + tuple(st_ms)))
     log("image: Ftot %.6g Jy, unpol %.6g Jy, LP %.3f%%, CP %.3f%%; geodesic steps %d, radiating %d" %
         (st["Ftot"], st["Ftot_unpol"], st["LP"], st["CP"], n_geo, n_rad))
 
-    roof = roofline(np.mean(np.array(ms_stage), axis=0), n_geo)
+    roof = roofline(np.mean(np.array(ms_stage), axis=0), n_geo, two_phase)
     batched = precision = adjoint = None
     if not args.no_batch and not args.small:
         batched = batch_measure(args, h, prims)
@@ -385,7 +410,8 @@ def main():
                            pixels=npix, Ftot_Jy=round(st["Ftot"], 9), geodesic_steps=n_geo,
                            radiating_steps=n_rad, ms_device=round(float(np.mean(ms_tot)), 4),
                            ms_geodesic=round(float(np.mean(ms_geo)), 4),
-                           ms_transfer=round(float(np.mean(ms_xfer)), 4)),
+                           ms_transfer=round(float(np.mean(ms_xfer)), 4),
+                           two_phase=bool(two_phase), ms_one_pass=round(float(np.mean(one[1:])), 4)),
                roofline=roof, cpu_baseline=cpu, batched=batched, mixed_precision=precision,
                adjoint=adjoint)
     print(json.dumps(out), flush=True)

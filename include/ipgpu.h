@@ -197,12 +197,32 @@ int ipgpu_bad_pixel(const ipgpu_ctx *ctx, int *i, int *j);
 int ipgpu_kernel_times(const ipgpu_ctx *ctx, double *ms_geodesic, double *ms_transfer,
                        double *ms_total);
 
-/* Per-stage device time of the last render (HIP events on the context's
- * stream), ms[0..3] = geodesic pass (k_trace), transfer step packets (k_scan
- * + k_prep), transfer recurrence (k_chain), output layout (k_finalize).
+/* Per-stage device time of the last render (HIP events).  Returns 0 after a
+ * one-pass render: ms[0..3] = geodesic pass (k_trace), transfer step packets
+ * (k_scan + k_prep), transfer recurrence (k_seg + k_chain), output layout
+ * (k_finalize).  Returns 1 after a two-phase render: ms[0] = both geodesic
+ * launches (k_trace_budget + k_trace_resume), ms[1] = transfer of the rays
+ * finished in the first pass (second stream, concurrent with the resumed
+ * trace), ms[2] = what follows the resumed trace (transfer of the resumed
+ * rays and the wait for the second stream), ms[3] = k_finalize.
  * Measurement only; the reference prints one wallclock figure
  * (src/main.c:1038-1040). */
 int ipgpu_stage_times(const ipgpu_ctx *ctx, double ms[4]);
+
+/* Tuning of ipgpu_render (no counterpart in the reference, whose pixel loop
+ * has none).  trace_budget: steps of the first geodesic pass of the
+ * two-phase render (default 4096; 0 = one pass; it applies when
+ * target_nturns < 0 and the record capacity exceeds it).  seg_min_steps:
+ * polarized rays longer than this have their transfer composed from
+ * 256-step segments (default 2048).  record_steps: geodesic records held
+ * per ray (default 8192, raised by a render whose longest ray needs more;
+ * that render retraces).  Results do not depend on trace_budget or
+ * record_steps; seg_min_steps reorders the transfer arithmetic of the rays
+ * it selects (agreement to roundoff).  A negative value leaves a setting
+ * unchanged.  The environment variables
+ * IPGPU_TRACE_BUDGET and IPGPU_SEG_MIN_STEPS set the defaults of a new
+ * context. */
+int ipgpu_set_tuning(ipgpu_ctx *ctx, int trace_budget, int seg_min_steps, int record_steps);
 
 /* Work counts of the last render: geodesic steps summed over rays (the
  * nstep of trace_geodesic) and radiating steps of the transfer pass (steps

@@ -64,27 +64,44 @@ IPG_HD void load_rec(const double *rec, int n, Rec &r)
 }
 
 /* Returns nstep.  Records beyond cap are not written (the caller retraces
- * with a larger cap; cap = 0 only counts the steps).  nt (may be null) receives the subring count per
- * step when P.target_nturns >= 0. */
+ * with a larger cap; cap = 0 only counts the steps).  nt (may be null)
+ * receives the subring count per step when P.target_nturns >= 0.
+ * With nend, the integration pauses after nend steps (*finished = 0); with
+ * n0 > 0 it resumes from record n0 (its X1..3 and K are the registers'
+ * values after that step bit for bit; X0 is never read), without nt. */
 IPG_HD int trace_ray(const ipgpu_geom &G, const ipgpu_image &P, int pi, int pj, int cap, double *rec,
-                     int *nt)
+                     int *nt, int n0 = 0, int nend = 0x7fffffff, int *finished = nullptr)
 {
-  double E[4][4];
   Metric m;
-  double X[4] = {P.Xcam[0], P.Xcam[1], P.Xcam[2], P.Xcam[3]};
-  camera_tetrad(G, X, P.old_centering, E, m);
-  double K[4];
-  init_K(P, E, pi, pj, K);
-  if (cap > 0) store_rec(rec, X, K, 0.);
-  if (nt && cap > 0) nt[0] = 0;
+  double X[4], K[4];
+  int nstep = 0;
+  if (n0 > 0) {
+    Rec r;
+    load_rec(rec, n0, r);
+    X[0] = 0.;
+    for (int k = 1; k < 4; k++) X[k] = r.X[k];
+    for (int k = 0; k < 4; k++) K[k] = r.K[k];
+    nstep = n0;
+  } else {
+    double E[4][4];
+    for (int k = 0; k < 4; k++) X[k] = P.Xcam[k];
+    camera_tetrad(G, X, P.old_centering, E, m);
+    init_K(P, E, pi, pj, K);
+    if (cap > 0) store_rec(rec, X, K, 0.);
+    if (nt && cap > 0) nt[0] = 0;
+  }
   int passed = (P.Xcam[2] < 0.5) ? 1 : ((P.Xcam[2] > 0.5) ? -1 : 0);
   int nturns = 0;
   double za = 0., zb = 0., x2a = 0., x2b = 0.;
-  int nstep = 0;
+  int done = 1;
   while (true) {
     metric(G, X[1], X[2], m);
     if (stop_backward(G, m.r, K)) break;
     if (nstep >= P.maxnstep - 1) break;
+    if (nstep >= nend) {
+      done = 0;
+      break;
+    }
     nstep++;
     double dl = stepsize(G, X, K, P.eps);
     double dlp = -dl;
@@ -121,6 +138,7 @@ IPG_HD int trace_ray(const ipgpu_geom &G, const ipgpu_image &P, int pi, int pj, 
       if (nstep < cap) nt[nstep] = nturns;
     }
   }
+  if (finished) *finished = done;
   return nstep;
 }
 
@@ -460,6 +478,149 @@ IPG_HD int chain_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const do
   }
   nrad = st.nrad;
   return chain_finish(G, P, st, out7);
+}
+
+/* ------------------------------------------- long rays in segments
+ *
+ * The polarized recurrence is affine in the 16 real parameters of N
+ * (d[4], re[6], im[6]): each step maps N -> A_q N + b_q (transport, then the
+ * plasma-frame Stokes step; the transport of N = 0 is 0, so transporting
+ * unconditionally is the same map).  A long ray's steps are cut into
+ * segments of SEG_LEN whose composed maps (16 columns from the basis states
+ * without sources, one from N = 0 with sources) are formed in parallel, then
+ * applied in order from the far end: the same result up to the roundoff of
+ * the regrouping. */
+constexpr int SEG_LEN = 256;
+constexpr int SEG_MIN_STEPS = 2048;  /* rays longer than this use segments */
+constexpr int NPAR = 16;
+constexpr int SEGCOL = NPAR + 1;
+
+IPG_HD double &herm_par(Herm &N, int b)
+{
+  return b < 4 ? N.d[b] : (b < 10 ? N.re[b - 4] : N.im[b - 10]);
+}
+
+IPG_HD double herm_par(const Herm &N, int b)
+{
+  return b < 4 ? N.d[b] : (b < 10 ? N.re[b - 4] : N.im[b - 10]);
+}
+
+/* Column b of the map of steps [q0, q1): applied to basis state b (b < 16,
+ * no sources) or to N = 0 with sources (b = 16).  Returns the steps'
+ * tetrad oddflag bits. */
+IPG_HD int segment_column(const double *pkt, int q0, int q1, int b, Herm &N)
+{
+  /* the basis state with constant indices only (a runtime index into N
+   * would put it in scratch memory) */
+#pragma unroll
+  for (int q = 0; q < 4; q++) N.d[q] = (b == q) ? 1. : 0.;
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    N.re[q] = (b == 4 + q) ? 1. : 0.;
+    N.im[q] = (b == 10 + q) ? 1. : 0.;
+  }
+  const bool src = (b == NPAR);
+  int flag = 0;
+  for (int q = q0; q < q1; q++) {
+    const double *pk = pkt + (size_t)q * PK;
+    if (q > 0) {
+      const double *pp = pkt + (size_t)(q - 1) * PK;
+      double Mi[4][4], Mh[4][4];
+      for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 4; k++) {
+          Mi[i][k] = pp[PK_MF + 4 * i + k];
+          Mh[i][k] = pk[PK_MH + 4 * i + k];
+        }
+      Herm Nh;
+      transport(N, N, Mi, pk[PK_HDL], Nh);
+      transport(N, Nh, Mh, pk[PK_FDL], N);
+    }
+    const int meta = (int)pk[PK_META];
+    if (meta & 1) {
+      double S0[4], dS[4];
+      herm_to_stokes(N, pk + PK_e1, pk + PK_e2, S0);
+      for (int i = 0; i < 4; i++)
+        dS[i] = (src ? pk[PK_CV + i] : 0.) + pk[PK_T + 4 * i] * S0[0] + pk[PK_T + 4 * i + 1] * S0[1]
+            + pk[PK_T + 4 * i + 2] * S0[2] + pk[PK_T + 4 * i + 3] * S0[3] - S0[i];
+      herm_add_stokes(N, pk + PK_E1, pk + PK_E2, dS);
+      flag |= meta >> 2;
+    }
+  }
+  return flag;
+}
+
+/* A long ray from its segment maps (segmap: SEGCOL x NPAR doubles per
+ * segment, column-major; segflag: the segments' oddflag bits), with the
+ * unpolarized I, tau and tauF along its packets, then project_N. */
+IPG_HD int chain_ray_segmented(const ipgpu_geom &G, const ipgpu_image &P, int ns, const double *pkt,
+                               const double *segmap, const int *segflag, double js, double ks,
+                               double out7[7], int &nrad)
+{
+  Herm N;
+  for (int q = 0; q < 4; q++) N.d[q] = 0.;
+  for (int q = 0; q < 6; q++) N.re[q] = N.im[q] = 0.;
+  int flag = 0;
+  const int nseg = (ns + SEG_LEN - 1) / SEG_LEN;
+  for (int sgi = 0; sgi < nseg; sgi++) {
+    const double *A = segmap + (size_t)sgi * SEGCOL * NPAR;
+    double x[NPAR];
+#pragma unroll
+    for (int b = 0; b < NPAR; b++) x[b] = herm_par(N, b);
+#pragma unroll
+    for (int p = 0; p < NPAR; p++) {
+      double acc = A[NPAR * NPAR + p];
+#pragma unroll
+      for (int b = 0; b < NPAR; b++) acc += A[b * NPAR + p] * x[b];
+      herm_par(N, p) = acc;
+    }
+    flag |= segflag[sgi];
+  }
+  double I = 0., tau = 0., tauF = 0.;
+  nrad = 0;
+  /* the unpolarized recurrence, its packet fields loaded 8 steps at a time
+   * so that the loads of a group are in flight together */
+  constexpr int GRP = 8;
+  for (int q0 = 0; q0 < ns; q0 += GRP) {
+    double mt[GRP], dl[GRP], ju[GRP], ku[GRP], dtf[GRP];
+#pragma unroll
+    for (int k = 0; k < GRP; k++) {
+      const double *pk = pkt + (size_t)(q0 + k < ns ? q0 + k : ns - 1) * PK;
+      mt[k] = pk[PK_META];
+      dl[k] = pk[PK_DL];
+      ju[k] = pk[PK_JU];
+      ku[k] = pk[PK_KU];
+      dtf[k] = pk[PK_DTF];
+    }
+#pragma unroll
+    for (int k = 0; k < GRP; k++) {
+      if (q0 + k >= ns) break;
+      const int meta = (int)mt[k];
+      if (meta & 1) {
+        nrad++;
+        double ji = (meta & 2) ? 0. : js, ki = ks;
+        js = ju[k];
+        ks = ku[k];
+        I = approximate_solve(I, ji, ki, (meta & 2) ? 0. : ju[k], ku[k], dl[k], tau);
+        tauF += dtf[k];
+      }
+    }
+  }
+  if (tauF > 1.e100 || tauF < -1.e100 || isnan(tauF)) flag |= 2;
+  if (isnan(N.d[0])) flag |= 4;
+  double S[4];
+  double E[4][4];
+  Metric m;
+  const double Xc[4] = {P.Xcam[0], P.Xcam[1], P.Xcam[2], P.Xcam[3]};
+  camera_tetrad(G, Xc, 0, E, m);
+  project(N, E, m, P.rotcam, S);
+  out7[0] = I;
+  out7[1] = S[0];
+  out7[2] = S[1];
+  out7[3] = S[2];
+  out7[4] = S[3];
+  out7[5] = tau;
+  out7[6] = tauF;
+  return flag;
 }
 
 /* One transfer step fused: the work of prep_step and chain_step in the

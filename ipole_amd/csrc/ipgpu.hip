@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -139,6 +140,27 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep(ipgpu_geom G, ipgpu_model M
                  aux ? aux + (size_t)g * AX : nullptr);
 }
 
+/* One lane per (segment, column): the composed maps of the long rays'
+ * segments.  seg_ray/seg_q0: ray and first step of each segment. */
+__global__ __launch_bounds__(WAVE) void k_seg(int nseg, const int *__restrict__ seg_ray,
+                                              const int *__restrict__ seg_q0, const long long *__restrict__ off,
+                                              const int *__restrict__ nstep_in, const double *__restrict__ pkt,
+                                              double *__restrict__ segmap, int *__restrict__ segflag)
+{
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nseg * SEGCOL) return;
+  const int sgi = (int)(t / SEGCOL), b = (int)(t % SEGCOL);
+  const int ray = seg_ray[sgi], q0 = seg_q0[sgi];
+  const int ns = nstep_in[ray];
+  const int q1 = q0 + SEG_LEN < ns ? q0 + SEG_LEN : ns;
+  Herm N;
+  int fl = segment_column(pkt + (size_t)off[ray] * PK, q0, q1, b, N);
+  double *o = segmap + ((size_t)sgi * SEGCOL + b) * NPAR;
+#pragma unroll
+  for (int p = 0; p < NPAR; p++) o[p] = herm_par(N, p);
+  if (b == NPAR) segflag[sgi] = fl;
+}
+
 /* One lane per ray: the recurrence of N and I over the ray's packets. */
 __global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P, int nrays,
                                                 int cap, int stride, const long long *__restrict__ off,
@@ -146,6 +168,9 @@ __global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ip
                                                 const double *__restrict__ traj,
                                                 const double *__restrict__ dump,
                                                 const double *__restrict__ pkt,
+                                                const int *__restrict__ seg_base,
+                                                const double *__restrict__ segmap,
+                                                const int *__restrict__ segflag,
                                                 double *__restrict__ out7, int *__restrict__ flags,
                                                 int *__restrict__ status,
                                                 unsigned long long *__restrict__ counts)
@@ -169,7 +194,12 @@ __global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ip
   else far_end_coefficients<false>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
   double res[7];
   int nrad;
-  flags[ray] = chain_ray(G, P, ns, pkt + (size_t)off[ray] * stride, stride, js, ks, res, nrad);
+  const int sb = seg_base ? seg_base[ray] : -1;
+  if (sb >= 0)
+    flags[ray] = chain_ray_segmented(G, P, ns, pkt + (size_t)off[ray] * stride, segmap + (size_t)sb * SEGCOL * NPAR,
+                                     segflag + sb, js, ks, res, nrad);
+  else
+    flags[ray] = chain_ray(G, P, ns, pkt + (size_t)off[ray] * stride, stride, js, ks, res, nrad);
   for (int q = 0; q < 7; q++) o[q] = res[q];
   atomicAdd(&counts[1], (unsigned long long)nrad);
 }
@@ -555,14 +585,151 @@ __global__ void k_bc_ne(ipgpu_geom G, int stage, double *dump)
   }
 }
 
+/* ------------------------------------------------ two-phase single image */
+
+/* Phase 1: every ray with a step budget; live[ray] = 1 where it paused. */
+__global__ __launch_bounds__(WAVE) void k_trace_budget(ipgpu_geom G, ipgpu_image P, int nrays, int cap, int nend,
+                                                       double *__restrict__ traj, int *__restrict__ nstep_out,
+                                                       int *__restrict__ live, unsigned long long *__restrict__ counts)
+{
+  int ray = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ray >= nrays) return;
+  int pi, pj;
+  ray_pixel(ray, P.nx, P.ny, pi, pj);
+  if (pi >= P.nx || pj >= P.ny) {
+    nstep_out[ray] = 0;
+    live[ray] = 0;
+    return;
+  }
+  int fin = 1;
+  int nstep = trace_ray(G, P, pi, pj, cap, traj + (size_t)ray * cap * REC, nullptr, 0, nend, &fin);
+  nstep_out[ray] = nstep;
+  live[ray] = fin ? 0 : 1;
+  atomicAdd(&counts[0], (unsigned long long)nstep);
+}
+
+/* Phase 2: the paused rays, resumed from their last record. */
+__global__ __launch_bounds__(WAVE) void k_trace_resume(ipgpu_geom G, ipgpu_image P, const int *__restrict__ list,
+                                                       int n, int cap, double *__restrict__ traj,
+                                                       int *__restrict__ nstep_io, int *__restrict__ status,
+                                                       unsigned long long *__restrict__ counts)
+{
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int ray = list[t];
+  const int n0 = nstep_io[ray];
+  int nstep = trace_ray(G, P, 0, 0, cap, traj + (size_t)ray * cap * REC, nullptr, n0);
+  nstep_io[ray] = nstep;
+  if (nstep >= cap) atomicAdd(&status[0], 1);
+  atomicAdd(&counts[0], (unsigned long long)(nstep - n0));
+}
+
+/* Block -> list element of a packet range starting at g0 (offL monotonic). */
+__global__ void k_blockmap_list(int n, const long long *__restrict__ offL, long long g0, int *__restrict__ blk)
+{
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  long long b0 = (offL[k] - g0 + PREP_BLOCK - 1) / PREP_BLOCK, b1 = (offL[k + 1] - g0 + PREP_BLOCK - 1) / PREP_BLOCK;
+  for (long long b = b0; b < b1; b++) blk[b] = k;
+}
+
+/* Step packets of the rays of a list (packets offL[k] .. offL[k+1] of ray
+ * list[k]), one lane per step. */
+template <bool F32>
+__global__ __launch_bounds__(PREP_BLOCK) void k_prep_list(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
+                                                          const int *__restrict__ list, int n,
+                                                          const long long *__restrict__ offL, long long g0,
+                                                          long long total, const int *__restrict__ blk, int cap,
+                                                          int stride, const int *__restrict__ nstep_in,
+                                                          const double *__restrict__ traj,
+                                                          const double *__restrict__ dump, double *__restrict__ pkt)
+{
+  long long g = g0 + (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
+  if (g >= g0 + total) return;
+  int k = blk[blockIdx.x];
+  while (k + 1 < n && offL[k + 1] <= g) k++;
+  const int ray = list[k];
+  const int ns = nstep_in[ray];
+  if (ns >= P.maxnstep - 1) return;
+  const int q = (int)(g - offL[k]);
+  prep_step<F32>(G, Mo, P, ns - q, traj + (size_t)ray * cap * REC, nullptr, dump, pkt + (size_t)g * stride,
+                 stride == PK, unit_scale());
+}
+
+/* The chains of the rays of a list, one lane per ray (segmented where
+ * seg_base[k] >= 0). */
+__global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
+                                                     const int *__restrict__ list, int n,
+                                                     const long long *__restrict__ offL, int cap, int stride,
+                                                     const int *__restrict__ nstep_in,
+                                                     const double *__restrict__ traj,
+                                                     const double *__restrict__ dump, const double *__restrict__ pkt,
+                                                     const int *__restrict__ seg_base,
+                                                     const double *__restrict__ segmap,
+                                                     const int *__restrict__ segflag, double *__restrict__ out7,
+                                                     int *__restrict__ flags, int *__restrict__ status,
+                                                     unsigned long long *__restrict__ counts)
+{
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int ray = list[k];
+  int pi, pj;
+  ray_pixel(ray, P.nx, P.ny, pi, pj);
+  if (pi >= P.nx || pj >= P.ny) return;
+  const int ns = nstep_in[ray];
+  double *o = out7 + (size_t)ray * 7;
+  if (ns >= P.maxnstep - 1) {
+    atomicMin(&status[1], pi * P.ny + pj + 1);
+    for (int q = 0; q < 7; q++) o[q] = 0.;
+    flags[ray] = 0;
+    return;
+  }
+  const double *rec = traj + (size_t)ray * cap * REC;
+  double js, ks;
+  if (P.transfer_fp32) far_end_coefficients<true>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
+  else far_end_coefficients<false>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
+  double res[7];
+  int nrad;
+  const int sb = seg_base ? seg_base[k] : -1;
+  if (sb >= 0)
+    flags[ray] = chain_ray_segmented(G, P, ns, pkt + (size_t)offL[k] * stride, segmap + (size_t)sb * SEGCOL * NPAR,
+                                     segflag + sb, js, ks, res, nrad);
+  else
+    flags[ray] = chain_ray(G, P, ns, pkt + (size_t)offL[k] * stride, stride, js, ks, res, nrad);
+  for (int q = 0; q < 7; q++) o[q] = res[q];
+  atomicAdd(&counts[1], (unsigned long long)nrad);
+}
+
+/* One lane per (segment, column), segments given by their packet offset. */
+__global__ __launch_bounds__(WAVE) void k_seg_off(int nseg, const long long *__restrict__ seg_off,
+                                                  const int *__restrict__ seg_q0, const int *__restrict__ seg_ns,
+                                                  const double *__restrict__ pkt, double *__restrict__ segmap,
+                                                  int *__restrict__ segflag)
+{
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nseg * SEGCOL) return;
+  const int sgi = (int)(t / SEGCOL), b = (int)(t % SEGCOL);
+  const int q0 = seg_q0[sgi], ns = seg_ns[sgi];
+  const int q1 = q0 + SEG_LEN < ns ? q0 + SEG_LEN : ns;
+  Herm N;
+  int fl = segment_column(pkt + (size_t)seg_off[sgi] * PK, q0, q1, b, N);
+  double *o = segmap + ((size_t)sgi * SEGCOL + b) * NPAR;
+#pragma unroll
+  for (int p = 0; p < NPAR; p++) o[p] = herm_par(N, p);
+  if (b == NPAR) segflag[sgi] = fl;
+}
+
 }  // namespace
 
 /* ------------------------------------------------------------------ C-ABI */
 
+constexpr int NEV = 10;
+
 struct ipgpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipStream_t stream2 = nullptr; /* finished rays of a two-phase render */
+  hipEvent_t ev[NEV] = {};
   char err[512] = {0};
   ipgpu_geom geom{};
   bool has_geom = false;
@@ -582,6 +749,9 @@ struct ipgpu_ctx {
   size_t cap_off = 0;
   int *d_blk = nullptr;
   size_t cap_blk = 0;
+  int *d_segbase = nullptr, *d_segray = nullptr, *d_segq0 = nullptr, *d_segflag = nullptr;
+  double *d_segmap = nullptr;
+  size_t cap_segbase = 0, cap_segray = 0, cap_segq0 = 0, cap_segflag = 0, cap_segmap = 0;
   int *d_nstep_ray = nullptr, *d_flag_ray = nullptr;
   double *d_out7 = nullptr;
   double *d_image = nullptr, *d_imageS = nullptr, *d_taus = nullptr;
@@ -595,6 +765,29 @@ struct ipgpu_ctx {
   float ms_geo = 0, ms_xfer = 0, ms_total = 0, ms_stage[4] = {0, 0, 0, 0};
   int bad_i = -1, bad_j = -1, bad_job = -1;
   int traj_cap_steps = 8192;
+  /* two-phase render: step budget of the first trace pass (0: one pass) and
+   * the ray length above which a chain is composed from segments */
+  int trace_budget = 4096;
+  int seg_min_steps = SEG_MIN_STEPS;
+  int *d_live = nullptr, *d_list = nullptr, *d_blk2 = nullptr;
+  long long *d_offl = nullptr;
+  size_t cap_live = 0, cap_list = 0, cap_blk2 = 0, cap_offl = 0;
+  double *d_pkt2 = nullptr;
+  size_t pkt2_doubles = 0;
+  struct SegSet {
+    long long *d_off = nullptr;
+    int *d_q0 = nullptr, *d_ns = nullptr, *d_flag = nullptr;
+    double *d_map = nullptr;
+    size_t cap_off = 0, cap_q0 = 0, cap_ns = 0, cap_flag = 0, cap_map = 0;
+    std::vector<long long> off;
+    std::vector<int> q0, ns;
+  } segset[2];
+  /* host copies read by pending asynchronous copies: they live until the
+   * render that filled them has synchronised */
+  std::vector<int> h_ns, h_ns2, h_live, h_list, h_base;
+  std::vector<long long> h_off;
+  float ms_phase[4] = {0, 0, 0, 0};
+  int two_phase = 0;
   /* batches */
   BGroup *d_groups = nullptr;
   size_t cap_groups = 0;
@@ -658,11 +851,17 @@ int ipgpu_create(ipgpu_ctx **out, int device)
     delete ctx;
     return IPGPU_ERR_HIP;
   }
-  for (int q = 0; q < 5; q++)
+  if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return IPGPU_ERR_HIP;
+  }
+  for (int q = 0; q < NEV; q++)
     if (hipEventCreate(&ctx->ev[q]) != hipSuccess) {
       delete ctx;
       return IPGPU_ERR_HIP;
     }
+  if (const char *e = getenv("IPGPU_TRACE_BUDGET")) ctx->trace_budget = atoi(e);
+  if (const char *e = getenv("IPGPU_SEG_MIN_STEPS")) ctx->seg_min_steps = atoi(e);
   if (hipMalloc((void **)&ctx->d_status, NSTATUS * sizeof(int)) != hipSuccess
       || hipMalloc((void **)&ctx->d_counts, 2 * sizeof(unsigned long long)) != hipSuccess) {
     delete ctx;
@@ -677,7 +876,14 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
   if (!ctx) return;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
-  void *ptrs[] = {ctx->d_dump, ctx->d_traj, ctx->d_pkt, ctx->d_aux, ctx->d_off, ctx->d_blk, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
+  if (ctx->stream2) hipStreamSynchronize(ctx->stream2);
+  for (auto &S : ctx->segset) {
+    void *sp[] = {S.d_off, S.d_q0, S.d_ns, S.d_flag, S.d_map};
+    for (void *p : sp)
+      if (p) hipFree(p);
+  }
+  void *ptrs[] = {ctx->d_live, ctx->d_list, ctx->d_blk2, ctx->d_offl, ctx->d_pkt2, ctx->d_dump, ctx->d_traj, ctx->d_pkt, ctx->d_aux, ctx->d_off, ctx->d_blk, ctx->d_segbase, ctx->d_segray,
+                  ctx->d_segq0, ctx->d_segflag, ctx->d_segmap, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
                   ctx->d_out7, ctx->d_image, ctx->d_imageS, ctx->d_taus, ctx->d_nstep_pix,
                   ctx->d_flag_pix, ctx->d_status, ctx->d_counts,
                   ctx->d_groups, ctx->d_jobs, ctx->d_items, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
@@ -685,9 +891,10 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
                   ctx->d_out4, ctx->d_target, ctx->d_scale};
   for (void *p : ptrs)
     if (p) hipFree(p);
-  for (int q = 0; q < 5; q++)
+  for (int q = 0; q < NEV; q++)
     if (ctx->ev[q]) hipEventDestroy(ctx->ev[q]);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
+  if (ctx->stream2) hipStreamDestroy(ctx->stream2);
   delete ctx;
 }
 
@@ -878,6 +1085,191 @@ static int prepare_packets(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, bool
   return 0;
 }
 
+/* Segment tables of the rays of a list whose chains are composed (rays
+ * longer than seg_min_steps); base[k] = first segment of element k or -1. */
+static void build_segments(ipgpu_ctx::SegSet &S, const int *list, int n, const long long *off,
+                           const std::vector<int> &ns, int seg_min, int maxnstep, int *base)
+{
+  S.off.clear();
+  S.q0.clear();
+  S.ns.clear();
+  for (int k = 0; k < n; k++) {
+    const int v = ns[list[k]];
+    base[k] = -1;
+    if (v > seg_min && v < maxnstep - 1) {
+      base[k] = (int)S.off.size();
+      for (int q0 = 0; q0 < v; q0 += SEG_LEN) {
+        S.off.push_back(off[k]);
+        S.q0.push_back(q0);
+        S.ns.push_back(v);
+      }
+    }
+  }
+}
+
+static int upload_segments(ipgpu_ctx *ctx, ipgpu_ctx::SegSet &S, hipStream_t st)
+{
+  const size_t nseg = S.off.size();
+  if (nseg == 0) return 0;
+  int rc;
+  if ((rc = grow(ctx, S.d_off, S.cap_off, nseg)) || (rc = grow(ctx, S.d_q0, S.cap_q0, nseg))
+      || (rc = grow(ctx, S.d_ns, S.cap_ns, nseg)) || (rc = grow(ctx, S.d_flag, S.cap_flag, nseg))
+      || (rc = grow(ctx, S.d_map, S.cap_map, nseg * SEGCOL * NPAR)))
+    return rc;
+  HIPCHK(ctx, hipMemcpyAsync(S.d_off, S.off.data(), nseg * sizeof(long long), hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(S.d_q0, S.q0.data(), nseg * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(S.d_ns, S.ns.data(), nseg * sizeof(int), hipMemcpyHostToDevice, st));
+  return 0;
+}
+
+/* Step packets, segment maps and chains of the n rays of a list (device
+ * list dl, offsets doff, host offsets hoff), all on stream st. */
+static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, int n, const int *dl,
+                         const long long *doff, long long total, int *&blk, size_t &cap_blk, double *pkt,
+                         const int *dbase, ipgpu_ctx::SegSet &S, int cap, int stride)
+{
+  if (n <= 0) return 0;
+  int rc;
+  if (total > 0) {
+    unsigned nb = (unsigned)((total + PREP_BLOCK - 1) / PREP_BLOCK);
+    if ((rc = grow(ctx, blk, cap_blk, (size_t)nb))) return rc;
+    k_blockmap_list<<<(n + 255) / 256, 256, 0, st>>>(n, doff, 0, blk);
+    HIPCHK(ctx, hipGetLastError());
+    auto kp = P.transfer_fp32 ? k_prep_list<true> : k_prep_list<false>;
+    kp<<<nb, PREP_BLOCK, 0, st>>>(ctx->geom, ctx->model, P, dl, n, doff, 0, total, blk, cap, stride,
+                                  ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt);
+    HIPCHK(ctx, hipGetLastError());
+  }
+  const int nseg = (int)S.off.size();
+  if (nseg > 0) {
+    long long lanes = (long long)nseg * SEGCOL;
+    k_seg_off<<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, st>>>(nseg, S.d_off, S.d_q0, S.d_ns, pkt,
+                                                                     S.d_map, S.d_flag);
+    HIPCHK(ctx, hipGetLastError());
+  }
+  k_chain_list<<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, dl, n, doff, cap, stride,
+                                                      ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt,
+                                                      nseg > 0 ? dbase : nullptr, S.d_map, S.d_flag,
+                                                      ctx->d_out7, ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
+  HIPCHK(ctx, hipGetLastError());
+  return 0;
+}
+
+/* Two-phase render of one image.  Phase 1 traces every ray for at most
+ * trace_budget steps.  The rays that finished are prepared and chained on
+ * stream2 while the paused rays resume on the main stream; those are
+ * prepared, segmented and chained once their traces end, and the streams
+ * join before k_finalize.  The arithmetic per ray is that of the one-pass
+ * render (a resumed trace continues from its stored record, which holds X
+ * and K exactly).  Sets fallback when a resumed ray outgrew the record
+ * capacity; the capacity is then raised for a one-pass render. */
+static int render_two_phase(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int cap, bool &fallback)
+{
+  int rc;
+  fallback = false;
+  const int stride = P.only_unpolarized ? PK_UNPOL : PK;
+  const bool segm = !P.only_unpolarized && !P.stokes_floors;
+  const int seg_min = segm ? ctx->seg_min_steps : 0x7fffffff;
+  hipStream_t sA = ctx->stream, sB = ctx->stream2;
+  hipEvent_t *ev = ctx->ev;
+  if ((rc = grow(ctx, ctx->d_traj, ctx->traj_doubles, (size_t)nrays * cap * REC))) return rc;
+  if ((rc = grow(ctx, ctx->d_live, ctx->cap_live, (size_t)nrays))) return rc;
+  if ((rc = grow(ctx, ctx->d_list, ctx->cap_list, (size_t)nrays))) return rc;
+  if ((rc = grow(ctx, ctx->d_offl, ctx->cap_offl, (size_t)nrays + 2))) return rc;
+  if ((rc = grow(ctx, ctx->d_segbase, ctx->cap_segbase, (size_t)nrays))) return rc;
+  static const int init[NSTATUS] = {0, 0x7fffffff, 0x7fffffff, 0};
+
+  /* phase 1 */
+  HIPCHK(ctx, hipEventRecord(ev[0], sA));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_status, init, sizeof(init), hipMemcpyHostToDevice, sA));
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_counts, 0, 2 * sizeof(unsigned long long), sA));
+  k_trace_budget<<<nrays / WAVE, WAVE, 0, sA>>>(ctx->geom, P, nrays, cap, ctx->trace_budget, ctx->d_traj,
+                                                 ctx->d_nstep_ray, ctx->d_live, ctx->d_counts);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ev[1], sA));
+  std::vector<int> &ns = ctx->h_ns, &live = ctx->h_live, &list = ctx->h_list, &base = ctx->h_base;
+  std::vector<long long> &off = ctx->h_off;
+  ns.resize(nrays);
+  live.resize(nrays);
+  HIPCHK(ctx, hipMemcpyAsync(ns.data(), ctx->d_nstep_ray, nrays * sizeof(int), hipMemcpyDeviceToHost, sA));
+  HIPCHK(ctx, hipMemcpyAsync(live.data(), ctx->d_live, nrays * sizeof(int), hipMemcpyDeviceToHost, sA));
+  HIPCHK(ctx, hipStreamSynchronize(sA));
+
+  /* lists: finished rays [0, nF), paused rays [nF, nrays); packet offsets
+   * of each list from 0 (finished: d_pkt, paused: d_pkt2) */
+  list.clear();
+  for (int r = 0; r < nrays; r++)
+    if (!live[r]) list.push_back(r);
+  const int nF = (int)list.size(), nL = nrays - nF;
+  for (int r = 0; r < nrays; r++)
+    if (live[r]) list.push_back(r);
+  off.assign((size_t)nrays + 2, 0);
+  for (int k = 0; k < nF; k++) off[k + 1] = off[k] + ns[list[k]];
+  const long long T1 = off[nF];
+  base.assign(nrays, -1);
+  build_segments(ctx->segset[0], list.data(), nF, off.data(), ns, seg_min, P.maxnstep, base.data());
+  if ((rc = grow(ctx, ctx->d_pkt, ctx->pkt_doubles, (size_t)(T1 > 0 ? T1 : 1) * stride))) return rc;
+
+  /* phase 2 (main stream): the paused rays resume */
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_list, list.data(), nrays * sizeof(int), hipMemcpyHostToDevice, sA));
+  HIPCHK(ctx, hipEventRecord(ev[2], sA));
+  if (nL > 0) {
+    k_trace_resume<<<(nL + WAVE - 1) / WAVE, WAVE, 0, sA>>>(ctx->geom, P, ctx->d_list + nF, nL, cap, ctx->d_traj,
+                                                            ctx->d_nstep_ray, ctx->d_status, ctx->d_counts);
+    HIPCHK(ctx, hipGetLastError());
+  }
+  HIPCHK(ctx, hipEventRecord(ev[3], sA));
+
+  /* the finished rays (second stream), after the list upload */
+  HIPCHK(ctx, hipStreamWaitEvent(sB, ev[2], 0));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_offl, off.data(), (nF + 1) * sizeof(long long), hipMemcpyHostToDevice, sB));
+  if (nF > 0)
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_segbase, base.data(), nF * sizeof(int), hipMemcpyHostToDevice, sB));
+  if ((rc = upload_segments(ctx, ctx->segset[0], sB))) return rc;
+  HIPCHK(ctx, hipEventRecord(ev[8], sB));
+  if ((rc = transfer_list(ctx, P, sB, nF, ctx->d_list, ctx->d_offl, T1, ctx->d_blk, ctx->cap_blk, ctx->d_pkt,
+                          ctx->d_segbase, ctx->segset[0], cap, stride)))
+    return rc;
+  HIPCHK(ctx, hipEventRecord(ev[5], sB));
+
+  /* the paused rays once traced */
+  int st0 = 0;
+  std::vector<int> &ns2 = ctx->h_ns2;
+  ns2.resize(nrays);
+  if (nL > 0) {
+    HIPCHK(ctx, hipMemcpyAsync(ns2.data(), ctx->d_nstep_ray, nrays * sizeof(int), hipMemcpyDeviceToHost, sA));
+    HIPCHK(ctx, hipMemcpyAsync(&st0, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, sA));
+    HIPCHK(ctx, hipStreamSynchronize(sA));
+  }
+  if (st0 > 0) {
+    int mx = 0;
+    for (int k = nF; k < nrays; k++) mx = ns2[list[k]] > mx ? ns2[list[k]] : mx;
+    ctx->traj_cap_steps = mx + 1 < P.maxnstep ? mx + 1 : P.maxnstep;
+    HIPCHK(ctx, hipStreamSynchronize(sB));
+    fallback = true;
+    return 0;
+  }
+  long long *offL = off.data() + nF + 1;
+  for (int k = 0; k < nL; k++) offL[k + 1] = offL[k] + ns2[list[nF + k]];
+  const long long T2 = offL[nL];
+  build_segments(ctx->segset[1], list.data() + nF, nL, offL, ns2, seg_min, P.maxnstep, base.data() + nF);
+  if (nL > 0) {
+    if ((rc = grow(ctx, ctx->d_pkt2, ctx->pkt2_doubles, (size_t)(T2 > 0 ? T2 : 1) * stride))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_offl + nF + 1, offL, (nL + 1) * sizeof(long long), hipMemcpyHostToDevice, sA));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_segbase + nF, base.data() + nF, nL * sizeof(int), hipMemcpyHostToDevice, sA));
+    if ((rc = upload_segments(ctx, ctx->segset[1], sA))) return rc;
+    if ((rc = transfer_list(ctx, P, sA, nL, ctx->d_list + nF, ctx->d_offl + nF + 1, T2, ctx->d_blk2, ctx->cap_blk2,
+                            ctx->d_pkt2, ctx->d_segbase + nF, ctx->segset[1], cap, stride)))
+      return rc;
+  } else {
+    ctx->segset[1].off.clear();
+  }
+  HIPCHK(ctx, hipEventRecord(ev[4], sA));
+  HIPCHK(ctx, hipStreamWaitEvent(sA, ev[5], 0));
+  HIPCHK(ctx, hipEventRecord(ev[6], sA));
+  return 0;
+}
+
 int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *imageS,
                  double *taus, int *nstep, int *flag)
 {
@@ -903,29 +1295,91 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
   if ((rc = grow(ctx, ctx->d_flag_pix, ctx->cap_flag_pix, npix))) return rc;
   if ((rc = grow(ctx, ctx->d_imageS, ctx->cap_imageS, npix * 5))) return rc;
 
-  int cap, stride;
-  unsigned long long total;
-  if ((rc = prepare_packets(ctx, P, nrays, false, cap, total, stride))) return rc;
-  k_chain<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, ctx->d_off,
-                                                   ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, ctx->d_pkt,
-                                                   ctx->d_out7, ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
-  HIPCHK(ctx, hipGetLastError());
-  HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+  int cap = ctx->traj_cap_steps < P.maxnstep ? ctx->traj_cap_steps : P.maxnstep;
+  const bool two = P.target_nturns < 0 && ctx->trace_budget > 0 && cap > ctx->trace_budget + 1;
+  bool fallback = false;
+  if (two && (rc = render_two_phase(ctx, P, nrays, cap, fallback))) return rc;
+  const bool one = !two || fallback;
+  if (one) {
+    int stride;
+    unsigned long long total;
+    if ((rc = prepare_packets(ctx, P, nrays, false, cap, total, stride))) return rc;
+    /* long rays of a polarized image: their chains run as composed segments */
+    int nseg = 0;
+    if (stride == PK && !P.stokes_floors) {
+      std::vector<int> ns(nrays), base(nrays, -1), sray, sq0;
+      HIPCHK(ctx, hipMemcpyAsync(ns.data(), ctx->d_nstep_ray, nrays * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      for (int r = 0; r < nrays; r++)
+        if (ns[r] > ctx->seg_min_steps && ns[r] < P.maxnstep - 1) {
+          base[r] = (int)sray.size();
+          for (int q0 = 0; q0 < ns[r]; q0 += SEG_LEN) {
+            sray.push_back(r);
+            sq0.push_back(q0);
+          }
+        }
+      nseg = (int)sray.size();
+      if (nseg > 0) {
+        if ((rc = grow(ctx, ctx->d_segbase, ctx->cap_segbase, (size_t)nrays))) return rc;
+        if ((rc = grow(ctx, ctx->d_segray, ctx->cap_segray, (size_t)nseg))) return rc;
+        if ((rc = grow(ctx, ctx->d_segq0, ctx->cap_segq0, (size_t)nseg))) return rc;
+        if ((rc = grow(ctx, ctx->d_segflag, ctx->cap_segflag, (size_t)nseg))) return rc;
+        if ((rc = grow(ctx, ctx->d_segmap, ctx->cap_segmap, (size_t)nseg * SEGCOL * NPAR))) return rc;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_segbase, base.data(), nrays * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_segray, sray.data(), nseg * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_segq0, sq0.data(), nseg * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        long long lanes = (long long)nseg * SEGCOL;
+        k_seg<<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>(nseg, ctx->d_segray, ctx->d_segq0,
+                                                                               ctx->d_off, ctx->d_nstep_ray, ctx->d_pkt,
+                                                                               ctx->d_segmap, ctx->d_segflag);
+        HIPCHK(ctx, hipGetLastError());
+        /* the host vectors above must outlive their copies */
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      }
+    }
+    k_chain<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, ctx->d_off,
+                                                     ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, ctx->d_pkt,
+                                                     nseg > 0 ? ctx->d_segbase : nullptr, ctx->d_segmap,
+                                                     ctx->d_segflag, ctx->d_out7, ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+  }
+  const int e1 = one ? 4 : 7;
   double f3 = pow(P.freqcgs, 3);
   k_finalize<<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(P, nrays, f3, ctx->d_out7, ctx->d_flag_ray,
                                                            ctx->d_nstep_ray, ctx->d_image, ctx->d_imageS,
                                                            ctx->d_taus, ctx->d_nstep_pix, ctx->d_flag_pix,
                                                            ctx->d_status);
   HIPCHK(ctx, hipGetLastError());
-  HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->ev[e1], ctx->stream));
   int status[NSTATUS];
   HIPCHK(ctx, hipMemcpyAsync(status, ctx->d_status, sizeof(status), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(ctx->counts, ctx->d_counts, sizeof(ctx->counts), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  for (int q = 0; q < 4; q++) hipEventElapsedTime(&ctx->ms_stage[q], ctx->ev[q], ctx->ev[q + 1]);
-  ctx->ms_geo = ctx->ms_stage[0];
-  ctx->ms_xfer = ctx->ms_stage[1] + ctx->ms_stage[2];
-  hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[4]);
+  if (one) {
+    /* trace, packets, chains, finalize */
+    for (int q = 0; q < 4; q++) hipEventElapsedTime(&ctx->ms_stage[q], ctx->ev[q], ctx->ev[q + 1]);
+    ctx->ms_geo = ctx->ms_stage[0];
+    ctx->ms_xfer = ctx->ms_stage[1] + ctx->ms_stage[2];
+  } else {
+    /* [0] both trace launches, [1] the finished rays' transfer on the second
+     * stream, [2] the paused rays' transfer after their trace, [3] finalize */
+    float a, b;
+    hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
+    hipEventElapsedTime(&b, ctx->ev[2], ctx->ev[3]);
+    ctx->ms_stage[0] = a + b;
+    hipEventElapsedTime(&ctx->ms_stage[1], ctx->ev[8], ctx->ev[5]);
+    hipEventElapsedTime(&ctx->ms_stage[2], ctx->ev[3], ctx->ev[6]);
+    hipEventElapsedTime(&ctx->ms_stage[3], ctx->ev[6], ctx->ev[7]);
+    hipEventElapsedTime(&ctx->ms_geo, ctx->ev[0], ctx->ev[3]);
+    hipEventElapsedTime(&ctx->ms_xfer, ctx->ev[3], ctx->ev[6]);
+    ctx->ms_phase[0] = a;
+    ctx->ms_phase[1] = b;
+    hipEventElapsedTime(&ctx->ms_phase[2], ctx->ev[3], ctx->ev[4]);
+    hipEventElapsedTime(&ctx->ms_phase[3], ctx->ev[0], ctx->ev[5]);
+  }
+  ctx->two_phase = !one;
+  hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[e1]);
   ctx->last_npix = (int)npix;
   ctx->last_unpol = P.only_unpolarized;
   ctx->have_result = 1;
@@ -1361,6 +1815,15 @@ int ipgpu_stage_times(const ipgpu_ctx *ctx, double ms[4])
 {
   if (!ctx || !ms) return IPGPU_ERR_ARG;
   for (int q = 0; q < 4; q++) ms[q] = ctx->ms_stage[q];
+  return ctx->two_phase;
+}
+
+int ipgpu_set_tuning(ipgpu_ctx *ctx, int trace_budget, int seg_min_steps, int record_steps)
+{
+  if (!ctx) return IPGPU_ERR_ARG;
+  if (trace_budget >= 0) ctx->trace_budget = trace_budget;
+  if (seg_min_steps >= 0) ctx->seg_min_steps = seg_min_steps;
+  if (record_steps >= 0) ctx->traj_cap_steps = record_steps > 2 ? record_steps : 2;
   return IPGPU_OK;
 }
 

@@ -232,10 +232,69 @@ int emu_adjoint(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P,
   return status;
 }
 
+/* Every ray through the segmented chain (composed segment maps; on the GPU
+ * only rays longer than SEG_MIN_STEPS take this path). */
+int emu_render_segmented(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, const double *dump,
+                         const int *pix, int npix, double *out7, int *nstep, int *flag)
+{
+  int status = 0;
+#pragma omp parallel
+  {
+    std::vector<double> rec((size_t)P->maxnstep * REC);
+    std::vector<double> pkt, segmap;
+    std::vector<int> segflag;
+#pragma omp for schedule(dynamic, 1)
+    for (int n = 0; n < npix; n++) {
+      int i = pix[2 * n], j = pix[2 * n + 1];
+      int ns = trace_ray(*G, *P, i, j, P->maxnstep, rec.data(), nullptr);
+      nstep[n] = ns;
+      if (ns >= P->maxnstep - 1) {
+#pragma omp critical
+        status = -10;
+        continue;
+      }
+      pkt.assign((size_t)(ns > 0 ? ns : 1) * PK, 0.);
+      for (int q = 0; q < ns; q++)
+        prep_step<false>(*G, *M, *P, ns - q, rec.data(), nullptr, dump, pkt.data() + (size_t)q * PK, true,
+                         unit_scale());
+      const int nseg = (ns + SEG_LEN - 1) / SEG_LEN;
+      segmap.assign((size_t)(nseg > 0 ? nseg : 1) * SEGCOL * NPAR, 0.);
+      segflag.assign(nseg > 0 ? nseg : 1, 0);
+      for (int sg = 0; sg < nseg; sg++)
+        for (int b = 0; b < SEGCOL; b++) {
+          Herm N;
+          int q1 = (sg + 1) * SEG_LEN < ns ? (sg + 1) * SEG_LEN : ns;
+          int fl = segment_column(pkt.data(), sg * SEG_LEN, q1, b, N);
+          for (int p = 0; p < NPAR; p++) segmap[((size_t)sg * SEGCOL + b) * NPAR + p] = herm_par(N, p);
+          if (b == NPAR) segflag[sg] = fl;
+        }
+      double js, ks;
+      far_end_coefficients<false>(*G, *M, *P, ns, rec.data(), dump, js, ks, unit_scale());
+      int nrad;
+      flag[n] = chain_ray_segmented(*G, *P, ns, pkt.data(), segmap.data(), segflag.data(), js, ks,
+                                    out7 + 7 * (size_t)n, nrad);
+    }
+  }
+  return status;
+}
+
 /* Trajectory of one pixel: nstep+1 records of REC doubles. */
 int emu_trace(const ipgpu_geom *G, const ipgpu_image *P, int i, int j, double *rec)
 {
   return trace_ray(*G, *P, i, j, P->maxnstep, rec, nullptr);
+}
+
+/* The same trajectory traced as the two-phase render does: at most budget
+ * steps, then resumed from the last record.  *paused = 1 if the first pass
+ * stopped at the budget. */
+int emu_trace_two_pass(const ipgpu_geom *G, const ipgpu_image *P, int i, int j, int budget, double *rec,
+                       int *paused)
+{
+  int fin = 1;
+  int ns = trace_ray(*G, *P, i, j, P->maxnstep, rec, nullptr, 0, budget, &fin);
+  *paused = !fin;
+  if (fin) return ns;
+  return trace_ray(*G, *P, 0, 0, P->maxnstep, rec, nullptr, ns);
 }
 
 }

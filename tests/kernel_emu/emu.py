@@ -32,6 +32,8 @@ def lib():
                                         C.POINTER(G.Image), dp, ip, C.c_int, C.c_double, dp, ip, ip]
         L.emu_adjoint.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), dp, dp, dp, dp,
                                   dp, dp]
+        L.emu_render_segmented.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), dp, ip,
+                                           C.c_int, dp, ip, ip]
         L.emu_trace.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Image), C.c_int, C.c_int, dp]
         _lib = L
     return _lib
@@ -101,6 +103,16 @@ def render(geom, model, image, dump, pix=None):
     return rc, out7, ns, fl
 
 
+def render_segmented(geom, model, image, dump):
+    """Every ray through the segmented (composed-map) chain."""
+    pix = np.array([(i, j) for i in range(image.nx) for j in range(image.ny)], np.int32)
+    n = len(pix)
+    out7, ns, fl = np.zeros((n, 7)), np.zeros(n, np.int32), np.zeros(n, np.int32)
+    rc = lib().emu_render_segmented(C.byref(geom), C.byref(model), C.byref(image), G.P(f64(dump)), G.IP(pix),
+                                    n, G.P(out7), G.IP(ns), G.IP(fl))
+    return rc, out7, ns, fl
+
+
 def render_scaled(geom, model, image_trace, image_job, dump, ne_scale=1.):
     """Batch-path arithmetic on the CPU (see emu_render_scaled)."""
     nx, ny = image_job.nx, image_job.ny
@@ -145,3 +157,10 @@ def trace(geom, image, i, j):
     rec = np.zeros(image.maxnstep * 8)
     ns = lib().emu_trace(C.byref(geom), C.byref(image), i, j, G.P(rec))
     return ns, rec[: 8 * (ns + 1)].reshape(ns + 1, 8)
+
+
+def trace_two_pass(geom, image, i, j, budget):
+    rec = np.zeros(image.maxnstep * 8)
+    paused = C.c_int(0)
+    ns = lib().emu_trace_two_pass(C.byref(geom), C.byref(image), i, j, budget, G.P(rec), C.byref(paused))
+    return ns, rec[: 8 * (ns + 1)].reshape(ns + 1, 8), bool(paused.value)

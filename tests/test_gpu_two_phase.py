@@ -1,0 +1,122 @@
+"""The two-phase render (first geodesic pass with a step budget, finished
+rays transferred on a second stream while the paused rays resume) against
+the one-pass render of the same image on the GPU: the per-ray arithmetic is
+the same, so every output array must be identical, whatever the budget or
+the record capacity (a resumed ray that outgrows it sends the render back
+to one pass with a larger capacity), given the same segment threshold (a
+composed chain agrees with the sequential one to roundoff only).  One case is
+also checked against the oracle at the parity bar."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import gpu_image_kwargs, small_dump
+from ipole_amd import ipgpu as G
+from test_gpu_parity import check
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def case():
+    h, prims = small_dump()
+    orc, fields = O.iharm_oracle(h, prims, M_unit=3e26)
+    return h, prims, orc
+
+
+def fresh(h, prims, **tune):
+    c = G.Context(0)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    c.set_geometry(g)
+    c.set_model(m)
+    c.upload_prims(prims, G.iharm_units(3e26))
+    if tune:
+        c.set_tuning(**tune)
+    return c, g, m
+
+
+def same(a, b):
+    for k in ("image", "imageS", "taus", "nstep", "flag"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("unpol", [0, 1])
+def test_two_phase_equals_one_pass(case, unpol):
+    h, prims, orc = case
+    say("case two-phase vs one-pass unpol=%d 32x32 th=163" % unpol)
+    c1, g, m = fresh(h, prims, trace_budget=0)
+    im, scale, _ = G.make_image(g, m.L_unit, nx=32, ny=32, thetacam=163., only_unpolarized=unpol)
+    refs = {}
+    for segmin in (2048, 100, 200):
+        c1.set_tuning(seg_min_steps=segmin)
+        refs[segmin] = c1.render(im)
+        assert c1.stage_times_mode()[1] is False
+    c1.close()
+    nmax = int(refs[2048]["nstep"].max())
+    assert nmax > 300
+    for budget, segmin in ((64, 2048), (300, 100), (nmax - 1, 200), (4096, 2048)):
+        say("  budget %d seg_min %d" % (budget, segmin))
+        c2, _, _ = fresh(h, prims, trace_budget=budget, seg_min_steps=segmin)
+        got = c2.render(im)
+        assert c2.stage_times_mode()[1] is True
+        same(refs[segmin], got)
+        c2.close()
+
+
+def test_two_phase_vs_oracle(case):
+    """Small budget (most rays resume) and a low segment threshold (most
+    chains composed), at the parity bar against the oracle."""
+    h, prims, orc = case
+    say("case two-phase budget 64 seg_min 64 vs oracle 32x32 th=60")
+    orc.activate()
+    p = O.default_params(nx=32, ny=32, thetacam=60., freqcgs=230e9)
+    ref = orc.render(p)
+    c, g, m = fresh(h, prims, trace_budget=64, seg_min_steps=64)
+    im, scale, _ = G.make_image(g, m.L_unit, **gpu_image_kwargs(p))
+    got = c.render(im)
+    assert c.stage_times_mode()[1] is True
+    check(ref, got, scale)
+    c.close()
+
+
+def test_two_phase_record_overflow(case):
+    """Resumed rays longer than the record capacity: the render retraces in
+    one pass with room for them and returns the one-pass result."""
+    h, prims, orc = case
+    say("case two-phase record overflow 24x24")
+    c1, g, m = fresh(h, prims, trace_budget=0)
+    im, _, _ = G.make_image(g, m.L_unit, nx=24, ny=24, thetacam=163.)
+    ref = c1.render(im)
+    c1.close()
+    assert ref["nstep"].max() > 200
+    c2, _, _ = fresh(h, prims, trace_budget=50, record_steps=200)
+    got = c2.render(im)
+    same(ref, got)
+    # the capacity was raised: the next render is two-phase throughout
+    got2 = c2.render(im)
+    assert c2.stage_times_mode()[1] is True
+    same(ref, got2)
+    c2.close()
+
+
+def test_two_phase_maxnstep(case):
+    """A resumed ray past maxnstep is reported with the one-pass pixel."""
+    h, prims, orc = case
+    say("case two-phase maxnstep 8x8")
+    errs = []
+    for budget in (0, 20):
+        c, g, m = fresh(h, prims, trace_budget=budget)
+        im, _, _ = G.make_image(g, m.L_unit, nx=8, ny=8, thetacam=163., maxnstep=50)
+        with pytest.raises(G.IpgpuError) as e:
+            c.render(im)
+        assert e.value.code == G.ERR_MAXNSTEP
+        errs.append(str(e.value))
+        c.close()
+    assert errs[0] == errs[1]

@@ -329,3 +329,46 @@ def test_emu_adjoint_vs_finite_differences():
         fd = (loss(dp) - loss(dm)) / (2 * hM)
         print("cell", c, gcell[c], fd)
         assert abs(gcell[c] / fd - 1) < 1e-4
+
+
+def test_emu_segmented_chain():
+    """The composed-segment chain (used on the GPU for rays longer than 2048
+    steps) for every ray of an image: against the sequential chain of the
+    same packets to roundoff, and against the oracle at the plain bar."""
+    h, prims = small_dump()
+    orc, fields = O.iharm_oracle(h, prims, M_unit=3e26)
+    p = O.default_params(nx=16, ny=16, thetacam=60., freqcgs=230e9)
+    ref = orc.render(p)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    im, scale, _ = G.make_image(g, m.L_unit, nx=16, ny=16, thetacam=60., freqcgs=230e9)
+    d = emu.pack_dump(fields)
+    rc, seq, ns, fl = emu.render(g, m, im, d)
+    rc2, seg, ns2, fl2 = emu.render_segmented(g, m, im, d)
+    assert rc == 0 and rc2 == 0 and np.array_equal(ns, ns2) and np.array_equal(fl, fl2)
+    assert ns.max() > 256   # some rays span several segments
+    den = np.maximum(np.abs(seq[:, 1]), 1e-6 * np.abs(seq[:, 1]).max())
+    assert (np.abs(seg[:, 1:5] - seq[:, 1:5]) / den[:, None]).max() < 1e-10
+    assert np.array_equal(seg[:, 0], seq[:, 0]) and np.array_equal(seg[:, 5], seq[:, 5])
+    F, errs = image_errors(ref, seg, 230e9 ** 3, scale)
+    assert F < 1e-6
+    for k, e in errs.items():
+        assert e.max() < 1e-5, (k, e.max())
+
+
+@pytest.mark.parametrize("budget", [1, 50, 200])
+def test_emu_trace_two_pass_bit_identical(budget):
+    """A trajectory traced for `budget` steps and resumed from its last
+    record (the two-phase render) is bit-identical to the one-pass trace:
+    same step count, same records."""
+    h, _ = small_dump()
+    g, m = G.iharm_geom(h), G.iharm_model()
+    im, _, _ = G.make_image(g, m.L_unit, nx=16, ny=16, thetacam=60., freqcgs=230e9)
+    paused_any = False
+    for i, j in [(0, 0), (3, 11), (8, 8), (15, 2), (7, 9)]:
+        ns, rec = emu.trace(g, im, i, j)
+        ns2, rec2, paused = emu.trace_two_pass(g, im, i, j, budget)
+        paused_any |= paused
+        assert paused == (ns > budget)
+        assert ns2 == ns
+        assert np.array_equal(rec, rec2)
+    assert paused_any
