@@ -53,25 +53,26 @@ def cpu_baseline(h, prims, cfg, sample_stride, nthreads):
                                                        nthreads))
 
 
-def roofline(stage_ms, n_geo, two_phase):
+def roofline(stage_ms, n_geo, streamed):
     """Roofline of the dominant kernel (longest stage): algorithmic FLOP per
     launch = per-unit count (op-counting build, ipole_amd/flops_per_step.json)
     x units (geodesic steps = transfer steps per image); traffic = measured
-    HBM bytes per launch (ipole_amd/hbm_traffic.json).  In the two-phase
-    render the geodesic pass is two launches (k_trace_budget, then
-    k_trace_resume on the paused rays), timed together; the transfer kernels
-    run beside the second one and are not timed one by one."""
+    HBM bytes per launch (ipole_amd/hbm_traffic.json).  In the streamed
+    render the geodesic pass is one launch per phase (k_trace_budget, then
+    k_trace_resume_mask on the rays still live), timed together; the
+    transfer kernels run beside the later phases and are not timed one by
+    one."""
     fl = json.load(open(os.path.join(ROOT, "ipole_amd", "flops_per_step.json")))["pol"]
     tr = json.load(open(os.path.join(ROOT, "ipole_amd", "hbm_traffic.json")))["bytes_per_launch"]
     per_unit = dict(k_trace=fl["F_trace_per_step"], k_prep=fl["F_prep_per_step"],
                     k_chain=fl["F_chain_per_step"])
-    if two_phase:
+    if streamed:
         t = dict(k_trace=stage_ms[0])
     else:
         t = dict(k_trace=stage_ms[0], k_prep=stage_ms[1], k_chain=stage_ms[2])
     ach = {k: per_unit[k] * n_geo / (t[k] * 1e-3) / 1e12 for k in t}
     dom = max(t, key=t.get)
-    name = "k_trace_budget+k_trace_resume" if two_phase and dom == "k_trace" else dom
+    name = "k_trace_budget+k_trace_resume_mask" if streamed and dom == "k_trace" else dom
     return dict(bound="mfma", achieved=round(ach[dom], 4), peak=PEAK_FP64, unit="TFLOP/s",
                 frac=round(ach[dom] / PEAK_FP64, 5), traffic=tr.get(dom), kernel=name,
                 note="FP64 compute roof (MI355X FP64 peak 78.6 TFLOP/s, which the FP64 vector ALU "
@@ -351,7 +352,7 @@ def main():
         ms_geo.append(a)
         ms_xfer.append(b)
         ms_tot.append(c)
-        st_, two_phase = ctx.stage_times_mode()
+        st_, streamed = ctx.stage_times_mode()
         ms_stage.append(st_)
     wall = time.perf_counter() - t0
     n_geo, n_rad = ctx.step_counts()
@@ -377,9 +378,9 @@ def main():
     ctx.close()
     ms_step = 1e3 * wall / args.steps
     st_ms = np.mean(np.array(ms_stage), axis=0)
-    if two_phase:
-        log("per step: wall %.3f ms, device %.3f ms (two-phase: trace launches %.3f, finished rays' "
-            "transfer on the second stream %.3f, tail after the resumed trace %.3f, k_finalize %.3f)" %
+    if streamed:
+        log("per step: wall %.3f ms, device %.3f ms (streamed: trace launches %.3f, span of the finished "
+            "rays' transfer %.3f, tail after the last trace launch %.3f, k_finalize %.3f)" %
             ((ms_step, np.mean(ms_tot)) + tuple(st_ms)))
     else:
         log("per step: wall %.3f ms, device %.3f ms (k_trace %.3f, k_prep %.3f, k_chain %.3f, k_finalize %.3f)" %
@@ -387,7 +388,7 @@ def main():
     log("image: Ftot %.6g Jy, unpol %.6g Jy, LP %.3f%%, CP %.3f%%; geodesic steps %d, radiating %d" %
         (st["Ftot"], st["Ftot_unpol"], st["LP"], st["CP"], n_geo, n_rad))
 
-    roof = roofline(np.mean(np.array(ms_stage), axis=0), n_geo, two_phase)
+    roof = roofline(np.mean(np.array(ms_stage), axis=0), n_geo, streamed)
     batched = precision = adjoint = None
     if not args.no_batch and not args.small:
         batched = batch_measure(args, h, prims)
@@ -411,7 +412,7 @@ def main():
                            radiating_steps=n_rad, ms_device=round(float(np.mean(ms_tot)), 4),
                            ms_geodesic=round(float(np.mean(ms_geo)), 4),
                            ms_transfer=round(float(np.mean(ms_xfer)), 4),
-                           two_phase=bool(two_phase), ms_one_pass=round(float(np.mean(one[1:])), 4)),
+                           streamed=bool(streamed), ms_one_pass=round(float(np.mean(one[1:])), 4)),
                roofline=roof, cpu_baseline=cpu, batched=batched, mixed_precision=precision,
                adjoint=adjoint)
     print(json.dumps(out), flush=True)

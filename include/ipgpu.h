@@ -200,29 +200,30 @@ int ipgpu_kernel_times(const ipgpu_ctx *ctx, double *ms_geodesic, double *ms_tra
 /* Per-stage device time of the last render (HIP events).  Returns 0 after a
  * one-pass render: ms[0..3] = geodesic pass (k_trace), transfer step packets
  * (k_scan + k_prep), transfer recurrence (k_seg + k_chain), output layout
- * (k_finalize).  Returns 1 after a two-phase render: ms[0] = both geodesic
- * launches (k_trace_budget + k_trace_resume), ms[1] = transfer of the rays
- * finished in the first pass (second stream, concurrent with the resumed
- * trace), ms[2] = what follows the resumed trace (transfer of the resumed
- * rays and the wait for the second stream), ms[3] = k_finalize.
- * Measurement only; the reference prints one wallclock figure
- * (src/main.c:1038-1040). */
+ * (k_finalize).  Returns 1 after a streamed render: ms[0] = the geodesic
+ * launches (k_trace_budget, k_trace_resume_mask per later phase), ms[1] =
+ * the span of the transfer of the rays finished in the bounded phases (on
+ * their own streams, concurrent with later phases), ms[2] = from the end of
+ * the last geodesic launch until the streams have joined (transfer of the
+ * longest rays, the wait), ms[3] = k_finalize.  Measurement only; the
+ * reference prints one wallclock figure (src/main.c:1038-1040). */
 int ipgpu_stage_times(const ipgpu_ctx *ctx, double ms[4]);
 
 /* Tuning of ipgpu_render (no counterpart in the reference, whose pixel loop
- * has none).  trace_budget: steps of the first geodesic pass of the
- * two-phase render (default 4096; 0 = one pass; it applies when
- * target_nturns < 0 and the record capacity exceeds it).  seg_min_steps:
+ * has none).  trace_budget: steps of the last bounded geodesic phase of
+ * the streamed render (phases at a quarter, a half and all of it, then to
+ * the end; default 4096; 0 = one pass; it applies when target_nturns < 0
+ * and the record capacity exceeds it).  seg_min_steps:
  * polarized rays longer than this have their transfer composed from
- * 256-step segments (default 2048).  record_steps: geodesic records held
- * per ray (default 8192, raised by a render whose longest ray needs more;
- * that render retraces).  Results do not depend on trace_budget or
- * record_steps; seg_min_steps reorders the transfer arithmetic of the rays
- * it selects (agreement to roundoff).  A negative value leaves a setting
- * unchanged.  The environment variables
- * IPGPU_TRACE_BUDGET and IPGPU_SEG_MIN_STEPS set the defaults of a new
- * context. */
-int ipgpu_set_tuning(ipgpu_ctx *ctx, int trace_budget, int seg_min_steps, int record_steps);
+ * segments (default 4096).  seg_len: steps per segment (default 256).
+ * record_steps: geodesic records held per ray (default 8192, raised by a
+ * render whose longest ray needs more; that render retraces).  Results do
+ * not depend on trace_budget or record_steps; seg_min_steps and seg_len
+ * regroup the transfer arithmetic of the rays selected (agreement to
+ * roundoff).  A negative value leaves a setting unchanged.  The environment
+ * variables IPGPU_TRACE_BUDGET, IPGPU_SEG_MIN_STEPS and IPGPU_SEG_LEN set the
+ * defaults of a new context. */
+int ipgpu_set_tuning(ipgpu_ctx *ctx, int trace_budget, int seg_min_steps, int seg_len, int record_steps);
 
 /* Work counts of the last render: geodesic steps summed over rays (the
  * nstep of trace_geodesic) and radiating steps of the transfer pass (steps

@@ -46,6 +46,9 @@ inline double rcp(double x) { return 1.0 / x; }
 inline double exp_(double x) { return exp(x); }
 inline void sincos_(double x, double &s, double &c) { sincos(x, &s, &c); }
 inline void sincospi_(double y, double &s, double &c) { sincos(3.141592653589793 * y, &s, &c); }
+inline double exp_r(double x) { return exp(x); }
+inline void sincos_r(double x, double &s, double &c) { sincos(x, &s, &c); }
+inline void sincospi_r(double y, double &s, double &c) { sincos(3.141592653589793 * y, &s, &c); }
 inline double log_(double x) { return log(x); }
 inline double pow_(double a, double b) { return pow(a, b); }
 inline double cbrt_(double x) { return cbrt(x); }
@@ -71,9 +74,14 @@ constexpr double LN2_HI = 0.6931471805599453;
 constexpr double LN2_LO = 2.3190468138462996e-17;
 constexpr double LOG2E = 1.4426950408889634;
 
-IPG_MATH_HD double exp_(double x)
+/* exp_r, sincos_r, sincospi_r: the arithmetic of exp_, sincos_ and
+ * sincospi_ without the library fallback, for callers whose arguments are
+ * finite and in range (|x| < 700, |x| < 1e5, |y| < 1e9; NaN propagates).
+ * Their bodies have no branch, so that the compiler can interleave two
+ * independent evaluations (the geodesic step evaluates the metric at two
+ * points at once). */
+IPG_MATH_HD double exp_r(double x)
 {
-  if (!(fabs(x) < 700.)) return exp(x);
   double k = rint(x * LOG2E);
   double r = fma(-k, LN2_HI, x);
   r = fma(-k, LN2_LO, r);
@@ -92,6 +100,12 @@ IPG_MATH_HD double exp_(double x)
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
   return ldexp(p, (int)k);
+}
+
+IPG_MATH_HD double exp_(double x)
+{
+  if (!(fabs(x) < 700.)) return exp(x);
+  return exp_r(x);
 }
 
 /* sin r and cos r for |r| <= pi/4 */
@@ -134,6 +148,17 @@ constexpr double TWO_OVER_PI = 0.6366197723675814;
 constexpr double PI_HI = 3.141592653589793;
 constexpr double PI_MID = 1.2246467991473532e-16;
 
+IPG_MATH_HD void sincos_r(double x, double &s, double &c)
+{
+  double k = rint(x * TWO_OVER_PI);
+  double r = fma(-k, PIO2_HI, x);
+  r = fma(-k, PIO2_MID, r);
+  r = fma(-k, PIO2_LO, r);
+  double sr, cr;
+  sincos_reduced(r, sr, cr);
+  quadrant((int)k, sr, cr, s, c);
+}
+
 IPG_MATH_HD void sincos_(double x, double &s, double &c)
 {
   if (!(fabs(x) < 1.e5)) {
@@ -141,10 +166,14 @@ IPG_MATH_HD void sincos_(double x, double &s, double &c)
     c = cos(x);
     return;
   }
-  double k = rint(x * TWO_OVER_PI);
-  double r = fma(-k, PIO2_HI, x);
-  r = fma(-k, PIO2_MID, r);
-  r = fma(-k, PIO2_LO, r);
+  sincos_r(x, s, c);
+}
+
+IPG_MATH_HD void sincospi_r(double y, double &s, double &c)
+{
+  double k = rint(2. * y);
+  double t = y - 0.5 * k;
+  double r = fma(t, PI_HI, t * PI_MID);
   double sr, cr;
   sincos_reduced(r, sr, cr);
   quadrant((int)k, sr, cr, s, c);
@@ -157,12 +186,7 @@ IPG_MATH_HD void sincospi_(double y, double &s, double &c)
     c = cos(PI_HI * y);
     return;
   }
-  double k = rint(2. * y);
-  double t = y - 0.5 * k;
-  double r = fma(t, PI_HI, t * PI_MID);
-  double sr, cr;
-  sincos_reduced(r, sr, cr);
-  quadrant((int)k, sr, cr, s, c);
+  sincospi_r(y, s, c);
 }
 
 constexpr double SQRT_HALF = 0.7071067811865476;

@@ -76,6 +76,32 @@ IPG_HD double powa(double x, double alpha)
   return pow(x, alpha);
 }
 
+/* x^n for 0 <= n < 64: the products of ipow_d in the same order, selected
+ * instead of branched on. */
+IPG_HD_INLINE double ipow_sel(double x, int n)
+{
+  double r = 1.0, b = x;
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    r = ((n >> i) & 1) ? r * b : r;
+    b = b * b;
+  }
+  return r;
+}
+
+/* Metric kinds with straight-line evaluation code (template argument of
+ * theta_t / metric_t): eMinkowski, MKS, FMKS with an integer poly_alpha - 1
+ * in [0, 64) (x^n by ipow_sel), other FMKS (powa). */
+enum { MK_MINK = 0, MK_MKS = 1, MK_FMKS_INT = 2, MK_FMKS = 3, MK_ANY = 4 };
+
+IPG_HD int metric_kind(const ipgpu_geom &G)
+{
+  if (G.metric == IPGPU_METRIC_EMINKOWSKI) return MK_MINK;
+  if (G.metric == IPGPU_METRIC_MKS) return MK_MKS;
+  double a1 = G.poly_alpha - 1.;
+  return (floor(a1) == a1 && a1 >= 0. && a1 < 64.) ? MK_FMKS_INT : MK_FMKS;
+}
+
 /* ------------------------------------------------------------------ metric */
 
 struct Metric {
@@ -93,16 +119,18 @@ struct Theta {
 
 /* theta(X1, X2) and derivatives for MKS/FMKS: src/coordinates.c:69-81
  * (values) and :278-305 (first derivatives; the second derivatives follow
- * from the same closed form). */
-IPG_HD void theta_mks(const ipgpu_geom &G, double X1, double X2, Theta &T)
+ * from the same closed form).  X2 in [0, 1] and X1 within the grid keep the
+ * in-range math in range. */
+template <int MK>
+IPG_HD_INLINE void theta_t(const ipgpu_geom &G, double X1, double X2, Theta &T)
 {
   double s2p, c2p;
-  sincospi_(2. * X2, s2p, c2p);
+  sincospi_r(2. * X2, s2p, c2p);
   double hs = G.hslope;
   double thG = PI * X2 + ((1. - hs) / 2.) * s2p;
   double dG = PI + (1. - hs) * PI * c2p;
   double ddG = -2. * PI * PI * (1. - hs) * s2p;
-  if (G.metric == IPGPU_METRIC_MKS) {
+  if constexpr (MK == MK_MKS || MK == MK_MINK) {
     T.th = thG;
     T.t1 = 0.;
     T.t2 = dG;
@@ -110,24 +138,25 @@ IPG_HD void theta_mks(const ipgpu_geom &G, double X1, double X2, Theta &T)
     T.t12 = 0.;
     T.t22 = ddG;
     return;
+  } else {
+    double y = 2. * X2 - 1.;
+    double al = G.poly_alpha, xt = G.poly_xt, pn = G.poly_norm;
+    double ixt = rcp(xt);
+    double P = y * ixt;
+    double Pa1 = (MK == MK_FMKS_INT) ? ipow_sel(P, (int)(al - 1.)) : powa(P, al - 1.);
+    double Pa = Pa1 * P;
+    double thJ = pn * y * (1. + Pa * rcp(al + 1.)) + 0.5 * PI;
+    double dJ = 2. * pn * (1. + Pa);
+    double ddJ = 4. * pn * al * Pa1 * ixt;
+    double E = exp_r(G.mks_smooth * (G.startx[1] - X1));
+    double ms = G.mks_smooth;
+    T.th = thG + E * (thJ - thG);
+    T.t1 = -ms * E * (thJ - thG);
+    T.t2 = dG + E * (dJ - dG);
+    T.t11 = ms * ms * E * (thJ - thG);
+    T.t12 = -ms * E * (dJ - dG);
+    T.t22 = ddG + E * (ddJ - ddG);
   }
-  double y = 2. * X2 - 1.;
-  double al = G.poly_alpha, xt = G.poly_xt, pn = G.poly_norm;
-  double ixt = rcp(xt);
-  double P = y * ixt;
-  double Pa1 = powa(P, al - 1.);
-  double Pa = Pa1 * P;
-  double thJ = pn * y * (1. + Pa * rcp(al + 1.)) + 0.5 * PI;
-  double dJ = 2. * pn * (1. + Pa);
-  double ddJ = 4. * pn * al * Pa1 * ixt;
-  double E = exp_(G.mks_smooth * (G.startx[1] - X1));
-  double ms = G.mks_smooth;
-  T.th = thG + E * (thJ - thG);
-  T.t1 = -ms * E * (thJ - thG);
-  T.t2 = dG + E * (dJ - dG);
-  T.t11 = ms * ms * E * (thJ - thG);
-  T.t12 = -ms * E * (dJ - dG);
-  T.t22 = ddG + E * (ddJ - ddG);
 }
 
 /* r, theta only (src/coordinates.c:46-97) */
@@ -153,14 +182,16 @@ IPG_HD void bl_coord(const ipgpu_geom &G, double X1, double X2, double &r, doubl
 
 /* Metric, inverse and X1/X2 derivatives at (X1, X2).  Same tensor as
  * gcov_func (src/coordinates.c:175-213, gcov_ks :216-239, set_dxdX
- * :263-348): g_MKS = J^T g_KS J with J = dx^KS/dX. */
-IPG_HD void metric(const ipgpu_geom &G, double X1, double X2, Metric &m)
+ * :263-348): g_MKS = J^T g_KS J with J = dx^KS/dX.  One straight-line body
+ * per metric kind (MK_*). */
+template <int MK>
+IPG_HD_INLINE void metric_t(const ipgpu_geom &G, double X1, double X2, Metric &m)
 {
-  double r = exp_(X1);
+  double r = exp_r(X1);
   m.r = r;
-  if (G.metric == IPGPU_METRIC_EMINKOWSKI) {
+  if constexpr (MK == MK_MINK) {
     double th = X2, s, c;
-    sincos_(th, s, c);
+    sincos_r(th, s, c);
     double r2 = r * r;
     m.th = th;
     for (int n = 0; n < 10; n++) {
@@ -184,13 +215,13 @@ IPG_HD void metric(const ipgpu_geom &G, double X1, double X2, Metric &m)
     m.d2[9] = 2. * r2 * s * c;
     m.sqrtmg = r2 * r * fabs(s);
     return;
-  }
+  } else {
   Theta T;
-  theta_mks(G, X1, X2, T);
+  theta_t<MK>(G, X1, X2, T);
   m.th = T.th;
   double a = G.a, a2 = a * a;
   double s, c;
-  sincos_(T.th, s, c);
+  sincos_r(T.th, s, c);
   double s2 = s * s, s2t = 2. * s * c;
   double Sig = r * r + a2 * c * c;
   double Sr = 2. * r, St = -2. * a2 * c * s;
@@ -262,6 +293,25 @@ IPG_HD void metric(const ipgpu_geom &G, double X1, double X2, Metric &m)
   m.gc[8] = u * c13;
   m.gc[9] = c33;
   m.sqrtmg = Sig * fabs(s) * r * fabs(t2);
+  }
+}
+
+IPG_HD_INLINE void metric(const ipgpu_geom &G, double X1, double X2, Metric &m)
+{
+  switch (metric_kind(G)) {
+  case MK_MINK: metric_t<MK_MINK>(G, X1, X2, m); break;
+  case MK_MKS: metric_t<MK_MKS>(G, X1, X2, m); break;
+  case MK_FMKS_INT: metric_t<MK_FMKS_INT>(G, X1, X2, m); break;
+  default: metric_t<MK_FMKS>(G, X1, X2, m); break;
+  }
+}
+
+/* metric_t for a kind fixed at compile time, metric() for MK_ANY */
+template <int MK>
+IPG_HD_INLINE void metric_k(const ipgpu_geom &G, double X1, double X2, Metric &m)
+{
+  if constexpr (MK == MK_ANY) metric(G, X1, X2, m);
+  else metric_t<MK>(G, X1, X2, m);
 }
 
 IPG_HD void lower(const double g[10], const double u[4], double l[4])

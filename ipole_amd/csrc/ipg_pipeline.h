@@ -68,34 +68,25 @@ IPG_HD void load_rec(const double *rec, int n, Rec &r)
  * receives the subring count per step when P.target_nturns >= 0.
  * With nend, the integration pauses after nend steps (*finished = 0); with
  * n0 > 0 it resumes from record n0 (its X1..3 and K are the registers'
- * values after that step bit for bit; X0 is never read), without nt. */
-IPG_HD int trace_ray(const ipgpu_geom &G, const ipgpu_image &P, int pi, int pj, int cap, double *rec,
-                     int *nt, int n0 = 0, int nend = 0x7fffffff, int *finished = nullptr)
+ * values after that step bit for bit; X0 is never read), without nt.
+ *
+ * push_photon (src/geodesics.c:15-49) in the order that shortens the
+ * dependent chain of a step: the end point X' = X + dl Kh is known once
+ * the half step is, so the metric at X' (the next step's start) is
+ * evaluated together with the metric at the half-step point, two
+ * independent straight-line evaluations the compiler interleaves.  The
+ * values are those of the plain order. */
+template <int MK, bool NT>
+IPG_HD_INLINE int trace_ray_t(const ipgpu_geom &G, const ipgpu_image &P, double X[4], double K[4], int nstep,
+                              int cap, double *rec, int *nt, int nend, int *finished)
 {
   Metric m;
-  double X[4], K[4];
-  int nstep = 0;
-  if (n0 > 0) {
-    Rec r;
-    load_rec(rec, n0, r);
-    X[0] = 0.;
-    for (int k = 1; k < 4; k++) X[k] = r.X[k];
-    for (int k = 0; k < 4; k++) K[k] = r.K[k];
-    nstep = n0;
-  } else {
-    double E[4][4];
-    for (int k = 0; k < 4; k++) X[k] = P.Xcam[k];
-    camera_tetrad(G, X, P.old_centering, E, m);
-    init_K(P, E, pi, pj, K);
-    if (cap > 0) store_rec(rec, X, K, 0.);
-    if (nt && cap > 0) nt[0] = 0;
-  }
   int passed = (P.Xcam[2] < 0.5) ? 1 : ((P.Xcam[2] > 0.5) ? -1 : 0);
   int nturns = 0;
   double za = 0., zb = 0., x2a = 0., x2b = 0.;
   int done = 1;
+  metric_k<MK>(G, X[1], X[2], m);
   while (true) {
-    metric(G, X[1], X[2], m);
     if (stop_backward(G, m.r, K)) break;
     if (nstep >= P.maxnstep - 1) break;
     if (nstep >= nend) {
@@ -105,15 +96,18 @@ IPG_HD int trace_ray(const ipgpu_geom &G, const ipgpu_image &P, int pi, int pj, 
     nstep++;
     double dl = stepsize(G, X, K, P.eps);
     double dlp = -dl;
-    double Xh[4], Kh[4];
+    double Xh[4], Kh[4], Xn[4];
     half_step(m, X, K, dlp, Xh, Kh);
-    metric(G, Xh[1], Xh[2], m);
+    for (int k = 0; k < 4; k++) Xn[k] = X[k] + dlp * Kh[k];
+    Metric mh;
+    metric_k<MK>(G, Xh[1], Xh[2], mh);
+    metric_k<MK>(G, Xn[1], Xn[2], m);
     double acc[4];
-    geodesic_accel(m, Kh, acc);
+    geodesic_accel(mh, Kh, acc);
     for (int k = 0; k < 4; k++) K[k] = K[k] - dlp * acc[k];
-    for (int k = 0; k < 4; k++) X[k] = X[k] + dlp * Kh[k];
+    for (int k = 0; k < 4; k++) X[k] = Xn[k];
     if (nstep < cap) store_rec(rec + (size_t)nstep * REC, X, K, dl);
-    if (nt) {
+    if constexpr (NT) {
       /* subring turning points, src/model_geodesics.c:129-165 */
       double rr, th;
       bl_coord(G, X[1], X[2], rr, th);
@@ -140,6 +134,39 @@ IPG_HD int trace_ray(const ipgpu_geom &G, const ipgpu_image &P, int pi, int pj, 
   }
   if (finished) *finished = done;
   return nstep;
+}
+
+IPG_HD int trace_ray(const ipgpu_geom &G, const ipgpu_image &P, int pi, int pj, int cap, double *rec,
+                     int *nt, int n0 = 0, int nend = 0x7fffffff, int *finished = nullptr)
+{
+  double X[4], K[4];
+  if (n0 > 0) {
+    Rec r;
+    load_rec(rec, n0, r);
+    X[0] = 0.;
+    for (int k = 1; k < 4; k++) X[k] = r.X[k];
+    for (int k = 0; k < 4; k++) K[k] = r.K[k];
+  } else {
+    double E[4][4];
+    Metric m;
+    for (int k = 0; k < 4; k++) X[k] = P.Xcam[k];
+    camera_tetrad(G, X, P.old_centering, E, m);
+    init_K(P, E, pi, pj, K);
+    if (cap > 0) store_rec(rec, X, K, 0.);
+    if (nt && cap > 0) nt[0] = 0;
+    n0 = 0;
+  }
+#define IPG_TRACE_CASE(MKV)                                                          \
+  case MKV:                                                                           \
+    return nt ? trace_ray_t<MKV, true>(G, P, X, K, n0, cap, rec, nt, nend, finished)   \
+              : trace_ray_t<MKV, false>(G, P, X, K, n0, cap, rec, nt, nend, finished);
+  switch (metric_kind(G)) {
+    IPG_TRACE_CASE(MK_MINK)
+    IPG_TRACE_CASE(MK_MKS)
+    IPG_TRACE_CASE(MK_FMKS_INT)
+    default: IPG_TRACE_CASE(MK_FMKS)
+  }
+#undef IPG_TRACE_CASE
 }
 
 /* Step packet: everything one transfer step needs that does not depend on
@@ -485,15 +512,26 @@ IPG_HD int chain_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const do
  * The polarized recurrence is affine in the 16 real parameters of N
  * (d[4], re[6], im[6]): each step maps N -> A_q N + b_q (transport, then the
  * plasma-frame Stokes step; the transport of N = 0 is 0, so transporting
- * unconditionally is the same map).  A long ray's steps are cut into
- * segments of SEG_LEN whose composed maps (16 columns from the basis states
- * without sources, one from N = 0 with sources) are formed in parallel, then
- * applied in order from the far end: the same result up to the roundoff of
- * the regrouping. */
-constexpr int SEG_LEN = 256;
-constexpr int SEG_MIN_STEPS = 2048;  /* rays longer than this use segments */
+ * unconditionally is the same map).  The unpolarized recurrence is affine
+ * in I (approximate_solve: I -> a I + b, with a, b functions of the
+ * coefficients), and tau, tauF are sums.  A long ray's steps are cut into
+ * segments of seg_len steps whose composed maps are formed in parallel
+ * (16 columns of the N map from the basis states without sources, one from
+ * N = 0 with sources, one column for I, tau, tauF), then applied in order
+ * from the far end: the same result up to the roundoff of the regrouping. */
+constexpr int SEG_LEN = 256;          /* default segment length */
+constexpr int SEG_MIN_STEPS = 4096;   /* default: rays longer than this use segments */
 constexpr int NPAR = 16;
-constexpr int SEGCOL = NPAR + 1;
+constexpr int SEG_I = NPAR + 1;       /* column of the unpolarized map */
+constexpr int SEGCOL = NPAR + 2;
+/* slots of the unpolarized column: the map I -> a I + b of the segment's
+ * radiating steps after its first one (which enters with the emissivity and
+ * absorptivity carried from earlier segments and is applied when the
+ * segments are composed), their tau, the segment's tauF and radiating step
+ * count, the first radiating step's data and the last one's ju, ku (the
+ * carry out) */
+constexpr int SI_A = 0, SI_B = 1, SI_TAU = 2, SI_TAUF = 3, SI_NRAD = 4, SI_M2 = 5, SI_J1 = 6, SI_K1 = 7,
+              SI_DL1 = 8, SI_JL = 9, SI_KL = 10;
 
 IPG_HD double &herm_par(Herm &N, int b)
 {
@@ -549,18 +587,80 @@ IPG_HD int segment_column(const double *pkt, int q0, int q1, int b, Herm &N)
   return flag;
 }
 
+/* approximate_solve as the affine map I -> a I + b it is (the same branch
+ * on dtau; the regrouping moves roundoff only). */
+IPG_HD void approximate_solve_affine(double ji, double ki, double jf, double kf, double dl, double &tau,
+                                     double &a, double &b)
+{
+  double javg = (ji + jf) / 2.;
+  double kavg = (ki + kf) / 2.;
+  double dtau = dl * kavg;
+  tau += dtau;
+  if (dtau < 1.e-3) {
+    double f = dl * (1. - (dtau * 0.5) * (1. - dtau * (1. / 3.)));
+    a = 1. - kavg * f;
+    b = javg * f;
+    return;
+  }
+  double efac = exp_(-dtau);
+  a = efac;
+  b = (javg * rcp(kavg)) * (1. - efac);
+}
+
+/* The unpolarized column of the map of steps [q0, q1) (slots SI_*).  The
+ * recurrence caches the emissivity and absorptivity of the last radiating
+ * step (src/ipolarray.c:140-146); within the segment that cache is local
+ * except at its first radiating step. */
+IPG_HD void segment_intensity(const double *pkt, int q0, int q1, double *o)
+{
+  double A = 1., B = 0., tau = 0., tauF = 0., js = 0., ks = 0.;
+  int nrad = 0;
+  for (int q = 0; q < NPAR; q++) o[q] = 0.;
+  for (int q = q0; q < q1; q++) {
+    const double *pk = pkt + (size_t)q * PK;
+    const int meta = (int)pk[PK_META];
+    if (meta & 1) {
+      const double ju = pk[PK_JU], ku = pk[PK_KU];
+      tauF += pk[PK_DTF];
+      if (nrad == 0) {
+        o[SI_M2] = (meta & 2) ? 1. : 0.;
+        o[SI_J1] = ju;
+        o[SI_K1] = ku;
+        o[SI_DL1] = pk[PK_DL];
+      } else {
+        double a, b;
+        approximate_solve_affine((meta & 2) ? 0. : js, ks, (meta & 2) ? 0. : ju, ku, pk[PK_DL], tau, a, b);
+        A = a * A;
+        B = a * B + b;
+      }
+      js = ju;
+      ks = ku;
+      nrad++;
+    }
+  }
+  o[SI_A] = A;
+  o[SI_B] = B;
+  o[SI_TAU] = tau;
+  o[SI_TAUF] = tauF;
+  o[SI_NRAD] = (double)nrad;
+  o[SI_JL] = js;
+  o[SI_KL] = ks;
+}
+
 /* A long ray from its segment maps (segmap: SEGCOL x NPAR doubles per
- * segment, column-major; segflag: the segments' oddflag bits), with the
- * unpolarized I, tau and tauF along its packets, then project_N. */
-IPG_HD int chain_ray_segmented(const ipgpu_geom &G, const ipgpu_image &P, int ns, const double *pkt,
-                               const double *segmap, const int *segflag, double js, double ks,
-                               double out7[7], int &nrad)
+ * segment, column-major, the last column the unpolarized map; segflag: the
+ * segments' oddflag bits), then project_N. */
+IPG_HD int chain_ray_segmented(const ipgpu_geom &G, const ipgpu_image &P, int ns, int seg_len,
+                               const double *segmap, const int *segflag, double js, double ks, double out7[7],
+                               int &nrad)
 {
   Herm N;
   for (int q = 0; q < 4; q++) N.d[q] = 0.;
   for (int q = 0; q < 6; q++) N.re[q] = N.im[q] = 0.;
   int flag = 0;
-  const int nseg = (ns + SEG_LEN - 1) / SEG_LEN;
+  double I = 0., tau = 0., tauF = 0.;
+  nrad = 0;
+  const int nseg = (ns + seg_len - 1) / seg_len;
   for (int sgi = 0; sgi < nseg; sgi++) {
     const double *A = segmap + (size_t)sgi * SEGCOL * NPAR;
     double x[NPAR];
@@ -573,37 +673,21 @@ IPG_HD int chain_ray_segmented(const ipgpu_geom &G, const ipgpu_image &P, int ns
       for (int b = 0; b < NPAR; b++) acc += A[b * NPAR + p] * x[b];
       herm_par(N, p) = acc;
     }
+    const double *u = A + SEG_I * NPAR;
+    const int nr = (int)u[SI_NRAD];
+    if (nr > 0) {
+      /* the segment's first radiating step with the carried js, ks, as the
+       * serial recurrence does it, then the rest of the segment */
+      const bool m2 = u[SI_M2] != 0.;
+      I = approximate_solve(I, m2 ? 0. : js, ks, m2 ? 0. : u[SI_J1], u[SI_K1], u[SI_DL1], tau);
+      I = u[SI_A] * I + u[SI_B];
+      tau += u[SI_TAU];
+      js = u[SI_JL];
+      ks = u[SI_KL];
+    }
+    tauF += u[SI_TAUF];
+    nrad += nr;
     flag |= segflag[sgi];
-  }
-  double I = 0., tau = 0., tauF = 0.;
-  nrad = 0;
-  /* the unpolarized recurrence, its packet fields loaded 8 steps at a time
-   * so that the loads of a group are in flight together */
-  constexpr int GRP = 8;
-  for (int q0 = 0; q0 < ns; q0 += GRP) {
-    double mt[GRP], dl[GRP], ju[GRP], ku[GRP], dtf[GRP];
-#pragma unroll
-    for (int k = 0; k < GRP; k++) {
-      const double *pk = pkt + (size_t)(q0 + k < ns ? q0 + k : ns - 1) * PK;
-      mt[k] = pk[PK_META];
-      dl[k] = pk[PK_DL];
-      ju[k] = pk[PK_JU];
-      ku[k] = pk[PK_KU];
-      dtf[k] = pk[PK_DTF];
-    }
-#pragma unroll
-    for (int k = 0; k < GRP; k++) {
-      if (q0 + k >= ns) break;
-      const int meta = (int)mt[k];
-      if (meta & 1) {
-        nrad++;
-        double ji = (meta & 2) ? 0. : js, ki = ks;
-        js = ju[k];
-        ks = ku[k];
-        I = approximate_solve(I, ji, ki, (meta & 2) ? 0. : ju[k], ku[k], dl[k], tau);
-        tauF += dtf[k];
-      }
-    }
   }
   if (tauF > 1.e100 || tauF < -1.e100 || isnan(tauF)) flag |= 2;
   if (isnan(N.d[0])) flag |= 4;

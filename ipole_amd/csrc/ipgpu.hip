@@ -140,27 +140,6 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep(ipgpu_geom G, ipgpu_model M
                  aux ? aux + (size_t)g * AX : nullptr);
 }
 
-/* One lane per (segment, column): the composed maps of the long rays'
- * segments.  seg_ray/seg_q0: ray and first step of each segment. */
-__global__ __launch_bounds__(WAVE) void k_seg(int nseg, const int *__restrict__ seg_ray,
-                                              const int *__restrict__ seg_q0, const long long *__restrict__ off,
-                                              const int *__restrict__ nstep_in, const double *__restrict__ pkt,
-                                              double *__restrict__ segmap, int *__restrict__ segflag)
-{
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)nseg * SEGCOL) return;
-  const int sgi = (int)(t / SEGCOL), b = (int)(t % SEGCOL);
-  const int ray = seg_ray[sgi], q0 = seg_q0[sgi];
-  const int ns = nstep_in[ray];
-  const int q1 = q0 + SEG_LEN < ns ? q0 + SEG_LEN : ns;
-  Herm N;
-  int fl = segment_column(pkt + (size_t)off[ray] * PK, q0, q1, b, N);
-  double *o = segmap + ((size_t)sgi * SEGCOL + b) * NPAR;
-#pragma unroll
-  for (int p = 0; p < NPAR; p++) o[p] = herm_par(N, p);
-  if (b == NPAR) segflag[sgi] = fl;
-}
-
 /* One lane per ray: the recurrence of N and I over the ray's packets. */
 __global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P, int nrays,
                                                 int cap, int stride, const long long *__restrict__ off,
@@ -168,7 +147,7 @@ __global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ip
                                                 const double *__restrict__ traj,
                                                 const double *__restrict__ dump,
                                                 const double *__restrict__ pkt,
-                                                const int *__restrict__ seg_base,
+                                                const int *__restrict__ seg_base, int seg_len,
                                                 const double *__restrict__ segmap,
                                                 const int *__restrict__ segflag,
                                                 double *__restrict__ out7, int *__restrict__ flags,
@@ -196,8 +175,8 @@ __global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ip
   int nrad;
   const int sb = seg_base ? seg_base[ray] : -1;
   if (sb >= 0)
-    flags[ray] = chain_ray_segmented(G, P, ns, pkt + (size_t)off[ray] * stride, segmap + (size_t)sb * SEGCOL * NPAR,
-                                     segflag + sb, js, ks, res, nrad);
+    flags[ray] = chain_ray_segmented(G, P, ns, seg_len, segmap + (size_t)sb * SEGCOL * NPAR, segflag + sb, js, ks,
+                                     res, nrad);
   else
     flags[ray] = chain_ray(G, P, ns, pkt + (size_t)off[ray] * stride, stride, js, ks, res, nrad);
   for (int q = 0; q < 7; q++) o[q] = res[q];
@@ -585,42 +564,57 @@ __global__ void k_bc_ne(ipgpu_geom G, int stage, double *dump)
   }
 }
 
-/* ------------------------------------------------ two-phase single image */
+/* ------------------------------------------------ streamed single image */
 
-/* Phase 1: every ray with a step budget; live[ray] = 1 where it paused. */
+/* First trace phase: every ray with a step budget; live[ray] = 1 where it
+ * paused. */
 __global__ __launch_bounds__(WAVE) void k_trace_budget(ipgpu_geom G, ipgpu_image P, int nrays, int cap, int nend,
                                                        double *__restrict__ traj, int *__restrict__ nstep_out,
-                                                       int *__restrict__ live, unsigned long long *__restrict__ counts)
+                                                       int *__restrict__ live, int *__restrict__ host_ns,
+                                                       int *__restrict__ host_live,
+                                                       unsigned long long *__restrict__ counts)
 {
   int ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= nrays) return;
   int pi, pj;
   ray_pixel(ray, P.nx, P.ny, pi, pj);
-  if (pi >= P.nx || pj >= P.ny) {
-    nstep_out[ray] = 0;
-    live[ray] = 0;
-    return;
+  int nstep = 0, lv = 0;
+  if (pi < P.nx && pj < P.ny) {
+    int fin = 1;
+    nstep = trace_ray(G, P, pi, pj, cap, traj + (size_t)ray * cap * REC, nullptr, 0, nend, &fin);
+    lv = fin ? 0 : 1;
+    atomicAdd(&counts[0], (unsigned long long)nstep);
   }
-  int fin = 1;
-  int nstep = trace_ray(G, P, pi, pj, cap, traj + (size_t)ray * cap * REC, nullptr, 0, nend, &fin);
   nstep_out[ray] = nstep;
-  live[ray] = fin ? 0 : 1;
-  atomicAdd(&counts[0], (unsigned long long)nstep);
+  live[ray] = lv;
+  host_ns[ray] = nstep;
+  host_live[ray] = lv;
 }
 
-/* Phase 2: the paused rays, resumed from their last record. */
-__global__ __launch_bounds__(WAVE) void k_trace_resume(ipgpu_geom G, ipgpu_image P, const int *__restrict__ list,
-                                                       int n, int cap, double *__restrict__ traj,
-                                                       int *__restrict__ nstep_io, int *__restrict__ status,
-                                                       unsigned long long *__restrict__ counts)
+/* A later trace phase over all rays: the rays still live resume from their
+ * last record for at most nend steps in all (nend < 0: to the end).  The
+ * host rows receive the step count of every ray integrated here and the
+ * live flag of every ray. */
+__global__ __launch_bounds__(WAVE) void k_trace_resume_mask(ipgpu_geom G, ipgpu_image P, int nrays, int cap,
+                                                            int nend, double *__restrict__ traj,
+                                                            int *__restrict__ nstep_io, int *__restrict__ live,
+                                                            int *__restrict__ host_ns, int *__restrict__ host_live,
+                                                            unsigned long long *__restrict__ counts)
 {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int ray = list[t];
+  int ray = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ray >= nrays) return;
+  if (!live[ray]) {
+    host_live[ray] = 0;   /* the host row is complete for every ray */
+    return;
+  }
   const int n0 = nstep_io[ray];
-  int nstep = trace_ray(G, P, 0, 0, cap, traj + (size_t)ray * cap * REC, nullptr, n0);
+  int fin = 1;
+  int nstep = trace_ray(G, P, 0, 0, cap, traj + (size_t)ray * cap * REC, nullptr, n0,
+                        nend < 0 ? 0x7fffffff : nend, &fin);
   nstep_io[ray] = nstep;
-  if (nstep >= cap) atomicAdd(&status[0], 1);
+  live[ray] = fin ? 0 : 1;
+  host_ns[ray] = nstep;
+  host_live[ray] = fin ? 0 : 1;
   atomicAdd(&counts[0], (unsigned long long)(nstep - n0));
 }
 
@@ -664,7 +658,7 @@ __global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model M
                                                      const int *__restrict__ nstep_in,
                                                      const double *__restrict__ traj,
                                                      const double *__restrict__ dump, const double *__restrict__ pkt,
-                                                     const int *__restrict__ seg_base,
+                                                     const int *__restrict__ seg_base, int seg_len,
                                                      const double *__restrict__ segmap,
                                                      const int *__restrict__ segflag, double *__restrict__ out7,
                                                      int *__restrict__ flags, int *__restrict__ status,
@@ -692,28 +686,39 @@ __global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model M
   int nrad;
   const int sb = seg_base ? seg_base[k] : -1;
   if (sb >= 0)
-    flags[ray] = chain_ray_segmented(G, P, ns, pkt + (size_t)offL[k] * stride, segmap + (size_t)sb * SEGCOL * NPAR,
-                                     segflag + sb, js, ks, res, nrad);
+    flags[ray] = chain_ray_segmented(G, P, ns, seg_len, segmap + (size_t)sb * SEGCOL * NPAR, segflag + sb, js, ks,
+                                     res, nrad);
   else
     flags[ray] = chain_ray(G, P, ns, pkt + (size_t)offL[k] * stride, stride, js, ks, res, nrad);
   for (int q = 0; q < 7; q++) o[q] = res[q];
   atomicAdd(&counts[1], (unsigned long long)nrad);
 }
 
-/* One lane per (segment, column), segments given by their packet offset. */
-__global__ __launch_bounds__(WAVE) void k_seg_off(int nseg, const long long *__restrict__ seg_off,
-                                                  const int *__restrict__ seg_q0, const int *__restrict__ seg_ns,
-                                                  const double *__restrict__ pkt, double *__restrict__ segmap,
-                                                  int *__restrict__ segflag)
+/* One lane per (segment, column): the composed maps of the long rays'
+ * segments (columns 0..15 the N map of the basis states, 16 its source
+ * term, 17 the unpolarized map).  Segments are given by their ray's packet
+ * offset, first step and the ray's step count. */
+__global__ __launch_bounds__(WAVE) void k_seg(int nseg, int seg_len, const long long *__restrict__ seg_off,
+                                              const int *__restrict__ seg_q0, const int *__restrict__ seg_ns,
+                                              const double *__restrict__ pkt, double *__restrict__ segmap,
+                                              int *__restrict__ segflag)
 {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nseg * SEGCOL) return;
   const int sgi = (int)(t / SEGCOL), b = (int)(t % SEGCOL);
   const int q0 = seg_q0[sgi], ns = seg_ns[sgi];
-  const int q1 = q0 + SEG_LEN < ns ? q0 + SEG_LEN : ns;
-  Herm N;
-  int fl = segment_column(pkt + (size_t)seg_off[sgi] * PK, q0, q1, b, N);
+  const int q1 = q0 + seg_len < ns ? q0 + seg_len : ns;
+  const double *pr = pkt + (size_t)seg_off[sgi] * PK;
   double *o = segmap + ((size_t)sgi * SEGCOL + b) * NPAR;
+  if (b == SEG_I) {
+    double u[NPAR];
+    segment_intensity(pr, q0, q1, u);
+#pragma unroll
+    for (int p = 0; p < NPAR; p++) o[p] = u[p];
+    return;
+  }
+  Herm N;
+  int fl = segment_column(pr, q0, q1, b, N);
 #pragma unroll
   for (int p = 0; p < NPAR; p++) o[p] = herm_par(N, p);
   if (b == NPAR) segflag[sgi] = fl;
@@ -723,12 +728,13 @@ __global__ __launch_bounds__(WAVE) void k_seg_off(int nseg, const long long *__r
 
 /* ------------------------------------------------------------------ C-ABI */
 
-constexpr int NEV = 10;
+constexpr int NEV = 24;
+constexpr int MAXPH = 4;   /* trace phases of the streamed render (the last unbounded) */
 
 struct ipgpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t stream2 = nullptr; /* finished rays of a two-phase render */
+  hipStream_t streamF[MAXPH - 1] = {}; /* transfer of the rays finished in each bounded phase */
   hipEvent_t ev[NEV] = {};
   char err[512] = {0};
   ipgpu_geom geom{};
@@ -749,9 +755,8 @@ struct ipgpu_ctx {
   size_t cap_off = 0;
   int *d_blk = nullptr;
   size_t cap_blk = 0;
-  int *d_segbase = nullptr, *d_segray = nullptr, *d_segq0 = nullptr, *d_segflag = nullptr;
-  double *d_segmap = nullptr;
-  size_t cap_segbase = 0, cap_segray = 0, cap_segq0 = 0, cap_segflag = 0, cap_segmap = 0;
+  int *d_segbase = nullptr;
+  size_t cap_segbase = 0;
   int *d_nstep_ray = nullptr, *d_flag_ray = nullptr;
   double *d_out7 = nullptr;
   double *d_image = nullptr, *d_imageS = nullptr, *d_taus = nullptr;
@@ -765,15 +770,18 @@ struct ipgpu_ctx {
   float ms_geo = 0, ms_xfer = 0, ms_total = 0, ms_stage[4] = {0, 0, 0, 0};
   int bad_i = -1, bad_j = -1, bad_job = -1;
   int traj_cap_steps = 8192;
-  /* two-phase render: step budget of the first trace pass (0: one pass) and
-   * the ray length above which a chain is composed from segments */
+  /* streamed render: step budget of the last bounded trace phase (0: one
+   * pass), the ray length above which a chain is composed from segments,
+   * the segment length */
   int trace_budget = 4096;
   int seg_min_steps = SEG_MIN_STEPS;
+  int seg_len = SEG_LEN;
   int *d_live = nullptr, *d_list = nullptr, *d_blk2 = nullptr;
   long long *d_offl = nullptr;
   size_t cap_live = 0, cap_list = 0, cap_blk2 = 0, cap_offl = 0;
   double *d_pkt2 = nullptr;
   size_t pkt2_doubles = 0;
+  long long last_total = 0;  /* packets of the previous render (arena size hint) */
   struct SegSet {
     long long *d_off = nullptr;
     int *d_q0 = nullptr, *d_ns = nullptr, *d_flag = nullptr;
@@ -781,13 +789,17 @@ struct ipgpu_ctx {
     size_t cap_off = 0, cap_q0 = 0, cap_ns = 0, cap_flag = 0, cap_map = 0;
     std::vector<long long> off;
     std::vector<int> q0, ns;
-  } segset[2];
+  } segset[MAXPH];
   /* host copies read by pending asynchronous copies: they live until the
    * render that filled them has synchronised */
-  std::vector<int> h_ns, h_ns2, h_live, h_list, h_base;
+  std::vector<int> h_ns, h_list, h_base;
   std::vector<long long> h_off;
-  float ms_phase[4] = {0, 0, 0, 0};
-  int two_phase = 0;
+  /* pinned host memory of the streamed render: per phase the step counts
+   * and live flags, then lists, packet offsets and segment bases */
+  void *h_pin = nullptr;
+  size_t h_pin_bytes = 0;
+  int streamed = 0;
+  int nphase = 0;
   /* batches */
   BGroup *d_groups = nullptr;
   size_t cap_groups = 0;
@@ -851,10 +863,11 @@ int ipgpu_create(ipgpu_ctx **out, int device)
     delete ctx;
     return IPGPU_ERR_HIP;
   }
-  if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess) {
-    delete ctx;
-    return IPGPU_ERR_HIP;
-  }
+  for (auto &st : ctx->streamF)
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      delete ctx;
+      return IPGPU_ERR_HIP;
+    }
   for (int q = 0; q < NEV; q++)
     if (hipEventCreate(&ctx->ev[q]) != hipSuccess) {
       delete ctx;
@@ -862,6 +875,7 @@ int ipgpu_create(ipgpu_ctx **out, int device)
     }
   if (const char *e = getenv("IPGPU_TRACE_BUDGET")) ctx->trace_budget = atoi(e);
   if (const char *e = getenv("IPGPU_SEG_MIN_STEPS")) ctx->seg_min_steps = atoi(e);
+  if (const char *e = getenv("IPGPU_SEG_LEN")) ctx->seg_len = atoi(e) > 1 ? atoi(e) : 1;
   if (hipMalloc((void **)&ctx->d_status, NSTATUS * sizeof(int)) != hipSuccess
       || hipMalloc((void **)&ctx->d_counts, 2 * sizeof(unsigned long long)) != hipSuccess) {
     delete ctx;
@@ -876,14 +890,15 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
   if (!ctx) return;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
-  if (ctx->stream2) hipStreamSynchronize(ctx->stream2);
+  for (auto st : ctx->streamF)
+    if (st) hipStreamSynchronize(st);
+  if (ctx->h_pin) hipHostFree(ctx->h_pin);
   for (auto &S : ctx->segset) {
     void *sp[] = {S.d_off, S.d_q0, S.d_ns, S.d_flag, S.d_map};
     for (void *p : sp)
       if (p) hipFree(p);
   }
-  void *ptrs[] = {ctx->d_live, ctx->d_list, ctx->d_blk2, ctx->d_offl, ctx->d_pkt2, ctx->d_dump, ctx->d_traj, ctx->d_pkt, ctx->d_aux, ctx->d_off, ctx->d_blk, ctx->d_segbase, ctx->d_segray,
-                  ctx->d_segq0, ctx->d_segflag, ctx->d_segmap, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
+  void *ptrs[] = {ctx->d_live, ctx->d_list, ctx->d_blk2, ctx->d_offl, ctx->d_pkt2, ctx->d_dump, ctx->d_traj, ctx->d_pkt, ctx->d_aux, ctx->d_off, ctx->d_blk, ctx->d_segbase, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
                   ctx->d_out7, ctx->d_image, ctx->d_imageS, ctx->d_taus, ctx->d_nstep_pix,
                   ctx->d_flag_pix, ctx->d_status, ctx->d_counts,
                   ctx->d_groups, ctx->d_jobs, ctx->d_items, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
@@ -894,7 +909,8 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
   for (int q = 0; q < NEV; q++)
     if (ctx->ev[q]) hipEventDestroy(ctx->ev[q]);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
-  if (ctx->stream2) hipStreamDestroy(ctx->stream2);
+  for (auto st : ctx->streamF)
+    if (st) hipStreamDestroy(st);
   delete ctx;
 }
 
@@ -1088,7 +1104,7 @@ static int prepare_packets(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, bool
 /* Segment tables of the rays of a list whose chains are composed (rays
  * longer than seg_min_steps); base[k] = first segment of element k or -1. */
 static void build_segments(ipgpu_ctx::SegSet &S, const int *list, int n, const long long *off,
-                           const std::vector<int> &ns, int seg_min, int maxnstep, int *base)
+                           const std::vector<int> &ns, int seg_min, int seg_len, int maxnstep, int *base)
 {
   S.off.clear();
   S.q0.clear();
@@ -1098,7 +1114,7 @@ static void build_segments(ipgpu_ctx::SegSet &S, const int *list, int n, const l
     base[k] = -1;
     if (v > seg_min && v < maxnstep - 1) {
       base[k] = (int)S.off.size();
-      for (int q0 = 0; q0 < v; q0 += SEG_LEN) {
+      for (int q0 = 0; q0 < v; q0 += seg_len) {
         S.off.push_back(off[k]);
         S.q0.push_back(q0);
         S.ns.push_back(v);
@@ -1125,14 +1141,12 @@ static int upload_segments(ipgpu_ctx *ctx, ipgpu_ctx::SegSet &S, hipStream_t st)
 /* Step packets, segment maps and chains of the n rays of a list (device
  * list dl, offsets doff, host offsets hoff), all on stream st. */
 static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, int n, const int *dl,
-                         const long long *doff, long long total, int *&blk, size_t &cap_blk, double *pkt,
-                         const int *dbase, ipgpu_ctx::SegSet &S, int cap, int stride)
+                         const long long *doff, long long total, int *blk, double *pkt, const int *dbase,
+                         ipgpu_ctx::SegSet &S, int cap, int stride)
 {
   if (n <= 0) return 0;
-  int rc;
   if (total > 0) {
-    unsigned nb = (unsigned)((total + PREP_BLOCK - 1) / PREP_BLOCK);
-    if ((rc = grow(ctx, blk, cap_blk, (size_t)nb))) return rc;
+    unsigned nb = (unsigned)((total + PREP_BLOCK - 1) / PREP_BLOCK);   /* blk holds nb entries */
     k_blockmap_list<<<(n + 255) / 256, 256, 0, st>>>(n, doff, 0, blk);
     HIPCHK(ctx, hipGetLastError());
     auto kp = P.transfer_fp32 ? k_prep_list<true> : k_prep_list<false>;
@@ -1143,130 +1157,185 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
   const int nseg = (int)S.off.size();
   if (nseg > 0) {
     long long lanes = (long long)nseg * SEGCOL;
-    k_seg_off<<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, st>>>(nseg, S.d_off, S.d_q0, S.d_ns, pkt,
-                                                                     S.d_map, S.d_flag);
+    k_seg<<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, st>>>(nseg, ctx->seg_len, S.d_off, S.d_q0, S.d_ns, pkt,
+                                                                 S.d_map, S.d_flag);
     HIPCHK(ctx, hipGetLastError());
   }
   k_chain_list<<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, dl, n, doff, cap, stride,
                                                       ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt,
-                                                      nseg > 0 ? dbase : nullptr, S.d_map, S.d_flag,
+                                                      nseg > 0 ? dbase : nullptr, ctx->seg_len, S.d_map, S.d_flag,
                                                       ctx->d_out7, ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
   HIPCHK(ctx, hipGetLastError());
   return 0;
 }
 
-/* Two-phase render of one image.  Phase 1 traces every ray for at most
- * trace_budget steps.  The rays that finished are prepared and chained on
- * stream2 while the paused rays resume on the main stream; those are
- * prepared, segmented and chained once their traces end, and the streams
- * join before k_finalize.  The arithmetic per ray is that of the one-pass
- * render (a resumed trace continues from its stored record, which holds X
- * and K exactly).  Sets fallback when a resumed ray outgrew the record
- * capacity; the capacity is then raised for a one-pass render. */
-static int render_two_phase(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int cap, bool &fallback)
+/* Streamed render of one image.  The geodesics are integrated in phases
+ * with growing step budgets (trace_budget/4, /2, trace_budget, then to the
+ * end), all queued at once on the main stream; each phase ends with a copy
+ * of the step counts and live flags to pinned host memory.  As soon as a
+ * bounded phase has ended, the rays it finished are prepared and chained on
+ * a stream of their own, concurrent with the later phases.  The rays alive
+ * after the last bounded phase (the longest) are prepared, segmented and
+ * chained on the main stream after their trace; the streams join before
+ * k_finalize.  The per-ray arithmetic is that of the one-pass render (a
+ * resumed trace continues from its stored record, which holds X and K
+ * exactly).  Sets fallback when a resumed ray outgrew the record capacity;
+ * the capacity is then raised for a one-pass render. */
+static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int cap, bool &fallback)
 {
   int rc;
   fallback = false;
   const int stride = P.only_unpolarized ? PK_UNPOL : PK;
   const bool segm = !P.only_unpolarized && !P.stokes_floors;
   const int seg_min = segm ? ctx->seg_min_steps : 0x7fffffff;
-  hipStream_t sA = ctx->stream, sB = ctx->stream2;
+  hipStream_t sA = ctx->stream;
   hipEvent_t *ev = ctx->ev;
+  /* budgets of the bounded phases */
+  int bud[MAXPH - 1], K = 0;
+  for (int d = 2; d >= 0; d--) {
+    int b = ctx->trace_budget >> d;
+    if (b >= 64 && (K == 0 || b > bud[K - 1])) bud[K++] = b;
+  }
+  if (K == 0) bud[K++] = ctx->trace_budget;
+  /* events: trace kernels [2k, 2k+1] (ev[0] opens phase 0), end of the
+   * finished-ray work of phase k 15+k, first finished-ray work 19, longest
+   * rays' work done 20, join 21, finalize 22 */
   if ((rc = grow(ctx, ctx->d_traj, ctx->traj_doubles, (size_t)nrays * cap * REC))) return rc;
   if ((rc = grow(ctx, ctx->d_live, ctx->cap_live, (size_t)nrays))) return rc;
   if ((rc = grow(ctx, ctx->d_list, ctx->cap_list, (size_t)nrays))) return rc;
-  if ((rc = grow(ctx, ctx->d_offl, ctx->cap_offl, (size_t)nrays + 2))) return rc;
+  if ((rc = grow(ctx, ctx->d_offl, ctx->cap_offl, (size_t)nrays + MAXPH + 1))) return rc;
   if ((rc = grow(ctx, ctx->d_segbase, ctx->cap_segbase, (size_t)nrays))) return rc;
-  static const int init[NSTATUS] = {0, 0x7fffffff, 0x7fffffff, 0};
+  if (ctx->last_total > 0) {
+    if ((rc = grow(ctx, ctx->d_pkt, ctx->pkt_doubles, (size_t)ctx->last_total * stride))) return rc;
+    if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)(ctx->last_total / PREP_BLOCK + 2 * MAXPH))))
+      return rc;
+  }
+  /* pinned, device-mapped, coherent host arena: the trace kernels write
+   * each phase's step counts and live flags into it (a copy command would
+   * hold the host until the phase ends) */
+  const size_t nI = (size_t)nrays;
+  const size_t need = sizeof(int) * nI * (2 * (MAXPH) + 2) + sizeof(long long) * (nI + MAXPH + 1);
+  if (ctx->h_pin_bytes < need) {
+    if (ctx->h_pin) hipHostFree(ctx->h_pin);
+    ctx->h_pin = nullptr;
+    ctx->h_pin_bytes = 0;
+    HIPCHK(ctx, hipHostMalloc(&ctx->h_pin, need, hipHostMallocMapped | hipHostMallocCoherent));
+    ctx->h_pin_bytes = need;
+  }
+  char *d_pin = nullptr;
+  HIPCHK(ctx, hipHostGetDevicePointer((void **)&d_pin, ctx->h_pin, 0));
+  long long *h_off = (long long *)ctx->h_pin;
+  const size_t ns_bytes = sizeof(long long) * (nI + MAXPH + 1);
+  int *h_ns = (int *)((char *)ctx->h_pin + ns_bytes); /* [K+1][nrays] */
+  int *h_live = h_ns + (size_t)MAXPH * nI;            /* [K][nrays] */
+  int *h_list = h_live + (size_t)MAXPH * nI;
+  int *h_base = h_list + nI;
+  int *dh_ns = (int *)(d_pin + ns_bytes), *dh_live = dh_ns + (size_t)MAXPH * nI;
 
-  /* phase 1 */
+  static const int init[NSTATUS] = {0, 0x7fffffff, 0x7fffffff, 0};
   HIPCHK(ctx, hipEventRecord(ev[0], sA));
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_status, init, sizeof(init), hipMemcpyHostToDevice, sA));
   HIPCHK(ctx, hipMemsetAsync(ctx->d_counts, 0, 2 * sizeof(unsigned long long), sA));
-  k_trace_budget<<<nrays / WAVE, WAVE, 0, sA>>>(ctx->geom, P, nrays, cap, ctx->trace_budget, ctx->d_traj,
-                                                 ctx->d_nstep_ray, ctx->d_live, ctx->d_counts);
-  HIPCHK(ctx, hipGetLastError());
-  HIPCHK(ctx, hipEventRecord(ev[1], sA));
-  std::vector<int> &ns = ctx->h_ns, &live = ctx->h_live, &list = ctx->h_list, &base = ctx->h_base;
-  std::vector<long long> &off = ctx->h_off;
-  ns.resize(nrays);
-  live.resize(nrays);
-  HIPCHK(ctx, hipMemcpyAsync(ns.data(), ctx->d_nstep_ray, nrays * sizeof(int), hipMemcpyDeviceToHost, sA));
-  HIPCHK(ctx, hipMemcpyAsync(live.data(), ctx->d_live, nrays * sizeof(int), hipMemcpyDeviceToHost, sA));
-  HIPCHK(ctx, hipStreamSynchronize(sA));
-
-  /* lists: finished rays [0, nF), paused rays [nF, nrays); packet offsets
-   * of each list from 0 (finished: d_pkt, paused: d_pkt2) */
-  list.clear();
-  for (int r = 0; r < nrays; r++)
-    if (!live[r]) list.push_back(r);
-  const int nF = (int)list.size(), nL = nrays - nF;
-  for (int r = 0; r < nrays; r++)
-    if (live[r]) list.push_back(r);
-  off.assign((size_t)nrays + 2, 0);
-  for (int k = 0; k < nF; k++) off[k + 1] = off[k] + ns[list[k]];
-  const long long T1 = off[nF];
-  base.assign(nrays, -1);
-  build_segments(ctx->segset[0], list.data(), nF, off.data(), ns, seg_min, P.maxnstep, base.data());
-  if ((rc = grow(ctx, ctx->d_pkt, ctx->pkt_doubles, (size_t)(T1 > 0 ? T1 : 1) * stride))) return rc;
-
-  /* phase 2 (main stream): the paused rays resume */
-  HIPCHK(ctx, hipMemcpyAsync(ctx->d_list, list.data(), nrays * sizeof(int), hipMemcpyHostToDevice, sA));
-  HIPCHK(ctx, hipEventRecord(ev[2], sA));
-  if (nL > 0) {
-    k_trace_resume<<<(nL + WAVE - 1) / WAVE, WAVE, 0, sA>>>(ctx->geom, P, ctx->d_list + nF, nL, cap, ctx->d_traj,
-                                                            ctx->d_nstep_ray, ctx->d_status, ctx->d_counts);
+  /* all trace phases, queued at once; phase k writes the step counts (and
+   * live flags) of the rays it integrated to the host arena's row k (the
+   * live-flag row of the last, unbounded phase is scratch) */
+  for (int k = 0; k <= K; k++) {
+    if (k > 0) HIPCHK(ctx, hipEventRecord(ev[2 * k], sA));
+    int *hn = dh_ns + (size_t)k * nI, *hl = dh_live + (size_t)(k < K ? k : MAXPH - 1) * nI;
+    if (k == 0)
+      k_trace_budget<<<nrays / WAVE, WAVE, 0, sA>>>(ctx->geom, P, nrays, cap, bud[0], ctx->d_traj, ctx->d_nstep_ray,
+                                                     ctx->d_live, hn, hl, ctx->d_counts);
+    else
+      k_trace_resume_mask<<<nrays / WAVE, WAVE, 0, sA>>>(ctx->geom, P, nrays, cap, k < K ? bud[k] : -1,
+                                                          ctx->d_traj, ctx->d_nstep_ray, ctx->d_live, hn, hl,
+                                                          ctx->d_counts);
     HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(ev[2 * k + 1], sA));
   }
-  HIPCHK(ctx, hipEventRecord(ev[3], sA));
 
-  /* the finished rays (second stream), after the list upload */
-  HIPCHK(ctx, hipStreamWaitEvent(sB, ev[2], 0));
-  HIPCHK(ctx, hipMemcpyAsync(ctx->d_offl, off.data(), (nF + 1) * sizeof(long long), hipMemcpyHostToDevice, sB));
-  if (nF > 0)
-    HIPCHK(ctx, hipMemcpyAsync(ctx->d_segbase, base.data(), nF * sizeof(int), hipMemcpyHostToDevice, sB));
-  if ((rc = upload_segments(ctx, ctx->segset[0], sB))) return rc;
-  HIPCHK(ctx, hipEventRecord(ev[8], sB));
-  if ((rc = transfer_list(ctx, P, sB, nF, ctx->d_list, ctx->d_offl, T1, ctx->d_blk, ctx->cap_blk, ctx->d_pkt,
-                          ctx->d_segbase, ctx->segset[0], cap, stride)))
-    return rc;
-  HIPCHK(ctx, hipEventRecord(ev[5], sB));
-
-  /* the paused rays once traced */
-  int st0 = 0;
-  std::vector<int> &ns2 = ctx->h_ns2;
-  ns2.resize(nrays);
-  if (nL > 0) {
-    HIPCHK(ctx, hipMemcpyAsync(ns2.data(), ctx->d_nstep_ray, nrays * sizeof(int), hipMemcpyDeviceToHost, sA));
-    HIPCHK(ctx, hipMemcpyAsync(&st0, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, sA));
-    HIPCHK(ctx, hipStreamSynchronize(sA));
+  /* the rays of each phase as it ends: list region [start, start + n),
+   * offsets at start + k, packets bump-allocated in d_pkt (phases up to K-1)
+   * or d_pkt2 (the last) */
+  std::vector<int> nsv(nI);
+  int start = 0;
+  long long pofs = 0, bofs = 0, total = 0;
+  bool any_f = false;
+  for (int k = 0; k <= K; k++) {
+    HIPCHK(ctx, hipEventSynchronize(ev[2 * k + 1]));
+    const volatile int *nsk = h_ns + (size_t)k * nI;
+    int n = 0;
+    for (int r = 0; r < nrays; r++) {
+      /* rays integrated in phase k (all in phase 0, else those live after
+       * phase k - 1) that it finished */
+      if (k > 0 && !((volatile int *)h_live)[(size_t)(k - 1) * nI + r]) continue;
+      if (k < K && ((volatile int *)h_live)[(size_t)k * nI + r]) continue;
+      if (start + n >= nrays) return set_err(ctx, IPGPU_ERR_STATE, "streamed render: inconsistent phase lists");
+      h_list[start + n++] = r;
+      nsv[r] = nsk[r];
+    }
+    if (k == K) {
+      /* a ray past the record capacity: one pass with room for it */
+      int mx = 0;
+      for (int q = 0; q < n; q++) mx = nsv[h_list[start + q]] > mx ? nsv[h_list[start + q]] : mx;
+      if (mx >= cap) {
+        ctx->traj_cap_steps = mx + 1 < P.maxnstep ? mx + 1 : P.maxnstep;
+        for (int q = 0; q < K; q++) HIPCHK(ctx, hipStreamSynchronize(ctx->streamF[q]));
+        fallback = true;
+        return 0;
+      }
+    }
+    long long *off = h_off + start + k;
+    off[0] = 0;
+    for (int q = 0; q < n; q++) off[q + 1] = off[q] + nsv[h_list[start + q]];
+    const long long T = off[n];
+    total += T;
+    build_segments(ctx->segset[k], h_list + start, n, off, nsv, seg_min, ctx->seg_len, P.maxnstep, h_base + start);
+    hipStream_t st = (k < K) ? ctx->streamF[k] : sA;
+    int *dl = ctx->d_list + start, *dbase = ctx->d_segbase + start;
+    long long *doff = ctx->d_offl + start + k;
+    double *pkt;
+    int *blk;
+    if (n > 0) {
+      if (k < K) {
+        const size_t nb = (size_t)((T + PREP_BLOCK - 1) / PREP_BLOCK);
+        if ((size_t)(pofs + T) * stride > ctx->pkt_doubles || bofs + nb > ctx->cap_blk) {
+          /* the arenas are full: earlier phases finish first */
+          for (int q = 0; q < k; q++) HIPCHK(ctx, hipStreamSynchronize(ctx->streamF[q]));
+          pofs = bofs = 0;
+          const size_t want = (size_t)(T > ctx->last_total ? T : ctx->last_total);
+          if ((rc = grow(ctx, ctx->d_pkt, ctx->pkt_doubles, (want > 0 ? want : 1) * stride))) return rc;
+          if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, want / PREP_BLOCK + 2 * MAXPH))) return rc;
+        }
+        pkt = ctx->d_pkt + (size_t)pofs * stride;
+        blk = ctx->d_blk + bofs;
+        pofs += T;
+        bofs += nb;
+        if (!any_f) HIPCHK(ctx, hipEventRecord(ev[19], st));
+        any_f = true;
+      } else {
+        if ((rc = grow(ctx, ctx->d_pkt2, ctx->pkt2_doubles, (size_t)(T > 0 ? T : 1) * stride))) return rc;
+        const size_t nb = (size_t)((T + PREP_BLOCK - 1) / PREP_BLOCK);
+        if ((rc = grow(ctx, ctx->d_blk2, ctx->cap_blk2, nb > 0 ? nb : 1))) return rc;
+        pkt = ctx->d_pkt2;
+        blk = ctx->d_blk2;
+      }
+      HIPCHK(ctx, hipMemcpyAsync(dl, h_list + start, n * sizeof(int), hipMemcpyHostToDevice, st));
+      HIPCHK(ctx, hipMemcpyAsync(doff, off, (n + 1) * sizeof(long long), hipMemcpyHostToDevice, st));
+      if (!ctx->segset[k].off.empty())
+        HIPCHK(ctx, hipMemcpyAsync(dbase, h_base + start, n * sizeof(int), hipMemcpyHostToDevice, st));
+      if ((rc = upload_segments(ctx, ctx->segset[k], st))) return rc;
+      if ((rc = transfer_list(ctx, P, st, n, dl, doff, T, blk, pkt, dbase, ctx->segset[k], cap, stride)))
+        return rc;
+    }
+    if (k < K) HIPCHK(ctx, hipEventRecord(ev[15 + k], st));
+    start += n;
   }
-  if (st0 > 0) {
-    int mx = 0;
-    for (int k = nF; k < nrays; k++) mx = ns2[list[k]] > mx ? ns2[list[k]] : mx;
-    ctx->traj_cap_steps = mx + 1 < P.maxnstep ? mx + 1 : P.maxnstep;
-    HIPCHK(ctx, hipStreamSynchronize(sB));
-    fallback = true;
-    return 0;
-  }
-  long long *offL = off.data() + nF + 1;
-  for (int k = 0; k < nL; k++) offL[k + 1] = offL[k] + ns2[list[nF + k]];
-  const long long T2 = offL[nL];
-  build_segments(ctx->segset[1], list.data() + nF, nL, offL, ns2, seg_min, P.maxnstep, base.data() + nF);
-  if (nL > 0) {
-    if ((rc = grow(ctx, ctx->d_pkt2, ctx->pkt2_doubles, (size_t)(T2 > 0 ? T2 : 1) * stride))) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(ctx->d_offl + nF + 1, offL, (nL + 1) * sizeof(long long), hipMemcpyHostToDevice, sA));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->d_segbase + nF, base.data() + nF, nL * sizeof(int), hipMemcpyHostToDevice, sA));
-    if ((rc = upload_segments(ctx, ctx->segset[1], sA))) return rc;
-    if ((rc = transfer_list(ctx, P, sA, nL, ctx->d_list + nF, ctx->d_offl + nF + 1, T2, ctx->d_blk2, ctx->cap_blk2,
-                            ctx->d_pkt2, ctx->d_segbase + nF, ctx->segset[1], cap, stride)))
-      return rc;
-  } else {
-    ctx->segset[1].off.clear();
-  }
-  HIPCHK(ctx, hipEventRecord(ev[4], sA));
-  HIPCHK(ctx, hipStreamWaitEvent(sA, ev[5], 0));
-  HIPCHK(ctx, hipEventRecord(ev[6], sA));
+  if (!any_f) HIPCHK(ctx, hipEventRecord(ev[19], sA));
+  ctx->last_total = total;
+  HIPCHK(ctx, hipEventRecord(ev[20], sA));
+  for (int k = 0; k < K; k++) HIPCHK(ctx, hipStreamWaitEvent(sA, ev[15 + k], 0));
+  HIPCHK(ctx, hipEventRecord(ev[21], sA));
+  ctx->nphase = K;
   return 0;
 }
 
@@ -1298,53 +1367,50 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
   int cap = ctx->traj_cap_steps < P.maxnstep ? ctx->traj_cap_steps : P.maxnstep;
   const bool two = P.target_nturns < 0 && ctx->trace_budget > 0 && cap > ctx->trace_budget + 1;
   bool fallback = false;
-  if (two && (rc = render_two_phase(ctx, P, nrays, cap, fallback))) return rc;
+  if (two && (rc = render_streamed(ctx, P, nrays, cap, fallback))) return rc;
   const bool one = !two || fallback;
   if (one) {
     int stride;
     unsigned long long total;
     if ((rc = prepare_packets(ctx, P, nrays, false, cap, total, stride))) return rc;
     /* long rays of a polarized image: their chains run as composed segments */
-    int nseg = 0;
+    ipgpu_ctx::SegSet &S = ctx->segset[0];
+    S.off.clear();
     if (stride == PK && !P.stokes_floors) {
-      std::vector<int> ns(nrays), base(nrays, -1), sray, sq0;
+      std::vector<int> &ns = ctx->h_ns, &list = ctx->h_list, &base = ctx->h_base;
+      std::vector<long long> &off = ctx->h_off;
+      ns.resize(nrays);
       HIPCHK(ctx, hipMemcpyAsync(ns.data(), ctx->d_nstep_ray, nrays * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
       HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-      for (int r = 0; r < nrays; r++)
-        if (ns[r] > ctx->seg_min_steps && ns[r] < P.maxnstep - 1) {
-          base[r] = (int)sray.size();
-          for (int q0 = 0; q0 < ns[r]; q0 += SEG_LEN) {
-            sray.push_back(r);
-            sq0.push_back(q0);
-          }
-        }
-      nseg = (int)sray.size();
-      if (nseg > 0) {
+      list.resize(nrays);
+      off.assign((size_t)nrays + 1, 0);
+      base.assign(nrays, -1);
+      for (int r = 0; r < nrays; r++) {
+        list[r] = r;
+        off[r + 1] = off[r] + ns[r];   /* the packet offsets k_scan forms */
+      }
+      build_segments(S, list.data(), nrays, off.data(), ns, ctx->seg_min_steps, ctx->seg_len, P.maxnstep,
+                     base.data());
+      if (!S.off.empty()) {
         if ((rc = grow(ctx, ctx->d_segbase, ctx->cap_segbase, (size_t)nrays))) return rc;
-        if ((rc = grow(ctx, ctx->d_segray, ctx->cap_segray, (size_t)nseg))) return rc;
-        if ((rc = grow(ctx, ctx->d_segq0, ctx->cap_segq0, (size_t)nseg))) return rc;
-        if ((rc = grow(ctx, ctx->d_segflag, ctx->cap_segflag, (size_t)nseg))) return rc;
-        if ((rc = grow(ctx, ctx->d_segmap, ctx->cap_segmap, (size_t)nseg * SEGCOL * NPAR))) return rc;
         HIPCHK(ctx, hipMemcpyAsync(ctx->d_segbase, base.data(), nrays * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(ctx, hipMemcpyAsync(ctx->d_segray, sray.data(), nseg * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(ctx, hipMemcpyAsync(ctx->d_segq0, sq0.data(), nseg * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-        long long lanes = (long long)nseg * SEGCOL;
-        k_seg<<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>(nseg, ctx->d_segray, ctx->d_segq0,
-                                                                               ctx->d_off, ctx->d_nstep_ray, ctx->d_pkt,
-                                                                               ctx->d_segmap, ctx->d_segflag);
+        if ((rc = upload_segments(ctx, S, ctx->stream))) return rc;
+        long long lanes = (long long)S.off.size() * SEGCOL;
+        k_seg<<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>((int)S.off.size(), ctx->seg_len, S.d_off,
+                                                                               S.d_q0, S.d_ns, ctx->d_pkt, S.d_map,
+                                                                               S.d_flag);
         HIPCHK(ctx, hipGetLastError());
-        /* the host vectors above must outlive their copies */
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
       }
     }
+    const bool segd = !S.off.empty();
     k_chain<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, ctx->d_off,
                                                      ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, ctx->d_pkt,
-                                                     nseg > 0 ? ctx->d_segbase : nullptr, ctx->d_segmap,
-                                                     ctx->d_segflag, ctx->d_out7, ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
+                                                     segd ? ctx->d_segbase : nullptr, ctx->seg_len, S.d_map,
+                                                     S.d_flag, ctx->d_out7, ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
   }
-  const int e1 = one ? 4 : 7;
+  const int e1 = one ? 4 : 22;
   double f3 = pow(P.freqcgs, 3);
   k_finalize<<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(P, nrays, f3, ctx->d_out7, ctx->d_flag_ray,
                                                            ctx->d_nstep_ray, ctx->d_image, ctx->d_imageS,
@@ -1362,23 +1428,29 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
     ctx->ms_geo = ctx->ms_stage[0];
     ctx->ms_xfer = ctx->ms_stage[1] + ctx->ms_stage[2];
   } else {
-    /* [0] both trace launches, [1] the finished rays' transfer on the second
-     * stream, [2] the paused rays' transfer after their trace, [3] finalize */
-    float a, b;
-    hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
-    hipEventElapsedTime(&b, ctx->ev[2], ctx->ev[3]);
-    ctx->ms_stage[0] = a + b;
-    hipEventElapsedTime(&ctx->ms_stage[1], ctx->ev[8], ctx->ev[5]);
-    hipEventElapsedTime(&ctx->ms_stage[2], ctx->ev[3], ctx->ev[6]);
-    hipEventElapsedTime(&ctx->ms_stage[3], ctx->ev[6], ctx->ev[7]);
-    hipEventElapsedTime(&ctx->ms_geo, ctx->ev[0], ctx->ev[3]);
-    hipEventElapsedTime(&ctx->ms_xfer, ctx->ev[3], ctx->ev[6]);
-    ctx->ms_phase[0] = a;
-    ctx->ms_phase[1] = b;
-    hipEventElapsedTime(&ctx->ms_phase[2], ctx->ev[3], ctx->ev[4]);
-    hipEventElapsedTime(&ctx->ms_phase[3], ctx->ev[0], ctx->ev[5]);
+    /* [0] the trace launches, [1] the span of the finished rays' transfer
+     * (their own streams, concurrent with later trace phases), [2] what
+     * follows the last trace launch until the streams have joined (the
+     * longest rays' transfer, the wait), [3] finalize */
+    const int K = ctx->nphase;
+    float a, tr = 0.f;
+    for (int k = 0; k <= K; k++) {
+      hipEventElapsedTime(&a, ctx->ev[k == 0 ? 0 : 2 * k], ctx->ev[2 * k + 1]);
+      tr += a;
+    }
+    ctx->ms_stage[0] = tr;
+    float f1 = 0.f;
+    for (int k = 0; k < K; k++) {
+      hipEventElapsedTime(&a, ctx->ev[19], ctx->ev[15 + k]);
+      f1 = a > f1 ? a : f1;
+    }
+    ctx->ms_stage[1] = f1;
+    hipEventElapsedTime(&ctx->ms_stage[2], ctx->ev[2 * K + 1], ctx->ev[21]);
+    hipEventElapsedTime(&ctx->ms_stage[3], ctx->ev[21], ctx->ev[22]);
+    hipEventElapsedTime(&ctx->ms_geo, ctx->ev[0], ctx->ev[2 * K + 1]);
+    ctx->ms_xfer = ctx->ms_stage[2];
   }
-  ctx->two_phase = !one;
+  ctx->streamed = !one;
   hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[e1]);
   ctx->last_npix = (int)npix;
   ctx->last_unpol = P.only_unpolarized;
@@ -1815,14 +1887,15 @@ int ipgpu_stage_times(const ipgpu_ctx *ctx, double ms[4])
 {
   if (!ctx || !ms) return IPGPU_ERR_ARG;
   for (int q = 0; q < 4; q++) ms[q] = ctx->ms_stage[q];
-  return ctx->two_phase;
+  return ctx->streamed;
 }
 
-int ipgpu_set_tuning(ipgpu_ctx *ctx, int trace_budget, int seg_min_steps, int record_steps)
+int ipgpu_set_tuning(ipgpu_ctx *ctx, int trace_budget, int seg_min_steps, int seg_len, int record_steps)
 {
   if (!ctx) return IPGPU_ERR_ARG;
   if (trace_budget >= 0) ctx->trace_budget = trace_budget;
   if (seg_min_steps >= 0) ctx->seg_min_steps = seg_min_steps;
+  if (seg_len >= 0) ctx->seg_len = seg_len > 1 ? seg_len : 1;
   if (record_steps >= 0) ctx->traj_cap_steps = record_steps > 2 ? record_steps : 2;
   return IPGPU_OK;
 }

@@ -100,7 +100,7 @@ def lib():
         L.ipgpu_scale_density.argtypes = [vp, dp]
         L.ipgpu_kernel_times.argtypes = [vp, dp, dp, dp]
         L.ipgpu_stage_times.argtypes = [vp, dp]
-        L.ipgpu_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.ipgpu_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.ipgpu_step_counts.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.ipgpu_stream.argtypes = [vp]
         L.ipgpu_stream.restype = vp
@@ -372,23 +372,24 @@ class Context:
 
     def stage_times(self):
         """ms of the last render's stages: k_trace, k_scan + k_prep,
-        k_seg + k_chain, k_finalize after a one-pass render; both trace
-        launches, the finished rays' transfer (second stream), the tail after
-        the resumed trace, k_finalize after a two-phase one."""
+        k_seg + k_chain, k_finalize after a one-pass render; the trace
+        launches, the span of the finished rays' transfer, the tail after the
+        last trace launch, k_finalize after a streamed one."""
         return self.stage_times_mode()[0]
 
     def stage_times_mode(self):
-        """(stage_times(), True if the last render was two-phase)."""
+        """(stage_times(), True if the last render was streamed)."""
         ms = (C.c_double * 4)()
         rc = self.L.ipgpu_stage_times(self.h, ms)
         self._chk(min(rc, 0))
         return tuple(ms), rc == 1
 
-    def set_tuning(self, trace_budget=-1, seg_min_steps=-1, record_steps=-1):
-        """Two-phase trace budget (0 = one pass), the ray length above which
-        polarized transfer is composed from segments, and the geodesic
-        records held per ray; -1 keeps a setting."""
-        self._chk(self.L.ipgpu_set_tuning(self.h, int(trace_budget), int(seg_min_steps), int(record_steps)))
+    def set_tuning(self, trace_budget=-1, seg_min_steps=-1, seg_len=-1, record_steps=-1):
+        """Streamed-render trace budget (0 = one pass), the ray length above which
+        polarized transfer is composed from segments, the segment length and
+        the geodesic records held per ray; -1 keeps a setting."""
+        self._chk(self.L.ipgpu_set_tuning(self.h, int(trace_budget), int(seg_min_steps), int(seg_len),
+                                          int(record_steps)))
 
     def step_counts(self):
         a, b = C.c_longlong(), C.c_longlong()

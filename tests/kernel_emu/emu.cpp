@@ -232,10 +232,11 @@ int emu_adjoint(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P,
   return status;
 }
 
-/* Every ray through the segmented chain (composed segment maps; on the GPU
- * only rays longer than SEG_MIN_STEPS take this path). */
+/* Every ray through the segmented chain (composed segment maps of seg_len
+ * steps; on the GPU only rays longer than the segment threshold take this
+ * path). */
 int emu_render_segmented(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, const double *dump,
-                         const int *pix, int npix, double *out7, int *nstep, int *flag)
+                         const int *pix, int npix, int seg_len, double *out7, int *nstep, int *flag)
 {
   int status = 0;
 #pragma omp parallel
@@ -257,21 +258,27 @@ int emu_render_segmented(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_
       for (int q = 0; q < ns; q++)
         prep_step<false>(*G, *M, *P, ns - q, rec.data(), nullptr, dump, pkt.data() + (size_t)q * PK, true,
                          unit_scale());
-      const int nseg = (ns + SEG_LEN - 1) / SEG_LEN;
+      const int nseg = (ns + seg_len - 1) / seg_len;
       segmap.assign((size_t)(nseg > 0 ? nseg : 1) * SEGCOL * NPAR, 0.);
       segflag.assign(nseg > 0 ? nseg : 1, 0);
-      for (int sg = 0; sg < nseg; sg++)
+      for (int sg = 0; sg < nseg; sg++) {
+        const int q0 = sg * seg_len, q1 = (sg + 1) * seg_len < ns ? (sg + 1) * seg_len : ns;
         for (int b = 0; b < SEGCOL; b++) {
+          double *o = segmap.data() + ((size_t)sg * SEGCOL + b) * NPAR;
+          if (b == SEG_I) {
+            segment_intensity(pkt.data(), q0, q1, o);
+            continue;
+          }
           Herm N;
-          int q1 = (sg + 1) * SEG_LEN < ns ? (sg + 1) * SEG_LEN : ns;
-          int fl = segment_column(pkt.data(), sg * SEG_LEN, q1, b, N);
-          for (int p = 0; p < NPAR; p++) segmap[((size_t)sg * SEGCOL + b) * NPAR + p] = herm_par(N, p);
+          int fl = segment_column(pkt.data(), q0, q1, b, N);
+          for (int p = 0; p < NPAR; p++) o[p] = herm_par(N, p);
           if (b == NPAR) segflag[sg] = fl;
         }
+      }
       double js, ks;
       far_end_coefficients<false>(*G, *M, *P, ns, rec.data(), dump, js, ks, unit_scale());
       int nrad;
-      flag[n] = chain_ray_segmented(*G, *P, ns, pkt.data(), segmap.data(), segflag.data(), js, ks,
+      flag[n] = chain_ray_segmented(*G, *P, ns, seg_len, segmap.data(), segflag.data(), js, ks,
                                     out7 + 7 * (size_t)n, nrad);
     }
   }

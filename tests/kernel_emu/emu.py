@@ -33,7 +33,7 @@ def lib():
         L.emu_adjoint.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), dp, dp, dp, dp,
                                   dp, dp]
         L.emu_render_segmented.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), dp, ip,
-                                           C.c_int, dp, ip, ip]
+                                           C.c_int, C.c_int, dp, ip, ip]
         L.emu_trace.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Image), C.c_int, C.c_int, dp]
         _lib = L
     return _lib
@@ -103,13 +103,13 @@ def render(geom, model, image, dump, pix=None):
     return rc, out7, ns, fl
 
 
-def render_segmented(geom, model, image, dump):
+def render_segmented(geom, model, image, dump, seg_len=256):
     """Every ray through the segmented (composed-map) chain."""
     pix = np.array([(i, j) for i in range(image.nx) for j in range(image.ny)], np.int32)
     n = len(pix)
     out7, ns, fl = np.zeros((n, 7)), np.zeros(n, np.int32), np.zeros(n, np.int32)
     rc = lib().emu_render_segmented(C.byref(geom), C.byref(model), C.byref(image), G.P(f64(dump)), G.IP(pix),
-                                    n, G.P(out7), G.IP(ns), G.IP(fl))
+                                    n, seg_len, G.P(out7), G.IP(ns), G.IP(fl))
     return rc, out7, ns, fl
 
 

@@ -1,6 +1,7 @@
-"""The two-phase render (first geodesic pass with a step budget, finished
-rays transferred on a second stream while the paused rays resume) against
-the one-pass render of the same image on the GPU: the per-ray arithmetic is
+"""The streamed render (geodesic phases with growing step budgets; the rays
+each bounded phase finishes are transferred on a stream of their own while
+the others resume; the longest rays last) against the one-pass render of
+the same image on the GPU: the per-ray arithmetic is
 the same, so every output array must be identical, whatever the budget or
 the record capacity (a resumed ray that outgrows it sends the render back
 to one pass with a larger capacity), given the same segment threshold (a
@@ -48,9 +49,9 @@ def same(a, b):
 
 
 @pytest.mark.parametrize("unpol", [0, 1])
-def test_two_phase_equals_one_pass(case, unpol):
+def test_streamed_equals_one_pass(case, unpol):
     h, prims, orc = case
-    say("case two-phase vs one-pass unpol=%d 32x32 th=163" % unpol)
+    say("case streamed vs one-pass unpol=%d 32x32 th=163" % unpol)
     c1, g, m = fresh(h, prims, trace_budget=0)
     im, scale, _ = G.make_image(g, m.L_unit, nx=32, ny=32, thetacam=163., only_unpolarized=unpol)
     refs = {}
@@ -70,11 +71,11 @@ def test_two_phase_equals_one_pass(case, unpol):
         c2.close()
 
 
-def test_two_phase_vs_oracle(case):
+def test_streamed_vs_oracle(case):
     """Small budget (most rays resume) and a low segment threshold (most
     chains composed), at the parity bar against the oracle."""
     h, prims, orc = case
-    say("case two-phase budget 64 seg_min 64 vs oracle 32x32 th=60")
+    say("case streamed budget 64 seg_min 64 vs oracle 32x32 th=60")
     orc.activate()
     p = O.default_params(nx=32, ny=32, thetacam=60., freqcgs=230e9)
     ref = orc.render(p)
@@ -86,11 +87,11 @@ def test_two_phase_vs_oracle(case):
     c.close()
 
 
-def test_two_phase_record_overflow(case):
+def test_streamed_record_overflow(case):
     """Resumed rays longer than the record capacity: the render retraces in
     one pass with room for them and returns the one-pass result."""
     h, prims, orc = case
-    say("case two-phase record overflow 24x24")
+    say("case streamed record overflow 24x24")
     c1, g, m = fresh(h, prims, trace_budget=0)
     im, _, _ = G.make_image(g, m.L_unit, nx=24, ny=24, thetacam=163.)
     ref = c1.render(im)
@@ -99,17 +100,17 @@ def test_two_phase_record_overflow(case):
     c2, _, _ = fresh(h, prims, trace_budget=50, record_steps=200)
     got = c2.render(im)
     same(ref, got)
-    # the capacity was raised: the next render is two-phase throughout
+    # the capacity was raised: the next render is streamed throughout
     got2 = c2.render(im)
     assert c2.stage_times_mode()[1] is True
     same(ref, got2)
     c2.close()
 
 
-def test_two_phase_maxnstep(case):
+def test_streamed_maxnstep(case):
     """A resumed ray past maxnstep is reported with the one-pass pixel."""
     h, prims, orc = case
-    say("case two-phase maxnstep 8x8")
+    say("case streamed maxnstep 8x8")
     errs = []
     for budget in (0, 20):
         c, g, m = fresh(h, prims, trace_budget=budget)
@@ -120,3 +121,23 @@ def test_two_phase_maxnstep(case):
         errs.append(str(e.value))
         c.close()
     assert errs[0] == errs[1]
+
+
+def test_streamed_context_reuse(case):
+    """One context rendering different images in turn (sizes, cameras,
+    budgets): each result equals a one-pass render of the same image.  The
+    per-phase host rows and device arenas of a render must not leak into the
+    next."""
+    h, prims, orc = case
+    say("case streamed context reuse")
+    c, g, m = fresh(h, prims, trace_budget=256, seg_min_steps=400)
+    ref_ctx, _, _ = fresh(h, prims, trace_budget=0, seg_min_steps=400)
+    for nx, th, budget in ((32, 163., 256), (24, 60., 256), (40, 130., 512), (16, 17., 300), (32, 163., 128)):
+        say("  %dx%d th=%g budget %d" % (nx, nx, th, budget))
+        c.set_tuning(trace_budget=budget)
+        im, _, _ = G.make_image(g, m.L_unit, nx=nx, ny=nx, thetacam=th)
+        got = c.render(im)
+        assert c.stage_times_mode()[1] is True
+        same(ref_ctx.render(im), got)
+    c.close()
+    ref_ctx.close()
