@@ -392,13 +392,19 @@ IPG_HD bool stop_backward(const ipgpu_geom &G, double r, const double K[4])
 
 /* First half of push_photon (src/geodesics.c:15-31): from (X, K) at the
  * start of a step of length dlp (negative when tracing backward). */
+IPG_HD_INLINE void half_step_acc(const double acc[4], const double X[4], const double K[4], double dlp,
+                                  double Xh[4], double Kh[4])
+{
+  for (int k = 0; k < 4; k++) Kh[k] = K[k] - 0.5 * dlp * acc[k];
+  for (int i = 0; i < 4; i++) Xh[i] = X[i] + 0.5 * dlp * K[i];
+}
+
 IPG_HD void half_step(const Metric &m0, const double X[4], const double K[4], double dlp,
                       double Xh[4], double Kh[4])
 {
   double acc[4];
   geodesic_accel(m0, K, acc);
-  for (int k = 0; k < 4; k++) Kh[k] = K[k] - 0.5 * dlp * acc[k];
-  for (int i = 0; i < 4; i++) Xh[i] = X[i] + 0.5 * dlp * K[i];
+  half_step_acc(acc, X, K, dlp, Xh, Kh);
 }
 
 /* --------------------------------------------------------------- tetrads */

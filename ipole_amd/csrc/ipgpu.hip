@@ -48,6 +48,105 @@ __device__ __forceinline__ void ray_pixel(int ray, int nx, int ny, int &i, int &
   j = tj * TILE + l % TILE;
 }
 
+/* The two lanes (2k, 2k+1) of a lane pair exchange a value (DPP quad
+ * permutation [1, 0, 3, 2] on both 32-bit halves). */
+__device__ __forceinline__ double pair_swap(double v)
+{
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, 0xB1, 0xF, 0xF, false);
+  const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, 0xB1, 0xF, 0xF, false);
+  return __hiloint2double(hi2, lo2);
+}
+
+/* trace_ray_t on a lane pair.  Both lanes hold the same X, K and take the
+ * same decisions; the two metric evaluations of a step run one per lane
+ * (the even lane at the half-step point, the odd lane at the step's end
+ * point, which is the next step's start), and the accelerations each lane
+ * forms from its metric are exchanged.  Half the instructions per lane of
+ * the one-lane step, with the same operations on the same values, so the
+ * records are bit-identical.  Each lane stores half of a record. */
+template <int MK>
+__device__ __forceinline__ int trace_pair_t(const ipgpu_geom &G, const ipgpu_image &P, double X[4], double K[4],
+                                            int nstep, int cap, double *rec, int nend, int *finished, bool odd)
+{
+  Metric m;
+  metric_k<MK>(G, X[1], X[2], m); /* both lanes: the start point */
+  int done = 1;
+  while (true) {
+    /* r at X: the odd lane's metric (both lanes' on the first pass) */
+    const double r_o = pair_swap(m.r);
+    const double rX = odd ? m.r : r_o;
+    if (stop_backward(G, rX, K)) break;
+    if (nstep >= P.maxnstep - 1) break;
+    if (nstep >= nend) {
+      done = 0;
+      break;
+    }
+    nstep++;
+    double dl = stepsize(G, X, K, P.eps);
+    double dlp = -dl;
+    double a0[4];
+    geodesic_accel(m, K, a0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const double o = pair_swap(a0[k]);
+      a0[k] = odd ? a0[k] : o;
+    }
+    double Xh[4], Kh[4], Xn[4];
+    half_step_acc(a0, X, K, dlp, Xh, Kh);
+    for (int k = 0; k < 4; k++) Xn[k] = X[k] + dlp * Kh[k];
+    metric_k<MK>(G, odd ? Xn[1] : Xh[1], odd ? Xn[2] : Xh[2], m);
+    double acc[4];
+    geodesic_accel(m, Kh, acc);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const double o = pair_swap(acc[k]);
+      acc[k] = odd ? o : acc[k];
+    }
+    for (int k = 0; k < 4; k++) K[k] = K[k] - dlp * acc[k];
+    for (int k = 0; k < 4; k++) X[k] = Xn[k];
+    if (nstep < cap) {
+      /* fields 0-3 from the even lane, 4-7 from the odd one */
+      double *o = rec + (size_t)nstep * REC + (odd ? 4 : 0);
+      o[0] = odd ? K[1] : X[1];
+      o[1] = odd ? K[2] : X[2];
+      o[2] = odd ? K[3] : X[3];
+      o[3] = odd ? dl : K[0];
+    }
+  }
+  if (finished) *finished = done;
+  return nstep;
+}
+
+/* trace_ray on a lane pair: the camera start or the resumption is formed by
+ * both lanes alike. */
+__device__ __forceinline__ int trace_pair(const ipgpu_geom &G, const ipgpu_image &P, int pi, int pj, int cap,
+                                          double *rec, int n0, int nend, int *finished, bool odd)
+{
+  double X[4], K[4];
+  if (n0 > 0) {
+    Rec r;
+    load_rec(rec, n0, r);
+    X[0] = 0.;
+    for (int k = 1; k < 4; k++) X[k] = r.X[k];
+    for (int k = 0; k < 4; k++) K[k] = r.K[k];
+  } else {
+    double E[4][4];
+    Metric m;
+    for (int k = 0; k < 4; k++) X[k] = P.Xcam[k];
+    camera_tetrad(G, X, P.old_centering, E, m);
+    init_K(P, E, pi, pj, K);
+    if (cap > 0 && !odd) store_rec(rec, X, K, 0.);
+    n0 = 0;
+  }
+  switch (metric_kind(G)) {
+  case MK_MINK: return trace_pair_t<MK_MINK>(G, P, X, K, n0, cap, rec, nend, finished, odd);
+  case MK_MKS: return trace_pair_t<MK_MKS>(G, P, X, K, n0, cap, rec, nend, finished, odd);
+  case MK_FMKS_INT: return trace_pair_t<MK_FMKS_INT>(G, P, X, K, n0, cap, rec, nend, finished, odd);
+  default: return trace_pair_t<MK_FMKS>(G, P, X, K, n0, cap, rec, nend, finished, odd);
+  }
+}
+
 /* the loss's component weights, by value */
 struct AdjW {
   double w[4];
@@ -69,6 +168,30 @@ __global__ __launch_bounds__(WAVE) void k_trace(ipgpu_geom G, ipgpu_image P, int
   double *rec = traj + (size_t)ray * cap * REC;
   int *nt = nturns_buf ? nturns_buf + (size_t)ray * cap : nullptr;
   int nstep = trace_ray(G, P, pi, pj, cap, rec, nt);
+  nstep_out[ray] = nstep;
+  if (nstep >= cap) atomicAdd(&status[0], 1);
+  atomicAdd(&counts[0], (unsigned long long)nstep);
+}
+
+/* k_trace without subring counts on lane pairs (trace_pair): the geodesics
+ * of the one-pass render are then those of the streamed render, bit for
+ * bit. */
+__global__ __launch_bounds__(WAVE) void k_trace_pair(ipgpu_geom G, ipgpu_image P, int nrays, int cap,
+                                                     double *__restrict__ traj, int *__restrict__ nstep_out,
+                                                     int *__restrict__ status, unsigned long long *__restrict__ counts)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ray = t >> 1;
+  const bool odd = t & 1;
+  if (ray >= nrays) return;
+  int pi, pj;
+  ray_pixel(ray, P.nx, P.ny, pi, pj);
+  if (pi >= P.nx || pj >= P.ny) {
+    if (!odd) nstep_out[ray] = 0;
+    return;
+  }
+  int nstep = trace_pair(G, P, pi, pj, cap, traj + (size_t)ray * cap * REC, 0, 0x7fffffff, nullptr, odd);
+  if (odd) return;
   nstep_out[ray] = nstep;
   if (nstep >= cap) atomicAdd(&status[0], 1);
   atomicAdd(&counts[0], (unsigned long long)nstep);
@@ -574,17 +697,21 @@ __global__ __launch_bounds__(WAVE) void k_trace_budget(ipgpu_geom G, ipgpu_image
                                                        int *__restrict__ host_live,
                                                        unsigned long long *__restrict__ counts)
 {
-  int ray = blockIdx.x * blockDim.x + threadIdx.x;
+  /* one lane pair per ray */
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ray = t >> 1;
+  const bool odd = t & 1;
   if (ray >= nrays) return;
   int pi, pj;
   ray_pixel(ray, P.nx, P.ny, pi, pj);
   int nstep = 0, lv = 0;
   if (pi < P.nx && pj < P.ny) {
     int fin = 1;
-    nstep = trace_ray(G, P, pi, pj, cap, traj + (size_t)ray * cap * REC, nullptr, 0, nend, &fin);
+    nstep = trace_pair(G, P, pi, pj, cap, traj + (size_t)ray * cap * REC, 0, nend, &fin, odd);
     lv = fin ? 0 : 1;
-    atomicAdd(&counts[0], (unsigned long long)nstep);
+    if (!odd) atomicAdd(&counts[0], (unsigned long long)nstep);
   }
+  if (odd) return;
   nstep_out[ray] = nstep;
   live[ray] = lv;
   host_ns[ray] = nstep;
@@ -601,16 +728,20 @@ __global__ __launch_bounds__(WAVE) void k_trace_resume_mask(ipgpu_geom G, ipgpu_
                                                             int *__restrict__ host_ns, int *__restrict__ host_live,
                                                             unsigned long long *__restrict__ counts)
 {
-  int ray = blockIdx.x * blockDim.x + threadIdx.x;
+  /* one lane pair per ray */
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ray = t >> 1;
+  const bool odd = t & 1;
   if (ray >= nrays) return;
   if (!live[ray]) {
-    host_live[ray] = 0;   /* the host row is complete for every ray */
+    if (!odd) host_live[ray] = 0;   /* the host row is complete for every ray */
     return;
   }
   const int n0 = nstep_io[ray];
   int fin = 1;
-  int nstep = trace_ray(G, P, 0, 0, cap, traj + (size_t)ray * cap * REC, nullptr, n0,
-                        nend < 0 ? 0x7fffffff : nend, &fin);
+  int nstep = trace_pair(G, P, 0, 0, cap, traj + (size_t)ray * cap * REC, n0, nend < 0 ? 0x7fffffff : nend, &fin,
+                         odd);
+  if (odd) return;
   nstep_io[ray] = nstep;
   live[ray] = fin ? 0 : 1;
   host_ns[ray] = nstep;
@@ -1039,9 +1170,12 @@ static int run_trace(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int cap, b
   }
   HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
   HIPCHK(ctx, hipMemsetAsync(ctx->d_counts, 0, 2 * sizeof(unsigned long long), ctx->stream));
-  k_trace<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, P, nrays, cap, ctx->d_traj,
-                                                   ctx->d_nstep_ray, turns ? ctx->d_nturns : nullptr,
-                                                   ctx->d_status, ctx->d_counts);
+  if (turns)
+    k_trace<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, P, nrays, cap, ctx->d_traj, ctx->d_nstep_ray,
+                                                     ctx->d_nturns, ctx->d_status, ctx->d_counts);
+  else
+    k_trace_pair<<<2 * nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, P, nrays, cap, ctx->d_traj,
+                                                             ctx->d_nstep_ray, ctx->d_status, ctx->d_counts);
   HIPCHK(ctx, hipGetLastError());
   return 0;
 }
@@ -1243,10 +1377,10 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
     if (k > 0) HIPCHK(ctx, hipEventRecord(ev[2 * k], sA));
     int *hn = dh_ns + (size_t)k * nI, *hl = dh_live + (size_t)(k < K ? k : MAXPH - 1) * nI;
     if (k == 0)
-      k_trace_budget<<<nrays / WAVE, WAVE, 0, sA>>>(ctx->geom, P, nrays, cap, bud[0], ctx->d_traj, ctx->d_nstep_ray,
-                                                     ctx->d_live, hn, hl, ctx->d_counts);
+      k_trace_budget<<<2 * nrays / WAVE, WAVE, 0, sA>>>(ctx->geom, P, nrays, cap, bud[0], ctx->d_traj,
+                                                         ctx->d_nstep_ray, ctx->d_live, hn, hl, ctx->d_counts);
     else
-      k_trace_resume_mask<<<nrays / WAVE, WAVE, 0, sA>>>(ctx->geom, P, nrays, cap, k < K ? bud[k] : -1,
+      k_trace_resume_mask<<<2 * nrays / WAVE, WAVE, 0, sA>>>(ctx->geom, P, nrays, cap, k < K ? bud[k] : -1,
                                                           ctx->d_traj, ctx->d_nstep_ray, ctx->d_live, hn, hl,
                                                           ctx->d_counts);
     HIPCHK(ctx, hipGetLastError());
