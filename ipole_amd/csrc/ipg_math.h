@@ -47,6 +47,7 @@ inline double exp_(double x) { return exp(x); }
 inline void sincos_(double x, double &s, double &c) { sincos(x, &s, &c); }
 inline void sincospi_(double y, double &s, double &c) { sincos(3.141592653589793 * y, &s, &c); }
 inline double exp_r(double x) { return exp(x); }
+inline double exp_neg(double x) { return exp(x); }
 inline void sincos_r(double x, double &s, double &c) { sincos(x, &s, &c); }
 inline void sincospi_r(double y, double &s, double &c) { sincos(3.141592653589793 * y, &s, &c); }
 inline double log_(double x) { return log(x); }
@@ -106,6 +107,13 @@ IPG_MATH_HD double exp_(double x)
 {
   if (!(fabs(x) < 700.)) return exp(x);
   return exp_r(x);
+}
+
+/* e^x for x <= 0 without a branch: arguments below -800 (whose e^x is 0)
+ * are clamped there, NaN propagates; the same value as exp_ on [-700, 0]. */
+IPG_MATH_HD double exp_neg(double x)
+{
+  return exp_r(x != x ? x : fmax(x, -800.));
 }
 
 /* sin r and cos r for |r| <= pi/4 */

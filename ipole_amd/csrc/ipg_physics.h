@@ -1227,7 +1227,7 @@ IPG_HD double approximate_solve(double Ii, double ji, double ki, double jf, doub
   double dtau = dl * kavg;
   tau += dtau;
   if (dtau < 1.e-3) return Ii + (javg - Ii * kavg) * dl * (1. - (dtau * 0.5) * (1. - dtau * (1. / 3.)));
-  double efac = exp_(-dtau);
+  double efac = exp_neg(-dtau);
   return Ii * efac + (javg * rcp(kavg)) * (1. - efac);
 }
 

@@ -20,6 +20,14 @@ namespace ipg {
 
 constexpr int REC = 8;
 
+/* keeps the scheduler from moving instructions across this point (bounds
+ * the registers a long unrolled sequence holds) */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define IPG_SCHED_BARRIER() __builtin_amdgcn_sched_barrier(0)
+#else
+#define IPG_SCHED_BARRIER() ((void)0)
+#endif
+
 struct Rec {
   double X[4], K[4], dl;
 };
@@ -221,11 +229,22 @@ constexpr int AX = 16;
 constexpr int AX_NU = 0, AX_STH = 1, AX_CTH = 2, AX_EDGE = 3, AX_NE = 4, AX_TH = 5, AX_B = 6, AX_CELL = 7,
               AX_D1 = 8, AX_D2 = 9, AX_D3 = 10, AX_SMOOTH = 11, AX_DRH = 12, AX_DRL = 13, AX_ZERO = 14;
 
-template <bool F32 = false>
+/* Packet fields are written through PkW: field f of the packet at p is
+ * p[f * PS] (PS = 1: one packet contiguous; PS = 64: the packets of the 64
+ * rays of a wave interleaved field by field, so that a wave's load or store
+ * of one field is 512 contiguous bytes). */
+template <int PS>
+struct PkW {
+  double *p;
+  IPG_HD double &operator[](int f) const { return p[(size_t)f * PS]; }
+};
+
+template <bool F32 = false, int PS = 1>
 IPG_HD_INLINE void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int n,
-                      const double *rec, const int *nt, const double *dump, double *pk, bool pol,
+                      const double *rec, const int *nt, const double *dump, double *pk_out, bool pol,
                       const Scale &sc, double *ax = nullptr)
 {
+  PkW<PS> pk{pk_out};
   if (ax)
     for (int q = 0; q < AX; q++) ax[q] = 0.;
   Rec ti, tf;
@@ -362,6 +381,14 @@ struct Pkt {
   double v[PK];
 };
 
+/* A packet stored with field stride PS (see PkW). */
+template <int PS>
+IPG_HD_INLINE void load_pkt_s(const double *p, Pkt &k)
+{
+#pragma unroll
+  for (int i = 0; i < PK; i++) k.v[i] = p[(size_t)i * PS];
+}
+
 IPG_HD void load_pkt(const double *p, int n, Pkt &k)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -486,7 +513,8 @@ IPG_HD int chain_finish(const ipgpu_geom &G, const ipgpu_image &P, const Chain &
 /* The state recurrence over the packets of one ray held in memory (packet
  * 0 is step ns, packet ns-1 is step 1), then project_N.  On the device the
  * next packet is loaded while the current step computes. */
-IPG_HD int chain_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const double *pkt, int stride,
+template <int PS = 1>
+IPG_HD_INLINE int chain_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const double *pkt, int stride,
                      double js, double ks, double out7[7], int &nrad)
 {
   const bool pol = stride == PK;
@@ -494,14 +522,26 @@ IPG_HD int chain_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const do
   chain_init(st, js, ks);
   if (pol) {
     Pkt cur, nxt;
-    if (ns > 0) load_pkt(pkt, PK, cur);
+    if constexpr (PS == 1) {
+      if (ns > 0) load_pkt(pkt, PK, cur);
+    } else {
+      if (ns > 0) load_pkt_s<PS>(pkt, cur);
+    }
     for (int q = 0; q < ns; q++) {
-      if (q + 1 < ns) load_pkt(pkt + (size_t)(q + 1) * PK, PK, nxt);
+      if (q + 1 < ns) {
+        if constexpr (PS == 1) load_pkt(pkt + (size_t)(q + 1) * PK, PK, nxt);
+        else load_pkt_s<PS>(pkt + (size_t)(q + 1) * PK * PS, nxt);
+      }
       chain_step(st, cur.v, true, P.stokes_floors);
       cur = nxt;
     }
   } else {
-    for (int q = 0; q < ns; q++) chain_step(st, pkt + (size_t)q * stride, false, P.stokes_floors);
+    for (int q = 0; q < ns; q++) {
+      const double *p = pkt + (size_t)q * stride * PS;
+      double u[PK_UNPOL];
+      for (int f = 0; f < PK_UNPOL; f++) u[f] = p[(size_t)f * PS];
+      chain_step(st, u, false, P.stokes_floors);
+    }
   }
   nrad = st.nrad;
   return chain_finish(G, P, st, out7);
@@ -546,6 +586,7 @@ IPG_HD double herm_par(const Herm &N, int b)
 /* Column b of the map of steps [q0, q1): applied to basis state b (b < 16,
  * no sources) or to N = 0 with sources (b = 16).  Returns the steps'
  * tetrad oddflag bits. */
+template <int PS = 1>
 IPG_HD int segment_column(const double *pkt, int q0, int q1, int b, Herm &N)
 {
   /* the basis state with constant indices only (a runtime index into N
@@ -560,27 +601,34 @@ IPG_HD int segment_column(const double *pkt, int q0, int q1, int b, Herm &N)
   const bool src = (b == NPAR);
   int flag = 0;
   for (int q = q0; q < q1; q++) {
-    const double *pk = pkt + (size_t)q * PK;
+    const double *pk = pkt + (size_t)q * PK * PS;
+    auto F = [pk](int f) { return pk[(size_t)f * PS]; };
     if (q > 0) {
-      const double *pp = pkt + (size_t)(q - 1) * PK;
+      const double *pp = pkt + (size_t)(q - 1) * PK * PS;
       double Mi[4][4], Mh[4][4];
       for (int i = 0; i < 4; i++)
         for (int k = 0; k < 4; k++) {
-          Mi[i][k] = pp[PK_MF + 4 * i + k];
-          Mh[i][k] = pk[PK_MH + 4 * i + k];
+          Mi[i][k] = pp[(size_t)(PK_MF + 4 * i + k) * PS];
+          Mh[i][k] = F(PK_MH + 4 * i + k);
         }
       Herm Nh;
-      transport(N, N, Mi, pk[PK_HDL], Nh);
-      transport(N, Nh, Mh, pk[PK_FDL], N);
+      transport(N, N, Mi, F(PK_HDL), Nh);
+      transport(N, Nh, Mh, F(PK_FDL), N);
     }
-    const int meta = (int)pk[PK_META];
+    const int meta = (int)F(PK_META);
     if (meta & 1) {
-      double S0[4], dS[4];
-      herm_to_stokes(N, pk + PK_e1, pk + PK_e2, S0);
+      double S0[4], dS[4], e1[4], e2[4], E1[4], E2[4];
+      for (int i = 0; i < 4; i++) {
+        e1[i] = F(PK_e1 + i);
+        e2[i] = F(PK_e2 + i);
+        E1[i] = F(PK_E1 + i);
+        E2[i] = F(PK_E2 + i);
+      }
+      herm_to_stokes(N, e1, e2, S0);
       for (int i = 0; i < 4; i++)
-        dS[i] = (src ? pk[PK_CV + i] : 0.) + pk[PK_T + 4 * i] * S0[0] + pk[PK_T + 4 * i + 1] * S0[1]
-            + pk[PK_T + 4 * i + 2] * S0[2] + pk[PK_T + 4 * i + 3] * S0[3] - S0[i];
-      herm_add_stokes(N, pk + PK_E1, pk + PK_E2, dS);
+        dS[i] = (src ? F(PK_CV + i) : 0.) + F(PK_T + 4 * i) * S0[0] + F(PK_T + 4 * i + 1) * S0[1]
+            + F(PK_T + 4 * i + 2) * S0[2] + F(PK_T + 4 * i + 3) * S0[3] - S0[i];
+      herm_add_stokes(N, E1, E2, dS);
       flag |= meta >> 2;
     }
   }
@@ -602,7 +650,7 @@ IPG_HD void approximate_solve_affine(double ji, double ki, double jf, double kf,
     b = javg * f;
     return;
   }
-  double efac = exp_(-dtau);
+  double efac = exp_neg(-dtau);
   a = efac;
   b = (javg * rcp(kavg)) * (1. - efac);
 }
@@ -611,25 +659,27 @@ IPG_HD void approximate_solve_affine(double ji, double ki, double jf, double kf,
  * recurrence caches the emissivity and absorptivity of the last radiating
  * step (src/ipolarray.c:140-146); within the segment that cache is local
  * except at its first radiating step. */
+template <int PS = 1>
 IPG_HD void segment_intensity(const double *pkt, int q0, int q1, double *o)
 {
   double A = 1., B = 0., tau = 0., tauF = 0., js = 0., ks = 0.;
   int nrad = 0;
   for (int q = 0; q < NPAR; q++) o[q] = 0.;
   for (int q = q0; q < q1; q++) {
-    const double *pk = pkt + (size_t)q * PK;
-    const int meta = (int)pk[PK_META];
+    const double *pk = pkt + (size_t)q * PK * PS;
+    auto F = [pk](int f) { return pk[(size_t)f * PS]; };
+    const int meta = (int)F(PK_META);
     if (meta & 1) {
-      const double ju = pk[PK_JU], ku = pk[PK_KU];
-      tauF += pk[PK_DTF];
+      const double ju = F(PK_JU), ku = F(PK_KU);
+      tauF += F(PK_DTF);
       if (nrad == 0) {
         o[SI_M2] = (meta & 2) ? 1. : 0.;
         o[SI_J1] = ju;
         o[SI_K1] = ku;
-        o[SI_DL1] = pk[PK_DL];
+        o[SI_DL1] = F(PK_DL);
       } else {
         double a, b;
-        approximate_solve_affine((meta & 2) ? 0. : js, ks, (meta & 2) ? 0. : ju, ku, pk[PK_DL], tau, a, b);
+        approximate_solve_affine((meta & 2) ? 0. : js, ks, (meta & 2) ? 0. : ju, ku, F(PK_DL), tau, a, b);
         A = a * A;
         B = a * B + b;
       }
@@ -663,16 +713,21 @@ IPG_HD int chain_ray_segmented(const ipgpu_geom &G, const ipgpu_image &P, int ns
   const int nseg = (ns + seg_len - 1) / seg_len;
   for (int sgi = 0; sgi < nseg; sgi++) {
     const double *A = segmap + (size_t)sgi * SEGCOL * NPAR;
-    double x[NPAR];
+    double x[NPAR], acc[NPAR];
 #pragma unroll
     for (int b = 0; b < NPAR; b++) x[b] = herm_par(N, b);
 #pragma unroll
-    for (int p = 0; p < NPAR; p++) {
-      double acc = A[NPAR * NPAR + p];
+    for (int p = 0; p < NPAR; p++) acc[p] = A[NPAR * NPAR + p];
+    /* column by column (acc[p] sums in the order b = 0..15), one column's
+     * loads at a time so that the map is never held in registers whole */
 #pragma unroll
-      for (int b = 0; b < NPAR; b++) acc += A[b * NPAR + p] * x[b];
-      herm_par(N, p) = acc;
+    for (int b = 0; b < NPAR; b++) {
+#pragma unroll
+      for (int p = 0; p < NPAR; p++) acc[p] += A[b * NPAR + p] * x[b];
+      IPG_SCHED_BARRIER();
     }
+#pragma unroll
+    for (int p = 0; p < NPAR; p++) herm_par(N, p) = acc[p];
     const double *u = A + SEG_I * NPAR;
     const int nr = (int)u[SI_NRAD];
     if (nr > 0) {

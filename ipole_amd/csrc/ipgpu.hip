@@ -263,7 +263,11 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep(ipgpu_geom G, ipgpu_model M
                  aux ? aux + (size_t)g * AX : nullptr);
 }
 
-/* One lane per ray: the recurrence of N and I over the ray's packets. */
+/* One lane per ray: the recurrence of N and I over the ray's packets
+ * (SEG = false: the rays with seg_base < 0) or the composition of its
+ * segment maps (SEG = true: the rays with seg_base >= 0).  Two instances,
+ * so that neither carries the other's registers. */
+template <bool SEG>
 __global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P, int nrays,
                                                 int cap, int stride, const long long *__restrict__ off,
                                                 const int *__restrict__ nstep_in,
@@ -279,6 +283,8 @@ __global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ip
 {
   int ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= nrays) return;
+  const int sb = seg_base ? seg_base[ray] : -1;
+  if ((sb >= 0) != SEG) return;
   int pi, pj;
   ray_pixel(ray, P.nx, P.ny, pi, pj);
   if (pi >= P.nx || pj >= P.ny) return;
@@ -292,12 +298,17 @@ __global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ip
   }
   const double *rec = traj + (size_t)ray * cap * REC;
   double js, ks;
-  if (P.transfer_fp32) far_end_coefficients<true>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
-  else far_end_coefficients<false>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
+  if constexpr (SEG) {
+    if (P.transfer_fp32) far_end_coefficients<true>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
+    else far_end_coefficients<false>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
+    IPG_SCHED_BARRIER();
+  } else {
+    if (P.transfer_fp32) far_end_coefficients<true>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
+    else far_end_coefficients<false>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
+  }
   double res[7];
   int nrad;
-  const int sb = seg_base ? seg_base[ray] : -1;
-  if (sb >= 0)
+  if constexpr (SEG)
     flags[ray] = chain_ray_segmented(G, P, ns, seg_len, segmap + (size_t)sb * SEGCOL * NPAR, segflag + sb, js, ks,
                                      res, nrad);
   else
@@ -749,43 +760,59 @@ __global__ __launch_bounds__(WAVE) void k_trace_resume_mask(ipgpu_geom G, ipgpu_
   atomicAdd(&counts[0], (unsigned long long)(nstep - n0));
 }
 
-/* Block -> list element of a packet range starting at g0 (offL monotonic). */
-__global__ void k_blockmap_list(int n, const long long *__restrict__ offL, long long g0, int *__restrict__ blk)
+/* Packets of a list of rays, wave-interleaved: the list is sorted by step
+ * count, longest first, and cut into waves of 64 elements; wave c owns
+ * rows W[c] .. W[c+1] (its first element's step count), and field f of step
+ * q of its lane l is at pkt[((W[c] + q) * stride + f) * 64 + l].  A wave of
+ * the chain kernel then loads one field of its 64 rays' packets as 512
+ * contiguous bytes, and a wave of the packet kernel stores it so. */
+constexpr int ILV = WAVE;
+
+/* Block (4 rows of 64 lanes) -> wave of the list whose rows contain the
+ * block's first row. */
+__global__ void k_blockmap_rows(int nw, const long long *__restrict__ W, int *__restrict__ blk)
 {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  long long b0 = (offL[k] - g0 + PREP_BLOCK - 1) / PREP_BLOCK, b1 = (offL[k + 1] - g0 + PREP_BLOCK - 1) / PREP_BLOCK;
-  for (long long b = b0; b < b1; b++) blk[b] = k;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nw) return;
+  constexpr int RPB = PREP_BLOCK / ILV;
+  long long b0 = (W[c] + RPB - 1) / RPB, b1 = (W[c + 1] + RPB - 1) / RPB;
+  for (long long b = b0; b < b1; b++) blk[b] = c;
 }
 
-/* Step packets of the rays of a list (packets offL[k] .. offL[k+1] of ray
- * list[k]), one lane per step. */
+/* Step packets of a list: one lane per (row, lane) slot; slots past a ray's
+ * step count are padding and do nothing. */
 template <bool F32>
 __global__ __launch_bounds__(PREP_BLOCK) void k_prep_list(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
                                                           const int *__restrict__ list, int n,
-                                                          const long long *__restrict__ offL, long long g0,
-                                                          long long total, const int *__restrict__ blk, int cap,
-                                                          int stride, const int *__restrict__ nstep_in,
+                                                          const long long *__restrict__ W, int nw, long long rows,
+                                                          const int *__restrict__ blk, int cap, int stride,
+                                                          const int *__restrict__ nstep_in,
                                                           const double *__restrict__ traj,
                                                           const double *__restrict__ dump, double *__restrict__ pkt)
 {
-  long long g = g0 + (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
-  if (g >= g0 + total) return;
-  int k = blk[blockIdx.x];
-  while (k + 1 < n && offL[k + 1] <= g) k++;
+  const long long t = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
+  const long long row = t / ILV;
+  const int l = (int)(t % ILV);
+  if (row >= rows) return;
+  int c = blk[blockIdx.x];
+  while (c + 1 < nw && W[c + 1] <= row) c++;
+  const int k = c * ILV + l;
+  if (k >= n) return;
   const int ray = list[k];
   const int ns = nstep_in[ray];
-  if (ns >= P.maxnstep - 1) return;
-  const int q = (int)(g - offL[k]);
-  prep_step<F32>(G, Mo, P, ns - q, traj + (size_t)ray * cap * REC, nullptr, dump, pkt + (size_t)g * stride,
-                 stride == PK, unit_scale());
+  const int q = (int)(row - W[c]);
+  if (q >= ns || ns >= P.maxnstep - 1) return;
+  prep_step<F32, ILV>(G, Mo, P, ns - q, traj + (size_t)ray * cap * REC, nullptr, dump,
+                      pkt + (size_t)(W[c] + q) * stride * ILV + l, stride == PK, unit_scale());
 }
 
-/* The chains of the rays of a list, one lane per ray (segmented where
- * seg_base[k] >= 0). */
+/* The chains of the rays of a list, one lane per ray: SEG = false the rays
+ * with seg_base[k] < 0 (recurrence over the packets), SEG = true those with
+ * seg_base[k] >= 0 (composition of the segment maps). */
+template <bool SEG>
 __global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
                                                      const int *__restrict__ list, int n,
-                                                     const long long *__restrict__ offL, int cap, int stride,
+                                                     const long long *__restrict__ W, int cap, int stride,
                                                      const int *__restrict__ nstep_in,
                                                      const double *__restrict__ traj,
                                                      const double *__restrict__ dump, const double *__restrict__ pkt,
@@ -797,6 +824,8 @@ __global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model M
 {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
+  const int sb = seg_base ? seg_base[k] : -1;
+  if ((sb >= 0) != SEG) return;
   const int ray = list[k];
   int pi, pj;
   ray_pixel(ray, P.nx, P.ny, pi, pj);
@@ -811,24 +840,32 @@ __global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model M
   }
   const double *rec = traj + (size_t)ray * cap * REC;
   double js, ks;
-  if (P.transfer_fp32) far_end_coefficients<true>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
-  else far_end_coefficients<false>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
+  if constexpr (SEG) {
+    if (P.transfer_fp32) far_end_coefficients<true>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
+    else far_end_coefficients<false>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
+    IPG_SCHED_BARRIER();
+  } else {
+    if (P.transfer_fp32) far_end_coefficients<true>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
+    else far_end_coefficients<false>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
+  }
   double res[7];
   int nrad;
-  const int sb = seg_base ? seg_base[k] : -1;
-  if (sb >= 0)
+  if constexpr (SEG)
     flags[ray] = chain_ray_segmented(G, P, ns, seg_len, segmap + (size_t)sb * SEGCOL * NPAR, segflag + sb, js, ks,
                                      res, nrad);
   else
-    flags[ray] = chain_ray(G, P, ns, pkt + (size_t)offL[k] * stride, stride, js, ks, res, nrad);
+    flags[ray] = chain_ray<ILV>(G, P, ns, pkt + (size_t)W[k / ILV] * stride * ILV + (k % ILV), stride, js, ks,
+                                res, nrad);
   for (int q = 0; q < 7; q++) o[q] = res[q];
   atomicAdd(&counts[1], (unsigned long long)nrad);
 }
 
 /* One lane per (segment, column): the composed maps of the long rays'
  * segments (columns 0..15 the N map of the basis states, 16 its source
- * term, 17 the unpolarized map).  Segments are given by their ray's packet
- * offset, first step and the ray's step count. */
+ * term, 17 the unpolarized map).  Segments are given by the position of
+ * their ray's first packet (in doubles), their first step and the ray's
+ * step count; PS is the packets' field stride. */
+template <int PS>
 __global__ __launch_bounds__(WAVE) void k_seg(int nseg, int seg_len, const long long *__restrict__ seg_off,
                                               const int *__restrict__ seg_q0, const int *__restrict__ seg_ns,
                                               const double *__restrict__ pkt, double *__restrict__ segmap,
@@ -839,17 +876,17 @@ __global__ __launch_bounds__(WAVE) void k_seg(int nseg, int seg_len, const long 
   const int sgi = (int)(t / SEGCOL), b = (int)(t % SEGCOL);
   const int q0 = seg_q0[sgi], ns = seg_ns[sgi];
   const int q1 = q0 + seg_len < ns ? q0 + seg_len : ns;
-  const double *pr = pkt + (size_t)seg_off[sgi] * PK;
+  const double *pr = pkt + seg_off[sgi];
   double *o = segmap + ((size_t)sgi * SEGCOL + b) * NPAR;
   if (b == SEG_I) {
     double u[NPAR];
-    segment_intensity(pr, q0, q1, u);
+    segment_intensity<PS>(pr, q0, q1, u);
 #pragma unroll
     for (int p = 0; p < NPAR; p++) o[p] = u[p];
     return;
   }
   Herm N;
-  int fl = segment_column(pr, q0, q1, b, N);
+  int fl = segment_column<PS>(pr, q0, q1, b, N);
 #pragma unroll
   for (int p = 0; p < NPAR; p++) o[p] = herm_par(N, p);
   if (b == NPAR) segflag[sgi] = fl;
@@ -923,8 +960,8 @@ struct ipgpu_ctx {
   } segset[MAXPH];
   /* host copies read by pending asynchronous copies: they live until the
    * render that filled them has synchronised */
-  std::vector<int> h_ns, h_list, h_base;
-  std::vector<long long> h_off;
+  std::vector<int> h_ns, h_list, h_base, h_cnt, h_tmp;
+  std::vector<long long> h_off, h_eoff;
   /* pinned host memory of the streamed render: per phase the step counts
    * and live flags, then lists, packet offsets and segment bases */
   void *h_pin = nullptr;
@@ -1236,7 +1273,8 @@ static int prepare_packets(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, bool
 }
 
 /* Segment tables of the rays of a list whose chains are composed (rays
- * longer than seg_min_steps); base[k] = first segment of element k or -1. */
+ * longer than seg_min_steps); off[k]: position of element k's first packet
+ * in doubles; base[k] = first segment of element k or -1. */
 static void build_segments(ipgpu_ctx::SegSet &S, const int *list, int n, const long long *off,
                            const std::vector<int> &ns, int seg_min, int seg_len, int maxnstep, int *base)
 {
@@ -1273,34 +1311,58 @@ static int upload_segments(ipgpu_ctx *ctx, ipgpu_ctx::SegSet &S, hipStream_t st)
 }
 
 /* Step packets, segment maps and chains of the n rays of a list (device
- * list dl, offsets doff, host offsets hoff), all on stream st. */
+ * list dl sorted longest first, wave rows W with nw + 1 entries, rows in
+ * all), on stream st; packets wave-interleaved at pkt, block map at blk. */
 static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, int n, const int *dl,
-                         const long long *doff, long long total, int *blk, double *pkt, const int *dbase,
+                         const long long *W, int nw, long long rows, int *blk, double *pkt, const int *dbase,
                          ipgpu_ctx::SegSet &S, int cap, int stride)
 {
   if (n <= 0) return 0;
-  if (total > 0) {
-    unsigned nb = (unsigned)((total + PREP_BLOCK - 1) / PREP_BLOCK);   /* blk holds nb entries */
-    k_blockmap_list<<<(n + 255) / 256, 256, 0, st>>>(n, doff, 0, blk);
+  if (rows > 0) {
+    const unsigned nb = (unsigned)((rows * ILV + PREP_BLOCK - 1) / PREP_BLOCK);   /* blk holds nb entries */
+    k_blockmap_rows<<<(nw + 255) / 256, 256, 0, st>>>(nw, W, blk);
     HIPCHK(ctx, hipGetLastError());
     auto kp = P.transfer_fp32 ? k_prep_list<true> : k_prep_list<false>;
-    kp<<<nb, PREP_BLOCK, 0, st>>>(ctx->geom, ctx->model, P, dl, n, doff, 0, total, blk, cap, stride,
-                                  ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt);
+    kp<<<nb, PREP_BLOCK, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, nw, rows, blk, cap, stride, ctx->d_nstep_ray,
+                                  ctx->d_traj, ctx->d_dump, pkt);
     HIPCHK(ctx, hipGetLastError());
   }
   const int nseg = (int)S.off.size();
   if (nseg > 0) {
     long long lanes = (long long)nseg * SEGCOL;
-    k_seg<<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, st>>>(nseg, ctx->seg_len, S.d_off, S.d_q0, S.d_ns, pkt,
-                                                                 S.d_map, S.d_flag);
+    k_seg<ILV><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, st>>>(nseg, ctx->seg_len, S.d_off, S.d_q0, S.d_ns,
+                                                                      pkt, S.d_map, S.d_flag);
     HIPCHK(ctx, hipGetLastError());
   }
-  k_chain_list<<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, dl, n, doff, cap, stride,
-                                                      ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt,
-                                                      nseg > 0 ? dbase : nullptr, ctx->seg_len, S.d_map, S.d_flag,
-                                                      ctx->d_out7, ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
+  /* composed rays (the longest, at the front of the list) and the rest */
+  if (nseg > 0) {
+    k_chain_list<true><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, cap, stride,
+                                                              ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt, dbase,
+                                                              ctx->seg_len, S.d_map, S.d_flag, ctx->d_out7,
+                                                              ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
+    HIPCHK(ctx, hipGetLastError());
+  }
+  k_chain_list<false><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, cap, stride,
+                                                             ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt,
+                                                             nseg > 0 ? dbase : nullptr, ctx->seg_len, S.d_map,
+                                                             S.d_flag, ctx->d_out7, ctx->d_flag_ray, ctx->d_status,
+                                                             ctx->d_counts);
   HIPCHK(ctx, hipGetLastError());
   return 0;
+}
+
+/* Sorts list[0..n) by ns, longest first (counting sort; ties by ray). */
+static void sort_longest_first(int *list, int n, const std::vector<int> &ns, std::vector<int> &cnt,
+                               std::vector<int> &tmp)
+{
+  int mx = 0;
+  for (int k = 0; k < n; k++) mx = ns[list[k]] > mx ? ns[list[k]] : mx;
+  cnt.assign((size_t)mx + 2, 0);
+  for (int k = 0; k < n; k++) cnt[mx - ns[list[k]] + 1]++;
+  for (int v = 1; v <= mx + 1; v++) cnt[v] += cnt[v - 1];
+  tmp.resize(n);
+  for (int k = 0; k < n; k++) tmp[cnt[mx - ns[list[k]]]++] = list[k];
+  for (int k = 0; k < n; k++) list[k] = tmp[k];
 }
 
 /* Streamed render of one image.  The geodesics are integrated in phases
@@ -1418,20 +1480,29 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
         return 0;
       }
     }
-    long long *off = h_off + start + k;
-    off[0] = 0;
-    for (int q = 0; q < n; q++) off[q + 1] = off[q] + nsv[h_list[start + q]];
-    const long long T = off[n];
+    /* longest first, waves of 64, rows per wave */
+    sort_longest_first(h_list + start, n, nsv, ctx->h_cnt, ctx->h_tmp);
+    const int nw = (n + ILV - 1) / ILV;
+    long long *W = h_off + start + k;
+    W[0] = 0;
+    for (int c = 0; c < nw; c++) W[c + 1] = W[c] + nsv[h_list[start + c * ILV]];
+    const long long rows = W[nw];
+    const long long T = rows * ILV;   /* packet slots, padding included */
     total += T;
-    build_segments(ctx->segset[k], h_list + start, n, off, nsv, seg_min, ctx->seg_len, P.maxnstep, h_base + start);
+    /* first packet of each element, in doubles */
+    std::vector<long long> &eoff = ctx->h_eoff;
+    eoff.resize(n);
+    for (int q = 0; q < n; q++) eoff[q] = W[q / ILV] * stride * ILV + q % ILV;
+    build_segments(ctx->segset[k], h_list + start, n, eoff.data(), nsv, seg_min, ctx->seg_len, P.maxnstep,
+                   h_base + start);
     hipStream_t st = (k < K) ? ctx->streamF[k] : sA;
     int *dl = ctx->d_list + start, *dbase = ctx->d_segbase + start;
-    long long *doff = ctx->d_offl + start + k;
+    long long *dW = ctx->d_offl + start + k;
     double *pkt;
     int *blk;
     if (n > 0) {
+      const size_t nb = (size_t)((T + PREP_BLOCK - 1) / PREP_BLOCK);
       if (k < K) {
-        const size_t nb = (size_t)((T + PREP_BLOCK - 1) / PREP_BLOCK);
         if ((size_t)(pofs + T) * stride > ctx->pkt_doubles || bofs + nb > ctx->cap_blk) {
           /* the arenas are full: earlier phases finish first */
           for (int q = 0; q < k; q++) HIPCHK(ctx, hipStreamSynchronize(ctx->streamF[q]));
@@ -1448,17 +1519,16 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
         any_f = true;
       } else {
         if ((rc = grow(ctx, ctx->d_pkt2, ctx->pkt2_doubles, (size_t)(T > 0 ? T : 1) * stride))) return rc;
-        const size_t nb = (size_t)((T + PREP_BLOCK - 1) / PREP_BLOCK);
         if ((rc = grow(ctx, ctx->d_blk2, ctx->cap_blk2, nb > 0 ? nb : 1))) return rc;
         pkt = ctx->d_pkt2;
         blk = ctx->d_blk2;
       }
       HIPCHK(ctx, hipMemcpyAsync(dl, h_list + start, n * sizeof(int), hipMemcpyHostToDevice, st));
-      HIPCHK(ctx, hipMemcpyAsync(doff, off, (n + 1) * sizeof(long long), hipMemcpyHostToDevice, st));
+      HIPCHK(ctx, hipMemcpyAsync(dW, W, (nw + 1) * sizeof(long long), hipMemcpyHostToDevice, st));
       if (!ctx->segset[k].off.empty())
         HIPCHK(ctx, hipMemcpyAsync(dbase, h_base + start, n * sizeof(int), hipMemcpyHostToDevice, st));
       if ((rc = upload_segments(ctx, ctx->segset[k], st))) return rc;
-      if ((rc = transfer_list(ctx, P, st, n, dl, doff, T, blk, pkt, dbase, ctx->segset[k], cap, stride)))
+      if ((rc = transfer_list(ctx, P, st, n, dl, dW, nw, rows, blk, pkt, dbase, ctx->segset[k], cap, stride)))
         return rc;
     }
     if (k < K) HIPCHK(ctx, hipEventRecord(ev[15 + k], st));
@@ -1522,6 +1592,7 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
       for (int r = 0; r < nrays; r++) {
         list[r] = r;
         off[r + 1] = off[r] + ns[r];   /* the packet offsets k_scan forms */
+        off[r] *= PK;                  /* in doubles */
       }
       build_segments(S, list.data(), nrays, off.data(), ns, ctx->seg_min_steps, ctx->seg_len, P.maxnstep,
                      base.data());
@@ -1530,17 +1601,25 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
         HIPCHK(ctx, hipMemcpyAsync(ctx->d_segbase, base.data(), nrays * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
         if ((rc = upload_segments(ctx, S, ctx->stream))) return rc;
         long long lanes = (long long)S.off.size() * SEGCOL;
-        k_seg<<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>((int)S.off.size(), ctx->seg_len, S.d_off,
-                                                                               S.d_q0, S.d_ns, ctx->d_pkt, S.d_map,
-                                                                               S.d_flag);
+        k_seg<1><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>((int)S.off.size(), ctx->seg_len,
+                                                                                  S.d_off, S.d_q0, S.d_ns, ctx->d_pkt,
+                                                                                  S.d_map, S.d_flag);
         HIPCHK(ctx, hipGetLastError());
       }
     }
     const bool segd = !S.off.empty();
-    k_chain<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, ctx->d_off,
-                                                     ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, ctx->d_pkt,
-                                                     segd ? ctx->d_segbase : nullptr, ctx->seg_len, S.d_map,
-                                                     S.d_flag, ctx->d_out7, ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
+    if (segd) {
+      k_chain<true><<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, ctx->d_off,
+                                                             ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, ctx->d_pkt,
+                                                             ctx->d_segbase, ctx->seg_len, S.d_map, S.d_flag,
+                                                             ctx->d_out7, ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
+      HIPCHK(ctx, hipGetLastError());
+    }
+    k_chain<false><<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, ctx->d_off,
+                                                            ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, ctx->d_pkt,
+                                                            segd ? ctx->d_segbase : nullptr, ctx->seg_len, S.d_map,
+                                                            S.d_flag, ctx->d_out7, ctx->d_flag_ray, ctx->d_status,
+                                                            ctx->d_counts);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
   }
