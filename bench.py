@@ -356,15 +356,14 @@ def main():
         ms_stage.append(st_)
     wall = time.perf_counter() - t0
     n_geo, n_rad = ctx.step_counts()
+    res = ctx.render(im)
     # the one-pass render of the same image, for the log
     ctx.set_tuning(trace_budget=0)
     one = []
     for s in range(3):
         ctx.render(im, fetch=False)
         one.append(ctx.kernel_times()[2])
-    ctx.set_tuning(trace_budget=4096)
     log("one-pass render of the same image: %.3f ms" % np.mean(one[1:]))
-    res = ctx.render(im)
     st = G.image_stats(res, scale, cfg["freqcgs"], 16.9e6)
     ns_img = res["nstep"]
     tiles = ns_img[:ns_img.shape[0] // 8 * 8, :ns_img.shape[1] // 8 * 8].reshape(
