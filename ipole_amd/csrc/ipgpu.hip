@@ -806,6 +806,39 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_list(ipgpu_geom G, ipgpu_mo
                       pkt + (size_t)(W[c] + q) * stride * ILV + l, stride == PK, unit_scale());
 }
 
+/* The same packets in the compact layout (ray after ray, packets of one ray
+ * contiguous: element k's packets at offL[k] .. offL[k+1]), one lane per
+ * step; for lists whose chains are all composed from segments, whose
+ * column lanes then read one ray's packets from few cache lines. */
+__global__ void k_blockmap_list(int n, const long long *__restrict__ offL, int *__restrict__ blk)
+{
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  long long b0 = (offL[k] + PREP_BLOCK - 1) / PREP_BLOCK, b1 = (offL[k + 1] + PREP_BLOCK - 1) / PREP_BLOCK;
+  for (long long b = b0; b < b1; b++) blk[b] = k;
+}
+
+template <bool F32>
+__global__ __launch_bounds__(PREP_BLOCK) void k_prep_compact(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
+                                                             const int *__restrict__ list, int n,
+                                                             const long long *__restrict__ offL, long long total,
+                                                             const int *__restrict__ blk, int cap,
+                                                             const int *__restrict__ nstep_in,
+                                                             const double *__restrict__ traj,
+                                                             const double *__restrict__ dump, double *__restrict__ pkt)
+{
+  long long g = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
+  if (g >= total) return;
+  int k = blk[blockIdx.x];
+  while (k + 1 < n && offL[k + 1] <= g) k++;
+  const int ray = list[k];
+  const int ns = nstep_in[ray];
+  if (ns >= P.maxnstep - 1) return;
+  const int q = (int)(g - offL[k]);
+  prep_step<F32>(G, Mo, P, ns - q, traj + (size_t)ray * cap * REC, nullptr, dump, pkt + (size_t)g * PK, true,
+                 unit_scale());
+}
+
 /* The chains of the rays of a list, one lane per ray: SEG = false the rays
  * with seg_base[k] < 0 (recurrence over the packets), SEG = true those with
  * seg_base[k] >= 0 (composition of the segment maps). */
@@ -1315,9 +1348,34 @@ static int upload_segments(ipgpu_ctx *ctx, ipgpu_ctx::SegSet &S, hipStream_t st)
  * all), on stream st; packets wave-interleaved at pkt, block map at blk. */
 static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, int n, const int *dl,
                          const long long *W, int nw, long long rows, int *blk, double *pkt, const int *dbase,
-                         ipgpu_ctx::SegSet &S, int cap, int stride)
+                         ipgpu_ctx::SegSet &S, int cap, int stride, bool compact)
 {
   if (n <= 0) return 0;
+  if (compact) {
+    /* every chain composed: W holds the n + 1 packet offsets, rows the total */
+    if (rows > 0) {
+      const unsigned nb = (unsigned)((rows + PREP_BLOCK - 1) / PREP_BLOCK);
+      k_blockmap_list<<<(n + 255) / 256, 256, 0, st>>>(n, W, blk);
+      HIPCHK(ctx, hipGetLastError());
+      auto kp = P.transfer_fp32 ? k_prep_compact<true> : k_prep_compact<false>;
+      kp<<<nb, PREP_BLOCK, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, rows, blk, cap, ctx->d_nstep_ray,
+                                    ctx->d_traj, ctx->d_dump, pkt);
+      HIPCHK(ctx, hipGetLastError());
+    }
+    const int nseg = (int)S.off.size();
+    if (nseg > 0) {
+      long long lanes = (long long)nseg * SEGCOL;
+      k_seg<1><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, st>>>(nseg, ctx->seg_len, S.d_off, S.d_q0, S.d_ns,
+                                                                      pkt, S.d_map, S.d_flag);
+      HIPCHK(ctx, hipGetLastError());
+    }
+    k_chain_list<true><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, cap, stride,
+                                                              ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt, dbase,
+                                                              ctx->seg_len, S.d_map, S.d_flag, ctx->d_out7,
+                                                              ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
+    HIPCHK(ctx, hipGetLastError());
+    return 0;
+  }
   if (rows > 0) {
     const unsigned nb = (unsigned)((rows * ILV + PREP_BLOCK - 1) / PREP_BLOCK);   /* blk holds nb entries */
     k_blockmap_rows<<<(nw + 255) / 256, 256, 0, st>>>(nw, W, blk);
@@ -1480,19 +1538,28 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
         return 0;
       }
     }
-    /* longest first, waves of 64, rows per wave */
+    /* longest first.  Lists whose chains are all composed from segments
+     * (the last phase's, when the segment threshold is at most the last
+     * budget) keep each ray's packets contiguous (offsets in W); the others
+     * are cut into waves of 64 with wave-interleaved packets (rows per wave
+     * in W). */
     sort_longest_first(h_list + start, n, nsv, ctx->h_cnt, ctx->h_tmp);
-    const int nw = (n + ILV - 1) / ILV;
+    bool compact = (k == K) && segm && n > 0;
+    for (int q = 0; compact && q < n; q++) compact = nsv[h_list[start + q]] > seg_min;
+    const int nw = compact ? n : (n + ILV - 1) / ILV;
     long long *W = h_off + start + k;
     W[0] = 0;
-    for (int c = 0; c < nw; c++) W[c + 1] = W[c] + nsv[h_list[start + c * ILV]];
+    if (compact)
+      for (int q = 0; q < n; q++) W[q + 1] = W[q] + nsv[h_list[start + q]];
+    else
+      for (int c = 0; c < nw; c++) W[c + 1] = W[c] + nsv[h_list[start + c * ILV]];
     const long long rows = W[nw];
-    const long long T = rows * ILV;   /* packet slots, padding included */
+    const long long T = compact ? rows : rows * ILV;   /* packet slots, padding included */
     total += T;
     /* first packet of each element, in doubles */
     std::vector<long long> &eoff = ctx->h_eoff;
     eoff.resize(n);
-    for (int q = 0; q < n; q++) eoff[q] = W[q / ILV] * stride * ILV + q % ILV;
+    for (int q = 0; q < n; q++) eoff[q] = compact ? W[q] * PK : W[q / ILV] * stride * ILV + q % ILV;
     build_segments(ctx->segset[k], h_list + start, n, eoff.data(), nsv, seg_min, ctx->seg_len, P.maxnstep,
                    h_base + start);
     hipStream_t st = (k < K) ? ctx->streamF[k] : sA;
@@ -1528,7 +1595,8 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
       if (!ctx->segset[k].off.empty())
         HIPCHK(ctx, hipMemcpyAsync(dbase, h_base + start, n * sizeof(int), hipMemcpyHostToDevice, st));
       if ((rc = upload_segments(ctx, ctx->segset[k], st))) return rc;
-      if ((rc = transfer_list(ctx, P, st, n, dl, dW, nw, rows, blk, pkt, dbase, ctx->segset[k], cap, stride)))
+      if ((rc = transfer_list(ctx, P, st, n, dl, dW, nw, rows, blk, pkt, dbase, ctx->segset[k], cap, stride,
+                              compact)))
         return rc;
     }
     if (k < K) HIPCHK(ctx, hipEventRecord(ev[15 + k], st));
