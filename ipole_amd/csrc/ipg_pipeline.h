@@ -521,19 +521,20 @@ IPG_HD_INLINE int chain_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, c
   Chain st;
   chain_init(st, js, ks);
   if (pol) {
-    Pkt cur, nxt;
-    if constexpr (PS == 1) {
-      if (ns > 0) load_pkt(pkt, PK, cur);
-    } else {
-      if (ns > 0) load_pkt_s<PS>(pkt, cur);
-    }
-    for (int q = 0; q < ns; q++) {
-      if (q + 1 < ns) {
-        if constexpr (PS == 1) load_pkt(pkt + (size_t)(q + 1) * PK, PK, nxt);
-        else load_pkt_s<PS>(pkt + (size_t)(q + 1) * PK * PS, nxt);
-      }
-      chain_step(st, cur.v, true, P.stokes_floors);
-      cur = nxt;
+    /* two packet buffers used in turn (no copy between them): the next
+     * step's packet is in flight while the current one computes */
+    Pkt a, b;
+    auto load = [&](int q, Pkt &k) {
+      if constexpr (PS == 1) load_pkt(pkt + (size_t)q * PK, PK, k);
+      else load_pkt_s<PS>(pkt + (size_t)q * PK * PS, k);
+    };
+    if (ns > 0) load(0, a);
+    for (int q = 0; q < ns; q += 2) {
+      if (q + 1 < ns) load(q + 1, b);
+      chain_step(st, a.v, true, P.stokes_floors);
+      if (q + 1 >= ns) break;
+      if (q + 2 < ns) load(q + 2, a);
+      chain_step(st, b.v, true, P.stokes_floors);
     }
   } else {
     for (int q = 0; q < ns; q++) {
