@@ -210,20 +210,22 @@ int ipgpu_kernel_times(const ipgpu_ctx *ctx, double *ms_geodesic, double *ms_tra
 int ipgpu_stage_times(const ipgpu_ctx *ctx, double ms[4]);
 
 /* Tuning of ipgpu_render (no counterpart in the reference, whose pixel loop
- * has none).  trace_budget: steps of the last bounded geodesic phase of
- * the streamed render (phases at a quarter, a half and all of it, then to
- * the end; default 4096; 0 = one pass; it applies when target_nturns < 0
- * and the record capacity exceeds it).  seg_min_steps:
- * polarized rays longer than this have their transfer composed from
- * segments (default 4096).  seg_len: steps per segment (default 256).
- * record_steps: geodesic records held per ray (default 8192, raised by a
- * render whose longest ray needs more; that render retraces).  Results do
- * not depend on trace_budget or record_steps; seg_min_steps and seg_len
- * regroup the transfer arithmetic of the rays selected (agreement to
- * roundoff).  A negative value leaves a setting unchanged.  The environment
- * variables IPGPU_TRACE_BUDGET, IPGPU_SEG_MIN_STEPS and IPGPU_SEG_LEN set the
- * defaults of a new context. */
-int ipgpu_set_tuning(ipgpu_ctx *ctx, int trace_budget, int seg_min_steps, int seg_len, int record_steps);
+ * has none).  trace_budget, trace_phases: the streamed render integrates the
+ * geodesics in trace_phases bounded phases of trace_budget j / trace_phases
+ * steps (j = 1 .. trace_phases, at most 4), then to the end (defaults 3584
+ * and 2; trace_budget 0 = one pass; streaming applies when target_nturns < 0
+ * and the record capacity exceeds the budget).  seg_min_steps: polarized
+ * rays longer than this have their transfer composed from segments (default
+ * 3584).  seg_len: steps per segment (default 384).  record_steps:
+ * geodesic records held per ray (default 8192, raised by a render whose
+ * longest ray needs more; that render retraces).  Results do not depend on
+ * the phases or the record capacity; seg_min_steps and seg_len regroup the
+ * transfer arithmetic of the rays selected (agreement to roundoff).  A
+ * negative value leaves a setting unchanged.  The environment variables
+ * IPGPU_TRACE_BUDGET, IPGPU_TRACE_PHASES, IPGPU_SEG_MIN_STEPS and
+ * IPGPU_SEG_LEN set the defaults of a new context. */
+int ipgpu_set_tuning(ipgpu_ctx *ctx, int trace_budget, int trace_phases, int seg_min_steps, int seg_len,
+                     int record_steps);
 
 /* Work counts of the last render: geodesic steps summed over rays (the
  * nstep of trace_geodesic) and radiating steps of the transfer pass (steps

@@ -560,8 +560,8 @@ IPG_HD_INLINE int chain_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, c
  * (16 columns of the N map from the basis states without sources, one from
  * N = 0 with sources, one column for I, tau, tauF), then applied in order
  * from the far end: the same result up to the roundoff of the regrouping. */
-constexpr int SEG_LEN = 256;          /* default segment length */
-constexpr int SEG_MIN_STEPS = 4096;   /* default: rays longer than this use segments */
+constexpr int SEG_LEN = 384;          /* default segment length */
+constexpr int SEG_MIN_STEPS = 3584;   /* default: rays longer than this use segments */
 constexpr int NPAR = 16;
 constexpr int SEG_I = NPAR + 1;       /* column of the unpolarized map */
 constexpr int SEGCOL = NPAR + 2;

@@ -930,7 +930,7 @@ __global__ __launch_bounds__(WAVE) void k_seg(int nseg, int seg_len, const long 
 /* ------------------------------------------------------------------ C-ABI */
 
 constexpr int NEV = 24;
-constexpr int MAXPH = 4;   /* trace phases of the streamed render (the last unbounded) */
+constexpr int MAXPH = 5;   /* trace phases of the streamed render (the last unbounded) */
 
 struct ipgpu_ctx {
   int device = 0;
@@ -974,7 +974,8 @@ struct ipgpu_ctx {
   /* streamed render: step budget of the last bounded trace phase (0: one
    * pass), the ray length above which a chain is composed from segments,
    * the segment length */
-  int trace_budget = 4096;
+  int trace_budget = 3584;
+  int trace_phases = 2;      /* bounded phases, budgets trace_budget j / trace_phases */
   int seg_min_steps = SEG_MIN_STEPS;
   int seg_len = SEG_LEN;
   int *d_live = nullptr, *d_list = nullptr, *d_blk2 = nullptr;
@@ -1077,6 +1078,7 @@ int ipgpu_create(ipgpu_ctx **out, int device)
   if (const char *e = getenv("IPGPU_TRACE_BUDGET")) ctx->trace_budget = atoi(e);
   if (const char *e = getenv("IPGPU_SEG_MIN_STEPS")) ctx->seg_min_steps = atoi(e);
   if (const char *e = getenv("IPGPU_SEG_LEN")) ctx->seg_len = atoi(e) > 1 ? atoi(e) : 1;
+  if (const char *e = getenv("IPGPU_TRACE_PHASES")) ctx->trace_phases = atoi(e);
   if (hipMalloc((void **)&ctx->d_status, NSTATUS * sizeof(int)) != hipSuccess
       || hipMalloc((void **)&ctx->d_counts, 2 * sizeof(unsigned long long)) != hipSuccess) {
     delete ctx;
@@ -1444,10 +1446,12 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
   const int seg_min = segm ? ctx->seg_min_steps : 0x7fffffff;
   hipStream_t sA = ctx->stream;
   hipEvent_t *ev = ctx->ev;
-  /* budgets of the bounded phases */
+  /* budgets of the bounded phases: trace_budget j / nb for j = 1 .. nb
+   * (nb = trace_phases), those below 64 steps dropped */
   int bud[MAXPH - 1], K = 0;
-  for (int d = 2; d >= 0; d--) {
-    int b = ctx->trace_budget >> d;
+  const int nb = ctx->trace_phases < 1 ? 1 : (ctx->trace_phases > MAXPH - 1 ? MAXPH - 1 : ctx->trace_phases);
+  for (int j = 1; j <= nb; j++) {
+    int b = (int)((long long)ctx->trace_budget * j / nb);
     if (b >= 64 && (K == 0 || b > bud[K - 1])) bud[K++] = b;
   }
   if (K == 0) bud[K++] = ctx->trace_budget;
@@ -2171,10 +2175,12 @@ int ipgpu_stage_times(const ipgpu_ctx *ctx, double ms[4])
   return ctx->streamed;
 }
 
-int ipgpu_set_tuning(ipgpu_ctx *ctx, int trace_budget, int seg_min_steps, int seg_len, int record_steps)
+int ipgpu_set_tuning(ipgpu_ctx *ctx, int trace_budget, int trace_phases, int seg_min_steps, int seg_len,
+                     int record_steps)
 {
   if (!ctx) return IPGPU_ERR_ARG;
   if (trace_budget >= 0) ctx->trace_budget = trace_budget;
+  if (trace_phases >= 0) ctx->trace_phases = trace_phases;
   if (seg_min_steps >= 0) ctx->seg_min_steps = seg_min_steps;
   if (seg_len >= 0) ctx->seg_len = seg_len > 1 ? seg_len : 1;
   if (record_steps >= 0) ctx->traj_cap_steps = record_steps > 2 ? record_steps : 2;

@@ -100,7 +100,7 @@ def lib():
         L.ipgpu_scale_density.argtypes = [vp, dp]
         L.ipgpu_kernel_times.argtypes = [vp, dp, dp, dp]
         L.ipgpu_stage_times.argtypes = [vp, dp]
-        L.ipgpu_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ipgpu_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.ipgpu_step_counts.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.ipgpu_stream.argtypes = [vp]
         L.ipgpu_stream.restype = vp
@@ -384,12 +384,13 @@ class Context:
         self._chk(min(rc, 0))
         return tuple(ms), rc == 1
 
-    def set_tuning(self, trace_budget=-1, seg_min_steps=-1, seg_len=-1, record_steps=-1):
-        """Streamed-render trace budget (0 = one pass), the ray length above which
-        polarized transfer is composed from segments, the segment length and
-        the geodesic records held per ray; -1 keeps a setting."""
-        self._chk(self.L.ipgpu_set_tuning(self.h, int(trace_budget), int(seg_min_steps), int(seg_len),
-                                          int(record_steps)))
+    def set_tuning(self, trace_budget=-1, trace_phases=-1, seg_min_steps=-1, seg_len=-1, record_steps=-1):
+        """Streamed-render trace budget (0 = one pass) and number of bounded
+        phases, the ray length above which polarized transfer is composed
+        from segments, the segment length and the geodesic records held per
+        ray; -1 keeps a setting."""
+        self._chk(self.L.ipgpu_set_tuning(self.h, int(trace_budget), int(trace_phases), int(seg_min_steps),
+                                          int(seg_len), int(record_steps)))
 
     def step_counts(self):
         a, b = C.c_longlong(), C.c_longlong()
