@@ -768,39 +768,50 @@ __global__ __launch_bounds__(WAVE) void k_trace_resume_mask(ipgpu_geom G, ipgpu_
  * contiguous bytes, and a wave of the packet kernel stores it so. */
 constexpr int ILV = WAVE;
 
-/* Block (4 rows of 64 lanes) -> wave of the list whose rows contain the
- * block's first row. */
-__global__ void k_blockmap_rows(int nw, const long long *__restrict__ W, int *__restrict__ blk)
+/* The packet kernel of an interleaved list works in tiles of 8 rays x 8
+ * consecutive steps (one hardware wave): a lane's neighbours are the next
+ * steps of its ray (the same trajectory lines, mostly the same dump cells)
+ * and a store of one field covers 8 rows of 64 contiguous bytes.  Wave c of
+ * the list has 8 ceil(len_c / 8) tiles, from tile TT[c] on. */
+constexpr int TQ = 8;                        /* steps per tile */
+constexpr int TL = WAVE / TQ;                /* rays per tile */
+constexpr int TPB = PREP_BLOCK / WAVE;       /* tiles per block */
+
+/* Block (TPB tiles) -> wave of the list whose tiles contain the block's
+ * first tile. */
+__global__ void k_blockmap_tiles(int nw, const long long *__restrict__ TT, int *__restrict__ blk)
 {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nw) return;
-  constexpr int RPB = PREP_BLOCK / ILV;
-  long long b0 = (W[c] + RPB - 1) / RPB, b1 = (W[c + 1] + RPB - 1) / RPB;
+  long long b0 = (TT[c] + TPB - 1) / TPB, b1 = (TT[c + 1] + TPB - 1) / TPB;
   for (long long b = b0; b < b1; b++) blk[b] = c;
 }
 
-/* Step packets of a list: one lane per (row, lane) slot; slots past a ray's
- * step count are padding and do nothing. */
+/* Step packets of an interleaved list: one lane per (step, ray) slot of a
+ * tile; slots past a ray's step count are padding and do nothing. */
 template <bool F32>
 __global__ __launch_bounds__(PREP_BLOCK) void k_prep_list(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
                                                           const int *__restrict__ list, int n,
-                                                          const long long *__restrict__ W, int nw, long long rows,
+                                                          const long long *__restrict__ W,
+                                                          const long long *__restrict__ TT, int nw, long long tiles,
                                                           const int *__restrict__ blk, int cap, int stride,
                                                           const int *__restrict__ nstep_in,
                                                           const double *__restrict__ traj,
                                                           const double *__restrict__ dump, double *__restrict__ pkt)
 {
   const long long t = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
-  const long long row = t / ILV;
-  const int l = (int)(t % ILV);
-  if (row >= rows) return;
+  const long long tile = t / WAVE;
+  const int j = (int)(t % WAVE);
+  if (tile >= tiles) return;
   int c = blk[blockIdx.x];
-  while (c + 1 < nw && W[c + 1] <= row) c++;
+  while (c + 1 < nw && TT[c + 1] <= tile) c++;
+  const int lt = (int)(tile - TT[c]);
+  const int l = (lt % (ILV / TL)) * TL + j / TQ;   /* lane of the list wave */
+  const int q = (lt / (ILV / TL)) * TQ + j % TQ;   /* step */
   const int k = c * ILV + l;
   if (k >= n) return;
   const int ray = list[k];
   const int ns = nstep_in[ray];
-  const int q = (int)(row - W[c]);
   if (q >= ns || ns >= P.maxnstep - 1) return;
   prep_step<F32, ILV>(G, Mo, P, ns - q, traj + (size_t)ray * cap * REC, nullptr, dump,
                       pkt + (size_t)(W[c] + q) * stride * ILV + l, stride == PK, unit_scale());
@@ -1346,11 +1357,12 @@ static int upload_segments(ipgpu_ctx *ctx, ipgpu_ctx::SegSet &S, hipStream_t st)
 }
 
 /* Step packets, segment maps and chains of the n rays of a list (device
- * list dl sorted longest first, wave rows W with nw + 1 entries, rows in
- * all), on stream st; packets wave-interleaved at pkt, block map at blk. */
+ * list dl sorted longest first, wave rows W and packet-kernel tiles TT with
+ * nw + 1 entries each, rows and tiles in all), on stream st; packets
+ * wave-interleaved at pkt, block map at blk. */
 static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, int n, const int *dl,
-                         const long long *W, int nw, long long rows, int *blk, double *pkt, const int *dbase,
-                         ipgpu_ctx::SegSet &S, int cap, int stride, bool compact)
+                         const long long *W, const long long *TT, long long tiles, int nw, long long rows, int *blk,
+                         double *pkt, const int *dbase, ipgpu_ctx::SegSet &S, int cap, int stride, bool compact)
 {
   if (n <= 0) return 0;
   if (compact) {
@@ -1379,12 +1391,12 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
     return 0;
   }
   if (rows > 0) {
-    const unsigned nb = (unsigned)((rows * ILV + PREP_BLOCK - 1) / PREP_BLOCK);   /* blk holds nb entries */
-    k_blockmap_rows<<<(nw + 255) / 256, 256, 0, st>>>(nw, W, blk);
+    const unsigned nb = (unsigned)((tiles + TPB - 1) / TPB);   /* blk holds nb entries */
+    k_blockmap_tiles<<<(nw + 255) / 256, 256, 0, st>>>(nw, TT, blk);
     HIPCHK(ctx, hipGetLastError());
     auto kp = P.transfer_fp32 ? k_prep_list<true> : k_prep_list<false>;
-    kp<<<nb, PREP_BLOCK, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, nw, rows, blk, cap, stride, ctx->d_nstep_ray,
-                                  ctx->d_traj, ctx->d_dump, pkt);
+    kp<<<nb, PREP_BLOCK, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, TT, nw, tiles, blk, cap, stride,
+                                  ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt);
     HIPCHK(ctx, hipGetLastError());
   }
   const int nseg = (int)S.off.size();
@@ -1461,18 +1473,20 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
   if ((rc = grow(ctx, ctx->d_traj, ctx->traj_doubles, (size_t)nrays * cap * REC))) return rc;
   if ((rc = grow(ctx, ctx->d_live, ctx->cap_live, (size_t)nrays))) return rc;
   if ((rc = grow(ctx, ctx->d_list, ctx->cap_list, (size_t)nrays))) return rc;
-  if ((rc = grow(ctx, ctx->d_offl, ctx->cap_offl, (size_t)nrays + MAXPH + 1))) return rc;
+  if ((rc = grow(ctx, ctx->d_offl, ctx->cap_offl, 2 * ((size_t)nrays + MAXPH + 1)))) return rc;
   if ((rc = grow(ctx, ctx->d_segbase, ctx->cap_segbase, (size_t)nrays))) return rc;
   if (ctx->last_total > 0) {
     if ((rc = grow(ctx, ctx->d_pkt, ctx->pkt_doubles, (size_t)ctx->last_total * stride))) return rc;
-    if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)(ctx->last_total / PREP_BLOCK + 2 * MAXPH))))
+    /* tiles: rows / 4 + 2 per list wave (ceil to 8 steps) per block of 4 */
+    if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk,
+                   (size_t)(ctx->last_total / PREP_BLOCK + 2 * ((long long)nrays / ILV + 2 * MAXPH)))))
       return rc;
   }
   /* pinned, device-mapped, coherent host arena: the trace kernels write
    * each phase's step counts and live flags into it (a copy command would
    * hold the host until the phase ends) */
   const size_t nI = (size_t)nrays;
-  const size_t need = sizeof(int) * nI * (2 * (MAXPH) + 2) + sizeof(long long) * (nI + MAXPH + 1);
+  const size_t need = sizeof(int) * nI * (2 * (MAXPH) + 2) + 2 * sizeof(long long) * (nI + MAXPH + 1);
   if (ctx->h_pin_bytes < need) {
     if (ctx->h_pin) hipHostFree(ctx->h_pin);
     ctx->h_pin = nullptr;
@@ -1483,7 +1497,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
   char *d_pin = nullptr;
   HIPCHK(ctx, hipHostGetDevicePointer((void **)&d_pin, ctx->h_pin, 0));
   long long *h_off = (long long *)ctx->h_pin;
-  const size_t ns_bytes = sizeof(long long) * (nI + MAXPH + 1);
+  const size_t ns_bytes = 2 * sizeof(long long) * (nI + MAXPH + 1);   /* W, then TT */
   int *h_ns = (int *)((char *)ctx->h_pin + ns_bytes); /* [K+1][nrays] */
   int *h_live = h_ns + (size_t)MAXPH * nI;            /* [K][nrays] */
   int *h_list = h_live + (size_t)MAXPH * nI;
@@ -1559,6 +1573,11 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
       for (int c = 0; c < nw; c++) W[c + 1] = W[c] + nsv[h_list[start + c * ILV]];
     const long long rows = W[nw];
     const long long T = compact ? rows : rows * ILV;   /* packet slots, padding included */
+    long long *TT = W + (nI + MAXPH + 1);
+    TT[0] = 0;
+    if (!compact)
+      for (int c = 0; c < nw; c++) TT[c + 1] = TT[c] + (long long)(ILV / TL) * ((W[c + 1] - W[c] + TQ - 1) / TQ);
+    const long long tiles = compact ? 0 : TT[nw];
     total += T;
     /* first packet of each element, in doubles */
     std::vector<long long> &eoff = ctx->h_eoff;
@@ -1568,11 +1587,11 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
                    h_base + start);
     hipStream_t st = (k < K) ? ctx->streamF[k] : sA;
     int *dl = ctx->d_list + start, *dbase = ctx->d_segbase + start;
-    long long *dW = ctx->d_offl + start + k;
+    long long *dW = ctx->d_offl + start + k, *dTT = dW + (nI + MAXPH + 1);
     double *pkt;
     int *blk;
     if (n > 0) {
-      const size_t nb = (size_t)((T + PREP_BLOCK - 1) / PREP_BLOCK);
+      const size_t nb = compact ? (size_t)((T + PREP_BLOCK - 1) / PREP_BLOCK) : (size_t)((tiles + TPB - 1) / TPB);
       if (k < K) {
         if ((size_t)(pofs + T) * stride > ctx->pkt_doubles || bofs + nb > ctx->cap_blk) {
           /* the arenas are full: earlier phases finish first */
@@ -1580,7 +1599,8 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
           pofs = bofs = 0;
           const size_t want = (size_t)(T > ctx->last_total ? T : ctx->last_total);
           if ((rc = grow(ctx, ctx->d_pkt, ctx->pkt_doubles, (want > 0 ? want : 1) * stride))) return rc;
-          if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, want / PREP_BLOCK + 2 * MAXPH))) return rc;
+          if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, want / PREP_BLOCK + 2 * ((size_t)nrays / ILV + 2 * MAXPH))))
+            return rc;
         }
         pkt = ctx->d_pkt + (size_t)pofs * stride;
         blk = ctx->d_blk + bofs;
@@ -1596,11 +1616,12 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
       }
       HIPCHK(ctx, hipMemcpyAsync(dl, h_list + start, n * sizeof(int), hipMemcpyHostToDevice, st));
       HIPCHK(ctx, hipMemcpyAsync(dW, W, (nw + 1) * sizeof(long long), hipMemcpyHostToDevice, st));
+      if (!compact) HIPCHK(ctx, hipMemcpyAsync(dTT, TT, (nw + 1) * sizeof(long long), hipMemcpyHostToDevice, st));
       if (!ctx->segset[k].off.empty())
         HIPCHK(ctx, hipMemcpyAsync(dbase, h_base + start, n * sizeof(int), hipMemcpyHostToDevice, st));
       if ((rc = upload_segments(ctx, ctx->segset[k], st))) return rc;
-      if ((rc = transfer_list(ctx, P, st, n, dl, dW, nw, rows, blk, pkt, dbase, ctx->segset[k], cap, stride,
-                              compact)))
+      if ((rc = transfer_list(ctx, P, st, n, dl, dW, dTT, tiles, nw, rows, blk, pkt, dbase, ctx->segset[k], cap,
+                              stride, compact)))
         return rc;
     }
     if (k < K) HIPCHK(ctx, hipEventRecord(ev[15 + k], st));
