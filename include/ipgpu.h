@@ -57,6 +57,9 @@ typedef struct ipgpu_model {
   double L_unit;                         /* cm per GM/c^2 */
   double th_beg, sigma_cut, sigma_cut_high;
   double sph_R0, sph_Ne0, sph_Thetae0, sph_B0;
+  /* sphere sigma and beta inside R0 (model/sphere/model.c:172-188), read
+   * only by the trace output: Thetae0 tp_over_te / beta0 and beta0 */
+  double sph_sigma, sph_beta;
 } ipgpu_model;
 
 /* Units and electron model for the on-device dump derivation
@@ -188,6 +191,47 @@ int ipgpu_adjoint(ipgpu_ctx *ctx, const ipgpu_image *img, const double *target, 
  * refresh its ghost copies: the per-cell density scale whose gradient
  * ipgpu_adjoint returns.  No counterpart in the reference. */
 int ipgpu_scale_density(ipgpu_ctx *ctx, const double *scale);
+
+/* Trace output of one pixel: everything dump_var_along records
+ * (src/io.c:279-543), the data of the reference's trace.h5 for pixel
+ * (i, j).  out holds cap >= maxnstep records of IPGPU_TRACE_NF doubles;
+ * record s has the values the reference stores at index s of its per-step
+ * arrays (filled for s = 1 .. nstep - 1, and e2Transported also at 0; the
+ * rest zero), at the offsets below.  rho, u and beta need the fields derived
+ * on the device (ipgpu_upload_prims), else they are 0.  *nstep receives the
+ * trajectory's step count.  One lane: a debugging and parity tool, not a
+ * fast path. */
+#define IPGPU_TRACE_NF 126
+#define IPGPU_TR_PRIMS 0      /* 8: rho u U1 U2 U3 B1 B2 B3 */
+#define IPGPU_TR_B 8
+#define IPGPU_TR_NE 9
+#define IPGPU_TR_THETAE 10
+#define IPGPU_TR_NU 11
+#define IPGPU_TR_BK_ANGLE 12
+#define IPGPU_TR_SIGMA 13
+#define IPGPU_TR_BETA 14
+#define IPGPU_TR_DL 15        /* cgs */
+#define IPGPU_TR_R 16
+#define IPGPU_TR_TH 17
+#define IPGPU_TR_PHI 18
+#define IPGPU_TR_J_UNPOL 19
+#define IPGPU_TR_K_UNPOL 20
+#define IPGPU_TR_I_UNPOL 21
+#define IPGPU_TR_X 22         /* 4 */
+#define IPGPU_TR_KCON 26      /* 4 */
+#define IPGPU_TR_J_INV 30     /* 4 */
+#define IPGPU_TR_ALPHA_INV 34 /* 4 */
+#define IPGPU_TR_RHO_INV 38   /* 4, [0] = 0 */
+#define IPGPU_TR_STOKES 42    /* 4 */
+#define IPGPU_TR_E2C 46       /* 4: e2Constructed */
+#define IPGPU_TR_E2C_COV 50   /* 4: e2ConstructedCov */
+#define IPGPU_TR_E2T 54       /* 4: e2Transported */
+#define IPGPU_TR_E2T_COV 58   /* 4: e2TransportedCov */
+#define IPGPU_TR_NTETRAD 62   /* 16 */
+#define IPGPU_TR_NCOORD 78    /* 16 */
+#define IPGPU_TR_ECON 94      /* 16 */
+#define IPGPU_TR_ECOV 110     /* 16 */
+int ipgpu_trace_pixel(ipgpu_ctx *ctx, const ipgpu_image *img, int i, int j, int cap, int *nstep, double *out);
 
 /* Pixel (i, j) that exceeded maxnstep in the last render (IPGPU_ERR_MAXNSTEP). */
 int ipgpu_bad_pixel(const ipgpu_ctx *ctx, int *i, int *j);

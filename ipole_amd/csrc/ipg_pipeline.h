@@ -1096,4 +1096,227 @@ IPG_HD void adjoint_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const
 }
 #endif
 
+/* ------------------------------------------------------- pixel trace output */
+
+/* Everything dump_var_along records for one pixel (src/io.c:279-543), one
+ * lane.  The geodesic is traced into rec (nstep + 1 records), then the
+ * records are walked as the reference does: i = nstep - 1 .. 1, quantities
+ * at point i, the coherency tensor before the step, then one transfer step
+ * from point i + 1 to i with the emissivity cache started afresh from point
+ * i + 1 (integrate_emission called with one step, src/io.c:428).  Fields per
+ * record: the IPGPU_TR_* offsets of include/ipgpu.h.  rec holds cap
+ * trajectory records, nt cap subring counts (read when target_nturns >= 0),
+ * out cap zeroed output records; prims_ok says the dump carries rho, u and
+ * beta (slots SLOT_RHO, SLOT_U, SLOT_BETA). */
+IPG_HD void trace_pixel_records(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int pi,
+                                int pj, int cap, int prims_ok, const double *dump, double *rec, int *nt,
+                                double *out, int *nstep_out)
+{
+  const int ns = trace_ray(G, P, pi, pj, cap, rec, P.target_nturns >= 0 ? nt : nullptr);
+  *nstep_out = ns;
+  if (ns + 1 > cap) return;
+  const int *ntp = P.target_nturns >= 0 ? nt : nullptr;
+  const bool pol = !P.only_unpolarized;
+  const double conv = Mo.L_unit * HPL / (ME * CL * CL);
+  /* X0 along the ray: the time component the records leave out, formed by
+   * the geodesic step's own arithmetic */
+  {
+    Rec r0;
+    load_rec(rec, 0, r0);
+    double X0 = P.Xcam[0];
+    out[IPGPU_TR_X] = X0;
+    for (int n = 1; n <= ns; n++) {
+      Rec a, b;
+      load_rec(rec, n - 1, a);
+      load_rec(rec, n, b);
+      double X[4] = {X0, a.X[1], a.X[2], a.X[3]};
+      Metric m;
+      metric(G, X[1], X[2], m);
+      double Xh[4], Kh[4];
+      half_step(m, X, a.K, -b.dl, Xh, Kh);
+      X0 = X0 + (-b.dl) * Kh[0];
+      out[(size_t)n * IPGPU_TRACE_NF + IPGPU_TR_X] = X0;
+    }
+  }
+  Herm N;
+  for (int q = 0; q < 4; q++) N.d[q] = 0.;
+  for (int q = 0; q < 6; q++) N.re[q] = N.im[q] = 0.;
+  for (int i = ns - 1; i > 0; --i) {
+    double *o = out + (size_t)i * IPGPU_TRACE_NF;
+    Rec t;
+    load_rec(rec, i, t);
+    double X[4] = {o[IPGPU_TR_X], t.X[1], t.X[2], t.X[3]};
+    Metric m;
+    metric(G, X[1], X[2], m);
+    Fluid F;
+    fluid(G, Mo, dump, X, m, F);
+    /* primitives, sigma, beta */
+    if (Mo.kind == IPGPU_MODEL_IHARM && in_domain(G, X[1], X[2])) {
+      Cell c;
+      locate(G, X[1], X[2], X[3], c);
+      double v[10];
+      gather10(G, dump, c, v);
+      for (int q = 0; q < 6; q++) o[IPGPU_TR_PRIMS + 2 + q] = v[q];
+      o[IPGPU_TR_SIGMA] = v[9];
+      if (prims_ok) {
+        o[IPGPU_TR_PRIMS + 0] = gather_slot(G, dump, c, SLOT_RHO);
+        o[IPGPU_TR_PRIMS + 1] = gather_slot(G, dump, c, SLOT_U);
+        o[IPGPU_TR_BETA] = gather_slot(G, dump, c, SLOT_BETA);
+      }
+    } else if (Mo.kind == IPGPU_MODEL_SPHERE && radiating_region(G, Mo, m.r, m.th)) {
+      o[IPGPU_TR_SIGMA] = Mo.sph_sigma;
+      o[IPGPU_TR_BETA] = Mo.sph_beta;
+    }
+    o[IPGPU_TR_B] = F.B;
+    o[IPGPU_TR_NE] = F.ne;
+    o[IPGPU_TR_THETAE] = F.thetae;
+    const Geo q = coef_geometry(F, t.K);
+    o[IPGPU_TR_NU] = q.nu;
+    {
+      /* get_bk_angle, src/radiation.c:47-72 */
+      double Bm = sqrt(fabs(F.Bcon[0] * F.Bcov[0] + F.Bcon[1] * F.Bcov[1] + F.Bcon[2] * F.Bcov[2]
+                            + F.Bcon[3] * F.Bcov[3]));
+      double ang = PI / 2.;
+      if (Bm != 0.) {
+        double k = fabs(t.K[0] * F.Ucov[0] + t.K[1] * F.Ucov[1] + t.K[2] * F.Ucov[2] + t.K[3] * F.Ucov[3]);
+        double mu = (t.K[0] * F.Bcov[0] + t.K[1] * F.Bcov[1] + t.K[2] * F.Bcov[2] + t.K[3] * F.Bcov[3]) / (k * Bm);
+        if (fabs(mu) > 1.) mu /= fabs(mu);
+        ang = acos(mu);
+      }
+      o[IPGPU_TR_BK_ANGLE] = ang;
+    }
+    /* jar_calc, get_jkinv */
+    Coef c;
+    double ju, ku;
+    coefficients(F, t.K, P.emission_type, true, ju, ku, c);
+    o[IPGPU_TR_J_INV + 0] = c.jI;
+    o[IPGPU_TR_J_INV + 1] = c.jQ;
+    o[IPGPU_TR_J_INV + 2] = c.jU;
+    o[IPGPU_TR_J_INV + 3] = c.jV;
+    o[IPGPU_TR_ALPHA_INV + 0] = c.aI;
+    o[IPGPU_TR_ALPHA_INV + 1] = c.aQ;
+    o[IPGPU_TR_ALPHA_INV + 2] = c.aU;
+    o[IPGPU_TR_ALPHA_INV + 3] = c.aV;
+    o[IPGPU_TR_RHO_INV + 1] = c.rQ;
+    o[IPGPU_TR_RHO_INV + 2] = c.rU;
+    o[IPGPU_TR_RHO_INV + 3] = c.rV;
+    {
+      Coef cu;
+      coefficients(F, t.K, P.emission_type, false, ju, ku, cu);
+      o[IPGPU_TR_J_UNPOL] = ju;
+      o[IPGPU_TR_K_UNPOL] = ku;
+    }
+    o[IPGPU_TR_DL] = t.dl * conv;
+    for (int k = 1; k < 4; k++) o[IPGPU_TR_X + k] = X[k];
+    for (int k = 0; k < 4; k++) o[IPGPU_TR_KCON + k] = t.K[k];
+    double rr, th;
+    bl_coord(G, X[1], X[2], rr, th);
+    o[IPGPU_TR_R] = rr;
+    o[IPGPU_TR_TH] = th;
+    o[IPGPU_TR_PHI] = X[3];
+    /* plasma tetrad at point i (no b^2 <= 0 replacement here, as in the
+     * reference's trace) */
+    double E[4][4], Ec[4][4];
+    plasma_tetrad(F.Ucon, t.K, F.Bcon, m.g, m.sqrtmg, E);
+    for (int a = 0; a < 4; a++) lower(m.g, E[a], Ec[a]);
+    for (int k = 0; k < 4; k++) Ec[0][k] = -Ec[0][k];
+    for (int a = 0; a < 4; a++)
+      for (int k = 0; k < 4; k++) {
+        o[IPGPU_TR_ECON + 4 * a + k] = E[a][k];
+        o[IPGPU_TR_ECOV + 4 * a + k] = Ec[a][k];
+      }
+    /* e2 carried by parallel_transport_vector twice (src/ipolarray.c:297-320,
+     * src/io.c:393-408): to the step's half point with the connection at X_i,
+     * then to X_{i-1} with the connection at the half point */
+    {
+      Rec tm;
+      load_rec(rec, i - 1, tm);
+      double Xp[4] = {out[(size_t)(i - 1) * IPGPU_TRACE_NF + IPGPU_TR_X], tm.X[1], tm.X[2], tm.X[3]};
+      Metric mp;
+      metric(G, Xp[1], Xp[2], mp);
+      double Xh[4], Kh[4];
+      half_step(mp, Xp, tm.K, -t.dl, Xh, Kh);   /* the geodesic step's half point (traj[i].Xhalf) */
+      double M0[4][4], Mh[4][4];
+      transport_matrix(m, t.K, M0);
+      Metric mh;
+      metric(G, Xh[1], Xh[2], mh);
+      transport_matrix(mh, Kh, Mh);
+      /* the reference passes the step in cgs and divides it back */
+      const double dlc = (t.dl * conv) / conv;
+      double e2m[4], e2f[4];
+      for (int a = 0; a < 4; a++) {
+        double sacc = 0.;
+        for (int b = 0; b < 4; b++) sacc += M0[a][b] * E[2][b];
+        e2m[a] = E[2][a] - sacc * dlc;
+      }
+      for (int a = 0; a < 4; a++) {
+        double sacc = 0.;
+        for (int b = 0; b < 4; b++) sacc += Mh[a][b] * e2m[b];
+        e2f[a] = E[2][a] - sacc * dlc;
+      }
+      double *om = out + (size_t)(i - 1) * IPGPU_TRACE_NF;
+      for (int k = 0; k < 4; k++) {
+        o[IPGPU_TR_E2C + k] = E[2][k];
+        o[IPGPU_TR_E2C_COV + k] = Ec[2][k];
+        om[IPGPU_TR_E2T + k] = e2f[k];
+      }
+      /* e2Transported at i (carried from i + 1), lowered at X_i */
+      double e2i[4] = {o[IPGPU_TR_E2T], o[IPGPU_TR_E2T + 1], o[IPGPU_TR_E2T + 2], o[IPGPU_TR_E2T + 3]};
+      double e2c[4];
+      lower(m.g, e2i, e2c);
+      for (int k = 0; k < 4; k++) o[IPGPU_TR_E2T_COV + k] = e2c[k];
+    }
+    /* N before the step: coordinate and tetrad components (real parts), the
+     * plasma-frame Stokes parameters */
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) {
+        double re, im;
+        herm_get(N, a, b, re, im);
+        o[IPGPU_TR_NCOORD + 4 * a + b] = re;
+      }
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) {
+        double sacc = 0.;
+        for (int mu = 0; mu < 4; mu++)
+          for (int nu = 0; nu < 4; nu++) {
+            double re, im;
+            herm_get(N, mu, nu, re, im);
+            sacc += Ec[a][mu] * Ec[b][nu] * re;
+          }
+        o[IPGPU_TR_NTETRAD + 4 * a + b] = sacc;
+      }
+    {
+      double S[4];
+      herm_to_stokes(N, Ec[1], Ec[2], S);
+      for (int k = 0; k < 4; k++) o[IPGPU_TR_STOKES + k] = S[k];
+    }
+    /* integrate_emission over one step, from point i + 1 to i: I, tau and
+     * tauF start from 0 and the cached coefficients from point i + 1; N
+     * carries over */
+    {
+      Pkt pk;
+      for (int f = 0; f < PK; f++) pk.v[f] = 0.;
+      prep_step<false>(G, Mo, P, i + 1, rec, ntp, dump, pk.v, pol, unit_scale());
+      Chain st;
+      double js, ks;
+      far_end_coefficients<false>(G, Mo, P, i + 1, rec, dump, js, ks, unit_scale());
+      chain_init(st, js, ks);
+      st.N = N;
+      st.Nnz = herm_nonzero(N);
+      Rec ti;
+      load_rec(rec, i + 1, ti);
+      Metric mi;
+      metric(G, ti.X[1], ti.X[2], mi);
+      transport_matrix(mi, ti.K, st.Mi);
+      chain_step(st, pk.v, pol, P.stokes_floors);
+      N = st.N;
+      o[IPGPU_TR_I_UNPOL] = st.I;
+    }
+  }
+  /* X holds index 1 .. nstep - 1 only */
+  out[IPGPU_TR_X] = 0.;
+  out[(size_t)ns * IPGPU_TRACE_NF + IPGPU_TR_X] = 0.;
+}
+
+
 }  // namespace ipg

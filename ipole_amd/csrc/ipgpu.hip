@@ -936,6 +936,17 @@ __global__ __launch_bounds__(WAVE) void k_seg(int nseg, int seg_len, const long 
   if (b == NPAR) segflag[sgi] = fl;
 }
 
+/* ------------------------------------------------------- pixel trace output */
+
+/* One lane works (trace_pixel_records): a debugging and parity tool. */
+__global__ void k_trace_pixel(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P, int pi, int pj, int cap, int prims_ok,
+                              const double *__restrict__ dump, double *__restrict__ rec, int *__restrict__ nt,
+                              double *__restrict__ out, int *__restrict__ nstep_out)
+{
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  trace_pixel_records(G, Mo, P, pi, pj, cap, prims_ok, dump, rec, nt, out, nstep_out);
+}
+
 }  // namespace
 
 /* ------------------------------------------------------------------ C-ABI */
@@ -1031,6 +1042,10 @@ struct ipgpu_ctx {
   Herm *d_ckpt = nullptr, *d_seg = nullptr;
   size_t cap_ckpt = 0, cap_seg = 0;
   double *d_gcell = nullptr, *d_out4 = nullptr, *d_target = nullptr, *d_scale = nullptr;
+  /* pixel trace output */
+  double *d_trrec = nullptr, *d_trout = nullptr;
+  int *d_trnt = nullptr;
+  size_t cap_trrec = 0, cap_trout = 0, cap_trnt = 0;
   size_t cap_gcell = 0, cap_out4 = 0, cap_target = 0, cap_scale = 0;
 };
 
@@ -1117,7 +1132,7 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
                   ctx->d_flag_pix, ctx->d_status, ctx->d_counts,
                   ctx->d_groups, ctx->d_jobs, ctx->d_items, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
                   ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_ckpt, ctx->d_seg, ctx->d_gcell,
-                  ctx->d_out4, ctx->d_target, ctx->d_scale};
+                  ctx->d_out4, ctx->d_target, ctx->d_scale, ctx->d_trrec, ctx->d_trout, ctx->d_trnt};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int q = 0; q < NEV; q++)
@@ -2186,6 +2201,35 @@ int ipgpu_kernel_times(const ipgpu_ctx *ctx, double *g, double *x, double *t)
   if (g) *g = ctx->ms_geo;
   if (x) *x = ctx->ms_xfer;
   if (t) *t = ctx->ms_total;
+  return IPGPU_OK;
+}
+
+int ipgpu_trace_pixel(ipgpu_ctx *ctx, const ipgpu_image *img, int i, int j, int cap, int *nstep, double *out)
+{
+  if (!ctx || !img || !nstep || !out) return IPGPU_ERR_ARG;
+  if (!ctx->has_geom || !ctx->has_model) return set_err(ctx, IPGPU_ERR_STATE, "geometry/model not set");
+  if (ctx->model.kind == IPGPU_MODEL_IHARM && !ctx->d_dump)
+    return set_err(ctx, IPGPU_ERR_STATE, "no fluid dump uploaded");
+  const ipgpu_image &P = *img;
+  if (i < 0 || j < 0 || i >= P.nx || j >= P.ny) return set_err(ctx, IPGPU_ERR_ARG, "pixel outside the image");
+  if (P.maxnstep < 2 || cap < P.maxnstep) return set_err(ctx, IPGPU_ERR_ARG, "trace capacity below maxnstep");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int rc;
+  if ((rc = grow(ctx, ctx->d_trrec, ctx->cap_trrec, (size_t)P.maxnstep * REC))) return rc;
+  if ((rc = grow(ctx, ctx->d_trout, ctx->cap_trout, (size_t)P.maxnstep * IPGPU_TRACE_NF))) return rc;
+  if ((rc = grow(ctx, ctx->d_trnt, ctx->cap_trnt, (size_t)P.maxnstep))) return rc;
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_trout, 0, (size_t)P.maxnstep * IPGPU_TRACE_NF * sizeof(double), ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
+  k_trace_pixel<<<1, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, i, j, P.maxnstep, ctx->has_units ? 1 : 0,
+                                              ctx->d_dump, ctx->d_trrec, ctx->d_trnt, ctx->d_trout, ctx->d_status);
+  HIPCHK(ctx, hipGetLastError());
+  int ns = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&ns, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  *nstep = ns;
+  if (ns >= P.maxnstep - 1) return set_err(ctx, IPGPU_ERR_MAXNSTEP, "MaxNStep exceeded in the traced pixel");
+  HIPCHK(ctx, hipMemcpy(out, ctx->d_trout, (size_t)P.maxnstep * IPGPU_TRACE_NF * sizeof(double),
+                        hipMemcpyDeviceToHost));
   return IPGPU_OK;
 }
 

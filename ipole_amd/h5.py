@@ -19,6 +19,20 @@ class DumpHeader(C.Structure):
                 ("t", C.c_double), ("dump_cadence", C.c_double)]
 
 
+class ImageMeta(C.Structure):
+    """ipole_image_meta of host/h5io.h: what the image and trace headers hold."""
+    _fields_ = ([("version", C.c_char * 32), ("githash", C.c_char * 32)]
+                + [(k, C.c_int) for k in ("nx", "ny", "only_unpolarized", "qu_conv", "model_kind",
+                                          "reverse_field")]
+                + [(k, C.c_double) for k in ("freqcgs", "scale", "dsource", "dx", "dy", "fovx_dsource",
+                                             "fovy_dsource", "rcam", "thetacam", "phicam", "rotcam",
+                                             "fovx", "fovy", "xoff", "yoff")]
+                + [("Xcam", D4)]
+                + [(k, C.c_double) for k in ("t", "sigma_cut", "trat_small", "trat_large", "beta_crit",
+                                             "L_unit", "M_unit", "T_unit", "Thetae_unit", "Mdot",
+                                             "MdotEdd", "Ladv")])
+
+
 _lib = None
 
 
@@ -32,6 +46,12 @@ def lib():
         L.ipole_h5_read_dump_header.argtypes = [C.c_char_p, C.POINTER(DumpHeader)]
         L.ipole_h5_read_dump_prims.argtypes = [C.c_char_p, C.POINTER(DumpHeader), dp]
         L.ipole_h5_read_image.argtypes = [C.c_char_p, C.c_int, C.c_int, dp, dp, dp, dp, dp]
+        L.ipole_h5_trace_open.argtypes = [C.c_char_p, C.POINTER(ImageMeta), C.c_int, C.c_int, C.c_int]
+        L.ipole_h5_trace_open.restype = C.c_int64
+        L.ipole_h5_trace_write.argtypes = [C.c_int64, C.c_int, C.c_int, C.c_int, dp]
+        L.ipole_h5_trace_close.argtypes = [C.c_int64]
+        L.ipole_h5_dataset_shape.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_int64)]
+        L.ipole_h5_read_doubles.argtypes = [C.c_char_p, C.c_char_p, dp]
         _lib = L
     return _lib
 
@@ -82,3 +102,42 @@ def read_image(path, nx, ny, polarized=True):
     if rc:
         raise IOError("could not read image %s" % path)
     return dict(image=image, imageS=imageS, taus=taus, Ftot=Ft.value, Ftot_unpol=Fu.value)
+
+
+def dataset_shape(path, name):
+    """Dims of a dataset, or None when it is missing."""
+    d = (C.c_int64 * 16)()
+    r = lib().ipole_h5_dataset_shape(path.encode(), name.encode(), d)
+    return None if r < 0 else tuple(int(d[k]) for k in range(r))
+
+
+def read_dataset(path, name):
+    """Any numeric dataset as float64."""
+    shape = dataset_shape(path, name)
+    if shape is None:
+        raise KeyError(name)
+    out = np.zeros(shape)
+    if lib().ipole_h5_read_doubles(path.encode(), name.encode(), out.ctypes.data_as(dp)):
+        raise IOError("could not read %s from %s" % (name, path))
+    return out
+
+
+def write_trace(path, meta, nx, ny, maxnstep, pixels):
+    """Trace file of src/io.c:279-543: pixels is a list of (i, j, nstep,
+    records) with records as Context.trace_pixel returns them; meta an
+    ImageMeta for the header (None: zeros).  An existing file is appended
+    to, as the reference does."""
+    L = lib()
+    meta = meta if meta is not None else ImageMeta()
+    f = L.ipole_h5_trace_open(path.encode(), C.byref(meta), nx, ny, maxnstep)
+    if f < 0:
+        raise IOError("could not open %s" % path)
+    try:
+        for i, j, ns, rec in pixels:
+            r = np.ascontiguousarray(rec, dtype=np.float64)
+            if r.shape[0] < ns + 1:
+                raise ValueError("records shorter than nstep + 1")
+            if L.ipole_h5_trace_write(f, i, j, ns, r.ctypes.data_as(dp)):
+                raise IOError("could not write pixel %d %d of %s" % (i, j, path))
+    finally:
+        L.ipole_h5_trace_close(f)

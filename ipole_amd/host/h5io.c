@@ -7,7 +7,11 @@
 
 #include <hdf5.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
+
+#include "ipgpu.h"
 
 static int w_scalar(hid_t loc, const char *name, hid_t type, const void *v)
 {
@@ -175,11 +179,11 @@ int ipole_h5_read_dump_prims(const char *path, const ipole_dump_header *h, doubl
 
 /* Image file, src/io.c:229-273 (dump) with the header of :140-191 and the
  * iharm model header of model/iharm/model.c:1254-1301. */
-int ipole_h5_write_image(const char *path, const ipole_image_meta *mt, const double *image,
-                         const double *imageS, const double *taus)
+/* write_header, src/io.c:140-191, with the model's output_hdf5
+ * (model/iharm/model.c:1254-1301): the /githash attribute, /header and, for
+ * iharm, Mdot, MdotEdd and Ladv at the root. */
+static int write_header(hid_t f, const ipole_image_meta *mt)
 {
-  hid_t f = H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
-  if (f < 0) return -1;
   int rc = 0;
   /* githash attribute on / */
   {
@@ -248,6 +252,15 @@ int ipole_h5_write_image(const char *path, const ipole_image_meta *mt, const dou
     H5Gclose(un);
   }
   H5Gclose(hd);
+  return rc;
+}
+
+int ipole_h5_write_image(const char *path, const ipole_image_meta *mt, const double *image,
+                         const double *imageS, const double *taus)
+{
+  hid_t f = H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+  if (f < 0) return -1;
+  int rc = write_header(f, mt);
   double Ftot_unpol = 0., Ftot = 0.;
   size_t nx = mt->nx, ny = mt->ny;
   for (size_t i = 0; i < nx; ++i)
@@ -301,4 +314,168 @@ int ipole_h5_read_image(const char *path, int nx, int ny, double *image, double 
   if (Ftot_unpol) rc |= r_scalar(f, "/Ftot_unpol", H5T_NATIVE_DOUBLE, Ftot_unpol);
   H5Fclose(f);
   return rc ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------ traces */
+
+/* Datasets of the trace file (src/io.c:445-520): name, offset of the field in
+ * a record of IPGPU_TRACE_NF doubles, and the per-step shape (scalar, 4, 8 or
+ * 4 x 4). */
+static const struct {
+  const char *name;
+  int off, rank, d0, d1;
+} TRACE_SETS[] = {
+    {"b", IPGPU_TR_B, 0, 0, 0},
+    {"ne", IPGPU_TR_NE, 0, 0, 0},
+    {"thetae", IPGPU_TR_THETAE, 0, 0, 0},
+    {"nu", IPGPU_TR_NU, 0, 0, 0},
+    {"b_k_angle", IPGPU_TR_BK_ANGLE, 0, 0, 0},
+    {"sigma", IPGPU_TR_SIGMA, 0, 0, 0},
+    {"beta", IPGPU_TR_BETA, 0, 0, 0},
+    {"dl", IPGPU_TR_DL, 0, 0, 0},
+    {"r", IPGPU_TR_R, 0, 0, 0},
+    {"th", IPGPU_TR_TH, 0, 0, 0},
+    {"phi", IPGPU_TR_PHI, 0, 0, 0},
+    {"j_unpol", IPGPU_TR_J_UNPOL, 0, 0, 0},
+    {"k_unpol", IPGPU_TR_K_UNPOL, 0, 0, 0},
+    {"I_unpol", IPGPU_TR_I_UNPOL, 0, 0, 0},
+    {"X", IPGPU_TR_X, 1, 4, 0},
+    {"Kcon", IPGPU_TR_KCON, 1, 4, 0},
+    {"j_inv", IPGPU_TR_J_INV, 1, 4, 0},
+    {"alpha_inv", IPGPU_TR_ALPHA_INV, 1, 4, 0},
+    {"rho_inv", IPGPU_TR_RHO_INV, 1, 4, 0},
+    {"stokes", IPGPU_TR_STOKES, 1, 4, 0},
+    {"e2Constructed", IPGPU_TR_E2C, 1, 4, 0},
+    {"e2ConstructedCov", IPGPU_TR_E2C_COV, 1, 4, 0},
+    {"e2Transported", IPGPU_TR_E2T, 1, 4, 0},
+    {"e2TransportedCov", IPGPU_TR_E2T_COV, 1, 4, 0},
+    {"prims", IPGPU_TR_PRIMS, 1, 8, 0},
+    {"ntetrad", IPGPU_TR_NTETRAD, 2, 4, 4},
+    {"ncoord", IPGPU_TR_NCOORD, 2, 4, 4},
+    {"Econ", IPGPU_TR_ECON, 2, 4, 4},
+    {"Ecov", IPGPU_TR_ECOV, 2, 4, 4},
+};
+#define NTRACE_SETS ((int)(sizeof(TRACE_SETS) / sizeof(TRACE_SETS[0])))
+
+int64_t ipole_h5_trace_open(const char *path, const ipole_image_meta *mt, int nx, int ny, int maxnstep)
+{
+  if (access(path, F_OK) != -1) { /* src/io.c:282-287: append to an existing file */
+    hid_t f = H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT);
+    return f < 0 ? -1 : (int64_t)f;
+  }
+  hid_t f = H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+  if (f < 0) return -1;
+  int rc = write_header(f, mt);
+  /* nstep per pixel, then the per-step sets chunked by 200 steps; chunks no
+   * pixel writes stay unallocated and read as 0 */
+  hsize_t d2[2] = {(hsize_t)nx, (hsize_t)ny};
+  hid_t sp = H5Screate_simple(2, d2, NULL);
+  hid_t ds = H5Dcreate2(f, "nstep", H5T_STD_I32LE, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+  if (ds < 0) rc = -1;
+  else H5Dclose(ds);
+  H5Sclose(sp);
+  const hsize_t ck = maxnstep < 200 ? (hsize_t)maxnstep : 200;
+  for (int q = 0; q < NTRACE_SETS && rc == 0; q++) {
+    hsize_t dims[5] = {(hsize_t)nx, (hsize_t)ny, (hsize_t)maxnstep, (hsize_t)TRACE_SETS[q].d0,
+                       (hsize_t)TRACE_SETS[q].d1};
+    hsize_t chunk[5] = {1, 1, ck, (hsize_t)TRACE_SETS[q].d0, (hsize_t)TRACE_SETS[q].d1};
+    int rank = 3 + TRACE_SETS[q].rank;
+    hid_t s3 = H5Screate_simple(rank, dims, NULL);
+    hid_t pl = H5Pcreate(H5P_DATASET_CREATE);
+    H5Pset_chunk(pl, rank, chunk);
+    hid_t d = H5Dcreate2(f, TRACE_SETS[q].name, H5T_IEEE_F64LE, s3, H5P_DEFAULT, pl, H5P_DEFAULT);
+    if (d < 0) rc = -1;
+    else H5Dclose(d);
+    H5Pclose(pl);
+    H5Sclose(s3);
+  }
+  if (rc) {
+    H5Fclose(f);
+    return -1;
+  }
+  return (int64_t)f;
+}
+
+int ipole_h5_trace_write(int64_t fid, int i, int j, int nstep, const double *rec)
+{
+  hid_t f = (hid_t)fid;
+  int rc = 0;
+  const size_t nsteps = (size_t)nstep + 1;
+  {
+    hid_t ds = H5Dopen2(f, "nstep", H5P_DEFAULT);
+    if (ds < 0) return -1;
+    hid_t fs = H5Dget_space(ds);
+    hsize_t st[2] = {(hsize_t)i, (hsize_t)j}, cnt[2] = {1, 1};
+    H5Sselect_hyperslab(fs, H5S_SELECT_SET, st, NULL, cnt, NULL);
+    hid_t ms = H5Screate_simple(2, cnt, NULL);
+    if (H5Dwrite(ds, H5T_NATIVE_INT, ms, fs, H5P_DEFAULT, &nstep) < 0) rc = -1;
+    H5Sclose(ms);
+    H5Sclose(fs);
+    H5Dclose(ds);
+  }
+  double *buf = malloc(sizeof(double) * nsteps * 16);
+  if (!buf) return -1;
+  for (int q = 0; q < NTRACE_SETS && rc == 0; q++) {
+    const int w = TRACE_SETS[q].rank == 0 ? 1 : (TRACE_SETS[q].rank == 1 ? TRACE_SETS[q].d0 : 16);
+    for (size_t s = 0; s < nsteps; s++)
+      for (int k = 0; k < w; k++) buf[s * w + k] = rec[s * IPGPU_TRACE_NF + TRACE_SETS[q].off + k];
+    hid_t ds = H5Dopen2(f, TRACE_SETS[q].name, H5P_DEFAULT);
+    if (ds < 0) {
+      rc = -1;
+      break;
+    }
+    hid_t fs = H5Dget_space(ds);
+    const int rank = H5Sget_simple_extent_ndims(fs);
+    hsize_t dims[5], st[5] = {(hsize_t)i, (hsize_t)j, 0, 0, 0}, cnt[5];
+    H5Sget_simple_extent_dims(fs, dims, NULL);
+    if ((hsize_t)i >= dims[0] || (hsize_t)j >= dims[1] || nsteps > dims[2]) rc = -1;
+    cnt[0] = cnt[1] = 1;
+    cnt[2] = nsteps;
+    for (int r = 3; r < rank; r++) cnt[r] = dims[r];
+    if (rc == 0) {
+      H5Sselect_hyperslab(fs, H5S_SELECT_SET, st, NULL, cnt, NULL);
+      hid_t ms = H5Screate_simple(rank, cnt, NULL);
+      if (H5Dwrite(ds, H5T_NATIVE_DOUBLE, ms, fs, H5P_DEFAULT, buf) < 0) rc = -1;
+      H5Sclose(ms);
+    }
+    H5Sclose(fs);
+    H5Dclose(ds);
+  }
+  free(buf);
+  return rc;
+}
+
+int ipole_h5_trace_close(int64_t fid) { return H5Fclose((hid_t)fid) < 0 ? -1 : 0; }
+
+int ipole_h5_dataset_shape(const char *path, const char *name, int64_t *dims)
+{
+  hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+  if (f < 0) return -1;
+  int rank = -1;
+  if (H5Lexists(f, name, H5P_DEFAULT) > 0) {
+    hid_t ds = H5Dopen2(f, name, H5P_DEFAULT);
+    if (ds >= 0) {
+      hid_t fs = H5Dget_space(ds);
+      hsize_t d[16];
+      rank = H5Sget_simple_extent_ndims(fs);
+      if (rank > 16) rank = -1;
+      else if (rank >= 0) {
+        H5Sget_simple_extent_dims(fs, d, NULL);
+        for (int r = 0; r < rank; r++) dims[r] = (int64_t)d[r];
+      }
+      H5Sclose(fs);
+      H5Dclose(ds);
+    }
+  }
+  H5Fclose(f);
+  return rank;
+}
+
+int ipole_h5_read_doubles(const char *path, const char *name, double *out)
+{
+  hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+  if (f < 0) return -1;
+  int rc = r_scalar(f, name, H5T_NATIVE_DOUBLE, out);
+  H5Fclose(f);
+  return rc;
 }

@@ -2,6 +2,8 @@
 #ifndef IPOLE_H5IO_H
 #define IPOLE_H5IO_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -32,6 +34,19 @@ int ipole_h5_write_image(const char *path, const ipole_image_meta *mt, const dou
 int ipole_h5_copy_fluid_header(const char *dump_path, const char *image_path);
 int ipole_h5_read_image(const char *path, int nx, int ny, double *image, double *imageS,
                         double *taus, double *Ftot, double *Ftot_unpol);
+
+/* Trace file, src/io.c:279-543: open appends to an existing file, else
+ * creates it with the image header and zero-filled datasets of nx x ny
+ * pixels and maxnstep steps (returns a file handle, or -1).  write stores
+ * pixel (i, j): its nstep and records 0 .. nstep of IPGPU_TRACE_NF doubles
+ * as ipgpu_trace_pixel returns them. */
+int64_t ipole_h5_trace_open(const char *path, const ipole_image_meta *mt, int nx, int ny, int maxnstep);
+int ipole_h5_trace_write(int64_t fid, int i, int j, int nstep, const double *rec);
+int ipole_h5_trace_close(int64_t fid);
+
+/* Any dataset: its rank and dims (up to 16), or -1; its values as doubles. */
+int ipole_h5_dataset_shape(const char *path, const char *name, int64_t *dims);
+int ipole_h5_read_doubles(const char *path, const char *name, double *out);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <unistd.h>
 
 #include "h5io.h"
 #include "ipgpu.h"
@@ -40,7 +41,8 @@ typedef struct {
   int emission_type, stokes_floors, isolate_counterjet, target_nturns, subring_dtheta;
   double eps, xoff, yoff;
   int maxnstep, trace, histo, histo_polar, nx_min, ny_min;
-  char outf[2048];
+  int trace_stride, trace_i, trace_j;
+  char outf[2048], trace_outf[2048];
   /* model keys: model/iharm/model.c:130-167, model/sphere/model.c:33-42 */
   char model[32], dump[2048];
   double MBH, M_unit, trat_small, trat_large, beta_crit, sigma_cut, sigma_cut_high;
@@ -67,6 +69,10 @@ static void defaults(params_t *p)
   p->nx_min = -1;
   p->ny_min = -1;
   strcpy(p->outf, "image.h5");
+  p->trace_stride = 1;
+  p->trace_i = -1;
+  p->trace_j = -1;
+  strcpy(p->trace_outf, "trace.h5");
   /* model defaults */
   strcpy(p->model, "iharm");
   strcpy(p->dump, "dump.h5");
@@ -140,6 +146,10 @@ static void try_set(const char *w, const char *v, params_t *p)
   set_d(w, v, "yoff", &p->yoff);
   set_s(w, v, "outfile", p->outf);
   set_i(w, v, "trace", &p->trace);
+  set_i(w, v, "trace_stride", &p->trace_stride);
+  set_i(w, v, "trace_i", &p->trace_i);
+  set_i(w, v, "trace_j", &p->trace_j);
+  set_s(w, v, "trace_outf", p->trace_outf);
   set_i(w, v, "histo", &p->histo);
   set_i(w, v, "histo_polar", &p->histo_polar);
   /* model keys */
@@ -244,7 +254,6 @@ int main(int argc, char *argv[])
   params_t par;
   defaults(&par);
   load_par_from_argv(argc, argv, &par);
-  if (par.trace) fprintf(stderr, "! trace output is not produced by this build\n");
   if (par.nx_min > 0 && par.nx_min != par.nx)
     fprintf(stderr, "! adaptive refinement is not produced by this build: rendering %dx%d\n", par.nx, par.ny);
   if (par.target_mdot > 0) {
@@ -342,6 +351,8 @@ int main(int argc, char *argv[])
     model.sph_B0 = CL * sqrt(8 * M_PI * (gam - 1.) * (MP + ME) / par.sph_beta0)
         * sqrt(model.sph_Ne0 * par.sph_Thetae0) / sqrt(THETAE_UNIT);
     model.sph_Thetae0 = par.sph_Thetae0;
+    model.sph_sigma = par.sph_Thetae0 * par.sph_tp_over_te / par.sph_beta0;
+    model.sph_beta = par.sph_beta0;
     geom.metric = IPGPU_METRIC_EMINKOWSKI;
     geom.N1 = 300;
     geom.N2 = 100;
@@ -479,8 +490,8 @@ int main(int argc, char *argv[])
     fprintf(stderr, "LP,CP [%%]: %g %g\n", LPfrac, CPfrac);
   }
 
-  if (!par.quench_output) {
-    ipole_image_meta mt;
+  ipole_image_meta mt;
+  {
     memset(&mt, 0, sizeof(mt));
     strcpy(mt.version, VERSION_STRING);
     strcpy(mt.githash, "ipole_amd");
@@ -515,6 +526,58 @@ int main(int argc, char *argv[])
     mt.M_unit = par.M_unit;
     mt.T_unit = T_unit;
     mt.Thetae_unit = Thetae_unit;
+  }
+
+  /* trace output of get_pixel, src/main.c:865-880, written with
+   * dump_var_along's layout (src/io.c:279-543): pixel (trace_i, trace_j)
+   * alone, or every trace_stride-th pixel in both directions */
+  if (par.trace) {
+    const int stride = par.trace_stride > 0 ? par.trace_stride : 1;
+    const int single = !(par.trace_i < 0 || par.trace_j < 0);
+    const int tnx = single ? 1 : par.nx / stride, tny = single ? 1 : par.ny / stride;
+    double *rec = malloc(sizeof(double) * (size_t)par.maxnstep * IPGPU_TRACE_NF);
+    int64_t tf = -1;
+    if (!rec) {
+      fprintf(stderr, "Could not allocate trace memory!");
+      exit(-1);
+    }
+    for (int i = 0; i < par.nx; i++)
+      for (int j = 0; j < par.ny; j++) {
+        int ti, tj;
+        if (single) {
+          if (i != par.trace_i || j != par.trace_j) continue;
+          ti = tj = 0;
+        } else {
+          if (i % stride != 0 || j % stride != 0) continue;
+          ti = i / stride;
+          tj = j / stride;
+          if (ti >= tnx || tj >= tny) continue; /* outside the file's nx/stride x ny/stride */
+        }
+        if (tf < 0) {
+          int existed = access(par.trace_outf, F_OK) != -1;
+          tf = ipole_h5_trace_open(par.trace_outf, &mt, tnx, tny, par.maxnstep);
+          if (tf < 0) {
+            fprintf(stderr, "! could not open %s\n", par.trace_outf);
+            exit(-1);
+          }
+          if (!existed && is_iharm) {
+            ipole_h5_trace_close(tf);
+            ipole_h5_copy_fluid_header(par.dump, par.trace_outf);
+            tf = ipole_h5_trace_open(par.trace_outf, &mt, tnx, tny, par.maxnstep);
+          }
+        }
+        int ns = 0;
+        CHECK(ipgpu_trace_pixel(ctx, &im, i, j, par.maxnstep, &ns, rec));
+        if (ipole_h5_trace_write(tf, ti, tj, ns, rec) != 0) {
+          fprintf(stderr, "! could not write the trace of pixel %d %d to %s\n", i, j, par.trace_outf);
+          exit(-1);
+        }
+      }
+    if (tf >= 0) ipole_h5_trace_close(tf);
+    free(rec);
+  }
+
+  if (!par.quench_output) {
     if (ipole_h5_write_image(par.outf, &mt, image, imageS, taus) != 0) {
       fprintf(stderr, "! could not write %s\n", par.outf);
       exit(-1);

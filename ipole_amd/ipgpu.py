@@ -38,7 +38,8 @@ class Geom(C.Structure):
 class Model(C.Structure):
     _fields_ = [("kind", C.c_int), ("L_unit", C.c_double), ("th_beg", C.c_double),
                 ("sigma_cut", C.c_double), ("sigma_cut_high", C.c_double), ("sph_R0", C.c_double),
-                ("sph_Ne0", C.c_double), ("sph_Thetae0", C.c_double), ("sph_B0", C.c_double)]
+                ("sph_Ne0", C.c_double), ("sph_Thetae0", C.c_double), ("sph_B0", C.c_double),
+                ("sph_sigma", C.c_double), ("sph_beta", C.c_double)]
 
 
 class Units(C.Structure):
@@ -61,6 +62,31 @@ class Job(C.Structure):
     _fields_ = [("image", Image), ("m_unit_scale", C.c_double), ("image_out", C.POINTER(C.c_double)),
                 ("imageS_out", C.POINTER(C.c_double)), ("taus_out", C.POINTER(C.c_double)),
                 ("nstep_out", C.POINTER(C.c_int)), ("flag_out", C.POINTER(C.c_int))]
+
+
+# Trace output record (include/ipgpu.h IPGPU_TR_*): dataset name of the
+# reference's trace file (src/io.c:445-520), offset in the record, per-step
+# shape.
+TRACE_NF = 126
+TRACE_SETS = [
+    ("prims", 0, (8,)), ("b", 8, ()), ("ne", 9, ()), ("thetae", 10, ()), ("nu", 11, ()),
+    ("b_k_angle", 12, ()), ("sigma", 13, ()), ("beta", 14, ()), ("dl", 15, ()), ("r", 16, ()),
+    ("th", 17, ()), ("phi", 18, ()), ("j_unpol", 19, ()), ("k_unpol", 20, ()), ("I_unpol", 21, ()),
+    ("X", 22, (4,)), ("Kcon", 26, (4,)), ("j_inv", 30, (4,)), ("alpha_inv", 34, (4,)),
+    ("rho_inv", 38, (4,)), ("stokes", 42, (4,)), ("e2Constructed", 46, (4,)),
+    ("e2ConstructedCov", 50, (4,)), ("e2Transported", 54, (4,)), ("e2TransportedCov", 58, (4,)),
+    ("ntetrad", 62, (4, 4)), ("ncoord", 78, (4, 4)), ("Econ", 94, (4, 4)), ("Ecov", 110, (4, 4)),
+]
+
+
+def trace_fields(rec):
+    """Splits trace records (nsteps, TRACE_NF) into the named per-step arrays."""
+    out = {}
+    for name, off, shape in TRACE_SETS:
+        w = int(np.prod(shape)) if shape else 1
+        a = rec[:, off:off + w]
+        out[name] = a.reshape((rec.shape[0],) + shape) if shape else a[:, 0]
+    return out
 
 
 class IpgpuError(RuntimeError):
@@ -105,6 +131,7 @@ def lib():
         L.ipgpu_stream.argtypes = [vp]
         L.ipgpu_stream.restype = vp
         L.ipgpu_native_coord.argtypes = [C.POINTER(Geom), C.c_double, C.c_double, C.c_double, dp]
+        L.ipgpu_trace_pixel.argtypes = [vp, C.POINTER(Image), C.c_int, C.c_int, C.c_int, ip, dp]
         _lib = L
     return _lib
 
@@ -172,6 +199,8 @@ def sphere_setup(R0=100., tau0=1.e-5, Thetae0=10., beta0=20., gam=13. / 9, tp_ov
     theta_unit = MP / ME * (gam - 1.) / (1. + tp_over_te)
     m.sph_B0 = CL * np.sqrt(8 * np.pi * (gam - 1.) * (MP + ME) / beta0) * np.sqrt(m.sph_Ne0 * Thetae0) / np.sqrt(theta_unit)
     m.sph_Thetae0 = Thetae0
+    m.sph_sigma = Thetae0 * tp_over_te / beta0
+    m.sph_beta = beta0
     g = Geom()
     g.metric = METRIC_EMINKOWSKI
     g.N1, g.N2, g.N3 = 300, 100, 1
@@ -391,6 +420,17 @@ class Context:
         ray; -1 keeps a setting."""
         self._chk(self.L.ipgpu_set_tuning(self.h, int(trace_budget), int(trace_phases), int(seg_min_steps),
                                           int(seg_len), int(record_steps)))
+
+    def trace_pixel(self, image, i, j):
+        """Trace output of pixel (i, j): (nstep, records) with records of
+        shape (nstep + 1, TRACE_NF), the values the reference's trace file
+        holds for the pixel (src/io.c:279-543); trace_fields names them."""
+        cap = image.maxnstep
+        rec = np.zeros((cap, TRACE_NF))
+        ns = C.c_int()
+        rc = self.L.ipgpu_trace_pixel(self.h, C.byref(image), int(i), int(j), cap, C.byref(ns), P(rec))
+        self._chk(rc)
+        return ns.value, rec[:ns.value + 1].copy()
 
     def step_counts(self):
         a, b = C.c_longlong(), C.c_longlong()

@@ -236,6 +236,7 @@ void or_fourv(const double X[4], const double K[4], double Ucon[4], double Ucov[
 /* model/iharm/model.c:545-553 */
 double or_sigma(const double X[4])
 {
+  if (OR_M.kind == OR_MODEL_SPHERE) return or_radiating_region(X) ? OR_M.sph_sigma : 0.;
   if (OR_M.kind != OR_MODEL_IHARM) return 0.;
   if (X_in_domain(X) == 0) return 0.;
   return interp_scalar(X, field(9));
@@ -285,6 +286,26 @@ double or_b(const double X[4])
       * OR_M.an_B_unit;
 }
 
+/* get_model_primitives (model/iharm/model.c:498-508) and get_model_beta
+ * (:555-565) of the iharm model: rho, u interpolated from rub[0], rub[1],
+ * U1..B3 from the model fields, beta from rub[2] (rub: 3 ghosted arrays, or
+ * NULL, which leaves rho, u and beta 0).  The primitives stay 0 outside the
+ * domain and for other models; beta is 0 there too. */
+void or_primitives(const double X[4], const double *rub, double p8[8], double *beta)
+{
+  for (int n = 0; n < 8; n++) p8[n] = 0.;
+  *beta = 0.;
+  if (OR_M.kind == OR_MODEL_SPHERE && or_radiating_region(X)) *beta = OR_M.sph_beta;
+  if (OR_M.kind != OR_MODEL_IHARM || X_in_domain(X) == 0) return;
+  const size_t nc = (size_t)(OR_G.N1 + 2) * (OR_G.N2 + 2) * (OR_G.N3 + 2);
+  for (int n = 0; n < 6; n++) p8[2 + n] = interp_scalar(X, field(n));
+  if (rub) {
+    p8[0] = interp_scalar(X, rub);
+    p8[1] = interp_scalar(X, rub + nc);
+    *beta = interp_scalar(X, rub + 2 * nc);
+  }
+}
+
 /* ---------------------------------------------------------------- setup */
 
 /* model/sphere/model.c:58-96 */
@@ -301,6 +322,8 @@ void or_sphere_setup(double R0, double tau0, double Thetae0, double beta0, doubl
   m->sph_B0 = OR_CL * sqrt(8 * M_PI * (gam - 1.) * (OR_MP + OR_ME) / beta0)
       * sqrt(m->sph_Ne0 * Thetae0) / sqrt(THETAE_UNIT);
   m->sph_Thetae0 = Thetae0;
+  m->sph_sigma = Thetae0 * tp_over_te / beta0;
+  m->sph_beta = beta0;
 }
 
 /* model/analytic/model.c:70-150 */
@@ -353,6 +376,15 @@ void or_analytic_setup(int model, double MBH_solar, double rho_unit, double freq
  * prims: [N1][N2][N3][nprim] with rho, u, U1..U3, B1..B3 first.
  * out11: 11 ghosted arrays U1 U2 U3 B1 B2 B3 ne thetae b sigma beta. */
 void or_iharm_derive(const double *prims, int nprim, const or_units_t *u, double *out11)
+{
+  or_iharm_derive_ru(prims, nprim, u, out11, NULL);
+}
+
+/* The same, also returning the ghosted rho and u arrays (out_ru: 2 x ncell,
+ * may be NULL), which get_model_primitives interpolates for the trace output
+ * (model/iharm/model.c:498-508). */
+void or_iharm_derive_ru(const double *prims, int nprim, const or_units_t *u, double *out11,
+                        double *out_ru)
 {
   const int N1 = OR_G.N1, N2 = OR_G.N2, N3 = OR_G.N3;
   const size_t nc = (size_t)(N1 + 2) * (N2 + 2) * (N3 + 2);
@@ -465,5 +497,9 @@ void or_iharm_derive(const double *prims, int nprim, const or_units_t *u, double
     beta[c] = fmax(beta_m, OR_SMALL);
   }
   for (int l = 0; l < 6; l++) memcpy(out11 + l * nc, p[2 + l], nc * sizeof(double));
+  if (out_ru) {
+    memcpy(out_ru, p[0], nc * sizeof(double));
+    memcpy(out_ru + nc, p[1], nc * sizeof(double));
+  }
   free(pbuf);
 }

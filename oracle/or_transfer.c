@@ -506,6 +506,125 @@ static int pixel(long i, long j, const or_params_t *p, const or_camera_t *cam, o
   return 0;
 }
 
+/* src/ipolarray.c:297-320: N carried by dl (cgs) with the connection at
+ * Xm, Km */
+static void transport_vector(const double Xm[4], const double Km[4], const double Ni[4],
+                             const double Nm[4], double Nf[4], double dl)
+{
+  double lconn[4][4][4];
+  or_connection(Xm, lconn);
+  for (int i = 0; i < 4; i++) Nf[i] = Ni[i];
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      for (int k = 0; k < 4; k++)
+        Nf[i] += -(lconn[i][j][k] * Nm[j] * Km[k]) * dl / (OR_M.L_unit * OR_HPL / (OR_ME * OR_CL * OR_CL));
+}
+
+/* src/radiation.c:47-72 */
+static double bk_angle(const double K[4], const double Ucov[4], const double Bcon[4],
+                       const double Bcov[4])
+{
+  double B = sqrt(fabs(Bcon[0] * Bcov[0] + Bcon[1] * Bcov[1] + Bcon[2] * Bcov[2] + Bcon[3] * Bcov[3]));
+  if (B == 0.) return M_PI / 2.;
+  double k = fabs(K[0] * Ucov[0] + K[1] * Ucov[1] + K[2] * Ucov[2] + K[3] * Ucov[3]);
+  double mu = (K[0] * Bcov[0] + K[1] * Bcov[1] + K[2] * Bcov[2] + K[3] * Bcov[3]) / (k * B);
+  if (fabs(mu) > 1.) mu /= fabs(mu);
+  return acos(mu);
+}
+
+/* Trace output of pixel (i, j): get_pixel (src/main.c:822-883) with
+ * params.trace, then the per-step arrays of dump_var_along
+ * (src/io.c:279-543) as records of OR_TR_NF doubles at the OR_TR_* offsets,
+ * record s holding index s of every array (zero where the reference leaves
+ * its calloc'ed arrays unwritten).  rub: the ghosted rho, u, beta arrays of
+ * or_iharm_derive_ru / its beta, or NULL (those entries stay 0).  Returns 0,
+ * -10 when maxnstep is exceeded, -1 when cap < nstep + 1. */
+int or_trace_pixel(const or_params_t *p, const or_camera_t *cam, long i, long j, const double *rub,
+                   int cap, double *out, int *nstep_out)
+{
+  double X[4] = {0.}, Kcon[4] = {0.};
+  or_init_XK(i, j, p, cam, X, Kcon);
+  for (int mu = 0; mu < 4; mu++) Kcon[mu] *= cam->freq;
+  or_traj_t *traj = calloc((size_t)p->maxnstep, sizeof(or_traj_t));
+  int nstep = or_trace(X, Kcon, traj, p->eps, p->maxnstep, cam->Xcam, p->subring_dtheta);
+  *nstep_out = nstep;
+  if (nstep >= p->maxnstep - 1) {
+    free(traj);
+    return -10;
+  }
+  if (cap < nstep + 1) {
+    free(traj);
+    return -1;
+  }
+  memset(out, 0, sizeof(double) * (size_t)cap * OR_TR_NF);
+  double complex N_coord[4][4], N_tetrad[4][4];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) N_coord[a][b] = 0.;
+  double Intensity = 0, Tau = 0, tauF = 0;
+  for (int s = nstep - 1; s > 0; --s) {
+    double *o = out + (size_t)s * OR_TR_NF;
+    const or_traj_t *t = &traj[s];
+    double Ucon[4], Ucov[4], Bcon[4], Bcov[4];
+    or_primitives(t->X, rub, o + OR_TR_PRIMS, o + OR_TR_BETA);
+    or_fourv(t->X, t->Kcon, Ucon, Ucov, Bcon, Bcov);
+    o[OR_TR_B] = or_b(t->X);
+    o[OR_TR_NE] = or_ne(t->X);
+    o[OR_TR_THETAE] = or_thetae(t->X);
+    /* get_fluid_nu, src/radiation.c:20-42 */
+    o[OR_TR_NU] = -(t->Kcon[0] * Ucov[0] + t->Kcon[1] * Ucov[1] + t->Kcon[2] * Ucov[2]
+                    + t->Kcon[3] * Ucov[3]) * OR_ME * OR_CL * OR_CL / OR_HPL;
+    o[OR_TR_BK_ANGLE] = bk_angle(t->Kcon, Ucov, Bcon, Bcov);
+    o[OR_TR_SIGMA] = or_sigma(t->X);
+    double c[11];
+    or_jar(p->emission_type, 1, t->X, t->Kcon, c);
+    for (int q = 0; q < 4; q++) {
+      o[OR_TR_J_INV + q] = c[q];
+      o[OR_TR_ALPHA_INV + q] = c[4 + q];
+    }
+    for (int q = 0; q < 3; q++) o[OR_TR_RHO_INV + 1 + q] = c[8 + q];
+    get_jkinv(t->X, t->Kcon, o + OR_TR_J_UNPOL, o + OR_TR_K_UNPOL, p->emission_type);
+    o[OR_TR_DL] = t->dl;
+    for (int mu = 0; mu < 4; mu++) {
+      o[OR_TR_X + mu] = t->X[mu];
+      o[OR_TR_KCON + mu] = t->Kcon[mu];
+    }
+    double g[4][4];
+    or_gcov(t->X, g);
+    or_bl_coord(t->X, o + OR_TR_R, o + OR_TR_TH);
+    o[OR_TR_PHI] = t->X[3];
+    double Econ[4][4], Ecov[4][4];
+    or_plasma_tetrad(Ucon, t->Kcon, Bcon, g, Econ, Ecov);
+    double e2Mid[4], e2Final[4];
+    transport_vector(t->X, t->Kcon, Econ[2], Econ[2], e2Mid, t->dl);
+    transport_vector(t->Xhalf, t->Kconhalf, Econ[2], e2Mid, e2Final, t->dl);
+    double *om = out + (size_t)(s - 1) * OR_TR_NF;
+    for (int mu = 0; mu < 4; mu++) {
+      o[OR_TR_E2C + mu] = Econ[2][mu];
+      o[OR_TR_E2C_COV + mu] = Ecov[2][mu];
+      om[OR_TR_E2T + mu] = e2Final[mu];
+      e2Final[mu] = o[OR_TR_E2T + mu];
+    }
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) {
+        o[OR_TR_ECON + 4 * a + b] = Econ[a][b];
+        o[OR_TR_ECOV + 4 * a + b] = Ecov[a][b];
+      }
+    or_lower(e2Final, g, o + OR_TR_E2T_COV);
+    coord_to_tetrad_rank2(N_coord, Ecov, N_tetrad);
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) {
+        o[OR_TR_NTETRAD + 4 * a + b] = creal(N_tetrad[a][b]);
+        o[OR_TR_NCOORD + 4 * a + b] = creal(N_coord[a][b]);
+      }
+    tensor_to_stokes(N_tetrad, o + OR_TR_STOKES, o + OR_TR_STOKES + 1, o + OR_TR_STOKES + 2,
+                     o + OR_TR_STOKES + 3);
+    or_integrate_emission(&traj[s], 1, &Intensity, &Tau, &tauF, N_coord, p);
+    o[OR_TR_I_UNPOL] = Intensity;
+  }
+  free(traj);
+  return 0;
+}
+
 /* Pixels listed as (i, j) pairs; returns 0 or -10 (maxnstep exceeded). */
 int or_render_pixels(const or_params_t *p, const or_camera_t *cam, const int *pix, int npix,
                      double *out7, int *nsteps, int *flags, int nthreads)

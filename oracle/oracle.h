@@ -72,6 +72,7 @@ typedef struct {
   /* sphere (model/sphere/model.c) */
   double sph_R0, sph_Ne0, sph_Thetae0, sph_B0;
   /* analytic (model/analytic/model.c) */
+  double sph_sigma, sph_beta;    /* model/sphere/model.c:172-188 */
   int an_model;
   double an_A, an_alpha, an_height, an_l0, an_freqcgs, an_RHO_unit, an_B_unit;
 } or_model_t;
@@ -153,6 +154,9 @@ double or_b(const double X[4]);
 double or_sigma(const double X[4]);
 void or_iharm_derive(const double *prims, int nprim, const or_units_t *u,
                      double *out11);
+void or_iharm_derive_ru(const double *prims, int nprim, const or_units_t *u,
+                        double *out11, double *out_ru);
+void or_primitives(const double X[4], const double *rub, double p8[8], double *beta);
 void or_sphere_setup(double R0, double tau0, double Thetae0, double beta0, double gam,
                      double tp_over_te, double MBH_solar, or_model_t *m);
 void or_analytic_setup(int model, double MBH_solar, double rho_unit, double freqcgs,
@@ -175,6 +179,40 @@ void or_project_N(const double X[4], double complex N[4][4], double *SI, double 
                   double *SU, double *SV, double rotcam);
 int or_integrate_emission(or_traj_t *traj, int nsteps, double *Intensity, double *tau,
                           double *tauF, double complex N[4][4], const or_params_t *p);
+
+/* trace output (src/io.c:279-543): per-step record of OR_TR_NF doubles */
+#define OR_TR_NF 126
+#define OR_TR_PRIMS 0
+#define OR_TR_B 8
+#define OR_TR_NE 9
+#define OR_TR_THETAE 10
+#define OR_TR_NU 11
+#define OR_TR_BK_ANGLE 12
+#define OR_TR_SIGMA 13
+#define OR_TR_BETA 14
+#define OR_TR_DL 15
+#define OR_TR_R 16
+#define OR_TR_TH 17
+#define OR_TR_PHI 18
+#define OR_TR_J_UNPOL 19
+#define OR_TR_K_UNPOL 20
+#define OR_TR_I_UNPOL 21
+#define OR_TR_X 22
+#define OR_TR_KCON 26
+#define OR_TR_J_INV 30
+#define OR_TR_ALPHA_INV 34
+#define OR_TR_RHO_INV 38
+#define OR_TR_STOKES 42
+#define OR_TR_E2C 46
+#define OR_TR_E2C_COV 50
+#define OR_TR_E2T 54
+#define OR_TR_E2T_COV 58
+#define OR_TR_NTETRAD 62
+#define OR_TR_NCOORD 78
+#define OR_TR_ECON 94
+#define OR_TR_ECOV 110
+int or_trace_pixel(const or_params_t *p, const or_camera_t *cam, long i, long j, const double *rub,
+                   int cap, double *out, int *nstep_out);
 
 /* image (src/main.c) */
 int or_setup_camera(const or_params_t *p, or_camera_t *cam);

@@ -40,7 +40,8 @@ class Model(C.Structure):
     _fields_ = [("kind", C.c_int), ("L_unit", C.c_double), ("fields", dp),
                 ("th_beg", C.c_double), ("sigma_cut", C.c_double), ("sigma_cut_high", C.c_double),
                 ("sph_R0", C.c_double), ("sph_Ne0", C.c_double), ("sph_Thetae0", C.c_double),
-                ("sph_B0", C.c_double), ("an_model", C.c_int), ("an_A", C.c_double),
+                ("sph_B0", C.c_double), ("sph_sigma", C.c_double), ("sph_beta", C.c_double),
+                ("an_model", C.c_int), ("an_A", C.c_double),
                 ("an_alpha", C.c_double), ("an_height", C.c_double), ("an_l0", C.c_double),
                 ("an_freqcgs", C.c_double), ("an_RHO_unit", C.c_double), ("an_B_unit", C.c_double)]
 
@@ -132,6 +133,12 @@ def load(path):
             getattr(L, f).argtypes = [dp]
             getattr(L, f).restype = C.c_double
         L.or_iharm_derive.argtypes = [dp, C.c_int, C.POINTER(Units), dp]
+        L.or_iharm_derive_ru.argtypes = [dp, C.c_int, C.POINTER(Units), dp, dp]
+        L.or_trace_pixel.argtypes = [C.POINTER(Params), C.POINTER(Camera), C.c_long, C.c_long, dp, C.c_int,
+                                     dp, ip]
+        L.or_trace_pixel.restype = C.c_int
+        L.or_integrate_emission.argtypes = [C.POINTER(Traj), C.c_int, dp, dp, dp, dp, C.POINTER(Params)]
+        L.or_integrate_emission.restype = C.c_int
         L.or_sphere_setup.argtypes = [C.c_double] * 7 + [C.POINTER(Model)]
         L.or_analytic_setup.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
                                         C.POINTER(Model), C.POINTER(Geom)]
@@ -176,6 +183,7 @@ class Oracle:
         self.geom = geom
         self.model = model
         self.fields = None if fields is None else f64(fields)
+        self.rub = None  # ghosted rho, u, beta for the trace output
         if self.fields is not None:
             self.model.fields = P(self.fields)
         self.activate()
@@ -272,6 +280,33 @@ class Oracle:
         o = [C.c_double() for _ in range(4)]
         self.L.or_project_N(P(f64(X)), N.ctypes.data_as(dp), *[C.byref(v) for v in o], rotcam)
         return np.array([v.value for v in o])
+
+    def integrate_emission(self, rows, nsteps, params, N=None):
+        """or_integrate_emission over trajectory rows as trace() returns them
+        (nturns, dl, X, Kcon, Xhalf, Kconhalf): steps rows[nsteps] -> rows[0].
+        Returns (I, tau, tauF, N, flag)."""
+        traj = (Traj * len(rows))()
+        for s, r in enumerate(rows):
+            traj[s].nturns = int(r[0])
+            traj[s].dl = r[1]
+            for mu in range(4):
+                traj[s].X[mu], traj[s].Kcon[mu] = r[2 + mu], r[6 + mu]
+                traj[s].Xhalf[mu], traj[s].Kconhalf[mu] = r[10 + mu], r[14 + mu]
+        N = np.zeros((4, 4), np.complex128) if N is None else np.array(N, np.complex128)
+        I_, tau, tauF = C.c_double(), C.c_double(), C.c_double()
+        fl = self.L.or_integrate_emission(traj, nsteps, C.byref(I_), C.byref(tau), C.byref(tauF),
+                                          N.ctypes.data_as(dp), C.byref(params))
+        return I_.value, tau.value, tauF.value, N, fl
+
+    def trace_pixel(self, params, cam, i, j):
+        """Trace output of pixel (i, j) (src/io.c:279-543): (rc, nstep,
+        records (nstep + 1, 126))."""
+        cap = params.maxnstep
+        out = np.zeros((cap, 126))
+        ns = C.c_int()
+        rub = None if self.rub is None else P(self.rub)
+        rc = self.L.or_trace_pixel(C.byref(params), C.byref(cam), i, j, rub, cap, P(out), C.byref(ns))
+        return rc, ns.value, out[:ns.value + 1].copy()
 
     def render(self, params, nthreads=0):
         n = params.nx * params.ny
@@ -416,11 +451,14 @@ def iharm_oracle(h, prims, M_unit, MBH=6.2e9, trat_small=1., trat_large=40., bet
     u = iharm_units(M_unit, MBH, trat_small, trat_large, beta_crit, h["gam"])
     nc = (g.N1 + 2) * (g.N2 + 2) * (g.N3 + 2)
     out = np.zeros(11 * nc)
+    ru = np.zeros(2 * nc)
     pr = f64(prims)
-    L.or_iharm_derive(P(pr), pr.shape[-1], C.byref(u), P(out))
+    L.or_iharm_derive_ru(P(pr), pr.shape[-1], C.byref(u), P(out), P(ru))
     m = Model()
     m.kind = MODEL_IHARM
     m.L_unit = GNEWT * MBH * MSUN / (CL * CL)
     m.th_beg, m.sigma_cut, m.sigma_cut_high = th_beg, sigma_cut, -1.
     fields = out[:10 * nc].copy()
-    return Oracle(g, m, fields, L=L), out.reshape(11, g.N1 + 2, g.N2 + 2, g.N3 + 2)
+    orc = Oracle(g, m, fields, L=L)
+    orc.rub = np.concatenate([ru, out[10 * nc:]])
+    return orc, out.reshape(11, g.N1 + 2, g.N2 + 2, g.N3 + 2)

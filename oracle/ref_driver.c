@@ -26,6 +26,7 @@
 #include "model_geodesics.h"
 #include "model_tetrads.h"
 #include "par.h"
+#include "radiation.h"
 #include "tetrads.h"
 
 #include <stdio.h>
@@ -167,6 +168,31 @@ int main(int argc, char *argv[])
       put(Uc, 4); put(Ul, 4); put(Bc, 4); put(Bl, 4);
       put1(get_model_ne(X)); put1(get_model_thetae(X)); put1(get_model_b(X));
       put1(get_model_sigma(X)); put1(radiating_region(X));
+    }
+  } else if (!strcmp(cmd, "trace_fields")) {
+    /* the per-point model values dump_var_along records (src/io.c:357-371):
+     * record X[4] K[4]; out: prims[8] sigma beta nu b_k_angle */
+    for (int q = 0; q < n; q++) {
+      double X[4], K[4], Uc[4], Ul[4], Bc[4], Bl[4], p8[8] = {0};
+      memcpy(X, rec + 8 * q, sizeof(X));
+      memcpy(K, rec + 8 * q + 4, sizeof(K));
+      get_model_primitives(X, p8);
+      get_model_fourv(X, K, Uc, Ul, Bc, Bl);
+      put(p8, 8); put1(get_model_sigma(X)); put1(get_model_beta(X));
+      put1(get_fluid_nu(K, Ul)); put1(get_bk_angle(X, K, Ul, Bc, Bl));
+    }
+  } else if (!strcmp(cmd, "ptvec")) {
+    /* parallel_transport_vector (src/ipolarray.c:297-320): record Xm Km Ni Nm
+     * dl; out Nf */
+    for (int q = 0; q < n; q++) {
+      const double *r0 = rec + 17 * q;
+      double Xm[4], Km[4], Ni[4], Nm[4], Nf[4], X0[4] = {0}, K0[4] = {0};
+      memcpy(Xm, r0, sizeof(Xm));
+      memcpy(Km, r0 + 4, sizeof(Km));
+      memcpy(Ni, r0 + 8, sizeof(Ni));
+      memcpy(Nm, r0 + 12, sizeof(Nm));
+      parallel_transport_vector(X0, Xm, X0, K0, Km, K0, Ni, Nm, Nf, r0[16]);
+      put(Nf, 4);
     }
   } else if (!strcmp(cmd, "tetrad")) {
     for (int q = 0; q < n; q++) {

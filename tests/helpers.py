@@ -43,3 +43,29 @@ def gpu_image_kwargs(p):
                 stokes_floors=p.stokes_floors, target_nturns=p.target_nturns,
                 subring_dtheta=p.subring_dtheta, counterjet=p.counterjet,
                 emission_type=p.emission_type, xoff=p.xoff, yoff=p.yoff)
+
+
+TRACE_GEO = ("X", "Kcon", "dl", "r", "th", "phi")
+TRACE_TETRAD = ("Econ", "Ecov", "e2Constructed", "e2ConstructedCov", "e2Transported", "e2TransportedCov",
+                "ntetrad", "ncoord", "stokes")
+
+
+def trace_compare(ref, got, tag, tol=1e-6, geo_tol=1e-7, tetrad_tol=1e-5):
+    """Trace records (nstep + 1, TRACE_NF) against the oracle's, per field as
+    |d| / max |ref| over the steps: the geodesic within geo_tol, the plasma
+    tetrad and what is projected on it within tetrad_tol (Gram-Schmidt
+    amplifies the positions' difference where K and B are nearly parallel),
+    everything else within tol.  Returns the per-field error."""
+    fr, fg = G.trace_fields(ref), G.trace_fields(got)
+    worst = {}
+    for name, _, _ in G.TRACE_SETS:
+        a, b = fr[name], fg[name]
+        sc = np.abs(a).max()
+        if sc == 0:
+            assert not np.any(b), (tag, name)
+            continue
+        e = float(np.abs(b - a).max() / sc)
+        worst[name] = e
+        t = geo_tol if name in TRACE_GEO else (tetrad_tol if name in TRACE_TETRAD else tol)
+        assert e < t, (tag, name, e)
+    return worst

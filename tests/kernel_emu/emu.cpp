@@ -304,4 +304,18 @@ int emu_trace_two_pass(const ipgpu_geom *G, const ipgpu_image *P, int i, int j, 
   return trace_ray(*G, *P, 0, 0, P->maxnstep, rec, nullptr, ns);
 }
 
+/* Trace output of one pixel as k_trace_pixel forms it: out receives
+ * maxnstep records of IPGPU_TRACE_NF doubles. */
+int emu_trace_pixel(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, const double *dump,
+                    int prims_ok, int i, int j, double *out)
+{
+  const int cap = P->maxnstep;
+  std::vector<double> rec((size_t)cap * REC);
+  std::vector<int> nt(cap, 0);
+  memset(out, 0, sizeof(double) * (size_t)cap * IPGPU_TRACE_NF);
+  int ns = 0;
+  trace_pixel_records(*G, *M, *P, i, j, cap, prims_ok, dump, rec.data(), nt.data(), out, &ns);
+  return ns;
+}
+
 }

@@ -35,6 +35,8 @@ def lib():
         L.emu_render_segmented.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), dp, ip,
                                            C.c_int, C.c_int, dp, ip, ip]
         L.emu_trace.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Image), C.c_int, C.c_int, dp]
+        L.emu_trace_pixel.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), dp, C.c_int,
+                                      C.c_int, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -139,6 +141,23 @@ def pack_dump_adjoint(fields11, units):
     d[:, 13] = dth * betasq / (1. + betasq)
     d[:, 14] = dth / (1. + betasq)
     return d.ravel()
+
+
+def pack_dump_trace(fields11, rho_u):
+    """16-slot cell records with rho, u, beta in slots 10, 11, 12, as the
+    device derivation leaves them (rho_u: the ghosted rho and u arrays)."""
+    d = pack_dump(fields11[:10]).reshape(-1, 16)
+    ru = np.asarray(rho_u).reshape(2, -1)
+    d[:, 10], d[:, 11] = ru[0], ru[1]
+    d[:, 12] = np.asarray(fields11[10]).ravel()
+    return d.ravel()
+
+
+def trace_pixel(geom, model, image, dump, i, j, prims_ok=True):
+    out = np.zeros((image.maxnstep, G.TRACE_NF))
+    ns = lib().emu_trace_pixel(C.byref(geom), C.byref(model), C.byref(image),
+                               None if dump is None else G.P(f64(dump)), int(prims_ok), i, j, G.P(out))
+    return ns, out[:ns + 1].copy()
 
 
 def adjoint(geom, model, image, dump, target, w=(1., 1., 1., 1.)):

@@ -8,7 +8,7 @@ from scipy import special
 
 import emu
 import oracle as O
-from helpers import small_dump
+from helpers import small_dump, trace_compare
 from ipole_amd import ipgpu as G
 
 
@@ -379,3 +379,43 @@ def test_emu_trace_two_pass_bit_identical(budget):
         assert ns2 == ns
         assert np.array_equal(rec, rec2)
     assert paused_any
+
+
+@pytest.mark.parametrize("cam", [dict(thetacam=60.), dict(thetacam=163., target_nturns=0),
+                                 dict(thetacam=90., only_unpolarized=1)])
+def test_emu_trace_pixel(cam):
+    """The trace output's per-step records (trace_pixel_records, as
+    k_trace_pixel runs it) against the oracle's or_trace_pixel, with the
+    dump's rho, u and beta where the device derivation puts them."""
+    h, prims = small_dump()
+    orc, fields = O.iharm_oracle(h, prims, M_unit=3e26)
+    p = O.default_params(nx=16, ny=16, freqcgs=230e9, **cam)
+    oc = orc.camera(p)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    im, _, _ = G.make_image(g, m.L_unit, nx=16, ny=16, freqcgs=230e9, **cam)
+    nc = fields[0].size
+    dump = emu.pack_dump_trace(fields, orc.rub[:2 * nc])
+    for ij in [(8, 7), (3, 12), (15, 0)]:
+        rc, ns, ref = orc.trace_pixel(p, oc, *ij)
+        nse, got = emu.trace_pixel(g, m, im, dump, *ij)
+        assert rc == 0 and nse == ns, (ij, nse, ns)
+        trace_compare(ref, got, ij)
+        nse2, got2 = emu.trace_pixel(g, m, im, dump, *ij, prims_ok=False)
+        f2 = G.trace_fields(got2)
+        assert not np.any(f2["prims"][:, :2]) and not np.any(f2["beta"])
+        assert np.array_equal(np.delete(got2, [0, 1, 14], axis=1), np.delete(got, [0, 1, 14], axis=1))
+
+
+def test_emu_trace_pixel_sphere():
+    kw = dict(R0=50., tau0=2.0143e-5, Thetae0=10., beta0=30.)
+    go, mo = O.sphere_model(**kw)
+    orc = O.Oracle(go, mo)
+    p = O.default_params(nx=16, ny=16, thetacam=60., dx=100., dy=100., freqcgs=230e9)
+    oc = orc.camera(p)
+    gs, ms = G.sphere_setup(**kw)
+    im, _, _ = G.make_image(gs, ms.L_unit, nx=16, ny=16, thetacam=60., dx=100., dy=100., freqcgs=230e9)
+    for ij in [(8, 8), (2, 9)]:
+        rc, ns, ref = orc.trace_pixel(p, oc, *ij)
+        nse, got = emu.trace_pixel(gs, ms, im, None, *ij)
+        assert rc == 0 and nse == ns
+        trace_compare(ref, got, ij)
