@@ -136,6 +136,37 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
 int ipgpu_fetch(ipgpu_ctx *ctx, double *image, double *imageS, double *taus, int *nstep,
                 int *flag);
 
+/* Adaptive image refinement (src/main.c:505-792; keys src/par.c:77-84,
+ * 172-177): the image's nx x ny is the finest level, nx_min x ny_min the
+ * coarsest; levels = log2(nx / nx_min) + 1 on the even parts of nx, nx_min,
+ * and an even side gains one internal row or column (the 2^N + 1 scheme).
+ * The coarsest grid is traced, then each finer level traces the pixels
+ * whose interpolation error exceeds both refine_abs (relative to the
+ * estimated mean intensity) and refine_rel, where the lower neighbour is
+ * brighter than refine_cut, and interpolates the others (linearly, or
+ * from the nearest traced neighbour).  The internal grid must be
+ * (nx_min - 1) 2^(levels - 1) + 1 on both axes.  nx_min <= 0 or
+ * nx_min == nx renders as ipgpu_render.  Outputs: the nx x ny image as
+ * ipgpu_render returns it (interpolated pixels have nstep = flag = 0) and
+ * interp (may be NULL): 1 where a pixel was interpolated.  image must not
+ * be NULL; the other outputs may be. */
+typedef struct ipgpu_refine {
+  int nx_min, ny_min;
+  double refine_abs, refine_rel, refine_cut;
+  int nearest_neighbor;
+} ipgpu_refine;
+#define IPGPU_MAX_LEVELS 16
+int ipgpu_render_adaptive(ipgpu_ctx *ctx, const ipgpu_image *img, const ipgpu_refine *rf, double *image,
+                          double *imageS, double *taus, int *nstep, int *flag, int *interp);
+
+/* After ipgpu_render_adaptive: the number of levels, the estimated mean
+ * intensity of the coarse pass (Iavg, src/main.c:617-632), and per level
+ * (index 0 the coarse pass) the pixels traced and the interpolated pixels
+ * of the internal grid so far (the count src/main.c:776-786 prints).
+ * traced and interpolated hold IPGPU_MAX_LEVELS entries; any may be NULL. */
+int ipgpu_adaptive_stats(const ipgpu_ctx *ctx, int *levels, double *Iavg, long long *traced,
+                         long long *interpolated);
+
 /* One job of a batched render: an image (camera, size, frequency and
  * options as for ipgpu_render) of the uploaded dump, with n_e scaled by
  * m_unit_scale and B by its square root, i.e. the dump's M_unit times

@@ -39,14 +39,27 @@ constexpr int TILE = 8;         /* pixels per tile side */
 constexpr int WAVE = 64;
 constexpr int NSTATUS = 4;      /* [0] overflowed rays [1] first maxnstep ray+1 [2] NaN pixel+1 */
 
-__device__ __forceinline__ void ray_pixel(int ray, int nx, int ny, int &i, int &j)
+/* The pixel a ray renders: with pix == nullptr the 8x8 tiles of the nx x ny
+ * image (false for the padding rays of the edge tiles); otherwise
+ * pix[ray] = (i << 16) | j, a pixel list (false for the padding entries, -1). */
+__device__ __forceinline__ bool ray_pixel(const int *pix, int ray, int nx, int ny, int &i, int &j)
 {
+  if (pix) {
+    const int v = pix[ray];
+    i = v >> 16;
+    j = v & 0xffff;
+    return v >= 0;
+  }
   int tiles_y = (ny + TILE - 1) / TILE;
   int t = ray / WAVE, l = ray % WAVE;
   int ti = t / tiles_y, tj = t % tiles_y;
   i = ti * TILE + l / TILE;
   j = tj * TILE + l % TILE;
+  return i < nx && j < ny;
 }
+
+/* pixel code of the status words: first bad pixel by atomicMin */
+__device__ __forceinline__ int pixel_code(int i, int j) { return ((i << 16) | j) + 1; }
 
 /* The two lanes (2k, 2k+1) of a lane pair exchange a value (DPP quad
  * permutation [1, 0, 3, 2] on both 32-bit halves). */
@@ -152,7 +165,7 @@ struct AdjW {
   double w[4];
 };
 
-__global__ __launch_bounds__(WAVE) void k_trace(ipgpu_geom G, ipgpu_image P, int nrays, int cap,
+__global__ __launch_bounds__(WAVE) void k_trace(ipgpu_geom G, ipgpu_image P, const int *__restrict__ pix, int nrays, int cap,
                                                 double *__restrict__ traj, int *__restrict__ nstep_out,
                                                 int *__restrict__ nturns_buf, int *__restrict__ status,
                                                 unsigned long long *__restrict__ counts)
@@ -160,8 +173,7 @@ __global__ __launch_bounds__(WAVE) void k_trace(ipgpu_geom G, ipgpu_image P, int
   int ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= nrays) return;
   int pi, pj;
-  ray_pixel(ray, P.nx, P.ny, pi, pj);
-  if (pi >= P.nx || pj >= P.ny) {
+  if (!ray_pixel(pix, ray, P.nx, P.ny, pi, pj)) {
     nstep_out[ray] = 0;
     return;
   }
@@ -176,7 +188,7 @@ __global__ __launch_bounds__(WAVE) void k_trace(ipgpu_geom G, ipgpu_image P, int
 /* k_trace without subring counts on lane pairs (trace_pair): the geodesics
  * of the one-pass render are then those of the streamed render, bit for
  * bit. */
-__global__ __launch_bounds__(WAVE) void k_trace_pair(ipgpu_geom G, ipgpu_image P, int nrays, int cap,
+__global__ __launch_bounds__(WAVE) void k_trace_pair(ipgpu_geom G, ipgpu_image P, const int *__restrict__ pix, int nrays, int cap,
                                                      double *__restrict__ traj, int *__restrict__ nstep_out,
                                                      int *__restrict__ status, unsigned long long *__restrict__ counts)
 {
@@ -185,8 +197,7 @@ __global__ __launch_bounds__(WAVE) void k_trace_pair(ipgpu_geom G, ipgpu_image P
   const bool odd = t & 1;
   if (ray >= nrays) return;
   int pi, pj;
-  ray_pixel(ray, P.nx, P.ny, pi, pj);
-  if (pi >= P.nx || pj >= P.ny) {
+  if (!ray_pixel(pix, ray, P.nx, P.ny, pi, pj)) {
     if (!odd) nstep_out[ray] = 0;
     return;
   }
@@ -268,7 +279,7 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep(ipgpu_geom G, ipgpu_model M
  * segment maps (SEG = true: the rays with seg_base >= 0).  Two instances,
  * so that neither carries the other's registers. */
 template <bool SEG>
-__global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P, int nrays,
+__global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P, const int *__restrict__ pix, int nrays,
                                                 int cap, int stride, const long long *__restrict__ off,
                                                 const int *__restrict__ nstep_in,
                                                 const double *__restrict__ traj,
@@ -286,12 +297,11 @@ __global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ip
   const int sb = seg_base ? seg_base[ray] : -1;
   if ((sb >= 0) != SEG) return;
   int pi, pj;
-  ray_pixel(ray, P.nx, P.ny, pi, pj);
-  if (pi >= P.nx || pj >= P.ny) return;
+  if (!ray_pixel(pix, ray, P.nx, P.ny, pi, pj)) return;
   int ns = nstep_in[ray];
   double *o = out7 + (size_t)ray * 7;
   if (ns >= P.maxnstep - 1) {
-    atomicMin(&status[1], pi * P.ny + pj + 1);
+    atomicMin(&status[1], pixel_code(pi, pj));
     for (int q = 0; q < 7; q++) o[q] = 0.;
     flags[ray] = 0;
     return;
@@ -318,7 +328,8 @@ __global__ __launch_bounds__(WAVE) void k_chain(ipgpu_geom G, ipgpu_model Mo, ip
 }
 
 /* save_pixel (src/main.c:885-910): reference layout, nu^3 scaling, QU sign */
-__global__ void k_finalize(ipgpu_image P, int nrays, double f3, const double *__restrict__ out7,
+__global__ void k_finalize(ipgpu_image P, const int *__restrict__ pix, int out_ny, int nrays, double f3,
+                           const double *__restrict__ out7,
                            const int *__restrict__ flags_ray, const int *__restrict__ nstep_ray,
                            double *__restrict__ image, double *__restrict__ imageS,
                            double *__restrict__ taus, int *__restrict__ nstep_pix,
@@ -327,9 +338,8 @@ __global__ void k_finalize(ipgpu_image P, int nrays, double f3, const double *__
   int ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= nrays) return;
   int i, j;
-  ray_pixel(ray, P.nx, P.ny, i, j);
-  if (i >= P.nx || j >= P.ny) return;
-  size_t p = (size_t)i * P.ny + j;
+  if (!ray_pixel(pix, ray, P.nx, P.ny, i, j)) return;
+  size_t p = (size_t)i * out_ny + j;
   const double *o = out7 + (size_t)ray * 7;
   image[p] = o[0] * f3;
   taus[p] = o[5];
@@ -342,7 +352,7 @@ __global__ void k_finalize(ipgpu_image P, int nrays, double f3, const double *__
     imageS[p * 5 + 2] = qs * o[3] * f3;
     imageS[p * 5 + 3] = o[4] * f3;
     imageS[p * 5 + 4] = o[6];
-    if (isnan(imageS[p * 5])) atomicMin(&status[2], (int)p + 1);
+    if (isnan(imageS[p * 5])) atomicMin(&status[2], pixel_code(i, j));
   }
 }
 
@@ -532,7 +542,7 @@ __global__ __launch_bounds__(WAVE) void k_trace_count(ipgpu_geom G, const BGroup
   const int g = group_of(groups, ngroups, r);
   const ipgpu_image P = groups[g].P;
   int pi, pj;
-  ray_pixel((int)(r - groups[g].ray_base), P.nx, P.ny, pi, pj);
+  ray_pixel(nullptr, (int)(r - groups[g].ray_base), P.nx, P.ny, pi, pj);
   if (pi >= P.nx || pj >= P.ny) {
     nstep_out[r] = 0;
     return;
@@ -555,7 +565,7 @@ __global__ __launch_bounds__(WAVE) void k_trace_store(ipgpu_geom G, const BGroup
   const int g = group_of(groups, ngroups, r);
   const ipgpu_image P = groups[g].P;
   int pi, pj;
-  ray_pixel((int)(r - groups[g].ray_base), P.nx, P.ny, pi, pj);
+  ray_pixel(nullptr, (int)(r - groups[g].ray_base), P.nx, P.ny, pi, pj);
   const int ns = nstep_in[r];
   double *rec = traj + (size_t)off[r] * REC;
   int *nt = (nturns_buf && P.target_nturns >= 0) ? nturns_buf + off[r] : nullptr;
@@ -589,7 +599,7 @@ __global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model M
   const BJob J = jobs[jb];
   const ipgpu_image &P = J.P;
   int pi, pj;
-  ray_pixel((int)(r - groups[J.group].ray_base), P.nx, P.ny, pi, pj);
+  ray_pixel(nullptr, (int)(r - groups[J.group].ray_base), P.nx, P.ny, pi, pj);
   const long long pix = (long long)pi * P.ny + pj;
   const size_t p = (size_t)(J.pix_off + pix);
   const int ns = nstep_in[r];
@@ -642,7 +652,7 @@ __global__ __launch_bounds__(WAVE) void k_adjoint(ipgpu_geom G, ipgpu_image P, i
   int ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= nrays) return;
   int pi, pj;
-  ray_pixel(ray, P.nx, P.ny, pi, pj);
+  ray_pixel(nullptr, ray, P.nx, P.ny, pi, pj);
   double *o = out4 + (size_t)ray * 4;
   o[0] = o[1] = o[2] = o[3] = 0.;
   if (pi >= P.nx || pj >= P.ny) return;
@@ -702,7 +712,7 @@ __global__ void k_bc_ne(ipgpu_geom G, int stage, double *dump)
 
 /* First trace phase: every ray with a step budget; live[ray] = 1 where it
  * paused. */
-__global__ __launch_bounds__(WAVE) void k_trace_budget(ipgpu_geom G, ipgpu_image P, int nrays, int cap, int nend,
+__global__ __launch_bounds__(WAVE) void k_trace_budget(ipgpu_geom G, ipgpu_image P, const int *__restrict__ pix, int nrays, int cap, int nend,
                                                        double *__restrict__ traj, int *__restrict__ nstep_out,
                                                        int *__restrict__ live, int *__restrict__ host_ns,
                                                        int *__restrict__ host_live,
@@ -714,9 +724,8 @@ __global__ __launch_bounds__(WAVE) void k_trace_budget(ipgpu_geom G, ipgpu_image
   const bool odd = t & 1;
   if (ray >= nrays) return;
   int pi, pj;
-  ray_pixel(ray, P.nx, P.ny, pi, pj);
   int nstep = 0, lv = 0;
-  if (pi < P.nx && pj < P.ny) {
+  if (ray_pixel(pix, ray, P.nx, P.ny, pi, pj)) {
     int fin = 1;
     nstep = trace_pair(G, P, pi, pj, cap, traj + (size_t)ray * cap * REC, 0, nend, &fin, odd);
     lv = fin ? 0 : 1;
@@ -855,7 +864,7 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_compact(ipgpu_geom G, ipgpu
  * seg_base[k] >= 0 (composition of the segment maps). */
 template <bool SEG>
 __global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
-                                                     const int *__restrict__ list, int n,
+                                                     const int *__restrict__ pix, const int *__restrict__ list, int n,
                                                      const long long *__restrict__ W, int cap, int stride,
                                                      const int *__restrict__ nstep_in,
                                                      const double *__restrict__ traj,
@@ -872,12 +881,11 @@ __global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model M
   if ((sb >= 0) != SEG) return;
   const int ray = list[k];
   int pi, pj;
-  ray_pixel(ray, P.nx, P.ny, pi, pj);
-  if (pi >= P.nx || pj >= P.ny) return;
+  if (!ray_pixel(pix, ray, P.nx, P.ny, pi, pj)) return;
   const int ns = nstep_in[ray];
   double *o = out7 + (size_t)ray * 7;
   if (ns >= P.maxnstep - 1) {
-    atomicMin(&status[1], pi * P.ny + pj + 1);
+    atomicMin(&status[1], pixel_code(pi, pj));
     for (int q = 0; q < 7; q++) o[q] = 0.;
     flags[ray] = 0;
     return;
@@ -934,6 +942,121 @@ __global__ __launch_bounds__(WAVE) void k_seg(int nseg, int seg_len, const long 
 #pragma unroll
   for (int p = 0; p < NPAR; p++) o[p] = herm_par(N, p);
   if (b == NPAR) segflag[sgi] = fl;
+}
+
+/* ------------------------------------------------------ adaptive refinement */
+
+struct RefineK {
+  int nxi, nyi, sp, nmin_y;       /* internal grid, base spacing, base columns */
+  double refine_abs, refine_rel, refine_cut;
+  int nearest;
+};
+
+/* Coarse-pass intensity estimate of src/main.c:579-632: the base rays'
+ * Intensity (before the nu^3 scaling) weighted by the pixels each stands
+ * for, summed in the coarse grid's order, times freqcgs^3 / (nx ny).  One
+ * lane: the sum's order is the reference's. */
+__global__ void k_refine_iavg(RefineK R, int nbase, double f3, const double *__restrict__ out7,
+                              double *__restrict__ iavg)
+{
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const size_t sp = (size_t)R.sp, nx = (size_t)R.nxi, ny = (size_t)R.nyi;
+  double tot = 0.;
+  for (int r = 0; r < nbase; r++) {
+    const size_t i = (size_t)(r / R.nmin_y) * sp, j = (size_t)(r % R.nmin_y) * sp;
+    const double I = out7[(size_t)r * 7];
+    double w;
+    if (i % (nx - 1) != 0 && j % (ny - 1) != 0) w = I * sp * sp;
+    else if ((i == 0 && j % (ny - 1) != 0) || (i % (nx - 1) != 0 && j == 0)) w = I * (sp / 2 + 1) * sp;
+    else if (i % (nx - 1) != 0 || j % (ny - 1) != 0) w = I * (sp / 2) * sp;
+    else if (i == 0 && j == 0) w = I * (sp / 2 + 1) * (sp / 2 + 1);
+    else if (i == 0 || j == 0) w = I * (sp / 2 + 1) * sp / 2;
+    else w = I * (sp * sp) / 4;
+    tot += w;
+  }
+  iavg[0] = tot * f3 / (double)(nx * ny);
+}
+
+/* One refinement level (src/main.c:635-776) at spacing ns: each pixel of the
+ * level not on the coarser grid either joins the list of rays to trace
+ * (both error estimates above their tolerances and the lower neighbour
+ * above refine_cut) or is interpolated from the coarser grid at once
+ * (save_pixelTransfer, lininterp2, lininterp4: src/main.c:912-1000).  The
+ * pixels a level reads all lie on the coarser grid, so its pixels are
+ * independent.  count[0]: traced, count[1]: interpolated. */
+__global__ void k_refine_level(RefineK R, int ns, int unpol, const double *__restrict__ iavg,
+                               double *__restrict__ image, double *__restrict__ imageS,
+                               double *__restrict__ taus, int *__restrict__ interp, int *__restrict__ list,
+                               unsigned int *__restrict__ count)
+{
+  const int na = (R.nxi - 1) / ns + 1, nb = (R.nyi - 1) / ns + 1;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)na * nb) return;
+  const int a = (int)(t / nb), b = (int)(t % nb);
+  const bool ca = (a & 1) == 0, cb = (b & 1) == 0;
+  if (ca && cb) return;                                  /* traced before */
+  const int i = a * ns, j = b * ns;
+  const size_t ny = (size_t)R.nyi;
+  auto at = [&](int ii, int jj) { return (size_t)ii * ny + jj; };
+  const double Iavg = iavg[0];
+  size_t src[4];
+  int nsrc;
+  double I1, err_abs, err_rel;
+  if (ca) {                                              /* on a coarser column */
+    src[0] = at(i, j - ns);
+    src[1] = at(i, j + ns);
+    nsrc = 2;
+    I1 = image[src[0]];
+    const double I2 = image[src[1]];
+    err_abs = (I2 - I1) / 2 / Iavg;
+    err_rel = (I2 - I1) / 2 / I1;
+  } else if (cb) {                                       /* on a coarser row */
+    src[0] = at(i - ns, j);
+    src[1] = at(i + ns, j);
+    nsrc = 2;
+    I1 = image[src[0]];
+    const double I2 = image[src[1]];
+    err_abs = (I2 - I1) / 2 / Iavg;
+    err_rel = (I2 - I1) / 2 / I1;
+  } else {                                               /* between four corners */
+    src[0] = at(i - ns, j - ns);
+    src[1] = at(i + ns, j - ns);
+    src[2] = at(i - ns, j + ns);
+    src[3] = at(i + ns, j + ns);
+    nsrc = 4;
+    I1 = image[src[0]];
+    const double I2 = image[src[1]], I3 = image[src[2]];
+    err_abs = ((I2 + I3) / 2 - I1) / Iavg;
+    err_rel = (I2 + I3) / (2 * I1) - 1.;
+  }
+  const size_t p = at(i, j);
+  if ((fabs(err_abs) > R.refine_abs && fabs(err_rel) > R.refine_rel) && fabs(I1) > R.refine_cut) {
+    list[atomicAdd(&count[0], 1u)] = (i << 16) | j;
+    interp[p] = 0;
+    return;
+  }
+  atomicAdd(&count[1], 1u);
+  interp[p] = 1;
+  if (R.nearest) {
+    const size_t q = src[0];
+    image[p] = image[q];
+    if (!unpol) {
+      taus[p] = taus[q];
+      for (int k = 0; k < 5; k++) imageS[p * 5 + k] = imageS[q * 5 + k];
+    }
+  } else if (nsrc == 2) {
+    image[p] = .5 * (image[src[0]] + image[src[1]]);
+    taus[p] = .5 * (taus[src[0]] + taus[src[1]]);
+    if (!unpol)
+      for (int k = 0; k < 5; k++) imageS[p * 5 + k] = .5 * (imageS[src[0] * 5 + k] + imageS[src[1] * 5 + k]);
+  } else {
+    image[p] = .25 * (image[src[0]] + image[src[1]] + image[src[2]] + image[src[3]]);
+    taus[p] = .25 * (taus[src[0]] + taus[src[1]] + taus[src[2]] + taus[src[3]]);
+    if (!unpol)
+      for (int k = 0; k < 5; k++)
+        imageS[p * 5 + k] = .25 * (imageS[src[0] * 5 + k] + imageS[src[1] * 5 + k] + imageS[src[2] * 5 + k]
+                                   + imageS[src[3] * 5 + k]);
+  }
 }
 
 /* ------------------------------------------------------- pixel trace output */
@@ -1042,6 +1165,20 @@ struct ipgpu_ctx {
   Herm *d_ckpt = nullptr, *d_seg = nullptr;
   size_t cap_ckpt = 0, cap_seg = 0;
   double *d_gcell = nullptr, *d_out4 = nullptr, *d_target = nullptr, *d_scale = nullptr;
+  /* the pixel list of the rays being rendered (nullptr: the image's tiles)
+   * and the row length of the output image they are written to */
+  const int *pix = nullptr;
+  int out_ny = 0;
+  /* adaptive refinement: pixel list, decision counters, per-level counts */
+  int *d_rlist = nullptr;
+  size_t cap_rlist = 0;
+  unsigned long long *d_rcount = nullptr;
+  double *d_iavg = nullptr;
+  int *d_interp = nullptr;
+  size_t cap_interp = 0;
+  int ref_levels = 0;
+  double ref_iavg = 0.;
+  long long ref_traced[IPGPU_MAX_LEVELS] = {0}, ref_interp[IPGPU_MAX_LEVELS] = {0};
   /* pixel trace output */
   double *d_trrec = nullptr, *d_trout = nullptr;
   int *d_trnt = nullptr;
@@ -1132,7 +1269,8 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
                   ctx->d_flag_pix, ctx->d_status, ctx->d_counts,
                   ctx->d_groups, ctx->d_jobs, ctx->d_items, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
                   ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_ckpt, ctx->d_seg, ctx->d_gcell,
-                  ctx->d_out4, ctx->d_target, ctx->d_scale, ctx->d_trrec, ctx->d_trout, ctx->d_trnt};
+                  ctx->d_out4, ctx->d_target, ctx->d_scale, ctx->d_trrec, ctx->d_trout, ctx->d_trnt,
+                  ctx->d_rlist, ctx->d_rcount, ctx->d_iavg, ctx->d_interp};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int q = 0; q < NEV; q++)
@@ -1269,10 +1407,10 @@ static int run_trace(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int cap, b
   HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
   HIPCHK(ctx, hipMemsetAsync(ctx->d_counts, 0, 2 * sizeof(unsigned long long), ctx->stream));
   if (turns)
-    k_trace<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, P, nrays, cap, ctx->d_traj, ctx->d_nstep_ray,
+    k_trace<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, P, ctx->pix, nrays, cap, ctx->d_traj, ctx->d_nstep_ray,
                                                      ctx->d_nturns, ctx->d_status, ctx->d_counts);
   else
-    k_trace_pair<<<2 * nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, P, nrays, cap, ctx->d_traj,
+    k_trace_pair<<<2 * nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, P, ctx->pix, nrays, cap, ctx->d_traj,
                                                              ctx->d_nstep_ray, ctx->d_status, ctx->d_counts);
   HIPCHK(ctx, hipGetLastError());
   return 0;
@@ -1398,7 +1536,7 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
                                                                       pkt, S.d_map, S.d_flag);
       HIPCHK(ctx, hipGetLastError());
     }
-    k_chain_list<true><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, cap, stride,
+    k_chain_list<true><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, ctx->pix, dl, n, W, cap, stride,
                                                               ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt, dbase,
                                                               ctx->seg_len, S.d_map, S.d_flag, ctx->d_out7,
                                                               ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
@@ -1423,13 +1561,13 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
   }
   /* composed rays (the longest, at the front of the list) and the rest */
   if (nseg > 0) {
-    k_chain_list<true><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, cap, stride,
+    k_chain_list<true><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, ctx->pix, dl, n, W, cap, stride,
                                                               ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt, dbase,
                                                               ctx->seg_len, S.d_map, S.d_flag, ctx->d_out7,
                                                               ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
     HIPCHK(ctx, hipGetLastError());
   }
-  k_chain_list<false><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, cap, stride,
+  k_chain_list<false><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, ctx->pix, dl, n, W, cap, stride,
                                                              ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt,
                                                              nseg > 0 ? dbase : nullptr, ctx->seg_len, S.d_map,
                                                              S.d_flag, ctx->d_out7, ctx->d_flag_ray, ctx->d_status,
@@ -1530,7 +1668,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
     if (k > 0) HIPCHK(ctx, hipEventRecord(ev[2 * k], sA));
     int *hn = dh_ns + (size_t)k * nI, *hl = dh_live + (size_t)(k < K ? k : MAXPH - 1) * nI;
     if (k == 0)
-      k_trace_budget<<<2 * nrays / WAVE, WAVE, 0, sA>>>(ctx->geom, P, nrays, cap, bud[0], ctx->d_traj,
+      k_trace_budget<<<2 * nrays / WAVE, WAVE, 0, sA>>>(ctx->geom, P, ctx->pix, nrays, cap, bud[0], ctx->d_traj,
                                                          ctx->d_nstep_ray, ctx->d_live, hn, hl, ctx->d_counts);
     else
       k_trace_resume_mask<<<2 * nrays / WAVE, WAVE, 0, sA>>>(ctx->geom, P, nrays, cap, k < K ? bud[k] : -1,
@@ -1651,6 +1789,8 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
   return 0;
 }
 
+static int render_rays(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays);
+
 int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *imageS,
                  double *taus, int *nstep, int *flag)
 {
@@ -1663,19 +1803,35 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
   if (P.emission_type != 4 && P.emission_type != 1)
     return set_err(ctx, IPGPU_ERR_ARG, "emission_type must be 4 (Dexter thermal) or 1 (Pandya thermal)");
   HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (P.nx > 32767 || P.ny > 32767) return set_err(ctx, IPGPU_ERR_ARG, "image side above 32767 pixels");
   int tiles = ((P.nx + TILE - 1) / TILE) * ((P.ny + TILE - 1) / TILE);
   int nrays = tiles * WAVE;
   size_t npix = (size_t)P.nx * P.ny;
   int rc;
-  if ((rc = grow(ctx, ctx->d_nstep_ray, ctx->cap_nstep_ray, (size_t)nrays))) return rc;
-  if ((rc = grow(ctx, ctx->d_flag_ray, ctx->cap_flag_ray, (size_t)nrays))) return rc;
-  if ((rc = grow(ctx, ctx->d_out7, ctx->cap_out7, (size_t)nrays * 7))) return rc;
   if ((rc = grow(ctx, ctx->d_image, ctx->cap_image, npix))) return rc;
   if ((rc = grow(ctx, ctx->d_taus, ctx->cap_taus, npix))) return rc;
   if ((rc = grow(ctx, ctx->d_nstep_pix, ctx->cap_nstep_pix, npix))) return rc;
   if ((rc = grow(ctx, ctx->d_flag_pix, ctx->cap_flag_pix, npix))) return rc;
   if ((rc = grow(ctx, ctx->d_imageS, ctx->cap_imageS, npix * 5))) return rc;
 
+  ctx->pix = nullptr;
+  ctx->out_ny = P.ny;
+  if ((rc = render_rays(ctx, P, nrays))) return rc;
+  ctx->last_npix = (int)npix;
+  ctx->last_unpol = P.only_unpolarized;
+  ctx->have_result = 1;
+  return ipgpu_fetch(ctx, image, imageS, taus, nstep, flag);
+}
+
+/* The rays of ctx->pix (or the image's tiles) through the streamed or the
+ * one-pass pipeline into the device image buffers (rows of ctx->out_ny).
+ * nrays is a multiple of WAVE; the image buffers are sized by the caller. */
+static int render_rays(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays)
+{
+  int rc;
+  if ((rc = grow(ctx, ctx->d_nstep_ray, ctx->cap_nstep_ray, (size_t)nrays))) return rc;
+  if ((rc = grow(ctx, ctx->d_flag_ray, ctx->cap_flag_ray, (size_t)nrays))) return rc;
+  if ((rc = grow(ctx, ctx->d_out7, ctx->cap_out7, (size_t)nrays * 7))) return rc;
   int cap = ctx->traj_cap_steps < P.maxnstep ? ctx->traj_cap_steps : P.maxnstep;
   const bool two = P.target_nturns < 0 && ctx->trace_budget > 0 && cap > ctx->trace_budget + 1;
   bool fallback = false;
@@ -1717,13 +1873,13 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
     }
     const bool segd = !S.off.empty();
     if (segd) {
-      k_chain<true><<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, ctx->d_off,
+      k_chain<true><<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, ctx->pix, nrays, cap, stride, ctx->d_off,
                                                              ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, ctx->d_pkt,
                                                              ctx->d_segbase, ctx->seg_len, S.d_map, S.d_flag,
                                                              ctx->d_out7, ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
       HIPCHK(ctx, hipGetLastError());
     }
-    k_chain<false><<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, ctx->d_off,
+    k_chain<false><<<nrays / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->model, P, ctx->pix, nrays, cap, stride, ctx->d_off,
                                                             ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, ctx->d_pkt,
                                                             segd ? ctx->d_segbase : nullptr, ctx->seg_len, S.d_map,
                                                             S.d_flag, ctx->d_out7, ctx->d_flag_ray, ctx->d_status,
@@ -1733,7 +1889,7 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
   }
   const int e1 = one ? 4 : 22;
   double f3 = pow(P.freqcgs, 3);
-  k_finalize<<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(P, nrays, f3, ctx->d_out7, ctx->d_flag_ray,
+  k_finalize<<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(P, ctx->pix, ctx->out_ny, nrays, f3, ctx->d_out7, ctx->d_flag_ray,
                                                            ctx->d_nstep_ray, ctx->d_image, ctx->d_imageS,
                                                            ctx->d_taus, ctx->d_nstep_pix, ctx->d_flag_pix,
                                                            ctx->d_status);
@@ -1773,24 +1929,167 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
   }
   ctx->streamed = !one;
   hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[e1]);
-  ctx->last_npix = (int)npix;
-  ctx->last_unpol = P.only_unpolarized;
-  ctx->have_result = 1;
   if (status[1] != 0x7fffffff) {
     int p = status[1] - 1;
-    ctx->bad_i = p / P.ny;
-    ctx->bad_j = p % P.ny;
+    ctx->bad_i = p >> 16;
+    ctx->bad_j = p & 0xffff;
     snprintf(ctx->err, sizeof(ctx->err), "MaxNStep exceeded in pixel %d %d", ctx->bad_i, ctx->bad_j);
     return IPGPU_ERR_MAXNSTEP;
   }
   if (status[2] != 0x7fffffff) {
     int p = status[2] - 1;
-    ctx->bad_i = p / P.ny;
-    ctx->bad_j = p % P.ny;
+    ctx->bad_i = p >> 16;
+    ctx->bad_j = p & 0xffff;
     snprintf(ctx->err, sizeof(ctx->err), "NaN in image at pixel %d %d", ctx->bad_i, ctx->bad_j);
     return IPGPU_ERR_NAN;
   }
-  return ipgpu_fetch(ctx, image, imageS, taus, nstep, flag);
+  return IPGPU_OK;
+}
+
+/* ------------------------------------------------------ adaptive refinement */
+
+int ipgpu_render_adaptive(ipgpu_ctx *ctx, const ipgpu_image *img, const ipgpu_refine *rf, double *image,
+                          double *imageS, double *taus, int *nstep, int *flag, int *interp)
+{
+  if (!ctx || !img || !rf || !image) return IPGPU_ERR_ARG;
+  const ipgpu_image &P = *img;
+  if (rf->nx_min <= 0 || rf->nx_min == P.nx) {
+    ctx->ref_levels = 1;
+    ctx->ref_iavg = 0.;
+    int rc = ipgpu_render(ctx, img, image, imageS, taus, nstep, flag);
+    ctx->ref_traced[0] = (long long)P.nx * P.ny;
+    ctx->ref_interp[0] = 0;
+    if (rc == IPGPU_OK && interp) memset(interp, 0, sizeof(int) * (size_t)P.nx * P.ny);
+    return rc;
+  }
+  if (!ctx->has_geom || !ctx->has_model) return set_err(ctx, IPGPU_ERR_STATE, "geometry/model not set");
+  if (ctx->model.kind == IPGPU_MODEL_IHARM && !ctx->d_dump)
+    return set_err(ctx, IPGPU_ERR_STATE, "no fluid dump uploaded");
+  if (P.nx <= 0 || P.ny <= 0 || P.maxnstep < 2 || rf->ny_min <= 0 || rf->nx_min > P.nx || rf->ny_min > P.ny)
+    return set_err(ctx, IPGPU_ERR_ARG, "bad image size, refinement size or maxnstep");
+  if (P.emission_type != 4 && P.emission_type != 1)
+    return set_err(ctx, IPGPU_ERR_ARG, "emission_type must be 4 (Dexter thermal) or 1 (Pandya thermal)");
+  /* src/main.c:104-124 */
+  const int ratio = (P.nx - P.nx % 2) / (rf->nx_min - rf->nx_min % 2);
+  if (ratio < 1) return set_err(ctx, IPGPU_ERR_ARG, "nx_min too large");
+  const int levels = (int)(log2((double)ratio) + 1);
+  if (levels > IPGPU_MAX_LEVELS) return set_err(ctx, IPGPU_ERR_ARG, "too many refinement levels");
+  const int nxi = (levels > 1 && P.nx % 2 == 0) ? P.nx + 1 : P.nx;
+  const int nyi = (levels > 1 && P.ny % 2 == 0) ? P.ny + 1 : P.ny;
+  const int nxmi = (levels > 1 && P.nx % 2 == 0) ? rf->nx_min + 1 : rf->nx_min;
+  const int nymi = (levels > 1 && P.ny % 2 == 0) ? rf->ny_min + 1 : rf->ny_min;
+  if (levels == 1) {
+    ipgpu_refine r1 = *rf;
+    r1.nx_min = -1;
+    return ipgpu_render_adaptive(ctx, img, &r1, image, imageS, taus, nstep, flag, interp);
+  }
+  const int sp = 1 << (levels - 1);
+  if (nxmi < 2 || nymi < 2 || nxi - 1 != (nxmi - 1) * sp || nyi - 1 != (nymi - 1) * sp || nxi > 32767
+      || nyi > 32767)
+    return set_err(ctx, IPGPU_ERR_ARG, "the internal grid must be (n_min - 1) 2^(levels - 1) + 1 on both axes");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t npi = (size_t)nxi * nyi;
+  int rc;
+  if ((rc = grow(ctx, ctx->d_image, ctx->cap_image, npi))) return rc;
+  if ((rc = grow(ctx, ctx->d_taus, ctx->cap_taus, npi))) return rc;
+  if ((rc = grow(ctx, ctx->d_nstep_pix, ctx->cap_nstep_pix, npi))) return rc;
+  if ((rc = grow(ctx, ctx->d_flag_pix, ctx->cap_flag_pix, npi))) return rc;
+  if ((rc = grow(ctx, ctx->d_imageS, ctx->cap_imageS, npi * 5))) return rc;
+  if ((rc = grow(ctx, ctx->d_interp, ctx->cap_interp, npi))) return rc;
+  if ((rc = grow(ctx, ctx->d_rlist, ctx->cap_rlist, npi + WAVE))) return rc;
+  if (!ctx->d_rcount) {
+    HIPCHK(ctx, hipMalloc((void **)&ctx->d_rcount, 2 * sizeof(unsigned long long)));
+    HIPCHK(ctx, hipMalloc((void **)&ctx->d_iavg, sizeof(double)));
+  }
+  hipStream_t st = ctx->stream;
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_image, 0, npi * sizeof(double), st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_taus, 0, npi * sizeof(double), st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_imageS, 0, npi * 5 * sizeof(double), st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_nstep_pix, 0, npi * sizeof(int), st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_flag_pix, 0, npi * sizeof(int), st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_interp, 0, npi * sizeof(int), st));
+  ctx->have_result = 0;       /* the device buffers hold the internal grid, not ipgpu_fetch's layout */
+  ctx->ref_levels = levels;
+  for (int l = 0; l < IPGPU_MAX_LEVELS; l++) ctx->ref_traced[l] = ctx->ref_interp[l] = 0;
+  /* coarse pass: the base grid in the reference's loop order (i outer) */
+  const int nbase = nxmi * nymi;
+  const int nbpad = (nbase + WAVE - 1) / WAVE * WAVE;
+  {
+    std::vector<int> hl(nbpad, -1);
+    for (int a = 0; a < nxmi; a++)
+      for (int b = 0; b < nymi; b++) hl[(size_t)a * nymi + b] = ((a * sp) << 16) | (b * sp);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_rlist, hl.data(), nbpad * sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+  }
+  ctx->pix = ctx->d_rlist;
+  ctx->out_ny = nyi;
+  rc = render_rays(ctx, P, nbpad);
+  ctx->pix = nullptr;
+  if (rc) return rc;
+  ctx->ref_traced[0] = nbase;
+  RefineK R{nxi, nyi, sp, nymi, rf->refine_abs, rf->refine_rel, rf->refine_cut, rf->nearest_neighbor};
+  const double f3 = pow(P.freqcgs, 3);
+  k_refine_iavg<<<1, WAVE, 0, st>>>(R, nbase, f3, ctx->d_out7, ctx->d_iavg);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipMemcpyAsync(&ctx->ref_iavg, ctx->d_iavg, sizeof(double), hipMemcpyDeviceToHost, st));
+  unsigned int *cnt = reinterpret_cast<unsigned int *>(ctx->d_rcount);
+  long long interp_total = 0;
+  for (int lev = 1; lev < levels; lev++) {
+    const int ns = sp >> lev;
+    const long long cand = (long long)((nxi - 1) / ns + 1) * ((nyi - 1) / ns + 1);
+    HIPCHK(ctx, hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned int), st));
+    k_refine_level<<<(unsigned)((cand + 255) / 256), 256, 0, st>>>(R, ns, P.only_unpolarized, ctx->d_iavg,
+                                                                   ctx->d_image, ctx->d_imageS, ctx->d_taus,
+                                                                   ctx->d_interp, ctx->d_rlist, cnt);
+    HIPCHK(ctx, hipGetLastError());
+    unsigned int hc[2];
+    HIPCHK(ctx, hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    const int n = (int)hc[0];
+    interp_total += hc[1];
+    ctx->ref_traced[lev] = n;
+    ctx->ref_interp[lev] = interp_total;
+    if (n == 0) continue;
+    const int npad = (n + WAVE - 1) / WAVE * WAVE;
+    if (npad > n) HIPCHK(ctx, hipMemsetAsync(ctx->d_rlist + n, 0xff, (size_t)(npad - n) * sizeof(int), st));
+    ctx->pix = ctx->d_rlist;
+    rc = render_rays(ctx, P, npad);
+    ctx->pix = nullptr;
+    if (rc) return rc;
+  }
+  /* the nx x ny part of the internal grid */
+  HIPCHK(ctx, hipMemcpy2DAsync(image, P.ny * sizeof(double), ctx->d_image, nyi * sizeof(double),
+                               P.ny * sizeof(double), P.nx, hipMemcpyDeviceToHost, st));
+  if (taus)
+    HIPCHK(ctx, hipMemcpy2DAsync(taus, P.ny * sizeof(double), ctx->d_taus, nyi * sizeof(double),
+                                 P.ny * sizeof(double), P.nx, hipMemcpyDeviceToHost, st));
+  if (imageS && !P.only_unpolarized)
+    HIPCHK(ctx, hipMemcpy2DAsync(imageS, P.ny * 5 * sizeof(double), ctx->d_imageS, nyi * 5 * sizeof(double),
+                                 P.ny * 5 * sizeof(double), P.nx, hipMemcpyDeviceToHost, st));
+  if (nstep)
+    HIPCHK(ctx, hipMemcpy2DAsync(nstep, P.ny * sizeof(int), ctx->d_nstep_pix, nyi * sizeof(int),
+                                 P.ny * sizeof(int), P.nx, hipMemcpyDeviceToHost, st));
+  if (flag)
+    HIPCHK(ctx, hipMemcpy2DAsync(flag, P.ny * sizeof(int), ctx->d_flag_pix, nyi * sizeof(int), P.ny * sizeof(int),
+                                 P.nx, hipMemcpyDeviceToHost, st));
+  if (interp)
+    HIPCHK(ctx, hipMemcpy2DAsync(interp, P.ny * sizeof(int), ctx->d_interp, nyi * sizeof(int), P.ny * sizeof(int),
+                                 P.nx, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  return IPGPU_OK;
+}
+
+int ipgpu_adaptive_stats(const ipgpu_ctx *ctx, int *levels, double *Iavg, long long *traced,
+                         long long *interpolated)
+{
+  if (!ctx) return IPGPU_ERR_ARG;
+  if (levels) *levels = ctx->ref_levels;
+  if (Iavg) *Iavg = ctx->ref_iavg;
+  for (int l = 0; l < IPGPU_MAX_LEVELS; l++) {
+    if (traced) traced[l] = ctx->ref_traced[l];
+    if (interpolated) interpolated[l] = ctx->ref_interp[l];
+  }
+  return IPGPU_OK;
 }
 
 int ipgpu_fetch(ipgpu_ctx *ctx, double *image, double *imageS, double *taus, int *nstep, int *flag)

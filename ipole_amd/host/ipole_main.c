@@ -41,7 +41,8 @@ typedef struct {
   int emission_type, stokes_floors, isolate_counterjet, target_nturns, subring_dtheta;
   double eps, xoff, yoff;
   int maxnstep, trace, histo, histo_polar, nx_min, ny_min;
-  int trace_stride, trace_i, trace_j;
+  int trace_stride, trace_i, trace_j, nearest_neighbor;
+  double refine_abs, refine_rel, refine_cut;
   char outf[2048], trace_outf[2048];
   /* model keys: model/iharm/model.c:130-167, model/sphere/model.c:33-42 */
   char model[32], dump[2048];
@@ -68,6 +69,9 @@ static void defaults(params_t *p)
   p->target_nturns = -1;
   p->nx_min = -1;
   p->ny_min = -1;
+  p->refine_rel = 1.e-1;
+  p->refine_abs = 1.e-5;
+  p->refine_cut = 0.0;
   strcpy(p->outf, "image.h5");
   p->trace_stride = 1;
   p->trace_i = -1;
@@ -138,6 +142,10 @@ static void try_set(const char *w, const char *v, params_t *p)
   set_i(w, v, "ny", &p->ny);
   set_i(w, v, "nx_min", &p->nx_min);
   set_i(w, v, "ny_min", &p->ny_min);
+  set_d(w, v, "refine_abs", &p->refine_abs);
+  set_d(w, v, "refine_rel", &p->refine_rel);
+  set_d(w, v, "refine_cut", &p->refine_cut);
+  set_i(w, v, "use_nearest_neighbor", &p->nearest_neighbor);
   set_i(w, v, "target_nturns", &p->target_nturns);
   set_i(w, v, "subring_dtheta", &p->subring_dtheta);
   set_d(w, v, "eps", &p->eps);
@@ -254,8 +262,19 @@ int main(int argc, char *argv[])
   params_t par;
   defaults(&par);
   load_par_from_argv(argc, argv, &par);
-  if (par.nx_min > 0 && par.nx_min != par.nx)
-    fprintf(stderr, "! adaptive refinement is not produced by this build: rendering %dx%d\n", par.nx, par.ny);
+  /* src/main.c:98-106 */
+  if (par.nx_min < 0) {
+    par.nx_min = par.nx;
+    par.ny_min = par.ny;
+  }
+  int refine_level = 1;
+  if (par.nx_min != par.nx || par.ny_min != par.ny) {
+    if (par.nx_min < 2 || par.nx_min > par.nx || par.ny_min < 2 || par.ny_min > par.ny) {
+      fprintf(stderr, "! nx_min, ny_min must lie in 2 .. nx, ny\n");
+      exit(1);
+    }
+    refine_level = (int)(log2((par.nx - par.nx % 2) / (par.nx_min - par.nx_min % 2)) + 1);
+  }
   if (par.target_mdot > 0) {
     fprintf(stderr, "! mdot normalization is not supported by this build; set M_unit\n");
     exit(1);
@@ -405,7 +424,16 @@ int main(int argc, char *argv[])
   fprintf(stderr, "intensity [cgs] to flux per pixel [Jy] conversion: %g\n", scale);
   fprintf(stderr, "Dsource: %g [cm]\n", Dsource);
   fprintf(stderr, "FOVx, FOVy: %g %g [GM/c^2]\n", DX, DY);
-  fprintf(stderr, "Resolution: %dx%d\n", par.nx, par.ny);
+  if (refine_level > 1) {
+    fprintf(stderr, "Resolution: %dx%d, refined up to %dx%d (%d levels)\n", par.nx_min, par.ny_min, par.nx,
+            par.ny, refine_level);
+    fprintf(stderr,
+            "Refinement when relative error is >%g%% or absolute error is >%g%% of estimated total flux, for "
+            "pixel brightness > %f%% of average\n",
+            par.refine_rel * 100, par.refine_abs * 100, par.refine_cut * 100);
+  } else {
+    fprintf(stderr, "Resolution: %dx%d\n", par.nx, par.ny);
+  }
 
   ipgpu_image im;
   memset(&im, 0, sizeof(im));
@@ -440,7 +468,33 @@ int main(int argc, char *argv[])
     exit(-1);
   }
   double init_time = now() - t_start;
-  int rc = ipgpu_render(ctx, &im, image, imageS, taus, nstep, flag);
+  int rc;
+  if (refine_level > 1) {
+    /* src/main.c:505-792 on the device: the coarse pass, then one pass per
+     * level over the pixels the error estimate selects */
+    ipgpu_refine rf = {par.nx_min, par.ny_min, par.refine_abs, par.refine_rel, par.refine_cut,
+                       par.nearest_neighbor};
+    rc = ipgpu_render_adaptive(ctx, &im, &rf, image, imageS, taus, nstep, flag, NULL);
+    if (rc == IPGPU_OK) {
+      int lv;
+      double Iavg;
+      long long tr[IPGPU_MAX_LEVELS], ip[IPGPU_MAX_LEVELS];
+      ipgpu_adaptive_stats(ctx, &lv, &Iavg, tr, ip);
+      fprintf(stderr, "\n\n");
+      const int nxi = (par.nx % 2 == 0) ? par.nx + 1 : par.nx;
+      const int nxmi = (par.nx % 2 == 0) ? par.nx_min + 1 : par.nx_min;
+      const long sp = (nxi - 1) / (nxmi - 1);
+      for (int l = 1; l < lv; l++) {
+        long ns = sp >> l;
+        fprintf(stderr, "Refining level %d of %d, spacing %ld,%ld\n", l + 1, lv, ns, ns);
+        long nxl = par.nx / ns, nyl = par.ny / ns;
+        fprintf(stderr, "\n%lld of %ld (%f%%) of computed pixels at %ldx%ld were interpolated\n\n", ip[l],
+                nxl * nyl, ((double)ip[l]) / (nxl * nyl) * 100, nxl, nyl);
+      }
+    }
+  } else {
+    rc = ipgpu_render(ctx, &im, image, imageS, taus, nstep, flag);
+  }
   if (rc == IPGPU_ERR_MAXNSTEP) {
     int bi, bj;
     ipgpu_bad_pixel(ctx, &bi, &bj);

@@ -19,7 +19,7 @@ ip = C.POINTER(C.c_int)
 METRIC_MKS, METRIC_FMKS, METRIC_EMINKOWSKI = 0, 2, 5
 MODEL_IHARM, MODEL_SPHERE = 1, 2
 NFIELDS = 10
-ERR_MAXNSTEP = -10
+ERR_ARG, ERR_HIP, ERR_NAN, ERR_NOMEM, ERR_STATE, ERR_MAXNSTEP = -1, -2, -3, -4, -5, -10
 
 # constants of src/constants.h
 CL, ME, MP, HPL, GNEWT = 2.99792458e10, 9.1093826e-28, 1.67262171e-24, 6.6260693e-27, 6.6742e-8
@@ -56,6 +56,22 @@ class Image(C.Structure):
                 ("only_unpolarized", C.c_int), ("qu_conv", C.c_int), ("stokes_floors", C.c_int),
                 ("target_nturns", C.c_int), ("subring_dtheta", C.c_int), ("counterjet", C.c_int),
                 ("emission_type", C.c_int), ("transfer_fp32", C.c_int)]
+
+
+class Refine(C.Structure):
+    """ipgpu_refine: adaptive refinement (src/par.c:77-84 defaults)."""
+    _fields_ = [("nx_min", C.c_int), ("ny_min", C.c_int), ("refine_abs", C.c_double),
+                ("refine_rel", C.c_double), ("refine_cut", C.c_double), ("nearest_neighbor", C.c_int)]
+
+
+def make_refine(nx_min, ny_min=None, refine_abs=1e-5, refine_rel=1e-1, refine_cut=0., nearest_neighbor=0):
+    r = Refine()
+    r.nx_min, r.ny_min = nx_min, nx_min if ny_min is None else ny_min
+    r.refine_abs, r.refine_rel, r.refine_cut, r.nearest_neighbor = refine_abs, refine_rel, refine_cut, nearest_neighbor
+    return r
+
+
+MAX_LEVELS = 16
 
 
 class Job(C.Structure):
@@ -132,6 +148,8 @@ def lib():
         L.ipgpu_stream.restype = vp
         L.ipgpu_native_coord.argtypes = [C.POINTER(Geom), C.c_double, C.c_double, C.c_double, dp]
         L.ipgpu_trace_pixel.argtypes = [vp, C.POINTER(Image), C.c_int, C.c_int, C.c_int, ip, dp]
+        L.ipgpu_render_adaptive.argtypes = [vp, C.POINTER(Image), C.POINTER(Refine), dp, dp, dp, ip, ip, ip]
+        L.ipgpu_adaptive_stats.argtypes = [vp, ip, dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         _lib = L
     return _lib
 
@@ -420,6 +438,32 @@ class Context:
         ray; -1 keeps a setting."""
         self._chk(self.L.ipgpu_set_tuning(self.h, int(trace_budget), int(trace_phases), int(seg_min_steps),
                                           int(seg_len), int(record_steps)))
+
+    def render_adaptive(self, image, refine):
+        """Adaptive refinement (src/main.c:505-792): dict of the nx x ny
+        image arrays as render() returns them plus interp (1 where a pixel
+        was interpolated) and the stats of adaptive_stats()."""
+        nx, ny = image.nx, image.ny
+        n = nx * ny
+        out = dict(image=np.zeros(n), imageS=np.zeros(5 * n), taus=np.zeros(n),
+                   nstep=np.zeros(n, np.int32), flag=np.zeros(n, np.int32), interp=np.zeros(n, np.int32))
+        rc = self.L.ipgpu_render_adaptive(self.h, C.byref(image), C.byref(refine), P(out["image"]),
+                                          P(out["imageS"]), P(out["taus"]), IP(out["nstep"]), IP(out["flag"]),
+                                          IP(out["interp"]))
+        if rc == ERR_MAXNSTEP:
+            i, j = C.c_int(), C.c_int()
+            self.L.ipgpu_bad_pixel(self.h, C.byref(i), C.byref(j))
+            raise IpgpuError(rc, "MaxNStep exceeded in pixel %d %d" % (i.value, j.value))
+        self._chk(rc)
+        res = {k: (v.reshape(nx, ny, 5) if k == "imageS" else v.reshape(nx, ny)) for k, v in out.items()}
+        res.update(self.adaptive_stats())
+        return res
+
+    def adaptive_stats(self):
+        lv, ia = C.c_int(), C.c_double()
+        tr, it = (C.c_longlong * MAX_LEVELS)(), (C.c_longlong * MAX_LEVELS)()
+        self._chk(self.L.ipgpu_adaptive_stats(self.h, C.byref(lv), C.byref(ia), tr, it))
+        return dict(levels=lv.value, Iavg=ia.value, traced=list(tr)[:lv.value], interpolated=list(it)[:lv.value])
 
     def trace_pixel(self, image, i, j):
         """Trace output of pixel (i, j): (nstep, records) with records of

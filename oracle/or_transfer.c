@@ -506,6 +506,195 @@ static int pixel(long i, long j, const or_params_t *p, const or_camera_t *cam, o
   return 0;
 }
 
+/* ------------------------------------------------------ adaptive refinement */
+
+/* save_pixel, src/main.c:885-910, into an image of row length ny */
+static void save_px(double *image, double *imageS, double *taus, size_t i, size_t j, size_t ny,
+                    const or_params_t *p, const double o[7])
+{
+  const double f3 = pow(p->freqcgs, 3);
+  image[i * ny + j] = o[0] * f3;
+  taus[i * ny + j] = o[5];
+  if (!p->only_unpolarized) {
+    double *S = imageS + (i * ny + j) * 5;
+    S[0] = o[1] * f3;
+    S[1] = (p->qu_conv == 0 ? -o[2] : o[2]) * f3;
+    S[2] = (p->qu_conv == 0 ? -o[3] : o[3]) * f3;
+    S[3] = o[4] * f3;
+    S[4] = o[6];
+  }
+}
+
+/* Adaptive refinement, src/main.c:98-124 (levels and the internal 2^N + 1
+ * grid) and :505-792 (the coarse pass, the intensity estimate and the
+ * levels), with save_pixelTransfer, lininterp2 and lininterp4 (:912-1000).
+ * The pixels a level traces are decided first and traced together: each
+ * decision reads only the coarser grid, so this equals the reference's
+ * interleaved loop.  Outputs are the nx x ny part of the internal grid;
+ * Iavg, traced[level] and interpolated[level] (the running count the
+ * reference prints) as ipgpu_adaptive_stats reports them.  Returns 0, -10
+ * (maxnstep), or -2 when the sizes do not form the 2^N + 1 scheme. */
+int or_render_adaptive(const or_params_t *p, or_camera_t *cam, int nx_min, int ny_min, double refine_abs,
+                       double refine_rel, double refine_cut, int nearest, double *image_out,
+                       double *imageS_out, double *taus_out, int *interp_out, double *Iavg_out,
+                       long long *traced, long long *interpolated, int nthreads)
+{
+  int rc = or_setup_camera(p, cam);
+  if (rc) return rc;
+  const int refine_level = (int)(log2((p->nx - p->nx % 2) / (nx_min - nx_min % 2)) + 1);
+  size_t nx, ny, nxmin, nymin;
+  if (refine_level > 1 && p->nx % 2 == 0) {
+    nx = p->nx + 1;
+    nxmin = nx_min + 1;
+  } else {
+    nx = p->nx;
+    nxmin = nx_min;
+  }
+  if (refine_level > 1 && p->ny % 2 == 0) {
+    ny = p->ny + 1;
+    nymin = ny_min + 1;
+  } else {
+    ny = p->ny;
+    nymin = ny_min;
+  }
+  const size_t sp = (nx - 1) / (nxmin - 1), spy = (ny - 1) / (nymin - 1);
+  if (refine_level < 2 || sp != ((size_t)1 << (refine_level - 1)) || spy != sp || (nx - 1) % (nxmin - 1)
+      || (ny - 1) % (nymin - 1))
+    return -2;
+  double *image = calloc(nx * ny, sizeof(double)), *taus = calloc(nx * ny, sizeof(double));
+  double *imageS = calloc(nx * ny * 5, sizeof(double));
+  int *interp = calloc(nx * ny, sizeof(int));
+  int *pix = malloc(sizeof(int) * 2 * nx * ny);
+  double *out7 = malloc(sizeof(double) * 7 * nx * ny);
+  int *ns = malloc(sizeof(int) * nx * ny), *fl = malloc(sizeof(int) * nx * ny);
+  double *prelim = calloc(nxmin * nymin, sizeof(double));
+  /* coarse pass */
+  int n = 0;
+  for (size_t i = 0; i < nx; i += sp)
+    for (size_t j = 0; j < ny; j += sp) {
+      pix[2 * n] = (int)i;
+      pix[2 * n + 1] = (int)j;
+      n++;
+    }
+  rc = or_render_pixels(p, cam, pix, n, out7, ns, fl, nthreads);
+  if (rc == 0) {
+    for (int q = 0; q < n; q++) {
+      const size_t i = pix[2 * q], j = pix[2 * q + 1];
+      save_px(image, imageS, taus, i, j, ny, p, out7 + 7 * (size_t)q);
+      const double Intensity = out7[7 * (size_t)q];
+      const size_t loc = i / sp * nymin + j / sp;
+      if (i % (nx - 1) != 0 && j % (ny - 1) != 0) {
+        prelim[loc] = Intensity * sp * sp;
+      } else if ((i == 0 && j % (ny - 1) != 0) || (i % (nx - 1) != 0 && j == 0)) {
+        prelim[loc] = Intensity * (sp / 2 + 1) * sp;
+      } else if (i % (nx - 1) != 0 || j % (ny - 1) != 0) {
+        prelim[loc] = Intensity * (sp / 2) * sp;
+      } else {
+        if (i == 0 && j == 0) prelim[loc] = Intensity * (sp / 2 + 1) * (sp / 2 + 1);
+        else if (i == 0 || j == 0) prelim[loc] = Intensity * (sp / 2 + 1) * sp / 2;
+        else prelim[loc] = Intensity * (sp * sp) / 4;
+      }
+    }
+    traced[0] = n;
+    interpolated[0] = 0;
+  }
+  double interp_tot = 0;
+  for (size_t q = 0; q < nxmin * nymin; q++) interp_tot += prelim[q];
+  const double Iavg = interp_tot * pow(p->freqcgs, 3) / (nx * ny);
+  *Iavg_out = Iavg;
+  long long total_interp = 0;
+  for (int lev = 1; lev < refine_level && rc == 0; lev++) {
+    const size_t nsx = sp / (size_t)pow(2, lev), prev = nsx * 2;
+    n = 0;
+    for (size_t i = 0; i < nx; i += nsx)
+      for (size_t j = 0; j < ny; j += nsx) {
+        double I1, I2, I3, err_abs, err_rel;
+        size_t s0, s1, s2 = 0, s3 = 0;
+        int four = 0;
+        if (i % prev == 0 && j % prev == 0) continue;
+        if (i % prev == 0) {
+          s0 = i * ny + j - nsx;
+          s1 = i * ny + j + nsx;
+          I1 = image[s0];
+          I2 = image[s1];
+          err_abs = (I2 - I1) / 2 / Iavg;
+          err_rel = (I2 - I1) / 2 / I1;
+        } else if (j % prev == 0) {
+          s0 = (i - nsx) * ny + j;
+          s1 = (i + nsx) * ny + j;
+          I1 = image[s0];
+          I2 = image[s1];
+          err_abs = (I2 - I1) / 2 / Iavg;
+          err_rel = (I2 - I1) / 2 / I1;
+        } else {
+          s0 = (i - nsx) * ny + j - nsx;
+          s1 = (i + nsx) * ny + j - nsx;
+          s2 = (i - nsx) * ny + j + nsx;
+          s3 = (i + nsx) * ny + (j + nsx);
+          four = 1;
+          I1 = image[s0];
+          I2 = image[s1];
+          I3 = image[s2];
+          err_abs = ((I2 + I3) / 2 - I1) / Iavg;
+          err_rel = (I2 + I3) / (2 * I1) - 1.;
+        }
+        const size_t t = i * ny + j;
+        if ((fabs(err_abs) > refine_abs && fabs(err_rel) > refine_rel) && fabs(I1) > refine_cut) {
+          pix[2 * n] = (int)i;
+          pix[2 * n + 1] = (int)j;
+          n++;
+          interp[t] = 0;
+          continue;
+        }
+        interp[t] = 1;
+        total_interp++;
+        if (nearest) {
+          image[t] = image[s0];
+          if (!p->only_unpolarized) {
+            taus[t] = taus[s0];
+            for (int k = 0; k < 5; k++) imageS[t * 5 + k] = imageS[s0 * 5 + k];
+          }
+        } else if (!four) {
+          image[t] = .5 * (image[s0] + image[s1]);
+          taus[t] = .5 * (taus[s0] + taus[s1]);
+          if (!p->only_unpolarized)
+            for (int k = 0; k < 5; k++) imageS[t * 5 + k] = .5 * (imageS[s0 * 5 + k] + imageS[s1 * 5 + k]);
+        } else {
+          image[t] = .25 * (image[s0] + image[s1] + image[s2] + image[s3]);
+          taus[t] = .25 * (taus[s0] + taus[s1] + taus[s2] + taus[s3]);
+          if (!p->only_unpolarized)
+            for (int k = 0; k < 5; k++)
+              imageS[t * 5 + k] = .25 * (imageS[s0 * 5 + k] + imageS[s1 * 5 + k] + imageS[s2 * 5 + k]
+                                         + imageS[s3 * 5 + k]);
+        }
+      }
+    if (n) rc = or_render_pixels(p, cam, pix, n, out7, ns, fl, nthreads);
+    if (rc == 0)
+      for (int q = 0; q < n; q++) save_px(image, imageS, taus, pix[2 * q], pix[2 * q + 1], ny, p, out7 + 7 * (size_t)q);
+    traced[lev] = n;
+    interpolated[lev] = total_interp;
+  }
+  for (size_t i = 0; i < (size_t)p->nx; i++)
+    for (size_t j = 0; j < (size_t)p->ny; j++) {
+      const size_t a = i * p->ny + j, b = i * ny + j;
+      image_out[a] = image[b];
+      taus_out[a] = taus[b];
+      interp_out[a] = interp[b];
+      if (!p->only_unpolarized)
+        for (int k = 0; k < 5; k++) imageS_out[a * 5 + k] = imageS[b * 5 + k];
+    }
+  free(image);
+  free(taus);
+  free(imageS);
+  free(interp);
+  free(pix);
+  free(out7);
+  free(ns);
+  free(fl);
+  free(prelim);
+  return rc;
+}
+
 /* src/ipolarray.c:297-320: N carried by dl (cgs) with the connection at
  * Xm, Km */
 static void transport_vector(const double Xm[4], const double Km[4], const double Ni[4],

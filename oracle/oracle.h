@@ -214,6 +214,11 @@ int or_integrate_emission(or_traj_t *traj, int nsteps, double *Intensity, double
 int or_trace_pixel(const or_params_t *p, const or_camera_t *cam, long i, long j, const double *rub,
                    int cap, double *out, int *nstep_out);
 
+int or_render_adaptive(const or_params_t *p, or_camera_t *cam, int nx_min, int ny_min, double refine_abs,
+                       double refine_rel, double refine_cut, int nearest, double *image, double *imageS,
+                       double *taus, int *interp, double *Iavg, long long *traced, long long *interpolated,
+                       int nthreads);
+
 /* image (src/main.c) */
 int or_setup_camera(const or_params_t *p, or_camera_t *cam);
 int or_render(const or_params_t *p, or_camera_t *cam, double *image, double *imageS,

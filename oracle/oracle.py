@@ -139,6 +139,10 @@ def load(path):
         L.or_trace_pixel.restype = C.c_int
         L.or_integrate_emission.argtypes = [C.POINTER(Traj), C.c_int, dp, dp, dp, dp, C.POINTER(Params)]
         L.or_integrate_emission.restype = C.c_int
+        L.or_render_adaptive.argtypes = [C.POINTER(Params), C.POINTER(Camera), C.c_int, C.c_int, C.c_double,
+                                         C.c_double, C.c_double, C.c_int, dp, dp, dp, ip, dp,
+                                         C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.c_int]
+        L.or_render_adaptive.restype = C.c_int
         L.or_sphere_setup.argtypes = [C.c_double] * 7 + [C.POINTER(Model)]
         L.or_analytic_setup.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
                                         C.POINTER(Model), C.POINTER(Geom)]
@@ -318,6 +322,24 @@ class Oracle:
         return dict(rc=rc, cam=cam, image=image.reshape(params.nx, params.ny),
                     imageS=imageS.reshape(params.nx, params.ny, 5), taus=taus.reshape(params.nx, params.ny),
                     nstep=nsteps.reshape(params.nx, params.ny), flag=flags.reshape(params.nx, params.ny))
+
+    def render_adaptive(self, params, nx_min, ny_min=None, refine_abs=1e-5, refine_rel=1e-1, refine_cut=0.,
+                        nearest=0, nthreads=0):
+        """Adaptive refinement of src/main.c:505-792 (or_render_adaptive)."""
+        nx, ny = params.nx, params.ny
+        n = nx * ny
+        image, imageS, taus = np.zeros(n), np.zeros(5 * n), np.zeros(n)
+        interp = np.zeros(n, np.int32)
+        Iavg = C.c_double()
+        tr, it = (C.c_longlong * 16)(), (C.c_longlong * 16)()
+        cam = Camera()
+        rc = self.L.or_render_adaptive(C.byref(params), C.byref(cam), nx_min, nx_min if ny_min is None else ny_min,
+                                       refine_abs, refine_rel, refine_cut, nearest, P(image), P(imageS), P(taus),
+                                       IP(interp), C.byref(Iavg), tr, it, nthreads)
+        lv = int(np.log2((nx - nx % 2) // (nx_min - nx_min % 2)) + 1)
+        return dict(rc=rc, cam=cam, image=image.reshape(nx, ny), imageS=imageS.reshape(nx, ny, 5),
+                    taus=taus.reshape(nx, ny), interp=interp.reshape(nx, ny), Iavg=Iavg.value, levels=lv,
+                    traced=list(tr)[:lv], interpolated=list(it)[:lv])
 
     def render_pixels(self, params, cam, pix, nthreads=0):
         pix = np.ascontiguousarray(pix, dtype=np.int32)
