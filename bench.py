@@ -271,6 +271,29 @@ def adjoint_measure(args):
                 density_grad_abs_max=float(np.abs(r["density"]).max()))
 
 
+def adaptive_measure(ctx, im, full, scale, cfg, reps=3):
+    """The same image refined adaptively from 20x20 with the reference's
+    refinement test settings (tests/refinement/refinement.par: refine_rel
+    1e-2, refine_abs 1e-6): wall time per render, rays traced, and the flux
+    against the full render."""
+    from ipole_amd import ipgpu as G
+    rf = G.make_refine(cfg["nx"] // 8, refine_abs=1e-6, refine_rel=1e-2)
+    ctx.render_adaptive(im, rf)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        a = ctx.render_adaptive(im, rf)
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    F_full = full["imageS"][..., 0].sum() * scale
+    F_a = a["imageS"][..., 0].sum() * scale
+    out = dict(nx_min=cfg["nx"] // 8, refine_rel=1e-2, refine_abs=1e-6, ms_wall=round(ms, 3),
+               levels=a["levels"], traced=[int(x) for x in a["traced"]], rays_traced=int(sum(a["traced"])),
+               Ftot_rel_to_full=round(F_a / F_full - 1, 8))
+    log("adaptive %dx%d from %dx%d: %.3f ms wall, %d rays traced %s, Ftot %+.2e vs the full render" %
+        (cfg["nx"], cfg["ny"], cfg["nx"] // 8, cfg["ny"] // 8, ms, out["rays_traced"], out["traced"],
+         out["Ftot_rel_to_full"]))
+    return out
+
+
 def bench_batch(args):
     """--workload batch: the batched figures as the bench line's value
     (FP64 batched pixels/s)."""
@@ -364,6 +387,8 @@ def main():
         ctx.render(im, fetch=False)
         one.append(ctx.kernel_times()[2])
     log("one-pass render of the same image: %.3f ms" % np.mean(one[1:]))
+    ctx.set_tuning(trace_budget=3584)
+    adaptive = adaptive_measure(ctx, im, res, scale, cfg)
     st = G.image_stats(res, scale, cfg["freqcgs"], 16.9e6)
     ns_img = res["nstep"]
     tiles = ns_img[:ns_img.shape[0] // 8 * 8, :ns_img.shape[1] // 8 * 8].reshape(
@@ -413,7 +438,7 @@ def main():
                            ms_transfer=round(float(np.mean(ms_xfer)), 4),
                            streamed=bool(streamed), ms_one_pass=round(float(np.mean(one[1:])), 4)),
                roofline=roof, cpu_baseline=cpu, batched=batched, mixed_precision=precision,
-               adjoint=adjoint)
+               adjoint=adjoint, adaptive=adaptive)
     print(json.dumps(out), flush=True)
 
 
