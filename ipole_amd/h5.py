@@ -33,6 +33,12 @@ class ImageMeta(C.Structure):
                                              "MdotEdd", "Ladv")])
 
 
+class KharmaInfo(C.Structure):
+    """ipole_kharma_info of host/h5io.h."""
+    _fields_ = [("format", C.c_char * 8), ("num_meshblocks", C.c_int), ("mb", C.c_int * 3), ("nghost", C.c_int),
+                ("has_electrons", C.c_int), ("game", C.c_double), ("gamp", C.c_double)]
+
+
 _lib = None
 
 
@@ -52,6 +58,10 @@ def lib():
         L.ipole_h5_trace_close.argtypes = [C.c_int64]
         L.ipole_h5_dataset_shape.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_int64)]
         L.ipole_h5_read_doubles.argtypes = [C.c_char_p, C.c_char_p, dp]
+        L.ipole_h5_read_kharma_header.argtypes = [C.c_char_p, C.POINTER(DumpHeader), C.POINTER(KharmaInfo)]
+        L.ipole_h5_read_kharma_prims.argtypes = [C.c_char_p, C.POINTER(DumpHeader), C.POINTER(KharmaInfo), dp]
+        L.ipole_h5_write_kharma.argtypes = [C.c_char_p, C.POINTER(DumpHeader), C.POINTER(KharmaInfo), C.c_char_p,
+                                            C.POINTER(C.c_int), dp]
         _lib = L
     return _lib
 
@@ -141,3 +151,67 @@ def write_trace(path, meta, nx, ny, maxnstep, pixels):
                 raise IOError("could not write pixel %d %d of %s" % (i, j, path))
     finally:
         L.ipole_h5_trace_close(f)
+
+
+def kharma_par(h, mb, electrons=None, nghost=4):
+    """The parameter file text a KHARMA dump carries (<parthenon/mesh>,
+    <parthenon/meshblock>, <coordinates>, <GRMHD>, <electrons> blocks) for
+    a grid of the dump header h."""
+    t = "fmks" if h["metric"] in ("FMKS", "MMKS") else "mks"
+    N, x0, dx = (h["N1"], h["N2"], h["N3"]), h["startx"], h["dx"]
+    lines = ["<parthenon/job>", "problem_id = torus", "", "<parthenon/mesh>", "refinement = none",
+             "nghost = %d" % nghost]
+    for d in range(3):
+        lines += ["nx%d = %d" % (d + 1, N[d]), "x%dmin = %.17g" % (d + 1, x0[d + 1]),
+                  "x%dmax = %.17g" % (d + 1, x0[d + 1] + N[d] * dx[d + 1])]
+    lines += ["", "<parthenon/meshblock>"] + ["nx%d = %d" % (d + 1, mb[d]) for d in range(3)]
+    lines += ["", "<coordinates>", "base = spherical_ks", "transform = %s" % t, "a = %.17g" % h["a"],
+              "r_in = %.17g" % h["Rin"], "r_out = %.17g" % h["Rout"], "hslope = %.17g" % h["hslope"]]
+    if t == "fmks":
+        lines += ["mks_smooth = %.17g" % h["mks_smooth"], "poly_xt = %.17g" % h["poly_xt"],
+                  "poly_alpha = %.17g" % h["poly_alpha"]]
+    lines += ["", "<GRMHD>", "cfl = 0.9", "gamma = %.17g" % h["gam"], ""]
+    if electrons is not None:
+        lines += ["<electrons>", "on = true", "gamma_e = %.17g" % electrons[0], "gamma_p = %.17g" % electrons[1], ""]
+    else:
+        lines += ["<electrons>", "on = false", ""]
+    return "\n".join(lines)
+
+
+def write_kharma(path, h, prims, mb, order=None, fmt="new", electrons=None):
+    """A synthetic KHARMA (Parthenon) dump of the layout load_kharma_data
+    reads, from an iharm-style header h and primitives [N1][N2][N3][8];
+    mb the meshblock size, order the stored blocks' grid positions."""
+    pr = np.ascontiguousarray(prims, dtype=np.float64)
+    s = header_struct(h)
+    s.n_prim = 8
+    ki = KharmaInfo()
+    ki.format = fmt.encode()
+    for d in range(3):
+        ki.mb[d] = mb[d]
+    oa = None
+    if order is not None:
+        oa = (C.c_int * len(order))(*[int(x) for x in order])
+    if lib().ipole_h5_write_kharma(path.encode(), C.byref(s), C.byref(ki), kharma_par(h, mb, electrons).encode(),
+                                   oa, pr.ctypes.data_as(dp)):
+        raise IOError("could not write %s" % path)
+
+
+def read_kharma(path, fmt="new"):
+    """(header dict, primitives [N1][N2][N3][8], info dict) of a KHARMA dump."""
+    s = DumpHeader()
+    ki = KharmaInfo()
+    ki.format = fmt.encode()
+    L = lib()
+    rc = L.ipole_h5_read_kharma_header(path.encode(), C.byref(s), C.byref(ki))
+    if rc:
+        raise IOError("could not read the KHARMA header of %s (%d)" % (path, rc))
+    pr = np.zeros((s.N1, s.N2, s.N3, 8))
+    if L.ipole_h5_read_kharma_prims(path.encode(), C.byref(s), C.byref(ki), pr.ctypes.data_as(dp)):
+        raise IOError("could not read the KHARMA primitives of %s" % path)
+    h = dict(metric=s.metric.decode(), N1=s.N1, N2=s.N2, N3=s.N3, n_prim=s.n_prim, gam=s.gam, a=s.a,
+             hslope=s.hslope, Rin=s.Rin, Rout=s.Rout, poly_xt=s.poly_xt, poly_alpha=s.poly_alpha,
+             mks_smooth=s.mks_smooth, startx=list(s.startx), dx=list(s.dx), t=s.t)
+    info = dict(num_meshblocks=ki.num_meshblocks, mb=list(ki.mb), game=ki.game, gamp=ki.gamp,
+                has_electrons=ki.has_electrons)
+    return h, pr, info

@@ -44,6 +44,30 @@ int64_t ipole_h5_trace_open(const char *path, const ipole_image_meta *mt, int nx
 int ipole_h5_trace_write(int64_t fid, int i, int j, int nstep, const double *rec);
 int ipole_h5_trace_close(int64_t fid);
 
+/* KHARMA (Parthenon) dumps, model/kharma/model.c:407-1010, 1251-1470.  The
+ * header reader fills the dump header from the parameter file stored in
+ * the file (grid, coordinates, adiabatic index) and the Info attributes;
+ * format ("new", or "old" for KHARMA v5 files) is read from *ki, the rest
+ * of *ki is filled.  Returns 0, -1 (unreadable), or -2 for a coordinate
+ * transform other than fmks / mks.  The primitives come back as
+ * ipole_h5_read_dump_prims returns an iharm dump's: [N1][N2][N3][8].  The
+ * writer makes files of that layout for synthetic inputs (order[b]: grid
+ * position, x1 fastest, of stored block b; NULL for in order). */
+typedef struct {
+  char format[8];
+  int num_meshblocks, mb[3], nghost, has_electrons;
+  double game, gamp;
+} ipole_kharma_info;
+int ipole_h5_read_kharma_header(const char *path, ipole_dump_header *h, ipole_kharma_info *ki);
+int ipole_h5_read_kharma_prims(const char *path, const ipole_dump_header *h, const ipole_kharma_info *ki,
+                               double *prims);
+int ipole_h5_write_kharma(const char *path, const ipole_dump_header *h, const ipole_kharma_info *ki,
+                          const char *par, const int *order, const double *prims);
+/* /fluid_header of an image of a KHARMA dump (model/kharma/model.c:1954-2121):
+ * its numeric keys. */
+int ipole_h5_write_kharma_fluid_header(const char *image_path, const ipole_dump_header *h,
+                                       const ipole_kharma_info *ki);
+
 /* Any dataset: its rank and dims (up to 16), or -1; its values as doubles. */
 int ipole_h5_dataset_shape(const char *path, const char *name, int64_t *dims);
 int ipole_h5_read_doubles(const char *path, const char *name, double *out);

@@ -45,7 +45,7 @@ typedef struct {
   double refine_abs, refine_rel, refine_cut;
   char outf[2048], trace_outf[2048];
   /* model keys: model/iharm/model.c:130-167, model/sphere/model.c:33-42 */
-  char model[32], dump[2048];
+  char model[32], dump[2048], kharma_format[8];
   double MBH, M_unit, trat_small, trat_large, beta_crit, sigma_cut, sigma_cut_high;
   double rmax_geo, rmin_geo, polar_cut, target_mdot;
   int reverse_field;
@@ -80,6 +80,7 @@ static void defaults(params_t *p)
   /* model defaults */
   strcpy(p->model, "iharm");
   strcpy(p->dump, "dump.h5");
+  strcpy(p->kharma_format, "new");
   p->MBH = 6.2e9;
   p->trat_small = 1.;
   p->trat_large = 40.;
@@ -170,6 +171,7 @@ static void try_set(const char *w, const char *v, params_t *p)
   set_d(w, v, "M_unit", &p->M_unit);
   set_d(w, v, "mdot", &p->target_mdot);
   set_s(w, v, "dump", p->dump);
+  if (!strcmp(w, "kharma_format")) snprintf(p->kharma_format, sizeof(p->kharma_format), "%s", v);
   set_d(w, v, "trat_small", &p->trat_small);
   set_d(w, v, "trat_large", &p->trat_large);
   set_d(w, v, "sigma_cut", &p->sigma_cut);
@@ -290,11 +292,29 @@ int main(int argc, char *argv[])
   memset(&geom, 0, sizeof(geom));
   memset(&model, 0, sizeof(model));
   double L_unit, T_unit = 0, Thetae_unit = 0, dump_t = 0;
+  ipole_dump_header kh_h;          /* the KHARMA dump's header, for the image file */
+  ipole_kharma_info kh_ki;
+  memset(&kh_h, 0, sizeof(kh_h));
+  memset(&kh_ki, 0, sizeof(kh_ki));
   int is_iharm = strcmp(par.model, "sphere") != 0;
+  const int is_kharma = !strcmp(par.model, "kharma");
   if (is_iharm) {
     /* init_model / init_iharm_grid / load_iharm_data, model/iharm/model.c:295-389, :848-1074 */
+    /* iharm dumps (model/iharm/model.c:848-1074) or, with model kharma,
+     * Parthenon files (model/kharma/model.c:772-1010): the same fluid model
+     * after loading */
     ipole_dump_header h;
-    if (ipole_h5_read_dump_header(par.dump, &h) != 0) {
+    ipole_kharma_info ki;
+    memset(&ki, 0, sizeof(ki));
+    snprintf(ki.format, sizeof(ki.format), "%s", par.kharma_format);
+    ki.game = 1.333333333333333;
+    ki.gamp = 1.666666666666667;
+    int hrc = is_kharma ? ipole_h5_read_kharma_header(par.dump, &h, &ki) : ipole_h5_read_dump_header(par.dump, &h);
+    if (hrc == -2) {
+      fprintf(stderr, "Unknown coordinate system in %s\n", par.dump);
+      exit(-1);
+    }
+    if (hrc != 0) {
       fprintf(stderr, "! unable to open file %s. Exiting!\n", par.dump);
       exit(-1);
     }
@@ -343,7 +363,8 @@ int main(int argc, char *argv[])
     fprintf(stderr, "L,T,M units: %g [cm] %g [s] %g [g]\n", L_unit, T_unit, par.M_unit);
     size_t n = (size_t)h.N1 * h.N2 * h.N3 * h.n_prim;
     double *prims = malloc(n * sizeof(double));
-    if (!prims || ipole_h5_read_dump_prims(par.dump, &h, prims) != 0) {
+    if (!prims || (is_kharma ? ipole_h5_read_kharma_prims(par.dump, &h, &ki, prims)
+                             : ipole_h5_read_dump_prims(par.dump, &h, prims)) != 0) {
       fprintf(stderr, "! unable to read prims from %s. Exiting!\n", par.dump);
       exit(-1);
     }
@@ -351,8 +372,10 @@ int main(int argc, char *argv[])
       for (size_t c = 0; c < (size_t)h.N1 * h.N2 * h.N3; c++)
         for (int l = 5; l < 8; l++) prims[c * h.n_prim + l] *= -1.0;
     dump_t = h.t;
+    kh_h = h;
+    kh_ki = ki;
     ipgpu_units u = {par.M_unit, par.MBH, par.trat_small, par.trat_large, par.beta_crit,
-                     h.gam, 1.333333333333333, 1.666666666666667};
+                     h.gam, ki.game, ki.gamp};
     CHECK(ipgpu_set_geometry(ctx, &geom));
     CHECK(ipgpu_set_model(ctx, &model));
     CHECK(ipgpu_upload_prims(ctx, prims, h.n_prim, &u));
@@ -616,7 +639,8 @@ int main(int argc, char *argv[])
           }
           if (!existed && is_iharm) {
             ipole_h5_trace_close(tf);
-            ipole_h5_copy_fluid_header(par.dump, par.trace_outf);
+            if (is_kharma) ipole_h5_write_kharma_fluid_header(par.trace_outf, &kh_h, &kh_ki);
+            else ipole_h5_copy_fluid_header(par.dump, par.trace_outf);
             tf = ipole_h5_trace_open(par.trace_outf, &mt, tnx, tny, par.maxnstep);
           }
         }
@@ -636,7 +660,8 @@ int main(int argc, char *argv[])
       fprintf(stderr, "! could not write %s\n", par.outf);
       exit(-1);
     }
-    if (is_iharm) ipole_h5_copy_fluid_header(par.dump, par.outf);
+    if (is_iharm && !is_kharma) ipole_h5_copy_fluid_header(par.dump, par.outf);
+    if (is_kharma) ipole_h5_write_kharma_fluid_header(par.outf, &kh_h, &kh_ki);
   }
   ipgpu_destroy(ctx);
   double t = now() - t_start;

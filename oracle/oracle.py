@@ -455,22 +455,24 @@ def iharm_geom(h, rmax_geo=100., rmin_geo=1.):
     return g
 
 
-def iharm_units(M_unit, MBH=6.2e9, trat_small=1., trat_large=40., beta_crit=1., gam=13. / 9):
+def iharm_units(M_unit, MBH=6.2e9, trat_small=1., trat_large=40., beta_crit=1., gam=13. / 9,
+                game=1.333333333333333, gamp=1.666666666666667):
     u = Units()
     u.M_unit, u.MBH_solar = M_unit, MBH
     u.trat_small, u.trat_large, u.beta_crit = trat_small, trat_large, beta_crit
-    u.gam, u.game, u.gamp = gam, 1.333333333333333, 1.666666666666667
+    u.gam, u.game, u.gamp = gam, game, gamp
     return u
 
 
 def iharm_oracle(h, prims, M_unit, MBH=6.2e9, trat_small=1., trat_large=40., beta_crit=1.,
-                 sigma_cut=1., th_beg=0.0174, rmax_geo=100., L=None):
+                 sigma_cut=1., th_beg=0.0174, rmax_geo=100., L=None, game=1.333333333333333,
+                 gamp=1.666666666666667):
     """Oracle set up for an iharm dump: derives the ghosted fields as
     model/iharm/model.c does and returns (Oracle, fields11)."""
     g = iharm_geom(h, rmax_geo=rmax_geo)
     L = L or lib()
     L.or_set_geom(C.byref(g))
-    u = iharm_units(M_unit, MBH, trat_small, trat_large, beta_crit, h["gam"])
+    u = iharm_units(M_unit, MBH, trat_small, trat_large, beta_crit, h["gam"], game, gamp)
     nc = (g.N1 + 2) * (g.N2 + 2) * (g.N3 + 2)
     out = np.zeros(11 * nc)
     ru = np.zeros(2 * nc)

@@ -74,3 +74,27 @@ def test_host_sphere(tmp_path):
     ref = O.Oracle(g, m).render(O.default_params(nx=16, ny=16, thetacam=60., dx=100., dy=100., freqcgs=230e9))
     F_ref = ref["imageS"][..., 0].sum() * ref["cam"].scale
     assert abs(img["Ftot"] / F_ref - 1) < 1e-6
+
+
+@pytest.mark.parametrize("fmt,electrons", [("new", None), ("old", (1.4, 1.6))])
+def test_host_kharma(tmp_path, fmt, electrons):
+    """--model=kharma: a synthetic KHARMA file (shuffled meshblocks) through
+    the host's reader and the device derivation, against the oracle on the
+    primitives and indices the reader returns."""
+    h, prims = small_dump()
+    dump = str(tmp_path / "k.h5")
+    order = np.random.default_rng(3).permutation(32)
+    h5.write_kharma(dump, h, prims, (12, 8, 16), order=order, fmt=fmt, electrons=electrons)
+    rc, err = run(["--model=kharma", "--kharma_format=%s" % fmt, "--dump=%s" % dump, "--M_unit=3e26",
+                   "--freqcgs=230e9", "--nx=24", "--ny=24", "--thetacam=150",
+                   "--outfile=%s" % (tmp_path / "img.h5")], cwd=str(tmp_path))
+    assert rc == 0, err[-3000:]
+    img = h5.read_image(str(tmp_path / "img.h5"), 24, 24)
+    hr, pr, info = h5.read_kharma(dump, fmt)
+    orc, _ = O.iharm_oracle(hr, pr, M_unit=3e26, game=info["game"], gamp=info["gamp"])
+    ref = orc.render(O.default_params(nx=24, ny=24, thetacam=150., freqcgs=230e9))
+    F_ref = ref["imageS"][..., 0].sum() * ref["cam"].scale
+    assert abs(img["Ftot"] / F_ref - 1) < 1e-6
+    assert np.allclose(img["imageS"], ref["imageS"], rtol=1e-4, atol=1e-6 * np.abs(ref["imageS"][..., 0]).max())
+    assert h5.read_dataset(str(tmp_path / "img.h5"), "/fluid_header/geom/fmks/a") == h["a"]
+    assert h5.read_dataset(str(tmp_path / "img.h5"), "/fluid_header/nx1_mb") == 12
