@@ -956,7 +956,7 @@ struct RefineK {
  * Intensity (before the nu^3 scaling) weighted by the pixels each stands
  * for, summed in the coarse grid's order, times freqcgs^3 / (nx ny).  One
  * lane: the sum's order is the reference's. */
-__global__ void k_refine_iavg(RefineK R, int nbase, double f3, const double *__restrict__ out7,
+__global__ void k_refine_iavg(RefineK R, int nbase, int full, double f3, const double *__restrict__ out7,
                               double *__restrict__ iavg)
 {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
@@ -964,7 +964,10 @@ __global__ void k_refine_iavg(RefineK R, int nbase, double f3, const double *__r
   double tot = 0.;
   for (int r = 0; r < nbase; r++) {
     const size_t i = (size_t)(r / R.nmin_y) * sp, j = (size_t)(r % R.nmin_y) * sp;
-    const double I = out7[(size_t)r * 7];
+    /* the ray of base pixel r: r itself in the coarse pass's list, or the
+     * pixel's place in a render of the whole grid (full) */
+    const size_t ray = full ? i * ny + j : (size_t)r;
+    const double I = out7[ray * 7];
     double w;
     if (i % (nx - 1) != 0 && j % (ny - 1) != 0) w = I * sp * sp;
     else if ((i == 0 && j % (ny - 1) != 0) || (i % (nx - 1) != 0 && j == 0)) w = I * (sp / 2 + 1) * sp;
@@ -983,10 +986,13 @@ __global__ void k_refine_iavg(RefineK R, int nbase, double f3, const double *__r
  * above refine_cut) or is interpolated from the coarser grid at once
  * (save_pixelTransfer, lininterp2, lininterp4: src/main.c:912-1000).  The
  * pixels a level reads all lie on the coarser grid, so its pixels are
- * independent.  count[0]: traced, count[1]: interpolated. */
+ * independent.  count[0]: traced, count[1]: interpolated.  With list null
+ * every pixel has been traced beforehand, and those refined keep their
+ * values. */
 __global__ void k_refine_level(RefineK R, int ns, int unpol, const double *__restrict__ iavg,
                                double *__restrict__ image, double *__restrict__ imageS,
-                               double *__restrict__ taus, int *__restrict__ interp, int *__restrict__ list,
+                               double *__restrict__ taus, int *__restrict__ nstep_pix,
+                               int *__restrict__ flag_pix, int *__restrict__ interp, int *__restrict__ list,
                                unsigned int *__restrict__ count)
 {
   const int na = (R.nxi - 1) / ns + 1, nb = (R.nyi - 1) / ns + 1;
@@ -1031,18 +1037,23 @@ __global__ void k_refine_level(RefineK R, int ns, int unpol, const double *__res
   }
   const size_t p = at(i, j);
   if ((fabs(err_abs) > R.refine_abs && fabs(err_rel) > R.refine_rel) && fabs(I1) > R.refine_cut) {
-    list[atomicAdd(&count[0], 1u)] = (i << 16) | j;
+    const unsigned slot = atomicAdd(&count[0], 1u);
+    if (list) list[slot] = (i << 16) | j;   /* no list: the pixel was traced already */
     interp[p] = 0;
     return;
   }
   atomicAdd(&count[1], 1u);
   interp[p] = 1;
+  nstep_pix[p] = 0;
+  flag_pix[p] = 0;
   if (R.nearest) {
     const size_t q = src[0];
     image[p] = image[q];
     if (!unpol) {
       taus[p] = taus[q];
       for (int k = 0; k < 5; k++) imageS[p * 5 + k] = imageS[q * 5 + k];
+    } else {
+      taus[p] = 0.;   /* save_pixelTransfer leaves tau as allocated (0) with only_intensity */
     }
   } else if (nsrc == 2) {
     image[p] = .5 * (image[src[0]] + image[src[1]]);
@@ -1177,6 +1188,7 @@ struct ipgpu_ctx {
   int *d_interp = nullptr;
   size_t cap_interp = 0;
   int ref_levels = 0;
+  long long refine_spec_max = 1ll << 20;   /* speculative refinement up to this many internal pixels */
   double ref_iavg = 0.;
   long long ref_traced[IPGPU_MAX_LEVELS] = {0}, ref_interp[IPGPU_MAX_LEVELS] = {0};
   /* pixel trace output */
@@ -1242,6 +1254,7 @@ int ipgpu_create(ipgpu_ctx **out, int device)
   if (const char *e = getenv("IPGPU_SEG_MIN_STEPS")) ctx->seg_min_steps = atoi(e);
   if (const char *e = getenv("IPGPU_SEG_LEN")) ctx->seg_len = atoi(e) > 1 ? atoi(e) : 1;
   if (const char *e = getenv("IPGPU_TRACE_PHASES")) ctx->trace_phases = atoi(e);
+  if (const char *e = getenv("IPGPU_REFINE_SPECULATE_MAX")) ctx->refine_spec_max = atoll(e);
   if (hipMalloc((void **)&ctx->d_status, NSTATUS * sizeof(int)) != hipSuccess
       || hipMalloc((void **)&ctx->d_counts, 2 * sizeof(unsigned long long)) != hipSuccess) {
     delete ctx;
@@ -1948,6 +1961,9 @@ static int render_rays(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays)
 
 /* ------------------------------------------------------ adaptive refinement */
 
+static int adaptive_fetch(ipgpu_ctx *ctx, const ipgpu_image &P, int nyi, double *image, double *imageS,
+                          double *taus, int *nstep, int *flag, int *interp);
+
 int ipgpu_render_adaptive(ipgpu_ctx *ctx, const ipgpu_image *img, const ipgpu_refine *rf, double *image,
                           double *imageS, double *taus, int *nstep, int *flag, int *interp)
 {
@@ -2011,8 +2027,61 @@ int ipgpu_render_adaptive(ipgpu_ctx *ctx, const ipgpu_image *img, const ipgpu_re
   ctx->have_result = 0;       /* the device buffers hold the internal grid, not ipgpu_fetch's layout */
   ctx->ref_levels = levels;
   for (int l = 0; l < IPGPU_MAX_LEVELS; l++) ctx->ref_traced[l] = ctx->ref_interp[l] = 0;
-  /* coarse pass: the base grid in the reference's loop order (i outer) */
   const int nbase = nxmi * nymi;
+  RefineK R{nxi, nyi, sp, nymi, rf->refine_abs, rf->refine_rel, rf->refine_cut, rf->nearest_neighbor};
+  const double f3 = pow(P.freqcgs, 3);
+  unsigned int *cnt = reinterpret_cast<unsigned int *>(ctx->d_rcount);
+  /* Speculative: every pixel of the internal grid in one render, then each
+   * level's decisions and interpolations in place.  A pixel's result does
+   * not depend on which pixels are traced, so the image is the one tracing
+   * level by level gives, for one render's latency instead of one per
+   * level (the levels' lists are short and their rays long).  Above
+   * refine_spec_max pixels, or when a pixel the levels would not trace
+   * fails (maxnstep, NaN), the levels' lists are traced instead. */
+  if ((long long)npi <= ctx->refine_spec_max) {
+    const int npad = (int)((npi + WAVE - 1) / WAVE * WAVE);
+    {
+      std::vector<int> hl(npad, -1);
+      for (int i = 0; i < nxi; i++)
+        for (int j = 0; j < nyi; j++) hl[(size_t)i * nyi + j] = (i << 16) | j;
+      HIPCHK(ctx, hipMemcpyAsync(ctx->d_rlist, hl.data(), npad * sizeof(int), hipMemcpyHostToDevice, st));
+      HIPCHK(ctx, hipStreamSynchronize(st));
+    }
+    ctx->pix = ctx->d_rlist;
+    ctx->out_ny = nyi;
+    rc = render_rays(ctx, P, npad);
+    ctx->pix = nullptr;
+    if (rc == IPGPU_OK) {
+      ctx->ref_traced[0] = nbase;
+      k_refine_iavg<<<1, WAVE, 0, st>>>(R, nbase, 1, f3, ctx->d_out7, ctx->d_iavg);
+      HIPCHK(ctx, hipGetLastError());
+      HIPCHK(ctx, hipMemcpyAsync(&ctx->ref_iavg, ctx->d_iavg, sizeof(double), hipMemcpyDeviceToHost, st));
+      long long interp_total = 0;
+      for (int lev = 1; lev < levels; lev++) {
+        const int ns = sp >> lev;
+        const long long cand = (long long)((nxi - 1) / ns + 1) * ((nyi - 1) / ns + 1);
+        HIPCHK(ctx, hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned int), st));
+        k_refine_level<<<(unsigned)((cand + 255) / 256), 256, 0, st>>>(
+            R, ns, P.only_unpolarized, ctx->d_iavg, ctx->d_image, ctx->d_imageS, ctx->d_taus, ctx->d_nstep_pix,
+            ctx->d_flag_pix, ctx->d_interp, nullptr, cnt);
+        HIPCHK(ctx, hipGetLastError());
+        unsigned int hc[2];
+        HIPCHK(ctx, hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        interp_total += hc[1];
+        ctx->ref_traced[lev] = hc[0];
+        ctx->ref_interp[lev] = interp_total;
+      }
+      return adaptive_fetch(ctx, P, nyi, image, imageS, taus, nstep, flag, interp);
+    }
+    if (rc != IPGPU_ERR_MAXNSTEP && rc != IPGPU_ERR_NAN) return rc;
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_image, 0, npi * sizeof(double), st));
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_taus, 0, npi * sizeof(double), st));
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_imageS, 0, npi * 5 * sizeof(double), st));
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_nstep_pix, 0, npi * sizeof(int), st));
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_flag_pix, 0, npi * sizeof(int), st));
+  }
+  /* coarse pass: the base grid in the reference's loop order (i outer) */
   const int nbpad = (nbase + WAVE - 1) / WAVE * WAVE;
   {
     std::vector<int> hl(nbpad, -1);
@@ -2027,20 +2096,17 @@ int ipgpu_render_adaptive(ipgpu_ctx *ctx, const ipgpu_image *img, const ipgpu_re
   ctx->pix = nullptr;
   if (rc) return rc;
   ctx->ref_traced[0] = nbase;
-  RefineK R{nxi, nyi, sp, nymi, rf->refine_abs, rf->refine_rel, rf->refine_cut, rf->nearest_neighbor};
-  const double f3 = pow(P.freqcgs, 3);
-  k_refine_iavg<<<1, WAVE, 0, st>>>(R, nbase, f3, ctx->d_out7, ctx->d_iavg);
+  k_refine_iavg<<<1, WAVE, 0, st>>>(R, nbase, 0, f3, ctx->d_out7, ctx->d_iavg);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipMemcpyAsync(&ctx->ref_iavg, ctx->d_iavg, sizeof(double), hipMemcpyDeviceToHost, st));
-  unsigned int *cnt = reinterpret_cast<unsigned int *>(ctx->d_rcount);
   long long interp_total = 0;
   for (int lev = 1; lev < levels; lev++) {
     const int ns = sp >> lev;
     const long long cand = (long long)((nxi - 1) / ns + 1) * ((nyi - 1) / ns + 1);
     HIPCHK(ctx, hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned int), st));
-    k_refine_level<<<(unsigned)((cand + 255) / 256), 256, 0, st>>>(R, ns, P.only_unpolarized, ctx->d_iavg,
-                                                                   ctx->d_image, ctx->d_imageS, ctx->d_taus,
-                                                                   ctx->d_interp, ctx->d_rlist, cnt);
+    k_refine_level<<<(unsigned)((cand + 255) / 256), 256, 0, st>>>(
+        R, ns, P.only_unpolarized, ctx->d_iavg, ctx->d_image, ctx->d_imageS, ctx->d_taus, ctx->d_nstep_pix,
+        ctx->d_flag_pix, ctx->d_interp, ctx->d_rlist, cnt);
     HIPCHK(ctx, hipGetLastError());
     unsigned int hc[2];
     HIPCHK(ctx, hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st));
@@ -2057,25 +2123,43 @@ int ipgpu_render_adaptive(ipgpu_ctx *ctx, const ipgpu_image *img, const ipgpu_re
     ctx->pix = nullptr;
     if (rc) return rc;
   }
-  /* the nx x ny part of the internal grid */
-  HIPCHK(ctx, hipMemcpy2DAsync(image, P.ny * sizeof(double), ctx->d_image, nyi * sizeof(double),
-                               P.ny * sizeof(double), P.nx, hipMemcpyDeviceToHost, st));
-  if (taus)
-    HIPCHK(ctx, hipMemcpy2DAsync(taus, P.ny * sizeof(double), ctx->d_taus, nyi * sizeof(double),
-                                 P.ny * sizeof(double), P.nx, hipMemcpyDeviceToHost, st));
-  if (imageS && !P.only_unpolarized)
-    HIPCHK(ctx, hipMemcpy2DAsync(imageS, P.ny * 5 * sizeof(double), ctx->d_imageS, nyi * 5 * sizeof(double),
-                                 P.ny * 5 * sizeof(double), P.nx, hipMemcpyDeviceToHost, st));
-  if (nstep)
-    HIPCHK(ctx, hipMemcpy2DAsync(nstep, P.ny * sizeof(int), ctx->d_nstep_pix, nyi * sizeof(int),
-                                 P.ny * sizeof(int), P.nx, hipMemcpyDeviceToHost, st));
-  if (flag)
-    HIPCHK(ctx, hipMemcpy2DAsync(flag, P.ny * sizeof(int), ctx->d_flag_pix, nyi * sizeof(int), P.ny * sizeof(int),
-                                 P.nx, hipMemcpyDeviceToHost, st));
-  if (interp)
-    HIPCHK(ctx, hipMemcpy2DAsync(interp, P.ny * sizeof(int), ctx->d_interp, nyi * sizeof(int), P.ny * sizeof(int),
-                                 P.nx, hipMemcpyDeviceToHost, st));
-  HIPCHK(ctx, hipStreamSynchronize(st));
+  return adaptive_fetch(ctx, P, nyi, image, imageS, taus, nstep, flag, interp);
+}
+
+/* the nx x ny part of the internal grid (rows of nyi) */
+static int adaptive_fetch(ipgpu_ctx *ctx, const ipgpu_image &P, int nyi, double *image, double *imageS,
+                          double *taus, int *nstep, int *flag, int *interp)
+{
+  /* each array whole into host memory in one copy, then its rows: strided
+   * copies into pageable memory go row by row */
+  hipStream_t st = ctx->stream;
+  const size_t nxi = (size_t)P.nx + ((ctx->ref_levels > 1 && P.nx % 2 == 0) ? 1 : 0);
+  const size_t npi = nxi * nyi;
+  std::vector<double> hd(npi * 5);
+  std::vector<int> hi(npi);
+  auto rows = [&](const void *src, void *dst, size_t elem, size_t per) {
+    for (size_t i = 0; i < (size_t)P.nx; i++)
+      memcpy((char *)dst + i * P.ny * per * elem, (const char *)src + i * nyi * per * elem, P.ny * per * elem);
+  };
+  auto getd = [&](const double *dsrc, double *dst, size_t per) -> int {
+    HIPCHK(ctx, hipMemcpyAsync(hd.data(), dsrc, npi * per * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    rows(hd.data(), dst, sizeof(double), per);
+    return 0;
+  };
+  auto geti = [&](const int *dsrc, int *dst) -> int {
+    HIPCHK(ctx, hipMemcpyAsync(hi.data(), dsrc, npi * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    rows(hi.data(), dst, sizeof(int), 1);
+    return 0;
+  };
+  int rc;
+  if ((rc = getd(ctx->d_image, image, 1))) return rc;
+  if (taus && (rc = getd(ctx->d_taus, taus, 1))) return rc;
+  if (imageS && !P.only_unpolarized && (rc = getd(ctx->d_imageS, imageS, 5))) return rc;
+  if (nstep && (rc = geti(ctx->d_nstep_pix, nstep))) return rc;
+  if (flag && (rc = geti(ctx->d_flag_pix, flag))) return rc;
+  if (interp && (rc = geti(ctx->d_interp, interp))) return rc;
   return IPGPU_OK;
 }
 

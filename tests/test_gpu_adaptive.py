@@ -116,6 +116,29 @@ def test_working_tolerances_vs_oracle(setup, cam):
             assert e.max() < 1e-4 and np.mean(e > 1e-6) < 0.01, k
 
 
+@pytest.mark.parametrize("nearest,unpol", [(0, 0), (1, 1)])
+def test_speculative_equals_levels(setup, nearest, unpol, monkeypatch):
+    """The whole grid traced at once and then refined in place gives the
+    image, flags and counts of tracing each level's list."""
+    h, orc, ctx = setup
+    p, im, _ = image(h, orc, only_unpolarized=unpol)
+    rf = G.make_refine(8, refine_abs=1e-6, refine_rel=1e-2, nearest_neighbor=nearest)
+    say("speculative vs levels, nearest=%d unpol=%d" % (nearest, unpol))
+    a = ctx.render_adaptive(im, rf)
+    monkeypatch.setenv("IPGPU_REFINE_SPECULATE_MAX", "0")
+    c2 = G.Context(0)
+    try:
+        c2.set_geometry(G.iharm_geom(h))
+        c2.set_model(G.iharm_model())
+        c2.upload_fields(O.iharm_oracle(h, small_dump()[1], M_unit=M_UNIT)[1][:10])
+        b = c2.render_adaptive(im, rf)
+    finally:
+        c2.close()
+    for k in ("image", "imageS", "taus", "nstep", "flag", "interp"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["traced"] == b["traced"] and a["interpolated"] == b["interpolated"] and a["Iavg"] == b["Iavg"]
+
+
 def test_bad_sizes(setup):
     h, orc, ctx = setup
     p = O.default_params(nx=30, ny=30, thetacam=163., freqcgs=230e9)
