@@ -369,18 +369,19 @@ IPG_HD void transport_matrix(const Metric &m, const double K[4], double M[4][4])
 /* src/model_geodesics.c:267-293 */
 IPG_HD double stepsize(const ipgpu_geom &G, const double X[4], const double K[4], double eps)
 {
+  /* 1 / (|dlx| + SS) with dlx = eps f / (|K| + SS) >= 0 is written
+   * (|K| + SS) / (eps f + SS (|K| + SS)): one reciprocal per direction
+   * instead of two (the same value up to roundoff) */
   const double SS = SMALL * SMALL;
   double deh = fmin(fabs(X[1] - G.cstartx[1]), 0.1);
-  double dlx1 = eps * (10 * deh) * rcp(fabs(K[1]) + SS);
   double cut = 0.02;
   double ilx2 = rcp(G.cstopx[2] - G.cstartx[2]);
   double dpole = fmin(fabs(X[2] * ilx2), fabs((G.cstopx[2] - X[2]) * ilx2));
   double d2fac = (dpole < cut) ? dpole * (1. / 3.) : fmin(cut / 3 + (dpole - cut) * 10., 1);
-  double dlx2 = eps * d2fac * rcp(fabs(K[2]) + SS);
-  double dlx3 = eps * rcp(fabs(K[3]) + SS);
-  double idlx1 = rcp(fabs(dlx1) + SS);
-  double idlx2 = rcp(fabs(dlx2) + SS);
-  double idlx3 = rcp(fabs(dlx3) + SS);
+  const double a1 = fabs(K[1]) + SS, a2 = fabs(K[2]) + SS, a3 = fabs(K[3]) + SS;
+  double idlx1 = a1 * rcp(eps * (10 * deh) + SS * a1);
+  double idlx2 = a2 * rcp(eps * d2fac + SS * a2);
+  double idlx3 = a3 * rcp(eps + SS * a3);
   return rcp(idlx1 + idlx2 + idlx3);
 }
 
