@@ -229,14 +229,32 @@ constexpr int AX = 16;
 constexpr int AX_NU = 0, AX_STH = 1, AX_CTH = 2, AX_EDGE = 3, AX_NE = 4, AX_TH = 5, AX_B = 6, AX_CELL = 7,
               AX_D1 = 8, AX_D2 = 9, AX_D3 = 10, AX_SMOOTH = 11, AX_DRH = 12, AX_DRL = 13, AX_ZERO = 14;
 
+/* Position of field f in a packet stored with field stride PS.  PS = 1:
+ * one packet contiguous.  PS = 64: the packets of the 64 rays of a wave
+ * interleaved by field pairs, fields 2m and 2m + 1 of lane l's packet at
+ * 128 m + 2 l and + 1 (the caller offsets lane l's packet by 2 l), so that
+ * a wave's load of a pair is one 16-byte access per lane, 1 KiB contiguous,
+ * and a packet 38 loads: few enough that the next step's packet stays in
+ * flight under the hardware's count of outstanding loads. */
+template <int PS>
+IPG_HD constexpr size_t pk_at(int f)
+{
+  return PS == 1 ? (size_t)f : (size_t)(f >> 1) * (2 * PS) + (f & 1);
+}
+
+/* Lane l's offset within a PS-interleaved packet row. */
+template <int PS>
+IPG_HD constexpr size_t pk_lane(int l)
+{
+  return PS == 1 ? 0 : (size_t)2 * l;
+}
+
 /* Packet fields are written through PkW: field f of the packet at p is
- * p[f * PS] (PS = 1: one packet contiguous; PS = 64: the packets of the 64
- * rays of a wave interleaved field by field, so that a wave's load or store
- * of one field is 512 contiguous bytes). */
+ * p[pk_at<PS>(f)]. */
 template <int PS>
 struct PkW {
   double *p;
-  IPG_HD double &operator[](int f) const { return p[(size_t)f * PS]; }
+  IPG_HD double &operator[](int f) const { return p[pk_at<PS>(f)]; }
 };
 
 template <bool F32 = false, int PS = 1>
@@ -381,12 +399,21 @@ struct Pkt {
   double v[PK];
 };
 
-/* A packet stored with field stride PS (see PkW). */
+/* A packet stored with field stride PS (see pk_at), one field pair per
+ * load. */
 template <int PS>
 IPG_HD_INLINE void load_pkt_s(const double *p, Pkt &k)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
-  for (int i = 0; i < PK; i++) k.v[i] = p[(size_t)i * PS];
+  for (int i = 0; i < PK / 2; i++) {
+    const double2 t = *reinterpret_cast<const double2 *>(p + pk_at<PS>(2 * i));
+    k.v[2 * i] = t.x;
+    k.v[2 * i + 1] = t.y;
+  }
+#else
+  for (int i = 0; i < PK; i++) k.v[i] = p[pk_at<PS>(i)];
+#endif
 }
 
 IPG_HD void load_pkt(const double *p, int n, Pkt &k)
@@ -528,19 +555,22 @@ IPG_HD_INLINE int chain_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, c
       if constexpr (PS == 1) load_pkt(pkt + (size_t)q * PK, PK, k);
       else load_pkt_s<PS>(pkt + (size_t)q * PK * PS, k);
     };
+    /* the loads are unconditional (past the last step they re-read the
+     * last packet), so that the compiler's count of loads in flight is the
+     * same on every path and the current step waits for its own packet only */
     if (ns > 0) load(0, a);
     for (int q = 0; q < ns; q += 2) {
-      if (q + 1 < ns) load(q + 1, b);
+      load(q + 1 < ns ? q + 1 : ns - 1, b);
       chain_step(st, a.v, true, P.stokes_floors);
       if (q + 1 >= ns) break;
-      if (q + 2 < ns) load(q + 2, a);
+      load(q + 2 < ns ? q + 2 : ns - 1, a);
       chain_step(st, b.v, true, P.stokes_floors);
     }
   } else {
     for (int q = 0; q < ns; q++) {
       const double *p = pkt + (size_t)q * stride * PS;
       double u[PK_UNPOL];
-      for (int f = 0; f < PK_UNPOL; f++) u[f] = p[(size_t)f * PS];
+      for (int f = 0; f < PK_UNPOL; f++) u[f] = p[pk_at<PS>(f)];
       chain_step(st, u, false, P.stokes_floors);
     }
   }
@@ -603,13 +633,13 @@ IPG_HD int segment_column(const double *pkt, int q0, int q1, int b, Herm &N)
   int flag = 0;
   for (int q = q0; q < q1; q++) {
     const double *pk = pkt + (size_t)q * PK * PS;
-    auto F = [pk](int f) { return pk[(size_t)f * PS]; };
+    auto F = [pk](int f) { return pk[pk_at<PS>(f)]; };
     if (q > 0) {
       const double *pp = pkt + (size_t)(q - 1) * PK * PS;
       double Mi[4][4], Mh[4][4];
       for (int i = 0; i < 4; i++)
         for (int k = 0; k < 4; k++) {
-          Mi[i][k] = pp[(size_t)(PK_MF + 4 * i + k) * PS];
+          Mi[i][k] = pp[pk_at<PS>(PK_MF + 4 * i + k)];
           Mh[i][k] = F(PK_MH + 4 * i + k);
         }
       Herm Nh;
@@ -668,7 +698,7 @@ IPG_HD void segment_intensity(const double *pkt, int q0, int q1, double *o)
   for (int q = 0; q < NPAR; q++) o[q] = 0.;
   for (int q = q0; q < q1; q++) {
     const double *pk = pkt + (size_t)q * PK * PS;
-    auto F = [pk](int f) { return pk[(size_t)f * PS]; };
+    auto F = [pk](int f) { return pk[pk_at<PS>(f)]; };
     const int meta = (int)F(PK_META);
     if (meta & 1) {
       const double ju = F(PK_JU), ku = F(PK_KU);

@@ -823,7 +823,7 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_list(ipgpu_geom G, ipgpu_mo
   const int ns = nstep_in[ray];
   if (q >= ns || ns >= P.maxnstep - 1) return;
   prep_step<F32, ILV>(G, Mo, P, ns - q, traj + (size_t)ray * cap * REC, nullptr, dump,
-                      pkt + (size_t)(W[c] + q) * stride * ILV + l, stride == PK, unit_scale());
+                      pkt + (size_t)(W[c] + q) * stride * ILV + pk_lane<ILV>(l), stride == PK, unit_scale());
 }
 
 /* The same packets in the compact layout (ray after ray, packets of one ray
@@ -906,7 +906,7 @@ __global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model M
     flags[ray] = chain_ray_segmented(G, P, ns, seg_len, segmap + (size_t)sb * SEGCOL * NPAR, segflag + sb, js, ks,
                                      res, nrad);
   else
-    flags[ray] = chain_ray<ILV>(G, P, ns, pkt + (size_t)W[k / ILV] * stride * ILV + (k % ILV), stride, js, ks,
+    flags[ray] = chain_ray<ILV>(G, P, ns, pkt + (size_t)W[k / ILV] * stride * ILV + pk_lane<ILV>(k % ILV), stride, js, ks,
                                 res, nrad);
   for (int q = 0; q < 7; q++) o[q] = res[q];
   atomicAdd(&counts[1], (unsigned long long)nrad);
@@ -1748,7 +1748,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
     /* first packet of each element, in doubles */
     std::vector<long long> &eoff = ctx->h_eoff;
     eoff.resize(n);
-    for (int q = 0; q < n; q++) eoff[q] = compact ? W[q] * PK : W[q / ILV] * stride * ILV + q % ILV;
+    for (int q = 0; q < n; q++) eoff[q] = compact ? W[q] * PK : W[q / ILV] * stride * ILV + (long long)pk_lane<ILV>(q % ILV);
     build_segments(ctx->segset[k], h_list + start, n, eoff.data(), nsv, seg_min, ctx->seg_len, P.maxnstep,
                    h_base + start);
     hipStream_t st = (k < K) ? ctx->streamF[k] : sA;
