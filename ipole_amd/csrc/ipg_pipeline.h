@@ -604,6 +604,18 @@ constexpr int SEGCOL = NPAR + 2;
 constexpr int SI_A = 0, SI_B = 1, SI_TAU = 2, SI_TAUF = 3, SI_NRAD = 4, SI_M2 = 5, SI_J1 = 6, SI_K1 = 7,
               SI_DL1 = 8, SI_JL = 9, SI_KL = 10;
 
+/* Segment sg of a chain of ns steps: packets [q0, q1).  The boundaries sit
+ * at trace steps that are multiples of seg_len counted from the camera (the
+ * far-end segment is the short one), so that the segments of a ray's first
+ * steps do not depend on its length. */
+IPG_HD void segment_bounds(int ns, int seg_len, int sg, int &q0, int &q1)
+{
+  const int nseg = (ns + seg_len - 1) / seg_len;
+  const int f = ns - (nseg - 1) * seg_len;
+  q0 = sg == 0 ? 0 : f + (sg - 1) * seg_len;
+  q1 = f + sg * seg_len;
+}
+
 IPG_HD double &herm_par(Herm &N, int b)
 {
   return b < 4 ? N.d[b] : (b < 10 ? N.re[b - 4] : N.im[b - 10]);
@@ -614,9 +626,44 @@ IPG_HD double herm_par(const Herm &N, int b)
   return b < 4 ? N.d[b] : (b < 10 ? N.re[b - 4] : N.im[b - 10]);
 }
 
+/* One step of a segment column on N from the step's packet v (fields by
+ * index): transport with the previous step's end-point matrix Mi (when
+ * transport_on), then the plasma-frame Stokes step with its source term
+ * when src. */
+IPG_HD_INLINE void segment_step(Herm &N, const double *v, const double Mi[4][4], bool transport_on, bool src,
+                                int &flag)
+{
+  if (transport_on) {
+    double Mh[4][4];
+    for (int i = 0; i < 4; i++)
+      for (int k = 0; k < 4; k++) Mh[i][k] = v[PK_MH + 4 * i + k];
+    Herm Nh;
+    transport(N, N, Mi, v[PK_HDL], Nh);
+    transport(N, Nh, Mh, v[PK_FDL], N);
+  }
+  const int meta = (int)v[PK_META];
+  if (meta & 1) {
+    double S0[4], dS[4], e1[4], e2[4], E1[4], E2[4];
+    for (int i = 0; i < 4; i++) {
+      e1[i] = v[PK_e1 + i];
+      e2[i] = v[PK_e2 + i];
+      E1[i] = v[PK_E1 + i];
+      E2[i] = v[PK_E2 + i];
+    }
+    herm_to_stokes(N, e1, e2, S0);
+    for (int i = 0; i < 4; i++)
+      dS[i] = (src ? v[PK_CV + i] : 0.) + v[PK_T + 4 * i] * S0[0] + v[PK_T + 4 * i + 1] * S0[1]
+          + v[PK_T + 4 * i + 2] * S0[2] + v[PK_T + 4 * i + 3] * S0[3] - S0[i];
+    herm_add_stokes(N, E1, E2, dS);
+    flag |= meta >> 2;
+  }
+}
+
 /* Column b of the map of steps [q0, q1): applied to basis state b (b < 16,
  * no sources) or to N = 0 with sources (b = 16).  Returns the steps'
- * tetrad oddflag bits. */
+ * tetrad oddflag bits.  Two packet buffers used in turn, as in chain_ray:
+ * the next step's packet is in flight while the current one computes (past
+ * the segment's last step the loads re-read its packet). */
 template <int PS = 1>
 IPG_HD int segment_column(const double *pkt, int q0, int q1, int b, Herm &N)
 {
@@ -631,36 +678,29 @@ IPG_HD int segment_column(const double *pkt, int q0, int q1, int b, Herm &N)
   }
   const bool src = (b == NPAR);
   int flag = 0;
-  for (int q = q0; q < q1; q++) {
-    const double *pk = pkt + (size_t)q * PK * PS;
-    auto F = [pk](int f) { return pk[pk_at<PS>(f)]; };
-    if (q > 0) {
-      const double *pp = pkt + (size_t)(q - 1) * PK * PS;
-      double Mi[4][4], Mh[4][4];
-      for (int i = 0; i < 4; i++)
-        for (int k = 0; k < 4; k++) {
-          Mi[i][k] = pp[pk_at<PS>(PK_MF + 4 * i + k)];
-          Mh[i][k] = F(PK_MH + 4 * i + k);
-        }
-      Herm Nh;
-      transport(N, N, Mi, F(PK_HDL), Nh);
-      transport(N, Nh, Mh, F(PK_FDL), N);
-    }
-    const int meta = (int)F(PK_META);
-    if (meta & 1) {
-      double S0[4], dS[4], e1[4], e2[4], E1[4], E2[4];
-      for (int i = 0; i < 4; i++) {
-        e1[i] = F(PK_e1 + i);
-        e2[i] = F(PK_e2 + i);
-        E1[i] = F(PK_E1 + i);
-        E2[i] = F(PK_E2 + i);
-      }
-      herm_to_stokes(N, e1, e2, S0);
-      for (int i = 0; i < 4; i++)
-        dS[i] = (src ? F(PK_CV + i) : 0.) + F(PK_T + 4 * i) * S0[0] + F(PK_T + 4 * i + 1) * S0[1]
-            + F(PK_T + 4 * i + 2) * S0[2] + F(PK_T + 4 * i + 3) * S0[3] - S0[i];
-      herm_add_stokes(N, E1, E2, dS);
-      flag |= meta >> 2;
+  /* end-point matrix of the step before the current one */
+  double Mi[4][4];
+  for (int i = 0; i < 4; i++)
+    for (int k = 0; k < 4; k++)
+      Mi[i][k] = q0 > 0 ? pkt[(size_t)(q0 - 1) * PK * PS + pk_at<PS>(PK_MF + 4 * i + k)] : 0.;
+  auto load = [&](int q, Pkt &k) {
+    if constexpr (PS == 1) load_pkt(pkt + (size_t)q * PK, PK, k);
+    else load_pkt_s<PS>(pkt + (size_t)q * PK * PS, k);
+  };
+  auto step = [&](int q, const Pkt &k) {
+    segment_step(N, k.v, Mi, q > 0, src, flag);
+    for (int i = 0; i < 4; i++)
+      for (int c = 0; c < 4; c++) Mi[i][c] = k.v[PK_MF + 4 * i + c];
+  };
+  if (q1 > q0) {
+    Pkt a, c;
+    load(q0, a);
+    for (int q = q0; q < q1; q += 2) {
+      load(q + 1 < q1 ? q + 1 : q1 - 1, c);
+      step(q, a);
+      if (q + 1 >= q1) break;
+      load(q + 2 < q1 ? q + 2 : q1 - 1, a);
+      step(q + 1, c);
     }
   }
   return flag;
@@ -696,21 +736,32 @@ IPG_HD void segment_intensity(const double *pkt, int q0, int q1, double *o)
   double A = 1., B = 0., tau = 0., tauF = 0., js = 0., ks = 0.;
   int nrad = 0;
   for (int q = 0; q < NPAR; q++) o[q] = 0.;
-  for (int q = q0; q < q1; q++) {
+  /* the five fields a step reads (meta, dl, ju, ku, dtauF), the next
+   * step's in flight while the current one computes */
+  auto load = [&](int q, double u[5]) {
     const double *pk = pkt + (size_t)q * PK * PS;
-    auto F = [pk](int f) { return pk[pk_at<PS>(f)]; };
-    const int meta = (int)F(PK_META);
+    u[0] = pk[pk_at<PS>(PK_META)];
+    u[1] = pk[pk_at<PS>(PK_DL)];
+    u[2] = pk[pk_at<PS>(PK_JU)];
+    u[3] = pk[pk_at<PS>(PK_KU)];
+    u[4] = pk[pk_at<PS>(PK_DTF)];
+  };
+  double cur[5], nxt[5];
+  if (q1 > q0) load(q0, cur);
+  for (int q = q0; q < q1; q++) {
+    load(q + 1 < q1 ? q + 1 : q1 - 1, nxt);
+    const int meta = (int)cur[0];
     if (meta & 1) {
-      const double ju = F(PK_JU), ku = F(PK_KU);
-      tauF += F(PK_DTF);
+      const double ju = cur[2], ku = cur[3];
+      tauF += cur[4];
       if (nrad == 0) {
         o[SI_M2] = (meta & 2) ? 1. : 0.;
         o[SI_J1] = ju;
         o[SI_K1] = ku;
-        o[SI_DL1] = F(PK_DL);
+        o[SI_DL1] = cur[1];
       } else {
         double a, b;
-        approximate_solve_affine((meta & 2) ? 0. : js, ks, (meta & 2) ? 0. : ju, ku, F(PK_DL), tau, a, b);
+        approximate_solve_affine((meta & 2) ? 0. : js, ks, (meta & 2) ? 0. : ju, ku, cur[1], tau, a, b);
         A = a * A;
         B = a * B + b;
       }
@@ -718,6 +769,7 @@ IPG_HD void segment_intensity(const double *pkt, int q0, int q1, double *o)
       ks = ku;
       nrad++;
     }
+    for (int f = 0; f < 5; f++) cur[f] = nxt[f];
   }
   o[SI_A] = A;
   o[SI_B] = B;
@@ -728,53 +780,73 @@ IPG_HD void segment_intensity(const double *pkt, int q0, int q1, double *o)
   o[SI_KL] = ks;
 }
 
+/* State of the composition of a long ray's segment maps. */
+struct SegChain {
+  Herm N;
+  double I, tau, tauF, js, ks;
+  int flag, nrad;
+};
+
+/* Applies one segment's map A (SEGCOL x NPAR doubles, column-major, the
+ * last column the unpolarized map) with oddflag bits fl. */
+IPG_HD_INLINE void compose_segment(SegChain &c, const double *A, int fl)
+{
+  double x[NPAR], acc[NPAR];
+#pragma unroll
+  for (int b = 0; b < NPAR; b++) x[b] = herm_par(c.N, b);
+#pragma unroll
+  for (int p = 0; p < NPAR; p++) acc[p] = A[NPAR * NPAR + p];
+  /* column by column (acc[p] sums in the order b = 0..15), one column's
+   * loads at a time so that the map is never held in registers whole */
+#pragma unroll
+  for (int b = 0; b < NPAR; b++) {
+#pragma unroll
+    for (int p = 0; p < NPAR; p++) acc[p] += A[b * NPAR + p] * x[b];
+    IPG_SCHED_BARRIER();
+  }
+#pragma unroll
+  for (int p = 0; p < NPAR; p++) herm_par(c.N, p) = acc[p];
+  const double *u = A + SEG_I * NPAR;
+  const int nr = (int)u[SI_NRAD];
+  if (nr > 0) {
+    /* the segment's first radiating step with the carried js, ks, as the
+     * serial recurrence does it, then the rest of the segment */
+    const bool m2 = u[SI_M2] != 0.;
+    c.I = approximate_solve(c.I, m2 ? 0. : c.js, c.ks, m2 ? 0. : u[SI_J1], u[SI_K1], u[SI_DL1], c.tau);
+    c.I = u[SI_A] * c.I + u[SI_B];
+    c.tau += u[SI_TAU];
+    c.js = u[SI_JL];
+    c.ks = u[SI_KL];
+  }
+  c.tauF += u[SI_TAUF];
+  c.nrad += nr;
+  c.flag |= fl;
+}
+
 /* A long ray from its segment maps (segmap: SEGCOL x NPAR doubles per
- * segment, column-major, the last column the unpolarized map; segflag: the
- * segments' oddflag bits), then project_N. */
+ * segment; segflag: the segments' oddflag bits), then project_N.  With
+ * npre > 0 the last npre segments (the ray's first steps from the camera)
+ * are read from pre_map, pre_flag instead. */
 IPG_HD int chain_ray_segmented(const ipgpu_geom &G, const ipgpu_image &P, int ns, int seg_len,
                                const double *segmap, const int *segflag, double js, double ks, double out7[7],
-                               int &nrad)
+                               int &nrad, const double *pre_map = nullptr, const int *pre_flag = nullptr,
+                               int npre = 0)
 {
-  Herm N;
-  for (int q = 0; q < 4; q++) N.d[q] = 0.;
-  for (int q = 0; q < 6; q++) N.re[q] = N.im[q] = 0.;
-  int flag = 0;
-  double I = 0., tau = 0., tauF = 0.;
-  nrad = 0;
+  SegChain c;
+  for (int q = 0; q < 4; q++) c.N.d[q] = 0.;
+  for (int q = 0; q < 6; q++) c.N.re[q] = c.N.im[q] = 0.;
+  c.I = c.tau = c.tauF = 0.;
+  c.js = js;
+  c.ks = ks;
+  c.flag = c.nrad = 0;
   const int nseg = (ns + seg_len - 1) / seg_len;
-  for (int sgi = 0; sgi < nseg; sgi++) {
-    const double *A = segmap + (size_t)sgi * SEGCOL * NPAR;
-    double x[NPAR], acc[NPAR];
-#pragma unroll
-    for (int b = 0; b < NPAR; b++) x[b] = herm_par(N, b);
-#pragma unroll
-    for (int p = 0; p < NPAR; p++) acc[p] = A[NPAR * NPAR + p];
-    /* column by column (acc[p] sums in the order b = 0..15), one column's
-     * loads at a time so that the map is never held in registers whole */
-#pragma unroll
-    for (int b = 0; b < NPAR; b++) {
-#pragma unroll
-      for (int p = 0; p < NPAR; p++) acc[p] += A[b * NPAR + p] * x[b];
-      IPG_SCHED_BARRIER();
-    }
-#pragma unroll
-    for (int p = 0; p < NPAR; p++) herm_par(N, p) = acc[p];
-    const double *u = A + SEG_I * NPAR;
-    const int nr = (int)u[SI_NRAD];
-    if (nr > 0) {
-      /* the segment's first radiating step with the carried js, ks, as the
-       * serial recurrence does it, then the rest of the segment */
-      const bool m2 = u[SI_M2] != 0.;
-      I = approximate_solve(I, m2 ? 0. : js, ks, m2 ? 0. : u[SI_J1], u[SI_K1], u[SI_DL1], tau);
-      I = u[SI_A] * I + u[SI_B];
-      tau += u[SI_TAU];
-      js = u[SI_JL];
-      ks = u[SI_KL];
-    }
-    tauF += u[SI_TAUF];
-    nrad += nr;
-    flag |= segflag[sgi];
-  }
+  const int ntail = nseg - npre;
+  for (int sgi = 0; sgi < ntail; sgi++) compose_segment(c, segmap + (size_t)sgi * SEGCOL * NPAR, segflag[sgi]);
+  for (int sgi = 0; sgi < npre; sgi++) compose_segment(c, pre_map + (size_t)sgi * SEGCOL * NPAR, pre_flag[sgi]);
+  Herm &N = c.N;
+  int flag = c.flag;
+  const double I = c.I, tau = c.tau, tauF = c.tauF;
+  nrad = c.nrad;
   if (tauF > 1.e100 || tauF < -1.e100 || isnan(tauF)) flag |= 2;
   if (isnan(N.d[0])) flag |= 4;
   double S[4];

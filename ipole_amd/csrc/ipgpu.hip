@@ -859,6 +859,23 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_compact(ipgpu_geom G, ipgpu
                  unit_scale());
 }
 
+/* Step packets of the first steps of the rays still being traced: n rays
+ * (plist), np1 packets each, packet q of ray j (step np1 - q) at
+ * pkt + (j np1 + q) PK.  Formed from the records the bounded phases stored,
+ * while the unbounded phase extends the same trajectories past them. */
+template <bool F32>
+__global__ __launch_bounds__(PREP_BLOCK) void k_prep_prefix(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
+                                                            const int *__restrict__ plist, int n, int np1, int cap,
+                                                            const double *__restrict__ traj,
+                                                            const double *__restrict__ dump, double *__restrict__ pkt)
+{
+  const long long g = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
+  if (g >= (long long)n * np1) return;
+  const int j = (int)(g / np1), q = (int)(g % np1);
+  prep_step<F32>(G, Mo, P, np1 - q, traj + (size_t)plist[j] * cap * REC, nullptr, dump, pkt + (size_t)g * PK, true,
+                 unit_scale());
+}
+
 /* The chains of the rays of a list, one lane per ray: SEG = false the rays
  * with seg_base[k] < 0 (recurrence over the packets), SEG = true those with
  * seg_base[k] >= 0 (composition of the segment maps). */
@@ -873,23 +890,28 @@ __global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model M
                                                      const double *__restrict__ segmap,
                                                      const int *__restrict__ segflag, double *__restrict__ out7,
                                                      int *__restrict__ flags, int *__restrict__ status,
-                                                     unsigned long long *__restrict__ counts)
+                                                     unsigned long long *__restrict__ counts,
+                                                     const int *__restrict__ pre_idx,
+                                                     const double *__restrict__ pre_map,
+                                                     const int *__restrict__ pre_flag, int npre)
 {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const int sb = seg_base ? seg_base[k] : -1;
-  if ((sb >= 0) != SEG) return;
   const int ray = list[k];
   int pi, pj;
   if (!ray_pixel(pix, ray, P.nx, P.ny, pi, pj)) return;
   const int ns = nstep_in[ray];
   double *o = out7 + (size_t)ray * 7;
+  /* a ray past maxnstep is reported by each instance that sees it (a list
+   * whose chains are all composed runs the SEG instance alone) */
   if (ns >= P.maxnstep - 1) {
     atomicMin(&status[1], pixel_code(pi, pj));
     for (int q = 0; q < 7; q++) o[q] = 0.;
     flags[ray] = 0;
     return;
   }
+  const int sb = seg_base ? seg_base[k] : -1;
+  if ((sb >= 0) != SEG) return;
   const double *rec = traj + (size_t)ray * cap * REC;
   double js, ks;
   if constexpr (SEG) {
@@ -904,7 +926,8 @@ __global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model M
   int nrad;
   if constexpr (SEG)
     flags[ray] = chain_ray_segmented(G, P, ns, seg_len, segmap + (size_t)sb * SEGCOL * NPAR, segflag + sb, js, ks,
-                                     res, nrad);
+                                     res, nrad, pre_idx ? pre_map + (size_t)pre_idx[k] * npre * SEGCOL * NPAR : nullptr,
+                                     pre_idx ? pre_flag + (size_t)pre_idx[k] * npre : nullptr, pre_idx ? npre : 0);
   else
     flags[ray] = chain_ray<ILV>(G, P, ns, pkt + (size_t)W[k / ILV] * stride * ILV + pk_lane<ILV>(k % ILV), stride, js, ks,
                                 res, nrad);
@@ -915,19 +938,18 @@ __global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model M
 /* One lane per (segment, column): the composed maps of the long rays'
  * segments (columns 0..15 the N map of the basis states, 16 its source
  * term, 17 the unpolarized map).  Segments are given by the position of
- * their ray's first packet (in doubles), their first step and the ray's
- * step count; PS is the packets' field stride. */
+ * their ray's first packet (in doubles) and their packets [q0, q1); PS is
+ * the packets' field stride. */
 template <int PS>
-__global__ __launch_bounds__(WAVE) void k_seg(int nseg, int seg_len, const long long *__restrict__ seg_off,
-                                              const int *__restrict__ seg_q0, const int *__restrict__ seg_ns,
+__global__ __launch_bounds__(WAVE) void k_seg(int nseg, const long long *__restrict__ seg_off,
+                                              const int *__restrict__ seg_q0, const int *__restrict__ seg_q1,
                                               const double *__restrict__ pkt, double *__restrict__ segmap,
                                               int *__restrict__ segflag)
 {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nseg * SEGCOL) return;
   const int sgi = (int)(t / SEGCOL), b = (int)(t % SEGCOL);
-  const int q0 = seg_q0[sgi], ns = seg_ns[sgi];
-  const int q1 = q0 + seg_len < ns ? q0 + seg_len : ns;
+  const int q0 = seg_q0[sgi], q1 = seg_q1[sgi];
   const double *pr = pkt + seg_off[sgi];
   double *o = segmap + ((size_t)sgi * SEGCOL + b) * NPAR;
   if (b == SEG_I) {
@@ -1140,14 +1162,23 @@ struct ipgpu_ctx {
   double *d_pkt2 = nullptr;
   size_t pkt2_doubles = 0;
   long long last_total = 0;  /* packets of the previous render (arena size hint) */
+  /* streamed render: the first steps of the rays still traced after the
+   * last bounded phase are prepared and composed while the unbounded phase
+   * runs (packets d_ppkt, list d_plist, list positions d_pidx) */
+  int tail_prefix = 1;
+  double *d_ppkt = nullptr;
+  size_t ppkt_doubles = 0;
+  int *d_plist = nullptr, *d_pidx = nullptr;
+  size_t cap_plist = 0, cap_pidx = 0;
+  std::vector<int> h_ppos;
   struct SegSet {
     long long *d_off = nullptr;
-    int *d_q0 = nullptr, *d_ns = nullptr, *d_flag = nullptr;
+    int *d_q0 = nullptr, *d_q1 = nullptr, *d_flag = nullptr;
     double *d_map = nullptr;
-    size_t cap_off = 0, cap_q0 = 0, cap_ns = 0, cap_flag = 0, cap_map = 0;
+    size_t cap_off = 0, cap_q0 = 0, cap_q1 = 0, cap_flag = 0, cap_map = 0;
     std::vector<long long> off;
-    std::vector<int> q0, ns;
-  } segset[MAXPH];
+    std::vector<int> q0, q1;
+  } segset[MAXPH], segpre;
   /* host copies read by pending asynchronous copies: they live until the
    * render that filled them has synchronised */
   std::vector<int> h_ns, h_list, h_base, h_cnt, h_tmp;
@@ -1255,6 +1286,7 @@ int ipgpu_create(ipgpu_ctx **out, int device)
   if (const char *e = getenv("IPGPU_SEG_LEN")) ctx->seg_len = atoi(e) > 1 ? atoi(e) : 1;
   if (const char *e = getenv("IPGPU_TRACE_PHASES")) ctx->trace_phases = atoi(e);
   if (const char *e = getenv("IPGPU_REFINE_SPECULATE_MAX")) ctx->refine_spec_max = atoll(e);
+  if (const char *e = getenv("IPGPU_TAIL_PREFIX")) ctx->tail_prefix = atoi(e);
   if (hipMalloc((void **)&ctx->d_status, NSTATUS * sizeof(int)) != hipSuccess
       || hipMalloc((void **)&ctx->d_counts, 2 * sizeof(unsigned long long)) != hipSuccess) {
     delete ctx;
@@ -1272,8 +1304,9 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
   for (auto st : ctx->streamF)
     if (st) hipStreamSynchronize(st);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
-  for (auto &S : ctx->segset) {
-    void *sp[] = {S.d_off, S.d_q0, S.d_ns, S.d_flag, S.d_map};
+  for (auto *S : {&ctx->segset[0], &ctx->segset[1], &ctx->segset[2], &ctx->segset[3], &ctx->segset[4],
+                  &ctx->segpre}) {
+    void *sp[] = {S->d_off, S->d_q0, S->d_q1, S->d_flag, S->d_map};
     for (void *p : sp)
       if (p) hipFree(p);
   }
@@ -1283,7 +1316,8 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
                   ctx->d_groups, ctx->d_jobs, ctx->d_items, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
                   ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_ckpt, ctx->d_seg, ctx->d_gcell,
                   ctx->d_out4, ctx->d_target, ctx->d_scale, ctx->d_trrec, ctx->d_trout, ctx->d_trnt,
-                  ctx->d_rlist, ctx->d_rcount, ctx->d_iavg, ctx->d_interp};
+                  ctx->d_rlist, ctx->d_rcount, ctx->d_iavg, ctx->d_interp, ctx->d_ppkt, ctx->d_plist,
+                  ctx->d_pidx};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int q = 0; q < NEV; q++)
@@ -1488,20 +1522,27 @@ static int prepare_packets(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, bool
  * longer than seg_min_steps); off[k]: position of element k's first packet
  * in doubles; base[k] = first segment of element k or -1. */
 static void build_segments(ipgpu_ctx::SegSet &S, const int *list, int n, const long long *off,
-                           const std::vector<int> &ns, int seg_min, int seg_len, int maxnstep, int *base)
+                           const std::vector<int> &ns, int seg_min, int seg_len, int maxnstep, int *base,
+                           int covered = 0)
 {
+  /* covered: a multiple of seg_len; the segments of each ray's first
+   * covered steps from the camera are formed elsewhere and left out (its
+   * packets here are those of the other ns - covered steps) */
   S.off.clear();
   S.q0.clear();
-  S.ns.clear();
+  S.q1.clear();
   for (int k = 0; k < n; k++) {
     const int v = ns[list[k]];
     base[k] = -1;
     if (v > seg_min && v < maxnstep - 1) {
       base[k] = (int)S.off.size();
-      for (int q0 = 0; q0 < v; q0 += seg_len) {
+      const int nseg = (v - covered + seg_len - 1) / seg_len;
+      for (int sg = 0; sg < nseg; sg++) {
+        int q0, q1;
+        segment_bounds(v, seg_len, sg, q0, q1);
         S.off.push_back(off[k]);
         S.q0.push_back(q0);
-        S.ns.push_back(v);
+        S.q1.push_back(q1);
       }
     }
   }
@@ -1513,22 +1554,35 @@ static int upload_segments(ipgpu_ctx *ctx, ipgpu_ctx::SegSet &S, hipStream_t st)
   if (nseg == 0) return 0;
   int rc;
   if ((rc = grow(ctx, S.d_off, S.cap_off, nseg)) || (rc = grow(ctx, S.d_q0, S.cap_q0, nseg))
-      || (rc = grow(ctx, S.d_ns, S.cap_ns, nseg)) || (rc = grow(ctx, S.d_flag, S.cap_flag, nseg))
+      || (rc = grow(ctx, S.d_q1, S.cap_q1, nseg)) || (rc = grow(ctx, S.d_flag, S.cap_flag, nseg))
       || (rc = grow(ctx, S.d_map, S.cap_map, nseg * SEGCOL * NPAR)))
     return rc;
   HIPCHK(ctx, hipMemcpyAsync(S.d_off, S.off.data(), nseg * sizeof(long long), hipMemcpyHostToDevice, st));
   HIPCHK(ctx, hipMemcpyAsync(S.d_q0, S.q0.data(), nseg * sizeof(int), hipMemcpyHostToDevice, st));
-  HIPCHK(ctx, hipMemcpyAsync(S.d_ns, S.ns.data(), nseg * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(S.d_q1, S.q1.data(), nseg * sizeof(int), hipMemcpyHostToDevice, st));
   return 0;
 }
+
+/* Segment maps of the rays' first steps formed ahead (k_prep_prefix and
+ * k_seg on another stream): element k's npre maps at map + idx[k] npre,
+ * complete when ev has fired. */
+struct PreMaps {
+  const int *idx;
+  const double *map;
+  const int *flag;
+  int npre;
+  hipEvent_t ev;
+};
 
 /* Step packets, segment maps and chains of the n rays of a list (device
  * list dl sorted longest first, wave rows W and packet-kernel tiles TT with
  * nw + 1 entries each, rows and tiles in all), on stream st; packets
- * wave-interleaved at pkt, block map at blk. */
+ * wave-interleaved at pkt, block map at blk.  pre (compact lists only):
+ * the maps of the rays' first steps, formed ahead. */
 static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, int n, const int *dl,
                          const long long *W, const long long *TT, long long tiles, int nw, long long rows, int *blk,
-                         double *pkt, const int *dbase, ipgpu_ctx::SegSet &S, int cap, int stride, bool compact)
+                         double *pkt, const int *dbase, ipgpu_ctx::SegSet &S, int cap, int stride, bool compact,
+                         const PreMaps *pre = nullptr, hipEvent_t after_prep = nullptr)
 {
   if (n <= 0) return 0;
   if (compact) {
@@ -1545,14 +1599,17 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
     const int nseg = (int)S.off.size();
     if (nseg > 0) {
       long long lanes = (long long)nseg * SEGCOL;
-      k_seg<1><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, st>>>(nseg, ctx->seg_len, S.d_off, S.d_q0, S.d_ns,
+      k_seg<1><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, st>>>(nseg, S.d_off, S.d_q0, S.d_q1,
                                                                       pkt, S.d_map, S.d_flag);
       HIPCHK(ctx, hipGetLastError());
     }
+    if (pre) HIPCHK(ctx, hipStreamWaitEvent(st, pre->ev, 0));
     k_chain_list<true><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, ctx->pix, dl, n, W, cap, stride,
                                                               ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt, dbase,
                                                               ctx->seg_len, S.d_map, S.d_flag, ctx->d_out7,
-                                                              ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
+                                                              ctx->d_flag_ray, ctx->d_status, ctx->d_counts,
+                                                              pre ? pre->idx : nullptr, pre ? pre->map : nullptr,
+                                                              pre ? pre->flag : nullptr, pre ? pre->npre : 0);
     HIPCHK(ctx, hipGetLastError());
     return 0;
   }
@@ -1565,10 +1622,11 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
                                   ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt);
     HIPCHK(ctx, hipGetLastError());
   }
+  if (after_prep) HIPCHK(ctx, hipEventRecord(after_prep, st));
   const int nseg = (int)S.off.size();
   if (nseg > 0) {
     long long lanes = (long long)nseg * SEGCOL;
-    k_seg<ILV><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, st>>>(nseg, ctx->seg_len, S.d_off, S.d_q0, S.d_ns,
+    k_seg<ILV><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, st>>>(nseg, S.d_off, S.d_q0, S.d_q1,
                                                                       pkt, S.d_map, S.d_flag);
     HIPCHK(ctx, hipGetLastError());
   }
@@ -1577,14 +1635,15 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
     k_chain_list<true><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, ctx->pix, dl, n, W, cap, stride,
                                                               ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt, dbase,
                                                               ctx->seg_len, S.d_map, S.d_flag, ctx->d_out7,
-                                                              ctx->d_flag_ray, ctx->d_status, ctx->d_counts);
+                                                              ctx->d_flag_ray, ctx->d_status, ctx->d_counts, nullptr,
+                                                              nullptr, nullptr, 0);
     HIPCHK(ctx, hipGetLastError());
   }
   k_chain_list<false><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, ctx->pix, dl, n, W, cap, stride,
                                                              ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt,
                                                              nseg > 0 ? dbase : nullptr, ctx->seg_len, S.d_map,
                                                              S.d_flag, ctx->d_out7, ctx->d_flag_ray, ctx->d_status,
-                                                             ctx->d_counts);
+                                                             ctx->d_counts, nullptr, nullptr, nullptr, 0);
   HIPCHK(ctx, hipGetLastError());
   return 0;
 }
@@ -1652,7 +1711,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
    * each phase's step counts and live flags into it (a copy command would
    * hold the host until the phase ends) */
   const size_t nI = (size_t)nrays;
-  const size_t need = sizeof(int) * nI * (2 * (MAXPH) + 2) + 2 * sizeof(long long) * (nI + MAXPH + 1);
+  const size_t need = sizeof(int) * nI * (2 * (MAXPH) + 4) + 2 * sizeof(long long) * (nI + MAXPH + 1);
   if (ctx->h_pin_bytes < need) {
     if (ctx->h_pin) hipHostFree(ctx->h_pin);
     ctx->h_pin = nullptr;
@@ -1668,7 +1727,16 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
   int *h_live = h_ns + (size_t)MAXPH * nI;            /* [K][nrays] */
   int *h_list = h_live + (size_t)MAXPH * nI;
   int *h_base = h_list + nI;
+  int *h_plist = h_base + nI, *h_pidx = h_plist + nI;
   int *dh_ns = (int *)(d_pin + ns_bytes), *dh_live = dh_ns + (size_t)MAXPH * nI;
+  /* the first Ppre steps (a multiple of the segment length, records all
+   * stored by the bounded phases) of the rays the unbounded phase traces
+   * are prepared and composed on stream streamF[K] while it runs */
+  const int Ppre = K >= 1 ? ((bud[K - 1] - 1) / ctx->seg_len) * ctx->seg_len : 0;
+  const bool pre_on = ctx->tail_prefix && segm && K >= 1 && K <= MAXPH - 2 && seg_min <= bud[K - 1]
+      && Ppre >= ctx->seg_len && Ppre + 1 < cap;
+  bool pre_launched = false, prep_ev = false;
+  int pre_npre = 0;
 
   static const int init[NSTATUS] = {0, 0x7fffffff, 0x7fffffff, 0};
   HIPCHK(ctx, hipEventRecord(ev[0], sA));
@@ -1717,7 +1785,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
       for (int q = 0; q < n; q++) mx = nsv[h_list[start + q]] > mx ? nsv[h_list[start + q]] : mx;
       if (mx >= cap) {
         ctx->traj_cap_steps = mx + 1 < P.maxnstep ? mx + 1 : P.maxnstep;
-        for (int q = 0; q < K; q++) HIPCHK(ctx, hipStreamSynchronize(ctx->streamF[q]));
+        for (int q = 0; q < K + (pre_launched ? 1 : 0); q++) HIPCHK(ctx, hipStreamSynchronize(ctx->streamF[q]));
         fallback = true;
         return 0;
       }
@@ -1730,11 +1798,20 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
     sort_longest_first(h_list + start, n, nsv, ctx->h_cnt, ctx->h_tmp);
     bool compact = (k == K) && segm && n > 0;
     for (int q = 0; compact && q < n; q++) compact = nsv[h_list[start + q]] > seg_min;
+    /* the rays' first Ppre steps were composed ahead: their packets here are
+     * those of the other steps */
+    const bool use_pre = pre_launched && compact;
+    const int covered = use_pre ? Ppre : 0;
+    if (use_pre)
+      for (int q = 0; q < n; q++) {
+        h_pidx[q] = ctx->h_ppos[h_list[start + q]];
+        if (h_pidx[q] < 0) return set_err(ctx, IPGPU_ERR_STATE, "streamed render: inconsistent prefix list");
+      }
     const int nw = compact ? n : (n + ILV - 1) / ILV;
     long long *W = h_off + start + k;
     W[0] = 0;
     if (compact)
-      for (int q = 0; q < n; q++) W[q + 1] = W[q] + nsv[h_list[start + q]];
+      for (int q = 0; q < n; q++) W[q + 1] = W[q] + nsv[h_list[start + q]] - covered;
     else
       for (int c = 0; c < nw; c++) W[c + 1] = W[c] + nsv[h_list[start + c * ILV]];
     const long long rows = W[nw];
@@ -1750,7 +1827,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
     eoff.resize(n);
     for (int q = 0; q < n; q++) eoff[q] = compact ? W[q] * PK : W[q / ILV] * stride * ILV + (long long)pk_lane<ILV>(q % ILV);
     build_segments(ctx->segset[k], h_list + start, n, eoff.data(), nsv, seg_min, ctx->seg_len, P.maxnstep,
-                   h_base + start);
+                   h_base + start, covered);
     hipStream_t st = (k < K) ? ctx->streamF[k] : sA;
     int *dl = ctx->d_list + start, *dbase = ctx->d_segbase + start;
     long long *dW = ctx->d_offl + start + k, *dTT = dW + (nI + MAXPH + 1);
@@ -1786,17 +1863,70 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
       if (!ctx->segset[k].off.empty())
         HIPCHK(ctx, hipMemcpyAsync(dbase, h_base + start, n * sizeof(int), hipMemcpyHostToDevice, st));
       if ((rc = upload_segments(ctx, ctx->segset[k], st))) return rc;
+      PreMaps pm{ctx->d_pidx, ctx->segpre.d_map, ctx->segpre.d_flag, pre_npre, ev[23]};
+      if (use_pre) {
+        if ((rc = grow(ctx, ctx->d_pidx, ctx->cap_pidx, (size_t)n))) return rc;
+        pm.idx = ctx->d_pidx;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_pidx, h_pidx, n * sizeof(int), hipMemcpyHostToDevice, st));
+      }
       if ((rc = transfer_list(ctx, P, st, n, dl, dW, dTT, tiles, nw, rows, blk, pkt, dbase, ctx->segset[k], cap,
-                              stride, compact)))
+                              stride, compact, use_pre ? &pm : nullptr,
+                              (k == K - 1 && pre_on && !compact) ? ev[10] : nullptr)))
         return rc;
+      prep_ev = k == K - 1 && pre_on && !compact;
     }
     if (k < K) HIPCHK(ctx, hipEventRecord(ev[15 + k], st));
+    if (k == K - 1 && pre_on) {
+      /* the rays still traced, whose first Ppre steps are all stored */
+      int np = 0;
+      ctx->h_ppos.assign(nI, -1);
+      for (int r = 0; r < nrays; r++)
+        if (((volatile int *)h_live)[(size_t)(K - 1) * nI + r]) {
+          ctx->h_ppos[r] = np;
+          h_plist[np++] = r;
+        }
+      if (np > 0) {
+        const int np1 = Ppre + 1, npre = Ppre / ctx->seg_len;
+        ipgpu_ctx::SegSet &S = ctx->segpre;
+        S.off.clear();
+        S.q0.clear();
+        S.q1.clear();
+        for (int j = 0; j < np; j++)
+          for (int sg = 0; sg < npre; sg++) {
+            S.off.push_back((long long)j * np1 * PK);
+            S.q0.push_back(1 + sg * ctx->seg_len);
+            S.q1.push_back(1 + (sg + 1) * ctx->seg_len);
+          }
+        hipStream_t sp = ctx->streamF[K];
+        /* after the packets of the rays this phase finished, which the
+         * chain that ends last waits for */
+        if (prep_ev) HIPCHK(ctx, hipStreamWaitEvent(sp, ev[10], 0));
+        if ((rc = grow(ctx, ctx->d_ppkt, ctx->ppkt_doubles, (size_t)np * np1 * PK))
+            || (rc = grow(ctx, ctx->d_plist, ctx->cap_plist, (size_t)np)))
+          return rc;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_plist, h_plist, np * sizeof(int), hipMemcpyHostToDevice, sp));
+        if ((rc = upload_segments(ctx, S, sp))) return rc;
+        const long long slots = (long long)np * np1;
+        auto kp = P.transfer_fp32 ? k_prep_prefix<true> : k_prep_prefix<false>;
+        kp<<<(unsigned)((slots + PREP_BLOCK - 1) / PREP_BLOCK), PREP_BLOCK, 0, sp>>>(
+            ctx->geom, ctx->model, P, ctx->d_plist, np, np1, cap, ctx->d_traj, ctx->d_dump, ctx->d_ppkt);
+        HIPCHK(ctx, hipGetLastError());
+        const long long lanes = (long long)S.off.size() * SEGCOL;
+        k_seg<1><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, sp>>>((int)S.off.size(), S.d_off, S.d_q0, S.d_q1,
+                                                                        ctx->d_ppkt, S.d_map, S.d_flag);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventRecord(ev[23], sp));
+        pre_launched = true;
+        pre_npre = npre;
+      }
+    }
     start += n;
   }
   if (!any_f) HIPCHK(ctx, hipEventRecord(ev[19], sA));
   ctx->last_total = total;
   HIPCHK(ctx, hipEventRecord(ev[20], sA));
   for (int k = 0; k < K; k++) HIPCHK(ctx, hipStreamWaitEvent(sA, ev[15 + k], 0));
+  if (pre_launched) HIPCHK(ctx, hipStreamWaitEvent(sA, ev[23], 0));
   HIPCHK(ctx, hipEventRecord(ev[21], sA));
   ctx->nphase = K;
   return 0;
@@ -1878,8 +2008,8 @@ static int render_rays(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays)
         HIPCHK(ctx, hipMemcpyAsync(ctx->d_segbase, base.data(), nrays * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
         if ((rc = upload_segments(ctx, S, ctx->stream))) return rc;
         long long lanes = (long long)S.off.size() * SEGCOL;
-        k_seg<1><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>((int)S.off.size(), ctx->seg_len,
-                                                                                  S.d_off, S.d_q0, S.d_ns, ctx->d_pkt,
+        k_seg<1><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>((int)S.off.size(), S.d_off,
+                                                                                  S.d_q0, S.d_q1, ctx->d_pkt,
                                                                                   S.d_map, S.d_flag);
         HIPCHK(ctx, hipGetLastError());
       }

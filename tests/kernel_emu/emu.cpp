@@ -262,7 +262,8 @@ int emu_render_segmented(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_
       segmap.assign((size_t)(nseg > 0 ? nseg : 1) * SEGCOL * NPAR, 0.);
       segflag.assign(nseg > 0 ? nseg : 1, 0);
       for (int sg = 0; sg < nseg; sg++) {
-        const int q0 = sg * seg_len, q1 = (sg + 1) * seg_len < ns ? (sg + 1) * seg_len : ns;
+        int q0, q1;
+        segment_bounds(ns, seg_len, sg, q0, q1);
         for (int b = 0; b < SEGCOL; b++) {
           double *o = segmap.data() + ((size_t)sg * SEGCOL + b) * NPAR;
           if (b == SEG_I) {

@@ -71,6 +71,34 @@ def test_streamed_equals_one_pass(case, unpol):
         c2.close()
 
 
+@pytest.mark.parametrize("seg_len", [32, 37])
+def test_streamed_tail_prefix(case, seg_len, monkeypatch):
+    """The first steps of the rays still traced after the last bounded phase
+    (here 288 or 296 of them, whole segments) are prepared and composed on a
+    stream of their own while the unbounded phase runs; the rest after it.
+    Segment boundaries are counted from the camera, so the image is that of
+    the one-pass render and of the streamed render without the split, bit
+    for bit."""
+    h, prims, orc = case
+    say("case streamed tail prefix seg_len %d 32x32 th=163" % seg_len)
+    tune = dict(seg_min_steps=100, seg_len=seg_len)
+    c1, g, m = fresh(h, prims, trace_budget=0, **tune)
+    im, _, _ = G.make_image(g, m.L_unit, nx=32, ny=32, thetacam=163.)
+    ref = c1.render(im)
+    c1.close()
+    assert int((ref["nstep"] > 300).sum()) > 10
+    c2, _, _ = fresh(h, prims, trace_budget=300, **tune)
+    got = c2.render(im)
+    assert c2.stage_times_mode()[1] is True
+    same(ref, got)
+    same(ref, c2.render(im))
+    c2.close()
+    monkeypatch.setenv("IPGPU_TAIL_PREFIX", "0")
+    c3, _, _ = fresh(h, prims, trace_budget=300, **tune)
+    same(ref, c3.render(im))
+    c3.close()
+
+
 def test_streamed_vs_oracle(case):
     """Small budget (most rays resume) and a low segment threshold (most
     chains composed), at the parity bar against the oracle."""
