@@ -58,6 +58,12 @@ __device__ __forceinline__ bool ray_pixel(const int *pix, int ray, int nx, int n
   return i < nx && j < ny;
 }
 
+/* Issue priority of a kernel's waves among the waves of other kernels on
+ * the same SIMD: the latency-bound kernels (the geodesic phases, the
+ * chains, the segment maps of the rays traced last) take the issue slots
+ * first and the packet kernels fill the rest. */
+#define IPG_LATENCY_CRITICAL() __builtin_amdgcn_s_setprio(3)
+
 /* pixel code of the status words: first bad pixel by atomicMin */
 __device__ __forceinline__ int pixel_code(int i, int j) { return ((i << 16) | j) + 1; }
 
@@ -718,6 +724,7 @@ __global__ __launch_bounds__(WAVE) void k_trace_budget(ipgpu_geom G, ipgpu_image
                                                        int *__restrict__ host_live,
                                                        unsigned long long *__restrict__ counts)
 {
+  IPG_LATENCY_CRITICAL();
   /* one lane pair per ray */
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int ray = t >> 1;
@@ -748,6 +755,7 @@ __global__ __launch_bounds__(WAVE) void k_trace_resume_mask(ipgpu_geom G, ipgpu_
                                                             int *__restrict__ host_ns, int *__restrict__ host_live,
                                                             unsigned long long *__restrict__ counts)
 {
+  IPG_LATENCY_CRITICAL();
   /* one lane pair per ray */
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int ray = t >> 1;
@@ -895,6 +903,7 @@ __global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model M
                                                      const double *__restrict__ pre_map,
                                                      const int *__restrict__ pre_flag, int npre)
 {
+  IPG_LATENCY_CRITICAL();
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int ray = list[k];
@@ -944,8 +953,9 @@ template <int PS>
 __global__ __launch_bounds__(WAVE) void k_seg(int nseg, const long long *__restrict__ seg_off,
                                               const int *__restrict__ seg_q0, const int *__restrict__ seg_q1,
                                               const double *__restrict__ pkt, double *__restrict__ segmap,
-                                              int *__restrict__ segflag)
+                                              int *__restrict__ segflag, int urgent)
 {
+  if (urgent) IPG_LATENCY_CRITICAL();
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nseg * SEGCOL) return;
   const int sgi = (int)(t / SEGCOL), b = (int)(t % SEGCOL);
@@ -1600,7 +1610,7 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
     if (nseg > 0) {
       long long lanes = (long long)nseg * SEGCOL;
       k_seg<1><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, st>>>(nseg, S.d_off, S.d_q0, S.d_q1,
-                                                                      pkt, S.d_map, S.d_flag);
+                                                                      pkt, S.d_map, S.d_flag, 1);
       HIPCHK(ctx, hipGetLastError());
     }
     if (pre) HIPCHK(ctx, hipStreamWaitEvent(st, pre->ev, 0));
@@ -1627,7 +1637,7 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
   if (nseg > 0) {
     long long lanes = (long long)nseg * SEGCOL;
     k_seg<ILV><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, st>>>(nseg, S.d_off, S.d_q0, S.d_q1,
-                                                                      pkt, S.d_map, S.d_flag);
+                                                                      pkt, S.d_map, S.d_flag, 1);
     HIPCHK(ctx, hipGetLastError());
   }
   /* composed rays (the longest, at the front of the list) and the rest */
@@ -1913,7 +1923,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
         HIPCHK(ctx, hipGetLastError());
         const long long lanes = (long long)S.off.size() * SEGCOL;
         k_seg<1><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, sp>>>((int)S.off.size(), S.d_off, S.d_q0, S.d_q1,
-                                                                        ctx->d_ppkt, S.d_map, S.d_flag);
+                                                                        ctx->d_ppkt, S.d_map, S.d_flag, 0);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(ev[23], sp));
         pre_launched = true;
@@ -2010,7 +2020,7 @@ static int render_rays(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays)
         long long lanes = (long long)S.off.size() * SEGCOL;
         k_seg<1><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>((int)S.off.size(), S.d_off,
                                                                                   S.d_q0, S.d_q1, ctx->d_pkt,
-                                                                                  S.d_map, S.d_flag);
+                                                                                  S.d_map, S.d_flag, 0);
         HIPCHK(ctx, hipGetLastError());
       }
     }
