@@ -580,6 +580,94 @@ __global__ __launch_bounds__(WAVE) void k_trace_store(ipgpu_geom G, const BGroup
   atomicAdd(&counts[0], (unsigned long long)nstep);
 }
 
+/* The batch's trace passes without subring counts run on lane pairs (two
+ * lanes per ray, trace_pair), whose records a resumed trace continues bit
+ * for bit.  Pass 1: every ray's step count and, with slot > 0, its first
+ * slot records in a slot of its own (traj + r slot REC); a ray of fewer
+ * than slot steps is then complete. */
+__global__ __launch_bounds__(WAVE) void k_trace_slot(ipgpu_geom G, const BGroup *__restrict__ groups,
+                                                     int ngroups, long long nr, int slot, double *__restrict__ traj,
+                                                     int *__restrict__ nstep_out)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long r = t >> 1;
+  const bool odd = t & 1;
+  if (r >= nr) return;
+  const int g = group_of(groups, ngroups, r);
+  const ipgpu_image P = groups[g].P;
+  int pi, pj;
+  ray_pixel(nullptr, (int)(r - groups[g].ray_base), P.nx, P.ny, pi, pj);
+  if (pi >= P.nx || pj >= P.ny) {
+    if (!odd) nstep_out[r] = 0;
+    return;
+  }
+  const int ns = trace_pair(G, P, pi, pj, slot, slot > 0 ? traj + (size_t)r * slot * REC : nullptr, 0, 0x7fffffff,
+                            nullptr, odd);
+  if (!odd) nstep_out[r] = ns;
+}
+
+/* Pass 2 on lane pairs: the records of the nr rays of order in their
+ * regions (offsets off, in records). */
+__global__ __launch_bounds__(WAVE) void k_trace_store_pair(ipgpu_geom G, const BGroup *__restrict__ groups,
+                                                           int ngroups, const int *__restrict__ order, int nr,
+                                                           const long long *__restrict__ off,
+                                                           const int *__restrict__ nstep_in,
+                                                           double *__restrict__ traj, int *__restrict__ status,
+                                                           unsigned long long *__restrict__ counts)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = t >> 1;
+  const bool odd = t & 1;
+  if (k >= nr) return;
+  const int r = order[k];
+  const int g = group_of(groups, ngroups, r);
+  const ipgpu_image P = groups[g].P;
+  int pi, pj;
+  ray_pixel(nullptr, (int)(r - groups[g].ray_base), P.nx, P.ny, pi, pj);
+  const int ns = nstep_in[r];
+  const int nstep = trace_pair(G, P, pi, pj, ns + 1, traj + (size_t)off[r] * REC, 0, 0x7fffffff, nullptr, odd);
+  if (odd) return;
+  if (nstep != ns) atomicAdd(&status[0], 1);   /* the two passes are the same arithmetic */
+  atomicAdd(&counts[0], (unsigned long long)nstep);
+}
+
+/* The slot records of the n rays of list (slot REC doubles each) copied to
+ * their full-length regions in traj2 (record offsets off2[k]). */
+__global__ void k_slot_copy(const int *__restrict__ list, int n, const long long *__restrict__ off2, int slot,
+                            const double *__restrict__ traj, double *__restrict__ traj2)
+{
+  const long long per = (long long)slot * REC;
+  const long long total = (long long)n * per;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int k = (int)(t / per);
+    const long long e = t - (long long)k * per;
+    traj2[off2[k] * REC + e] = traj[(long long)list[k] * per + e];
+  }
+}
+
+/* The rays of list longer than the slot, on lane pairs: resumed from their
+ * last slot record (slot - 1) in their full-length regions of traj2; a step
+ * count other than the first pass's is reported in status[0]. */
+__global__ __launch_bounds__(WAVE) void k_trace_resume_list(ipgpu_geom G, const BGroup *__restrict__ groups,
+                                                            int ngroups, const int *__restrict__ list, int n,
+                                                            const long long *__restrict__ off2, int slot,
+                                                            const int *__restrict__ nstep_in,
+                                                            double *__restrict__ traj2, int *__restrict__ status)
+{
+  IPG_LATENCY_CRITICAL();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = t >> 1;
+  const bool odd = t & 1;
+  if (k >= n) return;
+  const int r = list[k];
+  const int g = group_of(groups, ngroups, r);
+  const ipgpu_image P = groups[g].P;
+  const int ns = nstep_in[r];
+  const int nstep = trace_pair(G, P, 0, 0, ns + 1, traj2 + (size_t)off2[k] * REC, slot - 1, 0x7fffffff, nullptr, odd);
+  if (!odd && nstep != ns) atomicAdd(&status[0], 1);
+}
+
 /* One lane per (job, ray) work item: the fused transfer of the ray for the
  * job, then the job's pixel in the reference layout (save_pixel,
  * src/main.c:885-910).  bstatus[0]: min of (job << 32 | pixel) + 1 over
@@ -590,6 +678,7 @@ __global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model M
                                                      const int2 *__restrict__ items, int nitems,
                                                      const long long *__restrict__ off,
                                                      const double *__restrict__ traj,
+                                                     const double *__restrict__ traj2,
                                                      const int *__restrict__ nstep_in,
                                                      const int *__restrict__ nturns_buf,
                                                      const double *__restrict__ dump, double *__restrict__ image,
@@ -619,8 +708,10 @@ __global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model M
       for (int q = 0; q < 5; q++) imageS[p * 5 + q] = 0.;
     return;
   }
-  const double *rec = traj + (size_t)off[r] * REC;
-  const int *nt = (nturns_buf && P.target_nturns >= 0) ? nturns_buf + off[r] : nullptr;
+  /* off[r] >= 0: records at traj + off[r] REC; < 0: at traj2 + (-1 - off[r]) REC */
+  const long long ro = off[r];
+  const double *rec = ro >= 0 ? traj + (size_t)ro * REC : traj2 + (size_t)(-1 - ro) * REC;
+  const int *nt = (nturns_buf && P.target_nturns >= 0) ? nturns_buf + ro : nullptr;
   double o[7];
   int nrad;
   int flag = transfer_fused<F32>(G, Mo, P, ns, rec, nt, dump, J.sc, o, nrad);
@@ -1135,6 +1226,13 @@ struct ipgpu_ctx {
   double *d_traj = nullptr;
   size_t traj_doubles = 0;
   int *d_nturns = nullptr;
+  /* batches: records per ray of the first trace pass; the full-length
+   * records of the rays longer than that */
+  int batch_slot = 2048;
+  double *d_traj2 = nullptr;
+  size_t traj2_doubles = 0;
+  long long *d_off2 = nullptr;
+  size_t cap_off2 = 0;
   size_t nturns_ints = 0;
   double *d_pkt = nullptr;
   size_t pkt_doubles = 0;
@@ -1297,6 +1395,7 @@ int ipgpu_create(ipgpu_ctx **out, int device)
   if (const char *e = getenv("IPGPU_TRACE_PHASES")) ctx->trace_phases = atoi(e);
   if (const char *e = getenv("IPGPU_REFINE_SPECULATE_MAX")) ctx->refine_spec_max = atoll(e);
   if (const char *e = getenv("IPGPU_TAIL_PREFIX")) ctx->tail_prefix = atoi(e);
+  if (const char *e = getenv("IPGPU_BATCH_SLOT")) ctx->batch_slot = atoi(e);
   if (hipMalloc((void **)&ctx->d_status, NSTATUS * sizeof(int)) != hipSuccess
       || hipMalloc((void **)&ctx->d_counts, 2 * sizeof(unsigned long long)) != hipSuccess) {
     delete ctx;
@@ -1326,7 +1425,7 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
                   ctx->d_groups, ctx->d_jobs, ctx->d_items, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
                   ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_ckpt, ctx->d_seg, ctx->d_gcell,
                   ctx->d_out4, ctx->d_target, ctx->d_scale, ctx->d_trrec, ctx->d_trout, ctx->d_trnt,
-                  ctx->d_rlist, ctx->d_rcount, ctx->d_iavg, ctx->d_interp, ctx->d_ppkt, ctx->d_plist,
+                  ctx->d_rlist, ctx->d_rcount, ctx->d_iavg, ctx->d_interp, ctx->d_ppkt, ctx->d_plist, ctx->d_traj2, ctx->d_off2,
                   ctx->d_pidx};
   for (void *p : ptrs)
     if (p) hipFree(p);
@@ -2435,12 +2534,24 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
 
   float ms_geo = 0.f, ms_xfer = 0.f, t = 0.f;
   HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-  /* pass 1: step counts of every ray */
+  /* pass 1: step counts of every ray; when every ray's slot of records
+   * fits the budget, also its first slot records (most rays are then
+   * complete and only the longer ones are continued, from their last slot
+   * record), else a count alone and the records in pass 2 */
+  const long long budget = (long long)(ctx->batch_traj_bytes / (REC * sizeof(double)));
+  const int slot = ctx->batch_slot;
+  const bool try_slot = !turns && slot >= 2 && (long long)rays_total * slot <= budget;
+  const bool pairs = !turns;   /* the single-lane trace counts subring turns */
   if ((rc = grow(ctx, ctx->d_nstep_ray, ctx->cap_nstep_ray, (size_t)rays_total))) return rc;
   if ((rc = grow(ctx, ctx->d_off, ctx->cap_off, (size_t)rays_total))) return rc;
+  if (try_slot && (rc = grow(ctx, ctx->d_traj, ctx->traj_doubles, (size_t)rays_total * slot * REC))) return rc;
   HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-  k_trace_count<<<(unsigned)((rays_total + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>(
-      ctx->geom, ctx->d_groups, ngroups, rays_total, ctx->d_nstep_ray);
+  if (pairs)
+    k_trace_slot<<<(unsigned)((2 * rays_total + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>(
+        ctx->geom, ctx->d_groups, ngroups, rays_total, try_slot ? slot : 0, ctx->d_traj, ctx->d_nstep_ray);
+  else
+    k_trace_count<<<(unsigned)((rays_total + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>(
+        ctx->geom, ctx->d_groups, ngroups, rays_total, ctx->d_nstep_ray);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
   std::vector<int> ns_host(rays_total);
@@ -2466,13 +2577,84 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     if (valid[r]) bucket[mxs - ns_host[r] + 1]++;
   for (int k = 1; k <= mxs + 1; k++) bucket[k] += bucket[k - 1];
   const long long nvalid = bucket[mxs + 1];
+  long long nvalid_done = 0, steps_host = 0;
   std::vector<int> order(nvalid);
   for (long long r = 0; r < rays_total; r++)
     if (valid[r]) order[bucket[mxs - ns_host[r]]++] = (int)r;
   std::vector<long long> off_host(rays_total, 0);
-  const long long budget = (long long)(ctx->batch_traj_bytes / (REC * sizeof(double)));
   std::vector<int2> items;
-  for (long long a = 0; a < nvalid;) {
+  long long long_recs = 0, nlong = 0, slot_steps = 0;
+  if (try_slot)
+    for (long long q = 0; q < nvalid; q++) {
+      const int v = ns_host[order[q]];
+      slot_steps += v;
+      if (v >= slot) {
+        long_recs += v + 1;
+        nlong++;
+      }
+    }
+  if (try_slot && (long long)rays_total * slot + long_recs <= budget) {
+    /* every ray's records in place: its slot, or for the nlong longest (at
+     * the front of order) a full-length region of traj2 (off = -1 - offset) */
+    std::vector<long long> off2(nlong > 0 ? nlong : 1);
+    long long o2 = 0;
+    for (long long q = 0; q < nvalid; q++) {
+      const int r = order[q];
+      if (q < nlong) {
+        off2[q] = o2;
+        off_host[r] = -1 - o2;
+        o2 += ns_host[r] + 1;
+      } else {
+        off_host[r] = (long long)r * slot;
+      }
+    }
+    for (long long q = 0; q < nvalid; q++) {
+      int r = order[q];
+      int g = group_of_host(groups, r);
+      for (int jb : members[g]) items.push_back(make_int2(jb, r));
+    }
+    const int nitems = (int)items.size();
+    if ((rc = grow(ctx, ctx->d_items, ctx->cap_items, (size_t)(nitems > 0 ? nitems : 1)))) return rc;
+    if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)(nlong > 0 ? nlong : 1)))) return rc;
+    if ((rc = grow(ctx, ctx->d_off2, ctx->cap_off2, (size_t)(nlong > 0 ? nlong : 1)))) return rc;
+    if (nlong > 0 && (rc = grow(ctx, ctx->d_traj2, ctx->traj2_doubles, (size_t)long_recs * REC))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_off, off_host.data(), rays_total * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    if (nlong > 0) {
+      HIPCHK(ctx, hipMemcpyAsync(ctx->d_blk, order.data(), nlong * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+      HIPCHK(ctx, hipMemcpyAsync(ctx->d_off2, off2.data(), nlong * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
+      k_slot_copy<<<1024, 256, 0, ctx->stream>>>(ctx->d_blk, (int)nlong, ctx->d_off2, slot, ctx->d_traj, ctx->d_traj2);
+      HIPCHK(ctx, hipGetLastError());
+      k_trace_resume_list<<<(unsigned)((2 * nlong + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>(
+          ctx->geom, ctx->d_groups, ngroups, ctx->d_blk, (int)nlong, ctx->d_off2, slot, ctx->d_nstep_ray,
+          ctx->d_traj2, ctx->d_status);
+      HIPCHK(ctx, hipGetLastError());
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, items.data(), nitems * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    if (nitems > 0) {
+      auto kx = jobs[0].image.transfer_fp32 ? k_xfer_batch<true> : k_xfer_batch<false>;
+      kx<<<(nitems + WAVE - 1) / WAVE, WAVE, 0, ctx->stream>>>(
+          ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items, nitems, ctx->d_off, ctx->d_traj,
+          ctx->d_traj2, ctx->d_nstep_ray, nullptr, ctx->d_dump, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
+          ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts);
+      HIPCHK(ctx, hipGetLastError());
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+    int mism = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&mism, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (mism) return set_err(ctx, IPGPU_ERR_HIP, "geodesic step counts differ between the two trace passes");
+    hipEventElapsedTime(&t, ctx->ev[1], ctx->ev[2]);
+    ms_geo += t;
+    hipEventElapsedTime(&t, ctx->ev[4], ctx->ev[3]);
+    ms_xfer += t;
+    nvalid_done = nvalid;
+    steps_host = slot_steps;
+  }
+  for (long long a = nvalid_done; a < nvalid;) {
     /* the longest prefix of the remaining rays whose records fit */
     long long b = a, recs = 0;
     while (b < nvalid && (b == a || recs + ns_host[order[b]] + 1 <= budget)) {
@@ -2488,10 +2670,15 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_blk, order.data() + a, nr * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    k_trace_store<<<(nr + WAVE - 1) / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->d_groups, ngroups, ctx->d_blk, nr,
-                                                                   ctx->d_off, ctx->d_nstep_ray, ctx->d_traj,
-                                                                   turns ? ctx->d_nturns : nullptr, ctx->d_status,
-                                                                   ctx->d_counts);
+    if (pairs)
+      k_trace_store_pair<<<(2 * nr + WAVE - 1) / WAVE, WAVE, 0, ctx->stream>>>(
+          ctx->geom, ctx->d_groups, ngroups, ctx->d_blk, nr, ctx->d_off, ctx->d_nstep_ray, ctx->d_traj,
+          ctx->d_status, ctx->d_counts);
+    else
+      k_trace_store<<<(nr + WAVE - 1) / WAVE, WAVE, 0, ctx->stream>>>(ctx->geom, ctx->d_groups, ngroups, ctx->d_blk,
+                                                                     nr, ctx->d_off, ctx->d_nstep_ray, ctx->d_traj,
+                                                                     turns ? ctx->d_nturns : nullptr, ctx->d_status,
+                                                                     ctx->d_counts);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     /* work items (job, ray) of these rays, in the rays' (longest first) order */
@@ -2508,7 +2695,7 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     if (nitems > 0) {
       auto kx = jobs[0].image.transfer_fp32 ? k_xfer_batch<true> : k_xfer_batch<false>;
       kx<<<(nitems + WAVE - 1) / WAVE, WAVE, 0, ctx->stream>>>(
-          ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items, nitems, ctx->d_off, ctx->d_traj,
+          ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items, nitems, ctx->d_off, ctx->d_traj, nullptr,
           ctx->d_nstep_ray, turns ? ctx->d_nturns : nullptr, ctx->d_dump, ctx->d_bimage, ctx->d_bimageS,
           ctx->d_btaus, ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts);
       HIPCHK(ctx, hipGetLastError());
@@ -2529,6 +2716,7 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
   HIPCHK(ctx, hipMemcpyAsync(bst, ctx->d_bstatus, sizeof(bst), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(ctx->counts, ctx->d_counts, sizeof(ctx->counts), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->counts[0] += (unsigned long long)steps_host;   /* the rays whose records pass 1 stored */
   hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[3]);
   ctx->ms_geo = ms_geo;
   ctx->ms_xfer = ms_xfer;

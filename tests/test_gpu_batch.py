@@ -85,6 +85,63 @@ def test_batch_vs_oracle(setup, traj_mib):
     c.upload_prims(prims, G.iharm_units(M_UNIT))
 
 
+def test_batch_chunks_identical(setup):
+    """With little trajectory memory the rays are counted, then traced and
+    transferred in chunks; with much, their records stay where the first
+    pass put them: the images are the same."""
+    h, prims, c, g, m = setup
+    say("case batch chunking: 1 MiB, 3 MiB, 48 GiB of trajectory memory")
+    jobs = []
+    for (th, nx, ny, f, s, unpol, et) in JOBS:
+        p = O.default_params(nx=nx, ny=ny, thetacam=th, freqcgs=f, only_unpolarized=unpol, emission_type=et)
+        im, _, _ = G.make_image(g, m.L_unit, **gpu_image_kwargs(p))
+        jobs.append((im, s))
+    outs = []
+    for mib in (1, 3, 48 * 1024):
+        c.set_batch_memory(mib << 20)
+        outs.append(c.render_batch(jobs))
+    c.set_batch_memory(48 << 30)
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            for k in ("image", "imageS", "taus", "nstep", "flag"):
+                if a.get(k) is not None:
+                    assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("slot", [64, 301])
+def test_batch_slot_records(setup, slot, monkeypatch):
+    """The first trace pass stores each ray's first `slot` records in a slot
+    of its own; the rays longer than that are continued on lane pairs from
+    their last slot record in a full-length region.  The images equal those
+    of the count-then-store passes (a small trajectory memory forces them)."""
+    h, prims, c, g, m = setup
+    say("case batch slot records, slot %d" % slot)
+    jobs = []
+    for (th, nx, ny, f, s, unpol, et) in JOBS:
+        p = O.default_params(nx=nx, ny=ny, thetacam=th, freqcgs=f, only_unpolarized=unpol, emission_type=et)
+        im, _, _ = G.make_image(g, m.L_unit, **gpu_image_kwargs(p))
+        jobs.append((im, s))
+    c.set_batch_memory(1 << 20)
+    ref = c.render_batch(jobs)
+    steps_ref = c.step_counts()[0]
+    c.set_batch_memory(48 << 30)
+    monkeypatch.setenv("IPGPU_BATCH_SLOT", str(slot))
+    c2 = G.Context(0)
+    try:
+        c2.set_geometry(g)
+        c2.set_model(m)
+        c2.upload_prims(prims, G.iharm_units(M_UNIT))
+        got = c2.render_batch(jobs)
+        assert max(int(o["nstep"].max()) for o in got) >= slot      # some rays continued
+        for a, b in zip(ref, got):
+            for k in ("image", "imageS", "taus", "nstep", "flag"):
+                if a.get(k) is not None:
+                    assert np.array_equal(a[k], b[k]), k
+        assert c2.step_counts()[0] == steps_ref
+    finally:
+        c2.close()
+
+
 def test_batch_maxnstep(setup):
     h, prims, c, g, m = setup
     say("case batch maxnstep")
