@@ -135,13 +135,16 @@ def test_streamed_record_overflow(case):
     c2.close()
 
 
-def test_streamed_maxnstep(case):
-    """A resumed ray past maxnstep is reported with the one-pass pixel."""
+@pytest.mark.parametrize("seg_min", [3584, 20])
+def test_streamed_maxnstep(case, seg_min):
+    """A resumed ray past maxnstep is reported with the one-pass pixel, also
+    when every ray of the last phase's list is composed from segments (that
+    list runs the composing chain kernel alone)."""
     h, prims, orc = case
-    say("case streamed maxnstep 8x8")
+    say("case streamed maxnstep 8x8 seg_min %d" % seg_min)
     errs = []
     for budget in (0, 20):
-        c, g, m = fresh(h, prims, trace_budget=budget)
+        c, g, m = fresh(h, prims, trace_budget=budget, seg_min_steps=seg_min, seg_len=16)
         im, _, _ = G.make_image(g, m.L_unit, nx=8, ny=8, thetacam=163., maxnstep=50)
         with pytest.raises(G.IpgpuError) as e:
             c.render(im)
