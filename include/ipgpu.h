@@ -184,8 +184,11 @@ typedef struct ipgpu_job {
  * fast-light pixel loop of src/main.c:536-615 per job).  Jobs with the same
  * camera, image size and integration options share one geodesic trace at
  * the frequency of the first of them (the geodesic in X does not depend on
- * the frequency).  Geodesics are traced twice: once to count the steps,
- * then, longest first, into exact slices of one compact record buffer; the
+ * the frequency).  A first trace counts every ray's steps and, when a slot
+ * of records per ray fits the batch memory, keeps its first records there,
+ * so that only the longer rays are continued into exact regions; otherwise
+ * the rays are traced again, longest first, into exact slices of one
+ * compact record buffer, in as many passes as the memory requires.  The
  * transfer of every (job, ray) pair then runs in one fused launch, longest
  * rays first.  Results agree with per-job ipgpu_render to
  * the roundoff of the rescaling.  On IPGPU_ERR_MAXNSTEP the job and pixel
@@ -194,8 +197,10 @@ typedef struct ipgpu_job {
 int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs);
 int ipgpu_bad_job(const ipgpu_ctx *ctx, int *job, int *i, int *j);
 
-/* Device memory a batch may use for geodesic records per pass (default
- * 160 GiB); rays are processed in as many passes as this requires. */
+/* Device memory a batch may use for geodesic records (default 160 GiB):
+ * records kept in place from the first trace when a slot per ray plus the
+ * longer rays' regions fit, else exact slices in as many passes as this
+ * requires. */
 int ipgpu_set_batch_memory(ipgpu_ctx *ctx, size_t traj_bytes);
 
 /* Gradient of the image loss
