@@ -9,7 +9,10 @@
   the sample's summed I within 1e-6;
 * the streamed render (bounded phases, finished rays on streams of their
   own, the longest rays' first steps composed during the last phase)
-  against the one-pass render of the same image, every output bit for bit.
+  against the one-pass render of the same image, every output bit for bit;
+* a batch of three jobs on the same dump (its first trace keeps the records
+  in per-ray slots and continues the rays longer than a slot), FP64 and
+  mixed precision, against each job's plain render.
 """
 import sys
 
@@ -86,3 +89,40 @@ def test_fullsize_streamed_equals_one_pass(full):
     for k in ("image", "imageS", "taus", "nstep", "flag"):
         assert np.array_equal(streamed[k], one[k]), k
     assert int(streamed["nstep"].max()) > 10000
+
+
+def test_fullsize_batch_vs_plain(full):
+    """Three jobs on the full dump in one batch (two frequencies and M_unit
+    scales of the 160x160 camera, one 96x96 camera at 50 degrees), FP64 and
+    mixed precision, against the plain render of each job with the dump
+    derived at its M_unit: FP64 nstep identical, |dFtot|/Ftot < 1e-9, pixels
+    within 1e-6; mixed |dFtot|/Ftot within the mode's bar of 1e-4."""
+    orc, p, c, im = full
+    h, prims = synth.make_dump(*N, a=A, seed=SEED)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    specs = [(160, 163., 230e9, 1.0), (160, 163., 86e9, 0.4), (96, 50., 345e9, 2.5)]
+    say("full-size batch of %d jobs vs plain" % len(specs))
+    plain = []
+    for n, th, f, s in specs:
+        c.upload_prims(prims, G.iharm_units(M_UNIT * s, trat_large=TRAT))
+        ji, _, _ = G.make_image(g, m.L_unit, nx=n, ny=n, thetacam=th, freqcgs=f)
+        plain.append(c.render(ji))
+    c.upload_prims(prims, G.iharm_units(M_UNIT, trat_large=TRAT))
+    for f32 in (0, 1):
+        jobs = []
+        for n, th, f, s in specs:
+            ji, _, _ = G.make_image(g, m.L_unit, nx=n, ny=n, thetacam=th, freqcgs=f, transfer_fp32=f32)
+            jobs.append((ji, s))
+        got = c.render_batch(jobs)
+        for k, (ref, out) in enumerate(zip(plain, got)):
+            assert np.array_equal(ref["nstep"], out["nstep"])
+            dF = abs(out["imageS"][..., 0].sum() / ref["imageS"][..., 0].sum() - 1)
+            say("  %s job %d: dFtot %.2e, longest ray %d" % ("mixed" if f32 else "fp64", k, dF,
+                                                              int(out["nstep"].max())))
+            if f32:
+                assert dF < 1e-4
+            else:
+                assert dF < 1e-9
+                for name, (mx, frac) in stokes_errors(ref["imageS"], ref["image"], out["imageS"],
+                                                      out["image"]).items():
+                    assert mx < 1e-6, (k, name, mx)
