@@ -1467,24 +1467,34 @@ IPG_HD bool herm_nonzero(const Herm &N)
   return !z;
 }
 
-/* Nout = Nin - dl (M Nm + Nm M^T): push_polar, src/ipolarray.c:253-289 */
+/* Nout = Nin - dl (M Nm + Nm M^T): push_polar, src/ipolarray.c:253-289.
+ * Each component accumulates as a chain of fused multiply-adds, and the
+ * terms with the diagonal's zero imaginary part are left out (a product
+ * with a zero operand cannot be dropped by the compiler under IEEE rules). */
 IPG_HD void transport(const Herm &Nin, const Herm &Nm, const double M[4][4], double dl, Herm &Nout)
 {
+#pragma unroll
   for (int i = 0; i < 4; i++)
+#pragma unroll
     for (int j = i; j < 4; j++) {
       double sr = 0., si = 0.;
+#pragma unroll
       for (int k = 0; k < 4; k++) {
         double ar, ai, br, bi;
         herm_get(Nm, k, j, ar, ai);
         herm_get(Nm, i, k, br, bi);
-        sr += M[i][k] * ar + M[j][k] * br;
-        si += M[i][k] * ai + M[j][k] * bi;
+        sr = fma(M[i][k], ar, sr);
+        sr = fma(M[j][k], br, sr);
+        if (i != j) {
+          if (k != j) si = fma(M[i][k], ai, si);
+          if (k != i) si = fma(M[j][k], bi, si);
+        }
       }
       if (i == j) {
-        Nout.d[i] = Nin.d[i] - dl * sr;
+        Nout.d[i] = fma(-dl, sr, Nin.d[i]);
       } else {
-        Nout.re[off(i, j)] = Nin.re[off(i, j)] - dl * sr;
-        Nout.im[off(i, j)] = Nin.im[off(i, j)] - dl * si;
+        Nout.re[off(i, j)] = fma(-dl, sr, Nin.re[off(i, j)]);
+        Nout.im[off(i, j)] = fma(-dl, si, Nin.im[off(i, j)]);
       }
     }
 }
