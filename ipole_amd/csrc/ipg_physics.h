@@ -1501,23 +1501,36 @@ IPG_HD void transport(const Herm &Nin, const Herm &Nm, const double M[4][4], dou
 
 /* Stokes parameters of N in a tetrad with covariant rows e1 = Ecov[1],
  * e2 = Ecov[2] (complex_coord_to_tetrad_rank2 + tensor_to_stokes,
- * src/ipolarray.c:684-694, :710-728, restricted to the [1..2] block). */
+ * src/ipolarray.c:684-694, :710-728, restricted to the [1..2] block):
+ * T11 = e1.Re(N).e1, T22 = e2.Re(N).e2, Re T12 = e2.Re(N).e1 and
+ * Im T12 = e1.Im(N).e2, as matrix-vector products in fused multiply-adds. */
 IPG_HD void herm_to_stokes(const Herm &N, const double e1[4], const double e2[4], double S[4])
 {
-  double t11 = 0., t22 = 0., r12 = 0., i12 = 0.;
+  double w1[4], w2[4], u2[4];
+#pragma unroll
   for (int k = 0; k < 4; k++) {
-    t11 += N.d[k] * e1[k] * e1[k];
-    t22 += N.d[k] * e2[k] * e2[k];
-    r12 += N.d[k] * e1[k] * e2[k];
-  }
-  for (int k = 0; k < 4; k++)
-    for (int l = k + 1; l < 4; l++) {
-      double re = N.re[off(k, l)], im = N.im[off(k, l)];
-      t11 += 2. * re * e1[k] * e1[l];
-      t22 += 2. * re * e2[k] * e2[l];
-      r12 += re * (e1[k] * e2[l] + e1[l] * e2[k]);
-      i12 += im * (e1[k] * e2[l] - e1[l] * e2[k]);
+    double a1 = N.d[k] * e1[k], a2 = N.d[k] * e2[k], b2 = 0.;
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      if (l == k) continue;
+      double re, im;
+      herm_get(N, k, l, re, im);
+      a1 = fma(re, e1[l], a1);
+      a2 = fma(re, e2[l], a2);
+      b2 = fma(im, e2[l], b2);
     }
+    w1[k] = a1;
+    w2[k] = a2;
+    u2[k] = b2;
+  }
+  double t11 = e1[0] * w1[0], t22 = e2[0] * w2[0], r12 = e2[0] * w1[0], i12 = e1[0] * u2[0];
+#pragma unroll
+  for (int k = 1; k < 4; k++) {
+    t11 = fma(e1[k], w1[k], t11);
+    t22 = fma(e2[k], w2[k], t22);
+    r12 = fma(e2[k], w1[k], r12);
+    i12 = fma(e1[k], u2[k], i12);
+  }
   S[0] = (t11 + t22) / 2;
   S[1] = (t11 - t22) / 2;
   S[2] = r12;
@@ -1526,15 +1539,28 @@ IPG_HD void herm_to_stokes(const Herm &N, const double e1[4], const double e2[4]
 
 /* N += Econ^T dT Econ for a Stokes change dS in the [1..2] tetrad block
  * (stokes_to_tensor + complex_tetrad_to_coord_rank2, src/ipolarray.c:673-682,
- * :730-748, applied as a difference). */
+ * :730-748, applied as a difference): with a = dI + dQ, b = dI - dQ, u = dU,
+ * v = dV, Re dN_kl = p_k E1_l + q_k E2_l (p = a E1 + u E2, q = b E2 + u E1)
+ * and Im dN_kl = v (E2_k E1_l - E1_k E2_l). */
 IPG_HD void herm_add_stokes(Herm &N, const double E1[4], const double E2[4], const double dS[4])
 {
-  double a = dS[0] + dS[1], b = dS[0] - dS[1], u = dS[2], v = dS[3];
-  for (int k = 0; k < 4; k++) N.d[k] += a * E1[k] * E1[k] + b * E2[k] * E2[k] + 2. * u * E1[k] * E2[k];
+  const double a = dS[0] + dS[1], b = dS[0] - dS[1], u = dS[2], v = dS[3];
+  double pk[4], qk[4], vE1[4], vE2[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    pk[k] = fma(u, E2[k], a * E1[k]);
+    qk[k] = fma(u, E1[k], b * E2[k]);
+    vE1[k] = v * E1[k];
+    vE2[k] = v * E2[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) N.d[k] = fma(pk[k], E1[k], fma(qk[k], E2[k], N.d[k]));
+#pragma unroll
   for (int k = 0; k < 4; k++)
+#pragma unroll
     for (int l = k + 1; l < 4; l++) {
-      N.re[off(k, l)] += a * E1[k] * E1[l] + b * E2[k] * E2[l] + u * (E1[k] * E2[l] + E2[k] * E1[l]);
-      N.im[off(k, l)] += v * (E2[k] * E1[l] - E1[k] * E2[l]);
+      N.re[off(k, l)] = fma(pk[k], E1[l], fma(qk[k], E2[l], N.re[off(k, l)]));
+      N.im[off(k, l)] = fma(vE2[k], E1[l], fma(-vE1[k], E2[l], N.im[off(k, l)]));
     }
 }
 
