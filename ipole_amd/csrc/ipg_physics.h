@@ -327,18 +327,39 @@ IPG_HD double dot4(const double a[4], const double b[4])
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
 }
 
+/* Index lowering with the metrics metric_t forms (Kerr-Schild-based MKS and
+ * FMKS, and Minkowski), which have g_02 = g_23 = 0 (so have their X1, X2
+ * derivatives) and g^03 = 0: the products with these zeros are left out
+ * (the compiler may not drop a product with a zero under IEEE rules; the sums
+ * are those of lower() up to the sign of an exact zero). */
+IPG_HD void lower_g(const double g[10], const double u[4], double l[4])
+{
+  l[0] = g[0] * u[0] + g[1] * u[1] + g[3] * u[3];
+  l[1] = g[1] * u[0] + g[4] * u[1] + g[5] * u[2] + g[6] * u[3];
+  l[2] = g[5] * u[1] + g[7] * u[2];
+  l[3] = g[3] * u[0] + g[6] * u[1] + g[9] * u[3];
+}
+
+IPG_HD void raise_gc(const double gc[10], const double u[4], double l[4])
+{
+  l[0] = gc[0] * u[0] + gc[1] * u[1] + gc[2] * u[2];
+  l[1] = gc[1] * u[0] + gc[4] * u[1] + gc[5] * u[2] + gc[6] * u[3];
+  l[2] = gc[2] * u[0] + gc[5] * u[1] + gc[7] * u[2] + gc[8] * u[3];
+  l[3] = gc[6] * u[1] + gc[8] * u[2] + gc[9] * u[3];
+}
+
 /* acc^i = Gamma^i_jk K^j K^k (the contraction push_photon forms,
  * src/geodesics.c:22-26) from dg/dX1, dg/dX2. */
 IPG_HD void geodesic_accel(const Metric &m, const double K[4], double acc[4])
 {
   double v1[4], v2[4];
-  lower(m.d1, K, v1);
-  lower(m.d2, K, v2);
+  lower_g(m.d1, K, v1);
+  lower_g(m.d2, K, v2);
   double al[4];
   for (int l = 0; l < 4; l++) al[l] = K[1] * v1[l] + K[2] * v2[l];
   al[1] -= 0.5 * dot4(v1, K);
   al[2] -= 0.5 * dot4(v2, K);
-  lower(m.gc, al, acc);
+  raise_gc(m.gc, al, acc);
 }
 
 /* M^i_k = Gamma^i_kl K^l, the matrix parallel transport of N applies
@@ -346,22 +367,26 @@ IPG_HD void geodesic_accel(const Metric &m, const double K[4], double acc[4])
 IPG_HD void transport_matrix(const Metric &m, const double K[4], double M[4][4])
 {
   double v1[4], v2[4];
-  lower(m.d1, K, v1);
-  lower(m.d2, K, v2);
+  lower_g(m.d1, K, v1);
+  lower_g(m.d2, K, v2);
   double Ml[4][4];
   for (int l = 0; l < 4; l++)
     for (int k = 0; k < 4; k++) {
-      double t = K[1] * m.d1[sy(l, k)] + K[2] * m.d2[sy(l, k)];
+      /* dg_02, dg_23 vanish (lower_g) */
+      const bool z = sy(l, k) == 2 || sy(l, k) == 8;
+      double t = z ? 0. : K[1] * m.d1[sy(l, k)] + K[2] * m.d2[sy(l, k)];
       if (k == 1) t += v1[l];
       if (k == 2) t += v2[l];
       if (l == 1) t -= v1[k];
       if (l == 2) t -= v2[k];
       Ml[l][k] = 0.5 * t;
     }
-  for (int i = 0; i < 4; i++)
-    for (int k = 0; k < 4; k++)
-      M[i][k] = m.gc[sy(i, 0)] * Ml[0][k] + m.gc[sy(i, 1)] * Ml[1][k] + m.gc[sy(i, 2)] * Ml[2][k]
-          + m.gc[sy(i, 3)] * Ml[3][k];
+  for (int k = 0; k < 4; k++) {
+    const double col[4] = {Ml[0][k], Ml[1][k], Ml[2][k], Ml[3][k]};
+    double up[4];
+    raise_gc(m.gc, col, up);
+    for (int i = 0; i < 4; i++) M[i][k] = up[i];
+  }
 }
 
 /* ------------------------------------------------------------- geodesics */
