@@ -1229,6 +1229,9 @@ struct ipgpu_ctx {
   /* batches: records per ray of the first trace pass; the full-length
    * records of the rays longer than that */
   int batch_slot = 2048;
+  int2 *h_items = nullptr;     /* pinned (job, ray) items of the in-place path */
+  size_t h_items_cap = 0;
+  std::vector<int> h_gid;
   double *d_traj2 = nullptr;
   size_t traj2_doubles = 0;
   long long *d_off2 = nullptr;
@@ -1413,6 +1416,7 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
   for (auto st : ctx->streamF)
     if (st) hipStreamSynchronize(st);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
+  if (ctx->h_items) hipHostFree(ctx->h_items);
   for (auto *S : {&ctx->segset[0], &ctx->segset[1], &ctx->segset[2], &ctx->segset[3], &ctx->segset[4],
                   &ctx->segpre}) {
     void *sp[] = {S->d_off, S->d_q0, S->d_q1, S->d_flag, S->d_map};
@@ -2608,12 +2612,28 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
         off_host[r] = (long long)r * slot;
       }
     }
-    for (long long q = 0; q < nvalid; q++) {
-      int r = order[q];
-      int g = group_of_host(groups, r);
-      for (int jb : members[g]) items.push_back(make_int2(jb, r));
+    /* the (job, ray) items, built in pinned host memory (the upload is then
+     * a direct copy); every ray's group from a table of the groups' ranges */
+    std::vector<int> &gid = ctx->h_gid;
+    gid.resize(rays_total);
+    for (int g = 0; g < ngroups; g++)
+      for (long long r = groups[g].ray_base; r < groups[g].ray_base + groups[g].nrays; r++) gid[r] = g;
+    long long nit = 0;
+    for (long long q = 0; q < nvalid; q++) nit += (long long)members[gid[order[q]]].size();
+    if ((size_t)nit > ctx->h_items_cap) {
+      if (ctx->h_items) hipHostFree(ctx->h_items);
+      ctx->h_items = nullptr;
+      ctx->h_items_cap = 0;
+      HIPCHK(ctx, hipHostMalloc((void **)&ctx->h_items, (size_t)nit * sizeof(int2), hipHostMallocDefault));
+      ctx->h_items_cap = (size_t)nit;
     }
-    const int nitems = (int)items.size();
+    int2 *hit = ctx->h_items;
+    long long w = 0;
+    for (long long q = 0; q < nvalid; q++) {
+      const int r = order[q];
+      for (int jb : members[gid[r]]) hit[w++] = make_int2(jb, r);
+    }
+    const int nitems = (int)nit;
     if ((rc = grow(ctx, ctx->d_items, ctx->cap_items, (size_t)(nitems > 0 ? nitems : 1)))) return rc;
     if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)(nlong > 0 ? nlong : 1)))) return rc;
     if ((rc = grow(ctx, ctx->d_off2, ctx->cap_off2, (size_t)(nlong > 0 ? nlong : 1)))) return rc;
@@ -2632,7 +2652,7 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
       HIPCHK(ctx, hipGetLastError());
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, items.data(), nitems * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, hit, nitems * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
     if (nitems > 0) {
       auto kx = jobs[0].image.transfer_fp32 ? k_xfer_batch<true> : k_xfer_batch<false>;
