@@ -12,6 +12,18 @@
  * cosh(nu t) dt, which converges geometrically for this entire integrand.
  * K_2 follows by the upward recurrence K_2 = K_0 + (2/x) K_1 as GSL does.
  * Pinned in tests against scipy.special.kn and mpmath.besselk.
+ *
+ * How the fits are pinned (tests/test_oracle_fits.py, test_oracle_math.py):
+ * - j_I, j_Q, j_V (Dexter and Pandya) against the exact thermal integrals of
+ *   the ultrarelativistic single-electron kernels F, G and (4/3)(cot th/g) H,
+ *   which the tests derive from the radiation amplitudes of a helical orbit;
+ * - rho_Q, rho_V at Theta_e 3-30 in the high-frequency limit against the
+ *   relativistic thermal closed forms (K0/K2, K1/K2 + 6 Theta), and in the cold
+ *   limit; Kirchhoff's law for the absorptivities;
+ * - the finite-X shape functions f(X) of rho_Q and g(X) of rho_V, which are
+ *   fits to numerical results with no closed form, only against the
+ *   reference's source text (every literal and arithmetic operator of each
+ *   fit function, in order); the reference build cannot compile them (GSL).
  */
 #include "oracle.h"
 #include <math.h>

@@ -33,6 +33,11 @@ def test_gold2020_fluxes(model):
 # frequency [Hz], inclination [deg], Ftot, Ftot_unpol  (thermal.txt rows)
 SPHERE_ROWS = [
     (30419100000.0, 90.0, 31.3027514002403, 32.14457457894139),
+    # optically thick rows, where polarized transfer (Dexter j, Kirchhoff a_Q,
+    # a_V, Faraday rho_Q, rho_V) takes Ftot to 0.76-0.90 of Ftot_unpol
+    (39400900000.0, 90.0, 48.134475453534684, 53.32846962863743),
+    (49731200000.0, 90.0, 65.02591765835368, 81.08815286512608),
+    (65253900000.0, 90.0, 85.1660303167946, 111.54531719510815),
     (223005000000.0, 90.0, 20.307140822627037, 20.60142287648152),
     (603808000000.0, 90.0, 0.8517082421646595, 0.8580307947514274),
     (223005000000.0, 40.0, 3.8639049623979367, 3.900997941745764),
