@@ -106,14 +106,14 @@ def test_working_tolerances_vs_oracle(setup, cam):
     say("  traced per level %s" % a["traced"])
     I = np.abs(o["image"])
     e = np.abs(a["image"] - o["image"]) / np.maximum(I, 1e-6 * I.max())
-    assert e.max() < 1e-4 and np.mean(e > 1e-6) < 0.01
+    assert e.max() < 1e-5 and np.mean(e > 1e-6) < 0.01
     F_o, F_a = o["image"].sum() * scale, a["image"].sum() * scale
     assert abs(F_a / F_o - 1) < 1e-6
     if not cam:
         Io = np.abs(o["imageS"][..., 0])
         for k in range(4):
             e = np.abs(a["imageS"][..., k] - o["imageS"][..., k]) / np.maximum(Io, 1e-6 * Io.max())
-            assert e.max() < 1e-4 and np.mean(e > 1e-6) < 0.01, k
+            assert e.max() < 1e-5 and np.mean(e > 1e-6) < 0.01, k
 
 
 @pytest.mark.parametrize("nearest,unpol", [(0, 0), (1, 1)])

@@ -5,7 +5,7 @@
 * against the oracle on a strided sample of its pixels (every 8th row and
   column, 400 rays, the photon-ring rays of ~10^4 steps among them), at the
   small cases' parity bar: nstep identical, per pixel |dS| / max(|I|, 1e-6
-  max I) below 1e-4 everywhere and above 1e-6 on at most 1% of the pixels,
+  max I) below 1e-5 everywhere and above 1e-6 on at most 1% of the pixels,
   the sample's summed I within 1e-6;
 * the streamed render (bounded phases, finished rays on streams of their
   own, the longest rays' first steps composed during the last phase)
@@ -72,7 +72,7 @@ def test_fullsize_sample_vs_oracle(full):
     errs = stokes_errors(ref_S, ref_I, got_S, got_I)
     say("  errors %s" % errs)
     for k, (mx, frac) in errs.items():
-        assert mx < 1e-4 and frac <= 1e-2, (k, mx, frac)
+        assert mx < 1e-5 and frac <= 1e-2, (k, mx, frac)
 
 
 def test_fullsize_streamed_equals_one_pass(full):

@@ -45,8 +45,8 @@ def test_host_iharm(tmp_path):
     F_ref = ref["imageS"][..., 0].sum() * sc
     assert abs(img["Ftot"] / F_ref - 1) < 1e-6
     assert abs(float(m.group(2)) / F_ref - 1) < 1e-5   # printed with %g
-    assert np.allclose(img["imageS"], ref["imageS"], rtol=1e-4, atol=1e-6 * np.abs(ref["imageS"][..., 0]).max())
-    assert np.allclose(img["image"], ref["image"], rtol=1e-4, atol=1e-6 * ref["image"].max())
+    assert np.allclose(img["imageS"], ref["imageS"], rtol=1e-5, atol=1e-6 * np.abs(ref["imageS"][..., 0]).max())
+    assert np.allclose(img["image"], ref["image"], rtol=1e-5, atol=1e-6 * ref["image"].max())
 
 
 def test_host_unpol_quench_and_maxnstep(tmp_path):
@@ -95,6 +95,6 @@ def test_host_kharma(tmp_path, fmt, electrons):
     ref = orc.render(O.default_params(nx=24, ny=24, thetacam=150., freqcgs=230e9))
     F_ref = ref["imageS"][..., 0].sum() * ref["cam"].scale
     assert abs(img["Ftot"] / F_ref - 1) < 1e-6
-    assert np.allclose(img["imageS"], ref["imageS"], rtol=1e-4, atol=1e-6 * np.abs(ref["imageS"][..., 0]).max())
+    assert np.allclose(img["imageS"], ref["imageS"], rtol=1e-5, atol=1e-6 * np.abs(ref["imageS"][..., 0]).max())
     assert h5.read_dataset(str(tmp_path / "img.h5"), "/fluid_header/geom/fmks/a") == h["a"]
     assert h5.read_dataset(str(tmp_path / "img.h5"), "/fluid_header/nx1_mb") == 12

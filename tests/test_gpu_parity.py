@@ -9,7 +9,7 @@ the finite-difference round-off (~1e-9 in Gamma), which the geodesics carry
 to the image.  The reference algorithm moves by the same amount between two
 builds of itself (tests/test_oracle_spread.py: up to ~6e-6 in Q/U on 0.7% of
 pixels).  So: Ftot within 1e-6, nstep identical, at most 1% of pixels above
-1e-6 and none above 1e-4.
+1e-6 and none above 1e-5 (the spread's maximum, rounded up).
 """
 import sys
 
@@ -23,7 +23,7 @@ from ipole_amd import ipgpu as G
 pytestmark = pytest.mark.gpu
 
 PIX_TOL = 1e-6
-PIX_CEIL = 1e-4
+PIX_CEIL = 1e-5
 FTOT_TOL = 1e-6
 
 
@@ -81,10 +81,12 @@ def check(ref, got, scale, unpol=False, floor=None):
 
 @pytest.mark.parametrize("unpol", [0, 1])
 def test_sphere(ctx, unpol):
-    say("case sphere unpol=%d 32x32" % unpol)
+    """The isothermal sphere at its benchmark configuration: 64x64, 100 r_g
+    field of view, thetacam 60, 230 GHz, polarized and -unpol."""
+    say("case sphere unpol=%d 64x64" % unpol)
     go, mo = O.sphere_model(R0=50., tau0=2.0143e-5, Thetae0=10., beta0=30.)
     orc = O.Oracle(go, mo)
-    p = O.default_params(nx=32, ny=32, thetacam=60., dx=100., dy=100., freqcgs=230e9, only_unpolarized=unpol)
+    p = O.default_params(nx=64, ny=64, thetacam=60., dx=100., dy=100., freqcgs=230e9, only_unpolarized=unpol)
     ref = orc.render(p)
     gs, ms = G.sphere_setup(R0=50., tau0=2.0143e-5, Thetae0=10., beta0=30.)
     ctx.set_geometry(gs)
@@ -123,6 +125,32 @@ def test_iharm(ctx, dump_small, thetacam, unpol, etype):
     im, scale, _ = G.make_image(g, m.L_unit, **gpu_image_kwargs(p))
     got = ctx.render(im)
     check(ref, got, scale, unpol=bool(unpol))
+
+
+# image options the host accepts as par keys (src/par.c:139-210): the
+# counterjet selection (src/ipolarray.c:122-135), the Stokes floors of
+# evolve_N (src/ipolarray.c:524-543), the old camera centering
+# (src/model_geodesics.c:186-196), the QU sign convention and the camera
+# rotation (src/main.c:895-897, src/ipolarray.c:595-597) and the pixel
+# offsets of the image plane (src/model_geodesics.c:192-197)
+OPTIONS = [dict(counterjet=1), dict(counterjet=2), dict(stokes_floors=1), dict(old_centering=1),
+           dict(qu_conv=1), dict(xoff=3.5, yoff=-2.25), dict(rotcam=25.)]
+
+
+@pytest.mark.parametrize("opts", OPTIONS, ids=lambda o: "-".join("%s=%g" % kv for kv in o.items()))
+def test_iharm_options(ctx, dump_small, opts):
+    h, prims, orc, fields = dump_small
+    say("case iharm options %s 24x24 th=60" % opts)
+    orc.activate()
+    p = O.default_params(nx=24, ny=24, thetacam=60., freqcgs=230e9, **opts)
+    ref = orc.render(p)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    ctx.set_geometry(g)
+    ctx.set_model(m)
+    ctx.upload_prims(prims, G.iharm_units(3e26))
+    im, scale, _ = G.make_image(g, m.L_unit, **gpu_image_kwargs(p))
+    got = ctx.render(im)
+    check(ref, got, scale)
 
 
 def test_iharm_upload_fields(ctx, dump_small):

@@ -159,6 +159,30 @@ def test_emu_iharm(thetacam, etype, unpol):
         assert np.mean(e > 1e-6) < 1e-3, (k, np.mean(e > 1e-6))
 
 
+@pytest.mark.parametrize("opts", [dict(counterjet=1), dict(counterjet=2), dict(stokes_floors=1),
+                                  dict(old_centering=1), dict(xoff=3.5, yoff=-2.25), dict(rotcam=25.)],
+                         ids=lambda o: "-".join("%s=%g" % kv for kv in o.items()))
+def test_emu_iharm_options(opts):
+    """Image options (counterjet, Stokes floors, old centering, pixel offsets,
+    camera rotation) through the device code built for the CPU, against the
+    oracle; qu_conv acts in the output layout only (k_finalize), which the GPU
+    tests cover."""
+    h, prims = small_dump()
+    orc, fields = O.iharm_oracle(h, prims, M_unit=3e26)
+    p = O.default_params(nx=16, ny=16, thetacam=60., freqcgs=230e9, **opts)
+    ref = orc.render(p)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    kw = dict(opts)
+    im, scale, _ = G.make_image(g, m.L_unit, nx=16, ny=16, thetacam=60., freqcgs=230e9, **kw)
+    rc, out7, ns, fl = emu.render(g, m, im, emu.pack_dump(fields))
+    assert rc == 0 and np.array_equal(ns, ref["nstep"].ravel())
+    F, errs = image_errors(ref, out7, 230e9 ** 3, scale)
+    assert F < 1e-6
+    for k, e in errs.items():
+        assert e.max() < 1e-5, (k, e.max())
+        assert np.mean(e > 1e-6) < 1e-2, (k, np.mean(e > 1e-6))
+
+
 def test_pipeline_log_pow_cbrt_vs_mpmath():
     """log_, cbrt_ and pow_ of ipg_math.h (the thermal fits' powers and
     logarithms) against mpmath: log within 4 ulp of max(|log x|, ln 2 / 2^10)
