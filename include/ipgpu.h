@@ -203,6 +203,21 @@ int ipgpu_bad_job(const ipgpu_ctx *ctx, int *job, int *i, int *j);
  * requires. */
 int ipgpu_set_batch_memory(ipgpu_ctx *ctx, size_t traj_bytes);
 
+/* Device memory a single-image render (ipgpu_render) may use for its ray
+ * records and step packets (default 64 GiB).  An image whose rays need more
+ * is rendered in chunks of rays (whole 8x8 pixel tiles) that fit; the image
+ * does not depend on the chunking.  The reference needs one trajectory of
+ * maxnstep records per thread (src/main.c:835); this bounds the GPU's
+ * equivalent, which holds the records of every ray of a chunk at once. */
+int ipgpu_set_render_memory(ipgpu_ctx *ctx, size_t bytes);
+
+/* Work layout of the last ipgpu_render: chunks of rays it ran in, rays whose
+ * trace outgrew the per-ray record capacity and continued in full-length
+ * regions (the capacity follows the longest ray of the previous render), and
+ * the device memory the context holds (dump, records, packets, images).
+ * Measurement only; no counterpart in the reference. */
+int ipgpu_render_info(const ipgpu_ctx *ctx, int *chunks, int *continued_rays, size_t *device_bytes);
+
 /* Gradient of the image loss
  *   L = sum over pixels of sum_s w_s (S_s - target_s)^2,  s = I, Q, U, V,
  * (S in the layout and units of ipgpu_render's imageS) with respect to the
@@ -297,8 +312,9 @@ int ipgpu_stage_times(const ipgpu_ctx *ctx, double ms[4]);
  * and the record capacity exceeds the budget).  seg_min_steps: polarized
  * rays longer than this have their transfer composed from segments (default
  * 3584).  seg_len: steps per segment (default 384).  record_steps:
- * geodesic records held per ray (default 8192, raised by a render whose
- * longest ray needs more; that render retraces).  Results do not depend on
+ * geodesic records held per ray (default 8192; raised to the longest ray of
+ * a render for the next, within half the render memory; a ray past it
+ * continues in a full-length region of its own).  Results do not depend on
  * the phases or the record capacity; seg_min_steps and seg_len regroup the
  * transfer arithmetic of the rays selected (agreement to roundoff).  A
  * negative value leaves a setting unchanged.  The environment variables

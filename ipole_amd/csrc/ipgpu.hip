@@ -58,6 +58,19 @@ __device__ __forceinline__ bool ray_pixel(const int *pix, int ray, int nx, int n
   return i < nx && j < ny;
 }
 
+/* The records of a ray of a single-image render: its region of cap records
+ * in traj, or, for a ray that outgrew it (rbase[ray] >= 0), its full-length
+ * region at trajO + rbase[ray]. */
+__device__ __forceinline__ const double *ray_rec(const double *traj, int cap, const long long *rbase,
+                                                 const double *trajO, int ray)
+{
+  if (rbase) {
+    const long long b = rbase[ray];
+    if (b >= 0) return trajO + b;
+  }
+  return traj + (size_t)ray * cap * REC;
+}
+
 /* Issue priority of a kernel's waves among the waves of other kernels on
  * the same SIMD: the latency-bound kernels (the geodesic phases, the
  * chains, the segment maps of the rays traced last) take the issue slots
@@ -868,6 +881,31 @@ __global__ __launch_bounds__(WAVE) void k_trace_resume_mask(ipgpu_geom G, ipgpu_
   atomicAdd(&counts[0], (unsigned long long)(nstep - n0));
 }
 
+/* The rays of a single image whose trace outgrew the record capacity (the
+ * unbounded phase counts their steps but stores only cap records): their
+ * first cap records copied to full-length regions of trajO (record offsets
+ * off[k]) by k_slot_copy, then resumed here on lane pairs from record
+ * cap - 1 to the end, bit for bit the continuous trace.  rbase[ray]
+ * receives the region's offset in doubles; a step count other than the
+ * trace's is reported in status[0]. */
+__global__ __launch_bounds__(WAVE) void k_trace_continue(ipgpu_geom G, ipgpu_image P, const int *__restrict__ list,
+                                                         int n, const long long *__restrict__ off, int cap,
+                                                         const int *__restrict__ nstep_in, double *__restrict__ trajO,
+                                                         long long *__restrict__ rbase, int *__restrict__ status)
+{
+  IPG_LATENCY_CRITICAL();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = t >> 1;
+  const bool odd = t & 1;
+  if (k >= n) return;
+  const int r = list[k];
+  const int ns = nstep_in[r];
+  const int nstep = trace_pair(G, P, 0, 0, ns + 1, trajO + (size_t)off[k] * REC, cap - 1, 0x7fffffff, nullptr, odd);
+  if (odd) return;
+  rbase[r] = off[k] * REC;
+  if (nstep != ns) atomicAdd(&status[0], 1);
+}
+
 /* Packets of a list of rays, wave-interleaved: the list is sorted by step
  * count, longest first, and cut into waves of 64 elements; wave c owns
  * rows W[c] .. W[c+1] (its first element's step count), and field f of step
@@ -905,6 +943,8 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_list(ipgpu_geom G, ipgpu_mo
                                                           const int *__restrict__ blk, int cap, int stride,
                                                           const int *__restrict__ nstep_in,
                                                           const double *__restrict__ traj,
+                                                          const long long *__restrict__ rbase,
+                                                          const double *__restrict__ trajO,
                                                           const double *__restrict__ dump, double *__restrict__ pkt)
 {
   const long long t = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
@@ -921,7 +961,7 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_list(ipgpu_geom G, ipgpu_mo
   const int ray = list[k];
   const int ns = nstep_in[ray];
   if (q >= ns || ns >= P.maxnstep - 1) return;
-  prep_step<F32, ILV>(G, Mo, P, ns - q, traj + (size_t)ray * cap * REC, nullptr, dump,
+  prep_step<F32, ILV>(G, Mo, P, ns - q, ray_rec(traj, cap, rbase, trajO, ray), nullptr, dump,
                       pkt + (size_t)(W[c] + q) * stride * ILV + pk_lane<ILV>(l), stride == PK, unit_scale());
 }
 
@@ -944,6 +984,8 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_compact(ipgpu_geom G, ipgpu
                                                              const int *__restrict__ blk, int cap,
                                                              const int *__restrict__ nstep_in,
                                                              const double *__restrict__ traj,
+                                                             const long long *__restrict__ rbase,
+                                                             const double *__restrict__ trajO,
                                                              const double *__restrict__ dump, double *__restrict__ pkt)
 {
   long long g = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
@@ -954,8 +996,8 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_compact(ipgpu_geom G, ipgpu
   const int ns = nstep_in[ray];
   if (ns >= P.maxnstep - 1) return;
   const int q = (int)(g - offL[k]);
-  prep_step<F32>(G, Mo, P, ns - q, traj + (size_t)ray * cap * REC, nullptr, dump, pkt + (size_t)g * PK, true,
-                 unit_scale());
+  prep_step<F32>(G, Mo, P, ns - q, ray_rec(traj, cap, rbase, trajO, ray), nullptr, dump, pkt + (size_t)g * PK,
+                 true, unit_scale());
 }
 
 /* Step packets of the first steps of the rays still being traced: n rays
@@ -984,6 +1026,8 @@ __global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model M
                                                      const long long *__restrict__ W, int cap, int stride,
                                                      const int *__restrict__ nstep_in,
                                                      const double *__restrict__ traj,
+                                                     const long long *__restrict__ rbase,
+                                                     const double *__restrict__ trajO,
                                                      const double *__restrict__ dump, const double *__restrict__ pkt,
                                                      const int *__restrict__ seg_base, int seg_len,
                                                      const double *__restrict__ segmap,
@@ -1012,7 +1056,7 @@ __global__ __launch_bounds__(WAVE) void k_chain_list(ipgpu_geom G, ipgpu_model M
   }
   const int sb = seg_base ? seg_base[k] : -1;
   if ((sb >= 0) != SEG) return;
-  const double *rec = traj + (size_t)ray * cap * REC;
+  const double *rec = ray_rec(traj, cap, rbase, trajO, ray);
   double js, ks;
   if constexpr (SEG) {
     if (P.transfer_fp32) far_end_coefficients<true>(G, Mo, P, ns, rec, dump, js, ks, unit_scale());
@@ -1259,7 +1303,28 @@ struct ipgpu_ctx {
   int last_npix = 0, last_unpol = 0, have_result = 0;
   float ms_geo = 0, ms_xfer = 0, ms_total = 0, ms_stage[4] = {0, 0, 0, 0};
   int bad_i = -1, bad_j = -1, bad_job = -1;
+  /* records per ray of a single-image render: a hint (the longest ray of
+   * the last render, at least 8192, within the render memory); rays past it
+   * continue in full-length regions (d_trajO, offsets per ray d_rbase) */
   int traj_cap_steps = 8192;
+  double *d_trajO = nullptr;
+  size_t trajO_doubles = 0;
+  int *d_ovl = nullptr;
+  long long *d_ovoff = nullptr, *d_rbase = nullptr;
+  size_t cap_ovl = 0, cap_ovoff = 0, cap_rbase = 0;
+  std::vector<int> h_ovl;
+  std::vector<long long> h_ovoff;
+  int last_max_steps = 0, last_overflow = 0;
+  /* device memory a single-image render may use for its ray records and
+   * step packets; larger images are rendered in chunks of rays */
+  size_t render_bytes = (size_t)64 << 30;
+  double avg_steps_hint = 2048.;    /* mean step count per ray, from the last render */
+  double pkt_bytes_per_ray = 0.;    /* step packets per ray of the last streamed render */
+  int last_chunks = 1;
+  std::vector<int> h_pixlist;
+  int *d_pixlist = nullptr;
+  size_t cap_pixlist = 0;
+  size_t dev_bytes = 0;             /* device memory held by the context */
   /* streamed render: step budget of the last bounded trace phase (0: one
    * pass), the ray length above which a chain is composed from segments,
    * the segment length */
@@ -1359,11 +1424,15 @@ template <typename T>
 static int grow(ipgpu_ctx *ctx, T *&ptr, size_t &cap, size_t need)
 {
   if (need <= cap && ptr) return 0;
-  if (ptr) HIPCHK(ctx, hipFree(ptr));
+  if (ptr) {
+    HIPCHK(ctx, hipFree(ptr));
+    ctx->dev_bytes -= cap * sizeof(T);
+  }
   ptr = nullptr;
   cap = 0;
   HIPCHK(ctx, hipMalloc((void **)&ptr, need * sizeof(T)));
   cap = need;
+  ctx->dev_bytes += need * sizeof(T);
   return 0;
 }
 
@@ -1430,7 +1499,7 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
                   ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_ckpt, ctx->d_seg, ctx->d_gcell,
                   ctx->d_out4, ctx->d_target, ctx->d_scale, ctx->d_trrec, ctx->d_trout, ctx->d_trnt,
                   ctx->d_rlist, ctx->d_rcount, ctx->d_iavg, ctx->d_interp, ctx->d_ppkt, ctx->d_plist, ctx->d_traj2, ctx->d_off2,
-                  ctx->d_pidx};
+                  ctx->d_pidx, ctx->d_trajO, ctx->d_ovl, ctx->d_ovoff, ctx->d_rbase, ctx->d_pixlist};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int q = 0; q < NEV; q++)
@@ -1476,10 +1545,15 @@ static int ensure_dump(ipgpu_ctx *ctx)
 {
   size_t nc = ghosted_cells(ctx->geom);
   if (ctx->d_dump && ctx->dump_cells == nc) return 0;
-  if (ctx->d_dump) HIPCHK(ctx, hipFree(ctx->d_dump));
+  if (ctx->d_dump) {
+    HIPCHK(ctx, hipFree(ctx->d_dump));
+    ctx->dev_bytes -= ctx->dump_cells * CELL_STRIDE * sizeof(double);
+  }
   ctx->d_dump = nullptr;
+  ctx->dump_cells = 0;
   HIPCHK(ctx, hipMalloc((void **)&ctx->d_dump, nc * CELL_STRIDE * sizeof(double)));
   ctx->dump_cells = nc;
+  ctx->dev_bytes += nc * CELL_STRIDE * sizeof(double);
   return 0;
 }
 
@@ -1695,7 +1769,8 @@ struct PreMaps {
 static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, int n, const int *dl,
                          const long long *W, const long long *TT, long long tiles, int nw, long long rows, int *blk,
                          double *pkt, const int *dbase, ipgpu_ctx::SegSet &S, int cap, int stride, bool compact,
-                         const PreMaps *pre = nullptr, hipEvent_t after_prep = nullptr)
+                         const PreMaps *pre = nullptr, hipEvent_t after_prep = nullptr,
+                         const long long *rbase = nullptr, const double *trajO = nullptr)
 {
   if (n <= 0) return 0;
   if (compact) {
@@ -1706,7 +1781,7 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
       HIPCHK(ctx, hipGetLastError());
       auto kp = P.transfer_fp32 ? k_prep_compact<true> : k_prep_compact<false>;
       kp<<<nb, PREP_BLOCK, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, rows, blk, cap, ctx->d_nstep_ray,
-                                    ctx->d_traj, ctx->d_dump, pkt);
+                                    ctx->d_traj, rbase, trajO, ctx->d_dump, pkt);
       HIPCHK(ctx, hipGetLastError());
     }
     const int nseg = (int)S.off.size();
@@ -1718,8 +1793,8 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
     }
     if (pre) HIPCHK(ctx, hipStreamWaitEvent(st, pre->ev, 0));
     k_chain_list<true><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, ctx->pix, dl, n, W, cap, stride,
-                                                              ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt, dbase,
-                                                              ctx->seg_len, S.d_map, S.d_flag, ctx->d_out7,
+                                                              ctx->d_nstep_ray, ctx->d_traj, rbase, trajO, ctx->d_dump,
+                                                              pkt, dbase, ctx->seg_len, S.d_map, S.d_flag, ctx->d_out7,
                                                               ctx->d_flag_ray, ctx->d_status, ctx->d_counts,
                                                               pre ? pre->idx : nullptr, pre ? pre->map : nullptr,
                                                               pre ? pre->flag : nullptr, pre ? pre->npre : 0);
@@ -1732,7 +1807,7 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
     HIPCHK(ctx, hipGetLastError());
     auto kp = P.transfer_fp32 ? k_prep_list<true> : k_prep_list<false>;
     kp<<<nb, PREP_BLOCK, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, TT, nw, tiles, blk, cap, stride,
-                                  ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt);
+                                  ctx->d_nstep_ray, ctx->d_traj, rbase, trajO, ctx->d_dump, pkt);
     HIPCHK(ctx, hipGetLastError());
   }
   if (after_prep) HIPCHK(ctx, hipEventRecord(after_prep, st));
@@ -1746,15 +1821,15 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
   /* composed rays (the longest, at the front of the list) and the rest */
   if (nseg > 0) {
     k_chain_list<true><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, ctx->pix, dl, n, W, cap, stride,
-                                                              ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt, dbase,
-                                                              ctx->seg_len, S.d_map, S.d_flag, ctx->d_out7,
+                                                              ctx->d_nstep_ray, ctx->d_traj, rbase, trajO, ctx->d_dump,
+                                                              pkt, dbase, ctx->seg_len, S.d_map, S.d_flag, ctx->d_out7,
                                                               ctx->d_flag_ray, ctx->d_status, ctx->d_counts, nullptr,
                                                               nullptr, nullptr, 0);
     HIPCHK(ctx, hipGetLastError());
   }
   k_chain_list<false><<<(n + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, P, ctx->pix, dl, n, W, cap, stride,
-                                                             ctx->d_nstep_ray, ctx->d_traj, ctx->d_dump, pkt,
-                                                             nseg > 0 ? dbase : nullptr, ctx->seg_len, S.d_map,
+                                                             ctx->d_nstep_ray, ctx->d_traj, rbase, trajO, ctx->d_dump,
+                                                             pkt, nseg > 0 ? dbase : nullptr, ctx->seg_len, S.d_map,
                                                              S.d_flag, ctx->d_out7, ctx->d_flag_ray, ctx->d_status,
                                                              ctx->d_counts, nullptr, nullptr, nullptr, 0);
   HIPCHK(ctx, hipGetLastError());
@@ -1785,12 +1860,12 @@ static void sort_longest_first(int *list, int n, const std::vector<int> &ns, std
  * chained on the main stream after their trace; the streams join before
  * k_finalize.  The per-ray arithmetic is that of the one-pass render (a
  * resumed trace continues from its stored record, which holds X and K
- * exactly).  Sets fallback when a resumed ray outgrew the record capacity;
- * the capacity is then raised for a one-pass render. */
-static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int cap, bool &fallback)
+ * exactly).  A ray that outgrows the record capacity (cap records per ray)
+ * is continued from its last stored record in a full-length region of its
+ * own before its transfer. */
+static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int cap)
 {
   int rc;
-  fallback = false;
   const int stride = P.only_unpolarized ? PK_UNPOL : PK;
   const bool segm = !P.only_unpolarized && !P.stokes_floors;
   const int seg_min = segm ? ctx->seg_min_steps : 0x7fffffff;
@@ -1877,7 +1952,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
    * or d_pkt2 (the last) */
   std::vector<int> nsv(nI);
   int start = 0;
-  long long pofs = 0, bofs = 0, total = 0;
+  long long pofs = 0, bofs = 0, total = 0, total_f = 0, pre_slots = 0;
   bool any_f = false;
   for (int k = 0; k <= K; k++) {
     HIPCHK(ctx, hipEventSynchronize(ev[2 * k + 1]));
@@ -1892,16 +1967,46 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
       h_list[start + n++] = r;
       nsv[r] = nsk[r];
     }
+    const long long *rbase = nullptr;
     if (k == K) {
-      /* a ray past the record capacity: one pass with room for it */
+      /* rays past the record capacity: their first cap records move to
+       * full-length regions, where their traces continue */
+      std::vector<int> &ov = ctx->h_ovl;
+      std::vector<long long> &oo = ctx->h_ovoff;
+      ov.clear();
+      oo.assign(1, 0);
       int mx = 0;
-      for (int q = 0; q < n; q++) mx = nsv[h_list[start + q]] > mx ? nsv[h_list[start + q]] : mx;
-      if (mx >= cap) {
-        ctx->traj_cap_steps = mx + 1 < P.maxnstep ? mx + 1 : P.maxnstep;
-        for (int q = 0; q < K + (pre_launched ? 1 : 0); q++) HIPCHK(ctx, hipStreamSynchronize(ctx->streamF[q]));
-        fallback = true;
-        return 0;
+      for (int q = 0; q < n; q++) {
+        const int r = h_list[start + q], v = nsv[r];
+        mx = v > mx ? v : mx;
+        if (v >= cap && v < P.maxnstep - 1) {
+          ov.push_back(r);
+          oo.push_back(oo.back() + v + 1);
+        }
       }
+      ctx->last_max_steps = mx;
+      if (!ov.empty()) {
+        const int no = (int)ov.size();
+        if ((rc = grow(ctx, ctx->d_trajO, ctx->trajO_doubles, (size_t)oo.back() * REC))
+            || (rc = grow(ctx, ctx->d_ovl, ctx->cap_ovl, (size_t)no))
+            || (rc = grow(ctx, ctx->d_ovoff, ctx->cap_ovoff, (size_t)no))
+            || (rc = grow(ctx, ctx->d_rbase, ctx->cap_rbase, nI)))
+          return rc;
+        HIPCHK(ctx, hipMemsetAsync(ctx->d_rbase, 0xff, nI * sizeof(long long), sA));
+        HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), sA));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_ovl, ov.data(), no * sizeof(int), hipMemcpyHostToDevice, sA));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_ovoff, oo.data(), no * sizeof(long long), hipMemcpyHostToDevice, sA));
+        const long long cells = (long long)no * cap * REC;
+        k_slot_copy<<<(unsigned)((cells + 255) / 256 < 65536 ? (cells + 255) / 256 : 65536), 256, 0, sA>>>(
+            ctx->d_ovl, no, ctx->d_ovoff, cap, ctx->d_traj, ctx->d_trajO);
+        HIPCHK(ctx, hipGetLastError());
+        k_trace_continue<<<(2 * no + WAVE - 1) / WAVE, WAVE, 0, sA>>>(ctx->geom, P, ctx->d_ovl, no, ctx->d_ovoff, cap,
+                                                                      ctx->d_nstep_ray, ctx->d_trajO, ctx->d_rbase,
+                                                                      ctx->d_status);
+        HIPCHK(ctx, hipGetLastError());
+        rbase = ctx->d_rbase;
+      }
+      ctx->last_overflow = (int)ov.size();
     }
     /* longest first.  Lists whose chains are all composed from segments
      * (the last phase's, when the segment threshold is at most the last
@@ -1935,6 +2040,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
       for (int c = 0; c < nw; c++) TT[c + 1] = TT[c] + (long long)(ILV / TL) * ((W[c + 1] - W[c] + TQ - 1) / TQ);
     const long long tiles = compact ? 0 : TT[nw];
     total += T;
+    if (k < K) total_f += T;
     /* first packet of each element, in doubles */
     std::vector<long long> &eoff = ctx->h_eoff;
     eoff.resize(n);
@@ -1984,7 +2090,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
       }
       if ((rc = transfer_list(ctx, P, st, n, dl, dW, dTT, tiles, nw, rows, blk, pkt, dbase, ctx->segset[k], cap,
                               stride, compact, use_pre ? &pm : nullptr,
-                              (k == K - 1 && pre_on && !compact) ? ev[10] : nullptr)))
+                              (k == K - 1 && pre_on && !compact) ? ev[10] : nullptr, rbase, ctx->d_trajO)))
         return rc;
       prep_ev = k == K - 1 && pre_on && !compact;
     }
@@ -2014,6 +2120,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
         /* after the packets of the rays this phase finished, which the
          * chain that ends last waits for */
         if (prep_ev) HIPCHK(ctx, hipStreamWaitEvent(sp, ev[10], 0));
+        pre_slots = (long long)np * np1;
         if ((rc = grow(ctx, ctx->d_ppkt, ctx->ppkt_doubles, (size_t)np * np1 * PK))
             || (rc = grow(ctx, ctx->d_plist, ctx->cap_plist, (size_t)np)))
           return rc;
@@ -2036,7 +2143,10 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
     start += n;
   }
   if (!any_f) HIPCHK(ctx, hipEventRecord(ev[19], sA));
-  ctx->last_total = total;
+  /* the arena of the bounded phases' lists is sized by their packets in the
+   * next render; the chunk size by all packets of this one */
+  ctx->last_total = total_f;
+  ctx->pkt_bytes_per_ray = (double)(total + pre_slots) * stride * sizeof(double) / nrays;
   HIPCHK(ctx, hipEventRecord(ev[20], sA));
   for (int k = 0; k < K; k++) HIPCHK(ctx, hipStreamWaitEvent(sA, ev[15 + k], 0));
   if (pre_launched) HIPCHK(ctx, hipStreamWaitEvent(sA, ev[23], 0));
@@ -2046,6 +2156,96 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
 }
 
 static int render_rays(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays);
+
+/* Rays per chunk of a single-image render whose ray records and step
+ * packets fit the render memory. */
+static long long chunk_rays(const ipgpu_ctx *ctx, const ipgpu_image &P)
+{
+  const int cap = ctx->traj_cap_steps < P.maxnstep ? ctx->traj_cap_steps : P.maxnstep;
+  const int stride = P.only_unpolarized ? PK_UNPOL : PK;
+  /* per ray: cap records, packets of the mean step count (with the padding
+   * of the interleaved layout and the prefix packets of the longest rays),
+   * and the per-ray tables; 85% of the render memory, the rest for the
+   * buffers of the last phase's list and the segment maps */
+  const double pkt = ctx->pkt_bytes_per_ray > 0. ? 1.1 * ctx->pkt_bytes_per_ray
+                                                 : ctx->avg_steps_hint * stride * sizeof(double) * 1.25;
+  const double per_ray = (double)cap * REC * sizeof(double) + pkt + 512.;
+  const long long chunk = (long long)(0.85 * (double)ctx->render_bytes / per_ray) / WAVE * WAVE;
+  return chunk < WAVE ? WAVE : chunk;
+}
+
+/* The rays of the image in chunks (whole waves of its 8x8 tiles, as pixel
+ * lists) whose ray records and step packets fit the render memory, or all
+ * at once when they fit.  A pixel's result does not depend on the other
+ * rays of its render, so the image is the same either way. */
+static int render_image(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays)
+{
+  if (chunk_rays(ctx, P) >= nrays) {
+    ctx->pix = nullptr;
+    ctx->last_chunks = 1;
+    int rc = render_rays(ctx, P, nrays);
+    if (nrays > 0 && ctx->counts[0] > 0) ctx->avg_steps_hint = (double)ctx->counts[0] / nrays;
+    return rc;
+  }
+  /* tiles in the order q -> q s mod T (s coprime to T, near the golden
+   * section of T): any run of consecutive tiles is spread over the image,
+   * so every chunk has about the image's mix of short and long rays and the
+   * next chunk can be sized from the last */
+  std::vector<int> &pl = ctx->h_pixlist;
+  pl.resize(nrays);
+  const int tiles_y = (P.ny + TILE - 1) / TILE;
+  const long long T = nrays / WAVE;
+  long long stp = (long long)(0.6180339887498949 * (double)T + 0.5);
+  auto gcd = [](long long a, long long b) {
+    while (b) {
+      long long t = a % b;
+      a = b;
+      b = t;
+    }
+    return a;
+  };
+  if (stp < 1) stp = 1;
+  while (gcd(stp, T) != 1) stp++;
+  for (long long q = 0; q < T; q++) {
+    const long long t = (q * stp) % T;
+    for (int l = 0; l < WAVE; l++) {
+      const int i = (int)(t / tiles_y) * TILE + l / TILE, j = (int)(t % tiles_y) * TILE + l % TILE;
+      pl[q * WAVE + l] = (i < P.nx && j < P.ny) ? ((i << 16) | j) : -1;
+    }
+  }
+  int rc;
+  if ((rc = grow(ctx, ctx->d_pixlist, ctx->cap_pixlist, (size_t)nrays))) return rc;
+  unsigned long long counts[2] = {0, 0};
+  float ms_geo = 0, ms_xfer = 0, ms_total = 0;
+  int nchunks = 0;
+  for (long long r0 = 0; r0 < nrays;) {
+    /* each chunk sized with the record capacity and mean step count the
+     * previous one left */
+    const long long chunk = chunk_rays(ctx, P);
+    const int n = (int)(nrays - r0 < chunk ? nrays - r0 : chunk);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_pixlist + r0, pl.data() + r0, (size_t)n * sizeof(int),
+                               hipMemcpyHostToDevice, ctx->stream));
+    ctx->pix = ctx->d_pixlist + r0;
+    rc = render_rays(ctx, P, n);
+    ctx->pix = nullptr;
+    nchunks++;
+    counts[0] += ctx->counts[0];
+    counts[1] += ctx->counts[1];
+    ms_geo += ctx->ms_geo;
+    ms_xfer += ctx->ms_xfer;
+    ms_total += ctx->ms_total;
+    if (rc) break;
+    if (ctx->counts[0] > 0) ctx->avg_steps_hint = (double)ctx->counts[0] / n;
+    r0 += n;
+  }
+  ctx->counts[0] = counts[0];
+  ctx->counts[1] = counts[1];
+  ctx->ms_geo = ms_geo;
+  ctx->ms_xfer = ms_xfer;
+  ctx->ms_total = ms_total;
+  ctx->last_chunks = nchunks;
+  return rc;
+}
 
 int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *imageS,
                  double *taus, int *nstep, int *flag)
@@ -2070,9 +2270,8 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
   if ((rc = grow(ctx, ctx->d_flag_pix, ctx->cap_flag_pix, npix))) return rc;
   if ((rc = grow(ctx, ctx->d_imageS, ctx->cap_imageS, npix * 5))) return rc;
 
-  ctx->pix = nullptr;
   ctx->out_ny = P.ny;
-  if ((rc = render_rays(ctx, P, nrays))) return rc;
+  if ((rc = render_image(ctx, P, nrays))) return rc;
   ctx->last_npix = (int)npix;
   ctx->last_unpol = P.only_unpolarized;
   ctx->have_result = 1;
@@ -2090,9 +2289,9 @@ static int render_rays(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays)
   if ((rc = grow(ctx, ctx->d_out7, ctx->cap_out7, (size_t)nrays * 7))) return rc;
   int cap = ctx->traj_cap_steps < P.maxnstep ? ctx->traj_cap_steps : P.maxnstep;
   const bool two = P.target_nturns < 0 && ctx->trace_budget > 0 && cap > ctx->trace_budget + 1;
-  bool fallback = false;
-  if (two && (rc = render_streamed(ctx, P, nrays, cap, fallback))) return rc;
-  const bool one = !two || fallback;
+  ctx->last_overflow = 0;
+  if (two && (rc = render_streamed(ctx, P, nrays, cap))) return rc;
+  const bool one = !two;
   if (one) {
     int stride;
     unsigned long long total;
@@ -2185,6 +2384,15 @@ static int render_rays(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays)
   }
   ctx->streamed = !one;
   hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[e1]);
+  if (!one) {
+    if (status[0] != 0)
+      return set_err(ctx, IPGPU_ERR_STATE, "continued trace of a ray past the record capacity differs");
+    /* the next render's record capacity: this one's longest ray, if the
+     * records of a chunk of rays then stay within half the render memory */
+    const long long want = (long long)ctx->last_max_steps + 1;
+    const long long lim = (long long)(ctx->render_bytes / 2 / ((size_t)nrays * REC * sizeof(double)));
+    if (want > ctx->traj_cap_steps && want <= lim) ctx->traj_cap_steps = (int)want;
+  }
   if (status[1] != 0x7fffffff) {
     int p = status[1] - 1;
     ctx->bad_i = p >> 16;
@@ -2767,6 +2975,22 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     if (J.flag_out) HIPCHK(ctx, hipMemcpyAsync(J.flag_out, ctx->d_bflag + o, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return IPGPU_OK;
+}
+
+int ipgpu_set_render_memory(ipgpu_ctx *ctx, size_t bytes)
+{
+  if (!ctx || bytes < ((size_t)1 << 20)) return set_err(ctx, IPGPU_ERR_ARG, "render memory below 1 MiB");
+  ctx->render_bytes = bytes;
+  return IPGPU_OK;
+}
+
+int ipgpu_render_info(const ipgpu_ctx *ctx, int *chunks, int *continued_rays, size_t *device_bytes)
+{
+  if (!ctx) return IPGPU_ERR_ARG;
+  if (chunks) *chunks = ctx->last_chunks;
+  if (continued_rays) *continued_rays = ctx->last_overflow;
+  if (device_bytes) *device_bytes = ctx->dev_bytes;
   return IPGPU_OK;
 }
 

@@ -138,6 +138,8 @@ def lib():
         L.ipgpu_render_batch.argtypes = [vp, C.c_int, C.POINTER(Job)]
         L.ipgpu_bad_job.argtypes = [vp, ip, ip, ip]
         L.ipgpu_set_batch_memory.argtypes = [vp, C.c_size_t]
+        L.ipgpu_set_render_memory.argtypes = [vp, C.c_size_t]
+        L.ipgpu_render_info.argtypes = [vp, ip, ip, C.POINTER(C.c_size_t)]
         L.ipgpu_adjoint.argtypes = [vp, C.POINTER(Image), dp, dp, dp, dp, dp]
         L.ipgpu_scale_density.argtypes = [vp, dp]
         L.ipgpu_kernel_times.argtypes = [vp, dp, dp, dp]
@@ -411,6 +413,17 @@ class Context:
 
     def set_batch_memory(self, nbytes):
         self._chk(self.L.ipgpu_set_batch_memory(self.h, int(nbytes)))
+
+    def set_render_memory(self, nbytes):
+        """Device memory a single-image render may use for ray records and
+        step packets; larger images render in chunks of rays."""
+        self._chk(self.L.ipgpu_set_render_memory(self.h, int(nbytes)))
+
+    def render_info(self):
+        """(chunks, continued rays, device bytes held) of the last render."""
+        a, b, c = C.c_int(), C.c_int(), C.c_size_t()
+        self._chk(self.L.ipgpu_render_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def kernel_times(self):
         a, b, c = C.c_double(), C.c_double(), C.c_double()

@@ -8,8 +8,11 @@
   max I) below 1e-5 everywhere and above 1e-6 on at most 1% of the pixels,
   the sample's summed I within 1e-6;
 * the streamed render (bounded phases, finished rays on streams of their
-  own, the longest rays' first steps composed during the last phase)
-  against the one-pass render of the same image, every output bit for bit;
+  own, the longest rays' first steps composed during the last phase, rays
+  past the record capacity continued in regions of their own) against the
+  one-pass render of the same image, every output bit for bit;
+* the image rendered in chunks of rays under a small render memory against
+  the whole render, bit for bit;
 * a batch of three jobs on the same dump (its first trace keeps the records
   in per-ray slots and continues the rays longer than a slot), FP64 and
   mixed precision, against each job's plain render.
@@ -76,10 +79,28 @@ def test_fullsize_sample_vs_oracle(full):
 
 
 def test_fullsize_streamed_equals_one_pass(full):
+    """On a context of its own (so independent of the other tests): the
+    first render is streamed although the longest ray (11,267 steps) exceeds
+    the default record capacity of 8,192 (it continues in a region of its
+    own); it and the next render equal the one-pass render bit for bit."""
     orc, p, c, im = full
-    say("full-size streamed vs one-pass")
-    streamed = c.render(im)
-    assert c.stage_times_mode()[1] is True
+    say("full-size streamed (fresh context) vs one-pass")
+    h, prims = synth.make_dump(*N, a=A, seed=SEED)
+    c2 = G.Context(0)
+    try:
+        g, m = G.iharm_geom(h), G.iharm_model()
+        c2.set_geometry(g)
+        c2.set_model(m)
+        c2.upload_prims(prims, G.iharm_units(M_UNIT, trat_large=TRAT))
+        first = c2.render(im)
+        assert c2.stage_times_mode()[1] is True
+        chunks, cont, nbytes = c2.render_info()
+        say("  first render: %d chunk(s), %d continued rays, %.1f GB held" % (chunks, cont, nbytes / 1e9))
+        assert chunks == 1 and cont > 0
+        second = c2.render(im)
+        assert c2.stage_times_mode()[1] is True and c2.render_info()[1] == 0
+    finally:
+        c2.close()
     c.set_tuning(trace_budget=0)
     try:
         one = c.render(im)
@@ -87,8 +108,28 @@ def test_fullsize_streamed_equals_one_pass(full):
     finally:
         c.set_tuning(trace_budget=3584)
     for k in ("image", "imageS", "taus", "nstep", "flag"):
-        assert np.array_equal(streamed[k], one[k]), k
-    assert int(streamed["nstep"].max()) > 10000
+        assert np.array_equal(first[k], one[k]), k
+        assert np.array_equal(second[k], one[k]), k
+    assert int(first["nstep"].max()) > 10000
+
+
+def test_fullsize_chunked_equals_whole(full):
+    """The 160x160 image rendered in chunks of rays under a small render
+    memory (8 GiB) equals the render of all rays at once, bit for bit."""
+    orc, p, c, im = full
+    say("full-size chunked (8 GiB) vs whole")
+    whole = c.render(im)
+    assert c.render_info()[0] == 1
+    c.set_render_memory(8 << 30)
+    try:
+        parts = c.render(im)
+        chunks, cont, nbytes = c.render_info()
+    finally:
+        c.set_render_memory(64 << 30)
+    say("  %d chunks, %.1f GB held" % (chunks, nbytes / 1e9))
+    assert chunks >= 3
+    for k in ("image", "imageS", "taus", "nstep", "flag"):
+        assert np.array_equal(whole[k], parts[k]), k
 
 
 def test_fullsize_batch_vs_plain(full):
@@ -126,3 +167,48 @@ def test_fullsize_batch_vs_plain(full):
                 for name, (mx, frac) in stokes_errors(ref["imageS"], ref["image"], out["imageS"],
                                                       out["image"]).items():
                     assert mx < 1e-6, (k, name, mx)
+
+
+def test_fullsize_512_within_render_memory(full):
+    """A 512x512 image of the same dump under a render memory of 32 GiB
+    (all its rays at once would need ~400 GB of records and packets): it
+    runs in chunks, the context holds at most the budget plus the dump and
+    the image, and a strided sample of pixels (every 32nd row and column)
+    matches the oracle at the parity bar."""
+    orc, p, c, im = full
+    budget = 32 << 30
+    say("512x512 under %d GiB" % (budget >> 30))
+    h, prims = synth.make_dump(*N, a=A, seed=SEED)
+    p512 = O.default_params(nx=512, ny=512, thetacam=163., freqcgs=230e9)
+    c3 = G.Context(0)
+    try:
+        g, m = G.iharm_geom(h), G.iharm_model()
+        c3.set_geometry(g)
+        c3.set_model(m)
+        c3.upload_prims(prims, G.iharm_units(M_UNIT, trat_large=TRAT))
+        c3.set_render_memory(budget)
+        im512, _, _ = G.make_image(g, m.L_unit, **gpu_image_kwargs(p512))
+        got = c3.render(im512)
+        chunks, cont, nbytes = c3.render_info()
+    finally:
+        c3.close()
+    dump_bytes = 290 * 130 * 130 * 128
+    image_bytes = 512 * 512 * 9 * 8
+    say("  %d chunks, %d continued rays, %.2f GB held (budget %.2f GB + dump %.2f GB)"
+        % (chunks, cont, nbytes / 1e9, budget / 1e9, dump_bytes / 1e9))
+    assert chunks >= 8
+    assert nbytes <= budget + dump_bytes + image_bytes + (64 << 20), nbytes
+    S = 32
+    pix = np.array([(i, j) for i in range(0, 512, S) for j in range(0, 512, S)], np.int32)
+    rc, out7, ns, fl = orc.render_pixels(p512, orc.camera(p512), pix, nthreads=16)
+    assert rc == 0
+    n = 512 // S
+    f3 = p512.freqcgs ** 3
+    ref_I = (out7[:, 0] * f3).reshape(n, n)
+    ref_S = np.stack([out7[:, 1] * f3, -out7[:, 2] * f3, -out7[:, 3] * f3, out7[:, 4] * f3, out7[:, 6]],
+                     axis=1).reshape(n, n, 5)
+    assert np.array_equal(np.asarray(ns).reshape(n, n), got["nstep"][::S, ::S])
+    errs = stokes_errors(ref_S, ref_I, got["imageS"][::S, ::S], got["image"][::S, ::S])
+    say("  errors %s" % errs)
+    for k, (mx, frac) in errs.items():
+        assert mx < 1e-5 and frac <= 1e-2, (k, mx, frac)

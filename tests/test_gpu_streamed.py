@@ -115,22 +115,34 @@ def test_streamed_vs_oracle(case):
     c.close()
 
 
-def test_streamed_record_overflow(case):
-    """Resumed rays longer than the record capacity: the render retraces in
-    one pass with room for them and returns the one-pass result."""
+@pytest.mark.parametrize("tune", [dict(trace_budget=50, record_steps=200),
+                                  dict(trace_budget=128, trace_phases=2, seg_min_steps=100, seg_len=32,
+                                       record_steps=200)],
+                         ids=["one-phase", "prefix-composed"])
+def test_streamed_record_overflow(case, tune):
+    """Rays longer than the record capacity continue from their last stored
+    record in full-length regions of their own: the render stays streamed
+    and returns the one-pass result bit for bit, on a fresh context and on
+    the next render (whose capacity follows the longest ray, so nothing
+    continues).  The second tuning also forms the longest rays' first
+    segments during the unbounded phase."""
     h, prims, orc = case
-    say("case streamed record overflow 24x24")
-    c1, g, m = fresh(h, prims, trace_budget=0)
+    say("case streamed record overflow 24x24 %s" % tune)
+    c1, g, m = fresh(h, prims, trace_budget=0, seg_min_steps=tune.get("seg_min_steps", 3584),
+                     seg_len=tune.get("seg_len", 384))
     im, _, _ = G.make_image(g, m.L_unit, nx=24, ny=24, thetacam=163.)
     ref = c1.render(im)
     c1.close()
     assert ref["nstep"].max() > 200
-    c2, _, _ = fresh(h, prims, trace_budget=50, record_steps=200)
+    c2, _, _ = fresh(h, prims, **tune)
     got = c2.render(im)
+    assert c2.stage_times_mode()[1] is True
+    chunks, cont, _ = c2.render_info()
+    assert chunks == 1 and cont == int(np.sum(ref["nstep"] >= 200)) > 0, (chunks, cont)
     same(ref, got)
-    # the capacity was raised: the next render is streamed throughout
     got2 = c2.render(im)
     assert c2.stage_times_mode()[1] is True
+    assert c2.render_info()[1] == 0
     same(ref, got2)
     c2.close()
 
