@@ -136,6 +136,26 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
 int ipgpu_fetch(ipgpu_ctx *ctx, double *image, double *imageS, double *taus, int *nstep,
                 int *flag);
 
+/* ipgpu_render for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops and harnesses that forbid allocation).
+ * No counterpart in the reference.  The outputs are the caller's DEVICE
+ * arrays (image, taus, nstep, flag: nx*ny entries; imageS: nx*ny*5; any may
+ * be NULL), in ipgpu_render's layout and units; the inputs are the
+ * context's geometry, model and uploaded dump.  The call allocates nothing:
+ * its working memory is one static device array of IPGPU_SCRATCH_BYTES per
+ * GPU (held for as long as the process runs), shared by every call on that
+ * GPU and left as the last call used it, so calls that overlap on one GPU
+ * (from several contexts or threads) race.  The rays are traced once to
+ * count their steps and again, in as many chunks as the scratch needs, to
+ * store their records, and each ray's transfer runs in one lane
+ * (transfer_fused, the arithmetic of ipgpu_render_batch): results agree
+ * with ipgpu_render to roundoff.  Returns IPGPU_ERR_ARG when the image's
+ * per-ray tables and two rays of maxnstep records exceed the scratch (about
+ * 85 million rays at maxnstep 50000), or for target_nturns >= 0. */
+#define IPGPU_SCRATCH_BYTES ((size_t)1 << 30)
+int ipgpu_render_noalloc(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *imageS, double *taus,
+                         int *nstep, int *flag);
+
 /* Adaptive image refinement (src/main.c:505-792; keys src/par.c:77-84,
  * 172-177): the image's nx x ny is the finest level, nx_min x ny_min the
  * coarsest; levels = log2(nx / nx_min) + 1 on the even parts of nx, nx_min,

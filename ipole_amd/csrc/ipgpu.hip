@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ipg_physics.h"
@@ -1248,7 +1249,119 @@ __global__ void k_trace_pixel(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P, int p
   trace_pixel_records(G, Mo, P, pi, pj, cap, prims_ok, dump, rec, nt, out, nstep_out);
 }
 
+/* ------------------------------------ single-image render without allocation
+ *
+ * ipgpu_render_noalloc works in one fixed scratch array per GPU: status
+ * words at byte 0, counters at byte 64, per-ray step counts and record
+ * offsets, then an arena of ray records.  A count pass gives every ray's
+ * step count; the prefix sum S_r = off[r] + r places ray r's nstep + 1
+ * records in a virtual contiguous layout, and chunk c holds the rays with
+ * S_r in [c A', (c+1) A'), A' = arena records - (maxnstep + 1), so that every
+ * ray of a chunk fits the arena at S_r - c A'.  Per chunk the rays are
+ * traced again into the arena and transferred (transfer_fused) straight into
+ * the caller's output arrays. */
+
+/* status[1], status[2] (first maxnstep and NaN pixel codes) to "none",
+ * status[0] (store passes whose step count differs) and the counters to 0 */
+__global__ void k_na_init(int *__restrict__ status, unsigned long long *__restrict__ counts)
+{
+  if (threadIdx.x == 0) {
+    status[0] = 0;
+    status[1] = 0x7fffffff;
+    status[2] = 0x7fffffff;
+    status[3] = 0;
+    counts[0] = 0;
+    counts[1] = 0;
+  }
+}
+
+/* Count pass on lane pairs: every ray's step count, nothing stored. */
+__global__ __launch_bounds__(WAVE) void k_na_count(ipgpu_geom G, ipgpu_image P, int nrays, int *__restrict__ nstep_out,
+                                                   unsigned long long *__restrict__ counts)
+{
+  IPG_LATENCY_CRITICAL();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ray = t >> 1;
+  const bool odd = t & 1;
+  if (ray >= nrays) return;
+  int pi, pj, ns = 0;
+  if (ray_pixel(nullptr, ray, P.nx, P.ny, pi, pj)) ns = trace_pair(G, P, pi, pj, 0, nullptr, 0, 0x7fffffff, nullptr, odd);
+  if (odd) return;
+  nstep_out[ray] = ns;
+  atomicAdd(&counts[0], (unsigned long long)ns);
+}
+
+/* Store pass of chunk c on lane pairs: the records of its rays at
+ * arena + (S_r - c A') REC; a step count other than the count pass's is
+ * reported in status[0]. */
+__global__ __launch_bounds__(WAVE) void k_na_store(ipgpu_geom G, ipgpu_image P, int nrays, long long c, long long Ap,
+                                                   const int *__restrict__ nstep_in, const long long *__restrict__ off,
+                                                   double *__restrict__ arena, int *__restrict__ status)
+{
+  IPG_LATENCY_CRITICAL();
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ray = t >> 1;
+  const bool odd = t & 1;
+  if (ray >= nrays) return;
+  const long long S = off[ray] + ray;
+  if (S / Ap != c) return;
+  int pi, pj;
+  if (!ray_pixel(nullptr, ray, P.nx, P.ny, pi, pj)) return;
+  const int ns = nstep_in[ray];
+  if (ns >= P.maxnstep - 1) return;
+  const int n = trace_pair(G, P, pi, pj, ns + 1, arena + (size_t)(S - c * Ap) * REC, 0, 0x7fffffff, nullptr, odd);
+  if (!odd && n != ns) atomicAdd(&status[0], 1);
+}
+
+/* Transfer of chunk c, one lane per ray, into the caller's arrays in the
+ * layout of save_pixel (src/main.c:885-910); any output may be null. */
+template <bool F32>
+__global__ __launch_bounds__(WAVE) void k_na_xfer(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P, int nrays, long long c,
+                                                  long long Ap, const int *__restrict__ nstep_in,
+                                                  const long long *__restrict__ off, const double *__restrict__ arena,
+                                                  const double *__restrict__ dump, double *__restrict__ image,
+                                                  double *__restrict__ imageS, double *__restrict__ taus,
+                                                  int *__restrict__ nstep_pix, int *__restrict__ flag_pix,
+                                                  int *__restrict__ status, unsigned long long *__restrict__ counts)
+{
+  const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ray >= nrays) return;
+  const long long S = off[ray] + ray;
+  if (S / Ap != c) return;
+  int pi, pj;
+  if (!ray_pixel(nullptr, ray, P.nx, P.ny, pi, pj)) return;
+  const size_t p = (size_t)pi * P.ny + pj;
+  const int ns = nstep_in[ray];
+  if (nstep_pix) nstep_pix[p] = ns;
+  double o[7] = {0., 0., 0., 0., 0., 0., 0.};
+  int flag = 0, nrad = 0;
+  if (ns >= P.maxnstep - 1) atomicMin(&status[1], pixel_code(pi, pj));
+  else flag = transfer_fused<F32>(G, Mo, P, ns, arena + (size_t)(S - c * Ap) * REC, nullptr, dump, unit_scale(), o, nrad);
+  const double f3 = P.freqcgs * P.freqcgs * P.freqcgs;
+  if (image) image[p] = o[0] * f3;
+  if (taus) taus[p] = o[5];
+  if (flag_pix) flag_pix[p] = flag;
+  if (!P.only_unpolarized) {
+    const double qs = (P.qu_conv == 0) ? -1. : 1.;
+    if (imageS) {
+      imageS[p * 5 + 0] = o[1] * f3;
+      imageS[p * 5 + 1] = qs * o[2] * f3;
+      imageS[p * 5 + 2] = qs * o[3] * f3;
+      imageS[p * 5 + 3] = o[4] * f3;
+      imageS[p * 5 + 4] = o[6];
+    }
+    if (isnan(o[1])) atomicMin(&status[2], pixel_code(pi, pj));
+  }
+  atomicAdd(&counts[1], (unsigned long long)nrad);
+}
+
 }  // namespace
+
+/* The working memory of ipgpu_render_noalloc (see above), one copy in each
+ * GPU's memory for as long as the process runs.  Host code never uses this
+ * name as an address: the device address comes from hipGetSymbolAddress,
+ * once per device (noalloc_setup), and calls on one GPU share the array. */
+static __device__ __align__(256) unsigned char scratch[IPGPU_SCRATCH_BYTES];
 
 /* ------------------------------------------------------------------ C-ABI */
 
@@ -1325,6 +1438,7 @@ struct ipgpu_ctx {
   int *d_pixlist = nullptr;
   size_t cap_pixlist = 0;
   size_t dev_bytes = 0;             /* device memory held by the context */
+  void *d_scratch = nullptr;        /* this GPU's scratch (ipgpu_render_noalloc) */
   /* streamed render: step budget of the last bounded trace phase (0: one
    * pass), the ray length above which a chain is composed from segments,
    * the segment length */
@@ -1436,6 +1550,25 @@ static int grow(ipgpu_ctx *ctx, T *&ptr, size_t &cap, size_t need)
   return 0;
 }
 
+/* Device address of scratch on each GPU, looked up once per device with
+ * that device current. */
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch[MAX_DEVICES] = {};
+
+static int noalloc_setup(int device, void **ptr)
+{
+  if (device < 0 || device >= MAX_DEVICES) return IPGPU_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return IPGPU_ERR_HIP;
+    g_scratch[device] = p;
+  }
+  *ptr = g_scratch[device];
+  return IPGPU_OK;
+}
+
 extern "C" {
 
 int ipgpu_create(ipgpu_ctx **out, int device)
@@ -1448,6 +1581,11 @@ int ipgpu_create(ipgpu_ctx **out, int device)
   ipgpu_ctx *ctx = new ipgpu_ctx();
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return IPGPU_ERR_HIP;
+  }
+  if (noalloc_setup(device, &ctx->d_scratch) != IPGPU_OK) {
+    hipStreamDestroy(ctx->stream);
     delete ctx;
     return IPGPU_ERR_HIP;
   }
@@ -2623,6 +2761,96 @@ int ipgpu_adaptive_stats(const ipgpu_ctx *ctx, int *levels, double *Iavg, long l
   for (int l = 0; l < IPGPU_MAX_LEVELS; l++) {
     if (traced) traced[l] = ctx->ref_traced[l];
     if (interpolated) interpolated[l] = ctx->ref_interp[l];
+  }
+  return IPGPU_OK;
+}
+
+int ipgpu_render_noalloc(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *imageS, double *taus,
+                         int *nstep, int *flag)
+{
+  if (!ctx || !img) return IPGPU_ERR_ARG;
+  if (!ctx->has_geom || !ctx->has_model) return set_err(ctx, IPGPU_ERR_STATE, "geometry/model not set");
+  if (ctx->model.kind == IPGPU_MODEL_IHARM && !ctx->d_dump)
+    return set_err(ctx, IPGPU_ERR_STATE, "no fluid dump uploaded");
+  const ipgpu_image &P = *img;
+  if (P.nx <= 0 || P.ny <= 0 || P.maxnstep < 2) return set_err(ctx, IPGPU_ERR_ARG, "bad image size or maxnstep");
+  if (P.nx > 32767 || P.ny > 32767) return set_err(ctx, IPGPU_ERR_ARG, "image side above 32767 pixels");
+  if (P.emission_type != 4 && P.emission_type != 1)
+    return set_err(ctx, IPGPU_ERR_ARG, "emission_type must be 4 (Dexter thermal) or 1 (Pandya thermal)");
+  if (P.target_nturns >= 0)
+    return set_err(ctx, IPGPU_ERR_ARG, "ipgpu_render_noalloc: subring selection (target_nturns >= 0) not supported");
+  if (!ctx->d_scratch) return set_err(ctx, IPGPU_ERR_STATE, "scratch address not set up");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int tiles = ((P.nx + TILE - 1) / TILE) * ((P.ny + TILE - 1) / TILE);
+  const long long nrays = (long long)tiles * WAVE;
+  /* layout: status (byte 0), counters (64), step counts (256), offsets,
+   * then the records */
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t o_ns = 256, o_off = up(o_ns + (size_t)nrays * sizeof(int));
+  const size_t o_arena = up(o_off + ((size_t)nrays + 1) * sizeof(long long));
+  const long long rec_bytes = REC * (long long)sizeof(double);
+  const long long A = o_arena < IPGPU_SCRATCH_BYTES ? (long long)((IPGPU_SCRATCH_BYTES - o_arena) / rec_bytes) : 0;
+  const long long Ap = A - ((long long)P.maxnstep + 1);
+  if (Ap < (long long)P.maxnstep + 1) {
+    snprintf(ctx->err, sizeof(ctx->err),
+             "ipgpu_render_noalloc: %dx%d image with maxnstep %d needs more than the %zu-byte scratch "
+             "(per-ray tables of %zu bytes and room for two rays of maxnstep records)",
+             P.nx, P.ny, P.maxnstep, (size_t)IPGPU_SCRATCH_BYTES, o_arena);
+    return IPGPU_ERR_ARG;
+  }
+  char *base = (char *)ctx->d_scratch;
+  int *d_status = (int *)base;
+  unsigned long long *d_counts = (unsigned long long *)(base + 64);
+  int *d_ns = (int *)(base + o_ns);
+  long long *d_off = (long long *)(base + o_off);
+  double *arena = (double *)(base + o_arena);
+  hipStream_t st = ctx->stream;
+  HIPCHK(ctx, hipEventRecord(ctx->ev[0], st));
+  k_na_init<<<1, 64, 0, st>>>(d_status, d_counts);
+  HIPCHK(ctx, hipGetLastError());
+  k_na_count<<<(unsigned)(2 * nrays / WAVE), WAVE, 0, st>>>(ctx->geom, P, (int)nrays, d_ns, d_counts);
+  HIPCHK(ctx, hipGetLastError());
+  k_scan<<<1, SCAN_THREADS, 0, st>>>((int)nrays, d_ns, d_off);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipEventRecord(ctx->ev[1], st));
+  long long total = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&total, d_off + nrays, sizeof(long long), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  const long long nchunks = (total + nrays + Ap - 1) / Ap;
+  auto kx = P.transfer_fp32 ? k_na_xfer<true> : k_na_xfer<false>;
+  for (long long c = 0; c < nchunks; c++) {
+    k_na_store<<<(unsigned)(2 * nrays / WAVE), WAVE, 0, st>>>(ctx->geom, P, (int)nrays, c, Ap, d_ns, d_off, arena,
+                                                              d_status);
+    HIPCHK(ctx, hipGetLastError());
+    kx<<<(unsigned)(nrays / WAVE), WAVE, 0, st>>>(ctx->geom, ctx->model, P, (int)nrays, c, Ap, d_ns, d_off, arena,
+                                                   ctx->d_dump, image, P.only_unpolarized ? nullptr : imageS, taus,
+                                                   nstep, flag, d_status, d_counts);
+    HIPCHK(ctx, hipGetLastError());
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev[2], st));
+  int status[NSTATUS];
+  HIPCHK(ctx, hipMemcpyAsync(status, d_status, sizeof(status), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->counts, d_counts, sizeof(ctx->counts), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  hipEventElapsedTime(&ctx->ms_geo, ctx->ev[0], ctx->ev[1]);
+  hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[2]);
+  ctx->ms_xfer = ctx->ms_total - ctx->ms_geo;
+  ctx->last_chunks = (int)nchunks;
+  ctx->last_overflow = 0;
+  if (status[0] != 0) return set_err(ctx, IPGPU_ERR_STATE, "ipgpu_render_noalloc: store pass step count differs");
+  if (status[1] != 0x7fffffff) {
+    const int p = status[1] - 1;
+    ctx->bad_i = p >> 16;
+    ctx->bad_j = p & 0xffff;
+    snprintf(ctx->err, sizeof(ctx->err), "MaxNStep exceeded in pixel %d %d", ctx->bad_i, ctx->bad_j);
+    return IPGPU_ERR_MAXNSTEP;
+  }
+  if (status[2] != 0x7fffffff) {
+    const int p = status[2] - 1;
+    ctx->bad_i = p >> 16;
+    ctx->bad_j = p & 0xffff;
+    snprintf(ctx->err, sizeof(ctx->err), "NaN in image at pixel %d %d", ctx->bad_i, ctx->bad_j);
+    return IPGPU_ERR_NAN;
   }
   return IPGPU_OK;
 }

@@ -139,6 +139,7 @@ def lib():
         L.ipgpu_bad_job.argtypes = [vp, ip, ip, ip]
         L.ipgpu_set_batch_memory.argtypes = [vp, C.c_size_t]
         L.ipgpu_set_render_memory.argtypes = [vp, C.c_size_t]
+        L.ipgpu_render_noalloc.argtypes = [vp, C.POINTER(Image), vp, vp, vp, vp, vp]
         L.ipgpu_render_info.argtypes = [vp, ip, ip, C.POINTER(C.c_size_t)]
         L.ipgpu_adjoint.argtypes = [vp, C.POINTER(Image), dp, dp, dp, dp, dp]
         L.ipgpu_scale_density.argtypes = [vp, dp]
