@@ -258,6 +258,17 @@ int ipgpu_render_info(const ipgpu_ctx *ctx, int *chunks, int *continued_rays, si
 int ipgpu_adjoint(ipgpu_ctx *ctx, const ipgpu_image *img, const double *target, const double *weights,
                   double out4[4], double *grad_density, double *imageS);
 
+/* Vector-Jacobian product of the polarized render: given dL/dS for the
+ * image ipgpu_render would return (dL_dimageS: nx*ny*5 doubles in imageS
+ * layout; the tauF entries are ignored), the gradient of L with respect to
+ * the dump's M_unit, R_high and R_low (out3) and, if grad_density is not
+ * NULL, to a per-cell scale of n_e at s = 1 (N1*N2*N3 doubles, x3 fastest,
+ * ghost copies folded back), by the same reverse pass as ipgpu_adjoint.
+ * imageS (may be NULL) receives the image.  No counterpart in the
+ * reference; the same restrictions as ipgpu_adjoint. */
+int ipgpu_backward(ipgpu_ctx *ctx, const ipgpu_image *img, const double *dL_dimageS, double out3[3],
+                   double *grad_density, double *imageS);
+
 /* Multiply n_e of every interior cell by scale[N1*N2*N3] (x3 fastest) and
  * refresh its ghost copies: the per-cell density scale whose gradient
  * ipgpu_adjoint returns.  No counterpart in the reference. */

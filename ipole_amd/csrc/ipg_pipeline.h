@@ -225,9 +225,14 @@ IPG_HD void scale_rec(Rec &r, const Scale &sc)
  * Theta_e, B) and of their dependence on the dump (interpolation cell and
  * weights, sigma smoothing, dTheta_e/dR_high, dR_low).  ax[4] = 0 where
  * the step does not radiate or n_e = 0. */
-constexpr int AX = 16;
+/* adjoint record of a step: the coefficient map's inputs, the interpolation
+ * cell and weights, dTheta_e/dR_high, dTheta_e/dR_low, and the derivatives
+ * of the step's affine Stokes map S -> T S + cv with respect to p = ln n_e,
+ * Theta_e, ln B: dT/dp at AX_DT + 16 p (row-major), dcv/dp at AX_DCV + 4 p */
+constexpr int AX = 76;
 constexpr int AX_NU = 0, AX_STH = 1, AX_CTH = 2, AX_EDGE = 3, AX_NE = 4, AX_TH = 5, AX_B = 6, AX_CELL = 7,
-              AX_D1 = 8, AX_D2 = 9, AX_D3 = 10, AX_SMOOTH = 11, AX_DRH = 12, AX_DRL = 13, AX_ZERO = 14;
+              AX_D1 = 8, AX_D2 = 9, AX_D3 = 10, AX_SMOOTH = 11, AX_DRH = 12, AX_DRL = 13, AX_ZERO = 14,
+              AX_DT = 16, AX_DCV = 64;
 
 /* Position of field f in a packet stored with field stride PS.  PS = 1:
  * one packet contiguous.  PS = 64: the packets of the 64 rays of a wave
@@ -256,6 +261,54 @@ struct PkW {
   double *p;
   IPG_HD double &operator[](int f) const { return p[pk_at<PS>(f)]; }
 };
+
+#if !defined(IPG_OPCOUNT)
+/* Derivatives of the step's affine Stokes map S -> T S + cv (stokes_affine
+ * of the coefficients at the step) with respect to p = ln n_e, Theta_e and
+ * ln B, by central differences of the coefficient map (relative step
+ * 1e-5), from the inputs the record ax holds; written to ax[AX_DT + 16 p +
+ * 4 i + j] and ax[AX_DCV + 4 p + i]. */
+IPG_HD void step_param_maps(double *ax, int etype, double dl)
+{
+  Geo q;
+  q.nu = ax[AX_NU];
+  q.sth = ax[AX_STH];
+  q.cth = ax[AX_CTH];
+  q.edge = ax[AX_EDGE] != 0.;
+  const double Ne = ax[AX_NE], Th = ax[AX_TH], B = ax[AX_B];
+  const bool zero = ax[AX_ZERO] != 0.;
+  const double eps = 1.e-5;
+  for (int p = 0; p < 3; p++) {
+    /* ln(1+eps) - ln(1-eps) for the logarithmic directions, 2 eps Theta_e for Theta_e */
+    const double inv = 1. / ((p == 1) ? 2. * eps * Th : (log(1. + eps) - log(1. - eps)));
+    double *dT = ax + AX_DT + 16 * p, *dc = ax + AX_DCV + 4 * p;
+    for (int sgn = 0; sgn < 2; sgn++) {
+      const double f = sgn ? (1. - eps) : (1. + eps);
+      double ne = Ne, th = Th, b = B;
+      if (p == 0) ne *= f;
+      else if (p == 1) th *= f;
+      else b *= f;
+      double ju, ku;
+      Coef c;
+      coefficients_core(q, ne, th, b, etype, true, ju, ku, c);
+      if (zero) c.jI = c.jQ = c.jU = c.jV = 0.;
+      double T[4][4], cv[4];
+      stokes_affine(c, dl, T, cv);
+      if (sgn == 0) {
+        for (int i = 0; i < 4; i++) {
+          for (int j = 0; j < 4; j++) dT[4 * i + j] = T[i][j];
+          dc[i] = cv[i];
+        }
+      } else {
+        for (int i = 0; i < 4; i++) {
+          for (int j = 0; j < 4; j++) dT[4 * i + j] = (dT[4 * i + j] - T[i][j]) * inv;
+          dc[i] = (dc[i] - cv[i]) * inv;
+        }
+      }
+    }
+  }
+}
+#endif
 
 template <bool F32 = false, int PS = 1>
 IPG_HD_INLINE void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int n,
@@ -326,6 +379,9 @@ IPG_HD_INLINE void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const i
       ax[AX_DRH] = ext.dth_rh;
       ax[AX_DRL] = ext.dth_rl;
       ax[AX_ZERO] = zero_emission ? 1. : 0.;
+#if !defined(IPG_OPCOUNT)
+      if (pol) step_param_maps(ax, P.emission_type, dl_cgs);
+#endif
     }
     Coef c;
     double ju, ku;
@@ -1026,39 +1082,6 @@ IPG_HD void herm_axpy(Herm &y, double a, const Herm &x)
   }
 }
 
-/* dF/d(ln n_e), dF/dTheta_e, dF/d(ln B) of F = T(c) S0 + cv(c) at a step. */
-IPG_HD void step_param_derivs(const double *ax, int etype, double dl, const double S0[4], double dF[3][4])
-{
-  Geo q;
-  q.nu = ax[AX_NU];
-  q.sth = ax[AX_STH];
-  q.cth = ax[AX_CTH];
-  q.edge = ax[AX_EDGE] != 0.;
-  const double Ne = ax[AX_NE], Th = ax[AX_TH], B = ax[AX_B];
-  const bool zero = ax[AX_ZERO] != 0.;
-  const double eps = 1.e-5;
-  for (int p = 0; p < 3; p++) {
-    double Fv[2][4];
-    for (int sgn = 0; sgn < 2; sgn++) {
-      const double f = sgn ? (1. - eps) : (1. + eps);
-      double ne = Ne, th = Th, b = B;
-      if (p == 0) ne *= f;
-      else if (p == 1) th *= f;
-      else b *= f;
-      double ju, ku;
-      Coef c;
-      coefficients_core(q, ne, th, b, etype, true, ju, ku, c);
-      if (zero) c.jI = c.jQ = c.jU = c.jV = 0.;
-      double T[4][4], cv[4];
-      stokes_affine(c, dl, T, cv);
-      for (int i = 0; i < 4; i++) Fv[sgn][i] = cv[i] + T[i][0] * S0[0] + T[i][1] * S0[1] + T[i][2] * S0[2] + T[i][3] * S0[3];
-    }
-    /* ln(1+eps) - ln(1-eps) for the logarithmic directions, 2 eps Theta_e for Theta_e */
-    const double den = (p == 1) ? 2. * eps * Th : (log(1. + eps) - log(1. - eps));
-    for (int i = 0; i < 4; i++) dF[p][i] = (Fv[0][i] - Fv[1][i]) / den;
-  }
-}
-
 IPG_HD void atomic_add_f64(double *p, double v)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -1071,13 +1094,15 @@ IPG_HD void atomic_add_f64(double *p, double v)
 
 /* Loss and gradient of one ray.  pkt/aux: its packets and adjoint records
  * (packet 0 is step ns); tgt: the pixel's target I, Q, U, V (output units);
- * w: component weights; ckpt: room for ceil(ns / CK) states; seg: room for
- * CK states; gcell (ghosted cells) receives dL/d(n_e scale) of each cell.
- * out: loss, dL/dln(M_unit), dL/dR_high, dL/dR_low; Simg: the pixel's
- * I, Q, U, V. */
+ * w: component weights; or, with cot non-null, the gradient of an arbitrary
+ * loss with respect to the pixel's I, Q, U, V (output units), in place of
+ * 2 w (S - tgt); ckpt: room for ceil(ns / CK) states; seg: room for CK
+ * states; gcell (ghosted cells) receives dL/d(n_e scale) of each cell.
+ * out: loss (0 with cot), dL/dln(M_unit), dL/dR_high, dL/dR_low; Simg: the
+ * pixel's I, Q, U, V. */
 IPG_HD void adjoint_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const double *pkt, const double *aux,
-                        const double tgt[4], const double w[4], const double *dump, Herm *ckpt, Herm *seg,
-                        double *gcell, double out[4], double Simg[4])
+                        const double tgt[4], const double w[4], const double *cot, const double *dump, Herm *ckpt,
+                        Herm *seg, double *gcell, double out[4], double Simg[4])
 {
   out[0] = out[1] = out[2] = out[3] = 0.;
   /* forward, with checkpoints */
@@ -1099,9 +1124,13 @@ IPG_HD void adjoint_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const
   double sig[4];
   for (int i = 0; i < 4; i++) {
     Simg[i] = f3 * sgn[i] * Sc[i];
-    double r = Simg[i] - tgt[i];
-    out[0] += w[i] * r * r;
-    sig[i] = 2. * w[i] * r * f3 * sgn[i];
+    if (cot) {
+      sig[i] = cot[i] * f3 * sgn[i];
+    } else {
+      double r = Simg[i] - tgt[i];
+      out[0] += w[i] * r * r;
+      sig[i] = 2. * w[i] * r * f3 * sgn[i];
+    }
   }
   /* project_N^T: the rotation by -2 rotcam, then K_e^T with the camera rows */
   const double rc = -2. * P.rotcam, cr = cos(rc), sr = sin(rc);
@@ -1155,10 +1184,16 @@ IPG_HD void adjoint_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const
         for (int i = 0; i < 4; i++) mu[i] *= 2.;
         herm_to_stokes(Nt, pk + PK_e1, pk + PK_e2, S0);
         if (ax[AX_NE] > 0.) {
-          double dF[3][4];
-          step_param_derivs(ax, P.emission_type, pk[PK_DL], S0, dF);
+          /* g_p = mu . (dT/dp S0 + dcv/dp) with the maps the prep pass formed */
           double g[3];
-          for (int p = 0; p < 3; p++) g[p] = mu[0] * dF[p][0] + mu[1] * dF[p][1] + mu[2] * dF[p][2] + mu[3] * dF[p][3];
+          for (int p = 0; p < 3; p++) {
+            const double *dT = ax + AX_DT + 16 * p, *dc = ax + AX_DCV + 4 * p;
+            double acc = 0.;
+            for (int i = 0; i < 4; i++)
+              acc += mu[i] * (dc[i] + dT[4 * i] * S0[0] + dT[4 * i + 1] * S0[1] + dT[4 * i + 2] * S0[2]
+                              + dT[4 * i + 3] * S0[3]);
+            g[p] = acc;
+          }
           out[1] += g[0] + 0.5 * g[2];
           out[2] += g[1] * ax[AX_DRH];
           out[3] += g[1] * ax[AX_DRL];

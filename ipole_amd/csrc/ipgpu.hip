@@ -755,6 +755,7 @@ __global__ __launch_bounds__(WAVE) void k_adjoint(ipgpu_geom G, ipgpu_image P, i
                                                   const int *__restrict__ nstep_in,
                                                   const double *__restrict__ pkt, const double *__restrict__ aux,
                                                   const double *__restrict__ target, AdjW w,
+                                                  const double *__restrict__ cot,
                                                   const double *__restrict__ dump, Herm *__restrict__ ckpt,
                                                   Herm *__restrict__ segbuf, double *__restrict__ gcell,
                                                   double *__restrict__ out4, double *__restrict__ imageS,
@@ -773,9 +774,13 @@ __global__ __launch_bounds__(WAVE) void k_adjoint(ipgpu_geom G, ipgpu_image P, i
     atomicMin(&status[1], (int)p + 1);
     return;
   }
-  const double tg[4] = {target[5 * p], target[5 * p + 1], target[5 * p + 2], target[5 * p + 3]};
+  double tg[4] = {0., 0., 0., 0.}, cv[4] = {0., 0., 0., 0.};
+  for (int q = 0; q < 4; q++) {
+    if (cot) cv[q] = cot[5 * p + q];
+    else tg[q] = target[5 * p + q];
+  }
   double res[4], S[4];
-  adjoint_ray(G, P, ns, pkt + (size_t)off[ray] * PK, aux + (size_t)off[ray] * AX, tg, w.w, dump,
+  adjoint_ray(G, P, ns, pkt + (size_t)off[ray] * PK, aux + (size_t)off[ray] * AX, tg, w.w, cot ? cv : nullptr, dump,
               ckpt + off[ray] / CK + ray, segbuf + (size_t)ray * CK, gcell, res, S);
   for (int q = 0; q < 4; q++) {
     o[q] = res[q];
@@ -3292,10 +3297,13 @@ static void fold_ghosts(const ipgpu_geom &G, std::vector<double> &g, double *int
       for (int k = 0; k < N3; k++) interior[((size_t)i * N2 + j) * N3 + k] = at(i + 1, j + 1, k + 1);
 }
 
-int ipgpu_adjoint(ipgpu_ctx *ctx, const ipgpu_image *img, const double *target, const double *weights,
-                  double out4[4], double *grad_density, double *imageS)
+/* The adjoint pass shared by ipgpu_adjoint (target and weights: the L2
+ * loss) and ipgpu_backward (cot: the gradient of any loss with respect to
+ * the output image). */
+static int run_adjoint(ipgpu_ctx *ctx, const ipgpu_image *img, const double *target, const double *weights,
+                       const double *cot, double out4[4], double *grad_density, double *imageS)
 {
-  if (!ctx || !img || !target || !out4) return IPGPU_ERR_ARG;
+  if (!ctx || !img || !(target || cot) || !out4) return IPGPU_ERR_ARG;
   if (!ctx->has_geom || !ctx->has_model || !ctx->d_dump || ctx->model.kind != IPGPU_MODEL_IHARM)
     return set_err(ctx, IPGPU_ERR_STATE, "the adjoint needs an iharm dump");
   if (!ctx->has_units) return set_err(ctx, IPGPU_ERR_STATE, "the adjoint needs a dump from ipgpu_upload_prims");
@@ -3322,12 +3330,14 @@ int ipgpu_adjoint(ipgpu_ctx *ctx, const ipgpu_image *img, const double *target, 
   unsigned long long total;
   if ((rc = prepare_packets(ctx, P, nrays, true, cap, total, stride))) return rc;
   if ((rc = grow(ctx, ctx->d_ckpt, ctx->cap_ckpt, (size_t)(total / CK + nrays + 1)))) return rc;
-  HIPCHK(ctx, hipMemcpyAsync(ctx->d_target, target, npix * 5 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_target, cot ? cot : target, npix * 5 * sizeof(double), hipMemcpyHostToDevice,
+                             ctx->stream));
   HIPCHK(ctx, hipMemsetAsync(ctx->d_gcell, 0, nc * sizeof(double), ctx->stream));
   AdjW w;
   for (int q = 0; q < 4; q++) w.w[q] = weights ? weights[q] : 1.;
   k_adjoint<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(G, P, nrays, ctx->d_off, ctx->d_nstep_ray, ctx->d_pkt, ctx->d_aux,
-                                                     ctx->d_target, w, ctx->d_dump, ctx->d_ckpt, ctx->d_seg,
+                                                     cot ? nullptr : ctx->d_target, w, cot ? ctx->d_target : nullptr,
+                                                     ctx->d_dump, ctx->d_ckpt, ctx->d_seg,
                                                      ctx->d_gcell, ctx->d_out4, ctx->d_imageS, ctx->d_status);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
@@ -3359,6 +3369,24 @@ int ipgpu_adjoint(ipgpu_ctx *ctx, const ipgpu_image *img, const double *target, 
     fold_ghosts(G, g, grad_density);
   }
   if (imageS) HIPCHK(ctx, hipMemcpy(imageS, ctx->d_imageS, npix * 5 * sizeof(double), hipMemcpyDeviceToHost));
+  return IPGPU_OK;
+}
+
+int ipgpu_adjoint(ipgpu_ctx *ctx, const ipgpu_image *img, const double *target, const double *weights,
+                  double out4[4], double *grad_density, double *imageS)
+{
+  if (!target) return set_err(ctx, IPGPU_ERR_ARG, "ipgpu_adjoint: no target image");
+  return run_adjoint(ctx, img, target, weights, nullptr, out4, grad_density, imageS);
+}
+
+int ipgpu_backward(ipgpu_ctx *ctx, const ipgpu_image *img, const double *dL_dimageS, double out3[3],
+                   double *grad_density, double *imageS)
+{
+  if (!dL_dimageS || !out3) return set_err(ctx, IPGPU_ERR_ARG, "ipgpu_backward: no output gradient");
+  double out4[4];
+  int rc = run_adjoint(ctx, img, nullptr, nullptr, dL_dimageS, out4, grad_density, imageS);
+  if (rc) return rc;
+  for (int q = 0; q < 3; q++) out3[q] = out4[q + 1];
   return IPGPU_OK;
 }
 

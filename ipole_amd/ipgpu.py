@@ -142,6 +142,7 @@ def lib():
         L.ipgpu_render_noalloc.argtypes = [vp, C.POINTER(Image), vp, vp, vp, vp, vp]
         L.ipgpu_render_info.argtypes = [vp, ip, ip, C.POINTER(C.c_size_t)]
         L.ipgpu_adjoint.argtypes = [vp, C.POINTER(Image), dp, dp, dp, dp, dp]
+        L.ipgpu_backward.argtypes = [vp, C.POINTER(Image), dp, dp, dp, dp]
         L.ipgpu_scale_density.argtypes = [vp, dp]
         L.ipgpu_kernel_times.argtypes = [vp, dp, dp, dp]
         L.ipgpu_stage_times.argtypes = [vp, dp]
@@ -404,6 +405,28 @@ class Context:
             raise IpgpuError(rc, "MaxNStep exceeded")
         self._chk(rc)
         return dict(loss=out4[0], dM_unit=out4[1], dR_high=out4[2], dR_low=out4[3],
+                    density=dens.reshape(g.N1, g.N2, g.N3) if dens is not None else None,
+                    imageS=S.reshape(image.nx, image.ny, 5))
+
+    def backward(self, image, dL_dimageS, density=True):
+        """Vector-Jacobian product: dL/dimageS (nx, ny, 5 or 4) of the image
+        the current dump renders -> dict(dM_unit, dR_high, dR_low, density
+        (N1, N2, N3) dL/d(n_e scale) at 1 or None, imageS)."""
+        n = image.nx * image.ny
+        cot = np.zeros((n, 5))
+        c = np.asarray(dL_dimageS, dtype=np.float64).reshape(n, -1)
+        cot[:, :4] = c[:, :4]
+        cot = np.ascontiguousarray(cot).reshape(-1)
+        out3 = np.zeros(3)
+        g = self.geom
+        dens = np.zeros(g.N1 * g.N2 * g.N3) if density else None
+        S = np.zeros(5 * n)
+        rc = self.L.ipgpu_backward(self.h, C.byref(image), P(cot), P(out3), P(dens) if dens is not None else None,
+                                   P(S))
+        if rc == ERR_MAXNSTEP:
+            raise IpgpuError(rc, "MaxNStep exceeded")
+        self._chk(rc)
+        return dict(dM_unit=out3[0], dR_high=out3[1], dR_low=out3[2],
                     density=dens.reshape(g.N1, g.N2, g.N3) if dens is not None else None,
                     imageS=S.reshape(image.nx, image.ny, 5))
 

@@ -221,7 +221,7 @@ int emu_adjoint(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P,
       ckpt.resize(ns / CK + 1);
       double o[4];
       const double tg[4] = {target[5 * n], target[5 * n + 1], target[5 * n + 2], target[5 * n + 3]};
-      adjoint_ray(*G, *P, ns, pkt.data(), aux.data(), tg, w, dump, ckpt.data(), seg.data(), gcell, o,
+      adjoint_ray(*G, *P, ns, pkt.data(), aux.data(), tg, w, nullptr, dump, ckpt.data(), seg.data(), gcell, o,
                   Simg + 4 * (size_t)n);
       for (int q = 0; q < 4; q++) loc[q] += o[q];
     }
