@@ -83,19 +83,6 @@ def roofline(stage_ms, n_geo, streamed):
                      % (dom, per_unit[dom], ", ".join("%s %.3f" % (k, v) for k, v in ach.items())))
 
 
-def c2_jobs(seed=0, n=64):
-    """64 jobs on the 288x128x128 dump: thetacam in {17, 50, 90, 130, 163},
-    {86, 230, 345} GHz, M_unit log-uniform over 1e24-1e26, image sizes from
-    {64, 96, 128, 192, 256}^2, drawn from a seeded generator."""
-    rng = np.random.default_rng(seed)
-    th = rng.choice([17., 50., 90., 130., 163.], n)
-    fr = rng.choice([86e9, 230e9, 345e9], n)
-    mu = 10 ** rng.uniform(24., 26., n)
-    sz = rng.choice([64, 96, 128, 192, 256], n)
-    return [dict(thetacam=float(t), freqcgs=float(f), M_unit=float(m), n=int(z))
-            for t, f, m, z in zip(th, fr, mu, sz)]
-
-
 def batch_measure(args, h=None, prims=None):
     """Batched render of the 64-job set in one call (FP64, then mixed
     precision) against the same jobs rendered one by one with the plain path
@@ -105,7 +92,7 @@ def batch_measure(args, h=None, prims=None):
     cfg = dict(C1)
     if h is None:
         h, prims = synth.make_dump(*cfg["N"], a=cfg["a"], seed=cfg["seed"])
-    jobs = c2_jobs()
+    jobs = synth.batch_jobs()
     if args.small:
         jobs = jobs[:8]
     M_ref = 1e25

@@ -175,3 +175,17 @@ def make_dump(N1, N2, N3, a=0.9375, seed=1234, noise_sigma=0.3, vel_noise=0.05, 
     prims[..., 5:8] *= scale
     h["seed"] = seed
     return h, prims
+
+
+def batch_jobs(seed=0, n=64):
+    """The batching workload's job set on the 288x128x128 dump: n jobs with
+    thetacam from {17, 50, 90, 130, 163}, frequency from {86, 230, 345} GHz,
+    M_unit log-uniform over 1e24-1e26 and image side from {64, 96, 128, 192,
+    256}, drawn from a seeded generator."""
+    rng = np.random.default_rng(seed)
+    th = rng.choice([17., 50., 90., 130., 163.], n)
+    fr = rng.choice([86e9, 230e9, 345e9], n)
+    mu = 10 ** rng.uniform(24., 26., n)
+    sz = rng.choice([64, 96, 128, 192, 256], n)
+    return [dict(thetacam=float(t), freqcgs=float(f), M_unit=float(m), n=int(z))
+            for t, f, m, z in zip(th, fr, mu, sz)]
