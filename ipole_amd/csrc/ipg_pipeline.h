@@ -1092,76 +1092,47 @@ IPG_HD void atomic_add_f64(double *p, double v)
 #endif
 }
 
-/* Loss and gradient of one ray.  pkt/aux: its packets and adjoint records
- * (packet 0 is step ns); tgt: the pixel's target I, Q, U, V (output units);
- * w: component weights; or, with cot non-null, the gradient of an arbitrary
- * loss with respect to the pixel's I, Q, U, V (output units), in place of
- * 2 w (S - tgt); ckpt: room for ceil(ns / CK) states; seg: room for CK
- * states; gcell (ghosted cells) receives dL/d(n_e scale) of each cell.
- * out: loss (0 with cot), dL/dln(M_unit), dL/dR_high, dL/dR_low; Simg: the
- * pixel's I, Q, U, V. */
-IPG_HD void adjoint_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const double *pkt, const double *aux,
-                        const double tgt[4], const double w[4], const double *cot, const double *dump, Herm *ckpt,
-                        Herm *seg, double *gcell, double out[4], double Simg[4])
+/* The reverse pass over steps [q0, q1) of one ray's recurrence: from the
+ * state Nq0 before step q0 (checkpointed every CK steps on a forward pass,
+ * recomputed window by window) and the adjoint state L after step q1 - 1,
+ * accumulates g3 += (dL/dln M_unit, dL/dR_high, dL/dR_low) and the per-cell
+ * gradients, and leaves in L the adjoint state before step q0.  pkt/aux:
+ * the ray's packets and adjoint records (packet q is step ns - q); ckpt:
+ * room for ceil((q1 - q0) / CK) states; seg: room for CK states. */
+IPG_HD void adjoint_range(const ipgpu_geom &G, const ipgpu_image &P, const double *pkt, const double *aux, int q0,
+                          int q1, const Herm &Nq0, Herm &L, const double *dump, Herm *ckpt, Herm *seg, double *gcell,
+                          double g3[3])
 {
-  out[0] = out[1] = out[2] = out[3] = 0.;
-  /* forward, with checkpoints */
   Chain st;
   chain_init(st, 0., 0.);
-  for (int q = 0; q < ns; q++) {
-    if (q % CK == 0) ckpt[q / CK] = st.N;
+  st.N = Nq0;
+  st.Nnz = herm_nonzero(Nq0);
+  if (q0 > 0)
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) st.Mi[i][j] = pkt[(size_t)(q0 - 1) * PK + PK_MF + 4 * i + j];
+  for (int q = q0; q < q1; q++) {
+    if ((q - q0) % CK == 0) ckpt[(q - q0) / CK] = st.N;
     chain_step(st, pkt + (size_t)q * PK, true, 0);
   }
-  double E[4][4];
-  Metric mc;
-  const double Xc[4] = {P.Xcam[0], P.Xcam[1], P.Xcam[2], P.Xcam[3]};
-  camera_tetrad(G, Xc, 0, E, mc);
-  double Sc[4];
-  project(st.N, E, mc, P.rotcam, Sc);
-  const double f3 = P.freqcgs * P.freqcgs * P.freqcgs;
-  const double qs = (P.qu_conv == 0) ? -1. : 1.;
-  const double sgn[4] = {1., qs, qs, 1.};
-  double sig[4];
-  for (int i = 0; i < 4; i++) {
-    Simg[i] = f3 * sgn[i] * Sc[i];
-    if (cot) {
-      sig[i] = cot[i] * f3 * sgn[i];
-    } else {
-      double r = Simg[i] - tgt[i];
-      out[0] += w[i] * r * r;
-      sig[i] = 2. * w[i] * r * f3 * sgn[i];
-    }
-  }
-  /* project_N^T: the rotation by -2 rotcam, then K_e^T with the camera rows */
-  const double rc = -2. * P.rotcam, cr = cos(rc), sr = sin(rc);
-  double tau[4] = {0.5 * sig[0], 0.5 * (sig[1] * cr + sig[2] * sr), 0.5 * (-sig[1] * sr + sig[2] * cr), 0.5 * sig[3]};
-  double e1c[4], e2c[4];
-  lower(mc.g, E[1], e1c);
-  lower(mc.g, E[2], e2c);
-  Herm L;
-  for (int q = 0; q < 4; q++) L.d[q] = 0.;
-  for (int q = 0; q < 6; q++) L.re[q] = L.im[q] = 0.;
-  herm_add_stokes(L, e1c, e2c, tau);
-  /* backward, segment by segment */
   const long s2 = G.N3 + 2, s1 = (long)(G.N2 + 2) * (G.N3 + 2);
-  const int nseg = (ns + CK - 1) / CK;
-  for (int k = nseg - 1; k >= 0; k--) {
-    const int q0 = k * CK, q1 = (q0 + CK < ns) ? q0 + CK : ns;
+  const int nwin = (q1 - q0 + CK - 1) / CK;
+  for (int k = nwin - 1; k >= 0; k--) {
+    const int w0 = q0 + k * CK, w1 = (w0 + CK < q1) ? w0 + CK : q1;
     Chain rs;
     chain_init(rs, 0., 0.);
     rs.N = ckpt[k];
     rs.Nnz = herm_nonzero(rs.N);
-    if (q0 > 0)
+    if (w0 > 0)
       for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 4; j++) rs.Mi[i][j] = pkt[(size_t)(q0 - 1) * PK + PK_MF + 4 * i + j];
-    for (int q = q0; q < q1; q++) {
-      seg[q - q0] = rs.N;
+        for (int j = 0; j < 4; j++) rs.Mi[i][j] = pkt[(size_t)(w0 - 1) * PK + PK_MF + 4 * i + j];
+    for (int q = w0; q < w1; q++) {
+      seg[q - w0] = rs.N;
       chain_step(rs, pkt + (size_t)q * PK, true, 0);
     }
-    for (int q = q1 - 1; q >= q0; q--) {
+    for (int q = w1 - 1; q >= w0; q--) {
       const double *pk = pkt + (size_t)q * PK;
       const double *ax = aux + (size_t)q * AX;
-      const Herm &Nb = seg[q - q0];
+      const Herm &Nb = seg[q - w0];
       const bool nnz = herm_nonzero(Nb);
       double Mi[4][4], Mh[4][4];
       if (nnz) {
@@ -1194,9 +1165,9 @@ IPG_HD void adjoint_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const
                               + dT[4 * i + 3] * S0[3]);
             g[p] = acc;
           }
-          out[1] += g[0] + 0.5 * g[2];
-          out[2] += g[1] * ax[AX_DRH];
-          out[3] += g[1] * ax[AX_DRL];
+          g3[0] += g[0] + 0.5 * g[2];
+          g3[1] += g[1] * ax[AX_DRH];
+          g3[2] += g[1] * ax[AX_DRL];
           if (gcell && ax[AX_CELL] >= 0.) {
             const long base = (long)ax[AX_CELL];
             const double d[3] = {ax[AX_D1], ax[AX_D2], ax[AX_D3]};
@@ -1230,6 +1201,144 @@ IPG_HD void adjoint_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const
       }
     }
   }
+}
+
+/* The pixel's I, Q, U, V (output units) from the final state N and the
+ * adjoint state the loss puts on N: with cot null, the L2 loss against tgt
+ * with weights w (loss returned), else the given dL/d(I, Q, U, V). */
+IPG_HD double adjoint_seed(const ipgpu_geom &G, const ipgpu_image &P, const Herm &N, const double tgt[4],
+                           const double w[4], const double *cot, double Simg[4], Herm &L)
+{
+  double E[4][4];
+  Metric mc;
+  const double Xc[4] = {P.Xcam[0], P.Xcam[1], P.Xcam[2], P.Xcam[3]};
+  camera_tetrad(G, Xc, 0, E, mc);
+  double Sc[4];
+  project(N, E, mc, P.rotcam, Sc);
+  const double f3 = P.freqcgs * P.freqcgs * P.freqcgs;
+  const double qs = (P.qu_conv == 0) ? -1. : 1.;
+  const double sgn[4] = {1., qs, qs, 1.};
+  double sig[4], loss = 0.;
+  for (int i = 0; i < 4; i++) {
+    Simg[i] = f3 * sgn[i] * Sc[i];
+    if (cot) {
+      sig[i] = cot[i] * f3 * sgn[i];
+    } else {
+      double r = Simg[i] - tgt[i];
+      loss += w[i] * r * r;
+      sig[i] = 2. * w[i] * r * f3 * sgn[i];
+    }
+  }
+  /* project_N^T: the rotation by -2 rotcam, then K_e^T with the camera rows */
+  const double rc = -2. * P.rotcam, cr = cos(rc), sr = sin(rc);
+  double tau[4] = {0.5 * sig[0], 0.5 * (sig[1] * cr + sig[2] * sr), 0.5 * (-sig[1] * sr + sig[2] * cr), 0.5 * sig[3]};
+  double e1c[4], e2c[4];
+  lower(mc.g, E[1], e1c);
+  lower(mc.g, E[2], e2c);
+  for (int q = 0; q < 4; q++) L.d[q] = 0.;
+  for (int q = 0; q < 6; q++) L.re[q] = L.im[q] = 0.;
+  herm_add_stokes(L, e1c, e2c, tau);
+  return loss;
+}
+
+/* Loss and gradient of one ray in one lane (sequential: forward pass,
+ * then the reverse pass over all steps).  pkt/aux: its packets and adjoint
+ * records (packet 0 is step ns); tgt, w, cot: as adjoint_seed; ckpt: room
+ * for ceil(ns / CK) states; seg: room for CK states; gcell (ghosted cells)
+ * receives dL/d(n_e scale) of each cell.  out: loss, dL/dln(M_unit),
+ * dL/dR_high, dL/dR_low; Simg: the pixel's I, Q, U, V. */
+IPG_HD void adjoint_ray(const ipgpu_geom &G, const ipgpu_image &P, int ns, const double *pkt, const double *aux,
+                        const double tgt[4], const double w[4], const double *cot, const double *dump, Herm *ckpt,
+                        Herm *seg, double *gcell, double out[4], double Simg[4])
+{
+  Chain st;
+  chain_init(st, 0., 0.);
+  for (int q = 0; q < ns; q++) chain_step(st, pkt + (size_t)q * PK, true, 0);
+  Herm L;
+  out[0] = adjoint_seed(G, P, st.N, tgt, w, cot, Simg, L);
+  Herm Z;
+  for (int q = 0; q < 4; q++) Z.d[q] = 0.;
+  for (int q = 0; q < 6; q++) Z.re[q] = Z.im[q] = 0.;
+  double g3[3] = {0., 0., 0.};
+  adjoint_range(G, P, pkt, aux, 0, ns, Z, L, dump, ckpt, seg, gcell, g3);
+  out[1] = g3[0];
+  out[2] = g3[1];
+  out[3] = g3[2];
+}
+
+/* ---- the reverse pass in segments of a ray's steps (parallel in time)
+ *
+ * A ray's ns steps are cut into segments [k Ls, min((k+1) Ls, ns)) from the
+ * far end.  The map of segment k, N -> A_k N + b_k, is formed column by
+ * column (segment_column: A_k's 16 columns from the basis states, b_k from
+ * N = 0 with sources; the first segment starts from N = 0 and needs b_0
+ * only).  One lane per ray composes them: the state before every segment,
+ * the final state and the loss's adjoint state, then the adjoint states
+ * after every segment, backwards with the transposed maps.  The adjoint of
+ * the linear map A in the parameters of Hermitian N (d[4], re[6], im[6]),
+ * for the pairing Re tr(Lambda^dagger N) = sum_b W_b Lambda_b N_b with
+ * W = 1 on the diagonal and 2 off it, is W^-1 A^T W.  Every segment's
+ * reverse pass (adjoint_range) then runs in a lane of its own. */
+constexpr int ASEG_MAP = (NPAR + 1) * NPAR;   /* doubles per segment map: 16 columns + source */
+
+IPG_HD_INLINE double herm_w(int b) { return b < 4 ? 1. : 2.; }
+
+/* out = A N + b with the map of one segment */
+IPG_HD_INLINE void seg_map_apply(const double *map, const Herm &N, Herm &out)
+{
+#pragma unroll
+  for (int c = 0; c < NPAR; c++) {
+    double acc = map[NPAR * NPAR + c];
+#pragma unroll
+    for (int b = 0; b < NPAR; b++) acc += map[b * NPAR + c] * herm_par(N, b);
+    herm_par(out, c) = acc;
+  }
+}
+
+/* out = W^-1 A^T W L: the adjoint state before a segment from the one after */
+IPG_HD_INLINE void seg_map_apply_T(const double *map, const Herm &L, Herm &out)
+{
+#pragma unroll
+  for (int b = 0; b < NPAR; b++) {
+    double acc = 0.;
+#pragma unroll
+    for (int c = 0; c < NPAR; c++) acc += map[b * NPAR + c] * herm_w(c) * herm_par(L, c);
+    herm_par(out, b) = acc / herm_w(b);
+  }
+}
+
+/* The composition of one ray of nseg segments (maps at map + k ASEG_MAP):
+ * Nstart[k] the state before segment k, Lend[k] the adjoint state after
+ * it; returns the loss (0 with cot) and the pixel's I, Q, U, V. */
+IPG_HD double adjoint_compose(const ipgpu_geom &G, const ipgpu_image &P, int nseg, const double *map,
+                              const double tgt[4], const double w[4], const double *cot, Herm *Nstart, Herm *Lend,
+                              double Simg[4])
+{
+  Herm N;
+  for (int q = 0; q < 4; q++) N.d[q] = 0.;
+  for (int q = 0; q < 6; q++) N.re[q] = N.im[q] = 0.;
+  for (int k = 0; k < nseg; k++) {
+    Nstart[k] = N;
+    if (k == 0) {
+#pragma unroll
+      for (int c = 0; c < NPAR; c++) herm_par(N, c) = map[NPAR * NPAR + c];
+    } else {
+      Herm t;
+      seg_map_apply(map + (size_t)k * ASEG_MAP, N, t);
+      N = t;
+    }
+  }
+  Herm L;
+  const double loss = adjoint_seed(G, P, N, tgt, w, cot, Simg, L);
+  for (int k = nseg - 1; k >= 0; k--) {
+    Lend[k] = L;
+    if (k > 0) {
+      Herm t;
+      seg_map_apply_T(map + (size_t)k * ASEG_MAP, L, t);
+      L = t;
+    }
+  }
+  return loss;
 }
 #endif
 

@@ -747,27 +747,49 @@ __global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model M
 
 /* ------------------------------------------------------------- adjoint */
 
-/* One lane per ray: loss and gradient of its pixel (adjoint_ray), per-ray
- * sums to out4 (loss, dL/dln M_unit, dL/dR_high, dL/dR_low), the pixel's
- * I Q U V to imageS. */
-__global__ __launch_bounds__(WAVE) void k_adjoint(ipgpu_geom G, ipgpu_image P, int nrays,
-                                                  const long long *__restrict__ off,
-                                                  const int *__restrict__ nstep_in,
-                                                  const double *__restrict__ pkt, const double *__restrict__ aux,
-                                                  const double *__restrict__ target, AdjW w,
-                                                  const double *__restrict__ cot,
-                                                  const double *__restrict__ dump, Herm *__restrict__ ckpt,
-                                                  Herm *__restrict__ segbuf, double *__restrict__ gcell,
-                                                  double *__restrict__ out4, double *__restrict__ imageS,
-                                                  int *__restrict__ status)
+/* The reverse pass in segments (parallel in time).  Segment s of the
+ * adjoint covers packets [seg_q0[s], seg_q1[s]) of ray seg_ray[s]; ray r's
+ * segments are ray_seg0[r] .. ray_seg0[r+1] - 1, in order from the far end.
+ *
+ * One lane per (segment, column) of the segments' maps: column lane_col[l]
+ * of segment lane_seg[l] (0..15 from the basis states, 16 the source term;
+ * a ray's first segment has its source column only). */
+__global__ __launch_bounds__(WAVE) void k_adj_map(int nlanes, const int *__restrict__ lane_seg,
+                                                  const int *__restrict__ lane_col, const int *__restrict__ seg_ray,
+                                                  const int *__restrict__ seg_q0, const int *__restrict__ seg_q1,
+                                                  const long long *__restrict__ off, const double *__restrict__ pkt,
+                                                  double *__restrict__ map)
 {
-  int ray = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nlanes) return;
+  const int sg = lane_seg[l], b = lane_col[l];
+  const int r = seg_ray[sg];
+  Herm N;
+  segment_column<1>(pkt + (size_t)off[r] * PK, seg_q0[sg], seg_q1[sg], b, N);
+  double *o = map + (size_t)sg * ASEG_MAP + (size_t)b * NPAR;
+#pragma unroll
+  for (int p = 0; p < NPAR; p++) o[p] = herm_par(N, p);
+}
+
+/* One lane per ray: the composition of its segments' maps (adjoint_compose):
+ * states before its segments, its pixel's I, Q, U, V and loss, adjoint
+ * states after its segments.  cot (imageS layout) replaces the L2 loss
+ * against target with weights w when not null. */
+__global__ __launch_bounds__(WAVE) void k_adj_compose(ipgpu_geom G, ipgpu_image P, int nrays,
+                                                      const int *__restrict__ nstep_in,
+                                                      const int *__restrict__ ray_seg0,
+                                                      const double *__restrict__ map,
+                                                      const double *__restrict__ target, AdjW w,
+                                                      const double *__restrict__ cot, Herm *__restrict__ Nstart,
+                                                      Herm *__restrict__ Lend, double *__restrict__ loss,
+                                                      double *__restrict__ imageS, int *__restrict__ status)
+{
+  IPG_LATENCY_CRITICAL();
+  const int ray = blockIdx.x * blockDim.x + threadIdx.x;
   if (ray >= nrays) return;
+  loss[ray] = 0.;
   int pi, pj;
-  ray_pixel(nullptr, ray, P.nx, P.ny, pi, pj);
-  double *o = out4 + (size_t)ray * 4;
-  o[0] = o[1] = o[2] = o[3] = 0.;
-  if (pi >= P.nx || pj >= P.ny) return;
+  if (!ray_pixel(nullptr, ray, P.nx, P.ny, pi, pj)) return;
   const int ns = nstep_in[ray];
   const size_t p = (size_t)pi * P.ny + pj;
   if (ns >= P.maxnstep - 1) {
@@ -779,14 +801,35 @@ __global__ __launch_bounds__(WAVE) void k_adjoint(ipgpu_geom G, ipgpu_image P, i
     if (cot) cv[q] = cot[5 * p + q];
     else tg[q] = target[5 * p + q];
   }
-  double res[4], S[4];
-  adjoint_ray(G, P, ns, pkt + (size_t)off[ray] * PK, aux + (size_t)off[ray] * AX, tg, w.w, cot ? cv : nullptr, dump,
-              ckpt + off[ray] / CK + ray, segbuf + (size_t)ray * CK, gcell, res, S);
-  for (int q = 0; q < 4; q++) {
-    o[q] = res[q];
-    imageS[5 * p + q] = S[q];
-  }
+  const int s0 = ray_seg0[ray], nseg = ray_seg0[ray + 1] - s0;
+  double S[4];
+  loss[ray] = adjoint_compose(G, P, nseg, map + (size_t)s0 * ASEG_MAP, tg, w.w, cot ? cv : nullptr, Nstart + s0,
+                              Lend + s0, S);
+  for (int q = 0; q < 4; q++) imageS[5 * p + q] = S[q];
   imageS[5 * p + 4] = 0.;
+}
+
+/* One lane per segment: its reverse pass (adjoint_range) from the state
+ * before it and the adjoint state after it; g3 per segment, per-cell
+ * gradients by atomics. */
+__global__ __launch_bounds__(WAVE) void k_adj_seg(ipgpu_geom G, ipgpu_image P, int nseg,
+                                                  const int *__restrict__ seg_ray, const int *__restrict__ seg_q0,
+                                                  const int *__restrict__ seg_q1, const long long *__restrict__ off,
+                                                  const double *__restrict__ pkt, const double *__restrict__ aux,
+                                                  const double *__restrict__ dump, const Herm *__restrict__ Nstart,
+                                                  const Herm *__restrict__ Lend, int nck, Herm *__restrict__ ckpt,
+                                                  Herm *__restrict__ segbuf, double *__restrict__ gcell,
+                                                  double *__restrict__ g3)
+{
+  IPG_LATENCY_CRITICAL();
+  const int sg = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sg >= nseg) return;
+  const int r = seg_ray[sg];
+  Herm L = Lend[sg];
+  double g[3] = {0., 0., 0.};
+  adjoint_range(G, P, pkt + (size_t)off[r] * PK, aux + (size_t)off[r] * AX, seg_q0[sg], seg_q1[sg], Nstart[sg], L,
+                dump, ckpt + (size_t)sg * nck, segbuf + (size_t)sg * CK, gcell, g);
+  for (int q = 0; q < 3; q++) g3[(size_t)sg * 3 + q] = g[q];
 }
 
 /* n_e of interior cell (i, j, k) times scale[i][j][k] */
@@ -1501,6 +1544,14 @@ struct ipgpu_ctx {
   bool has_units = false;
   Herm *d_ckpt = nullptr, *d_seg = nullptr;
   size_t cap_ckpt = 0, cap_seg = 0;
+  /* adjoint in segments: steps per segment, segment and lane tables, maps,
+   * states before and adjoint states after each segment, per-segment sums */
+  int adj_seg_len = 512;
+  std::vector<int> h_aseg, h_alane, h_aray;
+  int *d_aint = nullptr;
+  double *d_amap = nullptr, *d_ag3 = nullptr;
+  Herm *d_aN = nullptr, *d_aL = nullptr;
+  size_t cap_aint = 0, cap_amap = 0, cap_ag3 = 0, cap_aN = 0, cap_aL = 0;
   double *d_gcell = nullptr, *d_out4 = nullptr, *d_target = nullptr, *d_scale = nullptr;
   /* the pixel list of the rays being rendered (nullptr: the image's tiles)
    * and the row length of the output image they are written to */
@@ -1611,6 +1662,7 @@ int ipgpu_create(ipgpu_ctx **out, int device)
   if (const char *e = getenv("IPGPU_REFINE_SPECULATE_MAX")) ctx->refine_spec_max = atoll(e);
   if (const char *e = getenv("IPGPU_TAIL_PREFIX")) ctx->tail_prefix = atoi(e);
   if (const char *e = getenv("IPGPU_BATCH_SLOT")) ctx->batch_slot = atoi(e);
+  if (const char *e = getenv("IPGPU_ADJ_SEG_LEN")) ctx->adj_seg_len = atoi(e) > 0 ? atoi(e) : 512;
   if (hipMalloc((void **)&ctx->d_status, NSTATUS * sizeof(int)) != hipSuccess
       || hipMalloc((void **)&ctx->d_counts, 2 * sizeof(unsigned long long)) != hipSuccess) {
     delete ctx;
@@ -1642,7 +1694,8 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
                   ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_ckpt, ctx->d_seg, ctx->d_gcell,
                   ctx->d_out4, ctx->d_target, ctx->d_scale, ctx->d_trrec, ctx->d_trout, ctx->d_trnt,
                   ctx->d_rlist, ctx->d_rcount, ctx->d_iavg, ctx->d_interp, ctx->d_ppkt, ctx->d_plist, ctx->d_traj2, ctx->d_off2,
-                  ctx->d_pidx, ctx->d_trajO, ctx->d_ovl, ctx->d_ovoff, ctx->d_rbase, ctx->d_pixlist};
+                  ctx->d_pidx, ctx->d_trajO, ctx->d_ovl, ctx->d_ovoff, ctx->d_rbase, ctx->d_pixlist,
+                  ctx->d_aint, ctx->d_amap, ctx->d_ag3, ctx->d_aN, ctx->d_aL};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int q = 0; q < NEV; q++)
@@ -3323,27 +3376,94 @@ static int run_adjoint(ipgpu_ctx *ctx, const ipgpu_image *img, const double *tar
   if ((rc = grow(ctx, ctx->d_nstep_ray, ctx->cap_nstep_ray, (size_t)nrays))) return rc;
   if ((rc = grow(ctx, ctx->d_imageS, ctx->cap_imageS, npix * 5))) return rc;
   if ((rc = grow(ctx, ctx->d_target, ctx->cap_target, npix * 5))) return rc;
-  if ((rc = grow(ctx, ctx->d_out4, ctx->cap_out4, (size_t)nrays * 4))) return rc;
+  if ((rc = grow(ctx, ctx->d_out4, ctx->cap_out4, (size_t)nrays))) return rc;
   if ((rc = grow(ctx, ctx->d_gcell, ctx->cap_gcell, nc))) return rc;
-  if ((rc = grow(ctx, ctx->d_seg, ctx->cap_seg, (size_t)nrays * CK))) return rc;
   int cap, stride;
   unsigned long long total;
   if ((rc = prepare_packets(ctx, P, nrays, true, cap, total, stride))) return rc;
-  if ((rc = grow(ctx, ctx->d_ckpt, ctx->cap_ckpt, (size_t)(total / CK + nrays + 1)))) return rc;
+  /* segments of every ray (prepare_packets has synchronised: the step
+   * counts are final) */
+  std::vector<int> &ns = ctx->h_ns;
+  ns.resize(nrays);
+  HIPCHK(ctx, hipMemcpy(ns.data(), ctx->d_nstep_ray, nrays * sizeof(int), hipMemcpyDeviceToHost));
+  const int Ls = ctx->adj_seg_len;
+  std::vector<int> &sr = ctx->h_aseg, &lanes = ctx->h_alane, &r0 = ctx->h_aray;
+  sr.clear();
+  lanes.clear();
+  r0.assign((size_t)nrays + 1, 0);
+  for (int r = 0; r < nrays; r++) {
+    r0[r] = (int)(sr.size() / 3);
+    if (ns[r] <= 0 || ns[r] >= P.maxnstep - 1) continue;
+    for (int q0 = 0, k = 0; q0 < ns[r]; q0 += Ls, k++) {
+      const int sg = (int)(sr.size() / 3);
+      sr.push_back(r);
+      sr.push_back(q0);
+      sr.push_back(q0 + Ls < ns[r] ? q0 + Ls : ns[r]);
+      for (int b = (k == 0 ? NPAR : 0); b <= NPAR; b++) {
+        lanes.push_back(sg);
+        lanes.push_back(b);
+      }
+    }
+  }
+  const int nseg = (int)(sr.size() / 3);
+  r0[nrays] = nseg;
+  const int nlanes = (int)(lanes.size() / 2);
+  const int nck = (Ls + CK - 1) / CK;
+  /* segment table: ray, q0, q1 rows; lanes: segment, column rows; ray
+   * starts */
+  if ((rc = grow(ctx, ctx->d_aint, ctx->cap_aint, (size_t)3 * nseg + 2 * nlanes + nrays + 1 + 3))) return rc;
+  int *d_sray = ctx->d_aint, *d_sq0 = d_sray + nseg, *d_sq1 = d_sq0 + nseg;
+  int *d_lseg = d_sq1 + nseg, *d_lcol = d_lseg + nlanes, *d_r0 = d_lcol + nlanes;
+  {
+    std::vector<int> &t = ctx->h_tmp;
+    t.resize((size_t)3 * nseg + 2 * nlanes + nrays + 1);
+    for (int k = 0; k < nseg; k++) {
+      t[k] = sr[3 * k];
+      t[nseg + k] = sr[3 * k + 1];
+      t[2 * nseg + k] = sr[3 * k + 2];
+    }
+    for (int l = 0; l < nlanes; l++) {
+      t[3 * (size_t)nseg + l] = lanes[2 * l];
+      t[3 * (size_t)nseg + nlanes + l] = lanes[2 * l + 1];
+    }
+    for (int r = 0; r <= nrays; r++) t[3 * (size_t)nseg + 2 * nlanes + r] = r0[r];
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_aint, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  }
+  const size_t nsg = nseg > 0 ? (size_t)nseg : 1;
+  if ((rc = grow(ctx, ctx->d_amap, ctx->cap_amap, nsg * ASEG_MAP)) || (rc = grow(ctx, ctx->d_aN, ctx->cap_aN, nsg))
+      || (rc = grow(ctx, ctx->d_aL, ctx->cap_aL, nsg)) || (rc = grow(ctx, ctx->d_ckpt, ctx->cap_ckpt, nsg * nck))
+      || (rc = grow(ctx, ctx->d_seg, ctx->cap_seg, nsg * CK)) || (rc = grow(ctx, ctx->d_ag3, ctx->cap_ag3, nsg * 3)))
+    return rc;
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_target, cot ? cot : target, npix * 5 * sizeof(double), hipMemcpyHostToDevice,
                              ctx->stream));
   HIPCHK(ctx, hipMemsetAsync(ctx->d_gcell, 0, nc * sizeof(double), ctx->stream));
   AdjW w;
   for (int q = 0; q < 4; q++) w.w[q] = weights ? weights[q] : 1.;
-  k_adjoint<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(G, P, nrays, ctx->d_off, ctx->d_nstep_ray, ctx->d_pkt, ctx->d_aux,
-                                                     cot ? nullptr : ctx->d_target, w, cot ? ctx->d_target : nullptr,
-                                                     ctx->d_dump, ctx->d_ckpt, ctx->d_seg,
-                                                     ctx->d_gcell, ctx->d_out4, ctx->d_imageS, ctx->d_status);
+  if (nlanes > 0) {
+    k_adj_map<<<(nlanes + WAVE - 1) / WAVE, WAVE, 0, ctx->stream>>>(nlanes, d_lseg, d_lcol, d_sray, d_sq0, d_sq1,
+                                                                    ctx->d_off, ctx->d_pkt, ctx->d_amap);
+    HIPCHK(ctx, hipGetLastError());
+  }
+  k_adj_compose<<<nrays / WAVE, WAVE, 0, ctx->stream>>>(G, P, nrays, ctx->d_nstep_ray, d_r0, ctx->d_amap,
+                                                         cot ? nullptr : ctx->d_target, w,
+                                                         cot ? ctx->d_target : nullptr, ctx->d_aN, ctx->d_aL,
+                                                         ctx->d_out4, ctx->d_imageS, ctx->d_status);
   HIPCHK(ctx, hipGetLastError());
+  if (nseg > 0) {
+    k_adj_seg<<<(nseg + WAVE - 1) / WAVE, WAVE, 0, ctx->stream>>>(G, P, nseg, d_sray, d_sq0, d_sq1, ctx->d_off,
+                                                                  ctx->d_pkt, ctx->d_aux, ctx->d_dump, ctx->d_aN,
+                                                                  ctx->d_aL, nck, ctx->d_ckpt, ctx->d_seg,
+                                                                  ctx->d_gcell, ctx->d_ag3);
+    HIPCHK(ctx, hipGetLastError());
+  }
   HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-  std::vector<double> o4((size_t)nrays * 4);
+  std::vector<double> lossv((size_t)nrays), g3((size_t)nseg * 3);
   int status[NSTATUS];
-  HIPCHK(ctx, hipMemcpyAsync(o4.data(), ctx->d_out4, o4.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(lossv.data(), ctx->d_out4, lossv.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  if (nseg > 0)
+    HIPCHK(ctx, hipMemcpyAsync(g3.data(), ctx->d_ag3, g3.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(status, ctx->d_status, sizeof(status), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[3]);
@@ -3357,8 +3477,9 @@ static int run_adjoint(ipgpu_ctx *ctx, const ipgpu_image *img, const double *tar
     return IPGPU_ERR_MAXNSTEP;
   }
   double acc[4] = {0., 0., 0., 0.};
-  for (int r = 0; r < nrays; r++)
-    for (int q = 0; q < 4; q++) acc[q] += o4[(size_t)r * 4 + q];
+  for (int r = 0; r < nrays; r++) acc[0] += lossv[r];
+  for (int k = 0; k < nseg; k++)
+    for (int q = 0; q < 3; q++) acc[q + 1] += g3[(size_t)k * 3 + q];
   out4[0] = acc[0];
   out4[1] = acc[1] / ctx->units.M_unit;
   out4[2] = acc[2];

@@ -193,7 +193,7 @@ int emu_render_scaled(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_ima
  * dL/dln(M_unit), dL/dR_high, dL/dR_low; gcell (ghosted cells, zeroed by
  * the caller) receives dL/d(n_e scale) per cell; Simg (npix x 4). */
 int emu_adjoint(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, const double *dump,
-                const double *target, const double *w, double *out, double *gcell, double *Simg)
+                const double *target, const double *w, double *out, double *gcell, double *Simg, int seg_len)
 {
   const int npix = P->nx * P->ny;
   int status = 0;
@@ -218,11 +218,37 @@ int emu_adjoint(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P,
       for (int q = 0; q < ns; q++)
         prep_step<false>(*G, *M, *P, ns - q, rec.data(), nullptr, dump, pkt.data() + (size_t)q * PK, true,
                          unit_scale(), aux.data() + (size_t)q * AX);
-      ckpt.resize(ns / CK + 1);
-      double o[4];
+      double o[4] = {0., 0., 0., 0.};
       const double tg[4] = {target[5 * n], target[5 * n + 1], target[5 * n + 2], target[5 * n + 3]};
-      adjoint_ray(*G, *P, ns, pkt.data(), aux.data(), tg, w, nullptr, dump, ckpt.data(), seg.data(), gcell, o,
-                  Simg + 4 * (size_t)n);
+      if (seg_len <= 0) {
+        ckpt.resize(ns / CK + 1);
+        adjoint_ray(*G, *P, ns, pkt.data(), aux.data(), tg, w, nullptr, dump, ckpt.data(), seg.data(), gcell, o,
+                    Simg + 4 * (size_t)n);
+      } else {
+        /* the GPU's segmented reverse pass: maps of segments of seg_len
+         * packets (the first one's source column only), composition, then
+         * each segment's reverse pass */
+        const int nseg = (ns + seg_len - 1) / seg_len;
+        std::vector<double> map((size_t)(nseg > 0 ? nseg : 1) * ASEG_MAP, 0.);
+        for (int k = 0; k < nseg; k++) {
+          const int q0 = k * seg_len, q1 = q0 + seg_len < ns ? q0 + seg_len : ns;
+          for (int b = (k == 0 ? NPAR : 0); b <= NPAR; b++) {
+            Herm N;
+            segment_column<1>(pkt.data(), q0, q1, b, N);
+            for (int c = 0; c < NPAR; c++) map[(size_t)k * ASEG_MAP + (size_t)b * NPAR + c] = herm_par(N, c);
+          }
+        }
+        std::vector<Herm> Ns(nseg > 0 ? nseg : 1), Ls(nseg > 0 ? nseg : 1);
+        o[0] = adjoint_compose(*G, *P, nseg, map.data(), tg, w, nullptr, Ns.data(), Ls.data(), Simg + 4 * (size_t)n);
+        ckpt.resize(seg_len / CK + 1);
+        for (int k = 0; k < nseg; k++) {
+          const int q0 = k * seg_len, q1 = q0 + seg_len < ns ? q0 + seg_len : ns;
+          double g3[3] = {0., 0., 0.};
+          adjoint_range(*G, *P, pkt.data(), aux.data(), q0, q1, Ns[k], Ls[k], dump, ckpt.data(), seg.data(), gcell,
+                        g3);
+          for (int q = 0; q < 3; q++) o[q + 1] += g3[q];
+        }
+      }
       for (int q = 0; q < 4; q++) loc[q] += o[q];
     }
 #pragma omp critical

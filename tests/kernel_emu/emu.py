@@ -160,14 +160,15 @@ def trace_pixel(geom, model, image, dump, i, j, prims_ok=True):
     return ns, out[:ns + 1].copy()
 
 
-def adjoint(geom, model, image, dump, target, w=(1., 1., 1., 1.)):
+def adjoint(geom, model, image, dump, target, w=(1., 1., 1., 1.), seg_len=0):
     """(loss, dL/dlnM, dL/dR_high, dL/dR_low), per-cell dL/d(ne scale) over
-    the ghosted cells, and the image's I Q U V."""
+    the ghosted cells, and the image's I Q U V; seg_len > 0: the reverse
+    pass in segments of that many steps, as on the GPU."""
     n = image.nx * image.ny
     nc = (geom.N1 + 2) * (geom.N2 + 2) * (geom.N3 + 2)
     out, gcell, S = np.zeros(4), np.zeros(nc), np.zeros((n, 4))
     rc = lib().emu_adjoint(C.byref(geom), C.byref(model), C.byref(image), G.P(f64(dump)),
-                           G.P(f64(target)), G.P(f64(w)), G.P(out), G.P(gcell), G.P(S))
+                           G.P(f64(target)), G.P(f64(w)), G.P(out), G.P(gcell), G.P(S), int(seg_len))
     assert rc == 0
     return out, gcell, S
 

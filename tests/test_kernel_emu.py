@@ -342,6 +342,14 @@ def test_emu_adjoint_vs_finite_differences():
     fd_Rh = (loss(dump_at(M, Rh * (1 + hM), Rl)) - loss(dump_at(M, Rh * (1 - hM), Rl))) / (2 * hM * Rh)
     fd_Rl = (loss(dump_at(M, Rh, Rl * (1 + hM))) - loss(dump_at(M, Rh, Rl * (1 - hM)))) / (2 * hM * Rl)
     print("lnM", out[1], fd_M, "Rh", out[2], fd_Rh, "Rl", out[3], fd_Rl)
+    # the reverse pass in segments (the GPU's path) gives the same gradient
+    # to roundoff, whatever the segment length
+    for L in (512, 64, 7):
+        o2, g2, S2 = emu.adjoint(g, m, im, d0, target.ravel(), w, seg_len=L)
+        assert np.allclose(S2, S, rtol=1e-10, atol=1e-12 * np.abs(S).max()), L
+        for q in range(4):
+            assert abs(o2[q] / out[q] - 1) < 1e-8, (L, q, o2[q], out[q])
+        assert np.allclose(g2, gcell, rtol=1e-7, atol=1e-9 * np.abs(gcell).max()), L
     assert abs(out[1] / fd_M - 1) < 1e-4
     assert abs(out[2] / fd_Rh - 1) < 1e-4
     assert abs(out[3] / fd_Rl - 1) < 1e-4
