@@ -929,7 +929,7 @@ IPG_HD int chain_ray_segmented(const ipgpu_geom &G, const ipgpu_image &P, int ns
 template <bool F32 = false>
 IPG_HD_INLINE void fused_step(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int n,
                               const double *rec, const int *nt, const double *dump, const Scale &sc,
-                              bool pol, Chain &st)
+                              bool pol, Chain &st, double (*Mi)[4])
 {
   Rec ti, tf;
   load_rec(rec, n, ti);
@@ -951,9 +951,15 @@ IPG_HD_INLINE void fused_step(const ipgpu_geom &G, const ipgpu_model &Mo, const 
       transport_matrix(mh, Kh, Mh);
     }
     Herm Nh;
-    transport(st.N, st.N, st.Mi, (0.5 * dl_cgs) / conv, Nh);
+    transport(st.N, st.N, Mi, (0.5 * dl_cgs) / conv, Nh);
     transport(st.N, Nh, Mh, dl_cgs / conv, st.N);
   }
+  /* FP64: the next step's transport matrix at this step's end point, formed
+   * here so that the metric's derivatives are not live through the rest of
+   * the step (transport only reads it while N != 0, so forming it for every
+   * step gives the same values); the mixed variant has the registers and
+   * forms it at the end, while N != 0 only */
+  if (!F32 && pol) transport_matrix(m, tf.K, Mi);
   if (radiating_region(G, Mo, m.r, m.th)) {
     st.nrad++;
     bool zero_emission = false;
@@ -970,15 +976,15 @@ IPG_HD_INLINE void fused_step(const ipgpu_geom &G, const ipgpu_model &Mo, const 
     F.B *= sc.b;
     Coef c;
     double jf, kf;
-    if (F32) coefficients_f32(F, tf.K, P.emission_type, pol, jf, kf, c);
-    else coefficients(F, tf.K, P.emission_type, pol, jf, kf, c);
-    double ji = zero_emission ? 0. : st.js, ki = st.ks;
-    st.js = jf;
-    st.ks = kf;
-    st.I = approximate_solve(st.I, ji, ki, zero_emission ? 0. : jf, kf, dl_cgs, st.tau);
-    if (pol) {
-      if (zero_emission) c.jI = c.jQ = c.jU = c.jV = 0.;
-      /* evolve_N, src/ipolarray.c:332-575 */
+    int flag = 0;
+    double S0[4], E1[4], E2[4];
+    if constexpr (F32) {
+      coefficients_f32(F, tf.K, P.emission_type, pol, jf, kf, c);
+    } else if (pol) {
+      /* evolve_N's plasma frame (src/ipolarray.c:370-388) before the fits:
+       * the four-vectors, the metric and the lowered tetrad rows are then
+       * dead while the fits and the Stokes map are formed */
+      const Geo q = F.ne > 0. ? coef_geometry(F, tf.K) : Geo{0., 1., 0., false};
       double Bcon[4] = {F.Bcon[0], F.Bcon[1], F.Bcon[2], F.Bcon[3]};
       double bsq = F.Bcon[0] * F.Bcov[0] + F.Bcon[1] * F.Bcov[1] + F.Bcon[2] * F.Bcov[2]
           + F.Bcon[3] * F.Bcov[3];
@@ -989,16 +995,54 @@ IPG_HD_INLINE void fused_step(const ipgpu_geom &G, const ipgpu_model &Mo, const 
         Bcon[3] = 1.;
       }
       double E[4][4];
-      int flag = plasma_tetrad(F.Ucon, tf.K, Bcon, m.g, m.sqrtmg, E);
+      flag = plasma_tetrad(F.Ucon, tf.K, Bcon, m.g, m.sqrtmg, E);
       if (isnan(F.Ucov[0])) flag |= 8;
       double e1[4], e2[4];
       lower(m.g, E[1], e1);
       lower(m.g, E[2], e2);
+      herm_to_stokes(st.N, e1, e2, S0);
+      for (int k = 0; k < 4; k++) {
+        E1[k] = E[1][k];
+        E2[k] = E[2][k];
+      }
+      if (F.ne > 0.) coefficients_core(q, F.ne, F.thetae, F.B, P.emission_type, pol, jf, kf, c);
+      else coefficients_core(Geo{0., 1., 0., false}, 0., 1., 0., P.emission_type, pol, jf, kf, c);
+    } else {
+      coefficients(F, tf.K, P.emission_type, pol, jf, kf, c);
+    }
+    double ji = zero_emission ? 0. : st.js, ki = st.ks;
+    st.js = jf;
+    st.ks = kf;
+    st.I = approximate_solve(st.I, ji, ki, zero_emission ? 0. : jf, kf, dl_cgs, st.tau);
+    if (pol) {
+      if (zero_emission) c.jI = c.jQ = c.jU = c.jV = 0.;
+      if constexpr (F32) {
+        /* evolve_N, src/ipolarray.c:332-575 */
+        double Bcon[4] = {F.Bcon[0], F.Bcon[1], F.Bcon[2], F.Bcon[3]};
+        double bsq = F.Bcon[0] * F.Bcov[0] + F.Bcon[1] * F.Bcov[1] + F.Bcon[2] * F.Bcov[2]
+            + F.Bcon[3] * F.Bcov[3];
+        if (bsq <= 0.) {
+          Bcon[0] = 0.;
+          Bcon[1] = 1.;
+          Bcon[2] = 1.;
+          Bcon[3] = 1.;
+        }
+        double E[4][4];
+        flag = plasma_tetrad(F.Ucon, tf.K, Bcon, m.g, m.sqrtmg, E);
+        if (isnan(F.Ucov[0])) flag |= 8;
+        double e1[4], e2[4];
+        lower(m.g, E[1], e1);
+        lower(m.g, E[2], e2);
+        herm_to_stokes(st.N, e1, e2, S0);
+        for (int k = 0; k < 4; k++) {
+          E1[k] = E[1][k];
+          E2[k] = E[2][k];
+        }
+      }
       double T[4][4], cv[4];
       if (F32) stokes_affine_f32(c, dl_cgs, T, cv);
       else stokes_affine(c, dl_cgs, T, cv);
-      double S0[4], S[4];
-      herm_to_stokes(st.N, e1, e2, S0);
+      double S[4];
       for (int i = 0; i < 4; i++) S[i] = cv[i] + T[i][0] * S0[0] + T[i][1] * S0[1] + T[i][2] * S0[2] + T[i][3] * S0[3];
       st.tauF += dl_cgs * fabs(c.rV);
       if (P.stokes_floors) {
@@ -1014,29 +1058,33 @@ IPG_HD_INLINE void fused_step(const ipgpu_geom &G, const ipgpu_model &Mo, const 
         }
       }
       double dS[4] = {S[0] - S0[0], S[1] - S0[1], S[2] - S0[2], S[3] - S0[3]};
-      herm_add_stokes(st.N, E[1], E[2], dS);
+      herm_add_stokes(st.N, E1, E2, dS);
       if (st.tauF > 1.e100 || st.tauF < -1.e100 || isnan(st.tauF)) flag |= 2;
       if (isnan(st.N.d[0]) || isnan(S[0])) flag |= 4;
       st.flag |= flag;
       st.Nnz = herm_nonzero(st.N);
     }
   }
-  if (pol && st.Nnz) transport_matrix(m, tf.K, st.Mi);
+  if (F32 && pol && st.Nnz) transport_matrix(m, tf.K, Mi);
 }
 
 /* Whole transfer of one ray, step by step in registers (no packet memory):
- * for launches with enough rays to fill the chip. */
+ * for launches with enough rays to fill the chip.  mi (may be null: the
+ * state's own) holds the 16 doubles of the transport matrix carried from
+ * one step to the next; a kernel passes a slot of LDS there to free the
+ * registers. */
 template <bool F32 = false>
-IPG_HD int transfer_fused(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int ns,
+IPG_HD_INLINE int transfer_fused(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int ns,
                           const double *rec, const int *nt, const double *dump, const Scale &sc,
-                          double out7[7], int &nrad)
+                          double out7[7], int &nrad, double *mi = nullptr)
 {
   const bool pol = !P.only_unpolarized;
   double js, ks;
   far_end_coefficients<F32>(G, Mo, P, ns, rec, dump, js, ks, sc);
   Chain st;
   chain_init(st, js, ks);
-  for (int q = 0; q < ns; q++) fused_step<F32>(G, Mo, P, ns - q, rec, nt, dump, sc, pol, st);
+  double(*Mi)[4] = mi ? reinterpret_cast<double(*)[4]>(mi) : st.Mi;
+  for (int q = 0; q < ns; q++) fused_step<F32>(G, Mo, P, ns - q, rec, nt, dump, sc, pol, st, Mi);
   nrad = st.nrad;
   return chain_finish(G, P, st, out7);
 }

@@ -726,9 +726,12 @@ __global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model M
   const long long ro = off[r];
   const double *rec = ro >= 0 ? traj + (size_t)ro * REC : traj2 + (size_t)(-1 - ro) * REC;
   const int *nt = (nturns_buf && P.target_nturns >= 0) ? nturns_buf + ro : nullptr;
+  /* FP64: the transport matrix carried between steps lives in LDS (one
+   * padded slot per lane), which keeps the step within the register file */
+  __shared__ double mi_lds[F32 ? 1 : WAVE * 17];
   double o[7];
   int nrad;
-  int flag = transfer_fused<F32>(G, Mo, P, ns, rec, nt, dump, J.sc, o, nrad);
+  int flag = transfer_fused<F32>(G, Mo, P, ns, rec, nt, dump, J.sc, o, nrad, F32 ? nullptr : mi_lds + 17 * threadIdx.x);
   const double f3 = P.freqcgs * P.freqcgs * P.freqcgs;
   image[p] = o[0] * f3;
   taus[p] = o[5];
@@ -1383,8 +1386,11 @@ __global__ __launch_bounds__(WAVE) void k_na_xfer(ipgpu_geom G, ipgpu_model Mo, 
   if (nstep_pix) nstep_pix[p] = ns;
   double o[7] = {0., 0., 0., 0., 0., 0., 0.};
   int flag = 0, nrad = 0;
+  __shared__ double mi_lds[F32 ? 1 : WAVE * 17];
   if (ns >= P.maxnstep - 1) atomicMin(&status[1], pixel_code(pi, pj));
-  else flag = transfer_fused<F32>(G, Mo, P, ns, arena + (size_t)(S - c * Ap) * REC, nullptr, dump, unit_scale(), o, nrad);
+  else
+    flag = transfer_fused<F32>(G, Mo, P, ns, arena + (size_t)(S - c * Ap) * REC, nullptr, dump, unit_scale(), o, nrad,
+                               F32 ? nullptr : mi_lds + 17 * threadIdx.x);
   const double f3 = P.freqcgs * P.freqcgs * P.freqcgs;
   if (image) image[p] = o[0] * f3;
   if (taus) taus[p] = o[5];
