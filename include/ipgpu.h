@@ -223,6 +223,29 @@ int ipgpu_bad_job(const ipgpu_ctx *ctx, int *job, int *i, int *j);
  * requires. */
 int ipgpu_set_batch_memory(ipgpu_ctx *ctx, size_t traj_bytes);
 
+/* Render the last batch again with new m_unit_scale values, reusing its
+ * geodesic records: the transfer alone runs (a captured HIP graph of the
+ * job-table upload, status clears and transfer launch, replayed on later
+ * calls).  jobs must hold the same images, in the same order, as the last
+ * ipgpu_render_batch, whose records must have stayed in place (one pass; it
+ * returns IPGPU_ERR_STATE after a batch in passes, or after ipgpu_render,
+ * ipgpu_render_adaptive, ipgpu_adjoint, ipgpu_backward, ipgpu_trace_pixel
+ * or ipgpu_set_geometry, which reuse or invalidate the records).  A new
+ * dump of the same grid (ipgpu_upload_prims, ipgpu_scale_density) or a new
+ * model is allowed: the geodesics depend on the geometry and cameras only.
+ * The M_unit search of the reference's users, one ipole run per trial
+ * M_unit (src/main.c:536-615 per run), becomes one trace and one transfer
+ * per trial. */
+int ipgpu_rerender_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs);
+
+/* Batch launch options (a negative value keeps the current one): the
+ * record slot per ray of the first trace (default 2048), the transfer as a
+ * persistent work-queue launch (1, default) or one wave per 64 items (0),
+ * the persistent waves (0: four per compute unit), and whether
+ * ipgpu_rerender_batch replays a captured graph (1, default) or launches
+ * eagerly (0).  No counterpart in the reference. */
+int ipgpu_set_batch_options(ipgpu_ctx *ctx, int slot, int queue, int queue_waves, int rerender_graph);
+
 /* Device memory a single-image render (ipgpu_render) may use for its ray
  * records and step packets (default 64 GiB).  An image whose rays need more
  * is rendered in chunks of rays (whole 8x8 pixel tiles) that fit; the image

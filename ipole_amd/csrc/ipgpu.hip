@@ -682,27 +682,20 @@ __global__ __launch_bounds__(WAVE) void k_trace_resume_list(ipgpu_geom G, const 
   if (!odd && nstep != ns) atomicAdd(&status[0], 1);
 }
 
-/* One lane per (job, ray) work item: the fused transfer of the ray for the
- * job, then the job's pixel in the reference layout (save_pixel,
- * src/main.c:885-910).  bstatus[0]: min of (job << 32 | pixel) + 1 over
- * rays past maxnstep; bstatus[1]: the same over NaN pixels. */
+/* The transfer of work item w (job, ray) and the job's pixel in the
+ * reference layout; mi: the lane's LDS slot for the carried transport
+ * matrix (FP64) or null. */
 template <bool F32>
-__global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model Mo, const BJob *__restrict__ jobs,
-                                                     const BGroup *__restrict__ groups,
-                                                     const int2 *__restrict__ items, int nitems,
-                                                     const long long *__restrict__ off,
-                                                     const double *__restrict__ traj,
-                                                     const double *__restrict__ traj2,
-                                                     const int *__restrict__ nstep_in,
-                                                     const int *__restrict__ nturns_buf,
-                                                     const double *__restrict__ dump, double *__restrict__ image,
-                                                     double *__restrict__ imageS, double *__restrict__ taus,
-                                                     int *__restrict__ nstep_pix, int *__restrict__ flag_pix,
-                                                     unsigned long long *__restrict__ bstatus,
-                                                     unsigned long long *__restrict__ counts)
+__device__ __forceinline__ void xfer_item(const ipgpu_geom &G, const ipgpu_model &Mo, const BJob *__restrict__ jobs,
+                                          const BGroup *__restrict__ groups, const int2 *__restrict__ items, int w,
+                                          const long long *__restrict__ off, const double *__restrict__ traj,
+                                          const double *__restrict__ traj2, const int *__restrict__ nstep_in,
+                                          const int *__restrict__ nturns_buf, const double *__restrict__ dump,
+                                          double *__restrict__ image, double *__restrict__ imageS,
+                                          double *__restrict__ taus, int *__restrict__ nstep_pix,
+                                          int *__restrict__ flag_pix, unsigned long long *__restrict__ bstatus,
+                                          unsigned long long *__restrict__ counts, double *mi)
 {
-  int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= nitems) return;
   const int2 it = items[w];
   const int jb = it.x, r = it.y;
   const BJob J = jobs[jb];
@@ -726,12 +719,9 @@ __global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model M
   const long long ro = off[r];
   const double *rec = ro >= 0 ? traj + (size_t)ro * REC : traj2 + (size_t)(-1 - ro) * REC;
   const int *nt = (nturns_buf && P.target_nturns >= 0) ? nturns_buf + ro : nullptr;
-  /* FP64: the transport matrix carried between steps lives in LDS (one
-   * padded slot per lane), which keeps the step within the register file */
-  __shared__ double mi_lds[F32 ? 1 : WAVE * 17];
   double o[7];
   int nrad;
-  int flag = transfer_fused<F32>(G, Mo, P, ns, rec, nt, dump, J.sc, o, nrad, F32 ? nullptr : mi_lds + 17 * threadIdx.x);
+  int flag = transfer_fused<F32>(G, Mo, P, ns, rec, nt, dump, J.sc, o, nrad, mi);
   const double f3 = P.freqcgs * P.freqcgs * P.freqcgs;
   image[p] = o[0] * f3;
   taus[p] = o[5];
@@ -746,6 +736,85 @@ __global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model M
     if (isnan(o[1])) atomicMin(&bstatus[1], ((unsigned long long)jb << 32 | (unsigned long long)pix) + 1ull);
   }
   atomicAdd(&counts[1], (unsigned long long)nrad);
+}
+
+/* One lane per (job, ray) work item: the fused transfer of the ray for the
+ * job, then the job's pixel in the reference layout (save_pixel,
+ * src/main.c:885-910).  bstatus[0]: min of (job << 32 | pixel) + 1 over
+ * rays past maxnstep; bstatus[1]: the same over NaN pixels. */
+template <bool F32>
+__global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model Mo, const BJob *__restrict__ jobs,
+                                                     const BGroup *__restrict__ groups,
+                                                     const int2 *__restrict__ items, int nitems,
+                                                     const long long *__restrict__ off,
+                                                     const double *__restrict__ traj,
+                                                     const double *__restrict__ traj2,
+                                                     const int *__restrict__ nstep_in,
+                                                     const int *__restrict__ nturns_buf,
+                                                     const double *__restrict__ dump, double *__restrict__ image,
+                                                     double *__restrict__ imageS, double *__restrict__ taus,
+                                                     int *__restrict__ nstep_pix, int *__restrict__ flag_pix,
+                                                     unsigned long long *__restrict__ bstatus,
+                                                     unsigned long long *__restrict__ counts)
+{
+  /* FP64: the transport matrix carried between steps lives in LDS (one
+   * padded slot per lane), which keeps the step within the register file */
+  __shared__ double mi_lds[F32 ? 1 : WAVE * 17];
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nitems) return;
+  xfer_item<F32>(G, Mo, jobs, groups, items, w, off, traj, traj2, nstep_in, nturns_buf, dump, image, imageS, taus,
+                 nstep_pix, flag_pix, bstatus, counts, F32 ? nullptr : mi_lds + 17 * threadIdx.x);
+}
+
+/* The same work as a persistent launch with a work queue: a fixed number of
+ * waves, each taking the next chunk of 64 items (an atomic ticket on
+ * queue[0]) until the list is done.  Items are ordered longest ray first,
+ * so the waves that finish early take the remaining short items. */
+template <bool F32>
+__global__ __launch_bounds__(WAVE) void k_xfer_queue(ipgpu_geom G, ipgpu_model Mo, const BJob *__restrict__ jobs,
+                                                     const BGroup *__restrict__ groups,
+                                                     const int2 *__restrict__ items, int nitems,
+                                                     unsigned int *__restrict__ queue,
+                                                     const long long *__restrict__ off,
+                                                     const double *__restrict__ traj,
+                                                     const double *__restrict__ traj2,
+                                                     const int *__restrict__ nstep_in,
+                                                     const int *__restrict__ nturns_buf,
+                                                     const double *__restrict__ dump, double *__restrict__ image,
+                                                     double *__restrict__ imageS, double *__restrict__ taus,
+                                                     int *__restrict__ nstep_pix, int *__restrict__ flag_pix,
+                                                     unsigned long long *__restrict__ bstatus,
+                                                     unsigned long long *__restrict__ counts)
+{
+  __shared__ double mi_lds[F32 ? 1 : WAVE * 17];
+  __shared__ unsigned int ticket;
+  const unsigned int nchunks = (unsigned int)((nitems + WAVE - 1) / WAVE);
+  while (true) {
+    if (threadIdx.x == 0) ticket = atomicAdd(&queue[0], 1u);
+    __syncthreads();
+    const unsigned int c = ticket;
+    __syncthreads();
+    if (c >= nchunks) break;   /* every lane of the wave leaves together */
+    const int w = (int)(c * WAVE + threadIdx.x);
+    if (w < nitems)
+      xfer_item<F32>(G, Mo, jobs, groups, items, w, off, traj, traj2, nstep_in, nturns_buf, dump, image, imageS,
+                     taus, nstep_pix, flag_pix, bstatus, counts, F32 ? nullptr : mi_lds + 17 * threadIdx.x);
+  }
+}
+
+/* queue[0] = 0; with bstatus, both status words to "none"; with counts,
+ * the radiating step count to 0 (a kernel rather than memsets, so that the
+ * clears can sit in a captured graph) */
+__global__ void k_queue_reset(unsigned int *queue, unsigned long long *bstatus, unsigned long long *counts)
+{
+  if (threadIdx.x == 0) {
+    queue[0] = 0u;
+    if (bstatus) {
+      bstatus[0] = ~0ull;
+      bstatus[1] = ~0ull;
+    }
+    if (counts) counts[1] = 0ull;
+  }
 }
 
 /* ------------------------------------------------------------- adjoint */
@@ -1545,6 +1614,36 @@ struct ipgpu_ctx {
   size_t cap_bimage = 0, cap_bimageS = 0, cap_btaus = 0, cap_bnstep = 0, cap_bflag = 0;
   unsigned long long *d_bstatus = nullptr;
   size_t batch_traj_bytes = (size_t)160 << 30; /* geodesic records per pass of a batch */
+  /* the batch transfer: persistent work-queue launch (1) or one wave per 64
+   * items (0); persistent waves (0: 4 per compute unit) */
+  int batch_queue = 1;
+  int batch_queue_waves = 0;
+  int ncu = 256;
+  unsigned int *d_queue = nullptr;
+  /* the last in-place batch, for ipgpu_rerender_batch: its images, trace
+   * scales, output offsets, items and the graph of its transfer (job table
+   * upload from pinned memory, queue reset, transfer) */
+  struct LastBatch {
+    bool valid = false;
+    int njobs = 0, nitems = 0, f32 = 0;
+    std::vector<ipgpu_image> img;
+    std::vector<BJob> bj;
+    long long pix_total = 0;
+    const double *traj2 = nullptr;
+  } blast;
+  /* what the captured transfer graph baked in: kernel arguments by value
+   * and the dump it reads */
+  struct GraphKey {
+    ipgpu_geom geom;
+    ipgpu_model model;
+    const double *dump;
+    int queue, waves;
+  } bkey;
+  BJob *h_bjobs = nullptr;          /* pinned job table the graph uploads */
+  size_t h_bjobs_cap = 0;
+  hipGraph_t bgraph = nullptr;
+  hipGraphExec_t bgraph_exec = nullptr;
+  int rerender_graph = 1;           /* replay the transfer from a captured graph */
   /* adjoint */
   ipgpu_units units{};
   bool has_units = false;
@@ -1646,6 +1745,11 @@ int ipgpu_create(ipgpu_ctx **out, int device)
     delete ctx;
     return IPGPU_ERR_HIP;
   }
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+    ctx->ncu = ncu;
+  if (const char *e = getenv("IPGPU_BATCH_QUEUE")) ctx->batch_queue = atoi(e);
+  if (const char *e = getenv("IPGPU_BATCH_QUEUE_WAVES")) ctx->batch_queue_waves = atoi(e);
   if (noalloc_setup(device, &ctx->d_scratch) != IPGPU_OK) {
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1687,6 +1791,9 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
     if (st) hipStreamSynchronize(st);
   if (ctx->h_pin) hipHostFree(ctx->h_pin);
   if (ctx->h_items) hipHostFree(ctx->h_items);
+  if (ctx->h_bjobs) hipHostFree(ctx->h_bjobs);
+  if (ctx->bgraph_exec) hipGraphExecDestroy(ctx->bgraph_exec);
+  if (ctx->bgraph) hipGraphDestroy(ctx->bgraph);
   for (auto *S : {&ctx->segset[0], &ctx->segset[1], &ctx->segset[2], &ctx->segset[3], &ctx->segset[4],
                   &ctx->segpre}) {
     void *sp[] = {S->d_off, S->d_q0, S->d_q1, S->d_flag, S->d_map};
@@ -1701,7 +1808,7 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
                   ctx->d_out4, ctx->d_target, ctx->d_scale, ctx->d_trrec, ctx->d_trout, ctx->d_trnt,
                   ctx->d_rlist, ctx->d_rcount, ctx->d_iavg, ctx->d_interp, ctx->d_ppkt, ctx->d_plist, ctx->d_traj2, ctx->d_off2,
                   ctx->d_pidx, ctx->d_trajO, ctx->d_ovl, ctx->d_ovoff, ctx->d_rbase, ctx->d_pixlist,
-                  ctx->d_aint, ctx->d_amap, ctx->d_ag3, ctx->d_aN, ctx->d_aL};
+                  ctx->d_aint, ctx->d_amap, ctx->d_ag3, ctx->d_aN, ctx->d_aL, ctx->d_queue};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int q = 0; q < NEV; q++)
@@ -1716,6 +1823,8 @@ const char *ipgpu_last_error(const ipgpu_ctx *ctx) { return ctx ? ctx->err : "nu
 
 void *ipgpu_stream(ipgpu_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
+static void batch_evict(ipgpu_ctx *ctx);
+
 int ipgpu_set_geometry(ipgpu_ctx *ctx, const ipgpu_geom *g)
 {
   if (!ctx || !g) return IPGPU_ERR_ARG;
@@ -1723,6 +1832,7 @@ int ipgpu_set_geometry(ipgpu_ctx *ctx, const ipgpu_geom *g)
       && g->metric != IPGPU_METRIC_EMINKOWSKI)
     return set_err(ctx, IPGPU_ERR_ARG, "unsupported metric (MKS, FMKS, eMinkowski)");
   if (g->N1 <= 0 || g->N2 <= 0 || g->N3 <= 0) return set_err(ctx, IPGPU_ERR_ARG, "bad grid size");
+  batch_evict(ctx);
   ctx->geom = *g;
   ctx->has_geom = true;
   return IPGPU_OK;
@@ -2461,6 +2571,7 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
   if (P.emission_type != 4 && P.emission_type != 1)
     return set_err(ctx, IPGPU_ERR_ARG, "emission_type must be 4 (Dexter thermal) or 1 (Pandya thermal)");
   HIPCHK(ctx, hipSetDevice(ctx->device));
+  batch_evict(ctx);
   if (P.nx > 32767 || P.ny > 32767) return set_err(ctx, IPGPU_ERR_ARG, "image side above 32767 pixels");
   int tiles = ((P.nx + TILE - 1) / TILE) * ((P.ny + TILE - 1) / TILE);
   int nrays = tiles * WAVE;
@@ -2621,6 +2732,7 @@ int ipgpu_render_adaptive(ipgpu_ctx *ctx, const ipgpu_image *img, const ipgpu_re
                           double *imageS, double *taus, int *nstep, int *flag, int *interp)
 {
   if (!ctx || !img || !rf || !image) return IPGPU_ERR_ARG;
+  batch_evict(ctx);
   const ipgpu_image &P = *img;
   if (rf->nx_min <= 0 || rf->nx_min == P.nx) {
     ctx->ref_levels = 1;
@@ -2947,6 +3059,14 @@ static bool same_trace(const ipgpu_image &a, const ipgpu_image &b)
       && a.subring_dtheta == b.subring_dtheta && a.freq > 0. && b.freq > 0.;
 }
 
+/* Every field of the image the same (the struct's padding aside). */
+static bool same_image(const ipgpu_image &a, const ipgpu_image &b)
+{
+  return same_trace(a, b) && a.freq == b.freq && a.freqcgs == b.freqcgs
+      && a.only_unpolarized == b.only_unpolarized && a.qu_conv == b.qu_conv && a.stokes_floors == b.stokes_floors
+      && a.counterjet == b.counterjet && a.emission_type == b.emission_type && a.transfer_fp32 == b.transfer_fp32;
+}
+
 static int group_of_host(const std::vector<BGroup> &groups, long long r)
 {
   int lo = 0, hi = (int)groups.size();
@@ -2967,6 +3087,53 @@ static void host_ray_pixel(int ray, int nx, int ny, int &i, int &j)
   j = (t % tiles_y) * TILE + l % TILE;
 }
 
+/* The transfer of a batch's nitems work items (d_items, d_jobs, records at
+ * d_traj / traj2), on the context's stream: the persistent work-queue
+ * launch or one wave per 64 items.  clear_status: first reset the status
+ * words and the radiating step count on the device. */
+static int batch_transfer(ipgpu_ctx *ctx, bool f32, int nitems, const double *traj2, const int *nturns,
+                          bool clear_status)
+{
+  hipStream_t st = ctx->stream;
+  if (clear_status || (ctx->batch_queue && nitems > 0)) {
+    k_queue_reset<<<1, WAVE, 0, st>>>(ctx->d_queue, clear_status ? ctx->d_bstatus : nullptr,
+                                      clear_status ? ctx->d_counts : nullptr);
+    HIPCHK(ctx, hipGetLastError());
+  }
+  if (nitems <= 0) return 0;
+  if (ctx->batch_queue) {
+    const int nw = ctx->batch_queue_waves > 0 ? ctx->batch_queue_waves : 4 * ctx->ncu;
+    auto kq = f32 ? k_xfer_queue<true> : k_xfer_queue<false>;
+    kq<<<nw, WAVE, 0, st>>>(ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items, nitems, ctx->d_queue,
+                            ctx->d_off, ctx->d_traj, traj2, ctx->d_nstep_ray, nturns, ctx->d_dump, ctx->d_bimage,
+                            ctx->d_bimageS, ctx->d_btaus, ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts);
+  } else {
+    auto kx = f32 ? k_xfer_batch<true> : k_xfer_batch<false>;
+    kx<<<(nitems + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items,
+                                                    nitems, ctx->d_off, ctx->d_traj, traj2, ctx->d_nstep_ray, nturns,
+                                                    ctx->d_dump, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
+                                                    ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts);
+  }
+  HIPCHK(ctx, hipGetLastError());
+  return 0;
+}
+
+static void drop_batch_graph(ipgpu_ctx *ctx)
+{
+  if (ctx->bgraph_exec) hipGraphExecDestroy(ctx->bgraph_exec);
+  if (ctx->bgraph) hipGraphDestroy(ctx->bgraph);
+  ctx->bgraph_exec = nullptr;
+  ctx->bgraph = nullptr;
+}
+
+/* Every entry point that reuses the ray buffers (records, offsets, step
+ * counts) or changes the geometry ends the resident batch. */
+static void batch_evict(ipgpu_ctx *ctx)
+{
+  ctx->blast.valid = false;
+  drop_batch_graph(ctx);
+}
+
 int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
 {
   if (!ctx || njobs <= 0 || !jobs) return IPGPU_ERR_ARG;
@@ -2984,6 +3151,7 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
   }
   HIPCHK(ctx, hipSetDevice(ctx->device));
   int rc;
+  batch_evict(ctx);
   /* trace groups */
   std::vector<BGroup> groups;
   std::vector<BJob> bj(njobs);
@@ -3030,6 +3198,7 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
   if ((rc = grow(ctx, ctx->d_bnstep, ctx->cap_bnstep, (size_t)pix_total))) return rc;
   if ((rc = grow(ctx, ctx->d_bflag, ctx->cap_bflag, (size_t)pix_total))) return rc;
   if (!ctx->d_bstatus) HIPCHK(ctx, hipMalloc((void **)&ctx->d_bstatus, 2 * sizeof(unsigned long long)));
+  if (!ctx->d_queue) HIPCHK(ctx, hipMalloc((void **)&ctx->d_queue, 64 * sizeof(unsigned int)));
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_groups, groups.data(), groups.size() * sizeof(BGroup), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_jobs, bj.data(), bj.size() * sizeof(BJob), hipMemcpyHostToDevice, ctx->stream));
   static const unsigned long long bst_init[2] = {~0ull, ~0ull};
@@ -3154,15 +3323,15 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, hit, nitems * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-    if (nitems > 0) {
-      auto kx = jobs[0].image.transfer_fp32 ? k_xfer_batch<true> : k_xfer_batch<false>;
-      kx<<<(nitems + WAVE - 1) / WAVE, WAVE, 0, ctx->stream>>>(
-          ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items, nitems, ctx->d_off, ctx->d_traj,
-          ctx->d_traj2, ctx->d_nstep_ray, nullptr, ctx->d_dump, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
-          ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts);
-      HIPCHK(ctx, hipGetLastError());
-    }
+    if ((rc = batch_transfer(ctx, jobs[0].image.transfer_fp32 != 0, nitems, nlong > 0 ? ctx->d_traj2 : nullptr,
+                             nullptr, false)))
+      return rc;
     HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+    /* the records, items and job table stay resident: ipgpu_rerender_batch
+     * can run this transfer again with new M_unit scales */
+    ctx->blast.valid = true;
+    ctx->blast.nitems = nitems;
+    ctx->blast.traj2 = nlong > 0 ? ctx->d_traj2 : nullptr;
     int mism = 0;
     HIPCHK(ctx, hipMemcpyAsync(&mism, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -3212,14 +3381,9 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     if ((rc = grow(ctx, ctx->d_items, ctx->cap_items, (size_t)(nitems > 0 ? nitems : 1)))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, items.data(), nitems * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-    if (nitems > 0) {
-      auto kx = jobs[0].image.transfer_fp32 ? k_xfer_batch<true> : k_xfer_batch<false>;
-      kx<<<(nitems + WAVE - 1) / WAVE, WAVE, 0, ctx->stream>>>(
-          ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items, nitems, ctx->d_off, ctx->d_traj, nullptr,
-          ctx->d_nstep_ray, turns ? ctx->d_nturns : nullptr, ctx->d_dump, ctx->d_bimage, ctx->d_bimageS,
-          ctx->d_btaus, ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts);
-      HIPCHK(ctx, hipGetLastError());
-    }
+    if ((rc = batch_transfer(ctx, jobs[0].image.transfer_fp32 != 0, nitems, nullptr, turns ? ctx->d_nturns : nullptr,
+                             false)))
+      return rc;
     HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
     int mism = 0;
     HIPCHK(ctx, hipMemcpyAsync(&mism, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -3256,6 +3420,14 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
                q == 0 ? "MaxNStep exceeded" : "NaN in image", ctx->bad_job, ctx->bad_i, ctx->bad_j);
       return q == 0 ? IPGPU_ERR_MAXNSTEP : IPGPU_ERR_NAN;
     }
+  if (ctx->blast.valid) {
+    ctx->blast.njobs = njobs;
+    ctx->blast.f32 = jobs[0].image.transfer_fp32 != 0;
+    ctx->blast.img.resize(njobs);
+    for (int j = 0; j < njobs; j++) ctx->blast.img[j] = jobs[j].image;
+    ctx->blast.bj = bj;
+    ctx->blast.pix_total = pix_total;
+  }
   for (int j = 0; j < njobs; j++) {
     const ipgpu_job &J = jobs[j];
     size_t n = (size_t)J.image.nx * J.image.ny, o = (size_t)bj[j].pix_off;
@@ -3267,6 +3439,129 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     if (J.flag_out) HIPCHK(ctx, hipMemcpyAsync(J.flag_out, ctx->d_bflag + o, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return IPGPU_OK;
+}
+
+/* The transfer of the resident batch as one stream sequence: the job table
+ * from pinned memory, the status clears and the transfer launch. */
+static int rerender_enqueue(ipgpu_ctx *ctx)
+{
+  const ipgpu_ctx::LastBatch &L = ctx->blast;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_jobs, ctx->h_bjobs, (size_t)L.njobs * sizeof(BJob), hipMemcpyHostToDevice,
+                             ctx->stream));
+  return batch_transfer(ctx, L.f32 != 0, L.nitems, L.traj2, nullptr, true);
+}
+
+int ipgpu_rerender_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
+{
+  if (!ctx || njobs <= 0 || !jobs) return IPGPU_ERR_ARG;
+  ipgpu_ctx::LastBatch &L = ctx->blast;
+  if (!L.valid)
+    return set_err(ctx, IPGPU_ERR_STATE, "no resident batch (none rendered, it ran in passes, or a render, "
+                                         "adjoint or geometry change has reused its buffers since)");
+  if (njobs != L.njobs) return set_err(ctx, IPGPU_ERR_ARG, "job count differs from the resident batch");
+  for (int j = 0; j < njobs; j++) {
+    if (!same_image(jobs[j].image, L.img[j]))
+      return set_err(ctx, IPGPU_ERR_ARG, "a job's image differs from the resident batch's");
+    if (!(jobs[j].m_unit_scale > 0.)) return set_err(ctx, IPGPU_ERR_ARG, "bad m_unit_scale");
+  }
+  if (!ctx->d_dump && ctx->model.kind == IPGPU_MODEL_IHARM)
+    return set_err(ctx, IPGPU_ERR_STATE, "no fluid dump uploaded");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if ((size_t)njobs > ctx->h_bjobs_cap) {
+    drop_batch_graph(ctx);   /* the graph reads the pinned table it was captured with */
+    if (ctx->h_bjobs) hipHostFree(ctx->h_bjobs);
+    ctx->h_bjobs = nullptr;
+    ctx->h_bjobs_cap = 0;
+    HIPCHK(ctx, hipHostMalloc((void **)&ctx->h_bjobs, (size_t)njobs * sizeof(BJob), hipHostMallocDefault));
+    ctx->h_bjobs_cap = (size_t)njobs;
+  }
+  for (int j = 0; j < njobs; j++) {
+    BJob b = L.bj[j];
+    b.sc.ne = jobs[j].m_unit_scale;
+    b.sc.b = sqrt(jobs[j].m_unit_scale);
+    ctx->h_bjobs[j] = b;
+  }
+  const int nw = ctx->batch_queue_waves > 0 ? ctx->batch_queue_waves : 4 * ctx->ncu;
+  ipgpu_ctx::GraphKey key;
+  memset(&key, 0, sizeof(key));
+  key.geom = ctx->geom;
+  key.model = ctx->model;
+  key.dump = ctx->d_dump;
+  key.queue = ctx->batch_queue;
+  key.waves = nw;
+  if (ctx->bgraph_exec && memcmp(&key, &ctx->bkey, sizeof(key)) != 0) drop_batch_graph(ctx);
+  HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+  int rc;
+  if (ctx->rerender_graph) {
+    if (!ctx->bgraph_exec) {
+      HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      rc = rerender_enqueue(ctx);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+      if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+      }
+      HIPCHK(ctx, e);
+      ctx->bgraph = g;
+      HIPCHK(ctx, hipGraphInstantiate(&ctx->bgraph_exec, g, nullptr, nullptr, 0));
+      ctx->bkey = key;
+    }
+    HIPCHK(ctx, hipGraphLaunch(ctx->bgraph_exec, ctx->stream));
+  } else if ((rc = rerender_enqueue(ctx))) {
+    return rc;
+  }
+  HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+  unsigned long long bst[2];
+  HIPCHK(ctx, hipMemcpyAsync(bst, ctx->d_bstatus, sizeof(bst), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->counts, ctx->d_counts, sizeof(ctx->counts), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->counts[0] = 0;   /* no geodesic is traced */
+  hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[3]);
+  ctx->ms_geo = 0.f;
+  ctx->ms_xfer = ctx->ms_total;
+  ctx->ms_stage[0] = 0.f;
+  ctx->ms_stage[1] = 0.f;
+  ctx->ms_stage[2] = ctx->ms_total;
+  ctx->ms_stage[3] = 0.f;
+  for (int q = 0; q < 2; q++)
+    if (bst[q] != ~0ull) {
+      unsigned long long k = bst[q] - 1;
+      ctx->bad_job = (int)(k >> 32);
+      int pix = (int)(k & 0xffffffffull);
+      const ipgpu_image &P = jobs[ctx->bad_job].image;
+      ctx->bad_i = pix / P.ny;
+      ctx->bad_j = pix % P.ny;
+      snprintf(ctx->err, sizeof(ctx->err), "%s in job %d pixel %d %d",
+               q == 0 ? "MaxNStep exceeded" : "NaN in image", ctx->bad_job, ctx->bad_i, ctx->bad_j);
+      return q == 0 ? IPGPU_ERR_MAXNSTEP : IPGPU_ERR_NAN;
+    }
+  for (int j = 0; j < njobs; j++) {
+    const ipgpu_job &J = jobs[j];
+    size_t n = (size_t)J.image.nx * J.image.ny, o = (size_t)L.bj[j].pix_off;
+    if (J.image_out) HIPCHK(ctx, hipMemcpyAsync(J.image_out, ctx->d_bimage + o, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (J.imageS_out && !J.image.only_unpolarized)
+      HIPCHK(ctx, hipMemcpyAsync(J.imageS_out, ctx->d_bimageS + 5 * o, 5 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (J.taus_out) HIPCHK(ctx, hipMemcpyAsync(J.taus_out, ctx->d_btaus + o, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (J.nstep_out) HIPCHK(ctx, hipMemcpyAsync(J.nstep_out, ctx->d_bnstep + o, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (J.flag_out) HIPCHK(ctx, hipMemcpyAsync(J.flag_out, ctx->d_bflag + o, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return IPGPU_OK;
+}
+
+int ipgpu_set_batch_options(ipgpu_ctx *ctx, int slot, int queue, int queue_waves, int rerender_graph)
+{
+  if (!ctx) return IPGPU_ERR_ARG;
+  if (slot > 0) {
+    if (slot < 2) return set_err(ctx, IPGPU_ERR_ARG, "record slot below 2");
+    ctx->batch_slot = slot;
+  }
+  if (queue >= 0) ctx->batch_queue = queue != 0;
+  if (queue_waves >= 0) ctx->batch_queue_waves = queue_waves;
+  if (rerender_graph >= 0) ctx->rerender_graph = rerender_graph != 0;
+  drop_batch_graph(ctx);
   return IPGPU_OK;
 }
 
@@ -3366,6 +3661,7 @@ static int run_adjoint(ipgpu_ctx *ctx, const ipgpu_image *img, const double *tar
   if (!ctx->has_geom || !ctx->has_model || !ctx->d_dump || ctx->model.kind != IPGPU_MODEL_IHARM)
     return set_err(ctx, IPGPU_ERR_STATE, "the adjoint needs an iharm dump");
   if (!ctx->has_units) return set_err(ctx, IPGPU_ERR_STATE, "the adjoint needs a dump from ipgpu_upload_prims");
+  batch_evict(ctx);
   const ipgpu_image &P = *img;
   if (P.nx <= 0 || P.ny <= 0 || P.maxnstep < 2) return set_err(ctx, IPGPU_ERR_ARG, "bad image size or maxnstep");
   if (P.only_unpolarized || P.transfer_fp32 || P.target_nturns >= 0 || P.counterjet)
@@ -3551,6 +3847,7 @@ int ipgpu_trace_pixel(ipgpu_ctx *ctx, const ipgpu_image *img, int i, int j, int 
   const ipgpu_image &P = *img;
   if (i < 0 || j < 0 || i >= P.nx || j >= P.ny) return set_err(ctx, IPGPU_ERR_ARG, "pixel outside the image");
   if (P.maxnstep < 2 || cap < P.maxnstep) return set_err(ctx, IPGPU_ERR_ARG, "trace capacity below maxnstep");
+  batch_evict(ctx);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   int rc;
   if ((rc = grow(ctx, ctx->d_trrec, ctx->cap_trrec, (size_t)P.maxnstep * REC))) return rc;

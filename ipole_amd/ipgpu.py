@@ -138,6 +138,8 @@ def lib():
         L.ipgpu_render_batch.argtypes = [vp, C.c_int, C.POINTER(Job)]
         L.ipgpu_bad_job.argtypes = [vp, ip, ip, ip]
         L.ipgpu_set_batch_memory.argtypes = [vp, C.c_size_t]
+        L.ipgpu_rerender_batch.argtypes = [vp, C.c_int, C.POINTER(Job)]
+        L.ipgpu_set_batch_options.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.ipgpu_set_render_memory.argtypes = [vp, C.c_size_t]
         L.ipgpu_render_noalloc.argtypes = [vp, C.POINTER(Image), vp, vp, vp, vp, vp]
         L.ipgpu_render_info.argtypes = [vp, ip, ip, C.POINTER(C.c_size_t)]
@@ -355,9 +357,7 @@ class Context:
                     taus=out["taus"].reshape(nx, ny), nstep=out["nstep"].reshape(nx, ny),
                     flag=out["flag"].reshape(nx, ny))
 
-    def render_batch(self, jobs, fetch=True):
-        """Batched render of [(Image, m_unit_scale), ...]; returns one dict per
-        job as render() does (or None without fetch)."""
+    def _batch_call(self, fn, jobs, fetch):
         n = len(jobs)
         arr = (Job * n)()
         outs = []
@@ -371,7 +371,7 @@ class Context:
                 arr[k].image_out, arr[k].imageS_out, arr[k].taus_out = P(o["image"]), P(o["imageS"]), P(o["taus"])
                 arr[k].nstep_out, arr[k].flag_out = IP(o["nstep"]), IP(o["flag"])
                 outs.append(o)
-        rc = self.L.ipgpu_render_batch(self.h, n, arr)
+        rc = fn(self.h, n, arr)
         if rc == ERR_MAXNSTEP:
             jb, i, j = C.c_int(), C.c_int(), C.c_int()
             self.L.ipgpu_bad_job(self.h, C.byref(jb), C.byref(i), C.byref(j))
@@ -386,6 +386,22 @@ class Context:
                             taus=o["taus"].reshape(nx, ny), nstep=o["nstep"].reshape(nx, ny),
                             flag=o["flag"].reshape(nx, ny)))
         return res
+
+    def render_batch(self, jobs, fetch=True):
+        """Batched render of [(Image, m_unit_scale), ...]; returns one dict per
+        job as render() does (or None without fetch)."""
+        return self._batch_call(self.L.ipgpu_render_batch, jobs, fetch)
+
+    def rerender_batch(self, jobs, fetch=True):
+        """The last batch again with new M_unit scales, [(Image, m_unit_scale),
+        ...] with the same images in the same order: its geodesic records are
+        reused and only the transfer runs."""
+        return self._batch_call(self.L.ipgpu_rerender_batch, jobs, fetch)
+
+    def set_batch_options(self, slot=-1, queue=-1, queue_waves=-1, rerender_graph=-1):
+        """Batch launch options; -1 keeps the current value."""
+        self._chk(self.L.ipgpu_set_batch_options(self.h, int(slot), int(queue), int(queue_waves),
+                                                 int(rerender_graph)))
 
     def adjoint(self, image, target_imageS, weights=None, density=True):
         """Loss L = sum w_s (S_s - target_s)^2 over pixels and I Q U V, and its

@@ -150,3 +150,91 @@ def test_batch_maxnstep(setup):
     with pytest.raises(G.IpgpuError) as e:
         c.render_batch([(ok, 1.), (bad, 1.)])
     assert e.value.code == G.ERR_MAXNSTEP and "job 1" in str(e.value)
+
+
+def batch_jobs_of(g, m, scales=None):
+    jobs = []
+    for k, (th, nx, ny, f, s, unpol, et) in enumerate(JOBS):
+        p = O.default_params(nx=nx, ny=ny, thetacam=th, freqcgs=f, only_unpolarized=unpol, emission_type=et)
+        im, _, _ = G.make_image(g, m.L_unit, **gpu_image_kwargs(p))
+        jobs.append((im, s if scales is None else scales[k]))
+    return jobs
+
+
+def same_outputs(a, b, tag):
+    for x, y in zip(a, b):
+        for k in ("image", "imageS", "taus", "nstep", "flag"):
+            assert np.array_equal(x[k], y[k]), (tag, k)
+
+
+def test_batch_queue_launch(setup):
+    """The transfer as a persistent work-queue launch (waves take chunks of
+    64 items from an atomic ticket) gives the images of the one-wave-per-64
+    launch, for any number of waves."""
+    h, prims, c, g, m = setup
+    jobs = batch_jobs_of(g, m)
+    try:
+        c.set_batch_options(queue=0)
+        ref = c.render_batch(jobs)
+        for waves in (1, 7, 0):
+            say("case batch queue launch, %d waves" % waves)
+            c.set_batch_options(queue=1, queue_waves=waves)
+            same_outputs(ref, c.render_batch(jobs), waves)
+    finally:
+        c.set_batch_options(queue=1, queue_waves=0)
+
+
+def test_rerender_batch(setup):
+    """ipgpu_rerender_batch: the last batch's records reused with new M_unit
+    scales equal a fresh batch at those scales, bit for bit, eagerly and from
+    the captured graph; the graph is replayed several times with other work
+    on the stream and host synchronisations between the replays, and every
+    replay is checked.  A new dump of the same grid is allowed; a render that
+    reuses the ray buffers ends the resident batch."""
+    h, prims, c, g, m = setup
+    sa = [1.0, 1.0, 0.3, 2.0, 1.0, 0.5]
+    sb = [0.7, 1.9, 0.45, 1.1, 3.0, 0.25]
+    ja, jb = batch_jobs_of(g, m, sa), batch_jobs_of(g, m, sb)
+    say("case rerender: fresh batches at two scale sets")
+    fresh_b = c.render_batch(jb)
+    fresh_a = c.render_batch(ja)
+    say("case rerender: eager")
+    c.set_batch_options(rerender_graph=0)
+    same_outputs(fresh_b, c.rerender_batch(jb), "eager b")
+    same_outputs(fresh_a, c.rerender_batch(ja), "eager a")
+    c.set_batch_options(rerender_graph=1)
+    say("case rerender: graph replays")
+    for k, (jobs, ref) in enumerate([(jb, fresh_b), (ja, fresh_a), (jb, fresh_b), (jb, fresh_b), (ja, fresh_a)]):
+        same_outputs(ref, c.rerender_batch(jobs), "replay %d" % k)
+        c.download_fields()                        # other work on the stream between the replays
+        assert c.step_counts()[0] == 0
+    say("case rerender: new dump of the same grid")
+    c.upload_prims(prims, G.iharm_units(2 * M_UNIT))
+    half = c.rerender_batch(batch_jobs_of(g, m, [s / 2 for s in sa]))
+    c.upload_prims(prims, G.iharm_units(M_UNIT))
+    for k, (x, y) in enumerate(zip(fresh_a, half)):
+        assert np.array_equal(x["nstep"], y["nstep"])
+        assert abs(y["image"].sum() / x["image"].sum() - 1) < 1e-9, k
+        e = stokes_errors(x["imageS"], x["image"], y["imageS"], y["image"])
+        for name, (mx, frac) in e.items():
+            if JOBS[k][5] and name != "unpol":
+                continue
+            assert mx < 1e-6, (k, name, mx)
+    say("case rerender: a different image, a plain render in between")
+    other = list(ja)
+    im0, _, _ = G.make_image(g, m.L_unit, nx=8, ny=8, thetacam=30., freqcgs=230e9)
+    other[0] = (im0, 1.)
+    with pytest.raises(G.IpgpuError) as e:
+        c.rerender_batch(other)
+    assert e.value.code == G.ERR_ARG
+    c.render(im0)
+    with pytest.raises(G.IpgpuError) as e:
+        c.rerender_batch(ja)
+    assert e.value.code == G.ERR_STATE
+    say("case rerender: a batch in passes is not resident")
+    c.set_batch_memory(1 << 20)
+    c.render_batch(ja)
+    with pytest.raises(G.IpgpuError) as e:
+        c.rerender_batch(ja)
+    assert e.value.code == G.ERR_STATE
+    c.set_batch_memory(48 << 30)
