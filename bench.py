@@ -73,7 +73,7 @@ def roofline(stage_ms, n_geo, streamed):
     ach = {k: per_unit[k] * n_geo / (t[k] * 1e-3) / 1e12 for k in t}
     dom = max(t, key=t.get)
     name = "k_trace_budget+k_trace_resume_mask" if streamed and dom == "k_trace" else dom
-    return dict(bound="mfma", achieved=round(ach[dom], 4), peak=PEAK_FP64, unit="TFLOP/s",
+    return dict(bound="fp64-valu", achieved=round(ach[dom], 4), peak=PEAK_FP64, unit="TFLOP/s",
                 frac=round(ach[dom] / PEAK_FP64, 5), traffic=tr.get(dom), kernel=name,
                 note="FP64 compute roof (MI355X FP64 peak 78.6 TFLOP/s, which the FP64 vector ALU "
                      "also reaches; these kernels use the vector ALU, no matrix instructions). "
@@ -96,6 +96,7 @@ def batch_measure(args, h=None, prims=None):
     if args.small:
         jobs = jobs[:8]
     M_ref = 1e25
+    fl = json.load(open(os.path.join(ROOT, "ipole_amd", "flops_per_step.json")))["pol"]
     ctx = G.Context(0)
     g, m = G.iharm_geom(h), G.iharm_model()
     ctx.set_geometry(g)
@@ -121,18 +122,39 @@ def batch_measure(args, h=None, prims=None):
     ctx.upload_prims(prims, G.iharm_units(M_ref, trat_large=cfg["trat_large"]))
     batch = [(im, jb["M_unit"] / M_ref) for jb, im in zip(jobs, ims)]
     nrep = max(1, min(args.steps, 3))
-    ctx.render_batch(batch, fetch=False)
+    tuned = ctx.autotune_batch(batch)
+    log("batch FP64 autotuned: %s" % tuned)
     t_dev = []
     for s in range(nrep):
         ctx.render_batch(batch, fetch=False)
         t_dev.append(ctx.kernel_times())
     res = ctx.render_batch(batch)
+    n_rad = ctx.step_counts()[1]
+    n_xfer = int(sum(int(r["nstep"].sum()) for r in res))
     dF = max(abs(r["imageS"][..., 0].sum() / f - 1) for r, f in zip(res, Fplain))
     ms = float(np.mean([t[2] for t in t_dev]))
+    ms_x = float(np.mean([t[1] for t in t_dev]))
     log("batch FP64: %.1f ms device (geodesic %.1f, transfer %.1f); max |dFtot|/Ftot vs plain %.2e" %
-        (ms, np.mean([t[0] for t in t_dev]), np.mean([t[1] for t in t_dev]), dF))
+        (ms, np.mean([t[0] for t in t_dev]), ms_x, dF))
+    # the fused transfer's algorithmic FLOP: radiating and non-radiating
+    # steps and rays at the op-counting build's per-unit counts
+    flop = (fl["F_fused_per_radiating_step"] * n_rad + fl["F_fused_per_nonradiating_step"] * (n_xfer - n_rad)
+            + fl["F_fused_per_ray"] * npix)
+    tflops = flop / (ms_x * 1e-3) / 1e12
+    # the M_unit search: the same jobs at new scales on the resident records
+    # (the transfer alone, replayed from its captured graph)
+    batch_b = [(im, 1.5 * jb["M_unit"] / M_ref) for jb, im in zip(jobs, ims)]
+    ctx.rerender_batch(batch_b, fetch=False)
+    t_rr = []
+    for s in range(nrep):
+        ctx.rerender_batch(batch if s % 2 else batch_b, fetch=False)
+        t_rr.append(ctx.kernel_times()[2])
+    ms_rr = float(np.mean(t_rr))
+    log("batch FP64 transfer: %d steps (%d radiating), %.3g FLOP, %.2f TFLOP/s (%.1f%% of FP64 peak); "
+        "rerender at new M_unit scales %.1f ms" % (n_xfer, n_rad, flop, tflops, 100 * tflops / PEAK_FP64, ms_rr))
     batch32 = [(im, jb["M_unit"] / M_ref) for jb, im in zip(jobs, ims32)]
-    ctx.render_batch(batch32, fetch=False)
+    tuned32 = ctx.autotune_batch(batch32)
+    log("batch mixed autotuned: %s" % tuned32)
     t32 = []
     for s in range(nrep):
         ctx.render_batch(batch32, fetch=False)
@@ -149,6 +171,16 @@ def batch_measure(args, h=None, prims=None):
                 plain_pixels_per_s=round(npix / (sum(plain_ms) * 1e-3), 1),
                 fp64_ms_device=round(ms, 3), fp64_pixels_per_s=round(npix / (ms * 1e-3), 1),
                 fp64_speedup_vs_plain=round(sum(plain_ms) / ms, 3), fp64_max_dFtot_vs_plain=dF,
+                fp64_autotuned=tuned, transfer_steps=n_xfer, radiating_steps=int(n_rad),
+                fp64_transfer_ms_device=round(ms_x, 3),
+                fp64_transfer_roofline=dict(
+                    bound="fp64-valu", kernel="k_xfer_queue<FP64>" if tuned["queue"] else "k_xfer_batch<FP64>",
+                    achieved=round(tflops, 3), peak=PEAK_FP64, unit="TFLOP/s", frac=round(tflops / PEAK_FP64, 4),
+                    flop=flop, note="FLOP = radiating steps x %.0f + non-radiating x %.0f + rays x %.0f "
+                                    "(op-counting build, ipole_amd/flops_per_step.json)"
+                                    % (fl["F_fused_per_radiating_step"], fl["F_fused_per_nonradiating_step"],
+                                       fl["F_fused_per_ray"])),
+                fp64_rerender_ms_device=round(ms_rr, 3), mixed_autotuned=tuned32,
                 mixed_ms_device=round(ms32, 3), mixed_pixels_per_s=round(npix / (ms32 * 1e-3), 1),
                 mixed_speedup_vs_plain=round(sum(plain_ms) / ms32, 3), mixed_max_dFtot_vs_plain=dF32)
 

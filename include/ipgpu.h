@@ -246,6 +246,17 @@ int ipgpu_rerender_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs);
  * eagerly (0).  No counterpart in the reference. */
 int ipgpu_set_batch_options(ipgpu_ctx *ctx, int slot, int queue, int queue_waves, int rerender_graph);
 
+/* Pick the batch options for this job set by measurement and keep them in
+ * the context: the record slot (1024, 2048 or 4096; whole batches timed on
+ * the device), then the transfer launch (one wave per 64 items, or the work
+ * queue at one or two times the waves the transfer kernel keeps resident;
+ * the transfer alone timed on the records left in place).  The last batch
+ * stays resident for ipgpu_rerender_batch.  chosen (optional): slot, queue,
+ * queue waves and resident waves per compute unit.  Output pointers of the
+ * jobs are ignored.  No counterpart in the reference, whose only launch
+ * choice is OpenMP's dynamic schedule over pixels (src/main.c:536). */
+int ipgpu_autotune_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs, int chosen[4]);
+
 /* Device memory a single-image render (ipgpu_render) may use for its ray
  * records and step packets (default 64 GiB).  An image whose rays need more
  * is rendered in chunks of rays (whole 8x8 pixel tiles) that fit; the image

@@ -1619,6 +1619,7 @@ struct ipgpu_ctx {
   int batch_queue = 1;
   int batch_queue_waves = 0;
   int ncu = 256;
+  int xq_occ[2] = {1, 1};   /* resident waves per compute unit of k_xfer_queue<FP64>, <F32> */
   unsigned int *d_queue = nullptr;
   /* the last in-place batch, for ipgpu_rerender_batch: its images, trace
    * scales, output offsets, items and the graph of its transfer (job table
@@ -1705,7 +1706,14 @@ static int grow(ipgpu_ctx *ctx, T *&ptr, size_t &cap, size_t need)
   }
   ptr = nullptr;
   cap = 0;
-  HIPCHK(ctx, hipMalloc((void **)&ptr, need * sizeof(T)));
+  hipError_t e = hipMalloc((void **)&ptr, need * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    ptr = nullptr;
+    snprintf(ctx->err, sizeof(ctx->err), "device allocation of %zu bytes failed (%s); the context holds %zu bytes",
+             need * sizeof(T), hipGetErrorString(e), ctx->dev_bytes);
+    return e == hipErrorOutOfMemory ? IPGPU_ERR_NOMEM : IPGPU_ERR_HIP;
+  }
   cap = need;
   ctx->dev_bytes += need * sizeof(T);
   return 0;
@@ -1748,6 +1756,12 @@ int ipgpu_create(ipgpu_ctx **out, int device)
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
     ctx->ncu = ncu;
+  for (int f = 0; f < 2; f++) {
+    int nb = 0;
+    hipError_t e = f ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_xfer_queue<true>, WAVE, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_xfer_queue<false>, WAVE, 0);
+    if (e == hipSuccess && nb > 0) ctx->xq_occ[f] = nb;
+  }
   if (const char *e = getenv("IPGPU_BATCH_QUEUE")) ctx->batch_queue = atoi(e);
   if (const char *e = getenv("IPGPU_BATCH_QUEUE_WAVES")) ctx->batch_queue_waves = atoi(e);
   if (noalloc_setup(device, &ctx->d_scratch) != IPGPU_OK) {
@@ -3102,7 +3116,7 @@ static int batch_transfer(ipgpu_ctx *ctx, bool f32, int nitems, const double *tr
   }
   if (nitems <= 0) return 0;
   if (ctx->batch_queue) {
-    const int nw = ctx->batch_queue_waves > 0 ? ctx->batch_queue_waves : 4 * ctx->ncu;
+    const int nw = ctx->batch_queue_waves > 0 ? ctx->batch_queue_waves : ctx->xq_occ[f32 ? 1 : 0] * ctx->ncu;
     auto kq = f32 ? k_xfer_queue<true> : k_xfer_queue<false>;
     kq<<<nw, WAVE, 0, st>>>(ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items, nitems, ctx->d_queue,
                             ctx->d_off, ctx->d_traj, traj2, ctx->d_nstep_ray, nturns, ctx->d_dump, ctx->d_bimage,
@@ -3211,13 +3225,32 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
    * fits the budget, also its first slot records (most rays are then
    * complete and only the longer ones are continued, from their last slot
    * record), else a count alone and the records in pass 2 */
-  const long long budget = (long long)(ctx->batch_traj_bytes / (REC * sizeof(double)));
+  /* the record budget: the set limit, or what the device can give the two
+   * record arenas (free memory plus what they hold now, less 4 GiB for the
+   * images, items and other buffers), whichever is smaller */
+  size_t traj_bytes = ctx->batch_traj_bytes;
+  {
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) == hipSuccess) {
+      const size_t held = (ctx->traj_doubles + ctx->traj2_doubles) * sizeof(double);
+      const size_t reserve = (size_t)4 << 30;
+      const size_t avail = freeb + held > 2 * reserve ? freeb + held - reserve : (freeb + held) / 2;
+      traj_bytes = std::min(traj_bytes, avail);
+    }
+  }
+  long long budget = (long long)(traj_bytes / (REC * sizeof(double)));
   const int slot = ctx->batch_slot;
-  const bool try_slot = !turns && slot >= 2 && (long long)rays_total * slot <= budget;
+  bool try_slot = !turns && slot >= 2 && (long long)rays_total * slot <= budget;
   const bool pairs = !turns;   /* the single-lane trace counts subring turns */
   if ((rc = grow(ctx, ctx->d_nstep_ray, ctx->cap_nstep_ray, (size_t)rays_total))) return rc;
   if ((rc = grow(ctx, ctx->d_off, ctx->cap_off, (size_t)rays_total))) return rc;
-  if (try_slot && (rc = grow(ctx, ctx->d_traj, ctx->traj_doubles, (size_t)rays_total * slot * REC))) return rc;
+  if (try_slot && (rc = grow(ctx, ctx->d_traj, ctx->traj_doubles, (size_t)rays_total * slot * REC))) {
+    /* the device could not give the slots after all: count first, and hold
+     * the passes to half of what failed */
+    if (rc != IPGPU_ERR_NOMEM) return rc;
+    try_slot = false;
+    budget = std::min(budget, (long long)rays_total * slot / 2);
+  }
   HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   if (pairs)
     k_trace_slot<<<(unsigned)((2 * rays_total + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>(
@@ -3266,7 +3299,13 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
         nlong++;
       }
     }
-  if (try_slot && (long long)rays_total * slot + long_recs <= budget) {
+  bool in_place = try_slot && (long long)rays_total * slot + long_recs <= budget;
+  if (in_place && nlong > 0 && (rc = grow(ctx, ctx->d_traj2, ctx->traj2_doubles, (size_t)long_recs * REC))) {
+    if (rc != IPGPU_ERR_NOMEM) return rc;
+    in_place = false;   /* the passes below, in the memory the slots held */
+    budget = std::min(budget, (long long)rays_total * slot);
+  }
+  if (in_place) {
     /* every ray's records in place: its slot, or for the nlong longest (at
      * the front of order) a full-length region of traj2 (off = -1 - offset) */
     std::vector<long long> off2(nlong > 0 ? nlong : 1);
@@ -3306,7 +3345,6 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     if ((rc = grow(ctx, ctx->d_items, ctx->cap_items, (size_t)(nitems > 0 ? nitems : 1)))) return rc;
     if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)(nlong > 0 ? nlong : 1)))) return rc;
     if ((rc = grow(ctx, ctx->d_off2, ctx->cap_off2, (size_t)(nlong > 0 ? nlong : 1)))) return rc;
-    if (nlong > 0 && (rc = grow(ctx, ctx->d_traj2, ctx->traj2_doubles, (size_t)long_recs * REC))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_off, off_host.data(), rays_total * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
@@ -3343,6 +3381,13 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     nvalid_done = nvalid;
     steps_host = slot_steps;
   }
+  if (nvalid_done < nvalid && ctx->d_traj2) {
+    /* passes use one record arena: the budget counted the second one's memory */
+    HIPCHK(ctx, hipFree(ctx->d_traj2));
+    ctx->dev_bytes -= ctx->traj2_doubles * sizeof(double);
+    ctx->d_traj2 = nullptr;
+    ctx->traj2_doubles = 0;
+  }
   for (long long a = nvalid_done; a < nvalid;) {
     /* the longest prefix of the remaining rays whose records fit */
     long long b = a, recs = 0;
@@ -3352,7 +3397,11 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
       b++;
     }
     const int nr = (int)(b - a);
-    if ((rc = grow(ctx, ctx->d_traj, ctx->traj_doubles, (size_t)recs * REC))) return rc;
+    if ((rc = grow(ctx, ctx->d_traj, ctx->traj_doubles, (size_t)recs * REC))) {
+      if (rc != IPGPU_ERR_NOMEM || b - a == 1) return rc;
+      budget = recs / 2;   /* a smaller pass over the same rays */
+      continue;
+    }
     if (turns && (rc = grow(ctx, ctx->d_nturns, ctx->nturns_ints, (size_t)recs))) return rc;
     if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)nr))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_off, off_host.data(), rays_total * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
@@ -3482,7 +3531,7 @@ int ipgpu_rerender_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     b.sc.b = sqrt(jobs[j].m_unit_scale);
     ctx->h_bjobs[j] = b;
   }
-  const int nw = ctx->batch_queue_waves > 0 ? ctx->batch_queue_waves : 4 * ctx->ncu;
+  const int nw = ctx->batch_queue_waves > 0 ? ctx->batch_queue_waves : ctx->xq_occ[L.f32 ? 1 : 0] * ctx->ncu;
   ipgpu_ctx::GraphKey key;
   memset(&key, 0, sizeof(key));
   key.geom = ctx->geom;
@@ -3562,6 +3611,74 @@ int ipgpu_set_batch_options(ipgpu_ctx *ctx, int slot, int queue, int queue_waves
   if (queue_waves >= 0) ctx->batch_queue_waves = queue_waves;
   if (rerender_graph >= 0) ctx->rerender_graph = rerender_graph != 0;
   drop_batch_graph(ctx);
+  return IPGPU_OK;
+}
+
+int ipgpu_autotune_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs, int chosen[4])
+{
+  if (!ctx || njobs <= 0 || !jobs) return IPGPU_ERR_ARG;
+  std::vector<ipgpu_job> q(jobs, jobs + njobs);
+  for (ipgpu_job &j : q) {
+    j.image_out = j.imageS_out = j.taus_out = nullptr;
+    j.nstep_out = j.flag_out = nullptr;
+  }
+  const int slot0 = ctx->batch_slot, queue0 = ctx->batch_queue, waves0 = ctx->batch_queue_waves;
+  const int graph0 = ctx->rerender_graph;
+  auto restore = [&](int rc) {
+    ctx->batch_slot = slot0;
+    ctx->batch_queue = queue0;
+    ctx->batch_queue_waves = waves0;
+    ctx->rerender_graph = graph0;
+    return rc;
+  };
+  /* 1. the record slot of the first trace: whole batches, device time */
+  static const int slots[3] = {1024, 2048, 4096};
+  int best_slot = slot0, rc;
+  float best = 3.4e38f;
+  for (int s : slots) {
+    ctx->batch_slot = s;
+    if ((rc = ipgpu_render_batch(ctx, njobs, q.data()))) return restore(rc);
+    /* the trace and transfer launches (the first use of a larger record
+     * arena also pays for its allocation, which later batches do not) */
+    if (ctx->ms_geo + ctx->ms_xfer < best) {
+      best = ctx->ms_geo + ctx->ms_xfer;
+      best_slot = s;
+    }
+  }
+  ctx->batch_slot = best_slot;
+  if (best_slot != slots[2] && (rc = ipgpu_render_batch(ctx, njobs, q.data()))) return restore(rc);
+  /* 2. the transfer launch: one wave per 64 items, or the work queue at one
+   * and two times the resident waves; the transfer alone, on the records
+   * left in place (a batch in passes keeps the default launch) */
+  int best_q = queue0, best_w = waves0;
+  if (ctx->blast.valid) {
+    const int occ = ctx->xq_occ[ctx->blast.f32 ? 1 : 0] * ctx->ncu;
+    const int cand[3][2] = {{0, 0}, {1, occ}, {1, 2 * occ}};
+    ctx->rerender_graph = 0;
+    best = 3.4e38f;
+    for (const auto &c : cand) {
+      ctx->batch_queue = c[0];
+      ctx->batch_queue_waves = c[1];
+      if ((rc = ipgpu_rerender_batch(ctx, njobs, q.data()))) return restore(rc);
+      if (ctx->ms_total < best) {
+        best = ctx->ms_total;
+        best_q = c[0];
+        best_w = c[1];
+      }
+    }
+    ctx->rerender_graph = graph0;
+  }
+  ctx->batch_queue = best_q;
+  ctx->batch_queue_waves = best_w;
+  drop_batch_graph(ctx);
+  if (chosen) {
+    chosen[0] = ctx->batch_slot;
+    chosen[1] = ctx->batch_queue;
+    chosen[2] = ctx->batch_queue > 0 ? (ctx->batch_queue_waves > 0 ? ctx->batch_queue_waves
+                                                                   : ctx->xq_occ[q[0].image.transfer_fp32 ? 1 : 0] * ctx->ncu)
+                                     : 0;
+    chosen[3] = ctx->xq_occ[q[0].image.transfer_fp32 ? 1 : 0];
+  }
   return IPGPU_OK;
 }
 

@@ -140,6 +140,7 @@ def lib():
         L.ipgpu_set_batch_memory.argtypes = [vp, C.c_size_t]
         L.ipgpu_rerender_batch.argtypes = [vp, C.c_int, C.POINTER(Job)]
         L.ipgpu_set_batch_options.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ipgpu_autotune_batch.argtypes = [vp, C.c_int, C.POINTER(Job), ip]
         L.ipgpu_set_render_memory.argtypes = [vp, C.c_size_t]
         L.ipgpu_render_noalloc.argtypes = [vp, C.POINTER(Image), vp, vp, vp, vp, vp]
         L.ipgpu_render_info.argtypes = [vp, ip, ip, C.POINTER(C.c_size_t)]
@@ -397,6 +398,18 @@ class Context:
         ...] with the same images in the same order: its geodesic records are
         reused and only the transfer runs."""
         return self._batch_call(self.L.ipgpu_rerender_batch, jobs, fetch)
+
+    def autotune_batch(self, jobs):
+        """Measure and keep the batch options for [(Image, m_unit_scale), ...]:
+        returns dict(slot, queue, queue_waves, resident_waves_per_cu)."""
+        n = len(jobs)
+        arr = (Job * n)()
+        for k, (image, mscale) in enumerate(jobs):
+            arr[k].image = image
+            arr[k].m_unit_scale = mscale
+        ch = np.zeros(4, np.int32)
+        self._chk(self.L.ipgpu_autotune_batch(self.h, n, arr, IP(ch)))
+        return dict(slot=int(ch[0]), queue=int(ch[1]), queue_waves=int(ch[2]), resident_waves_per_cu=int(ch[3]))
 
     def set_batch_options(self, slot=-1, queue=-1, queue_waves=-1, rerender_graph=-1):
         """Batch launch options; -1 keeps the current value."""

@@ -33,13 +33,14 @@ def count(geom, model, image, dump, pix):
     subprocess.run(["make", "-s", "-C", HERE, "libopcount.so"], check=True)
     L = C.CDLL(os.path.join(HERE, "libopcount.so"))
     L.opcount_render.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), G.dp, G.ip,
-                                 C.c_int, G.dp]
-    out = np.zeros(9)
+                                 C.c_int, G.dp, G.dp]
+    out = np.zeros(10)
     pix = np.ascontiguousarray(pix, np.int32)
+    per_ray = np.zeros((len(pix), 3))
     rc = L.opcount_render(C.byref(geom), C.byref(model), C.byref(image),
-                          G.P(dump) if dump is not None else None, G.IP(pix), len(pix), G.P(out))
+                          G.P(dump) if dump is not None else None, G.IP(pix), len(pix), G.P(out), G.P(per_ray))
     assert rc == 0
-    return out
+    return out, per_ray
 
 
 def main():
@@ -56,8 +57,14 @@ def main():
     for unpol in (0, 1):
         im, _, _ = G.make_image(g, m.L_unit, nx=160, ny=160, thetacam=163., freqcgs=230e9,
                                 only_unpolarized=unpol)
-        o = count(g, m, im, dump, pix)
+        o, per_ray = count(g, m, im, dump, pix)
         n_geo, n_step, n_rad = o[0], o[1], o[2]
+        # the fused transfer's flops per ray as a radiating and a non-radiating
+        # step cost plus a per-ray cost (far-end coefficients, projection),
+        # least squares over the rays
+        A = np.column_stack([per_ray[:, 1], per_ray[:, 0] - per_ray[:, 1], np.ones(len(per_ray))])
+        (f_rad, f_non, f_ray), *_ = np.linalg.lstsq(A, per_ray[:, 2], rcond=None)
+        fit_err = float(np.abs(A @ [f_rad, f_non, f_ray] - per_ray[:, 2]).sum() / per_ray[:, 2].sum())
         res["unpol" if unpol else "pol"] = dict(
             geodesic_steps=int(n_geo), transfer_steps=int(n_step), radiating_steps=int(n_rad), rays=int(o[8]),
             F_trace_per_step=o[3] / n_geo,
@@ -65,6 +72,9 @@ def main():
             F_prep_per_nonradiating_step=o[5] / max(n_step - n_rad, 1),
             F_prep_per_step=(o[4] + o[5]) / n_step,
             F_chain_per_step=(o[6] + o[7]) / n_step,
+            F_fused_per_step=o[9] / n_step,
+            F_fused_per_radiating_step=f_rad, F_fused_per_nonradiating_step=f_non, F_fused_per_ray=f_ray,
+            F_fused_fit_rel_err=fit_err,
         )
         print(("unpol" if unpol else "pol"), json.dumps(res["unpol" if unpol else "pol"]), flush=True)
     out = dict(
@@ -76,7 +86,14 @@ def main():
         units=dict(F_trace_per_step="FLOP per geodesic step of k_trace (stepsize + midpoint push)",
                    F_prep_per_step="FLOP per transfer step of k_prep (mix of radiating and not)",
                    F_chain_per_step="FLOP per transfer step of k_chain, including its per-ray "
-                                    "far-end coefficients and projection"),
+                                    "far-end coefficients and projection",
+                   F_fused_per_step="FLOP per transfer step of the FP64 fused transfer of the batch "
+                                    "(k_xfer_batch / k_xfer_queue), far-end coefficients included",
+                   F_fused_per_radiating_step="the fused transfer's FLOP per ray fitted (least squares over "
+                                              "the rays) as radiating steps x this + non-radiating steps x "
+                                              "F_fused_per_nonradiating_step + F_fused_per_ray; "
+                                              "F_fused_fit_rel_err is the fit's summed |residual| over the "
+                                              "summed count"),
         **res)
     path = os.path.join(ROOT, "ipole_amd", "flops_per_step.json")
     with open(path, "w") as f:

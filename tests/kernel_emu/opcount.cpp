@@ -90,9 +90,11 @@ extern "C" {
  * records, or null for the sphere).  out[0..8]: geodesic steps, transfer
  * steps, radiating steps, flops of trace_ray, of prep_step on radiating
  * steps, of prep_step on non-radiating steps, of chain_ray, of
- * far_end_coefficients, number of rays. */
+ * far_end_coefficients, number of rays, of transfer_fused (the batch's
+ * FP64 fused transfer of the same ray).  per_ray (optional, 3 per pixel):
+ * the pixel's transfer steps, radiating steps and transfer_fused flops. */
 int opcount_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, const double *dump,
-                   const int *pix, int npix, double *out)
+                   const int *pix, int npix, double *out, double *per_ray)
 {
   std::vector<Cd> rec((size_t)P->maxnstep * REC);
   std::vector<int> nt(P->target_nturns >= 0 ? P->maxnstep : 1);
@@ -100,7 +102,7 @@ int opcount_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image 
   const int stride = pol ? PK : PK_UNPOL;
   std::vector<Cd> pkt;
   const Cd *d = reinterpret_cast<const Cd *>(dump);
-  for (int q = 0; q < 9; q++) out[q] = 0.;
+  for (int q = 0; q < 10; q++) out[q] = 0.;
   for (int n = 0; n < npix; n++) {
     int i = pix[2 * n], j = pix[2 * n + 1];
     int *ntp = P->target_nturns >= 0 ? nt.data() : nullptr;
@@ -133,6 +135,16 @@ int opcount_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image 
     chain_ray(*G, *P, ns, pkt.data(), stride, js, ks, o7, nrad);
     out[6] += g_ops.flops() - c.flops();
     out[8] += 1;
+    Ops e = g_ops;
+    Cd f7[7];
+    int nrad_f;
+    transfer_fused<false>(*G, *M, *P, ns, rec.data(), ntp, d, unit_scale(), f7, nrad_f);
+    out[9] += g_ops.flops() - e.flops();
+    if (per_ray) {
+      per_ray[3 * n] = ns;
+      per_ray[3 * n + 1] = nrad_f;
+      per_ray[3 * n + 2] = g_ops.flops() - e.flops();
+    }
   }
   return 0;
 }

@@ -238,3 +238,22 @@ def test_rerender_batch(setup):
         c.rerender_batch(ja)
     assert e.value.code == G.ERR_STATE
     c.set_batch_memory(48 << 30)
+
+
+def test_autotune_batch(setup):
+    """ipgpu_autotune_batch picks a slot and a transfer launch by timing them,
+    keeps them, and leaves the batch resident: the images are those of the
+    default options."""
+    h, prims, c, g, m = setup
+    say("case batch autotune")
+    jobs = batch_jobs_of(g, m)
+    ref = c.render_batch(jobs)
+    try:
+        ch = c.autotune_batch(jobs)
+        say("  chosen %s" % ch)
+        assert ch["slot"] in (1024, 2048, 4096) and ch["queue"] in (0, 1)
+        assert ch["resident_waves_per_cu"] >= 1
+        same_outputs(ref, c.rerender_batch(jobs), "resident after tuning")
+        same_outputs(ref, c.render_batch(jobs), "tuned options")
+    finally:
+        c.set_batch_options(slot=2048, queue=1, queue_waves=0)
