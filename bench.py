@@ -383,6 +383,8 @@ def main():
                                 freqcgs=cfg["freqcgs"])
     npix = cfg["nx"] * cfg["ny"]
 
+    tuned = ctx.autotune_render(im)
+    log("autotuned streamed render: %s (%.2f ms)" % (tuned, ctx.kernel_times()[2]))
     for w in range(args.warmup):
         ctx.render(im, fetch=False)
         log("warmup %d done: %.2f ms" % (w, ctx.kernel_times()[2]))
@@ -406,7 +408,7 @@ def main():
         ctx.render(im, fetch=False)
         one.append(ctx.kernel_times()[2])
     log("one-pass render of the same image: %.3f ms" % np.mean(one[1:]))
-    ctx.set_tuning(trace_budget=3584)
+    ctx.set_tuning(trace_budget=tuned["trace_budget"])
     adaptive = adaptive_measure(ctx, im, res, scale, cfg)
     st = G.image_stats(res, scale, cfg["freqcgs"], 16.9e6)
     ns_img = res["nstep"]
@@ -455,7 +457,8 @@ def main():
                            radiating_steps=n_rad, ms_device=round(float(np.mean(ms_tot)), 4),
                            ms_geodesic=round(float(np.mean(ms_geo)), 4),
                            ms_transfer=round(float(np.mean(ms_xfer)), 4),
-                           streamed=bool(streamed), ms_one_pass=round(float(np.mean(one[1:])), 4)),
+                           streamed=bool(streamed), autotuned=tuned,
+                           ms_one_pass=round(float(np.mean(one[1:])), 4)),
                roofline=roof, cpu_baseline=cpu, batched=batched, mixed_precision=precision,
                adjoint=adjoint, adaptive=adaptive)
     print(json.dumps(out), flush=True)

@@ -388,6 +388,16 @@ int ipgpu_stage_times(const ipgpu_ctx *ctx, double ms[4]);
 int ipgpu_set_tuning(ipgpu_ctx *ctx, int trace_budget, int trace_phases, int seg_min_steps, int seg_len,
                      int record_steps);
 
+/* Pick the streamed render's settings for this image by measurement and
+ * keep them in the context: the segment length (256, 384, 512), the segment
+ * threshold (the last phase's rays, or also the phase before's), then the
+ * bounded phases' budget (3/4 or 5/4 of the current), one at a time, each
+ * timed as the better of two renders on the device.  chosen (optional):
+ * trace_budget, trace_phases, seg_min_steps, seg_len.  The images do not
+ * depend on the budget; the segment settings regroup the transfer
+ * arithmetic (agreement to roundoff).  No counterpart in the reference. */
+int ipgpu_autotune_render(ipgpu_ctx *ctx, const ipgpu_image *img, int chosen[4]);
+
 /* Work counts of the last render: geodesic steps summed over rays (the
  * nstep of trace_geodesic) and radiating steps of the transfer pass (steps
  * with radiating_region true, src/ipolarray.c:122). */

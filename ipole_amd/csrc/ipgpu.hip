@@ -4004,6 +4004,85 @@ int ipgpu_set_tuning(ipgpu_ctx *ctx, int trace_budget, int trace_phases, int seg
   return IPGPU_OK;
 }
 
+int ipgpu_autotune_render(ipgpu_ctx *ctx, const ipgpu_image *img, int chosen[4])
+{
+  if (!ctx || !img) return IPGPU_ERR_ARG;
+  const int tb0 = ctx->trace_budget, ph0 = ctx->trace_phases, sm0 = ctx->seg_min_steps, sl0 = ctx->seg_len;
+  /* device time of one setting: the better of two renders after a first
+   * one that grows the arenas for it */
+  auto timed = [&](float &ms) -> int {
+    float best = 3.4e38f;
+    for (int rep = 0; rep < 3; rep++) {
+      int rc = ipgpu_render(ctx, img, nullptr, nullptr, nullptr, nullptr, nullptr);
+      if (rc) return rc;
+      if (rep > 0 && ctx->ms_total < best) best = ctx->ms_total;
+    }
+    ms = best;
+    return 0;
+  };
+  auto fail = [&](int rc) {
+    ctx->trace_budget = tb0;
+    ctx->trace_phases = ph0;
+    ctx->seg_min_steps = sm0;
+    ctx->seg_len = sl0;
+    return rc;
+  };
+  int rc;
+  float best;
+  if ((rc = timed(best))) return fail(rc);
+  /* one setting at a time, the others held at the best so far: segment
+   * length, the segment threshold (every ray of the last phase, or also
+   * those of the phase before), then the budget of the bounded phases */
+  const int seg_lens[2] = {256, 512};
+  for (int v : seg_lens) {
+    const int keep = ctx->seg_len;
+    ctx->seg_len = v;
+    float ms;
+    if ((rc = timed(ms))) return fail(rc);
+    if (ms < best) best = ms;
+    else ctx->seg_len = keep;
+  }
+  {
+    const int keep = ctx->seg_min_steps;
+    const int last = ctx->trace_budget, nb = ctx->trace_phases > 1 ? ctx->trace_phases : 1;
+    const int cand[2] = {last, (int)((long long)last * (nb > 1 ? nb - 1 : 1) / (nb > 1 ? nb : 2))};
+    for (int v : cand) {
+      if (v == keep) continue;
+      ctx->seg_min_steps = v;
+      float ms;
+      if ((rc = timed(ms))) return fail(rc);
+      if (ms < best) best = ms;
+      else ctx->seg_min_steps = keep;
+    }
+  }
+  {
+    const int keep_b = ctx->trace_budget, keep_m = ctx->seg_min_steps;
+    const int cand[2] = {keep_b * 3 / 4, keep_b * 5 / 4};
+    for (int v : cand) {
+      if (v < 256) continue;
+      ctx->trace_budget = v;
+      /* a threshold at the old budget follows it */
+      if (keep_m == keep_b) ctx->seg_min_steps = v;
+      float ms;
+      if ((rc = timed(ms))) return fail(rc);
+      if (ms < best) {
+        best = ms;
+        break;
+      }
+      ctx->trace_budget = keep_b;
+      ctx->seg_min_steps = keep_m;
+    }
+  }
+  if (chosen) {
+    chosen[0] = ctx->trace_budget;
+    chosen[1] = ctx->trace_phases;
+    chosen[2] = ctx->seg_min_steps;
+    chosen[3] = ctx->seg_len;
+  }
+  ctx->ms_total = best;
+  return IPGPU_OK;
+}
+
 /* native_coord + root_find, src/coordinates.c:375-442 */
 int ipgpu_native_coord(const ipgpu_geom *G, double r, double thd, double phid, double X[4])
 {

@@ -141,6 +141,7 @@ def lib():
         L.ipgpu_rerender_batch.argtypes = [vp, C.c_int, C.POINTER(Job)]
         L.ipgpu_set_batch_options.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.ipgpu_autotune_batch.argtypes = [vp, C.c_int, C.POINTER(Job), ip]
+        L.ipgpu_autotune_render.argtypes = [vp, C.POINTER(Image), ip]
         L.ipgpu_set_render_memory.argtypes = [vp, C.c_size_t]
         L.ipgpu_render_noalloc.argtypes = [vp, C.POINTER(Image), vp, vp, vp, vp, vp]
         L.ipgpu_render_info.argtypes = [vp, ip, ip, C.POINTER(C.c_size_t)]
@@ -410,6 +411,13 @@ class Context:
         ch = np.zeros(4, np.int32)
         self._chk(self.L.ipgpu_autotune_batch(self.h, n, arr, IP(ch)))
         return dict(slot=int(ch[0]), queue=int(ch[1]), queue_waves=int(ch[2]), resident_waves_per_cu=int(ch[3]))
+
+    def autotune_render(self, image):
+        """Measure and keep the streamed render's settings for this image:
+        returns dict(trace_budget, trace_phases, seg_min_steps, seg_len)."""
+        ch = np.zeros(4, np.int32)
+        self._chk(self.L.ipgpu_autotune_render(self.h, C.byref(image), IP(ch)))
+        return dict(trace_budget=int(ch[0]), trace_phases=int(ch[1]), seg_min_steps=int(ch[2]), seg_len=int(ch[3]))
 
     def set_batch_options(self, slot=-1, queue=-1, queue_waves=-1, rerender_graph=-1):
         """Batch launch options; -1 keeps the current value."""

@@ -184,3 +184,30 @@ def test_streamed_context_reuse(case):
         same(ref_ctx.render(im), got)
     c.close()
     ref_ctx.close()
+
+
+def test_autotune_render(case):
+    """ipgpu_autotune_render keeps a measured setting from its candidates; the
+    image after tuning agrees with the default setting's to the roundoff of
+    the segment regrouping, step counts and flags identical, and with the
+    oracle at the parity bar."""
+    h, prims, orc = case
+    say("case autotune render 48x48 th=163")
+    c, g, m = fresh(h, prims, trace_budget=512, trace_phases=2, seg_min_steps=512, seg_len=128)
+    try:
+        im, scale, _ = G.make_image(g, m.L_unit, nx=48, ny=48, thetacam=163.)
+        ref = c.render(im)
+        ch = c.autotune_render(im)
+        say("  chosen %s" % ch)
+        assert ch["seg_len"] in (128, 256, 512) and ch["trace_phases"] == 2
+        assert ch["trace_budget"] in (384, 512, 640)
+        assert ch["seg_min_steps"] in (256, 512, ch["trace_budget"])
+        got = c.render(im)
+        assert np.array_equal(ref["nstep"], got["nstep"]) and np.array_equal(ref["flag"], got["flag"])
+        e = np.abs(got["imageS"][..., :4] - ref["imageS"][..., :4]).max() / np.abs(ref["imageS"][..., 0]).max()
+        assert e < 1e-12, e
+    finally:
+        c.close()
+    orc.activate()
+    p = O.default_params(nx=48, ny=48, thetacam=163., freqcgs=230e9)
+    check(orc.render(p), got, scale)
