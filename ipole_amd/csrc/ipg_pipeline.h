@@ -844,24 +844,11 @@ struct SegChain {
 };
 
 /* Applies one segment's map A (SEGCOL x NPAR doubles, column-major, the
- * last column the unpolarized map) with oddflag bits fl. */
+ * last column the unpolarized map) with oddflag bits fl.  The unpolarized
+ * part comes first: after the N map, its branch kept the map's loads and
+ * the products live across it (a 450-byte spill per lane). */
 IPG_HD_INLINE void compose_segment(SegChain &c, const double *A, int fl)
 {
-  double x[NPAR], acc[NPAR];
-#pragma unroll
-  for (int b = 0; b < NPAR; b++) x[b] = herm_par(c.N, b);
-#pragma unroll
-  for (int p = 0; p < NPAR; p++) acc[p] = A[NPAR * NPAR + p];
-  /* column by column (acc[p] sums in the order b = 0..15), one column's
-   * loads at a time so that the map is never held in registers whole */
-#pragma unroll
-  for (int b = 0; b < NPAR; b++) {
-#pragma unroll
-    for (int p = 0; p < NPAR; p++) acc[p] += A[b * NPAR + p] * x[b];
-    IPG_SCHED_BARRIER();
-  }
-#pragma unroll
-  for (int p = 0; p < NPAR; p++) herm_par(c.N, p) = acc[p];
   const double *u = A + SEG_I * NPAR;
   const int nr = (int)u[SI_NRAD];
   if (nr > 0) {
@@ -877,6 +864,21 @@ IPG_HD_INLINE void compose_segment(SegChain &c, const double *A, int fl)
   c.tauF += u[SI_TAUF];
   c.nrad += nr;
   c.flag |= fl;
+  double x[NPAR], acc[NPAR];
+#pragma unroll
+  for (int b = 0; b < NPAR; b++) x[b] = herm_par(c.N, b);
+#pragma unroll
+  for (int p = 0; p < NPAR; p++) acc[p] = A[NPAR * NPAR + p];
+  /* column by column (acc[p] sums in the order b = 0..15), one column's
+   * loads at a time so that the map is never held in registers whole */
+#pragma unroll
+  for (int b = 0; b < NPAR; b++) {
+#pragma unroll
+    for (int p = 0; p < NPAR; p++) acc[p] += A[b * NPAR + p] * x[b];
+    IPG_SCHED_BARRIER();
+  }
+#pragma unroll
+  for (int p = 0; p < NPAR; p++) herm_par(c.N, p) = acc[p];
 }
 
 /* A long ray from its segment maps (segmap: SEGCOL x NPAR doubles per
