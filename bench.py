@@ -339,9 +339,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain device-to-device copy of MIB MiB (the reference "
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
-    ap.add_argument("--workload", choices=["c1", "batch", "extras"], default="c1",
+    ap.add_argument("--workload", choices=["c1", "batch", "extras", "adjoint", "precision"], default="c1",
                     help="c1: one 160x160 image (default); batch: the 64-job batch against the "
-                         "same jobs rendered one by one")
+                         "same jobs rendered one by one; adjoint, precision: those figures alone; "
+                         "extras: both")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this path is data-parallel over images, and no "
@@ -363,6 +364,12 @@ def main():
         return bench_batch(args)
     if args.workload == "extras":
         print(json.dumps(dict(precision=precision_measure(args), adjoint=adjoint_measure(args))), flush=True)
+        return
+    if args.workload == "adjoint":
+        print(json.dumps(dict(adjoint=adjoint_measure(args))), flush=True)
+        return
+    if args.workload == "precision":
+        print(json.dumps(dict(precision=precision_measure(args))), flush=True)
         return
 
     from ipole_amd import ipgpu as G
@@ -416,7 +423,7 @@ def main():
         ns_img.shape[0] // 8, 8, ns_img.shape[1] // 8, 8).max(axis=(1, 3))
     log("nstep: max %d, mean %.1f, p99 %d; sum over 8x8 tiles of the tile max %d" %
         (ns_img.max(), ns_img.mean(), np.percentile(ns_img, 99), tiles.sum()))
-    for T in (1024, 2048, 4096):
+    for T in sorted({1024, 2048, 4096, tuned["seg_min_steps"]}):
         sel = ns_img > T
         log("  rays > %d steps: %d, their steps %d (%.1f%%)" %
             (T, sel.sum(), ns_img[sel].sum(), 100. * ns_img[sel].sum() / ns_img.sum()))

@@ -34,7 +34,7 @@ def count(geom, model, image, dump, pix):
     L = C.CDLL(os.path.join(HERE, "libopcount.so"))
     L.opcount_render.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), G.dp, G.ip,
                                  C.c_int, G.dp, G.dp]
-    out = np.zeros(10)
+    out = np.zeros(12)
     pix = np.ascontiguousarray(pix, np.int32)
     per_ray = np.zeros((len(pix), 3))
     rc = L.opcount_render(C.byref(geom), C.byref(model), C.byref(image),
@@ -75,6 +75,7 @@ def main():
             F_fused_per_step=o[9] / n_step,
             F_fused_per_radiating_step=f_rad, F_fused_per_nonradiating_step=f_non, F_fused_per_ray=f_ray,
             F_fused_fit_rel_err=fit_err,
+            F_seg_per_column_step=o[10] / (17 * n_step), F_seg_intensity_per_step=o[11] / n_step,
         )
         print(("unpol" if unpol else "pol"), json.dumps(res["unpol" if unpol else "pol"]), flush=True)
     out = dict(
@@ -93,7 +94,10 @@ def main():
                                               "the rays) as radiating steps x this + non-radiating steps x "
                                               "F_fused_per_nonradiating_step + F_fused_per_ray; "
                                               "F_fused_fit_rel_err is the fit's summed |residual| over the "
-                                              "summed count"),
+                                              "summed count",
+                   F_seg_per_column_step="FLOP per step of one of the 17 N-map columns of k_seg "
+                                         "(segment_column), 384-step segments",
+                   F_seg_intensity_per_step="FLOP per step of k_seg's unpolarized column (segment_intensity)"),
         **res)
     path = os.path.join(ROOT, "ipole_amd", "flops_per_step.json")
     with open(path, "w") as f:

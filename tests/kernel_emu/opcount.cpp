@@ -91,7 +91,9 @@ extern "C" {
  * steps, radiating steps, flops of trace_ray, of prep_step on radiating
  * steps, of prep_step on non-radiating steps, of chain_ray, of
  * far_end_coefficients, number of rays, of transfer_fused (the batch's
- * FP64 fused transfer of the same ray).  per_ray (optional, 3 per pixel):
+ * FP64 fused transfer of the same ray), of segment_column over the ray's
+ * 384-step segments (17 columns), of segment_intensity over them.
+ * per_ray (optional, 3 per pixel):
  * the pixel's transfer steps, radiating steps and transfer_fused flops. */
 int opcount_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, const double *dump,
                    const int *pix, int npix, double *out, double *per_ray)
@@ -102,7 +104,7 @@ int opcount_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image 
   const int stride = pol ? PK : PK_UNPOL;
   std::vector<Cd> pkt;
   const Cd *d = reinterpret_cast<const Cd *>(dump);
-  for (int q = 0; q < 10; q++) out[q] = 0.;
+  for (int q = 0; q < 12; q++) out[q] = 0.;
   for (int n = 0; n < npix; n++) {
     int i = pix[2 * n], j = pix[2 * n + 1];
     int *ntp = P->target_nturns >= 0 ? nt.data() : nullptr;
@@ -139,11 +141,33 @@ int opcount_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image 
     Cd f7[7];
     int nrad_f;
     transfer_fused<false>(*G, *M, *P, ns, rec.data(), ntp, d, unit_scale(), f7, nrad_f);
-    out[9] += g_ops.flops() - e.flops();
+    const double f_fused = g_ops.flops() - e.flops();
+    out[9] += f_fused;
+    if (pol) {
+      const int nseg = (ns + SEG_LEN - 1) / SEG_LEN;
+      Ops g = g_ops;
+      for (int sg = 0; sg < nseg; sg++) {
+        int q0, q1;
+        segment_bounds(ns, SEG_LEN, sg, q0, q1);
+        for (int b = 0; b <= NPAR; b++) {
+          Herm N;
+          segment_column<1>(pkt.data(), q0, q1, b, N);
+        }
+      }
+      out[10] += g_ops.flops() - g.flops();
+      Ops h = g_ops;
+      for (int sg = 0; sg < nseg; sg++) {
+        int q0, q1;
+        segment_bounds(ns, SEG_LEN, sg, q0, q1);
+        Cd u[NPAR];
+        segment_intensity<1>(pkt.data(), q0, q1, u);
+      }
+      out[11] += g_ops.flops() - h.flops();
+    }
     if (per_ray) {
       per_ray[3 * n] = ns;
       per_ray[3 * n + 1] = nrad_f;
-      per_ray[3 * n + 2] = g_ops.flops() - e.flops();
+      per_ray[3 * n + 2] = f_fused;
     }
   }
   return 0;
