@@ -90,9 +90,11 @@ typedef struct ipgpu_image {
   int subring_dtheta;
   int counterjet;
   int emission_type;         /* 4 (Dexter thermal, default) or 1 (Pandya thermal) */
-  int transfer_fp32;         /* 1: transfer coefficients and Stokes-step exponentials in
-                                FP32 (geodesics, connection, tetrads, N in FP64); no
-                                counterpart in the reference, which is FP64 throughout */
+  int transfer_fp32;         /* 1: the coefficients' shape functions and the plasma-frame
+                                Stokes step (rotation, absorption/emission solution, its
+                                matrix and source terms) in FP32; geodesics, connection,
+                                tetrads and N in FP64.  No counterpart in the reference,
+                                which is FP64 throughout */
 } ipgpu_image;
 
 typedef struct ipgpu_ctx ipgpu_ctx;
@@ -258,8 +260,9 @@ int ipgpu_set_batch_options(ipgpu_ctx *ctx, int slot, int queue, int queue_waves
 int ipgpu_autotune_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs, int chosen[4]);
 
 /* Device memory a single-image render (ipgpu_render) may use for its ray
- * records and step packets (default 64 GiB).  An image whose rays need more
- * is rendered in chunks of rays (whole 8x8 pixel tiles) that fit; the image
+ * records and step packets (default 60% of the device's memory, and never
+ * more than the device has free).  An image whose rays need more is
+ * rendered in chunks of rays (whole 8x8 pixel tiles) that fit; the image
  * does not depend on the chunking.  The reference needs one trajectory of
  * maxnstep records per thread (src/main.c:835); this bounds the GPU's
  * equivalent, which holds the records of every ray of a chunk at once. */

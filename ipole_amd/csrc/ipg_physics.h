@@ -1374,87 +1374,117 @@ IPG_HD void stokes_affine(const Coef &c, double x, double T[4][4], double cv[4])
   }
 }
 
-/* stokes_affine with its exponentials and the rotation's sine and cosine
- * in FP32 (the matrices themselves, and everything they act on, in FP64). */
+/* (1 - e^-d) / d from em = e^-d - 1, with the thin-step series of
+ * approximate_solve below d = 1e-3 (src/ipolarray.c:616-620) */
+IPG_HD float one_minus_exp_over(float d, float em)
+{
+  return d < 1.e-3f ? 1.f - (d * 0.5f) * (1.f - d * (1.f / 3.f)) : -em / d;
+}
+
+/* stokes_affine in FP32: the Faraday rotation, the absorption/emission
+ * solution and their products (T and cv) are formed in single precision
+ * from dimensionless inputs.  The coefficients are turned into optical
+ * depths and rotation angles of the step (a x, rho x: formed in FP64, then
+ * rounded) and unit vectors, the emission into j x scaled by a power of two
+ * 2^-e that brings its largest component near 1 (j / nu^2 x spans
+ * ~1e-40..1e-20 in these units, outside FP32's normal range); sums and
+ * differences of optical depths (tau_I -+ tau_P) are taken in FP64 before
+ * rounding, and the source terms are written with (1 - e^-tau) / tau so that
+ * no product of two small depths (which would underflow) is formed.  cv is
+ * scaled back by 2^e exactly.  The branch tests (rho^2, aP^2 against 1e-80; the thin-step
+ * switch at 1e-3) are the reference's, on the FP64 coefficients. */
 IPG_HD void stokes_affine_f32(const Coef &c, double x, double T[4][4], double cv[4])
 {
-  double R3[3][3];
+  /* Faraday half step: angle vector rho x / 2 */
+  float R3[3][3];
   {
-    const double r[3] = {c.rQ, c.rU, c.rV};
-    const double rho2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-    const double X[3][3] = {{0., -r[2], r[1]}, {r[2], 0., -r[0]}, {-r[1], r[0], 0.}};
+    const double rho2 = c.rQ * c.rQ + c.rU * c.rU + c.rV * c.rV;
     const double hx = 0.5 * x;
-    if (rho2 > 1e-80) {
-      double rho = sqrt(rho2);
-      float shf, chf;
-      sincosf((float)(0.5 * rho * hx), &shf, &chf);
-      double sh = shf, ch = chf;
-      double s = 2. * sh * ch, cc = 1. - 2. * sh * sh;
-      double w = 2. * sh * sh / rho2, v = s / rho;
+    const float t[3] = {(float)(c.rQ * hx), (float)(c.rU * hx), (float)(c.rV * hx)};
+    const float X[3][3] = {{0.f, -t[2], t[1]}, {t[2], 0.f, -t[0]}, {-t[1], t[0], 0.f}};
+    const float th = (float)(sqrt(rho2) * hx);     /* rotation angle of the half step */
+    if (rho2 > 1e-80 && th > 1e-18f) {
+      const float ith = 1.f / th;
+      float sh, ch;
+      sincosf(0.5f * th, &sh, &ch);
+      const float sn = 2.f * sh * ch, cc = 1.f - 2.f * sh * sh;
+      const float q = sh * ith;                      /* w = (1 - cos th) / th^2 = 2 q^2 */
+      const float w = 2.f * q * q, v = sn * ith;
       for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) R3[i][j] = (i == j ? cc : 0.) + w * r[i] * r[j] + v * X[i][j];
+        for (int k = 0; k < 3; k++) R3[i][k] = (i == k ? cc : 0.f) + w * t[i] * t[k] + v * X[i][k];
     } else {
+      /* the reference's small-rho form (below 1e-18 rad the rotation is this
+       * to FP32's precision) */
       for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) R3[i][j] = (i == j ? 1. : 0.) + hx * X[i][j];
+        for (int k = 0; k < 3; k++) R3[i][k] = (i == k ? 1.f : 0.f) + X[i][k];
     }
   }
-  double A[4][4], b[4];
+  /* emission j x, scaled */
+  const double jx[4] = {c.jI * x, c.jQ * x, c.jU * x, c.jV * x};
+  double jmax = fmax(fmax(fabs(jx[0]), fabs(jx[1])), fmax(fabs(jx[2]), fabs(jx[3])));
+  int e = 0;
+  if (jmax > 0. && jmax < 1e300) (void)frexp(jmax, &e);
+  const double down = ldexp(1., -e), up = ldexp(1., e);
+  const float jn[4] = {(float)(jx[0] * down), (float)(jx[1] * down), (float)(jx[2] * down), (float)(jx[3] * down)};
+  float A[4][4], b[4];
+  for (int p = 0; p < 4; p++)
+    for (int q = 0; q < 4; q++) A[p][q] = 0.f;
   {
-    const double j[4] = {c.jI, c.jQ, c.jU, c.jV};
-    const double a[3] = {c.aQ, c.aU, c.aV};
-    const double aI = c.aI;
-    const double aP2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
-    for (int p = 0; p < 4; p++)
-      for (int q = 0; q < 4; q++) A[p][q] = 0.;
+    const double aP2 = c.aQ * c.aQ + c.aU * c.aU + c.aV * c.aV;
     if (aP2 > 1e-80) {
-      double aP = sqrt(aP2);
-      double tauP = aP * x, tauI = aI * x;
-      /* e^(-tauI -+ tauP) - 1 and e^(-tauI) - 1 in FP32 (expm1, so that thin
+      const double aP = sqrt(aP2);
+      const float tI = (float)(c.aI * x);
+      const float dm = (float)((c.aI - aP) * x);      /* tau_I - tau_P */
+      const float n[3] = {(float)(c.aQ / aP), (float)(c.aU / aP), (float)(c.aV / aP)};
+      /* e^(-tau_I -+ tau_P) - 1 and e^(-tau_I) - 1 (expm1, so that thin
        * steps keep their relative accuracy) */
-      float emm = expm1f((float)(-tauI + tauP)), emp = expm1f((float)(-tauI - tauP)), em = expm1f((float)(-tauI));
-      double efacm = 1. + (double)emm, efacp = 1. + (double)emp, efac = 1. + (double)em;
-      double ch = 0.5 * (efacm + efacp), shp = (efacp - efacm) / (2. * aP);
-      double iaP2 = 1. / aP2;
+      const float tS = (float)((c.aI + aP) * x);      /* tau_I + tau_P */
+      const float emm = expm1f(-dm), emp = expm1f(-tS), em = expm1f(-tI);
+      const float efacm = 1.f + emm, efacp = 1.f + emp, efac = 1.f + em;
+      const float ch = 0.5f * (efacm + efacp), sh = 0.5f * (efacp - efacm);
       A[0][0] = ch;
       for (int k = 0; k < 3; k++) {
-        A[0][k + 1] = shp * a[k];
-        A[k + 1][0] = shp * a[k];
-        for (int l = 0; l < 3; l++) A[k + 1][l + 1] = (k == l ? efac : 0.) + a[k] * a[l] * (ch - efac) * iaP2;
+        A[0][k + 1] = sh * n[k];
+        A[k + 1][0] = sh * n[k];
+        for (int l = 0; l < 3; l++) A[k + 1][l + 1] = (k == l ? efac : 0.f) + n[k] * n[l] * (ch - efac);
       }
-      double afacm = -(double)emm, afac = -(double)em, afacp = -(double)emp;
-      double adj = a[0] * j[1] + a[1] * j[2] + a[2] * j[3];
-      double m = afacm * (aP * j[0] - adj) / (2. * aP * (aI - aP));
-      double p = afacp * (aP * j[0] + adj) / (2. * aP * (aI + aP));
-      b[0] = m + p;
-      for (int k = 0; k < 3; k++) b[k + 1] = afac * (j[k + 1] / aI - a[k] * adj / (aI * aP2)) + (p - m) * a[k] / aP;
+      /* the source terms as (1 - e^-tau) / tau times scaled emission, so
+       * that no product of two small optical depths is formed */
+      const float nj = n[0] * jn[1] + n[1] * jn[2] + n[2] * jn[3];
+      const float m = 0.5f * one_minus_exp_over(dm, emm) * (jn[0] - nj);
+      const float pp = 0.5f * one_minus_exp_over(tS, emp) * (jn[0] + nj);
+      const float pI = one_minus_exp_over(tI, em);
+      b[0] = m + pp;
+      for (int k = 0; k < 3; k++) b[k + 1] = pI * (jn[k + 1] - n[k] * nj) + (pp - m) * n[k];
     } else {
-      double dtau = x * aI;
-      double em = (double)expm1f((float)(-dtau));
+      const double dtau = x * c.aI;
+      const float em = expm1f((float)(-dtau));
+      const float g = one_minus_exp_over((float)dtau, em);
       for (int k = 0; k < 4; k++) {
-        A[k][k] = 1. + em;
-        b[k] = (dtau < 1.e-3) ? j[k] * x * (1. - (dtau * 0.5) * (1. - dtau * (1. / 3.))) : (j[k] / aI) * (-em);
+        A[k][k] = 1.f + em;
+        b[k] = jn[k] * g;
       }
     }
   }
-  double R[4][4];
+  float R[4][4];
   for (int i = 0; i < 4; i++)
-    for (int k = 0; k < 4; k++) R[i][k] = (i == 0 || k == 0) ? (i == k ? 1. : 0.) : R3[i - 1][k - 1];
-  double RA[4][4];
+    for (int k = 0; k < 4; k++) R[i][k] = (i == 0 || k == 0) ? (i == k ? 1.f : 0.f) : R3[i - 1][k - 1];
+  float RA[4][4];
   for (int i = 0; i < 4; i++)
     for (int k = 0; k < 4; k++) {
-      double s = 0.;
+      float s = 0.f;
       for (int l = 0; l < 4; l++) s += R[i][l] * A[l][k];
       RA[i][k] = s;
     }
   for (int i = 0; i < 4; i++) {
     for (int k = 0; k < 4; k++) {
-      double s = 0.;
+      float s = 0.f;
       for (int l = 0; l < 4; l++) s += RA[i][l] * R[l][k];
       T[i][k] = s;
     }
-    double s = 0.;
+    float s = 0.f;
     for (int l = 0; l < 4; l++) s += R[i][l] * b[l];
-    cv[i] = s;
+    cv[i] = (double)s * up;
   }
 }
 

@@ -1756,6 +1756,13 @@ int ipgpu_create(ipgpu_ctx **out, int device)
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
     ctx->ncu = ncu;
+  {
+    /* single-image render memory: 60% of the device's (173 GB of an
+     * MI355X's 288), so that the benchmarked images run whole */
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) == hipSuccess && totalb > ((size_t)4 << 30))
+      ctx->render_bytes = (size_t)(0.6 * (double)totalb);
+  }
   for (int f = 0; f < 2; f++) {
     int nb = 0;
     hipError_t e = f ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_xfer_queue<true>, WAVE, 0)
@@ -2496,7 +2503,17 @@ static long long chunk_rays(const ipgpu_ctx *ctx, const ipgpu_image &P)
   const double pkt = ctx->pkt_bytes_per_ray > 0. ? 1.1 * ctx->pkt_bytes_per_ray
                                                  : ctx->avg_steps_hint * stride * sizeof(double) * 1.25;
   const double per_ray = (double)cap * REC * sizeof(double) + pkt + 512.;
-  const long long chunk = (long long)(0.85 * (double)ctx->render_bytes / per_ray) / WAVE * WAVE;
+  /* the render memory, or what the device can give the render's arenas
+   * (free memory plus what they hold now, less 4 GiB), whichever is smaller */
+  double budget = (double)ctx->render_bytes;
+  size_t freeb = 0, totalb = 0;
+  if (hipMemGetInfo(&freeb, &totalb) == hipSuccess) {
+    const double held = (double)(ctx->traj_doubles + ctx->pkt_doubles + ctx->pkt2_doubles + ctx->ppkt_doubles
+                                 + ctx->trajO_doubles) * sizeof(double);
+    const double avail = (double)freeb + held - (double)((size_t)4 << 30);
+    if (avail < budget) budget = avail > 0. ? avail : 0.;
+  }
+  const long long chunk = (long long)(0.85 * budget / per_ray) / WAVE * WAVE;
   return chunk < WAVE ? WAVE : chunk;
 }
 

@@ -97,6 +97,19 @@ void emu_stokes_affine(const double *c11, double x, const double *S_in, double *
     S_out[i] = cv[i] + T[i][0] * S_in[0] + T[i][1] * S_in[1] + T[i][2] * S_in[2] + T[i][3] * S_in[3];
 }
 
+/* The same with the mixed-precision step (stokes_affine_f32). */
+void emu_stokes_affine_f32(const double *c11, double x, const double *S_in, double *S_out)
+{
+  Coef c;
+  c.jI = c11[0]; c.jQ = c11[1]; c.jU = c11[2]; c.jV = c11[3];
+  c.aI = c11[4]; c.aQ = c11[5]; c.aU = c11[6]; c.aV = c11[7];
+  c.rQ = c11[8]; c.rU = c11[9]; c.rV = c11[10];
+  double T[4][4], cv[4];
+  stokes_affine_f32(c, x, T, cv);
+  for (int i = 0; i < 4; i++)
+    S_out[i] = cv[i] + T[i][0] * S_in[0] + T[i][1] * S_in[1] + T[i][2] * S_in[2] + T[i][3] * S_in[3];
+}
+
 int emu_plasma_tetrad(const ipgpu_geom *G, const double *X, const double *U, const double *K,
                       const double *B, double *E16)
 {

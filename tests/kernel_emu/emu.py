@@ -25,6 +25,7 @@ def lib():
         L.emu_point.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), dp, C.c_int, dp, dp, dp]
         L.emu_math.argtypes = [C.c_int, dp, C.c_int, dp]
         L.emu_stokes_affine.argtypes = [dp, C.c_double, dp, dp]
+        L.emu_stokes_affine_f32.argtypes = [dp, C.c_double, dp, dp]
         L.emu_plasma_tetrad.argtypes = [C.POINTER(G.Geom), dp, dp, dp, dp, dp]
         L.emu_render.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), dp, ip,
                                  C.c_int, dp, ip, ip]
@@ -89,6 +90,12 @@ def math(kind, x):
 def stokes_affine(c11, x, S):
     out = np.zeros(4)
     lib().emu_stokes_affine(G.P(f64(c11)), x, G.P(f64(S)), G.P(out))
+    return out
+
+
+def stokes_affine_f32(c11, x, S):
+    out = np.zeros(4)
+    lib().emu_stokes_affine_f32(G.P(f64(c11)), x, G.P(f64(S)), G.P(out))
     return out
 
 

@@ -1,8 +1,10 @@
-"""The mixed-precision transfer (ipgpu_image.transfer_fp32: coefficients'
-shape functions and the Stokes step's exponentials in FP32; geodesics,
+"""The mixed-precision transfer (ipgpu_image.transfer_fp32: the coefficients'
+shape functions and the plasma-frame Stokes step in FP32; geodesics,
 connection, tetrads and N in FP64) on the GPU, single image and batch,
-against the FP64 oracle: image Ftot, LP, CP and EVPA within 1e-5 (the mode's
-bar is |dFtot|/Ftot <= 1e-4; measured ~1e-7)."""
+against the FP64 oracle: image Ftot (polarized and unpolarized) within 1e-5
+(the mode's bar is |dFtot|/Ftot <= 1e-4; measured ~4e-7), LP, CP and EVPA
+within the mode's 1e-4 (measured up to ~1e-5 rad in EVPA, where the FP32
+Faraday rotation's rounding accumulates over the ray)."""
 import sys
 
 import numpy as np
@@ -57,4 +59,4 @@ def test_fp32_single_and_batch(setup, thetacam, etype):
         e = errors(ref, got)
         print("  %s %s" % (name, e), flush=True)
         for k, v in e.items():
-            assert v < 1e-5, (name, k, v)
+            assert v < (1e-5 if k in ("Ftot", "Funpol") else 1e-4), (name, k, v)

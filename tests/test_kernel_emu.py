@@ -106,6 +106,50 @@ def test_stokes_affine_vs_oracle_step():
     assert n == 3 * 3 * 6 * 6
 
 
+def test_stokes_affine_f32_vs_oracle_step():
+    """The mixed-precision step (stokes_affine_f32: Faraday rotation,
+    absorption/emission solution and their products in FP32 from optical
+    depths, angles and scaled emission) against the oracle's FP64 sequential
+    step, over the branches of test_stokes_affine_vs_oracle_step and at the
+    coefficient scales of a dump in cgs units (j ~ 1e-30..1e-40, alpha and
+    rho ~ 1e-16..1e-12 per cm, steps of 1e9..1e14 cm, including coefficients
+    just above the 1e-80 thresholds): within a few FP32 ulps of the state's
+    scale."""
+    rng = np.random.default_rng(12)
+    n = 0
+    worst = 0.
+    cases = []
+    for rot in (0., 1e-45, 1.):
+        for apol in (0., 1e-45, 0.5):
+            for x in (1e-9, 3e-6, 2e-4, 0.05, 1.5, 30.):
+                cases.append((rot, apol, x, 1., 1.))
+    for js in (1e-30, 1e-40):
+        for ascale in (1e-16, 1e-12, 1.5e-40):
+            for x in (1e9, 1e12, 1e14):
+                cases.append((1., 0.5, x, js, ascale))
+    for rot, apol, x, js, ascale in cases:
+        for _ in range(6):
+            c = np.zeros(11)
+            c[4] = rng.uniform(0.3, 2.) * ascale
+            a = rng.normal(size=3)
+            c[5:8] = a / np.linalg.norm(a) * apol * c[4] * rng.uniform(0.1, 0.95)
+            c[0] = rng.uniform(0.1, 1.) * js
+            jp = rng.normal(size=3)
+            c[1:4] = jp / np.linalg.norm(jp) * 0.6 * c[0]
+            c[8:11] = rng.normal(size=3) * rot * ascale
+            S0 = rng.normal(size=4) * js * x
+            S0[0] = abs(S0[0]) + 2. * js * x
+            ref = O.stokes_step(c, x, S0)
+            got = emu.stokes_affine_f32(c, x, S0)
+            scale = max(np.abs(ref).max(), np.abs(S0).max(), 1e-300)
+            err = np.abs(got - ref).max() / scale
+            assert np.all(np.isfinite(got)) and err <= 2e-6, (rot, apol, x, js, ascale, got, ref)
+            worst = max(worst, err)
+            n += 1
+    assert n == 6 * len(cases) and worst > 0.
+    print("max error / scale %.2e over %d steps" % (worst, n))
+
+
 def image_errors(ref, out7, f3, scale, unpol=False):
     I_ref = ref["image"].ravel()
     I_got = out7[:, 0] * f3
@@ -283,7 +327,7 @@ def image_summary(I, Q, U, V):
 
 @pytest.mark.parametrize("thetacam,etype", [(163., 4), (60., 1)])
 def test_emu_fp32_transfer(thetacam, etype):
-    """FP32 transfer coefficients and Stokes-step exponentials (FP64
+    """FP32 coefficient shape functions and plasma-frame Stokes step (FP64
     geodesics, connection, tetrads and N), split and fused forms, against
     the FP64 oracle: Ftot, LP, CP and EVPA well inside the 1e-4 bar of the
     mixed-precision mode (measured ~1e-7 here)."""
