@@ -978,7 +978,9 @@ __global__ __launch_bounds__(WAVE) void k_trace_resume_mask(ipgpu_geom G, ipgpu_
                                                             int nend, double *__restrict__ traj,
                                                             int *__restrict__ nstep_io, int *__restrict__ live,
                                                             int *__restrict__ host_ns, int *__restrict__ host_live,
-                                                            unsigned long long *__restrict__ counts)
+                                                            unsigned long long *__restrict__ counts,
+                                                            const long long *__restrict__ rbase,
+                                                            double *__restrict__ trajO, int ocap)
 {
   IPG_LATENCY_CRITICAL();
   /* one lane pair per ray */
@@ -992,14 +994,66 @@ __global__ __launch_bounds__(WAVE) void k_trace_resume_mask(ipgpu_geom G, ipgpu_
   }
   const int n0 = nstep_io[ray];
   int fin = 1;
-  int nstep = trace_pair(G, P, 0, 0, cap, traj + (size_t)ray * cap * REC, n0, nend < 0 ? 0x7fffffff : nend, &fin,
-                         odd);
+  /* a ray with a pooled region (the unbounded phase) continues there, with
+   * the region's capacity; the others in their slots */
+  double *rec = traj + (size_t)ray * cap * REC;
+  int rcap = cap;
+  if (rbase) {
+    const long long b = rbase[ray];
+    if (b >= 0) {
+      rec = trajO + b;
+      rcap = ocap;
+    }
+  }
+  int nstep = trace_pair(G, P, 0, 0, rcap, rec, n0, nend < 0 ? 0x7fffffff : nend, &fin, odd);
   if (odd) return;
   nstep_io[ray] = nstep;
   live[ray] = fin ? 0 : 1;
   host_ns[ray] = nstep;
   host_live[ray] = fin ? 0 : 1;
   atomicAdd(&counts[0], (unsigned long long)(nstep - n0));
+}
+
+/* When the last bounded phase has ended: each ray still live takes, in
+ * order of arrival, one of nreg full-length regions of trajO (reg doubles
+ * each) for its records, or none once they are used up.  rbase of every ray
+ * is written here (region offset in doubles, or -1), and its host row;
+ * list[s] is the ray of region s. */
+__global__ void k_pool_assign(int nrays, const int *__restrict__ live, int nreg, long long reg,
+                              unsigned int *__restrict__ ctr, long long *__restrict__ rbase,
+                              long long *__restrict__ host_rbase, int *__restrict__ list)
+{
+  const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ray >= nrays) return;
+  long long b = -1;
+  if (live[ray]) {
+    const unsigned int s = atomicAdd(ctr, 1u);
+    if (s < (unsigned int)nreg) {
+      b = (long long)s * reg;
+      list[s] = ray;
+    }
+  }
+  rbase[ray] = b;
+  host_rbase[ray] = b;
+}
+
+/* One workgroup per assigned region: the ray's records so far (at most cap,
+ * its slot) copied from its slot in traj to its region, where the unbounded
+ * phase resumes it and writes the rest. */
+__global__ __launch_bounds__(256) void k_pool_copy(int nreg, const unsigned int *__restrict__ ctr,
+                                                   const int *__restrict__ list, const int *__restrict__ nstep,
+                                                   int cap, const double *__restrict__ traj,
+                                                   double *__restrict__ trajO, long long reg)
+{
+  const unsigned int used = min(*ctr, (unsigned int)nreg);
+  const unsigned int s = blockIdx.x;
+  if (s >= used) return;
+  const int ray = list[s];
+  const int nrec = min(nstep[ray] + 1, cap);
+  const long long n = (long long)nrec * REC / 2;      /* 16-byte pairs */
+  const double2 *src = reinterpret_cast<const double2 *>(traj + (size_t)ray * cap * REC);
+  double2 *dst = reinterpret_cast<double2 *>(trajO + (size_t)s * reg);
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
 /* The rays of a single image whose trace outgrew the record capacity (the
@@ -1551,6 +1605,17 @@ struct ipgpu_ctx {
   std::vector<int> h_ovl;
   std::vector<long long> h_ovoff;
   int last_max_steps = 0, last_overflow = 0;
+  /* the streamed render's record pool: each ray still live when the last
+   * bounded phase ends gets a full-length region of d_trajO, assigned on
+   * the device (k_pool_assign), so the per-ray slots need hold only the
+   * bounded phases' records */
+  int pool_on = 1;
+  int pool_max = -1;                /* at most this many regions (-1: no limit; 0: no pool) */
+  double live_frac = 1.0 / 16;      /* rays live after the last bounded phase, of the last render */
+  unsigned int *d_pool_ctr = nullptr;
+  int *d_pool_list = nullptr;
+  size_t cap_pool_list = 0, cap_pool_ctr = 0;
+  int last_pool_regions = 0, last_pool_used = 0;
   /* device memory a single-image render may use for its ray records and
    * step packets; larger images are rendered in chunks of rays */
   size_t render_bytes = (size_t)64 << 30;
@@ -1719,6 +1784,30 @@ static int grow(ipgpu_ctx *ctx, T *&ptr, size_t &cap, size_t need)
   return 0;
 }
 
+/* grow that keeps the first `keep` elements (copied on the stream, which
+ * must hold no other work on the buffer in flight). */
+template <typename T>
+static int grow_keep(ipgpu_ctx *ctx, T *&ptr, size_t &cap, size_t need, size_t keep, hipStream_t st)
+{
+  if (need <= cap && ptr) return 0;
+  if (!ptr || keep == 0) return grow(ctx, ptr, cap, need);
+  T *np = nullptr;
+  hipError_t e = hipMalloc((void **)&np, need * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    snprintf(ctx->err, sizeof(ctx->err), "device allocation of %zu bytes failed (%s); the context holds %zu bytes",
+             need * sizeof(T), hipGetErrorString(e), ctx->dev_bytes);
+    return e == hipErrorOutOfMemory ? IPGPU_ERR_NOMEM : IPGPU_ERR_HIP;
+  }
+  HIPCHK(ctx, hipMemcpyAsync(np, ptr, std::min(keep, cap) * sizeof(T), hipMemcpyDeviceToDevice, st));
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  HIPCHK(ctx, hipFree(ptr));
+  ctx->dev_bytes += (need - cap) * sizeof(T);
+  ptr = np;
+  cap = need;
+  return 0;
+}
+
 /* Device address of scratch on each GPU, looked up once per device with
  * that device current. */
 constexpr int MAX_DEVICES = 64;
@@ -1770,6 +1859,7 @@ int ipgpu_create(ipgpu_ctx **out, int device)
     if (e == hipSuccess && nb > 0) ctx->xq_occ[f] = nb;
   }
   if (const char *e = getenv("IPGPU_BATCH_QUEUE")) ctx->batch_queue = atoi(e);
+  if (const char *e = getenv("IPGPU_RECORD_POOL_MAX")) ctx->pool_max = atoi(e);
   if (const char *e = getenv("IPGPU_BATCH_QUEUE_WAVES")) ctx->batch_queue_waves = atoi(e);
   if (noalloc_setup(device, &ctx->d_scratch) != IPGPU_OK) {
     hipStreamDestroy(ctx->stream);
@@ -1829,7 +1919,8 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
                   ctx->d_out4, ctx->d_target, ctx->d_scale, ctx->d_trrec, ctx->d_trout, ctx->d_trnt,
                   ctx->d_rlist, ctx->d_rcount, ctx->d_iavg, ctx->d_interp, ctx->d_ppkt, ctx->d_plist, ctx->d_traj2, ctx->d_off2,
                   ctx->d_pidx, ctx->d_trajO, ctx->d_ovl, ctx->d_ovoff, ctx->d_rbase, ctx->d_pixlist,
-                  ctx->d_aint, ctx->d_amap, ctx->d_ag3, ctx->d_aN, ctx->d_aL, ctx->d_queue};
+                  ctx->d_aint, ctx->d_amap, ctx->d_ag3, ctx->d_aN, ctx->d_aL, ctx->d_queue, ctx->d_pool_ctr,
+                  ctx->d_pool_list};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int q = 0; q < NEV; q++)
@@ -2183,6 +2274,24 @@ static void sort_longest_first(int *list, int n, const std::vector<int> &ns, std
   for (int k = 0; k < n; k++) list[k] = tmp[k];
 }
 
+/* Full-length record regions for the rays the unbounded phase of a
+ * streamed render traces (0: no pool; the per-ray slots then follow the
+ * longest ray, as the one-pass render's do): the last render's share of
+ * such rays with a quarter more, at most a quarter of the render memory. */
+static int pool_regions(const ipgpu_ctx *ctx, const ipgpu_image &P, int nrays)
+{
+  if (!ctx->pool_on || ctx->pool_max == 0 || P.target_nturns >= 0 || ctx->trace_budget <= 0
+      || ctx->trace_budget + 1 >= P.maxnstep)
+    return 0;
+  const double reg_bytes = (double)(P.maxnstep + 1) * REC * sizeof(double);
+  const double want = (double)nrays * ctx->live_frac * 1.25 + 64.;
+  const double lim = 0.25 * (double)ctx->render_bytes / reg_bytes;
+  double n = want < lim ? want : lim;
+  if (n > nrays) n = nrays;
+  if (ctx->pool_max > 0 && n > ctx->pool_max) n = ctx->pool_max;
+  return n >= 1. ? (int)n : 0;
+}
+
 /* Streamed render of one image.  The geodesics are integrated in phases
  * with growing step budgets (trace_budget/4, /2, trace_budget, then to the
  * end), all queued at once on the main stream; each phase ends with a copy
@@ -2196,7 +2305,7 @@ static void sort_longest_first(int *list, int n, const std::vector<int> &ns, std
  * exactly).  A ray that outgrows the record capacity (cap records per ray)
  * is continued from its last stored record in a full-length region of its
  * own before its transfer. */
-static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int cap)
+static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int cap, int nreg)
 {
   int rc;
   const int stride = P.only_unpolarized ? PK_UNPOL : PK;
@@ -2232,7 +2341,8 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
    * each phase's step counts and live flags into it (a copy command would
    * hold the host until the phase ends) */
   const size_t nI = (size_t)nrays;
-  const size_t need = sizeof(int) * nI * (2 * (MAXPH) + 4) + 2 * sizeof(long long) * (nI + MAXPH + 1);
+  const size_t need = sizeof(int) * nI * (2 * (MAXPH) + 4) + 2 * sizeof(long long) * (nI + MAXPH + 1)
+      + sizeof(long long) * nI;
   if (ctx->h_pin_bytes < need) {
     if (ctx->h_pin) hipHostFree(ctx->h_pin);
     ctx->h_pin = nullptr;
@@ -2250,6 +2360,21 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
   int *h_base = h_list + nI;
   int *h_plist = h_base + nI, *h_pidx = h_plist + nI;
   int *dh_ns = (int *)(d_pin + ns_bytes), *dh_live = dh_ns + (size_t)MAXPH * nI;
+  /* pooled regions' offsets per ray, after the int rows (8-byte aligned) */
+  const size_t rb_bytes = ns_bytes + sizeof(int) * nI * (2 * (MAXPH) + 4);
+  const long long *h_rbase = (const long long *)((char *)ctx->h_pin + rb_bytes);
+  long long *dh_rbase = (long long *)(d_pin + rb_bytes);
+  /* the record pool of the unbounded phase */
+  const long long reg = (long long)(P.maxnstep + 1) * REC;   /* doubles per region */
+  const long long pool_doubles = (long long)nreg * reg;
+  if (nreg > 0) {
+    if ((rc = grow(ctx, ctx->d_trajO, ctx->trajO_doubles, (size_t)pool_doubles))
+        || (rc = grow(ctx, ctx->d_pool_list, ctx->cap_pool_list, (size_t)nreg))
+        || (rc = grow(ctx, ctx->d_pool_ctr, ctx->cap_pool_ctr, (size_t)1))
+        || (rc = grow(ctx, ctx->d_rbase, ctx->cap_rbase, nI)))
+      return rc;
+  }
+  ctx->last_pool_regions = nreg;
   /* the first Ppre steps (a multiple of the segment length, records all
    * stored by the bounded phases) of the rays the unbounded phase traces
    * are prepared and composed on stream streamF[K] while it runs */
@@ -2266,16 +2391,27 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
   /* all trace phases, queued at once; phase k writes the step counts (and
    * live flags) of the rays it integrated to the host arena's row k (the
    * live-flag row of the last, unbounded phase is scratch) */
+  if (nreg > 0) HIPCHK(ctx, hipMemsetAsync(ctx->d_pool_ctr, 0, sizeof(unsigned int), sA));
   for (int k = 0; k <= K; k++) {
+    const bool pooled = k == K && nreg > 0;
+    if (pooled) {
+      /* the rays still live get their regions, and their records so far */
+      k_pool_assign<<<(nrays + 255) / 256, 256, 0, sA>>>(nrays, ctx->d_live, nreg, reg, ctx->d_pool_ctr,
+                                                        ctx->d_rbase, dh_rbase, ctx->d_pool_list);
+      HIPCHK(ctx, hipGetLastError());
+      k_pool_copy<<<nreg, 256, 0, sA>>>(nreg, ctx->d_pool_ctr, ctx->d_pool_list, ctx->d_nstep_ray, cap,
+                                          ctx->d_traj, ctx->d_trajO, reg);
+      HIPCHK(ctx, hipGetLastError());
+    }
     if (k > 0) HIPCHK(ctx, hipEventRecord(ev[2 * k], sA));
     int *hn = dh_ns + (size_t)k * nI, *hl = dh_live + (size_t)(k < K ? k : MAXPH - 1) * nI;
     if (k == 0)
       k_trace_budget<<<2 * nrays / WAVE, WAVE, 0, sA>>>(ctx->geom, P, ctx->pix, nrays, cap, bud[0], ctx->d_traj,
                                                          ctx->d_nstep_ray, ctx->d_live, hn, hl, ctx->d_counts);
     else
-      k_trace_resume_mask<<<2 * nrays / WAVE, WAVE, 0, sA>>>(ctx->geom, P, nrays, cap, k < K ? bud[k] : -1,
-                                                          ctx->d_traj, ctx->d_nstep_ray, ctx->d_live, hn, hl,
-                                                          ctx->d_counts);
+      k_trace_resume_mask<<<2 * nrays / WAVE, WAVE, 0, sA>>>(
+          ctx->geom, P, nrays, cap, k < K ? bud[k] : -1, ctx->d_traj, ctx->d_nstep_ray, ctx->d_live, hn, hl,
+          ctx->d_counts, pooled ? ctx->d_rbase : nullptr, pooled ? ctx->d_trajO : nullptr, P.maxnstep + 1);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(ev[2 * k + 1], sA));
   }
@@ -2302,30 +2438,39 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
     }
     const long long *rbase = nullptr;
     if (k == K) {
-      /* rays past the record capacity: their first cap records move to
-       * full-length regions, where their traces continue */
+      /* rays past the record capacity without a pooled region: their first
+       * cap records move to full-length regions (after the pool's), where
+       * their traces continue */
       std::vector<int> &ov = ctx->h_ovl;
       std::vector<long long> &oo = ctx->h_ovoff;
       ov.clear();
-      oo.assign(1, 0);
+      oo.assign(1, pool_doubles / REC);
       int mx = 0;
       for (int q = 0; q < n; q++) {
         const int r = h_list[start + q], v = nsv[r];
         mx = v > mx ? v : mx;
-        if (v >= cap && v < P.maxnstep - 1) {
+        const bool in_pool = nreg > 0 && ((const volatile long long *)h_rbase)[r] >= 0;
+        if (v >= cap && v < P.maxnstep - 1 && !in_pool) {
           ov.push_back(r);
           oo.push_back(oo.back() + v + 1);
         }
       }
       ctx->last_max_steps = mx;
+      if (nreg > 0) {
+        /* the share of rays the unbounded phase traced sizes the next pool */
+        ctx->live_frac = std::max((double)n / nrays, 1.0 / 4096);
+        ctx->last_pool_used = std::min(n, nreg);
+        rbase = ctx->d_rbase;
+      }
       if (!ov.empty()) {
         const int no = (int)ov.size();
-        if ((rc = grow(ctx, ctx->d_trajO, ctx->trajO_doubles, (size_t)oo.back() * REC))
+        if ((rc = grow_keep(ctx, ctx->d_trajO, ctx->trajO_doubles, (size_t)oo.back() * REC, (size_t)pool_doubles,
+                            sA))
             || (rc = grow(ctx, ctx->d_ovl, ctx->cap_ovl, (size_t)no))
             || (rc = grow(ctx, ctx->d_ovoff, ctx->cap_ovoff, (size_t)no))
             || (rc = grow(ctx, ctx->d_rbase, ctx->cap_rbase, nI)))
           return rc;
-        HIPCHK(ctx, hipMemsetAsync(ctx->d_rbase, 0xff, nI * sizeof(long long), sA));
+        if (nreg == 0) HIPCHK(ctx, hipMemsetAsync(ctx->d_rbase, 0xff, nI * sizeof(long long), sA));
         HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), sA));
         HIPCHK(ctx, hipMemcpyAsync(ctx->d_ovl, ov.data(), no * sizeof(int), hipMemcpyHostToDevice, sA));
         HIPCHK(ctx, hipMemcpyAsync(ctx->d_ovoff, oo.data(), no * sizeof(long long), hipMemcpyHostToDevice, sA));
@@ -2494,7 +2639,14 @@ static int render_rays(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays);
  * packets fit the render memory. */
 static long long chunk_rays(const ipgpu_ctx *ctx, const ipgpu_image &P)
 {
-  const int cap = ctx->traj_cap_steps < P.maxnstep ? ctx->traj_cap_steps : P.maxnstep;
+  int cap = ctx->traj_cap_steps < P.maxnstep ? ctx->traj_cap_steps : P.maxnstep;
+  /* with the record pool: slots of the bounded phases' records, plus the
+   * pool's share of full-length regions per ray */
+  double pool_per_ray = 0.;
+  if (pool_regions(ctx, P, 1 << 20) > 0) {
+    cap = ctx->trace_budget + 1;
+    pool_per_ray = ctx->live_frac * 1.25 * (double)(P.maxnstep + 1) * REC * sizeof(double);
+  }
   const int stride = P.only_unpolarized ? PK_UNPOL : PK;
   /* per ray: cap records, packets of the mean step count (with the padding
    * of the interleaved layout and the prefix packets of the longest rays),
@@ -2502,7 +2654,7 @@ static long long chunk_rays(const ipgpu_ctx *ctx, const ipgpu_image &P)
    * buffers of the last phase's list and the segment maps */
   const double pkt = ctx->pkt_bytes_per_ray > 0. ? 1.1 * ctx->pkt_bytes_per_ray
                                                  : ctx->avg_steps_hint * stride * sizeof(double) * 1.25;
-  const double per_ray = (double)cap * REC * sizeof(double) + pkt + 512.;
+  const double per_ray = (double)cap * REC * sizeof(double) + pool_per_ray + pkt + 512.;
   /* the render memory, or what the device can give the render's arenas
    * (free memory plus what they hold now, less 4 GiB), whichever is smaller */
   double budget = (double)ctx->render_bytes;
@@ -2632,9 +2784,11 @@ static int render_rays(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays)
   if ((rc = grow(ctx, ctx->d_flag_ray, ctx->cap_flag_ray, (size_t)nrays))) return rc;
   if ((rc = grow(ctx, ctx->d_out7, ctx->cap_out7, (size_t)nrays * 7))) return rc;
   int cap = ctx->traj_cap_steps < P.maxnstep ? ctx->traj_cap_steps : P.maxnstep;
-  const bool two = P.target_nturns < 0 && ctx->trace_budget > 0 && cap > ctx->trace_budget + 1;
+  const int nreg = pool_regions(ctx, P, nrays);
+  if (nreg > 0) cap = ctx->trace_budget + 1;   /* slots for the bounded phases' records */
+  const bool two = P.target_nturns < 0 && ctx->trace_budget > 0 && (nreg > 0 || cap > ctx->trace_budget + 1);
   ctx->last_overflow = 0;
-  if (two && (rc = render_streamed(ctx, P, nrays, cap))) return rc;
+  if (two && (rc = render_streamed(ctx, P, nrays, cap, nreg))) return rc;
   const bool one = !two;
   if (one) {
     int stride;

@@ -78,29 +78,38 @@ def test_fullsize_sample_vs_oracle(full):
         assert mx < 1e-5 and frac <= 1e-2, (k, mx, frac)
 
 
-def test_fullsize_streamed_equals_one_pass(full):
+def test_fullsize_streamed_equals_one_pass(full, monkeypatch):
     """On a context of its own (so independent of the other tests): the
-    first render is streamed although the longest ray (11,267 steps) exceeds
-    the default record capacity of 8,192 (it continues in a region of its
-    own); it and the next render equal the one-pass render bit for bit."""
+    first render is streamed, its per-ray slots holding the bounded phases'
+    3,585 records and the 808 rays still live after them (up to 11,267
+    steps) taking pooled full-length regions, so nothing is retraced; it and
+    the next render equal the one-pass render bit for bit.  Without the pool
+    (IPGPU_RECORD_POOL_MAX=0) the longest rays exceed the default record
+    capacity of 8,192 on the first render and continue from their last
+    record in regions of their own, with the same result."""
     orc, p, c, im = full
     say("full-size streamed (fresh context) vs one-pass")
     h, prims = synth.make_dump(*N, a=A, seed=SEED)
-    c2 = G.Context(0)
-    try:
-        g, m = G.iharm_geom(h), G.iharm_model()
-        c2.set_geometry(g)
-        c2.set_model(m)
-        c2.upload_prims(prims, G.iharm_units(M_UNIT, trat_large=TRAT))
-        first = c2.render(im)
-        assert c2.stage_times_mode()[1] is True
-        chunks, cont, nbytes = c2.render_info()
-        say("  first render: %d chunk(s), %d continued rays, %.1f GB held" % (chunks, cont, nbytes / 1e9))
-        assert chunks == 1 and cont > 0
-        second = c2.render(im)
-        assert c2.stage_times_mode()[1] is True and c2.render_info()[1] == 0
-    finally:
-        c2.close()
+    renders = []
+    for pool_max in ("-1", "0"):
+        monkeypatch.setenv("IPGPU_RECORD_POOL_MAX", pool_max)
+        c2 = G.Context(0)
+        try:
+            g, m = G.iharm_geom(h), G.iharm_model()
+            c2.set_geometry(g)
+            c2.set_model(m)
+            c2.upload_prims(prims, G.iharm_units(M_UNIT, trat_large=TRAT))
+            first = c2.render(im)
+            assert c2.stage_times_mode()[1] is True
+            chunks, cont, nbytes = c2.render_info()
+            say("  pool max %s, first render: %d chunk(s), %d continued rays, %.1f GB held" %
+                (pool_max, chunks, cont, nbytes / 1e9))
+            assert chunks == 1 and (cont == 0 if pool_max == "-1" else cont > 0)
+            second = c2.render(im)
+            assert c2.stage_times_mode()[1] is True and c2.render_info()[1] == 0
+            renders += [first, second]
+        finally:
+            c2.close()
     c.set_tuning(trace_budget=0)
     try:
         one = c.render(im)
@@ -108,9 +117,9 @@ def test_fullsize_streamed_equals_one_pass(full):
     finally:
         c.set_tuning(trace_budget=3584)
     for k in ("image", "imageS", "taus", "nstep", "flag"):
-        assert np.array_equal(first[k], one[k]), k
-        assert np.array_equal(second[k], one[k]), k
-    assert int(first["nstep"].max()) > 10000
+        for r in renders:
+            assert np.array_equal(r[k], one[k]), k
+    assert int(one["nstep"].max()) > 10000
 
 
 def test_fullsize_chunked_equals_whole(full):

@@ -119,15 +119,17 @@ def test_streamed_vs_oracle(case):
                                   dict(trace_budget=128, trace_phases=2, seg_min_steps=100, seg_len=32,
                                        record_steps=200)],
                          ids=["one-phase", "prefix-composed"])
-def test_streamed_record_overflow(case, tune):
-    """Rays longer than the record capacity continue from their last stored
-    record in full-length regions of their own: the render stays streamed
-    and returns the one-pass result bit for bit, on a fresh context and on
-    the next render (whose capacity follows the longest ray, so nothing
-    continues).  The second tuning also forms the longest rays' first
-    segments during the unbounded phase."""
+def test_streamed_record_overflow(case, tune, monkeypatch):
+    """Without the record pool (IPGPU_RECORD_POOL_MAX=0), rays longer than
+    the record capacity continue from their last stored record in
+    full-length regions of their own: the render stays streamed and returns
+    the one-pass result bit for bit, on a fresh context and on the next
+    render (whose capacity follows the longest ray, so nothing continues).
+    The second tuning also forms the longest rays' first segments during the
+    unbounded phase."""
     h, prims, orc = case
     say("case streamed record overflow 24x24 %s" % tune)
+    monkeypatch.setenv("IPGPU_RECORD_POOL_MAX", "0")
     c1, g, m = fresh(h, prims, trace_budget=0, seg_min_steps=tune.get("seg_min_steps", 3584),
                      seg_len=tune.get("seg_len", 384))
     im, _, _ = G.make_image(g, m.L_unit, nx=24, ny=24, thetacam=163.)
@@ -145,6 +147,45 @@ def test_streamed_record_overflow(case, tune):
     assert c2.render_info()[1] == 0
     same(ref, got2)
     c2.close()
+
+
+@pytest.mark.parametrize("pool_max", [-1, 2])
+@pytest.mark.parametrize("tune", [dict(trace_budget=50),
+                                  dict(trace_budget=128, trace_phases=2, seg_min_steps=100, seg_len=32)],
+                         ids=["one-phase", "prefix-composed"])
+def test_streamed_record_pool(case, tune, pool_max, monkeypatch):
+    """The record pool: the per-ray slots hold the bounded phases' records,
+    and each ray still live when the last bounded phase ends gets a
+    full-length region assigned on the device, its records copied there
+    before the unbounded phase continues it.  The render equals the one-pass
+    render bit for bit.  The pool is sized from the previous render's share
+    of long rays (a first guess on a fresh context), so here, where most
+    rays outlast the bounded phase, the first render continues the rays
+    beyond its guess as outgrown ones and the second has a region for every
+    ray; with the pool held to 2 regions the other long rays continue as
+    outgrown ones (their count reported) on every render, still bit for
+    bit."""
+    h, prims, orc = case
+    say("case streamed record pool 24x24 %s pool_max %d" % (tune, pool_max))
+    c1, g, m = fresh(h, prims, trace_budget=0, seg_min_steps=tune.get("seg_min_steps", 3584),
+                     seg_len=tune.get("seg_len", 384))
+    im, _, _ = G.make_image(g, m.L_unit, nx=24, ny=24, thetacam=163.)
+    ref = c1.render(im)
+    c1.close()
+    n_long = int(np.sum(ref["nstep"] > tune["trace_budget"]))
+    assert n_long > 2
+    monkeypatch.setenv("IPGPU_RECORD_POOL_MAX", str(pool_max))
+    c2, _, _ = fresh(h, prims, **tune)
+    try:
+        for rep in range(2):
+            got = c2.render(im)
+            assert c2.stage_times_mode()[1] is True
+            chunks, cont, _ = c2.render_info()
+            want = n_long - 2 if pool_max > 0 else (0 if rep > 0 else cont)
+            assert chunks == 1 and cont == want and (rep > 0 or cont < n_long), (rep, chunks, cont, n_long)
+            same(ref, got)
+    finally:
+        c2.close()
 
 
 @pytest.mark.parametrize("seg_min", [3584, 20])
