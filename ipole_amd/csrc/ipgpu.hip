@@ -1808,6 +1808,23 @@ static int grow_keep(ipgpu_ctx *ctx, T *&ptr, size_t &cap, size_t need, size_t k
   return 0;
 }
 
+/* grow for the streamed render's arenas: when it must reallocate, a quarter
+ * more than asked (as far as 95% of the render memory leaves room), so that the
+ * next renders, whose sizes follow this one's, do not reallocate again (a
+ * reallocation of tens of GB stalls the render for hundreds of ms). */
+template <typename T>
+static int grow_hr(ipgpu_ctx *ctx, T *&ptr, size_t &cap, size_t need)
+{
+  if (need <= cap && ptr) return 0;
+  const size_t held = (ctx->traj_doubles + ctx->pkt_doubles + ctx->pkt2_doubles + ctx->ppkt_doubles
+                       + ctx->trajO_doubles) * sizeof(double) - cap * sizeof(T) + need * sizeof(T);
+  size_t extra = need / 4;
+  const size_t lim = ctx->render_bytes / 20 * 19;   /* 95%: the rest for per-ray tables and maps */
+  const size_t room = lim > held ? (lim - held) / sizeof(T) : 0;
+  if (extra > room) extra = room;
+  return grow(ctx, ptr, cap, need + extra);
+}
+
 /* Device address of scratch on each GPU, looked up once per device with
  * that device current. */
 constexpr int MAX_DEVICES = 64;
@@ -2325,13 +2342,13 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
   /* events: trace kernels [2k, 2k+1] (ev[0] opens phase 0), end of the
    * finished-ray work of phase k 15+k, first finished-ray work 19, longest
    * rays' work done 20, join 21, finalize 22 */
-  if ((rc = grow(ctx, ctx->d_traj, ctx->traj_doubles, (size_t)nrays * cap * REC))) return rc;
+  if ((rc = grow_hr(ctx, ctx->d_traj, ctx->traj_doubles, (size_t)nrays * cap * REC))) return rc;
   if ((rc = grow(ctx, ctx->d_live, ctx->cap_live, (size_t)nrays))) return rc;
   if ((rc = grow(ctx, ctx->d_list, ctx->cap_list, (size_t)nrays))) return rc;
   if ((rc = grow(ctx, ctx->d_offl, ctx->cap_offl, 2 * ((size_t)nrays + MAXPH + 1)))) return rc;
   if ((rc = grow(ctx, ctx->d_segbase, ctx->cap_segbase, (size_t)nrays))) return rc;
   if (ctx->last_total > 0) {
-    if ((rc = grow(ctx, ctx->d_pkt, ctx->pkt_doubles, (size_t)ctx->last_total * stride))) return rc;
+    if ((rc = grow_hr(ctx, ctx->d_pkt, ctx->pkt_doubles, (size_t)ctx->last_total * stride))) return rc;
     /* tiles: rows / 4 + 2 per list wave (ceil to 8 steps) per block of 4 */
     if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk,
                    (size_t)(ctx->last_total / PREP_BLOCK + 2 * ((long long)nrays / ILV + 2 * MAXPH)))))
@@ -2368,7 +2385,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
   const long long reg = (long long)(P.maxnstep + 1) * REC;   /* doubles per region */
   const long long pool_doubles = (long long)nreg * reg;
   if (nreg > 0) {
-    if ((rc = grow(ctx, ctx->d_trajO, ctx->trajO_doubles, (size_t)pool_doubles))
+    if ((rc = grow_hr(ctx, ctx->d_trajO, ctx->trajO_doubles, (size_t)pool_doubles))
         || (rc = grow(ctx, ctx->d_pool_list, ctx->cap_pool_list, (size_t)nreg))
         || (rc = grow(ctx, ctx->d_pool_ctr, ctx->cap_pool_ctr, (size_t)1))
         || (rc = grow(ctx, ctx->d_rbase, ctx->cap_rbase, nI)))
@@ -2538,7 +2555,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
           for (int q = 0; q < k; q++) HIPCHK(ctx, hipStreamSynchronize(ctx->streamF[q]));
           pofs = bofs = 0;
           const size_t want = (size_t)(T > ctx->last_total ? T : ctx->last_total);
-          if ((rc = grow(ctx, ctx->d_pkt, ctx->pkt_doubles, (want > 0 ? want : 1) * stride))) return rc;
+          if ((rc = grow_hr(ctx, ctx->d_pkt, ctx->pkt_doubles, (want > 0 ? want : 1) * stride))) return rc;
           if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, want / PREP_BLOCK + 2 * ((size_t)nrays / ILV + 2 * MAXPH))))
             return rc;
         }
@@ -2549,7 +2566,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
         if (!any_f) HIPCHK(ctx, hipEventRecord(ev[19], st));
         any_f = true;
       } else {
-        if ((rc = grow(ctx, ctx->d_pkt2, ctx->pkt2_doubles, (size_t)(T > 0 ? T : 1) * stride))) return rc;
+        if ((rc = grow_hr(ctx, ctx->d_pkt2, ctx->pkt2_doubles, (size_t)(T > 0 ? T : 1) * stride))) return rc;
         if ((rc = grow(ctx, ctx->d_blk2, ctx->cap_blk2, nb > 0 ? nb : 1))) return rc;
         pkt = ctx->d_pkt2;
         blk = ctx->d_blk2;
