@@ -1099,35 +1099,19 @@ IPG_HD void bessel_k012_f(float x, float ks[3], float &emx)
   ks[2] = k0s + (2.0f / x) * k1s;
 }
 
-IPG_HD void coefficients_f32(const Fluid &F, const double K[4], int emission_type, bool pol, double &ju,
-                             double &ku, Coef &c)
+/* The fits of coefficients_f32 from the fluid-frame frequency and angle
+ * (coef_geometry) and the local n_e, Theta_e and B. */
+IPG_HD void coefficients_f32_core(const Geo &q, double Ne, double Th, double B, int emission_type, bool pol,
+                                  double &ju, double &ku, Coef &c)
 {
   ju = 0.;
   ku = 0.;
   c.jI = c.jQ = c.jU = c.jV = c.aI = c.aQ = c.aU = c.aV = c.rQ = c.rU = c.rV = 0.;
-  double Ne = F.ne;
   if (Ne <= 0.) return;
-  double kdotu = K[0] * F.Ucov[0] + K[1] * F.Ucov[1] + K[2] * F.Ucov[2] + K[3] * F.Ucov[3];
-  double nu = -kdotu * (ME * CL * CL / HPL);
-  double nusq = nu * nu;
-  double sth, cth;
-  bool edge = false;
-  {
-    double Bm = sqrt(fabs(F.Bcon[0] * F.Bcov[0] + F.Bcon[1] * F.Bcov[1] + F.Bcon[2] * F.Bcov[2]
-                          + F.Bcon[3] * F.Bcov[3]));
-    if (Bm == 0.) {
-      sth = 1.;
-      cth = COS_PI_2;
-    } else {
-      double mu = (K[0] * F.Bcov[0] + K[1] * F.Bcov[1] + K[2] * F.Bcov[2] + K[3] * F.Bcov[3])
-          * rcp(fabs(kdotu) * Bm);
-      if (fabs(mu) > 1.) mu = (mu > 0.) ? 1. : -1.;
-      edge = (mu == 1. || mu == -1.);
-      sth = sqrt((1. - mu) * (1. + mu));
-      cth = mu;
-    }
-  }
-  const double B = F.B, Th = F.thetae;
+  const double nu = q.nu;
+  const double nusq = nu * nu;
+  const double sth = q.sth, cth = q.cth;
+  const bool edge = q.edge;
   const float Thf = (float)Th, sthf = (float)sth, cthf = (float)cth;
   float ks[3], emx;
   bessel_k012_f(1.0f / Thf, ks, emx);
@@ -1240,6 +1224,16 @@ IPG_HD void coefficients_f32(const Fluid &F, const double K[4], int emission_typ
     c.rV = (2. * PI / CL) * nu * eps12 * nu;
     c.rU = 0. * nu;
   }
+}
+
+IPG_HD void coefficients_f32(const Fluid &F, const double K[4], int emission_type, bool pol, double &ju,
+                             double &ku, Coef &c)
+{
+  if (F.ne <= 0.) {
+    coefficients_f32_core(Geo{0., 1., 0., false}, 0., 1., 0., emission_type, pol, ju, ku, c);
+    return;
+  }
+  coefficients_f32_core(coef_geometry(F, K), F.ne, F.thetae, F.B, emission_type, pol, ju, ku, c);
 }
 
 /* ------------------------------------------------------------ radiative step */

@@ -1091,6 +1091,259 @@ IPG_HD_INLINE int transfer_fused(const ipgpu_geom &G, const ipgpu_model &Mo, con
   return chain_finish(G, P, st, out7);
 }
 
+/* ------------------------------------------------ several jobs of one ray
+ *
+ * The jobs of a batch that share a geodesic trace (same camera, different
+ * frequencies, M_unit values or image options) share every part of a
+ * transfer step that depends on the geometry alone: the metric at the step's
+ * end and half-step points, the transport matrices (Gamma K is linear in K,
+ * and its product with the step dl, the only way it is used, is invariant
+ * under K -> f K, dl -> dl / f), the fluid state at the point, the
+ * fluid-frame angle and the plasma tetrad (functions of K's direction).
+ * transfer_fused_multi runs up to MJ jobs of one ray in one lane: the shared
+ * part once per step, then per job the fits at its frequency (nu scaled by
+ * f), n_e and B scales, the Stokes step with its dl, and its coherency tensor,
+ * which the caller keeps in memory (nst: element e of job j at
+ * nst[(16 j + e) * stride]; on the device a lane's slots in LDS).  Each job's
+ * results agree with transfer_fused on the job's rescaled records to the
+ * roundoff of the rescaling. */
+constexpr int MJ = 3;
+/* job loops unrolled, so that the per-job arrays are indexed by constants
+ * (a run-time index would put them in scratch memory) */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define IPG_UNROLL_MJ _Pragma("unroll")
+#else
+#define IPG_UNROLL_MJ
+#endif
+
+struct JobCfg {
+  Scale sc;
+  int etype, pol, floors, cj;
+};
+
+struct JobAcc {
+  double I, tau, tauF, js, ks;
+  int flag, nrad, Nnz;
+};
+
+IPG_HD_INLINE void nst_load(const double *nst, int stride, int j, Herm &N)
+{
+#pragma unroll
+  for (int e = 0; e < 16; e++) herm_par(N, e) = nst[(size_t)(16 * j + e) * stride];
+}
+
+IPG_HD_INLINE void nst_store(double *nst, int stride, int j, const Herm &N)
+{
+#pragma unroll
+  for (int e = 0; e < 16; e++) nst[(size_t)(16 * j + e) * stride] = herm_par(N, e);
+}
+
+template <bool F32 = false>
+IPG_HD_INLINE void fused_step_multi(const ipgpu_geom &G, const ipgpu_model &Mo, int nj, const JobCfg *cfg,
+                                    int target_nturns, int n, const double *rec, const int *nt,
+                                    const double *dump, JobAcc *acc, double *nst, int stride, double (*Mi)[4],
+                                    bool any_pol)
+{
+  Rec ti, tf;
+  load_rec(rec, n, ti);
+  load_rec(rec, n - 1, tf);
+  const double conv = Mo.L_unit * HPL / (ME * CL * CL);
+  Metric m;
+  metric(G, tf.X[1], tf.X[2], m);
+  bool anyN = false;
+  IPG_UNROLL_MJ
+  for (int j = 0; j < MJ; j++) anyN = anyN || (j < nj && cfg[j].pol && acc[j].Nnz);
+  if (anyN) {
+    /* push_polar twice, src/ipolarray.c:85-89, with the trace's K and dl */
+    double Mh[4][4];
+    {
+      double Xh[4], Kh[4];
+      half_step(m, tf.X, tf.K, -ti.dl, Xh, Kh);
+      Metric mh;
+      metric(G, Xh[1], Xh[2], mh);
+      transport_matrix(mh, Kh, Mh);
+    }
+    const double dl_cgs = ti.dl * conv;
+    IPG_UNROLL_MJ
+    for (int j = 0; j < MJ; j++) {
+      if (j >= nj || !(cfg[j].pol && acc[j].Nnz)) continue;
+      Herm N, Nh;
+      nst_load(nst, stride, j, N);
+      transport(N, N, Mi, (0.5 * dl_cgs) / conv, Nh);
+      transport(N, Nh, Mh, dl_cgs / conv, N);
+      nst_store(nst, stride, j, N);
+    }
+  }
+  if (!F32 && any_pol) transport_matrix(m, tf.K, Mi);
+  bool anyNnz = false;
+  if (radiating_region(G, Mo, m.r, m.th)) {
+    Fluid F;
+    fluid(G, Mo, dump, tf.X, m, F);
+    const Geo q0 = F.ne > 0. ? coef_geometry(F, tf.K) : Geo{0., 1., 0., false};
+    double E1[4], E2[4], e1[4], e2[4];
+    int tflag = 0;
+    if (any_pol) {
+      /* evolve_N's plasma frame (src/ipolarray.c:370-388) */
+      double Bcon[4] = {F.Bcon[0], F.Bcon[1], F.Bcon[2], F.Bcon[3]};
+      double bsq = F.Bcon[0] * F.Bcov[0] + F.Bcon[1] * F.Bcov[1] + F.Bcon[2] * F.Bcov[2] + F.Bcon[3] * F.Bcov[3];
+      if (bsq <= 0.) {
+        Bcon[0] = 0.;
+        Bcon[1] = 1.;
+        Bcon[2] = 1.;
+        Bcon[3] = 1.;
+      }
+      double E[4][4];
+      tflag = plasma_tetrad(F.Ucon, tf.K, Bcon, m.g, m.sqrtmg, E);
+      if (isnan(F.Ucov[0])) tflag |= 8;
+      lower(m.g, E[1], e1);
+      lower(m.g, E[2], e2);
+      for (int k = 0; k < 4; k++) {
+        E1[k] = E[1][k];
+        E2[k] = E[2][k];
+      }
+    }
+    const double half_x2 = (G.cstopx[2] - G.cstartx[2]) / 2;
+    bool zero_turns = target_nturns >= 0 && nt && nt[n] != target_nturns;
+    IPG_UNROLL_MJ
+    for (int j = 0; j < MJ; j++) {
+      if (j >= nj) break;
+      const JobCfg &J = cfg[j];
+      JobAcc &a = acc[j];
+      a.nrad++;
+      bool zero_emission = zero_turns;
+      if (J.cj == 1) {
+        if (tf.X[2] < half_x2) zero_emission = true;
+      } else if (J.cj == 2) {
+        if (tf.X[2] > half_x2) zero_emission = true;
+      }
+      const double dl_cgs = (ti.dl * J.sc.ikf) * conv;
+      const double ne = F.ne * J.sc.ne, B = F.B * J.sc.b;
+      Geo q = q0;
+      q.nu *= J.sc.kf;
+      Coef c;
+      double jf, kf;
+      if (ne > 0.) {
+        if (F32) coefficients_f32_core(q, ne, F.thetae, B, J.etype, J.pol, jf, kf, c);
+        else coefficients_core(q, ne, F.thetae, B, J.etype, J.pol, jf, kf, c);
+      } else {
+        if (F32) coefficients_f32_core(Geo{0., 1., 0., false}, 0., 1., 0., J.etype, J.pol, jf, kf, c);
+        else coefficients_core(Geo{0., 1., 0., false}, 0., 1., 0., J.etype, J.pol, jf, kf, c);
+      }
+      const double ji = zero_emission ? 0. : a.js, ki = a.ks;
+      a.js = jf;
+      a.ks = kf;
+      a.I = approximate_solve(a.I, ji, ki, zero_emission ? 0. : jf, kf, dl_cgs, a.tau);
+      if (J.pol) {
+        if (zero_emission) c.jI = c.jQ = c.jU = c.jV = 0.;
+        Herm N;
+        nst_load(nst, stride, j, N);
+        double S0[4];
+        herm_to_stokes(N, e1, e2, S0);
+        double T[4][4], cv[4];
+        if (F32) stokes_affine_f32(c, dl_cgs, T, cv);
+        else stokes_affine(c, dl_cgs, T, cv);
+        double S[4];
+        for (int i = 0; i < 4; i++)
+          S[i] = cv[i] + T[i][0] * S0[0] + T[i][1] * S0[1] + T[i][2] * S0[2] + T[i][3] * S0[3];
+        a.tauF += dl_cgs * fabs(c.rV);
+        if (J.floors) {
+          if (S[0] < 0) {
+            S[0] = S[1] = S[2] = S[3] = 0.;
+          } else {
+            double pf = sqrt(S[1] * S[1] + S[2] * S[2] + S[3] * S[3]) / S[0];
+            if (pf > 1.) {
+              S[1] /= pf;
+              S[2] /= pf;
+              S[3] /= pf;
+            }
+          }
+        }
+        double dS[4] = {S[0] - S0[0], S[1] - S0[1], S[2] - S0[2], S[3] - S0[3]};
+        herm_add_stokes(N, E1, E2, dS);
+        int flag = tflag;
+        if (a.tauF > 1.e100 || a.tauF < -1.e100 || isnan(a.tauF)) flag |= 2;
+        if (isnan(N.d[0]) || isnan(S[0])) flag |= 4;
+        a.flag |= flag;
+        a.Nnz = herm_nonzero(N);
+        nst_store(nst, stride, j, N);
+      }
+      anyNnz = anyNnz || (J.pol && a.Nnz);
+    }
+  } else {
+    IPG_UNROLL_MJ
+    for (int j = 0; j < MJ; j++) anyNnz = anyNnz || (j < nj && cfg[j].pol && acc[j].Nnz);
+  }
+  if (F32 && anyNnz) transport_matrix(m, tf.K, Mi);
+}
+
+/* The whole transfer of one ray for nj <= MJ jobs (cfg, their images P[j]
+ * for the projection), outputs out7[7 j ..]; returns nothing, flags and
+ * radiating-step counts per job in flags[j], nrad[j].  mi: the transport
+ * matrix carried between steps (16 doubles), nst: the jobs' coherency
+ * tensors (see above). */
+template <bool F32 = false>
+IPG_HD_INLINE void transfer_fused_multi(const ipgpu_geom &G, const ipgpu_model &Mo, int nj, const JobCfg *cfg,
+                                        const ipgpu_image *const *P, int ns, const double *rec, const int *nt,
+                                        const double *dump, double *out7, int *flags, int *nrad, double *mi,
+                                        double *nst, int stride)
+{
+  JobAcc acc[MJ];
+  bool any_pol = false;
+  {
+    /* the far end's unpolarized coefficients, per job */
+    Rec t;
+    load_rec(rec, ns, t);
+    Metric m;
+    metric(G, t.X[1], t.X[2], m);
+    Fluid F;
+    fluid(G, Mo, dump, t.X, m, F);
+    const Geo q0 = F.ne > 0. ? coef_geometry(F, t.K) : Geo{0., 1., 0., false};
+    IPG_UNROLL_MJ
+    for (int j = 0; j < MJ; j++) {
+      if (j >= nj) break;
+      const JobCfg &J = cfg[j];
+      const double ne = F.ne * J.sc.ne, B = F.B * J.sc.b;
+      Geo q = q0;
+      q.nu *= J.sc.kf;
+      Coef c;
+      double js, ks;
+      if (ne > 0.) {
+        if (F32) coefficients_f32_core(q, ne, F.thetae, B, J.etype, false, js, ks, c);
+        else coefficients_core(q, ne, F.thetae, B, J.etype, false, js, ks, c);
+      } else {
+        if (F32) coefficients_f32_core(Geo{0., 1., 0., false}, 0., 1., 0., J.etype, false, js, ks, c);
+        else coefficients_core(Geo{0., 1., 0., false}, 0., 1., 0., J.etype, false, js, ks, c);
+      }
+      JobAcc &a = acc[j];
+      a.I = a.tau = a.tauF = 0.;
+      a.js = js;
+      a.ks = ks;
+      a.flag = a.nrad = a.Nnz = 0;
+      Herm N;
+      for (int q2 = 0; q2 < 4; q2++) N.d[q2] = 0.;
+      for (int q2 = 0; q2 < 6; q2++) N.re[q2] = N.im[q2] = 0.;
+      nst_store(nst, stride, j, N);
+      any_pol = any_pol || J.pol;
+    }
+  }
+  double(*Mi)[4] = reinterpret_cast<double(*)[4]>(mi);
+  const int tn = P[0]->target_nturns;
+  for (int q = 0; q < ns; q++)
+    fused_step_multi<F32>(G, Mo, nj, cfg, tn, ns - q, rec, nt, dump, acc, nst, stride, Mi, any_pol);
+  IPG_UNROLL_MJ
+  for (int j = 0; j < MJ; j++) {
+    if (j >= nj) break;
+    Chain st;
+    nst_load(nst, stride, j, st.N);
+    st.I = acc[j].I;
+    st.tau = acc[j].tau;
+    st.tauF = acc[j].tauF;
+    st.flag = acc[j].flag;
+    flags[j] = chain_finish(G, *P[j], st, out7 + 7 * j);
+    nrad[j] = acc[j].nrad;
+  }
+}
+
 #if !defined(IPG_OPCOUNT)
 /* ------------------------------------------------------------------ adjoint
  *

@@ -802,6 +802,135 @@ __global__ __launch_bounds__(WAVE) void k_xfer_queue(ipgpu_geom G, ipgpu_model M
   }
 }
 
+/* The transfer of one ray for a subset of the jobs of its trace group
+ * (transfer_fused_multi): item (ray, subset), subset {n, job0, job1, job2}. */
+template <bool F32>
+__device__ __forceinline__ void xfer_multi_item(const ipgpu_geom &G, const ipgpu_model &Mo, const BJob *jobs,
+                                                const BGroup *groups, const int2 *items, const int4 *subsets,
+                                                int w, const long long *off, const double *traj,
+                                                const double *traj2, const int *nstep_in, const int *nturns_buf,
+                                                const double *dump, double *image, double *imageS, double *taus,
+                                                int *nstep_pix, int *flag_pix, unsigned long long *bstatus,
+                                                unsigned long long *counts, double *mi, double *nst)
+{
+  const int2 it = items[w];
+  const int r = it.x;
+  const int4 sb = subsets[it.y];
+  const int nj = sb.x;
+  const int jbs[MJ] = {sb.y, sb.z, sb.w};
+  const ipgpu_image &P0 = jobs[jbs[0]].P;
+  int pi, pj;
+  ray_pixel(nullptr, (int)(r - groups[jobs[jbs[0]].group].ray_base), P0.nx, P0.ny, pi, pj);
+  const long long pix = (long long)pi * P0.ny + pj;
+  const int ns = nstep_in[r];
+  if (ns >= P0.maxnstep - 1) {
+#pragma unroll
+    for (int j = 0; j < MJ; j++) {
+      if (j >= nj) break;
+      const BJob &J = jobs[jbs[j]];
+      const size_t p = (size_t)(J.pix_off + pix);
+      atomicMin(&bstatus[0], ((unsigned long long)jbs[j] << 32 | (unsigned long long)pix) + 1ull);
+      nstep_pix[p] = ns;
+      image[p] = 0.;
+      taus[p] = 0.;
+      flag_pix[p] = 0;
+      if (!J.P.only_unpolarized)
+        for (int q = 0; q < 5; q++) imageS[p * 5 + q] = 0.;
+    }
+    return;
+  }
+  /* off[r] >= 0: records at traj + off[r] REC; < 0: at traj2 + (-1 - off[r]) REC */
+  const long long ro = off[r];
+  const double *rec = ro >= 0 ? traj + (size_t)ro * REC : traj2 + (size_t)(-1 - ro) * REC;
+  const int *nt = (nturns_buf && P0.target_nturns >= 0) ? nturns_buf + ro : nullptr;
+  JobCfg cfg[MJ];
+  const ipgpu_image *Pp[MJ];
+#pragma unroll
+  for (int j = 0; j < MJ; j++) {
+    const BJob &J = jobs[jbs[j < nj ? j : 0]];
+    cfg[j].sc = J.sc;
+    cfg[j].etype = J.P.emission_type;
+    cfg[j].pol = !J.P.only_unpolarized;
+    cfg[j].floors = J.P.stokes_floors;
+    cfg[j].cj = J.P.counterjet;
+    Pp[j] = &J.P;
+  }
+  double o[7 * MJ];
+  int fl[MJ], nr[MJ];
+  transfer_fused_multi<F32>(G, Mo, nj, cfg, Pp, ns, rec, nt, dump, o, fl, nr, mi, nst, WAVE);
+  unsigned long long nrad = 0;
+#pragma unroll
+  for (int j = 0; j < MJ; j++) {
+    if (j >= nj) break;
+    const BJob &J = jobs[jbs[j]];
+    const ipgpu_image &P = J.P;
+    const size_t p = (size_t)(J.pix_off + pix);
+    const double f3 = P.freqcgs * P.freqcgs * P.freqcgs;
+    const double *oj = o + 7 * j;
+    nstep_pix[p] = ns;
+    image[p] = oj[0] * f3;
+    taus[p] = oj[5];
+    flag_pix[p] = fl[j];
+    if (!P.only_unpolarized) {
+      const double qs = (P.qu_conv == 0) ? -1. : 1.;
+      imageS[p * 5 + 0] = oj[1] * f3;
+      imageS[p * 5 + 1] = qs * oj[2] * f3;
+      imageS[p * 5 + 2] = qs * oj[3] * f3;
+      imageS[p * 5 + 3] = oj[4] * f3;
+      imageS[p * 5 + 4] = oj[6];
+      if (isnan(oj[1])) atomicMin(&bstatus[1], ((unsigned long long)jbs[j] << 32 | (unsigned long long)pix) + 1ull);
+    }
+    nrad += (unsigned long long)nr[j];
+  }
+  atomicAdd(&counts[1], nrad);
+}
+
+/* One lane per (ray, job subset) item, in waves of 64 items or, with a
+ * queue, as a persistent launch taking chunks of 64 items by an atomic
+ * ticket.  The transport matrix carried between steps and the jobs'
+ * coherency tensors live in LDS (a lane's slots, lane-minor). */
+template <bool F32>
+__global__ __launch_bounds__(WAVE) void k_xfer_multi(ipgpu_geom G, ipgpu_model Mo, const BJob *__restrict__ jobs,
+                                                     const BGroup *__restrict__ groups,
+                                                     const int2 *__restrict__ items, const int4 *__restrict__ subsets,
+                                                     int nitems, unsigned int *__restrict__ queue,
+                                                     const long long *__restrict__ off,
+                                                     const double *__restrict__ traj,
+                                                     const double *__restrict__ traj2,
+                                                     const int *__restrict__ nstep_in,
+                                                     const int *__restrict__ nturns_buf,
+                                                     const double *__restrict__ dump, double *__restrict__ image,
+                                                     double *__restrict__ imageS, double *__restrict__ taus,
+                                                     int *__restrict__ nstep_pix, int *__restrict__ flag_pix,
+                                                     unsigned long long *__restrict__ bstatus,
+                                                     unsigned long long *__restrict__ counts)
+{
+  __shared__ double mi_lds[WAVE * 17];
+  __shared__ double nst_lds[MJ * 16 * WAVE];
+  __shared__ unsigned int ticket;
+  double *mi = mi_lds + 17 * threadIdx.x;
+  double *nst = nst_lds + threadIdx.x;
+  if (!queue) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nitems) return;
+    xfer_multi_item<F32>(G, Mo, jobs, groups, items, subsets, w, off, traj, traj2, nstep_in, nturns_buf, dump,
+                         image, imageS, taus, nstep_pix, flag_pix, bstatus, counts, mi, nst);
+    return;
+  }
+  const unsigned int nchunks = (unsigned int)((nitems + WAVE - 1) / WAVE);
+  while (true) {
+    if (threadIdx.x == 0) ticket = atomicAdd(&queue[0], 1u);
+    __syncthreads();
+    const unsigned int c = ticket;
+    __syncthreads();
+    if (c >= nchunks) break;   /* every lane of the wave leaves together */
+    const int w = (int)(c * WAVE + threadIdx.x);
+    if (w < nitems)
+      xfer_multi_item<F32>(G, Mo, jobs, groups, items, subsets, w, off, traj, traj2, nstep_in, nturns_buf, dump,
+                           image, imageS, taus, nstep_pix, flag_pix, bstatus, counts, mi, nst);
+  }
+}
+
 /* queue[0] = 0; with bstatus, both status words to "none"; with counts,
  * the radiating step count to 0 (a kernel rather than memsets, so that the
  * clears can sit in a captured graph) */
@@ -1683,15 +1812,21 @@ struct ipgpu_ctx {
    * items (0); persistent waves (0: 4 per compute unit) */
   int batch_queue = 1;
   int batch_queue_waves = 0;
+  /* the transfer of up to MJ jobs of a trace group per lane (shared
+   * geometry), or one job per lane */
+  int batch_multi = 1;
+  int4 *d_subsets = nullptr;
+  size_t cap_subsets = 0;
   int ncu = 256;
   int xq_occ[2] = {1, 1};   /* resident waves per compute unit of k_xfer_queue<FP64>, <F32> */
+  int xm_occ[2] = {1, 1};   /* the same of k_xfer_multi */
   unsigned int *d_queue = nullptr;
   /* the last in-place batch, for ipgpu_rerender_batch: its images, trace
    * scales, output offsets, items and the graph of its transfer (job table
    * upload from pinned memory, queue reset, transfer) */
   struct LastBatch {
     bool valid = false;
-    int njobs = 0, nitems = 0, f32 = 0;
+    int njobs = 0, nitems = 0, f32 = 0, multi = 0;
     std::vector<ipgpu_image> img;
     std::vector<BJob> bj;
     long long pix_total = 0;
@@ -1874,9 +2009,14 @@ int ipgpu_create(ipgpu_ctx **out, int device)
     hipError_t e = f ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_xfer_queue<true>, WAVE, 0)
                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_xfer_queue<false>, WAVE, 0);
     if (e == hipSuccess && nb > 0) ctx->xq_occ[f] = nb;
+    nb = 0;
+    e = f ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_xfer_multi<true>, WAVE, 0)
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_xfer_multi<false>, WAVE, 0);
+    if (e == hipSuccess && nb > 0) ctx->xm_occ[f] = nb;
   }
   if (const char *e = getenv("IPGPU_BATCH_QUEUE")) ctx->batch_queue = atoi(e);
   if (const char *e = getenv("IPGPU_RECORD_POOL_MAX")) ctx->pool_max = atoi(e);
+  if (const char *e = getenv("IPGPU_BATCH_MULTI")) ctx->batch_multi = atoi(e);
   if (const char *e = getenv("IPGPU_BATCH_QUEUE_WAVES")) ctx->batch_queue_waves = atoi(e);
   if (noalloc_setup(device, &ctx->d_scratch) != IPGPU_OK) {
     hipStreamDestroy(ctx->stream);
@@ -1937,7 +2077,7 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
                   ctx->d_rlist, ctx->d_rcount, ctx->d_iavg, ctx->d_interp, ctx->d_ppkt, ctx->d_plist, ctx->d_traj2, ctx->d_off2,
                   ctx->d_pidx, ctx->d_trajO, ctx->d_ovl, ctx->d_ovoff, ctx->d_rbase, ctx->d_pixlist,
                   ctx->d_aint, ctx->d_amap, ctx->d_ag3, ctx->d_aN, ctx->d_aL, ctx->d_queue, ctx->d_pool_ctr,
-                  ctx->d_pool_list};
+                  ctx->d_pool_list, ctx->d_subsets};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int q = 0; q < NEV; q++)
@@ -3293,8 +3433,8 @@ static void host_ray_pixel(int ray, int nx, int ny, int &i, int &j)
  * d_traj / traj2), on the context's stream: the persistent work-queue
  * launch or one wave per 64 items.  clear_status: first reset the status
  * words and the radiating step count on the device. */
-static int batch_transfer(ipgpu_ctx *ctx, bool f32, int nitems, const double *traj2, const int *nturns,
-                          bool clear_status)
+static int batch_transfer(ipgpu_ctx *ctx, bool f32, bool multi, int nitems, const double *traj2,
+                          const int *nturns, bool clear_status)
 {
   hipStream_t st = ctx->stream;
   if (clear_status || (ctx->batch_queue && nitems > 0)) {
@@ -3303,6 +3443,18 @@ static int batch_transfer(ipgpu_ctx *ctx, bool f32, int nitems, const double *tr
     HIPCHK(ctx, hipGetLastError());
   }
   if (nitems <= 0) return 0;
+  if (multi) {
+    /* items (ray, job subset) */
+    auto km = f32 ? k_xfer_multi<true> : k_xfer_multi<false>;
+    const int nw = ctx->batch_queue_waves > 0 ? ctx->batch_queue_waves : ctx->xm_occ[f32 ? 1 : 0] * ctx->ncu;
+    const unsigned grid = ctx->batch_queue ? (unsigned)nw : (unsigned)((nitems + WAVE - 1) / WAVE);
+    km<<<grid, WAVE, 0, st>>>(ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items, ctx->d_subsets,
+                              nitems, ctx->batch_queue ? ctx->d_queue : nullptr, ctx->d_off, ctx->d_traj, traj2,
+                              ctx->d_nstep_ray, nturns, ctx->d_dump, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
+                              ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts);
+    HIPCHK(ctx, hipGetLastError());
+    return 0;
+  }
   if (ctx->batch_queue) {
     const int nw = ctx->batch_queue_waves > 0 ? ctx->batch_queue_waves : ctx->xq_occ[f32 ? 1 : 0] * ctx->ncu;
     auto kq = f32 ? k_xfer_queue<true> : k_xfer_queue<false>;
@@ -3392,6 +3544,41 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
   }
   const long long rays_total = groups.back().ray_base + groups.back().nrays;
   const int ngroups = (int)groups.size();
+  /* the multi-job transfer: each group's jobs in subsets of at most MJ, one
+   * work item per (ray, subset) */
+  const bool multi = ctx->batch_multi != 0;
+  std::vector<int4> subsets;
+  std::vector<std::vector<int>> gsub(ngroups);
+  if (multi) {
+    for (int g = 0; g < ngroups; g++)
+      for (size_t a = 0; a < members[g].size(); a += MJ) {
+        const int n = (int)std::min((size_t)MJ, members[g].size() - a);
+        int4 v;
+        v.x = n;
+        v.y = members[g][a];
+        v.z = members[g][a + (n > 1 ? 1 : 0)];
+        v.w = members[g][a + (n > 2 ? 2 : 0)];
+        gsub[g].push_back((int)subsets.size());
+        subsets.push_back(v);
+      }
+    if ((rc = grow(ctx, ctx->d_subsets, ctx->cap_subsets, subsets.size()))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_subsets, subsets.data(), subsets.size() * sizeof(int4), hipMemcpyHostToDevice,
+                               ctx->stream));
+  }
+  /* a ray's items: (job, ray) each, or (ray, subset) in multi mode.  Items
+   * go longest ray first (the launch's tail is then the short rays); in
+   * multi mode rays within 256 steps of each other have their items ordered
+   * by subset size, so that a wave's lanes mostly run as many jobs */
+  auto ray_items = [&](int r, int g, int sz, auto &&emit) {
+    if (!multi) {
+      if (sz == 1)
+        for (int jb : members[g]) emit(make_int2(jb, r));
+      return;
+    }
+    for (int sid : gsub[g])
+      if (subsets[sid].x == sz) emit(make_int2(r, sid));
+  };
+  const int max_sz = multi ? MJ : 1;
   if ((rc = grow(ctx, ctx->d_groups, ctx->cap_groups, groups.size()))) return rc;
   if ((rc = grow(ctx, ctx->d_jobs, ctx->cap_jobs, (size_t)njobs))) return rc;
   if ((rc = grow(ctx, ctx->d_bimage, ctx->cap_bimage, (size_t)pix_total))) return rc;
@@ -3515,7 +3702,8 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     for (int g = 0; g < ngroups; g++)
       for (long long r = groups[g].ray_base; r < groups[g].ray_base + groups[g].nrays; r++) gid[r] = g;
     long long nit = 0;
-    for (long long q = 0; q < nvalid; q++) nit += (long long)members[gid[order[q]]].size();
+    for (long long q = 0; q < nvalid; q++)
+      nit += multi ? (long long)gsub[gid[order[q]]].size() : (long long)members[gid[order[q]]].size();
     if ((size_t)nit > ctx->h_items_cap) {
       if (ctx->h_items) hipHostFree(ctx->h_items);
       ctx->h_items = nullptr;
@@ -3525,9 +3713,15 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     }
     int2 *hit = ctx->h_items;
     long long w = 0;
-    for (long long q = 0; q < nvalid; q++) {
-      const int r = order[q];
-      for (int jb : members[gid[r]]) hit[w++] = make_int2(jb, r);
+    for (long long q0 = 0; q0 < nvalid;) {
+      long long q1 = q0;
+      while (q1 < nvalid && (ns_host[order[q1]] >> 8) == (ns_host[order[q0]] >> 8)) q1++;
+      for (int sz = max_sz; sz >= 1; sz--)
+        for (long long q = q0; q < q1; q++) {
+          const int r = order[q];
+          ray_items(r, gid[r], sz, [&](int2 v) { hit[w++] = v; });
+        }
+      q0 = q1;
     }
     const int nitems = (int)nit;
     if ((rc = grow(ctx, ctx->d_items, ctx->cap_items, (size_t)(nitems > 0 ? nitems : 1)))) return rc;
@@ -3549,7 +3743,7 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, hit, nitems * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-    if ((rc = batch_transfer(ctx, jobs[0].image.transfer_fp32 != 0, nitems, nlong > 0 ? ctx->d_traj2 : nullptr,
+    if ((rc = batch_transfer(ctx, jobs[0].image.transfer_fp32 != 0, multi, nitems, nlong > 0 ? ctx->d_traj2 : nullptr,
                              nullptr, false)))
       return rc;
     HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
@@ -3558,6 +3752,7 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     ctx->blast.valid = true;
     ctx->blast.nitems = nitems;
     ctx->blast.traj2 = nlong > 0 ? ctx->d_traj2 : nullptr;
+    ctx->blast.multi = multi;
     int mism = 0;
     HIPCHK(ctx, hipMemcpyAsync(&mism, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -3609,16 +3804,22 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     /* work items (job, ray) of these rays, in the rays' (longest first) order */
     items.clear();
-    for (long long q = a; q < b; q++) {
-      int r = order[q];
-      int g = group_of_host(groups, r);
-      for (int jb : members[g]) items.push_back(make_int2(jb, r));
+    for (long long q0 = a; q0 < b;) {
+      long long q1 = q0;
+      while (q1 < b && (ns_host[order[q1]] >> 8) == (ns_host[order[q0]] >> 8)) q1++;
+      for (int sz = max_sz; sz >= 1; sz--)
+        for (long long q = q0; q < q1; q++) {
+          const int r = order[q];
+          ray_items(r, group_of_host(groups, r), sz, [&](int2 v) { items.push_back(v); });
+        }
+      q0 = q1;
     }
     const int nitems = (int)items.size();
     if ((rc = grow(ctx, ctx->d_items, ctx->cap_items, (size_t)(nitems > 0 ? nitems : 1)))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, items.data(), nitems * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-    if ((rc = batch_transfer(ctx, jobs[0].image.transfer_fp32 != 0, nitems, nullptr, turns ? ctx->d_nturns : nullptr,
+    if ((rc = batch_transfer(ctx, jobs[0].image.transfer_fp32 != 0, multi, nitems, nullptr,
+                             turns ? ctx->d_nturns : nullptr,
                              false)))
       return rc;
     HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
@@ -3686,7 +3887,7 @@ static int rerender_enqueue(ipgpu_ctx *ctx)
   const ipgpu_ctx::LastBatch &L = ctx->blast;
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_jobs, ctx->h_bjobs, (size_t)L.njobs * sizeof(BJob), hipMemcpyHostToDevice,
                              ctx->stream));
-  return batch_transfer(ctx, L.f32 != 0, L.nitems, L.traj2, nullptr, true);
+  return batch_transfer(ctx, L.f32 != 0, L.multi != 0, L.nitems, L.traj2, nullptr, true);
 }
 
 int ipgpu_rerender_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
@@ -3719,7 +3920,8 @@ int ipgpu_rerender_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     b.sc.b = sqrt(jobs[j].m_unit_scale);
     ctx->h_bjobs[j] = b;
   }
-  const int nw = ctx->batch_queue_waves > 0 ? ctx->batch_queue_waves : ctx->xq_occ[L.f32 ? 1 : 0] * ctx->ncu;
+  const int nw = ctx->batch_queue_waves > 0 ? ctx->batch_queue_waves
+                                            : (L.multi ? ctx->xm_occ : ctx->xq_occ)[L.f32 ? 1 : 0] * ctx->ncu;
   ipgpu_ctx::GraphKey key;
   memset(&key, 0, sizeof(key));
   key.geom = ctx->geom;

@@ -201,6 +201,56 @@ int emu_render_scaled(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_ima
   return status;
 }
 
+/* The multi-job transfer (transfer_fused_multi) of nj <= MJ jobs over the
+ * rays traced with Ptrace: out7 (npix x nj x 7), flag (npix x nj). */
+int emu_render_multi(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *Ptrace, int nj,
+                     const ipgpu_image *Pjobs, const double *ne_s, const double *dump, const int *pix, int npix,
+                     double *out7, int *nstep, int *flag)
+{
+  if (nj < 1 || nj > MJ) return -1;
+  JobCfg cfg[MJ];
+  const ipgpu_image *Pp[MJ];
+  for (int j = 0; j < nj; j++) {
+    cfg[j].sc.kf = Pjobs[j].freq / Ptrace->freq;
+    cfg[j].sc.ikf = 1. / cfg[j].sc.kf;
+    cfg[j].sc.ne = ne_s[j];
+    cfg[j].sc.b = sqrt(ne_s[j]);
+    cfg[j].etype = Pjobs[j].emission_type;
+    cfg[j].pol = !Pjobs[j].only_unpolarized;
+    cfg[j].floors = Pjobs[j].stokes_floors;
+    cfg[j].cj = Pjobs[j].counterjet;
+    Pp[j] = &Pjobs[j];
+  }
+  const bool f32 = Pjobs[0].transfer_fp32 != 0;
+  int status = 0;
+#pragma omp parallel
+  {
+    std::vector<double> rec((size_t)Ptrace->maxnstep * REC);
+    std::vector<int> nt(Ptrace->target_nturns >= 0 ? Ptrace->maxnstep : 1);
+    double mi[16], nst[16 * MJ];
+#pragma omp for schedule(dynamic, 1)
+    for (int n = 0; n < npix; n++) {
+      int i = pix[2 * n], j = pix[2 * n + 1];
+      int *ntp = Ptrace->target_nturns >= 0 ? nt.data() : nullptr;
+      int ns = trace_ray(*G, *Ptrace, i, j, Ptrace->maxnstep, rec.data(), ntp);
+      nstep[n] = ns;
+      if (ns >= Ptrace->maxnstep - 1) {
+#pragma omp critical
+        status = -10;
+        continue;
+      }
+      int nrad[MJ];
+      if (f32)
+        transfer_fused_multi<true>(*G, *M, nj, cfg, Pp, ns, rec.data(), ntp, dump, out7 + 7 * (size_t)nj * n,
+                                   flag + (size_t)nj * n, nrad, mi, nst, 1);
+      else
+        transfer_fused_multi<false>(*G, *M, nj, cfg, Pp, ns, rec.data(), ntp, dump, out7 + 7 * (size_t)nj * n,
+                                    flag + (size_t)nj * n, nrad, mi, nst, 1);
+    }
+  }
+  return status;
+}
+
 /* Loss and gradients of the L2 image loss against target (npix x 5, the
  * reference layout) over all pixels of the image: out[0..3] = loss,
  * dL/dln(M_unit), dL/dR_high, dL/dR_low; gcell (ghosted cells, zeroed by

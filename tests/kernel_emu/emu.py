@@ -26,6 +26,8 @@ def lib():
         L.emu_math.argtypes = [C.c_int, dp, C.c_int, dp]
         L.emu_stokes_affine.argtypes = [dp, C.c_double, dp, dp]
         L.emu_stokes_affine_f32.argtypes = [dp, C.c_double, dp, dp]
+        L.emu_render_multi.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), C.c_int,
+                                       C.POINTER(G.Image), dp, dp, ip, C.c_int, dp, ip, ip]
         L.emu_plasma_tetrad.argtypes = [C.POINTER(G.Geom), dp, dp, dp, dp, dp]
         L.emu_render.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), dp, ip,
                                  C.c_int, dp, ip, ip]
@@ -119,6 +121,22 @@ def render_segmented(geom, model, image, dump, seg_len=256):
     out7, ns, fl = np.zeros((n, 7)), np.zeros(n, np.int32), np.zeros(n, np.int32)
     rc = lib().emu_render_segmented(C.byref(geom), C.byref(model), C.byref(image), G.P(f64(dump)), G.IP(pix),
                                     n, seg_len, G.P(out7), G.IP(ns), G.IP(fl))
+    return rc, out7, ns, fl
+
+
+def render_multi(geom, model, image_trace, image_jobs, ne_scales, dump):
+    """Multi-job transfer on the CPU (see emu_render_multi): out7 (npix, nj,
+    7), nstep (npix), flag (npix, nj)."""
+    nj = len(image_jobs)
+    nx, ny = image_trace.nx, image_trace.ny
+    pix = np.array([(i, j) for i in range(nx) for j in range(ny)], np.int32)
+    n = len(pix)
+    arr = (G.Image * nj)(*image_jobs)
+    out7, ns, fl = np.zeros((n, nj, 7)), np.zeros(n, np.int32), np.zeros((n, nj), np.int32)
+    d = f64(dump) if dump is not None else None
+    rc = lib().emu_render_multi(C.byref(geom), C.byref(model), C.byref(image_trace), nj, arr,
+                                G.P(f64(ne_scales)), G.P(d) if d is not None else None, G.IP(pix), n,
+                                G.P(out7), G.IP(ns), G.IP(fl))
     return rc, out7, ns, fl
 
 

@@ -318,6 +318,41 @@ def test_emu_batch_rescaling(unpol):
         assert np.mean(e > 1e-6) < 1e-3, (k, np.mean(e > 1e-6))
 
 
+@pytest.mark.parametrize("f32", [0, 1])
+def test_emu_multi_job_transfer(f32):
+    """Three jobs of one trace group in one lane (transfer_fused_multi: the
+    geometry of a step once, the fits, Stokes step and coherency tensor per
+    job) against each job's own fused transfer on its rescaled records: the
+    job at the trace's frequency bit for bit, the others (86 GHz trace reused
+    at 230 and 345 GHz, other M_unit scales, one unpolarized with the
+    counterjet cut, one Pandya) to the roundoff of the rescaling; then two
+    jobs, and one."""
+    h, prims = small_dump()
+    _, fields = O.iharm_oracle(h, prims, M_unit=3e26)
+    d = emu.pack_dump(fields)
+    g, m = G.iharm_geom(h), G.iharm_model()
+    kw = dict(nx=16, ny=16, thetacam=163., transfer_fp32=f32)
+    im_t = G.make_image(g, m.L_unit, freqcgs=86e9, **kw)[0]
+    jobs = [G.make_image(g, m.L_unit, freqcgs=86e9, **kw)[0],
+            G.make_image(g, m.L_unit, freqcgs=230e9, only_unpolarized=1, counterjet=1, **kw)[0],
+            G.make_image(g, m.L_unit, freqcgs=345e9, emission_type=1, **kw)[0]]
+    scales = [1.0, 0.4, 2.5]
+    for nj in (3, 2, 1):
+        rc, out7, ns, fl = emu.render_multi(g, m, im_t, jobs[:nj], scales[:nj], d)
+        assert rc == 0
+        for j in range(nj):
+            rc1, ref7, ns1, fl1 = emu.render_scaled(g, m, im_t, jobs[j], d, ne_scale=scales[j])
+            assert rc1 == 0 and np.array_equal(ns, ns1) and np.array_equal(fl[:, j], fl1), (nj, j)
+            got = out7[:, j, :]
+            if j == 0:
+                assert np.array_equal(got, ref7), (nj, j)
+            else:
+                scale = np.abs(ref7[:, :5]).max(axis=0)
+                err = np.abs(got[:, :5] - ref7[:, :5]).max(axis=0) / np.where(scale > 0, scale, 1.)
+                assert err.max() < 1e-11, (nj, j, err)
+                assert np.allclose(got[:, 5:], ref7[:, 5:], rtol=1e-11, atol=0.), (nj, j)
+
+
 def image_summary(I, Q, U, V):
     """Ftot-proportional sum, net LP and CP fractions and EVPA (rad) of an
     image, as print_image_stats forms them (src/main.c:1001-1042)."""
