@@ -145,16 +145,39 @@ def batch_measure(args, h=None, prims=None):
         t_dev.append(ctx.kernel_times())
     res = ctx.render_batch(batch)
     n_rad = ctx.step_counts()[1]
+    bc = ctx.batch_counts()
     n_xfer = int(sum(int(r["nstep"].sum()) for r in res))
     dF = max(abs(r["imageS"][..., 0].sum() / f - 1) for r, f in zip(res, Fplain))
     ms = float(np.mean([t[2] for t in t_dev]))
     ms_x = float(np.mean([t[1] for t in t_dev]))
     log("batch FP64: %.1f ms device (geodesic %.1f, transfer %.1f); max |dFtot|/Ftot vs plain %.2e" %
         (ms, np.mean([t[0] for t in t_dev]), ms_x, dF))
-    # the fused transfer's algorithmic FLOP: radiating and non-radiating
-    # steps and rays at the op-counting build's per-unit counts
-    flop = (fl["F_fused_per_radiating_step"] * n_rad + fl["F_fused_per_nonradiating_step"] * (n_xfer - n_rad)
-            + fl["F_fused_per_ray"] * npix)
+    # the transfer's algorithmic FLOP at the op-counting build's per-unit
+    # counts: with several jobs of a ray in one lane, the shared part per
+    # item (its steps, radiating steps, the item) and the per-job part per
+    # (job, ray) pair; each job on its own, the single-job fused transfer's
+    flop_single = (fl["F_fused_per_radiating_step"] * n_rad + fl["F_fused_per_nonradiating_step"] * (n_xfer - n_rad)
+                   + fl["F_fused_per_ray"] * npix)
+    if bc["items"] < bc["job_items"]:
+        flop = (fl["F_multi_shared_per_radiating_step"] * bc["item_radiating_steps"]
+                + fl["F_multi_shared_per_nonradiating_step"] * (bc["item_steps"] - bc["item_radiating_steps"])
+                + fl["F_multi_shared_per_ray"] * bc["items"]
+                + fl["F_multi_job_per_radiating_step"] * n_rad
+                + fl["F_multi_job_per_nonradiating_step"] * (bc["job_steps"] - n_rad)
+                + fl["F_multi_job_per_ray"] * bc["job_items"])
+        kname = "k_xfer_multi<FP64>"
+        fnote = ("FLOP = per item: radiating steps x %.0f + non-radiating x %.0f + %.0f (the geometry its jobs "
+                 "share), plus per (job, ray): radiating steps x %.0f + non-radiating x %.0f + %.0f "
+                 "(op-counting build, ipole_amd/flops_per_step.json); the jobs one by one would need %.3g"
+                 % (fl["F_multi_shared_per_radiating_step"], fl["F_multi_shared_per_nonradiating_step"],
+                    fl["F_multi_shared_per_ray"], fl["F_multi_job_per_radiating_step"],
+                    fl["F_multi_job_per_nonradiating_step"], fl["F_multi_job_per_ray"], flop_single))
+    else:
+        flop = flop_single
+        kname = "k_xfer_queue<FP64>" if tuned["queue"] else "k_xfer_batch<FP64>"
+        fnote = ("FLOP = radiating steps x %.0f + non-radiating x %.0f + rays x %.0f (op-counting build, "
+                 "ipole_amd/flops_per_step.json)" % (fl["F_fused_per_radiating_step"],
+                                                     fl["F_fused_per_nonradiating_step"], fl["F_fused_per_ray"]))
     tflops = flop / (ms_x * 1e-3) / 1e12
     # the M_unit search: the same jobs at new scales on the resident records
     # (the transfer alone, replayed from its captured graph)
@@ -188,13 +211,10 @@ def batch_measure(args, h=None, prims=None):
                 fp64_speedup_vs_plain=round(sum(plain_ms) / ms, 3), fp64_max_dFtot_vs_plain=dF,
                 fp64_autotuned=tuned, transfer_steps=n_xfer, radiating_steps=int(n_rad),
                 fp64_transfer_ms_device=round(ms_x, 3),
+                transfer_items=bc,
                 fp64_transfer_roofline=dict(
-                    bound="fp64-valu", kernel="k_xfer_queue<FP64>" if tuned["queue"] else "k_xfer_batch<FP64>",
-                    achieved=round(tflops, 3), peak=PEAK_FP64, unit="TFLOP/s", frac=round(tflops / PEAK_FP64, 4),
-                    flop=flop, note="FLOP = radiating steps x %.0f + non-radiating x %.0f + rays x %.0f "
-                                    "(op-counting build, ipole_amd/flops_per_step.json)"
-                                    % (fl["F_fused_per_radiating_step"], fl["F_fused_per_nonradiating_step"],
-                                       fl["F_fused_per_ray"])),
+                    bound="fp64-valu", kernel=kname, achieved=round(tflops, 3), peak=PEAK_FP64, unit="TFLOP/s",
+                    frac=round(tflops / PEAK_FP64, 4), flop=flop, note=fnote),
                 fp64_rerender_ms_device=round(ms_rr, 3), mixed_autotuned=tuned32,
                 mixed_ms_device=round(ms32, 3), mixed_pixels_per_s=round(npix / (ms32 * 1e-3), 1),
                 mixed_speedup_vs_plain=round(sum(plain_ms) / ms32, 3), mixed_max_dFtot_vs_plain=dF32)

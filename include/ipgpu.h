@@ -259,6 +259,14 @@ int ipgpu_set_batch_options(ipgpu_ctx *ctx, int slot, int queue, int queue_waves
  * choice is OpenMP's dynamic schedule over pixels (src/main.c:536). */
 int ipgpu_autotune_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs, int chosen[4]);
 
+/* The last batch's transfer work, for measurement: out[0] work items (one
+ * lane each: a ray with up to three jobs of its trace group, whose geometry
+ * they share, or with IPGPU_BATCH_MULTI=0 one job), out[1] the items'
+ * transfer steps, out[2] their radiating steps (an item's counted once),
+ * out[3] (job, ray) pairs, out[4] their transfer steps.  No counterpart in
+ * the reference. */
+int ipgpu_batch_counts(const ipgpu_ctx *ctx, long long out[5]);
+
 /* Device memory a single-image render (ipgpu_render) may use for its ray
  * records and step packets (default 60% of the device's memory, and never
  * more than the device has free).  An image whose rays need more is

@@ -811,7 +811,8 @@ __device__ __forceinline__ void xfer_multi_item(const ipgpu_geom &G, const ipgpu
                                                 const double *traj2, const int *nstep_in, const int *nturns_buf,
                                                 const double *dump, double *image, double *imageS, double *taus,
                                                 int *nstep_pix, int *flag_pix, unsigned long long *bstatus,
-                                                unsigned long long *counts, double *mi, double *nst)
+                                                unsigned long long *counts, unsigned long long *bcounts, double *mi,
+                                                double *nst)
 {
   const int2 it = items[w];
   const int r = it.x;
@@ -883,6 +884,7 @@ __device__ __forceinline__ void xfer_multi_item(const ipgpu_geom &G, const ipgpu
     nrad += (unsigned long long)nr[j];
   }
   atomicAdd(&counts[1], nrad);
+  atomicAdd(&bcounts[0], (unsigned long long)nr[0]);   /* the ray's radiating steps, once */
 }
 
 /* One lane per (ray, job subset) item, in waves of 64 items or, with a
@@ -903,7 +905,8 @@ __global__ __launch_bounds__(WAVE) void k_xfer_multi(ipgpu_geom G, ipgpu_model M
                                                      double *__restrict__ imageS, double *__restrict__ taus,
                                                      int *__restrict__ nstep_pix, int *__restrict__ flag_pix,
                                                      unsigned long long *__restrict__ bstatus,
-                                                     unsigned long long *__restrict__ counts)
+                                                     unsigned long long *__restrict__ counts,
+                                                     unsigned long long *__restrict__ bcounts)
 {
   __shared__ double mi_lds[WAVE * 17];
   __shared__ double nst_lds[MJ * 16 * WAVE];
@@ -914,7 +917,7 @@ __global__ __launch_bounds__(WAVE) void k_xfer_multi(ipgpu_geom G, ipgpu_model M
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nitems) return;
     xfer_multi_item<F32>(G, Mo, jobs, groups, items, subsets, w, off, traj, traj2, nstep_in, nturns_buf, dump,
-                         image, imageS, taus, nstep_pix, flag_pix, bstatus, counts, mi, nst);
+                         image, imageS, taus, nstep_pix, flag_pix, bstatus, counts, bcounts, mi, nst);
     return;
   }
   const unsigned int nchunks = (unsigned int)((nitems + WAVE - 1) / WAVE);
@@ -927,14 +930,15 @@ __global__ __launch_bounds__(WAVE) void k_xfer_multi(ipgpu_geom G, ipgpu_model M
     const int w = (int)(c * WAVE + threadIdx.x);
     if (w < nitems)
       xfer_multi_item<F32>(G, Mo, jobs, groups, items, subsets, w, off, traj, traj2, nstep_in, nturns_buf, dump,
-                           image, imageS, taus, nstep_pix, flag_pix, bstatus, counts, mi, nst);
+                           image, imageS, taus, nstep_pix, flag_pix, bstatus, counts, bcounts, mi, nst);
   }
 }
 
 /* queue[0] = 0; with bstatus, both status words to "none"; with counts,
  * the radiating step count to 0 (a kernel rather than memsets, so that the
  * clears can sit in a captured graph) */
-__global__ void k_queue_reset(unsigned int *queue, unsigned long long *bstatus, unsigned long long *counts)
+__global__ void k_queue_reset(unsigned int *queue, unsigned long long *bstatus, unsigned long long *counts,
+                              unsigned long long *bcounts)
 {
   if (threadIdx.x == 0) {
     queue[0] = 0u;
@@ -943,6 +947,7 @@ __global__ void k_queue_reset(unsigned int *queue, unsigned long long *bstatus, 
       bstatus[1] = ~0ull;
     }
     if (counts) counts[1] = 0ull;
+    if (bcounts) bcounts[0] = 0ull;
   }
 }
 
@@ -1817,6 +1822,10 @@ struct ipgpu_ctx {
   int batch_multi = 1;
   int4 *d_subsets = nullptr;
   size_t cap_subsets = 0;
+  /* the last batch's work: items, their transfer steps and radiating steps
+   * (a multi-job item counts once), (job, ray) pairs and their steps */
+  unsigned long long *d_bcounts = nullptr;
+  long long bstat[5] = {0, 0, 0, 0, 0};
   int ncu = 256;
   int xq_occ[2] = {1, 1};   /* resident waves per compute unit of k_xfer_queue<FP64>, <F32> */
   int xm_occ[2] = {1, 1};   /* the same of k_xfer_multi */
@@ -2077,7 +2086,7 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
                   ctx->d_rlist, ctx->d_rcount, ctx->d_iavg, ctx->d_interp, ctx->d_ppkt, ctx->d_plist, ctx->d_traj2, ctx->d_off2,
                   ctx->d_pidx, ctx->d_trajO, ctx->d_ovl, ctx->d_ovoff, ctx->d_rbase, ctx->d_pixlist,
                   ctx->d_aint, ctx->d_amap, ctx->d_ag3, ctx->d_aN, ctx->d_aL, ctx->d_queue, ctx->d_pool_ctr,
-                  ctx->d_pool_list, ctx->d_subsets};
+                  ctx->d_pool_list, ctx->d_subsets, ctx->d_bcounts};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int q = 0; q < NEV; q++)
@@ -3439,7 +3448,7 @@ static int batch_transfer(ipgpu_ctx *ctx, bool f32, bool multi, int nitems, cons
   hipStream_t st = ctx->stream;
   if (clear_status || (ctx->batch_queue && nitems > 0)) {
     k_queue_reset<<<1, WAVE, 0, st>>>(ctx->d_queue, clear_status ? ctx->d_bstatus : nullptr,
-                                      clear_status ? ctx->d_counts : nullptr);
+                                      clear_status ? ctx->d_counts : nullptr, clear_status ? ctx->d_bcounts : nullptr);
     HIPCHK(ctx, hipGetLastError());
   }
   if (nitems <= 0) return 0;
@@ -3451,7 +3460,7 @@ static int batch_transfer(ipgpu_ctx *ctx, bool f32, bool multi, int nitems, cons
     km<<<grid, WAVE, 0, st>>>(ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items, ctx->d_subsets,
                               nitems, ctx->batch_queue ? ctx->d_queue : nullptr, ctx->d_off, ctx->d_traj, traj2,
                               ctx->d_nstep_ray, nturns, ctx->d_dump, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
-                              ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts);
+                              ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts, ctx->d_bcounts);
     HIPCHK(ctx, hipGetLastError());
     return 0;
   }
@@ -3588,6 +3597,9 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
   if ((rc = grow(ctx, ctx->d_bflag, ctx->cap_bflag, (size_t)pix_total))) return rc;
   if (!ctx->d_bstatus) HIPCHK(ctx, hipMalloc((void **)&ctx->d_bstatus, 2 * sizeof(unsigned long long)));
   if (!ctx->d_queue) HIPCHK(ctx, hipMalloc((void **)&ctx->d_queue, 64 * sizeof(unsigned int)));
+  if (!ctx->d_bcounts) HIPCHK(ctx, hipMalloc((void **)&ctx->d_bcounts, 2 * sizeof(unsigned long long)));
+  HIPCHK(ctx, hipMemsetAsync(ctx->d_bcounts, 0, 2 * sizeof(unsigned long long), ctx->stream));
+  for (long long &v : ctx->bstat) v = 0;
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_groups, groups.data(), groups.size() * sizeof(BGroup), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_jobs, bj.data(), bj.size() * sizeof(BJob), hipMemcpyHostToDevice, ctx->stream));
   static const unsigned long long bst_init[2] = {~0ull, ~0ull};
@@ -3724,6 +3736,14 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
       q0 = q1;
     }
     const int nitems = (int)nit;
+    for (long long k = 0; k < nit; k++) {
+      const int2 v = hit[k];
+      const int r = multi ? v.x : v.y, nj = multi ? subsets[v.y].x : 1;
+      ctx->bstat[0] += 1;
+      ctx->bstat[1] += ns_host[r];
+      ctx->bstat[3] += nj;
+      ctx->bstat[4] += (long long)ns_host[r] * nj;
+    }
     if ((rc = grow(ctx, ctx->d_items, ctx->cap_items, (size_t)(nitems > 0 ? nitems : 1)))) return rc;
     if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)(nlong > 0 ? nlong : 1)))) return rc;
     if ((rc = grow(ctx, ctx->d_off2, ctx->cap_off2, (size_t)(nlong > 0 ? nlong : 1)))) return rc;
@@ -3815,6 +3835,13 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
       q0 = q1;
     }
     const int nitems = (int)items.size();
+    for (const int2 &v : items) {
+      const int r = multi ? v.x : v.y, nj = multi ? subsets[v.y].x : 1;
+      ctx->bstat[0] += 1;
+      ctx->bstat[1] += ns_host[r];
+      ctx->bstat[3] += nj;
+      ctx->bstat[4] += (long long)ns_host[r] * nj;
+    }
     if ((rc = grow(ctx, ctx->d_items, ctx->cap_items, (size_t)(nitems > 0 ? nitems : 1)))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, items.data(), nitems * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
@@ -3837,8 +3864,11 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
   unsigned long long bst[2];
   HIPCHK(ctx, hipMemcpyAsync(bst, ctx->d_bstatus, sizeof(bst), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(ctx->counts, ctx->d_counts, sizeof(ctx->counts), hipMemcpyDeviceToHost, ctx->stream));
+  unsigned long long bc = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&bc, ctx->d_bcounts, sizeof(bc), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->counts[0] += (unsigned long long)steps_host;   /* the rays whose records pass 1 stored */
+  ctx->bstat[2] = multi ? (long long)bc : (long long)ctx->counts[1];
   hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[3]);
   ctx->ms_geo = ms_geo;
   ctx->ms_xfer = ms_xfer;
@@ -3955,8 +3985,11 @@ int ipgpu_rerender_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
   unsigned long long bst[2];
   HIPCHK(ctx, hipMemcpyAsync(bst, ctx->d_bstatus, sizeof(bst), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(ctx->counts, ctx->d_counts, sizeof(ctx->counts), hipMemcpyDeviceToHost, ctx->stream));
+  unsigned long long bc = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&bc, ctx->d_bcounts, sizeof(bc), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->counts[0] = 0;   /* no geodesic is traced */
+  ctx->bstat[2] = L.multi ? (long long)bc : (long long)ctx->counts[1];
   hipEventElapsedTime(&ctx->ms_total, ctx->ev[0], ctx->ev[3]);
   ctx->ms_geo = 0.f;
   ctx->ms_xfer = ctx->ms_total;
@@ -4069,6 +4102,13 @@ int ipgpu_autotune_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs, int c
                                      : 0;
     chosen[3] = ctx->xq_occ[q[0].image.transfer_fp32 ? 1 : 0];
   }
+  return IPGPU_OK;
+}
+
+int ipgpu_batch_counts(const ipgpu_ctx *ctx, long long out[5])
+{
+  if (!ctx || !out) return IPGPU_ERR_ARG;
+  for (int k = 0; k < 5; k++) out[k] = ctx->bstat[k];
   return IPGPU_OK;
 }
 

@@ -142,6 +142,7 @@ def lib():
         L.ipgpu_set_batch_options.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.ipgpu_autotune_batch.argtypes = [vp, C.c_int, C.POINTER(Job), ip]
         L.ipgpu_autotune_render.argtypes = [vp, C.POINTER(Image), ip]
+        L.ipgpu_batch_counts.argtypes = [vp, C.POINTER(C.c_longlong)]
         L.ipgpu_set_render_memory.argtypes = [vp, C.c_size_t]
         L.ipgpu_render_noalloc.argtypes = [vp, C.POINTER(Image), vp, vp, vp, vp, vp]
         L.ipgpu_render_info.argtypes = [vp, ip, ip, C.POINTER(C.c_size_t)]
@@ -411,6 +412,14 @@ class Context:
         ch = np.zeros(4, np.int32)
         self._chk(self.L.ipgpu_autotune_batch(self.h, n, arr, IP(ch)))
         return dict(slot=int(ch[0]), queue=int(ch[1]), queue_waves=int(ch[2]), resident_waves_per_cu=int(ch[3]))
+
+    def batch_counts(self):
+        """The last batch's transfer work: dict(items, item_steps,
+        item_radiating_steps, job_items, job_steps)."""
+        out = (C.c_longlong * 5)()
+        self._chk(self.L.ipgpu_batch_counts(self.h, out))
+        return dict(items=out[0], item_steps=out[1], item_radiating_steps=out[2], job_items=out[3],
+                    job_steps=out[4])
 
     def autotune_render(self, image):
         """Measure and keep the streamed render's settings for this image:

@@ -33,14 +33,16 @@ def count(geom, model, image, dump, pix):
     subprocess.run(["make", "-s", "-C", HERE, "libopcount.so"], check=True)
     L = C.CDLL(os.path.join(HERE, "libopcount.so"))
     L.opcount_render.argtypes = [C.POINTER(G.Geom), C.POINTER(G.Model), C.POINTER(G.Image), G.dp, G.ip,
-                                 C.c_int, G.dp, G.dp]
+                                 C.c_int, G.dp, G.dp, G.dp]
     out = np.zeros(12)
     pix = np.ascontiguousarray(pix, np.int32)
     per_ray = np.zeros((len(pix), 3))
+    per_ray_multi = np.zeros((len(pix), 3))
     rc = L.opcount_render(C.byref(geom), C.byref(model), C.byref(image),
-                          G.P(dump) if dump is not None else None, G.IP(pix), len(pix), G.P(out), G.P(per_ray))
+                          G.P(dump) if dump is not None else None, G.IP(pix), len(pix), G.P(out), G.P(per_ray),
+                          G.P(per_ray_multi))
     assert rc == 0
-    return out, per_ray
+    return out, per_ray, per_ray_multi
 
 
 def main():
@@ -57,7 +59,7 @@ def main():
     for unpol in (0, 1):
         im, _, _ = G.make_image(g, m.L_unit, nx=160, ny=160, thetacam=163., freqcgs=230e9,
                                 only_unpolarized=unpol)
-        o, per_ray = count(g, m, im, dump, pix)
+        o, per_ray, per_ray_multi = count(g, m, im, dump, pix)
         n_geo, n_step, n_rad = o[0], o[1], o[2]
         # the fused transfer's flops per ray as a radiating and a non-radiating
         # step cost plus a per-ray cost (far-end coefficients, projection),
@@ -65,6 +67,12 @@ def main():
         A = np.column_stack([per_ray[:, 1], per_ray[:, 0] - per_ray[:, 1], np.ones(len(per_ray))])
         (f_rad, f_non, f_ray), *_ = np.linalg.lstsq(A, per_ray[:, 2], rcond=None)
         fit_err = float(np.abs(A @ [f_rad, f_non, f_ray] - per_ray[:, 2]).sum() / per_ray[:, 2].sum())
+        # the multi-job transfer: per ray, flops of 1, 2, 3 jobs = shared + n x per job
+        jm = (per_ray_multi[:, 2] - per_ray_multi[:, 0]) / 2.
+        gm = per_ray_multi[:, 0] - jm
+        (g_rad, g_non, g_ray), *_ = np.linalg.lstsq(A, gm, rcond=None)
+        (j_rad, j_non, j_ray), *_ = np.linalg.lstsq(A, jm, rcond=None)
+        lin_err = float(np.abs(per_ray_multi[:, 1] - (gm + 2. * jm)).sum() / per_ray_multi[:, 1].sum())
         res["unpol" if unpol else "pol"] = dict(
             geodesic_steps=int(n_geo), transfer_steps=int(n_step), radiating_steps=int(n_rad), rays=int(o[8]),
             F_trace_per_step=o[3] / n_geo,
@@ -76,6 +84,10 @@ def main():
             F_fused_per_radiating_step=f_rad, F_fused_per_nonradiating_step=f_non, F_fused_per_ray=f_ray,
             F_fused_fit_rel_err=fit_err,
             F_seg_per_column_step=o[10] / (17 * n_step), F_seg_intensity_per_step=o[11] / n_step,
+            F_multi_shared_per_radiating_step=g_rad, F_multi_shared_per_nonradiating_step=g_non,
+            F_multi_shared_per_ray=g_ray, F_multi_job_per_radiating_step=j_rad,
+            F_multi_job_per_nonradiating_step=j_non, F_multi_job_per_ray=j_ray,
+            F_multi_linearity_rel_err=lin_err,
         )
         print(("unpol" if unpol else "pol"), json.dumps(res["unpol" if unpol else "pol"]), flush=True)
     out = dict(
@@ -97,7 +109,12 @@ def main():
                                               "summed count",
                    F_seg_per_column_step="FLOP per step of one of the 17 N-map columns of k_seg "
                                          "(segment_column), 384-step segments",
-                   F_seg_intensity_per_step="FLOP per step of k_seg's unpolarized column (segment_intensity)"),
+                   F_seg_intensity_per_step="FLOP per step of k_seg's unpolarized column (segment_intensity)",
+                   F_multi_shared_per_radiating_step="the multi-job fused transfer (k_xfer_multi) of n jobs of a "
+                                                     "ray costs the shared part (per radiating step, "
+                                                     "non-radiating step and ray) plus n times the per-job part "
+                                                     "(F_multi_job_*), fitted from 1 and 3 jobs; "
+                                                     "F_multi_linearity_rel_err checks 2 jobs against the fit"),
         **res)
     path = os.path.join(ROOT, "ipole_amd", "flops_per_step.json")
     with open(path, "w") as f:

@@ -66,6 +66,7 @@ inline Cd rint(Cd a) { return Cd(std::rint(a.v)); }
 inline Cd fmin(Cd a, Cd b) { return Cd(std::fmin(a.v, b.v)); }
 inline Cd fmax(Cd a, Cd b) { return Cd(std::fmax(a.v, b.v)); }
 inline Cd ldexp(Cd a, int e) { return Cd(std::ldexp(a.v, e)); }
+inline Cd frexp(Cd a, int *e) { return Cd(std::frexp(a.v, e)); }
 inline Cd fma(Cd a, Cd b, Cd c) { g_ops.fma_++; return Cd(std::fma(a.v, b.v, c.v)); }
 inline Cd fmod(Cd a, Cd b) { g_ops.div++; return Cd(std::fmod(a.v, b.v)); }
 inline bool isnan(Cd a) { return std::isnan(a.v); }
@@ -93,10 +94,13 @@ extern "C" {
  * far_end_coefficients, number of rays, of transfer_fused (the batch's
  * FP64 fused transfer of the same ray), of segment_column over the ray's
  * 384-step segments (17 columns), of segment_intensity over them.
+ * per_ray_multi (optional, 3 per pixel): transfer_fused_multi's flops for
+ * the first 1, 2, 3 of three jobs (the image's frequency and M_unit, 230/86
+ * of its frequency at 0.4 M_unit, 345/86 at 2.5 M_unit).
  * per_ray (optional, 3 per pixel):
  * the pixel's transfer steps, radiating steps and transfer_fused flops. */
 int opcount_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, const double *dump,
-                   const int *pix, int npix, double *out, double *per_ray)
+                   const int *pix, int npix, double *out, double *per_ray, double *per_ray_multi)
 {
   std::vector<Cd> rec((size_t)P->maxnstep * REC);
   std::vector<int> nt(P->target_nturns >= 0 ? P->maxnstep : 1);
@@ -163,6 +167,33 @@ int opcount_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image 
         segment_intensity<1>(pkt.data(), q0, q1, u);
       }
       out[11] += g_ops.flops() - h.flops();
+    }
+    if (per_ray_multi) {
+      ipgpu_image Pj[MJ];
+      JobCfg cfg[MJ];
+      const ipgpu_image *Pp[MJ];
+      const double ratio[MJ] = {1., 230. / 86., 345. / 86.}, sne[MJ] = {1., 0.4, 2.5};
+      for (int j = 0; j < MJ; j++) {
+        Pj[j] = *P;
+        Pj[j].freq = P->freq * ratio[j];
+        Pj[j].freqcgs = P->freqcgs * ratio[j];
+        cfg[j].sc.kf = ratio[j];
+        cfg[j].sc.ikf = 1. / ratio[j];
+        cfg[j].sc.ne = sne[j];
+        cfg[j].sc.b = sqrt(sne[j]);
+        cfg[j].etype = P->emission_type;
+        cfg[j].pol = pol;
+        cfg[j].floors = P->stokes_floors;
+        cfg[j].cj = P->counterjet;
+        Pp[j] = &Pj[j];
+      }
+      for (int nj = 1; nj <= MJ; nj++) {
+        Cd mi[16], nst[16 * MJ], o7[7 * MJ];
+        int fl[MJ], nr[MJ];
+        Ops a = g_ops;
+        transfer_fused_multi<false>(*G, *M, nj, cfg, Pp, ns, rec.data(), ntp, d, o7, fl, nr, mi, nst, 1);
+        per_ray_multi[3 * n + nj - 1] = g_ops.flops() - a.flops();
+      }
     }
     if (per_ray) {
       per_ray[3 * n] = ns;
