@@ -276,6 +276,18 @@ int ipgpu_batch_counts(const ipgpu_ctx *ctx, long long out[5]);
  * equivalent, which holds the records of every ray of a chunk at once. */
 int ipgpu_set_render_memory(ipgpu_ctx *ctx, size_t bytes);
 
+/* How the single-image render's step-packet kernels read the dump cells of
+ * the trilinear interpolation (src/grid.c:15-44, interp_scalar): on = 1
+ * stages in LDS, per group of 8 consecutive steps of a ray, the box of
+ * cells that spans their interpolation corners (at most 24 cells, loaded
+ * with direct-to-LDS loads ahead of the step's metric; larger boxes read
+ * the dump directly); on = 0 (default) reads each step's 8 corner records
+ * from the dump.  Both give the same interpolated fields; the polarized
+ * images of the two settings agree to ~1e-11 relative (the compiler fuses
+ * the step's multiply-adds differently in the two kernel instances), the
+ * unpolarized images bit for bit.  No counterpart in the reference. */
+int ipgpu_set_gather_staging(ipgpu_ctx *ctx, int on);
+
 /* Work layout of the last ipgpu_render: chunks of rays it ran in, rays whose
  * trace outgrew the per-ray record capacity and continued in full-length
  * regions (the capacity follows the longest ray of the previous render), and

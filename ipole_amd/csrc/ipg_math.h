@@ -35,6 +35,33 @@
 
 namespace ipg {
 
+/* The library routines the fast paths below fall back to outside their
+ * ranges, called out of line on the device: each fast path's rare branch is
+ * then a call instead of an inlined copy of the routine (the packet and
+ * batch-transfer kernels inline dozens of call sites; their code shrinks by
+ * about a third, which keeps more of it in the instruction cache). */
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __attribute__((noinline)) double lib_exp(double x) { return exp(x); }
+__device__ __attribute__((noinline)) double lib_log(double x) { return log(x); }
+__device__ __attribute__((noinline)) double lib_pow(double a, double b) { return pow(a, b); }
+__device__ __attribute__((noinline)) double lib_cbrt(double x) { return cbrt(x); }
+__device__ __attribute__((noinline)) void lib_sincos(double x, double *s, double *c)
+{
+  *s = sin(x);
+  *c = cos(x);
+}
+#else
+inline double lib_exp(double x) { return exp(x); }
+inline double lib_log(double x) { return log(x); }
+inline double lib_pow(double a, double b) { return pow(a, b); }
+inline double lib_cbrt(double x) { return cbrt(x); }
+inline void lib_sincos(double x, double *s, double *c)
+{
+  *s = sin(x);
+  *c = cos(x);
+}
+#endif
+
 #if !defined(IPG_OPCOUNT)
 typedef double ipg_f64; /* plain double for compile-time constant tables */
 #endif
@@ -105,7 +132,7 @@ IPG_MATH_HD double exp_r(double x)
 
 IPG_MATH_HD double exp_(double x)
 {
-  if (!(fabs(x) < 700.)) return exp(x);
+  if (!(fabs(x) < 700.)) return lib_exp(x);
   return exp_r(x);
 }
 
@@ -170,8 +197,7 @@ IPG_MATH_HD void sincos_r(double x, double &s, double &c)
 IPG_MATH_HD void sincos_(double x, double &s, double &c)
 {
   if (!(fabs(x) < 1.e5)) {
-    s = sin(x);
-    c = cos(x);
+    lib_sincos(x, &s, &c);
     return;
   }
   sincos_r(x, s, c);
@@ -205,7 +231,7 @@ constexpr double SQRT_HALF = 0.7071067811865476;
  * library routine. */
 IPG_MATH_HD double log_(double x)
 {
-  if (!(x >= 1.e-300 && x <= 1.e300)) return log(x);
+  if (!(x >= 1.e-300 && x <= 1.e300)) return lib_log(x);
   int e;
   double m = frexp(x, &e);
   if (m < SQRT_HALF) {
@@ -233,14 +259,14 @@ IPG_MATH_HD double log_(double x)
 /* a^b for a > 0 as e^(b log a); a = 0 gives 0 for b > 0 as pow does. */
 IPG_MATH_HD double pow_(double a, double b)
 {
-  if (!(a > 0.)) return pow(a, b);
+  if (!(a > 0.)) return lib_pow(a, b);
   return exp_(b * log_(a));
 }
 
 /* cube root of x > 0 (library routine otherwise) */
 IPG_MATH_HD double cbrt_(double x)
 {
-  if (!(x > 0.)) return cbrt(x);
+  if (!(x > 0.)) return lib_cbrt(x);
   double y = exp_(log_(x) * (1.0 / 3.0));
   /* one Newton step on y^3 = x restores the last bits lost to the
    * log/exp round trip */

@@ -616,24 +616,33 @@ IPG_HD bool in_domain(const ipgpu_geom &G, double X1, double X2)
  * U1 U2 U3 B1 B2 B3 ne thetae b sigma and 6 unused slots. */
 constexpr int CELL_STRIDE = 16;
 
-/* Trilinear interpolation of the 10 fields at once (src/grid.c:15-44). */
-IPG_HD void gather10(const ipgpu_geom &G, const double *dump, const Cell &c, double v[10])
+/* Trilinear interpolation of the 10 fields at once (src/grid.c:15-44).
+ * at(n, f) is field f of corner n = 4 di + 2 dj + dk, the cell record at
+ * (i + di, j + dj, k + dk); one body for every source of the corner
+ * records, with its multiply-adds written out (the compiler's own fusing
+ * of a*b + c*d depends on the code around it), so that every source gives
+ * the same roundings. */
+template <class At>
+IPG_HD_INLINE void interp10(const Cell &c, const At &at, double v[10])
 {
-  const long s3 = 1, s2 = (G.N3 + 2), s1 = (long)(G.N2 + 2) * (G.N3 + 2);
-  const long base = (long)c.i * s1 + (long)c.j * s2 + c.k * s3;
   const double b1 = 1. - c.d1, b2 = 1. - c.d2;
   const double w00 = b1 * b2, w01 = b1 * c.d2, w10 = c.d1 * b2, w11 = c.d1 * c.d2;
   const double z0 = 1. - c.d3, z1 = c.d3;
-  const double *p000 = dump + CELL_STRIDE * base;
-  const double *p010 = dump + CELL_STRIDE * (base + s2);
-  const double *p100 = dump + CELL_STRIDE * (base + s1);
-  const double *p110 = dump + CELL_STRIDE * (base + s1 + s2);
   for (int f = 0; f < 10; f++) {
-    double lo = p000[f] * w00 + p010[f] * w01 + p100[f] * w10 + p110[f] * w11;
-    double hi = p000[CELL_STRIDE + f] * w00 + p010[CELL_STRIDE + f] * w01
-        + p100[CELL_STRIDE + f] * w10 + p110[CELL_STRIDE + f] * w11;
-    v[f] = z0 * lo + z1 * hi;
+    const double lo = fma(at(6, f), w11, fma(at(4, f), w10, fma(at(2, f), w01, at(0, f) * w00)));
+    const double hi = fma(at(7, f), w11, fma(at(5, f), w10, fma(at(3, f), w01, at(1, f) * w00)));
+    v[f] = fma(z1, hi, z0 * lo);
   }
+}
+
+/* The corner records straight from the dump. */
+IPG_HD void gather10(const ipgpu_geom &G, const double *dump, const Cell &c, double v[10])
+{
+  const long s2 = (G.N3 + 2), s1 = (long)(G.N2 + 2) * (G.N3 + 2);
+  const double *p = dump + CELL_STRIDE * ((long)c.i * s1 + (long)c.j * s2 + c.k);
+  const long o[8] = {0, CELL_STRIDE, CELL_STRIDE * s2, CELL_STRIDE * (s2 + 1), CELL_STRIDE * s1,
+                     CELL_STRIDE * (s1 + 1), CELL_STRIDE * (s1 + s2), CELL_STRIDE * (s1 + s2 + 1)};
+  interp10(c, [&](int n, int f) { return p[o[n] + f]; }, v);
 }
 
 /* model/iharm/model.c:567-575 */
@@ -682,9 +691,12 @@ struct FluidExt {
 };
 
 /* Fluid state at a point: get_model_fourv/ne/thetae/b in one evaluation
- * (model/iharm/model.c:420-593; model/sphere/model.c:126-211). */
+ * (model/iharm/model.c:420-593; model/sphere/model.c:126-211).  With
+ * use_vpre, vpre holds the 10 interpolated fields at X that the caller has
+ * formed already (the values gather10 would return). */
 IPG_HD void fluid(const ipgpu_geom &G, const ipgpu_model &M, const double *dump, const double X[4],
-                  const Metric &m, Fluid &F, FluidExt *ext = nullptr)
+                  const Metric &m, Fluid &F, FluidExt *ext = nullptr, const double *vpre = nullptr,
+                  bool use_vpre = false)
 {
   if (ext) {
     ext->cell = -1.;
@@ -723,7 +735,10 @@ IPG_HD void fluid(const ipgpu_geom &G, const ipgpu_model &M, const double *dump,
   Cell c;
   locate(G, X[1], X[2], X[3], c);
   double v[10];
-  gather10(G, dump, c, v);
+  if (use_vpre)
+    for (int f = 0; f < 10; f++) v[f] = vpre[f];
+  else
+    gather10(G, dump, c, v);
   double VdotV = 0.;
   for (int i = 1; i < 4; i++)
     for (int j = 1; j < 4; j++) VdotV += m.g[sy(i, j)] * v[i - 1] * v[j - 1];

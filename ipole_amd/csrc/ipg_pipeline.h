@@ -310,37 +310,58 @@ IPG_HD void step_param_maps(double *ax, int etype, double dl)
 }
 #endif
 
-template <bool F32 = false, int PS = 1>
+/* The source of the step's records and interpolated dump fields when the
+ * caller has loaded or staged them (records / fetch fill their arguments
+ * and return true); NoVG: prep_step loads the records and gathers the
+ * fields from the dump itself. */
+struct NoVG {
+  IPG_HD bool records(Rec &, Rec &) const { return false; }
+  IPG_HD bool fetch(double *) const { return false; }
+};
+
+template <bool F32 = false, int PS = 1, class VG = NoVG>
 IPG_HD_INLINE void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int n,
                       const double *rec, const int *nt, const double *dump, double *pk_out, bool pol,
-                      const Scale &sc, double *ax = nullptr)
+                      const Scale &sc, double *ax = nullptr, const VG &vg = VG())
 {
   PkW<PS> pk{pk_out};
   if (ax)
     for (int q = 0; q < AX; q++) ax[q] = 0.;
   Rec ti, tf;
-  load_rec(rec, n, ti);
-  load_rec(rec, n - 1, tf);
+  if (!vg.records(ti, tf)) {
+    load_rec(rec, n, ti);
+    load_rec(rec, n - 1, tf);
+  }
   scale_rec(ti, sc);
   scale_rec(tf, sc);
   const double conv = Mo.L_unit * HPL / (ME * CL * CL);
   const double dl_cgs = ti.dl * conv;
   Metric m;
   metric(G, tf.X[1], tf.X[2], m);
+  /* staged fields are read here, before the packet stores begin */
+  double vst[10];
+  const bool staged = vg.fetch(vst);
   int meta = 0;
   pk[PK_DL] = dl_cgs;
   if (pol) {
-    double Mf[4][4], Mh[4][4], Xh[4], Kh[4];
-    transport_matrix(m, tf.K, Mf);
+    /* each transport matrix stored as soon as it is formed: the step-end
+     * matrix is not held while the half-step metric is evaluated */
+    double Xh[4], Kh[4];
+    {
+      double Mf[4][4];
+      transport_matrix(m, tf.K, Mf);
+      for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 4; k++) pk[PK_MF + 4 * i + k] = Mf[i][k];
+    }
     half_step(m, tf.X, tf.K, -ti.dl, Xh, Kh);
-    Metric mh;
-    metric(G, Xh[1], Xh[2], mh);
-    transport_matrix(mh, Kh, Mh);
-    for (int i = 0; i < 4; i++)
-      for (int k = 0; k < 4; k++) {
-        pk[PK_MF + 4 * i + k] = Mf[i][k];
-        pk[PK_MH + 4 * i + k] = Mh[i][k];
-      }
+    {
+      Metric mh;
+      metric(G, Xh[1], Xh[2], mh);
+      double Mh[4][4];
+      transport_matrix(mh, Kh, Mh);
+      for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 4; k++) pk[PK_MH + 4 * i + k] = Mh[i][k];
+    }
     pk[PK_HDL] = (0.5 * dl_cgs) / conv;
     pk[PK_FDL] = dl_cgs / conv;
     pk[PK_DTF] = 0.;
@@ -359,7 +380,7 @@ IPG_HD_INLINE void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const i
     if (zero_emission) meta |= 2;
     Fluid F;
     FluidExt ext;
-    fluid(G, Mo, dump, tf.X, m, F, ax ? &ext : nullptr);
+    fluid(G, Mo, dump, tf.X, m, F, ax ? &ext : nullptr, vst, staged);
     F.ne *= sc.ne;
     F.B *= sc.b;
     if (ax && F.ne > 0.) {

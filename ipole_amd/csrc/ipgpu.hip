@@ -267,9 +267,149 @@ __global__ void k_blockmap(int nrays, const long long *__restrict__ off, int *__
   for (long long b = b0; b < b1; b++) blk_ray[b] = r;
 }
 
+/* LDS-staged dump gathers of the packet kernels.  In their waves each group
+ * of 8 lanes holds 8 consecutive steps of one ray, whose interpolation
+ * corners fall in a few neighbouring cells: the group stages the box of
+ * cells that spans them (at most SG_CAP cells, else it gathers from the
+ * dump as before) in LDS with direct-to-LDS loads, issued before the step's
+ * metric is formed, and each lane then reads its 8 corners from LDS.  Cell
+ * v of group g (box order: i, then j, then k fastest) is loaded by lane
+ * 8 g + (v & 7); its 16-byte chunk ch (fields 2 ch, 2 ch + 1) lands in the
+ * wave's 1 KiB row (v >> 3) * 5 + ch, at that lane's 16 bytes.  On the
+ * 160x160 FMKS image ~80% of the steps' groups fit in 24 cells, which cuts
+ * the corner-line requests of the gathers ~3x and turns them into
+ * 80-byte runs. */
+constexpr int SG_CAP = 24;                   /* cells per group box */
+constexpr int SG_CPL = (SG_CAP + 7) / 8;     /* cells per lane */
+constexpr int SG_ROWS = 5 * SG_CPL;          /* 1 KiB LDS rows per wave */
+constexpr int SG_WAVE_DBL = SG_ROWS * 128;   /* doubles of LDS per wave */
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
+
+/* min / max over the 8 lanes of a group: DPP quad permutations [1,0,3,2]
+ * and [2,3,0,1], then the half-row mirror (lane i <- lane 7 - i) */
+__device__ __forceinline__ int group8_min(int x)
+{
+  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false));
+  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false));
+  return min(x, __builtin_amdgcn_update_dpp(x, x, 0x141, 0xF, 0xF, false));
+}
+__device__ __forceinline__ int group8_max(int x)
+{
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false));
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false));
+  return max(x, __builtin_amdgcn_update_dpp(x, x, 0x141, 0xF, 0xF, false));
+}
+
+/* A lane's view of its group's staged box; fetch() is prep_step's VG. */
+struct StagedCells {
+  const double *lds;   /* the wave's LDS rows */
+  Cell c;              /* the lane's interpolation cell */
+  int lo_i, lo_j, lo_k, bj, bk, slot0;
+  bool use;            /* the group is staged and the lane needs the fields */
+  Rec ti, tf;          /* the step's records, loaded before the staging loads
+                        * (vmcnt is in order: a later record load would wait
+                        * for them) */
+  __device__ bool records(Rec &a, Rec &b) const
+  {
+    a = ti;
+    b = tf;
+    return true;
+  }
+  __device__ bool fetch(double v[10]) const
+  {
+    if (!use) return false;
+    /* the direct-to-LDS loads count in vmcnt; nothing else reads them */
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    int off[8];
+    for (int n = 0; n < 8; n++) {
+      const int cv = ((c.i + (n >> 2) - lo_i) * bj + (c.j + ((n >> 1) & 1) - lo_j)) * bk + (c.k + (n & 1) - lo_k);
+      off[n] = (cv >> 3) * 5 * 128 + 2 * (slot0 + (cv & 7));
+    }
+    interp10(c, [&](int n, int f) { return lds[off[n] + (f >> 1) * 128 + (f & 1)]; }, v);
+    return true;
+  }
+};
+
+/* Called by every lane of the wave (need = false for lanes without a step
+ * or outside the dump); X1..X3 of the lane's interpolation point.  Forms
+ * the group's box, issues its loads and fills S. */
+__device__ __forceinline__ void stage_cells(const ipgpu_geom &G, const double *dump, bool need, double X1,
+                                            double X2, double X3, double *lds_wave, StagedCells &S)
+{
+  const int lane = (int)(threadIdx.x & (WAVE - 1));
+  const int t = lane & 7;
+  if (need) locate(G, X1, X2, X3, S.c);
+  const int BIG = 1 << 28;
+  const int lo_i = group8_min(need ? S.c.i : BIG), hi_i = group8_max(need ? S.c.i : -BIG);
+  const int lo_j = group8_min(need ? S.c.j : BIG), hi_j = group8_max(need ? S.c.j : -BIG);
+  const int lo_k = group8_min(need ? S.c.k : BIG), hi_k = group8_max(need ? S.c.k : -BIG);
+  const bool any = hi_i >= lo_i;
+  const int bi = hi_i - lo_i + 2, bj = hi_j - lo_j + 2, bk = hi_k - lo_k + 2;
+  const bool on = any && bi * bj <= SG_CAP && bi * bj * bk <= SG_CAP;
+  const int V = on ? bi * bj * bk : 0;
+  S.lds = lds_wave;
+  S.lo_i = lo_i;
+  S.lo_j = lo_j;
+  S.lo_k = lo_k;
+  S.bj = bj;
+  S.bk = bk;
+  S.slot0 = lane & ~7;
+  S.use = on && need;
+  const long s2 = G.N3 + 2, s1 = (long)(G.N2 + 2) * (G.N3 + 2);
+  for (int m = 0; m < SG_CPL; m++) {
+    const int v = m * 8 + t;
+    if (v < V) {
+      const int a = v / (bj * bk), r = v - a * (bj * bk), b = r / bk, kk = r - b * bk;
+      const double *p = dump + CELL_STRIDE * ((long)(lo_i + a) * s1 + (long)(lo_j + b) * s2 + (lo_k + kk));
+      for (int ch = 0; ch < 5; ch++)
+        __builtin_amdgcn_global_load_lds((glb_void_t *)(p + 2 * ch), (lds_void_t *)(lds_wave + (m * 5 + ch) * 128),
+                                         16, 0, 0);
+    }
+  }
+}
+
+/* stage_cells for step n of a ray's records (its interpolation point is
+ * record n - 1, as in prep_step), whose records S keeps for prep_step;
+ * act = false: the lane has no step. */
+__device__ __forceinline__ void stage_step(const ipgpu_geom &G, const ipgpu_model &Mo, const double *dump, bool act,
+                                           const double *rec, int n, const double *dummy, double *lds_wave,
+                                           StagedCells &S)
+{
+  /* loaded on every path (a lane without a step reads a stand-in pair of
+   * records, dummy: any two records that exist), so that no record load is
+   * still counted when the staging loads are issued */
+  load_rec(act ? rec : dummy, act ? n : 1, S.ti);
+  load_rec(act ? rec : dummy, act ? n - 1 : 0, S.tf);
+  /* each record value passes through an empty asm here, on every path, so
+   * the record loads are waited for before the staging loads are issued
+   * (a first use in a later branch would wait for those too) */
+  for (int k = 1; k < 4; k++) {
+    asm volatile("" : "+v"(S.ti.X[k]));
+    asm volatile("" : "+v"(S.tf.X[k]));
+  }
+  for (int k = 0; k < 4; k++) {
+    asm volatile("" : "+v"(S.ti.K[k]));
+    asm volatile("" : "+v"(S.tf.K[k]));
+  }
+  asm volatile("" : "+v"(S.ti.dl));
+  asm volatile("" : "+v"(S.tf.dl));
+  const double X1 = S.tf.X[1], X2 = S.tf.X[2], X3 = S.tf.X[3];
+  const bool need = act && Mo.kind != IPGPU_MODEL_SPHERE && in_domain(G, X1, X2);
+  stage_cells(G, dump, need, X1, X2, X3, lds_wave, S);
+}
+
+/* The wave's LDS rows of a packet kernel's block */
+__device__ __forceinline__ double *stage_rows(double *block_lds)
+{
+  return block_lds + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE)) * SG_WAVE_DBL;
+}
+
 /* One lane per transfer step of every ray: the state-independent part of
- * integrate_emission (transport matrices, fluid, coefficients, tetrad). */
-template <bool F32>
+ * integrate_emission (transport matrices, fluid, coefficients, tetrad).
+ * STAGE: the dump gathers staged in LDS as in the packet-list kernels. */
+template <bool F32, bool STAGE>
 __global__ __launch_bounds__(PREP_BLOCK) void k_prep(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P, int nrays,
                                                      int cap, int stride, long long total,
                                                      const long long *__restrict__ off,
@@ -280,18 +420,29 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep(ipgpu_geom G, ipgpu_model M
                                                      const double *__restrict__ dump, double *__restrict__ pkt,
                                                      double *__restrict__ aux)
 {
+  __shared__ double sg_lds[STAGE ? (PREP_BLOCK / WAVE) * SG_WAVE_DBL : 1];
   long long g = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
-  if (g >= total) return;
+  bool act = g < total;
   /* ray = last r with off[r] <= g, from the block's first ray forward */
   int ray = blk_ray[blockIdx.x];
-  while (ray + 1 < nrays && off[ray + 1] <= g) ray++;
+  if (act)
+    while (ray + 1 < nrays && off[ray + 1] <= g) ray++;
   const int ns = nstep_in[ray];
-  if (ns >= P.maxnstep - 1) return;
+  act = act && ns < P.maxnstep - 1;
   const int q = (int)(g - off[ray]);
   const double *rec = traj + (size_t)ray * cap * REC;
   const int *nt = nturns_buf ? nturns_buf + (size_t)ray * cap : nullptr;
-  prep_step<F32>(G, Mo, P, ns - q, rec, nt, dump, pkt + (size_t)g * stride, stride == PK, unit_scale(),
-                 aux ? aux + (size_t)g * AX : nullptr);
+  double *ax = aux ? aux + (size_t)g * AX : nullptr;
+  if constexpr (STAGE) {
+    StagedCells S;
+    stage_step(G, Mo, dump, act, rec, ns - q, traj, stage_rows(sg_lds), S);
+    if (!act) return;
+    prep_step<F32, 1, StagedCells>(G, Mo, P, ns - q, rec, nt, dump, pkt + (size_t)g * stride, stride == PK,
+                                   unit_scale(), ax, S);
+  } else {
+    if (!act) return;
+    prep_step<F32>(G, Mo, P, ns - q, rec, nt, dump, pkt + (size_t)g * stride, stride == PK, unit_scale(), ax);
+  }
 }
 
 /* One lane per ray: the recurrence of N and I over the ray's packets
@@ -1243,8 +1394,10 @@ __global__ void k_blockmap_tiles(int nw, const long long *__restrict__ TT, int *
 }
 
 /* Step packets of an interleaved list: one lane per (step, ray) slot of a
- * tile; slots past a ray's step count are padding and do nothing. */
-template <bool F32>
+ * tile; slots past a ray's step count are padding and do nothing.  STAGE:
+ * the dump gathers staged in LDS per 8-lane group (stage_cells), for which
+ * every lane of a wave stays until its group's loads are issued. */
+template <bool F32, bool STAGE>
 __global__ __launch_bounds__(PREP_BLOCK) void k_prep_list(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
                                                           const int *__restrict__ list, int n,
                                                           const long long *__restrict__ W,
@@ -1256,22 +1409,32 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_list(ipgpu_geom G, ipgpu_mo
                                                           const double *__restrict__ trajO,
                                                           const double *__restrict__ dump, double *__restrict__ pkt)
 {
+  __shared__ double sg_lds[STAGE ? TPB * SG_WAVE_DBL : 1];
   const long long t = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
   const long long tile = t / WAVE;
   const int j = (int)(t % WAVE);
-  if (tile >= tiles) return;
+  if (tile >= tiles) return;   /* a whole wave */
   int c = blk[blockIdx.x];
   while (c + 1 < nw && TT[c + 1] <= tile) c++;
   const int lt = (int)(tile - TT[c]);
   const int l = (lt % (ILV / TL)) * TL + j / TQ;   /* lane of the list wave */
   const int q = (lt / (ILV / TL)) * TQ + j % TQ;   /* step */
   const int k = c * ILV + l;
-  if (k >= n) return;
-  const int ray = list[k];
-  const int ns = nstep_in[ray];
-  if (q >= ns || ns >= P.maxnstep - 1) return;
-  prep_step<F32, ILV>(G, Mo, P, ns - q, ray_rec(traj, cap, rbase, trajO, ray), nullptr, dump,
-                      pkt + (size_t)(W[c] + q) * stride * ILV + pk_lane<ILV>(l), stride == PK, unit_scale());
+  bool act = k < n;
+  const int ray = act ? list[k] : 0;
+  const int ns = act ? nstep_in[ray] : 0;
+  act = act && q < ns && ns < P.maxnstep - 1;
+  const double *rec = act ? ray_rec(traj, cap, rbase, trajO, ray) : nullptr;
+  double *pk = pkt + (size_t)(W[c] + q) * stride * ILV + pk_lane<ILV>(l);
+  if constexpr (STAGE) {
+    StagedCells S;
+    stage_step(G, Mo, dump, act, rec, ns - q, traj, stage_rows(sg_lds), S);
+    if (!act) return;
+    prep_step<F32, ILV, StagedCells>(G, Mo, P, ns - q, rec, nullptr, dump, pk, stride == PK, unit_scale(), nullptr, S);
+  } else {
+    if (!act) return;
+    prep_step<F32, ILV>(G, Mo, P, ns - q, rec, nullptr, dump, pk, stride == PK, unit_scale());
+  }
 }
 
 /* The same packets in the compact layout (ray after ray, packets of one ray
@@ -1286,7 +1449,7 @@ __global__ void k_blockmap_list(int n, const long long *__restrict__ offL, int *
   for (long long b = b0; b < b1; b++) blk[b] = k;
 }
 
-template <bool F32>
+template <bool F32, bool STAGE>
 __global__ __launch_bounds__(PREP_BLOCK) void k_prep_compact(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
                                                              const int *__restrict__ list, int n,
                                                              const long long *__restrict__ offL, long long total,
@@ -1297,33 +1460,54 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_compact(ipgpu_geom G, ipgpu
                                                              const double *__restrict__ trajO,
                                                              const double *__restrict__ dump, double *__restrict__ pkt)
 {
+  __shared__ double sg_lds[STAGE ? TPB * SG_WAVE_DBL : 1];
   long long g = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
-  if (g >= total) return;
+  bool act = g < total;
   int k = blk[blockIdx.x];
-  while (k + 1 < n && offL[k + 1] <= g) k++;
+  if (act)
+    while (k + 1 < n && offL[k + 1] <= g) k++;
   const int ray = list[k];
   const int ns = nstep_in[ray];
-  if (ns >= P.maxnstep - 1) return;
+  act = act && ns < P.maxnstep - 1;
   const int q = (int)(g - offL[k]);
-  prep_step<F32>(G, Mo, P, ns - q, ray_rec(traj, cap, rbase, trajO, ray), nullptr, dump, pkt + (size_t)g * PK,
-                 true, unit_scale());
+  const double *rec = act ? ray_rec(traj, cap, rbase, trajO, ray) : nullptr;
+  if constexpr (STAGE) {
+    StagedCells S;
+    stage_step(G, Mo, dump, act, rec, ns - q, traj, stage_rows(sg_lds), S);
+    if (!act) return;
+    prep_step<F32, 1, StagedCells>(G, Mo, P, ns - q, rec, nullptr, dump, pkt + (size_t)g * PK, true, unit_scale(),
+                                   nullptr, S);
+  } else {
+    if (!act) return;
+    prep_step<F32>(G, Mo, P, ns - q, rec, nullptr, dump, pkt + (size_t)g * PK, true, unit_scale());
+  }
 }
 
 /* Step packets of the first steps of the rays still being traced: n rays
  * (plist), np1 packets each, packet q of ray j (step np1 - q) at
  * pkt + (j np1 + q) PK.  Formed from the records the bounded phases stored,
  * while the unbounded phase extends the same trajectories past them. */
-template <bool F32>
+template <bool F32, bool STAGE>
 __global__ __launch_bounds__(PREP_BLOCK) void k_prep_prefix(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
                                                             const int *__restrict__ plist, int n, int np1, int cap,
                                                             const double *__restrict__ traj,
                                                             const double *__restrict__ dump, double *__restrict__ pkt)
 {
+  __shared__ double sg_lds[STAGE ? TPB * SG_WAVE_DBL : 1];
   const long long g = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
-  if (g >= (long long)n * np1) return;
-  const int j = (int)(g / np1), q = (int)(g % np1);
-  prep_step<F32>(G, Mo, P, np1 - q, traj + (size_t)plist[j] * cap * REC, nullptr, dump, pkt + (size_t)g * PK, true,
-                 unit_scale());
+  const bool act = g < (long long)n * np1;
+  const int j = act ? (int)(g / np1) : 0, q = act ? (int)(g % np1) : 0;
+  const double *rec = act ? traj + (size_t)plist[j] * cap * REC : nullptr;
+  if constexpr (STAGE) {
+    StagedCells S;
+    stage_step(G, Mo, dump, act, rec, np1 - q, traj, stage_rows(sg_lds), S);
+    if (!act) return;
+    prep_step<F32, 1, StagedCells>(G, Mo, P, np1 - q, rec, nullptr, dump, pkt + (size_t)g * PK, true, unit_scale(),
+                                   nullptr, S);
+  } else {
+    if (!act) return;
+    prep_step<F32>(G, Mo, P, np1 - q, rec, nullptr, dump, pkt + (size_t)g * PK, true, unit_scale());
+  }
 }
 
 /* The chains of the rays of a list, one lane per ray: SEG = false the rays
@@ -1778,6 +1962,7 @@ struct ipgpu_ctx {
    * last bounded phase are prepared and composed while the unbounded phase
    * runs (packets d_ppkt, list d_plist, list positions d_pidx) */
   int tail_prefix = 1;
+  int gather_stage = 0;   /* packet kernels stage their dump gathers in LDS */
   double *d_ppkt = nullptr;
   size_t ppkt_doubles = 0;
   int *d_plist = nullptr, *d_pidx = nullptr;
@@ -2284,7 +2469,8 @@ static int prepare_packets(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, bool
     if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)nb))) return rc;
     k_blockmap<<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(nrays, ctx->d_off, ctx->d_blk);
     HIPCHK(ctx, hipGetLastError());
-    auto kp = P.transfer_fp32 ? k_prep<true> : k_prep<false>;
+    auto kp = P.transfer_fp32 ? (ctx->gather_stage ? k_prep<true, true> : k_prep<true, false>)
+                              : (ctx->gather_stage ? k_prep<false, true> : k_prep<false, false>);
     kp<<<nb, PREP_BLOCK, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, (long long)total,
                                                ctx->d_off, ctx->d_blk, ctx->d_nstep_ray, ctx->d_traj,
                                                turns ? ctx->d_nturns : nullptr, ctx->d_dump, ctx->d_pkt,
@@ -2369,7 +2555,8 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
       const unsigned nb = (unsigned)((rows + PREP_BLOCK - 1) / PREP_BLOCK);
       k_blockmap_list<<<(n + 255) / 256, 256, 0, st>>>(n, W, blk);
       HIPCHK(ctx, hipGetLastError());
-      auto kp = P.transfer_fp32 ? k_prep_compact<true> : k_prep_compact<false>;
+      auto kp = P.transfer_fp32 ? (ctx->gather_stage ? k_prep_compact<true, true> : k_prep_compact<true, false>)
+                                : (ctx->gather_stage ? k_prep_compact<false, true> : k_prep_compact<false, false>);
       kp<<<nb, PREP_BLOCK, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, rows, blk, cap, ctx->d_nstep_ray,
                                     ctx->d_traj, rbase, trajO, ctx->d_dump, pkt);
       HIPCHK(ctx, hipGetLastError());
@@ -2395,7 +2582,8 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
     const unsigned nb = (unsigned)((tiles + TPB - 1) / TPB);   /* blk holds nb entries */
     k_blockmap_tiles<<<(nw + 255) / 256, 256, 0, st>>>(nw, TT, blk);
     HIPCHK(ctx, hipGetLastError());
-    auto kp = P.transfer_fp32 ? k_prep_list<true> : k_prep_list<false>;
+    auto kp = P.transfer_fp32 ? (ctx->gather_stage ? k_prep_list<true, true> : k_prep_list<true, false>)
+                              : (ctx->gather_stage ? k_prep_list<false, true> : k_prep_list<false, false>);
     kp<<<nb, PREP_BLOCK, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, TT, nw, tiles, blk, cap, stride,
                                   ctx->d_nstep_ray, ctx->d_traj, rbase, trajO, ctx->d_dump, pkt);
     HIPCHK(ctx, hipGetLastError());
@@ -2771,7 +2959,8 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
         HIPCHK(ctx, hipMemcpyAsync(ctx->d_plist, h_plist, np * sizeof(int), hipMemcpyHostToDevice, sp));
         if ((rc = upload_segments(ctx, S, sp))) return rc;
         const long long slots = (long long)np * np1;
-        auto kp = P.transfer_fp32 ? k_prep_prefix<true> : k_prep_prefix<false>;
+        auto kp = P.transfer_fp32 ? (ctx->gather_stage ? k_prep_prefix<true, true> : k_prep_prefix<true, false>)
+                                  : (ctx->gather_stage ? k_prep_prefix<false, true> : k_prep_prefix<false, false>);
         kp<<<(unsigned)((slots + PREP_BLOCK - 1) / PREP_BLOCK), PREP_BLOCK, 0, sp>>>(
             ctx->geom, ctx->model, P, ctx->d_plist, np, np1, cap, ctx->d_traj, ctx->d_dump, ctx->d_ppkt);
         HIPCHK(ctx, hipGetLastError());
@@ -4116,6 +4305,13 @@ int ipgpu_set_render_memory(ipgpu_ctx *ctx, size_t bytes)
 {
   if (!ctx || bytes < ((size_t)1 << 20)) return set_err(ctx, IPGPU_ERR_ARG, "render memory below 1 MiB");
   ctx->render_bytes = bytes;
+  return IPGPU_OK;
+}
+
+int ipgpu_set_gather_staging(ipgpu_ctx *ctx, int on)
+{
+  if (!ctx) return IPGPU_ERR_ARG;
+  ctx->gather_stage = on ? 1 : 0;
   return IPGPU_OK;
 }
 

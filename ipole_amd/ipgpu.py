@@ -144,6 +144,7 @@ def lib():
         L.ipgpu_autotune_render.argtypes = [vp, C.POINTER(Image), ip]
         L.ipgpu_batch_counts.argtypes = [vp, C.POINTER(C.c_longlong)]
         L.ipgpu_set_render_memory.argtypes = [vp, C.c_size_t]
+        L.ipgpu_set_gather_staging.argtypes = [vp, C.c_int]
         L.ipgpu_render_noalloc.argtypes = [vp, C.POINTER(Image), vp, vp, vp, vp, vp]
         L.ipgpu_render_info.argtypes = [vp, ip, ip, C.POINTER(C.c_size_t)]
         L.ipgpu_adjoint.argtypes = [vp, C.POINTER(Image), dp, dp, dp, dp, dp]
@@ -488,6 +489,11 @@ class Context:
         """Device memory a single-image render may use for ray records and
         step packets; larger images render in chunks of rays."""
         self._chk(self.L.ipgpu_set_render_memory(self.h, int(nbytes)))
+
+    def set_gather_staging(self, on):
+        """Packet kernels read the interpolation's dump cells staged in LDS
+        per 8-step group (True) or straight from the dump (False, default)."""
+        self._chk(self.L.ipgpu_set_gather_staging(self.h, int(bool(on))))
 
     def render_info(self):
         """(chunks, continued rays, device bytes held) of the last render."""

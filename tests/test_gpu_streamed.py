@@ -99,6 +99,47 @@ def test_streamed_tail_prefix(case, seg_len, monkeypatch):
     c3.close()
 
 
+@pytest.mark.parametrize("unpol", [0, 1])
+def test_gather_staging(case, unpol):
+    """Packet kernels reading the interpolation's dump cells staged in LDS
+    per 8-step group: the streamed render (prefix, composed tails, several
+    phases) equals the one-pass render bit for bit under the same setting,
+    matches the oracle at the parity bar, and agrees with the direct-gather
+    render: unpolarized outputs bit for bit, polarized Stokes to roundoff
+    (the two kernel instances fuse the step's multiply-adds differently)."""
+    h, prims, orc = case
+    say("case gather staging unpol=%d 32x32 th=163" % unpol)
+    tune = dict(seg_min_steps=100, seg_len=32)
+    p = O.default_params(nx=32, ny=32, thetacam=163., freqcgs=230e9, only_unpolarized=unpol)
+    c1, g, m = fresh(h, prims, trace_budget=0, **tune)
+    im, scale, _ = G.make_image(g, m.L_unit, **gpu_image_kwargs(p))
+    direct = c1.render(im)
+    c1.set_gather_staging(True)
+    ref = c1.render(im)
+    assert c1.stage_times_mode()[1] is False
+    c1.close()
+    for budget in (64, 300):
+        say("  streamed budget %d" % budget)
+        c2, _, _ = fresh(h, prims, trace_budget=budget, **tune)
+        c2.set_gather_staging(True)
+        got = c2.render(im)
+        assert c2.stage_times_mode()[1] is True
+        same(ref, got)
+        c2.close()
+    for k in ("image", "taus", "nstep", "flag"):
+        assert np.array_equal(ref[k], direct[k]), k
+    if unpol:
+        assert np.array_equal(ref["imageS"], direct["imageS"])
+    else:
+        a, b = ref["imageS"][..., :4], direct["imageS"][..., :4]
+        den = np.maximum(np.abs(b[..., :1]), 1e-300)
+        err = float((np.abs(a - b) / den).max())
+        say("  staged vs direct polarized max rel %.3e" % err)
+        assert err < 1e-9
+        orc.activate()
+        check(orc.render(p), ref, scale)
+
+
 def test_streamed_vs_oracle(case):
     """Small budget (most rays resume) and a low segment threshold (most
     chains composed), at the parity bar against the oracle."""
