@@ -373,6 +373,9 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=2,
                     help="CPU baseline sample: every n-th pixel row (all columns)")
     ap.add_argument("--small", action="store_true", help="small dump for quick checks")
+    ap.add_argument("--gather-staging", action="store_true",
+                    help="the 160x160 render's step-packet kernels read the dump cells staged in LDS "
+                         "(ipgpu_set_gather_staging) instead of straight from the dump")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain device-to-device copy of MIB MiB (the reference "
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -426,6 +429,8 @@ def main():
     im, scale, _ = G.make_image(g, m.L_unit, nx=cfg["nx"], ny=cfg["ny"], thetacam=cfg["thetacam"],
                                 freqcgs=cfg["freqcgs"])
     npix = cfg["nx"] * cfg["ny"]
+    if args.gather_staging:
+        ctx.set_gather_staging(True)
 
     tuned = ctx.autotune_render(im)
     log("autotuned streamed render: %s (%.2f ms)" % (tuned, ctx.kernel_times()[2]))
@@ -501,7 +506,7 @@ def main():
                            radiating_steps=n_rad, ms_device=round(float(np.mean(ms_tot)), 4),
                            ms_geodesic=round(float(np.mean(ms_geo)), 4),
                            ms_transfer=round(float(np.mean(ms_xfer)), 4),
-                           streamed=bool(streamed), autotuned=tuned,
+                           streamed=bool(streamed), autotuned=tuned, gather_staging=bool(args.gather_staging),
                            ms_one_pass=round(float(np.mean(one[1:])), 4)),
                roofline=roof, cpu_baseline=cpu, batched=batched, mixed_precision=precision,
                adjoint=adjoint, adaptive=adaptive)
