@@ -13,6 +13,12 @@
  *   exp_(x)      Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, with ln 2
  *                split in two parts, the Taylor series of e^r to r^13
  *                (truncation < 4e-18), and scaling by 2^k.
+ *
+ * The series are evaluated in Estrin's scheme (terms in pairs, then
+ * combined with the argument's powers r^2, r^4, r^8), whose dependency
+ * depth is about the logarithm of the term count: a dependent FP64 FMA
+ * issues every ~6 cycles on gfx950 against ~4 for independent ones, and
+ * the geodesic, chain and batch-transfer lanes run one wave per SIMD.
  *   sincos_(x)   reduction x = k pi/2 + r with pi/2 in three parts (FMA),
  *                |r| <= pi/4, Taylor series of sin to r^17 and cos to r^18
  *                (truncation < 1e-19), quadrant select by k mod 4.
@@ -113,20 +119,20 @@ IPG_MATH_HD double exp_r(double x)
   double k = rint(x * LOG2E);
   double r = fma(-k, LN2_HI, x);
   r = fma(-k, LN2_LO, r);
-  double p = 1.0 / 6227020800.0;         /* 1/13! */
-  p = fma(p, r, 1.0 / 479001600.0);      /* 1/12! */
-  p = fma(p, r, 1.0 / 39916800.0);       /* 1/11! */
-  p = fma(p, r, 1.0 / 3628800.0);        /* 1/10! */
-  p = fma(p, r, 1.0 / 362880.0);         /* 1/9! */
-  p = fma(p, r, 1.0 / 40320.0);          /* 1/8! */
-  p = fma(p, r, 1.0 / 5040.0);           /* 1/7! */
-  p = fma(p, r, 1.0 / 720.0);            /* 1/6! */
-  p = fma(p, r, 1.0 / 120.0);            /* 1/5! */
-  p = fma(p, r, 1.0 / 24.0);             /* 1/4! */
-  p = fma(p, r, 1.0 / 6.0);              /* 1/3! */
-  p = fma(p, r, 0.5);                    /* 1/2! */
-  p = fma(p, r, 1.0);
-  p = fma(p, r, 1.0);
+  /* q = sum_{n=2}^{13} r^(n-2)/n! in Estrin's scheme (pairs in r, then
+   * in r^2, r^4, r^8: a dependency depth of 5 instead of 12, for the
+   * lanes that run one wave per SIMD and wait on each dependent FMA), then
+   * p = 1 + r (1 + r q) as the last two steps of Horner's rule */
+  const double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
+  const double a0 = fma(1.0 / 6.0, r, 0.5);                       /* 1/2!, 1/3! */
+  const double a1 = fma(1.0 / 120.0, r, 1.0 / 24.0);              /* 1/4!, 1/5! */
+  const double a2 = fma(1.0 / 5040.0, r, 1.0 / 720.0);            /* 1/6!, 1/7! */
+  const double a3 = fma(1.0 / 362880.0, r, 1.0 / 40320.0);        /* 1/8!, 1/9! */
+  const double a4 = fma(1.0 / 39916800.0, r, 1.0 / 3628800.0);    /* 1/10!, 1/11! */
+  const double a5 = fma(1.0 / 6227020800.0, r, 1.0 / 479001600.0);/* 1/12!, 1/13! */
+  const double b0 = fma(a1, r2, a0), b1 = fma(a3, r2, a2), b2 = fma(a5, r2, a4);
+  const double q = fma(b2, r8, fma(b1, r4, b0));
+  const double p = fma(fma(q, r, 1.0), r, 1.0);
   return ldexp(p, (int)k);
 }
 
@@ -146,24 +152,19 @@ IPG_MATH_HD double exp_neg(double x)
 /* sin r and cos r for |r| <= pi/4 */
 IPG_MATH_HD void sincos_reduced(double r, double &s, double &c)
 {
-  double z = r * r;
-  double ps = 1.0 / 355687428096000.0;   /* 1/17! */
-  ps = fma(ps, z, -1.0 / 1307674368000.0);
-  ps = fma(ps, z, 1.0 / 6227020800.0);
-  ps = fma(ps, z, -1.0 / 39916800.0);
-  ps = fma(ps, z, 1.0 / 362880.0);
-  ps = fma(ps, z, -1.0 / 5040.0);
-  ps = fma(ps, z, 1.0 / 120.0);
-  ps = fma(ps, z, -1.0 / 6.0);
+  /* both series in z = r^2 in Estrin's scheme (pairs, then z^2, z^4) */
+  const double z = r * r, z2 = z * z, z4 = z2 * z2;
+  const double s0 = fma(1.0 / 120.0, z, -1.0 / 6.0);                          /* -1/3!, 1/5! */
+  const double s1 = fma(1.0 / 362880.0, z, -1.0 / 5040.0);                    /* -1/7!, 1/9! */
+  const double s2 = fma(1.0 / 6227020800.0, z, -1.0 / 39916800.0);            /* -1/11!, 1/13! */
+  const double s3 = fma(1.0 / 355687428096000.0, z, -1.0 / 1307674368000.0);  /* -1/15!, 1/17! */
+  const double ps = fma(fma(s3, z2, s2), z4, fma(s1, z2, s0));
   s = fma(r * z, ps, r);
-  double pc = -1.0 / 6402373705728000.0; /* -1/18! */
-  pc = fma(pc, z, 1.0 / 20922789888000.0);
-  pc = fma(pc, z, -1.0 / 87178291200.0);
-  pc = fma(pc, z, 1.0 / 479001600.0);
-  pc = fma(pc, z, -1.0 / 3628800.0);
-  pc = fma(pc, z, 1.0 / 40320.0);
-  pc = fma(pc, z, -1.0 / 720.0);
-  pc = fma(pc, z, 1.0 / 24.0);
+  const double c0 = fma(-1.0 / 720.0, z, 1.0 / 24.0);                         /* 1/4!, -1/6! */
+  const double c1 = fma(-1.0 / 3628800.0, z, 1.0 / 40320.0);                  /* 1/8!, -1/10! */
+  const double c2 = fma(-1.0 / 87178291200.0, z, 1.0 / 479001600.0);          /* 1/12!, -1/14! */
+  const double c3 = fma(-1.0 / 6402373705728000.0, z, 1.0 / 20922789888000.0);/* 1/16!, -1/18! */
+  const double pc = fma(fma(c3, z2, c2), z4, fma(c1, z2, c0));
   c = fma(z, fma(z, pc, -0.5), 1.0);
 }
 
@@ -239,18 +240,14 @@ IPG_MATH_HD double log_(double x)
     e -= 1;
   }
   double s = (m - 1.) * rcp(m + 1.);
-  double z = s * s;
-  double p = 1.0 / 23.0;
-  p = fma(p, z, 1.0 / 21.0);
-  p = fma(p, z, 1.0 / 19.0);
-  p = fma(p, z, 1.0 / 17.0);
-  p = fma(p, z, 1.0 / 15.0);
-  p = fma(p, z, 1.0 / 13.0);
-  p = fma(p, z, 1.0 / 11.0);
-  p = fma(p, z, 1.0 / 9.0);
-  p = fma(p, z, 1.0 / 7.0);
-  p = fma(p, z, 1.0 / 5.0);
-  p = fma(p, z, 1.0 / 3.0);
+  /* sum_{n=0}^{10} z^n / (2n + 3) in Estrin's scheme (pairs, then z^2,
+   * z^4, z^8) */
+  const double z = s * s, z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
+  const double a0 = fma(1.0 / 5.0, z, 1.0 / 3.0), a1 = fma(1.0 / 9.0, z, 1.0 / 7.0);
+  const double a2 = fma(1.0 / 13.0, z, 1.0 / 11.0), a3 = fma(1.0 / 17.0, z, 1.0 / 15.0);
+  const double a4 = fma(1.0 / 21.0, z, 1.0 / 19.0);
+  const double b0 = fma(a1, z2, a0), b1 = fma(a3, z2, a2), b2 = fma(1.0 / 23.0, z2, a4);
+  const double p = fma(b2, z8, fma(b1, z4, b0));
   double at2 = 2. * fma(s * z, p, s);
   double de = (double)e;
   return fma(de, LN2_HI, fma(de, LN2_LO, at2));
