@@ -99,8 +99,8 @@ def test_streamed_tail_prefix(case, seg_len, monkeypatch):
     c3.close()
 
 
-@pytest.mark.parametrize("unpol", [0, 1])
-def test_gather_staging(case, unpol):
+@pytest.mark.parametrize("unpol,fp32", [(0, 0), (1, 0), (0, 1)], ids=["pol", "unpol", "pol-mixed"])
+def test_gather_staging(case, unpol, fp32):
     """Packet kernels reading the interpolation's dump cells staged in LDS
     per 8-step group: the streamed render (prefix, composed tails, several
     phases) equals the one-pass render bit for bit under the same setting,
@@ -108,11 +108,11 @@ def test_gather_staging(case, unpol):
     render: unpolarized outputs bit for bit, polarized Stokes to roundoff
     (the two kernel instances fuse the step's multiply-adds differently)."""
     h, prims, orc = case
-    say("case gather staging unpol=%d 32x32 th=163" % unpol)
+    say("case gather staging unpol=%d fp32=%d 32x32 th=163" % (unpol, fp32))
     tune = dict(seg_min_steps=100, seg_len=32)
     p = O.default_params(nx=32, ny=32, thetacam=163., freqcgs=230e9, only_unpolarized=unpol)
     c1, g, m = fresh(h, prims, trace_budget=0, **tune)
-    im, scale, _ = G.make_image(g, m.L_unit, **gpu_image_kwargs(p))
+    im, scale, _ = G.make_image(g, m.L_unit, transfer_fp32=fp32, **gpu_image_kwargs(p))
     direct = c1.render(im)
     c1.set_gather_staging(True)
     ref = c1.render(im)
@@ -126,8 +126,17 @@ def test_gather_staging(case, unpol):
         assert c2.stage_times_mode()[1] is True
         same(ref, got)
         c2.close()
-    for k in ("image", "taus", "nstep", "flag"):
+    for k in ("nstep", "flag"):
         assert np.array_equal(ref[k], direct[k]), k
+    if fp32:
+        # the FP32 fits and Stokes step of the two kernel instances may round
+        # differently at FP32 precision
+        for k in ("image", "taus"):
+            e = np.abs(ref[k] - direct[k]) / np.maximum(np.abs(direct[k]), 1e-300)
+            assert float(e.max()) < 1e-6, (k, float(e.max()))
+    else:
+        for k in ("image", "taus"):
+            assert np.array_equal(ref[k], direct[k]), k
     if unpol:
         assert np.array_equal(ref["imageS"], direct["imageS"])
     else:
@@ -135,9 +144,10 @@ def test_gather_staging(case, unpol):
         den = np.maximum(np.abs(b[..., :1]), 1e-300)
         err = float((np.abs(a - b) / den).max())
         say("  staged vs direct polarized max rel %.3e" % err)
-        assert err < 1e-9
-        orc.activate()
-        check(orc.render(p), ref, scale)
+        assert err < (1e-6 if fp32 else 1e-9)
+        if not fp32:
+            orc.activate()
+            check(orc.render(p), ref, scale)
 
 
 def test_streamed_vs_oracle(case):
