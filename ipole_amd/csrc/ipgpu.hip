@@ -676,6 +676,9 @@ __global__ void k_unpack(size_t ncell, const double *__restrict__ dump, double *
 
 /* ------------------------------------------------------------- batches */
 
+/* Step-count bands of the batch transfer's item order (2^ITEM_BAND steps). */
+constexpr int ITEM_BAND = 8;
+
 /* A trace group: one camera/image geometry traced once for all its jobs. */
 struct BGroup {
   ipgpu_image P;      /* image parameters at the trace frequency */
@@ -3770,8 +3773,15 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
   }
   /* a ray's items: (job, ray) each, or (ray, subset) in multi mode.  Items
    * go longest ray first (the launch's tail is then the short rays); in
-   * multi mode rays within 256 steps of each other have their items ordered
-   * by subset size, so that a wave's lanes mostly run as many jobs */
+   * multi mode rays in the same band of 2^ITEM_BAND steps have their items
+   * ordered by subset size, so that a wave's lanes mostly run as many jobs.
+   * A wave's lanes run in lockstep, each step with the largest subset among
+   * its lanes still running: in a model of the 64-job set (items' step
+   * counts and subset sizes, waves of 64 in this order) that costs 4.7%
+   * over the items' own work with bands of 256 steps and 2.5% with 32, but
+   * measured, 32-step bands made the FP64 transfer slower (357 -> 366 ms):
+   * wider bands keep a wave's items on neighbouring rays of one trace
+   * group, whose records and dump cells the caches then share */
   auto ray_items = [&](int r, int g, int sz, auto &&emit) {
     if (!multi) {
       if (sz == 1)
@@ -3948,7 +3958,7 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     long long w = 0;
     for (long long q0 = 0; q0 < nvalid;) {
       long long q1 = q0;
-      while (q1 < nvalid && (ns_host[order[q1]] >> 8) == (ns_host[order[q0]] >> 8)) q1++;
+      while (q1 < nvalid && (ns_host[order[q1]] >> ITEM_BAND) == (ns_host[order[q0]] >> ITEM_BAND)) q1++;
       for (int sz = max_sz; sz >= 1; sz--)
         for (long long q = q0; q < q1; q++) {
           const int r = order[q];
@@ -4047,7 +4057,7 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     items.clear();
     for (long long q0 = a; q0 < b;) {
       long long q1 = q0;
-      while (q1 < b && (ns_host[order[q1]] >> 8) == (ns_host[order[q0]] >> 8)) q1++;
+      while (q1 < b && (ns_host[order[q1]] >> ITEM_BAND) == (ns_host[order[q0]] >> ITEM_BAND)) q1++;
       for (int sz = max_sz; sz >= 1; sz--)
         for (long long q = q0; q < q1; q++) {
           const int r = order[q];
