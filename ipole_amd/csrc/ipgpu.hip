@@ -2622,17 +2622,25 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
   return 0;
 }
 
-/* Sorts list[0..n) by ns, longest first (counting sort; ties by ray). */
+/* Step-count bands of the streamed render's lists (2^LIST_BAND steps). */
+constexpr int LIST_BAND = 6;
+
+/* Sorts list[0..n) by ns >> band, longest first (counting sort; ties in
+ * list order).  With band > 0 the rays of a wave of 64 are neighbours in the
+ * list's original order (ray index) within a band of similar lengths: the
+ * packet kernel's tiles of 8 rays then follow nearby trajectories through
+ * the same dump cells (on the 160x160 image, 64-step bands cut k_prep_list's
+ * L1 misses by a third and its time by 2%, against the exact order). */
 static void sort_longest_first(int *list, int n, const std::vector<int> &ns, std::vector<int> &cnt,
-                               std::vector<int> &tmp)
+                               std::vector<int> &tmp, int band = 0)
 {
   int mx = 0;
-  for (int k = 0; k < n; k++) mx = ns[list[k]] > mx ? ns[list[k]] : mx;
+  for (int k = 0; k < n; k++) mx = (ns[list[k]] >> band) > mx ? (ns[list[k]] >> band) : mx;
   cnt.assign((size_t)mx + 2, 0);
-  for (int k = 0; k < n; k++) cnt[mx - ns[list[k]] + 1]++;
+  for (int k = 0; k < n; k++) cnt[mx - (ns[list[k]] >> band) + 1]++;
   for (int v = 1; v <= mx + 1; v++) cnt[v] += cnt[v - 1];
   tmp.resize(n);
-  for (int k = 0; k < n; k++) tmp[cnt[mx - ns[list[k]]]++] = list[k];
+  for (int k = 0; k < n; k++) tmp[cnt[mx - (ns[list[k]] >> band)]++] = list[k];
   for (int k = 0; k < n; k++) list[k] = tmp[k];
 }
 
@@ -2853,9 +2861,9 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
      * budget) keep each ray's packets contiguous (offsets in W); the others
      * are cut into waves of 64 with wave-interleaved packets (rows per wave
      * in W). */
-    sort_longest_first(h_list + start, n, nsv, ctx->h_cnt, ctx->h_tmp);
     bool compact = (k == K) && segm && n > 0;
     for (int q = 0; compact && q < n; q++) compact = nsv[h_list[start + q]] > seg_min;
+    sort_longest_first(h_list + start, n, nsv, ctx->h_cnt, ctx->h_tmp, compact ? 0 : LIST_BAND);
     /* the rays' first Ppre steps were composed ahead: their packets here are
      * those of the other steps */
     const bool use_pre = pre_launched && compact;
@@ -2871,7 +2879,12 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
     if (compact)
       for (int q = 0; q < n; q++) W[q + 1] = W[q] + nsv[h_list[start + q]] - covered;
     else
-      for (int c = 0; c < nw; c++) W[c + 1] = W[c] + nsv[h_list[start + c * ILV]];
+      for (int c = 0; c < nw; c++) {
+        /* the wave's rows: its longest ray's steps */
+        int mx = 0;
+        for (int q = c * ILV; q < n && q < (c + 1) * ILV; q++) mx = std::max(mx, nsv[h_list[start + q]]);
+        W[c + 1] = W[c] + mx;
+      }
     const long long rows = W[nw];
     const long long T = compact ? rows : rows * ILV;   /* packet slots, padding included */
     long long *TT = W + (nI + MAXPH + 1);
