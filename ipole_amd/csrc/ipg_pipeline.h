@@ -1161,13 +1161,10 @@ IPG_HD_INLINE void nst_store(double *nst, int stride, int j, const Herm &N)
 
 template <bool F32 = false>
 IPG_HD_INLINE void fused_step_multi(const ipgpu_geom &G, const ipgpu_model &Mo, int nj, const JobCfg *cfg,
-                                    int target_nturns, int n, const double *rec, const int *nt,
+                                    int target_nturns, int n, const Rec &ti, const Rec &tf, const int *nt,
                                     const double *dump, JobAcc *acc, double *nst, int stride, double (*Mi)[4],
                                     bool any_pol)
 {
-  Rec ti, tf;
-  load_rec(rec, n, ti);
-  load_rec(rec, n - 1, tf);
   const double conv = Mo.L_unit * HPL / (ME * CL * CL);
   Metric m;
   metric(G, tf.X[1], tf.X[2], m);
@@ -1349,8 +1346,22 @@ IPG_HD_INLINE void transfer_fused_multi(const ipgpu_geom &G, const ipgpu_model &
   }
   double(*Mi)[4] = reinterpret_cast<double(*)[4]>(mi);
   const int tn = P[0]->target_nturns;
-  for (int q = 0; q < ns; q++)
-    fused_step_multi<F32>(G, Mo, nj, cfg, tn, ns - q, rec, nt, dump, acc, nst, stride, Mi, any_pol);
+  /* a step's end record is loaded one step ahead (the records of step n:
+   * n at its start, n - 1 at its end), so that its latency hides behind the
+   * step before (mixed transfer -2.5%); index 0 stands in past the last
+   * step.  (In transfer_fused the carried record makes the mixed-precision
+   * kernels spill, so it loads its records in the step.) */
+  Rec ti, tf;
+  load_rec(rec, ns, ti);
+  load_rec(rec, ns >= 1 ? ns - 1 : 0, tf);
+  for (int q = 0; q < ns; q++) {
+    const int n = ns - q;
+    Rec ahead;
+    load_rec(rec, n >= 2 ? n - 2 : 0, ahead);
+    fused_step_multi<F32>(G, Mo, nj, cfg, tn, n, ti, tf, nt, dump, acc, nst, stride, Mi, any_pol);
+    ti = tf;
+    tf = ahead;
+  }
   IPG_UNROLL_MJ
   for (int j = 0; j < MJ; j++) {
     if (j >= nj) break;
