@@ -645,6 +645,21 @@ IPG_HD void gather10(const ipgpu_geom &G, const double *dump, const Cell &c, dou
   interp10(c, [&](int n, int f) { return p[o[n] + f]; }, v);
 }
 
+/* The cell records' 10 fields rounded to FP32, 16 floats (64 bytes) per
+ * cell: what the mixed-precision batch transfer gathers (half the bytes and
+ * registers of the FP64 records; the interpolation stays FP64). */
+constexpr int CELL32 = 16;
+
+/* gather10 over the FP32 records (the same corners and interpolation). */
+IPG_HD void gather10_f32(const ipgpu_geom &G, const float *d32, const Cell &c, double v[10])
+{
+  const long s2 = (G.N3 + 2), s1 = (long)(G.N2 + 2) * (G.N3 + 2);
+  const float *p = d32 + CELL32 * ((long)c.i * s1 + (long)c.j * s2 + c.k);
+  const long o[8] = {0, CELL32, CELL32 * s2, CELL32 * (s2 + 1), CELL32 * s1,
+                     CELL32 * (s1 + 1), CELL32 * (s1 + s2), CELL32 * (s1 + s2 + 1)};
+  interp10(c, [&](int n, int f) { return (double)p[o[n] + f]; }, v);
+}
+
 /* model/iharm/model.c:567-575 */
 IPG_HD double sigma_smoothfac(const ipgpu_model &M, double sigma)
 {

@@ -944,6 +944,24 @@ IPG_HD int chain_ray_segmented(const ipgpu_geom &G, const ipgpu_image &P, int ns
   return flag;
 }
 
+/* The fluid state at a fused transfer step: the mixed-precision instance
+ * (F32) with d32 given gathers the FP32 copy of the cell records, any other
+ * the FP64 records. */
+template <bool F32>
+IPG_HD_INLINE void fluid_step(const ipgpu_geom &G, const ipgpu_model &Mo, const double *dump, const float *d32,
+                              const double X[4], const Metric &m, Fluid &F)
+{
+  double v[10];
+  bool pre = false;
+  if (F32 && d32 && Mo.kind != IPGPU_MODEL_SPHERE && in_domain(G, X[1], X[2])) {
+    Cell c;
+    locate(G, X[1], X[2], X[3], c);
+    gather10_f32(G, d32, c, v);
+    pre = true;
+  }
+  fluid(G, Mo, dump, X, m, F, nullptr, v, pre);
+}
+
 /* One transfer step fused: the work of prep_step and chain_step in the
  * order that keeps the fewest values live, with nothing written to memory.
  * The same operations on the same values as the split path; the half-step
@@ -952,7 +970,7 @@ IPG_HD int chain_ray_segmented(const ipgpu_geom &G, const ipgpu_image &P, int ns
 template <bool F32 = false>
 IPG_HD_INLINE void fused_step(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int n,
                               const double *rec, const int *nt, const double *dump, const Scale &sc,
-                              bool pol, Chain &st, double (*Mi)[4])
+                              bool pol, Chain &st, double (*Mi)[4], const float *d32)
 {
   Rec ti, tf;
   load_rec(rec, n, ti);
@@ -994,7 +1012,7 @@ IPG_HD_INLINE void fused_step(const ipgpu_geom &G, const ipgpu_model &Mo, const 
       if (tf.X[2] > half_x2) zero_emission = true;
     }
     Fluid F;
-    fluid(G, Mo, dump, tf.X, m, F);
+    fluid_step<F32>(G, Mo, dump, d32, tf.X, m, F);
     F.ne *= sc.ne;
     F.B *= sc.b;
     Coef c;
@@ -1099,7 +1117,7 @@ IPG_HD_INLINE void fused_step(const ipgpu_geom &G, const ipgpu_model &Mo, const 
 template <bool F32 = false>
 IPG_HD_INLINE int transfer_fused(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int ns,
                           const double *rec, const int *nt, const double *dump, const Scale &sc,
-                          double out7[7], int &nrad, double *mi = nullptr)
+                          double out7[7], int &nrad, double *mi = nullptr, const float *d32 = nullptr)
 {
   const bool pol = !P.only_unpolarized;
   double js, ks;
@@ -1107,7 +1125,7 @@ IPG_HD_INLINE int transfer_fused(const ipgpu_geom &G, const ipgpu_model &Mo, con
   Chain st;
   chain_init(st, js, ks);
   double(*Mi)[4] = mi ? reinterpret_cast<double(*)[4]>(mi) : st.Mi;
-  for (int q = 0; q < ns; q++) fused_step<F32>(G, Mo, P, ns - q, rec, nt, dump, sc, pol, st, Mi);
+  for (int q = 0; q < ns; q++) fused_step<F32>(G, Mo, P, ns - q, rec, nt, dump, sc, pol, st, Mi, d32);
   nrad = st.nrad;
   return chain_finish(G, P, st, out7);
 }
@@ -1163,7 +1181,7 @@ template <bool F32 = false>
 IPG_HD_INLINE void fused_step_multi(const ipgpu_geom &G, const ipgpu_model &Mo, int nj, const JobCfg *cfg,
                                     int target_nturns, int n, const Rec &ti, const Rec &tf, const int *nt,
                                     const double *dump, JobAcc *acc, double *nst, int stride, double (*Mi)[4],
-                                    bool any_pol)
+                                    bool any_pol, const float *d32)
 {
   const double conv = Mo.L_unit * HPL / (ME * CL * CL);
   Metric m;
@@ -1196,7 +1214,7 @@ IPG_HD_INLINE void fused_step_multi(const ipgpu_geom &G, const ipgpu_model &Mo, 
   bool anyNnz = false;
   if (radiating_region(G, Mo, m.r, m.th)) {
     Fluid F;
-    fluid(G, Mo, dump, tf.X, m, F);
+    fluid_step<F32>(G, Mo, dump, d32, tf.X, m, F);
     const Geo q0 = F.ne > 0. ? coef_geometry(F, tf.K) : Geo{0., 1., 0., false};
     double E1[4], E2[4], e1[4], e2[4];
     int tflag = 0;
@@ -1303,7 +1321,7 @@ template <bool F32 = false>
 IPG_HD_INLINE void transfer_fused_multi(const ipgpu_geom &G, const ipgpu_model &Mo, int nj, const JobCfg *cfg,
                                         const ipgpu_image *const *P, int ns, const double *rec, const int *nt,
                                         const double *dump, double *out7, int *flags, int *nrad, double *mi,
-                                        double *nst, int stride)
+                                        double *nst, int stride, const float *d32 = nullptr)
 {
   JobAcc acc[MJ];
   bool any_pol = false;
@@ -1358,7 +1376,7 @@ IPG_HD_INLINE void transfer_fused_multi(const ipgpu_geom &G, const ipgpu_model &
     const int n = ns - q;
     Rec ahead;
     load_rec(rec, n >= 2 ? n - 2 : 0, ahead);
-    fused_step_multi<F32>(G, Mo, nj, cfg, tn, n, ti, tf, nt, dump, acc, nst, stride, Mi, any_pol);
+    fused_step_multi<F32>(G, Mo, nj, cfg, tn, n, ti, tf, nt, dump, acc, nst, stride, Mi, any_pol, d32);
     ti = tf;
     tf = ahead;
   }

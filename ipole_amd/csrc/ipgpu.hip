@@ -851,7 +851,8 @@ __device__ __forceinline__ void xfer_item(const ipgpu_geom &G, const ipgpu_model
                                           double *__restrict__ image, double *__restrict__ imageS,
                                           double *__restrict__ taus, int *__restrict__ nstep_pix,
                                           int *__restrict__ flag_pix, unsigned long long *__restrict__ bstatus,
-                                          unsigned long long *__restrict__ counts, double *mi)
+                                          unsigned long long *__restrict__ counts, double *mi,
+                                          const float *__restrict__ d32)
 {
   const int2 it = items[w];
   const int jb = it.x, r = it.y;
@@ -878,7 +879,7 @@ __device__ __forceinline__ void xfer_item(const ipgpu_geom &G, const ipgpu_model
   const int *nt = (nturns_buf && P.target_nturns >= 0) ? nturns_buf + ro : nullptr;
   double o[7];
   int nrad;
-  int flag = transfer_fused<F32>(G, Mo, P, ns, rec, nt, dump, J.sc, o, nrad, mi);
+  int flag = transfer_fused<F32>(G, Mo, P, ns, rec, nt, dump, J.sc, o, nrad, mi, d32);
   const double f3 = P.freqcgs * P.freqcgs * P.freqcgs;
   image[p] = o[0] * f3;
   taus[p] = o[5];
@@ -912,7 +913,8 @@ __global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model M
                                                      double *__restrict__ imageS, double *__restrict__ taus,
                                                      int *__restrict__ nstep_pix, int *__restrict__ flag_pix,
                                                      unsigned long long *__restrict__ bstatus,
-                                                     unsigned long long *__restrict__ counts)
+                                                     unsigned long long *__restrict__ counts,
+                                                     const float *__restrict__ d32)
 {
   /* FP64: the transport matrix carried between steps lives in LDS (one
    * padded slot per lane), which keeps the step within the register file */
@@ -920,7 +922,7 @@ __global__ __launch_bounds__(WAVE) void k_xfer_batch(ipgpu_geom G, ipgpu_model M
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nitems) return;
   xfer_item<F32>(G, Mo, jobs, groups, items, w, off, traj, traj2, nstep_in, nturns_buf, dump, image, imageS, taus,
-                 nstep_pix, flag_pix, bstatus, counts, F32 ? nullptr : mi_lds + 17 * threadIdx.x);
+                 nstep_pix, flag_pix, bstatus, counts, F32 ? nullptr : mi_lds + 17 * threadIdx.x, d32);
 }
 
 /* The same work as a persistent launch with a work queue: a fixed number of
@@ -941,7 +943,8 @@ __global__ __launch_bounds__(WAVE) void k_xfer_queue(ipgpu_geom G, ipgpu_model M
                                                      double *__restrict__ imageS, double *__restrict__ taus,
                                                      int *__restrict__ nstep_pix, int *__restrict__ flag_pix,
                                                      unsigned long long *__restrict__ bstatus,
-                                                     unsigned long long *__restrict__ counts)
+                                                     unsigned long long *__restrict__ counts,
+                                                     const float *__restrict__ d32)
 {
   __shared__ double mi_lds[F32 ? 1 : WAVE * 17];
   __shared__ unsigned int ticket;
@@ -955,7 +958,7 @@ __global__ __launch_bounds__(WAVE) void k_xfer_queue(ipgpu_geom G, ipgpu_model M
     const int w = (int)(c * WAVE + threadIdx.x);
     if (w < nitems)
       xfer_item<F32>(G, Mo, jobs, groups, items, w, off, traj, traj2, nstep_in, nturns_buf, dump, image, imageS,
-                     taus, nstep_pix, flag_pix, bstatus, counts, F32 ? nullptr : mi_lds + 17 * threadIdx.x);
+                     taus, nstep_pix, flag_pix, bstatus, counts, F32 ? nullptr : mi_lds + 17 * threadIdx.x, d32);
   }
 }
 
@@ -969,7 +972,7 @@ __device__ __forceinline__ void xfer_multi_item(const ipgpu_geom &G, const ipgpu
                                                 const double *dump, double *image, double *imageS, double *taus,
                                                 int *nstep_pix, int *flag_pix, unsigned long long *bstatus,
                                                 unsigned long long *counts, unsigned long long *bcounts, double *mi,
-                                                double *nst)
+                                                double *nst, const float *d32)
 {
   const int2 it = items[w];
   const int r = it.x;
@@ -1015,7 +1018,7 @@ __device__ __forceinline__ void xfer_multi_item(const ipgpu_geom &G, const ipgpu
   }
   double o[7 * MJ];
   int fl[MJ], nr[MJ];
-  transfer_fused_multi<F32>(G, Mo, nj, cfg, Pp, ns, rec, nt, dump, o, fl, nr, mi, nst, WAVE);
+  transfer_fused_multi<F32>(G, Mo, nj, cfg, Pp, ns, rec, nt, dump, o, fl, nr, mi, nst, WAVE, d32);
   unsigned long long nrad = 0;
 #pragma unroll
   for (int j = 0; j < MJ; j++) {
@@ -1063,7 +1066,8 @@ __global__ __launch_bounds__(WAVE) void k_xfer_multi(ipgpu_geom G, ipgpu_model M
                                                      int *__restrict__ nstep_pix, int *__restrict__ flag_pix,
                                                      unsigned long long *__restrict__ bstatus,
                                                      unsigned long long *__restrict__ counts,
-                                                     unsigned long long *__restrict__ bcounts)
+                                                     unsigned long long *__restrict__ bcounts,
+                                                     const float *__restrict__ d32)
 {
   __shared__ double mi_lds[WAVE * 17];
   __shared__ double nst_lds[MJ * 16 * WAVE];
@@ -1074,7 +1078,7 @@ __global__ __launch_bounds__(WAVE) void k_xfer_multi(ipgpu_geom G, ipgpu_model M
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nitems) return;
     xfer_multi_item<F32>(G, Mo, jobs, groups, items, subsets, w, off, traj, traj2, nstep_in, nturns_buf, dump,
-                         image, imageS, taus, nstep_pix, flag_pix, bstatus, counts, bcounts, mi, nst);
+                         image, imageS, taus, nstep_pix, flag_pix, bstatus, counts, bcounts, mi, nst, d32);
     return;
   }
   const unsigned int nchunks = (unsigned int)((nitems + WAVE - 1) / WAVE);
@@ -1087,8 +1091,18 @@ __global__ __launch_bounds__(WAVE) void k_xfer_multi(ipgpu_geom G, ipgpu_model M
     const int w = (int)(c * WAVE + threadIdx.x);
     if (w < nitems)
       xfer_multi_item<F32>(G, Mo, jobs, groups, items, subsets, w, off, traj, traj2, nstep_in, nturns_buf, dump,
-                           image, imageS, taus, nstep_pix, flag_pix, bstatus, counts, bcounts, mi, nst);
+                           image, imageS, taus, nstep_pix, flag_pix, bstatus, counts, bcounts, mi, nst, d32);
   }
+}
+
+/* The FP32 copy of the cell records the mixed-precision batch transfer
+ * gathers: the 10 fields of each of the nc cells rounded to nearest, the
+ * other 6 floats of its 64 bytes zero. */
+__global__ void k_dump_f32(long long nc, const double *__restrict__ d, float *__restrict__ o)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  for (int f = 0; f < CELL32; f++) o[i * CELL32 + f] = f < 10 ? (float)d[i * CELL_STRIDE + f] : 0.f;
 }
 
 /* queue[0] = 0; with bstatus, both status words to "none"; with counts,
@@ -1880,6 +1894,8 @@ struct ipgpu_ctx {
   ipgpu_model model{};
   bool has_model = false;
   double *d_dump = nullptr;
+  float *d_dump32 = nullptr;        /* FP32 copy of the records (mixed batch transfer) */
+  size_t dump32_cells = 0;
   size_t dump_cells = 0;
   double *d_traj = nullptr;
   size_t traj_doubles = 0;
@@ -2040,6 +2056,7 @@ struct ipgpu_ctx {
     ipgpu_geom geom;
     ipgpu_model model;
     const double *dump;
+    const float *dump32;
     int queue, waves;
   } bkey;
   BJob *h_bjobs = nullptr;          /* pinned job table the graph uploads */
@@ -2270,7 +2287,7 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
     for (void *p : sp)
       if (p) hipFree(p);
   }
-  void *ptrs[] = {ctx->d_live, ctx->d_list, ctx->d_blk2, ctx->d_offl, ctx->d_pkt2, ctx->d_dump, ctx->d_traj, ctx->d_pkt, ctx->d_aux, ctx->d_off, ctx->d_blk, ctx->d_segbase, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
+  void *ptrs[] = {ctx->d_live, ctx->d_list, ctx->d_blk2, ctx->d_offl, ctx->d_pkt2, ctx->d_dump, ctx->d_dump32, ctx->d_traj, ctx->d_pkt, ctx->d_aux, ctx->d_off, ctx->d_blk, ctx->d_segbase, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
                   ctx->d_out7, ctx->d_image, ctx->d_imageS, ctx->d_taus, ctx->d_nstep_pix,
                   ctx->d_flag_pix, ctx->d_status, ctx->d_counts,
                   ctx->d_groups, ctx->d_jobs, ctx->d_items, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
@@ -2337,6 +2354,24 @@ static int ensure_dump(ipgpu_ctx *ctx)
   HIPCHK(ctx, hipMalloc((void **)&ctx->d_dump, nc * CELL_STRIDE * sizeof(double)));
   ctx->dump_cells = nc;
   ctx->dev_bytes += nc * CELL_STRIDE * sizeof(double);
+  return 0;
+}
+
+/* The FP32 copy of the cell records for a mixed-precision batch (its
+ * contents are written before each transfer launch that reads it). */
+static int ensure_dump32(ipgpu_ctx *ctx)
+{
+  if (!ctx->d_dump || ctx->model.kind != IPGPU_MODEL_IHARM) return 0;
+  if (ctx->d_dump32 && ctx->dump32_cells == ctx->dump_cells) return 0;
+  if (ctx->d_dump32) {
+    HIPCHK(ctx, hipFree(ctx->d_dump32));
+    ctx->dev_bytes -= ctx->dump32_cells * CELL32 * sizeof(float);
+  }
+  ctx->d_dump32 = nullptr;
+  ctx->dump32_cells = 0;
+  HIPCHK(ctx, hipMalloc((void **)&ctx->d_dump32, ctx->dump_cells * CELL32 * sizeof(float)));
+  ctx->dump32_cells = ctx->dump_cells;
+  ctx->dev_bytes += ctx->dump_cells * CELL32 * sizeof(float);
   return 0;
 }
 
@@ -3662,6 +3697,14 @@ static int batch_transfer(ipgpu_ctx *ctx, bool f32, bool multi, int nitems, cons
     HIPCHK(ctx, hipGetLastError());
   }
   if (nitems <= 0) return 0;
+  /* mixed precision: the FP32 copy of the records, from the dump as it is */
+  const float *d32 = nullptr;
+  if (f32 && ctx->d_dump32 && ctx->dump32_cells == ctx->dump_cells && ctx->model.kind == IPGPU_MODEL_IHARM) {
+    const long long nc = (long long)ctx->dump_cells;
+    k_dump_f32<<<(unsigned)((nc + 255) / 256), 256, 0, st>>>(nc, ctx->d_dump, ctx->d_dump32);
+    HIPCHK(ctx, hipGetLastError());
+    d32 = ctx->d_dump32;
+  }
   if (multi) {
     /* items (ray, job subset) */
     auto km = f32 ? k_xfer_multi<true> : k_xfer_multi<false>;
@@ -3670,7 +3713,7 @@ static int batch_transfer(ipgpu_ctx *ctx, bool f32, bool multi, int nitems, cons
     km<<<grid, WAVE, 0, st>>>(ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items, ctx->d_subsets,
                               nitems, ctx->batch_queue ? ctx->d_queue : nullptr, ctx->d_off, ctx->d_traj, traj2,
                               ctx->d_nstep_ray, nturns, ctx->d_dump, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
-                              ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts, ctx->d_bcounts);
+                              ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts, ctx->d_bcounts, d32);
     HIPCHK(ctx, hipGetLastError());
     return 0;
   }
@@ -3679,13 +3722,15 @@ static int batch_transfer(ipgpu_ctx *ctx, bool f32, bool multi, int nitems, cons
     auto kq = f32 ? k_xfer_queue<true> : k_xfer_queue<false>;
     kq<<<nw, WAVE, 0, st>>>(ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items, nitems, ctx->d_queue,
                             ctx->d_off, ctx->d_traj, traj2, ctx->d_nstep_ray, nturns, ctx->d_dump, ctx->d_bimage,
-                            ctx->d_bimageS, ctx->d_btaus, ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts);
+                            ctx->d_bimageS, ctx->d_btaus, ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts,
+                            d32);
   } else {
     auto kx = f32 ? k_xfer_batch<true> : k_xfer_batch<false>;
     kx<<<(nitems + WAVE - 1) / WAVE, WAVE, 0, st>>>(ctx->geom, ctx->model, ctx->d_jobs, ctx->d_groups, ctx->d_items,
                                                     nitems, ctx->d_off, ctx->d_traj, traj2, ctx->d_nstep_ray, nturns,
                                                     ctx->d_dump, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
-                                                    ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts);
+                                                    ctx->d_bnstep, ctx->d_bflag, ctx->d_bstatus, ctx->d_counts,
+                                                    d32);
   }
   HIPCHK(ctx, hipGetLastError());
   return 0;
@@ -3725,6 +3770,7 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
   HIPCHK(ctx, hipSetDevice(ctx->device));
   int rc;
   batch_evict(ctx);
+  if (jobs[0].image.transfer_fp32 && (rc = ensure_dump32(ctx))) return rc;
   /* trace groups */
   std::vector<BGroup> groups;
   std::vector<BJob> bj(njobs);
@@ -4206,6 +4252,10 @@ int ipgpu_rerender_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     b.sc.b = sqrt(jobs[j].m_unit_scale);
     ctx->h_bjobs[j] = b;
   }
+  if (L.f32) {
+    int rc0 = ensure_dump32(ctx);   /* before any capture: it may allocate */
+    if (rc0) return rc0;
+  }
   const int nw = ctx->batch_queue_waves > 0 ? ctx->batch_queue_waves
                                             : (L.multi ? ctx->xm_occ : ctx->xq_occ)[L.f32 ? 1 : 0] * ctx->ncu;
   ipgpu_ctx::GraphKey key;
@@ -4213,6 +4263,7 @@ int ipgpu_rerender_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
   key.geom = ctx->geom;
   key.model = ctx->model;
   key.dump = ctx->d_dump;
+  key.dump32 = L.f32 ? ctx->d_dump32 : nullptr;
   key.queue = ctx->batch_queue;
   key.waves = nw;
   if (ctx->bgraph_exec && memcmp(&key, &ctx->bkey, sizeof(key)) != 0) drop_batch_graph(ctx);

@@ -11,6 +11,19 @@
 
 using namespace ipg;
 
+/* The FP32 copy of the cell records that the library's mixed-precision
+ * batch transfer gathers (k_dump_f32 on the device). */
+static std::vector<float> dump_f32(const ipgpu_geom &G, const ipgpu_model &M, const double *dump)
+{
+  std::vector<float> o;
+  if (M.kind != IPGPU_MODEL_IHARM || !dump) return o;
+  const size_t nc = (size_t)(G.N1 + 2) * (G.N2 + 2) * (G.N3 + 2);
+  o.assign(nc * CELL32, 0.f);
+  for (size_t i = 0; i < nc; i++)
+    for (int f = 0; f < 10; f++) o[i * CELL32 + f] = (float)dump[i * CELL_STRIDE + f];
+  return o;
+}
+
 extern "C" {
 
 /* metric + derivatives + inverse at (X1, X2): 42 doubles (r th g gc d1 d2 sqrtmg) */
@@ -177,6 +190,8 @@ int emu_render_scaled(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_ima
   sc.ikf = 1. / sc.kf;
   sc.ne = ne_s;
   sc.b = sqrt(ne_s);
+  const std::vector<float> d32 = Pjob->transfer_fp32 ? dump_f32(*G, *M, dump) : std::vector<float>();
+  const float *d32p = d32.empty() ? nullptr : d32.data();
 #pragma omp parallel
   {
     std::vector<double> rec((size_t)Ptrace->maxnstep * REC);
@@ -194,7 +209,8 @@ int emu_render_scaled(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_ima
       }
       int nrad;
       flag[n] = Pjob->transfer_fp32
-          ? transfer_fused<true>(*G, *M, *Pjob, ns, rec.data(), ntp, dump, sc, out7 + 7 * (size_t)n, nrad)
+          ? transfer_fused<true>(*G, *M, *Pjob, ns, rec.data(), ntp, dump, sc, out7 + 7 * (size_t)n, nrad, nullptr,
+                                 d32p)
           : transfer_fused<false>(*G, *M, *Pjob, ns, rec.data(), ntp, dump, sc, out7 + 7 * (size_t)n, nrad);
     }
   }
@@ -222,6 +238,8 @@ int emu_render_multi(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_imag
     Pp[j] = &Pjobs[j];
   }
   const bool f32 = Pjobs[0].transfer_fp32 != 0;
+  const std::vector<float> d32 = f32 ? dump_f32(*G, *M, dump) : std::vector<float>();
+  const float *d32p = d32.empty() ? nullptr : d32.data();
   int status = 0;
 #pragma omp parallel
   {
@@ -242,7 +260,7 @@ int emu_render_multi(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_imag
       int nrad[MJ];
       if (f32)
         transfer_fused_multi<true>(*G, *M, nj, cfg, Pp, ns, rec.data(), ntp, dump, out7 + 7 * (size_t)nj * n,
-                                   flag + (size_t)nj * n, nrad, mi, nst, 1);
+                                   flag + (size_t)nj * n, nrad, mi, nst, 1, d32p);
       else
         transfer_fused_multi<false>(*G, *M, nj, cfg, Pp, ns, rec.data(), ntp, dump, out7 + 7 * (size_t)nj * n,
                                     flag + (size_t)nj * n, nrad, mi, nst, 1);
