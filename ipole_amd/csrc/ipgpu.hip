@@ -3997,6 +3997,24 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
         off_host[r] = (long long)r * slot;
       }
     }
+    /* the long rays' continuation goes out first: the host builds the
+     * transfer's items while it runs */
+    if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)(nlong > 0 ? nlong : 1)))) return rc;
+    if ((rc = grow(ctx, ctx->d_off2, ctx->cap_off2, (size_t)(nlong > 0 ? nlong : 1)))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_off, off_host.data(), rays_total * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    if (nlong > 0) {
+      HIPCHK(ctx, hipMemcpyAsync(ctx->d_blk, order.data(), nlong * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+      HIPCHK(ctx, hipMemcpyAsync(ctx->d_off2, off2.data(), nlong * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
+      k_slot_copy<<<1024, 256, 0, ctx->stream>>>(ctx->d_blk, (int)nlong, ctx->d_off2, slot, ctx->d_traj, ctx->d_traj2);
+      HIPCHK(ctx, hipGetLastError());
+      k_trace_resume_list<<<(unsigned)((2 * nlong + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>(
+          ctx->geom, ctx->d_groups, ngroups, ctx->d_blk, (int)nlong, ctx->d_off2, slot, ctx->d_nstep_ray,
+          ctx->d_traj2, ctx->d_status);
+      HIPCHK(ctx, hipGetLastError());
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     /* the (job, ray) items, built in pinned host memory (the upload is then
      * a direct copy); every ray's group from a table of the groups' ranges */
     std::vector<int> &gid = ctx->h_gid;
@@ -4047,22 +4065,6 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
       ctx->bstat[4] += (long long)ns_host[r] * nj;
     }
     if ((rc = grow(ctx, ctx->d_items, ctx->cap_items, (size_t)(nitems > 0 ? nitems : 1)))) return rc;
-    if ((rc = grow(ctx, ctx->d_blk, ctx->cap_blk, (size_t)(nlong > 0 ? nlong : 1)))) return rc;
-    if ((rc = grow(ctx, ctx->d_off2, ctx->cap_off2, (size_t)(nlong > 0 ? nlong : 1)))) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(ctx->d_off, off_host.data(), rays_total * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
-    HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    if (nlong > 0) {
-      HIPCHK(ctx, hipMemcpyAsync(ctx->d_blk, order.data(), nlong * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-      HIPCHK(ctx, hipMemcpyAsync(ctx->d_off2, off2.data(), nlong * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
-      k_slot_copy<<<1024, 256, 0, ctx->stream>>>(ctx->d_blk, (int)nlong, ctx->d_off2, slot, ctx->d_traj, ctx->d_traj2);
-      HIPCHK(ctx, hipGetLastError());
-      k_trace_resume_list<<<(unsigned)((2 * nlong + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>(
-          ctx->geom, ctx->d_groups, ngroups, ctx->d_blk, (int)nlong, ctx->d_off2, slot, ctx->d_nstep_ray,
-          ctx->d_traj2, ctx->d_status);
-      HIPCHK(ctx, hipGetLastError());
-    }
-    HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, hit, nitems * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
     if ((rc = batch_transfer(ctx, jobs[0].image.transfer_fp32 != 0, multi, nitems, nlong > 0 ? ctx->d_traj2 : nullptr,
