@@ -152,11 +152,13 @@ def test_batch_maxnstep(setup):
     assert e.value.code == G.ERR_MAXNSTEP and "job 1" in str(e.value)
 
 
-def batch_jobs_of(g, m, scales=None):
+def batch_jobs_of(g, m, scales=None, fp32=0):
     jobs = []
     for k, (th, nx, ny, f, s, unpol, et) in enumerate(JOBS):
         p = O.default_params(nx=nx, ny=ny, thetacam=th, freqcgs=f, only_unpolarized=unpol, emission_type=et)
-        im, _, _ = G.make_image(g, m.L_unit, **gpu_image_kwargs(p))
+        kw = gpu_image_kwargs(p)
+        kw["transfer_fp32"] = fp32
+        im, _, _ = G.make_image(g, m.L_unit, **kw)
         jobs.append((im, s if scales is None else scales[k]))
     return jobs
 
@@ -238,6 +240,38 @@ def test_rerender_batch(setup):
         c.rerender_batch(ja)
     assert e.value.code == G.ERR_STATE
     c.set_batch_memory(48 << 30)
+
+
+def test_rerender_batch_mixed(setup):
+    """The mixed-precision batch gathers an FP32 copy of the cell records,
+    written by a kernel before each transfer launch, also inside the
+    rerender's captured graph: replays equal fresh mixed batches bit for bit,
+    and a replay after a new dump of the same grid reads the new dump (twice
+    the M_unit at half the scales gives the same images to the FP32 rounding
+    of the records; a stale copy would give half the density)."""
+    h, prims, c, g, m = setup
+    sa = [1.0, 1.0, 0.3, 2.0, 1.0, 0.5]
+    sb = [0.7, 1.9, 0.45, 1.1, 3.0, 0.25]
+    ja, jb = batch_jobs_of(g, m, sa, fp32=1), batch_jobs_of(g, m, sb, fp32=1)
+    say("case mixed rerender: fresh batches at two scale sets")
+    fresh_b = c.render_batch(jb)
+    fresh_a = c.render_batch(ja)
+    c.set_batch_options(rerender_graph=1)
+    say("case mixed rerender: graph replays")
+    for k, (jobs, ref) in enumerate([(jb, fresh_b), (ja, fresh_a), (jb, fresh_b), (ja, fresh_a)]):
+        same_outputs(ref, c.rerender_batch(jobs), "mixed replay %d" % k)
+        c.download_fields()
+    say("case mixed rerender: new dump of the same grid")
+    c.upload_prims(prims, G.iharm_units(2 * M_UNIT))
+    try:
+        half = c.rerender_batch(batch_jobs_of(g, m, [s / 2 for s in sa], fp32=1))
+    finally:
+        c.upload_prims(prims, G.iharm_units(M_UNIT))
+    for k, (x, y) in enumerate(zip(fresh_a, half)):
+        assert np.array_equal(x["nstep"], y["nstep"])
+        assert abs(y["image"].sum() / x["image"].sum() - 1) < 1e-5, k
+        if not JOBS[k][5]:
+            assert abs(y["imageS"][..., 0].sum() / x["imageS"][..., 0].sum() - 1) < 1e-5, k
 
 
 def test_autotune_batch(setup):
