@@ -245,8 +245,7 @@ def precision_measure(args, oracle_stride=10):
     # one image on a dedicated device: a render-memory budget of 85% of it
     # holds the 400x400 image's records and packets at once (~237 GB; under
     # the library's default of 60% it renders in two chunks, ~10% slower)
-    import torch
-    ctx.set_render_memory(int(0.85 * torch.cuda.get_device_properties(0).total_memory))
+    ctx.set_render_memory(int(0.85 * G.device_total_memory(0)))
     res, ms = {}, {}
     for fp32 in (0, 1):
         im, scale, _ = G.make_image(g, m.L_unit, nx=400, ny=400, thetacam=17., freqcgs=230e9, transfer_fp32=fp32)

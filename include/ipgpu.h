@@ -92,9 +92,13 @@ typedef struct ipgpu_image {
   int emission_type;         /* 4 (Dexter thermal, default) or 1 (Pandya thermal) */
   int transfer_fp32;         /* 1: the coefficients' shape functions and the plasma-frame
                                 Stokes step (rotation, absorption/emission solution, its
-                                matrix and source terms) in FP32; geodesics, connection,
-                                tetrads and N in FP64.  No counterpart in the reference,
-                                which is FP64 throughout */
+                                matrix and source terms) in FP32, and the fluid gather
+                                reads the cell records rounded to FP32 (a copy the
+                                library writes from the dump before each render or batch
+                                transfer; not in ipgpu_render_noalloc, which allocates
+                                nothing and reads the FP64 records); geodesics,
+                                connection, interpolation, tetrads and N in FP64.  No
+                                counterpart in the reference, which is FP64 throughout */
 } ipgpu_image;
 
 typedef struct ipgpu_ctx ipgpu_ctx;

@@ -322,7 +322,7 @@ struct NoVG {
 template <bool F32 = false, int PS = 1, class VG = NoVG>
 IPG_HD_INLINE void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const ipgpu_image &P, int n,
                       const double *rec, const int *nt, const double *dump, double *pk_out, bool pol,
-                      const Scale &sc, double *ax = nullptr, const VG &vg = VG())
+                      const Scale &sc, double *ax = nullptr, const VG &vg = VG(), const float *d32 = nullptr)
 {
   PkW<PS> pk{pk_out};
   if (ax)
@@ -380,7 +380,15 @@ IPG_HD_INLINE void prep_step(const ipgpu_geom &G, const ipgpu_model &Mo, const i
     if (zero_emission) meta |= 2;
     Fluid F;
     FluidExt ext;
-    fluid(G, Mo, dump, tf.X, m, F, ax ? &ext : nullptr, vst, staged);
+    /* mixed precision with d32: the corners from the FP32 copy of the records */
+    bool pre = staged;
+    if (F32 && !staged && d32 && Mo.kind != IPGPU_MODEL_SPHERE && in_domain(G, tf.X[1], tf.X[2])) {
+      Cell c32;
+      locate(G, tf.X[1], tf.X[2], tf.X[3], c32);
+      gather10_f32(G, d32, c32, vst);
+      pre = true;
+    }
+    fluid(G, Mo, dump, tf.X, m, F, ax ? &ext : nullptr, vst, pre);
     F.ne *= sc.ne;
     F.B *= sc.b;
     if (ax && F.ne > 0.) {

@@ -418,7 +418,7 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep(ipgpu_geom G, ipgpu_model M
                                                      const double *__restrict__ traj,
                                                      const int *__restrict__ nturns_buf,
                                                      const double *__restrict__ dump, double *__restrict__ pkt,
-                                                     double *__restrict__ aux)
+                                                     double *__restrict__ aux, const float *__restrict__ d32)
 {
   __shared__ double sg_lds[STAGE ? (PREP_BLOCK / WAVE) * SG_WAVE_DBL : 1];
   long long g = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
@@ -441,7 +441,8 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep(ipgpu_geom G, ipgpu_model M
                                    unit_scale(), ax, S);
   } else {
     if (!act) return;
-    prep_step<F32>(G, Mo, P, ns - q, rec, nt, dump, pkt + (size_t)g * stride, stride == PK, unit_scale(), ax);
+    prep_step<F32>(G, Mo, P, ns - q, rec, nt, dump, pkt + (size_t)g * stride, stride == PK, unit_scale(), ax,
+                   NoVG(), d32);
   }
 }
 
@@ -1427,7 +1428,8 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_list(ipgpu_geom G, ipgpu_mo
                                                           const double *__restrict__ traj,
                                                           const long long *__restrict__ rbase,
                                                           const double *__restrict__ trajO,
-                                                          const double *__restrict__ dump, double *__restrict__ pkt)
+                                                          const double *__restrict__ dump, double *__restrict__ pkt,
+                                                          const float *__restrict__ d32)
 {
   __shared__ double sg_lds[STAGE ? TPB * SG_WAVE_DBL : 1];
   const long long t = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
@@ -1453,7 +1455,7 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_list(ipgpu_geom G, ipgpu_mo
     prep_step<F32, ILV, StagedCells>(G, Mo, P, ns - q, rec, nullptr, dump, pk, stride == PK, unit_scale(), nullptr, S);
   } else {
     if (!act) return;
-    prep_step<F32, ILV>(G, Mo, P, ns - q, rec, nullptr, dump, pk, stride == PK, unit_scale());
+    prep_step<F32, ILV>(G, Mo, P, ns - q, rec, nullptr, dump, pk, stride == PK, unit_scale(), nullptr, NoVG(), d32);
   }
 }
 
@@ -1478,7 +1480,8 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_compact(ipgpu_geom G, ipgpu
                                                              const double *__restrict__ traj,
                                                              const long long *__restrict__ rbase,
                                                              const double *__restrict__ trajO,
-                                                             const double *__restrict__ dump, double *__restrict__ pkt)
+                                                             const double *__restrict__ dump, double *__restrict__ pkt,
+                                                             const float *__restrict__ d32)
 {
   __shared__ double sg_lds[STAGE ? TPB * SG_WAVE_DBL : 1];
   long long g = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
@@ -1499,7 +1502,8 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_compact(ipgpu_geom G, ipgpu
                                    nullptr, S);
   } else {
     if (!act) return;
-    prep_step<F32>(G, Mo, P, ns - q, rec, nullptr, dump, pkt + (size_t)g * PK, true, unit_scale());
+    prep_step<F32>(G, Mo, P, ns - q, rec, nullptr, dump, pkt + (size_t)g * PK, true, unit_scale(), nullptr, NoVG(),
+                   d32);
   }
 }
 
@@ -1511,7 +1515,8 @@ template <bool F32, bool STAGE>
 __global__ __launch_bounds__(PREP_BLOCK) void k_prep_prefix(ipgpu_geom G, ipgpu_model Mo, ipgpu_image P,
                                                             const int *__restrict__ plist, int n, int np1, int cap,
                                                             const double *__restrict__ traj,
-                                                            const double *__restrict__ dump, double *__restrict__ pkt)
+                                                            const double *__restrict__ dump, double *__restrict__ pkt,
+                                                            const float *__restrict__ d32)
 {
   __shared__ double sg_lds[STAGE ? TPB * SG_WAVE_DBL : 1];
   const long long g = (long long)blockIdx.x * PREP_BLOCK + threadIdx.x;
@@ -1526,7 +1531,8 @@ __global__ __launch_bounds__(PREP_BLOCK) void k_prep_prefix(ipgpu_geom G, ipgpu_
                                    nullptr, S);
   } else {
     if (!act) return;
-    prep_step<F32>(G, Mo, P, np1 - q, rec, nullptr, dump, pkt + (size_t)g * PK, true, unit_scale());
+    prep_step<F32>(G, Mo, P, np1 - q, rec, nullptr, dump, pkt + (size_t)g * PK, true, unit_scale(), nullptr, NoVG(),
+                   d32);
   }
 }
 
@@ -1894,7 +1900,8 @@ struct ipgpu_ctx {
   ipgpu_model model{};
   bool has_model = false;
   double *d_dump = nullptr;
-  float *d_dump32 = nullptr;        /* FP32 copy of the records (mixed batch transfer) */
+  float *d_dump32 = nullptr;        /* FP32 copy of the records (mixed-precision transfers) */
+  const float *r_d32 = nullptr;     /* the copy the single-image render in progress reads, or null */
   size_t dump32_cells = 0;
   size_t dump_cells = 0;
   double *d_traj = nullptr;
@@ -2517,7 +2524,7 @@ static int prepare_packets(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, bool
     kp<<<nb, PREP_BLOCK, 0, ctx->stream>>>(ctx->geom, ctx->model, P, nrays, cap, stride, (long long)total,
                                                ctx->d_off, ctx->d_blk, ctx->d_nstep_ray, ctx->d_traj,
                                                turns ? ctx->d_nturns : nullptr, ctx->d_dump, ctx->d_pkt,
-                                               aux ? ctx->d_aux : nullptr);
+                                               aux ? ctx->d_aux : nullptr, ctx->r_d32);
     HIPCHK(ctx, hipGetLastError());
   }
   HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
@@ -2601,7 +2608,7 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
       auto kp = P.transfer_fp32 ? (ctx->gather_stage ? k_prep_compact<true, true> : k_prep_compact<true, false>)
                                 : (ctx->gather_stage ? k_prep_compact<false, true> : k_prep_compact<false, false>);
       kp<<<nb, PREP_BLOCK, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, rows, blk, cap, ctx->d_nstep_ray,
-                                    ctx->d_traj, rbase, trajO, ctx->d_dump, pkt);
+                                    ctx->d_traj, rbase, trajO, ctx->d_dump, pkt, ctx->r_d32);
       HIPCHK(ctx, hipGetLastError());
     }
     const int nseg = (int)S.off.size();
@@ -2628,7 +2635,7 @@ static int transfer_list(ipgpu_ctx *ctx, const ipgpu_image &P, hipStream_t st, i
     auto kp = P.transfer_fp32 ? (ctx->gather_stage ? k_prep_list<true, true> : k_prep_list<true, false>)
                               : (ctx->gather_stage ? k_prep_list<false, true> : k_prep_list<false, false>);
     kp<<<nb, PREP_BLOCK, 0, st>>>(ctx->geom, ctx->model, P, dl, n, W, TT, nw, tiles, blk, cap, stride,
-                                  ctx->d_nstep_ray, ctx->d_traj, rbase, trajO, ctx->d_dump, pkt);
+                                  ctx->d_nstep_ray, ctx->d_traj, rbase, trajO, ctx->d_dump, pkt, ctx->r_d32);
     HIPCHK(ctx, hipGetLastError());
   }
   if (after_prep) HIPCHK(ctx, hipEventRecord(after_prep, st));
@@ -3018,7 +3025,7 @@ static int render_streamed(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays, int 
         auto kp = P.transfer_fp32 ? (ctx->gather_stage ? k_prep_prefix<true, true> : k_prep_prefix<true, false>)
                                   : (ctx->gather_stage ? k_prep_prefix<false, true> : k_prep_prefix<false, false>);
         kp<<<(unsigned)((slots + PREP_BLOCK - 1) / PREP_BLOCK), PREP_BLOCK, 0, sp>>>(
-            ctx->geom, ctx->model, P, ctx->d_plist, np, np1, cap, ctx->d_traj, ctx->d_dump, ctx->d_ppkt);
+            ctx->geom, ctx->model, P, ctx->d_plist, np, np1, cap, ctx->d_traj, ctx->d_dump, ctx->d_ppkt, ctx->r_d32);
         HIPCHK(ctx, hipGetLastError());
         const long long lanes = (long long)S.off.size() * SEGCOL;
         k_seg<1><<<(unsigned)((lanes + WAVE - 1) / WAVE), WAVE, 0, sp>>>((int)S.off.size(), S.d_off, S.d_q0, S.d_q1,
@@ -3191,6 +3198,21 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
 static int render_rays(ipgpu_ctx *ctx, const ipgpu_image &P, int nrays)
 {
   int rc;
+  /* mixed precision: the packet kernels gather the FP32 copy of the cell
+   * records, written here from the dump as it is (the context forgets it
+   * when this render returns) */
+  struct D32Scope {
+    ipgpu_ctx *c;
+    ~D32Scope() { c->r_d32 = nullptr; }
+  } d32_scope{ctx};
+  ctx->r_d32 = nullptr;
+  if (P.transfer_fp32 && ctx->model.kind == IPGPU_MODEL_IHARM && ctx->d_dump) {
+    if ((rc = ensure_dump32(ctx))) return rc;
+    const long long nc = (long long)ctx->dump_cells;
+    k_dump_f32<<<(unsigned)((nc + 255) / 256), 256, 0, ctx->stream>>>(nc, ctx->d_dump, ctx->d_dump32);
+    HIPCHK(ctx, hipGetLastError());
+    ctx->r_d32 = ctx->d_dump32;
+  }
   if ((rc = grow(ctx, ctx->d_nstep_ray, ctx->cap_nstep_ray, (size_t)nrays))) return rc;
   if ((rc = grow(ctx, ctx->d_flag_ray, ctx->cap_flag_ray, (size_t)nrays))) return rc;
   if ((rc = grow(ctx, ctx->d_out7, ctx->cap_out7, (size_t)nrays * 7))) return rc;

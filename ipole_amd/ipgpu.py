@@ -164,6 +164,20 @@ def lib():
     return _lib
 
 
+def device_total_memory(device=0):
+    """Total memory of a device in bytes, from the HIP runtime that
+    libipgpu.so links (a process may also hold torch's own copy of the
+    runtime; this asks the library's, whatever was initialised first)."""
+    lib()
+    hip = C.CDLL("libamdhip64.so.7")   # the copy the library loaded (same soname)
+    hip.hipDeviceTotalMem.argtypes = [C.POINTER(C.c_size_t), C.c_int]
+    n = C.c_size_t(0)
+    rc = hip.hipDeviceTotalMem(C.byref(n), device)
+    if rc != 0:
+        raise RuntimeError("hipDeviceTotalMem failed (%d)" % rc)
+    return n.value
+
+
 def P(a):
     return a.ctypes.data_as(dp)
 

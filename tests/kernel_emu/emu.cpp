@@ -139,6 +139,8 @@ int emu_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, 
                const int *pix, int npix, double *out7, int *nstep, int *flag)
 {
   int status = 0;
+  const std::vector<float> d32 = P->transfer_fp32 ? dump_f32(*G, *M, dump) : std::vector<float>();
+  const float *d32p = d32.empty() ? nullptr : d32.data();
 #pragma omp parallel
   {
     std::vector<double> rec((size_t)P->maxnstep * REC);
@@ -162,7 +164,8 @@ int emu_render(const ipgpu_geom *G, const ipgpu_model *M, const ipgpu_image *P, 
       double js, ks;
       if (P->transfer_fp32) {
         for (int q = 0; q < ns; q++)
-          prep_step<true>(*G, *M, *P, ns - q, rec.data(), ntp, dump, pkt.data() + (size_t)q * stride, pol, unit_scale());
+          prep_step<true>(*G, *M, *P, ns - q, rec.data(), ntp, dump, pkt.data() + (size_t)q * stride, pol, unit_scale(),
+                          nullptr, NoVG(), d32p);
         far_end_coefficients<true>(*G, *M, *P, ns, rec.data(), dump, js, ks, unit_scale());
       } else {
         for (int q = 0; q < ns; q++)

@@ -106,7 +106,9 @@ def test_gather_staging(case, unpol, fp32):
     phases) equals the one-pass render bit for bit under the same setting,
     matches the oracle at the parity bar, and agrees with the direct-gather
     render: unpolarized outputs bit for bit, polarized Stokes to roundoff
-    (the two kernel instances fuse the step's multiply-adds differently)."""
+    (the two kernel instances fuse the step's multiply-adds differently); in
+    mixed precision to the mode's per-pixel bar (the direct gather reads the
+    FP32 copy of the records, the staged one the FP64 records)."""
     h, prims, orc = case
     say("case gather staging unpol=%d fp32=%d 32x32 th=163" % (unpol, fp32))
     tune = dict(seg_min_steps=100, seg_len=32)
@@ -129,11 +131,13 @@ def test_gather_staging(case, unpol, fp32):
     for k in ("nstep", "flag"):
         assert np.array_equal(ref[k], direct[k]), k
     if fp32:
-        # the FP32 fits and Stokes step of the two kernel instances may round
-        # differently at FP32 precision
+        # mixed precision: the direct gather reads the FP32 copy of the cell
+        # records, the staged one stages the FP64 records, so the two differ
+        # by the fields' FP32 rounding; compared at the mode's per-pixel bar
         for k in ("image", "taus"):
             e = np.abs(ref[k] - direct[k]) / np.maximum(np.abs(direct[k]), 1e-300)
-            assert float(e.max()) < 1e-6, (k, float(e.max()))
+            say("  staged vs direct %s max rel %.3e" % (k, float(e.max())))
+            assert float(e.max()) < 1e-4, (k, float(e.max()))
     else:
         for k in ("image", "taus"):
             assert np.array_equal(ref[k], direct[k]), k
@@ -144,7 +148,7 @@ def test_gather_staging(case, unpol, fp32):
         den = np.maximum(np.abs(b[..., :1]), 1e-300)
         err = float((np.abs(a - b) / den).max())
         say("  staged vs direct polarized max rel %.3e" % err)
-        assert err < (1e-6 if fp32 else 1e-9)
+        assert err < (1e-4 if fp32 else 1e-9)
         if not fp32:
             orc.activate()
             check(orc.render(p), ref, scale)

@@ -363,10 +363,10 @@ def image_summary(I, Q, U, V):
 @pytest.mark.parametrize("thetacam,etype", [(163., 4), (60., 1)])
 def test_emu_fp32_transfer(thetacam, etype):
     """FP32 coefficient shape functions and plasma-frame Stokes step (FP64
-    geodesics, connection, tetrads and N), split and fused forms (the fused
-    batch form gathers the FP32 copy of the cell records), against the FP64
-    oracle: Ftot, LP, CP and EVPA well inside the 1e-4 bar of the
-    mixed-precision mode (measured ~1e-7 here)."""
+    geodesics, connection, tetrads and N), split and fused forms (both
+    gather the FP32 copy of the cell records, as the GPU's mixed-precision
+    kernels do), against the FP64 oracle: Ftot, LP, CP and EVPA well inside
+    the 1e-4 bar of the mixed-precision mode (measured ~1e-7 here)."""
     h, prims = small_dump()
     orc, fields = O.iharm_oracle(h, prims, M_unit=3e26)
     p = O.default_params(nx=20, ny=20, thetacam=thetacam, freqcgs=230e9, emission_type=etype)
