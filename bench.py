@@ -7,7 +7,9 @@ in HBM after on-device derivation, imaged at 160x160 pixels, thetacam 163,
 (geodesic pass + polarized transfer pass + output layout) with the inputs
 already on the device; outputs stay on the device.
 
-Prints one JSON line (see DESIGN.md for the field definitions).
+Prints one JSON line: the metric and its value (pixels/s of the image), the
+roofline entry of the geodesic launches, the CPU baseline, and the batch,
+mixed-precision, adjoint and adaptive measurements.
 """
 import argparse
 import json

@@ -1,7 +1,7 @@
 """GPU parity: the HIP library (through its C-ABI) against the CPU oracle on
 the same seeded inputs.
 
-Bar (north star): per pixel I, Q, U, V within 1e-6 of the oracle, measured as
+Bar: per pixel I, Q, U, V within 1e-6 of the oracle, measured as
 |dS| / max(|I_px|, 1e-6 max I), and Ftot within 1e-6 relative.  The GPU
 forms the connection from exact metric derivatives while the oracle (like
 the reference) differences the metric with DEL = 1e-7, so the two differ at
