@@ -122,11 +122,17 @@ def test_fullsize_streamed_equals_one_pass(full, monkeypatch):
     assert int(one["nstep"].max()) > 10000
 
 
-def test_fullsize_chunked_equals_whole(full):
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_fullsize_chunked_equals_whole(full, fp32):
     """The 160x160 image rendered in chunks of rays under a small render
-    memory (8 GiB) equals the render of all rays at once, bit for bit."""
+    memory (8 GiB) equals the render of all rays at once, bit for bit; in
+    mixed precision too, where every chunk's render rewrites the FP32 copy of
+    the cell records its packet kernels gather."""
     orc, p, c, im = full
-    say("full-size chunked (8 GiB) vs whole")
+    if fp32:
+        im = type(im).from_buffer_copy(im)
+        im.transfer_fp32 = 1
+    say("full-size chunked (8 GiB) vs whole, fp32=%d" % fp32)
     whole = c.render(im)
     assert c.render_info()[0] == 1
     c.set_render_memory(8 << 30)
