@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "ipg_physics.h"
@@ -91,6 +92,55 @@ __device__ __forceinline__ double pair_swap(double v)
   return __hiloint2double(hi2, lo2);
 }
 
+/* Where trace_pair_t puts the records past its capacity: nowhere
+ * (NoOverflow), or in chunks of OVF_CH records drawn from an arena
+ * (OverflowChunks, the batch's first pass). */
+struct NoOverflow {
+  __device__ void store(int, bool, double, double, double, double) {}
+};
+
+constexpr int OVF_CH = 1024;   /* records per overflow chunk (64 KiB) */
+
+/* A ray's records from step cap on, chunk by chunk: the pair's even lane
+ * draws a chunk index from the arena's counter (nchunks of them), the odd
+ * lane receives it, and table[c] keeps chunk c's index for the ray.  When a
+ * chunk cannot be had (the arena used up, or more than maxc chunks) the ray
+ * stores no more and ok is false; it is then continued as without chunks. */
+struct OverflowChunks {
+  double *arena;
+  unsigned long long *counter;
+  unsigned long long nchunks;
+  int *table;
+  int maxc, cap;
+  double *cur;
+  bool ok;
+  __device__ __forceinline__ void store(int nstep, bool odd, double a, double b, double c, double d)
+  {
+    const int rel = nstep - cap;
+    const int within = rel & (OVF_CH - 1);
+    if (within == 0) {
+      const int ci = rel / OVF_CH;
+      double id = -1.;
+      if (ok && ci < maxc && !odd) {
+        const unsigned long long t = atomicAdd(counter, 1ull);
+        id = t < nchunks ? (double)t : -1.;
+      }
+      const double io = pair_swap(id);
+      id = odd ? io : id;
+      if (!odd && ci < maxc) table[ci] = (int)id;
+      if (id < 0.) ok = false;
+      cur = ok ? arena + (size_t)(long long)id * OVF_CH * REC : nullptr;
+    }
+    if (ok) {
+      double *o = cur + (size_t)within * REC + (odd ? 4 : 0);
+      o[0] = a;
+      o[1] = b;
+      o[2] = c;
+      o[3] = d;
+    }
+  }
+};
+
 /* trace_ray_t on a lane pair.  Both lanes hold the same X, K and take the
  * same decisions; the two metric evaluations of a step run one per lane
  * (the even lane at the half-step point, the odd lane at the step's end
@@ -98,9 +148,10 @@ __device__ __forceinline__ double pair_swap(double v)
  * forms from its metric are exchanged.  Half the instructions per lane of
  * the one-lane step, with the same operations on the same values, so the
  * records are bit-identical.  Each lane stores half of a record. */
-template <int MK>
+template <int MK, class OV = NoOverflow>
 __device__ __forceinline__ int trace_pair_t(const ipgpu_geom &G, const ipgpu_image &P, double X[4], double K[4],
-                                            int nstep, int cap, double *rec, int nend, int *finished, bool odd)
+                                            int nstep, int cap, double *rec, int nend, int *finished, bool odd,
+                                            OV *ov = nullptr)
 {
   Metric m;
   metric_k<MK>(G, X[1], X[2], m); /* both lanes: the start point */
@@ -145,6 +196,8 @@ __device__ __forceinline__ int trace_pair_t(const ipgpu_geom &G, const ipgpu_ima
       o[1] = odd ? K[2] : X[2];
       o[2] = odd ? K[3] : X[3];
       o[3] = odd ? dl : K[0];
+    } else if constexpr (!std::is_same<OV, NoOverflow>::value) {
+      if (cap > 0) ov->store(nstep, odd, odd ? K[1] : X[1], odd ? K[2] : X[2], odd ? K[3] : X[3], odd ? dl : K[0]);
     }
   }
   if (finished) *finished = done;
@@ -153,8 +206,9 @@ __device__ __forceinline__ int trace_pair_t(const ipgpu_geom &G, const ipgpu_ima
 
 /* trace_ray on a lane pair: the camera start or the resumption is formed by
  * both lanes alike. */
+template <class OV = NoOverflow>
 __device__ __forceinline__ int trace_pair(const ipgpu_geom &G, const ipgpu_image &P, int pi, int pj, int cap,
-                                          double *rec, int n0, int nend, int *finished, bool odd)
+                                          double *rec, int n0, int nend, int *finished, bool odd, OV *ov = nullptr)
 {
   double X[4], K[4];
   if (n0 > 0) {
@@ -173,10 +227,10 @@ __device__ __forceinline__ int trace_pair(const ipgpu_geom &G, const ipgpu_image
     n0 = 0;
   }
   switch (metric_kind(G)) {
-  case MK_MINK: return trace_pair_t<MK_MINK>(G, P, X, K, n0, cap, rec, nend, finished, odd);
-  case MK_MKS: return trace_pair_t<MK_MKS>(G, P, X, K, n0, cap, rec, nend, finished, odd);
-  case MK_FMKS_INT: return trace_pair_t<MK_FMKS_INT>(G, P, X, K, n0, cap, rec, nend, finished, odd);
-  default: return trace_pair_t<MK_FMKS>(G, P, X, K, n0, cap, rec, nend, finished, odd);
+  case MK_MINK: return trace_pair_t<MK_MINK, OV>(G, P, X, K, n0, cap, rec, nend, finished, odd, ov);
+  case MK_MKS: return trace_pair_t<MK_MKS, OV>(G, P, X, K, n0, cap, rec, nend, finished, odd, ov);
+  case MK_FMKS_INT: return trace_pair_t<MK_FMKS_INT, OV>(G, P, X, K, n0, cap, rec, nend, finished, odd, ov);
+  default: return trace_pair_t<MK_FMKS, OV>(G, P, X, K, n0, cap, rec, nend, finished, odd, ov);
   }
 }
 
@@ -756,7 +810,10 @@ __global__ __launch_bounds__(WAVE) void k_trace_store(ipgpu_geom G, const BGroup
  * than slot steps is then complete. */
 __global__ __launch_bounds__(WAVE) void k_trace_slot(ipgpu_geom G, const BGroup *__restrict__ groups,
                                                      int ngroups, long long nr, int slot, double *__restrict__ traj,
-                                                     int *__restrict__ nstep_out, const int *__restrict__ wtile)
+                                                     int *__restrict__ nstep_out, const int *__restrict__ wtile,
+                                                     double *__restrict__ arena, unsigned long long *ovf_counter,
+                                                     unsigned long long nchunks, int *__restrict__ ovf_table,
+                                                     int maxc, int *__restrict__ ovf_ok)
 {
   /* a wave traces half a tile of 64 rays; wtile: the tiles in dispatch
    * order (null: in index order) */
@@ -773,8 +830,17 @@ __global__ __launch_bounds__(WAVE) void k_trace_slot(ipgpu_geom G, const BGroup 
     if (!odd) nstep_out[r] = 0;
     return;
   }
-  const int ns = trace_pair(G, P, pi, pj, slot, slot > 0 ? traj + (size_t)r * slot * REC : nullptr, 0, 0x7fffffff,
-                            nullptr, odd);
+  int ns;
+  if (arena) {
+    /* the records past the slot go to chunks of the arena as the ray runs */
+    OverflowChunks ov{arena, ovf_counter, nchunks, ovf_table + (size_t)r * maxc, maxc, slot, nullptr, slot > 0};
+    ns = trace_pair(G, P, pi, pj, slot, slot > 0 ? traj + (size_t)r * slot * REC : nullptr, 0, 0x7fffffff, nullptr,
+                    odd, &ov);
+    if (!odd) ovf_ok[r] = ov.ok ? 1 : 0;
+  } else {
+    ns = trace_pair(G, P, pi, pj, slot, slot > 0 ? traj + (size_t)r * slot * REC : nullptr, 0, 0x7fffffff, nullptr,
+                    odd);
+  }
   if (!odd) nstep_out[r] = ns;
 }
 
@@ -815,6 +881,33 @@ __global__ void k_slot_copy(const int *__restrict__ list, int n, const long long
     const int k = (int)(t / per);
     const long long e = t - (long long)k * per;
     traj2[off2[k] * REC + e] = traj[(long long)list[k] * per + e];
+  }
+}
+
+/* The n rays of list whose records past the slot are all in arena chunks
+ * (table: maxc chunk indices per ray): slot records and chunks copied to
+ * their full-length regions (offsets off2, in records) of traj2, one block
+ * per ray, 16 bytes per thread and step. */
+__global__ void k_ovf_gather(const int *__restrict__ list, int n, const long long *__restrict__ off2, int slot,
+                             const int *__restrict__ nstep, const double *__restrict__ traj,
+                             const double *__restrict__ arena, const int *__restrict__ table, int maxc,
+                             double *__restrict__ traj2)
+{
+  const int k = blockIdx.x;
+  if (k >= n) return;
+  const int r = list[k];
+  const long long nq = (long long)(nstep[r] + 1) * (REC / 2);
+  double2 *dst = reinterpret_cast<double2 *>(traj2 + (size_t)off2[k] * REC);
+  for (long long u = threadIdx.x; u < nq; u += blockDim.x) {
+    const int rn = (int)(u / (REC / 2)), part = (int)(u % (REC / 2));
+    const double *src;
+    if (rn < slot) {
+      src = traj + ((size_t)r * slot + rn) * REC;
+    } else {
+      const int rel = rn - slot;
+      src = arena + ((size_t)table[(size_t)r * maxc + rel / OVF_CH] * OVF_CH + (rel % OVF_CH)) * REC;
+    }
+    dst[u] = reinterpret_cast<const double2 *>(src)[part];
   }
 }
 
@@ -1901,6 +1994,17 @@ struct ipgpu_ctx {
   bool has_model = false;
   double *d_dump = nullptr;
   float *d_dump32 = nullptr;        /* FP32 copy of the records (mixed-precision transfers) */
+  /* the batch's first-pass overflow: arena chunks for the records past the
+   * slot, their table, a per-ray flag and the chunk counter */
+  double *d_ovf_arena = nullptr;
+  size_t ovf_arena_doubles = 0;
+  int *d_ovf_table = nullptr, *d_ovf_ok = nullptr, *d_glist = nullptr;
+  size_t cap_ovf_table = 0, cap_ovf_ok = 0, cap_glist = 0;
+  long long *d_goff2 = nullptr;
+  size_t cap_goff2 = 0;
+  unsigned long long *d_ovf_cnt = nullptr;
+  size_t cap_ovf_cnt = 0;
+  size_t ovf_arena_bytes = (size_t)8 << 30;
   const float *r_d32 = nullptr;     /* the copy the single-image render in progress reads, or null */
   size_t dump32_cells = 0;
   size_t dump_cells = 0;
@@ -2266,6 +2370,7 @@ int ipgpu_create(ipgpu_ctx **out, int device)
   if (const char *e = getenv("IPGPU_REFINE_SPECULATE_MAX")) ctx->refine_spec_max = atoll(e);
   if (const char *e = getenv("IPGPU_TAIL_PREFIX")) ctx->tail_prefix = atoi(e);
   if (const char *e = getenv("IPGPU_BATCH_SLOT")) ctx->batch_slot = atoi(e);
+  if (const char *e = getenv("IPGPU_BATCH_ARENA_MIB")) ctx->ovf_arena_bytes = (size_t)std::max(0, atoi(e)) << 20;
   if (const char *e = getenv("IPGPU_ADJ_SEG_LEN")) ctx->adj_seg_len = atoi(e) > 0 ? atoi(e) : 512;
   if (hipMalloc((void **)&ctx->d_status, NSTATUS * sizeof(int)) != hipSuccess
       || hipMalloc((void **)&ctx->d_counts, 2 * sizeof(unsigned long long)) != hipSuccess) {
@@ -2294,7 +2399,8 @@ void ipgpu_destroy(ipgpu_ctx *ctx)
     for (void *p : sp)
       if (p) hipFree(p);
   }
-  void *ptrs[] = {ctx->d_live, ctx->d_list, ctx->d_blk2, ctx->d_offl, ctx->d_pkt2, ctx->d_dump, ctx->d_dump32, ctx->d_traj, ctx->d_pkt, ctx->d_aux, ctx->d_off, ctx->d_blk, ctx->d_segbase, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
+  void *ptrs[] = {ctx->d_ovf_arena, ctx->d_ovf_table, ctx->d_ovf_ok, ctx->d_glist, ctx->d_goff2, ctx->d_ovf_cnt,
+                  ctx->d_live, ctx->d_list, ctx->d_blk2, ctx->d_offl, ctx->d_pkt2, ctx->d_dump, ctx->d_dump32, ctx->d_traj, ctx->d_pkt, ctx->d_aux, ctx->d_off, ctx->d_blk, ctx->d_segbase, ctx->d_nturns, ctx->d_nstep_ray, ctx->d_flag_ray,
                   ctx->d_out7, ctx->d_image, ctx->d_imageS, ctx->d_taus, ctx->d_nstep_pix,
                   ctx->d_flag_pix, ctx->d_status, ctx->d_counts,
                   ctx->d_groups, ctx->d_jobs, ctx->d_items, ctx->d_bimage, ctx->d_bimageS, ctx->d_btaus,
@@ -3949,18 +4055,44 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     if ((rc = grow(ctx, ctx->d_wtile, ctx->cap_wtile, (size_t)ntiles))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_wtile, wt.data(), ntiles * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   }
+  /* the records past the slot: chunks of an arena of ovf_arena_bytes, so
+   * that the rays longer than the slot need no second trace (a ray that
+   * cannot get all its chunks is continued as before) */
+  double *arena = nullptr;
+  unsigned long long nchunks = 0;
+  int maxc = 0;
+  if (pairs && try_slot && ctx->ovf_arena_bytes >= (size_t)OVF_CH * REC * sizeof(double)) {
+    int mxn = 0;
+    for (int g = 0; g < ngroups; g++) mxn = std::max(mxn, groups[g].P.maxnstep);
+    maxc = std::min(64, std::max(1, (mxn - slot + OVF_CH) / OVF_CH));
+    nchunks = ctx->ovf_arena_bytes / ((size_t)OVF_CH * REC * sizeof(double));
+    if ((rc = grow(ctx, ctx->d_ovf_arena, ctx->ovf_arena_doubles, (size_t)nchunks * OVF_CH * REC))
+        || (rc = grow(ctx, ctx->d_ovf_table, ctx->cap_ovf_table, (size_t)rays_total * maxc))
+        || (rc = grow(ctx, ctx->d_ovf_ok, ctx->cap_ovf_ok, (size_t)rays_total))
+        || (rc = grow(ctx, ctx->d_ovf_cnt, ctx->cap_ovf_cnt, (size_t)1))) {
+      if (rc != IPGPU_ERR_NOMEM) return rc;
+      nchunks = 0;   /* no arena: every long ray is continued */
+    } else {
+      HIPCHK(ctx, hipMemsetAsync(ctx->d_ovf_cnt, 0, sizeof(unsigned long long), ctx->stream));
+      arena = ctx->d_ovf_arena;
+    }
+  }
   HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   if (pairs)
     k_trace_slot<<<(unsigned)(2 * ntiles), WAVE, 0, ctx->stream>>>(ctx->geom, ctx->d_groups, ngroups, rays_total,
                                                                   try_slot ? slot : 0, ctx->d_traj, ctx->d_nstep_ray,
-                                                                  ctx->d_wtile);
+                                                                  ctx->d_wtile, arena, ctx->d_ovf_cnt, nchunks,
+                                                                  ctx->d_ovf_table, maxc, ctx->d_ovf_ok);
   else
     k_trace_count<<<(unsigned)((rays_total + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>(
         ctx->geom, ctx->d_groups, ngroups, rays_total, ctx->d_nstep_ray);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-  std::vector<int> ns_host(rays_total);
+  std::vector<int> ns_host(rays_total), ovf_host(arena ? rays_total : 0);
   HIPCHK(ctx, hipMemcpyAsync(ns_host.data(), ctx->d_nstep_ray, rays_total * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  if (arena)
+    HIPCHK(ctx, hipMemcpyAsync(ovf_host.data(), ctx->d_ovf_ok, rays_total * sizeof(int), hipMemcpyDeviceToHost,
+                               ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   hipEventElapsedTime(&t, ctx->ev[1], ctx->ev[2]);
   ms_geo += t;
@@ -4027,14 +4159,44 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     HIPCHK(ctx, hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     if (nlong > 0) {
-      HIPCHK(ctx, hipMemcpyAsync(ctx->d_blk, order.data(), nlong * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-      HIPCHK(ctx, hipMemcpyAsync(ctx->d_off2, off2.data(), nlong * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
-      k_slot_copy<<<1024, 256, 0, ctx->stream>>>(ctx->d_blk, (int)nlong, ctx->d_off2, slot, ctx->d_traj, ctx->d_traj2);
-      HIPCHK(ctx, hipGetLastError());
-      k_trace_resume_list<<<(unsigned)((2 * nlong + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>(
-          ctx->geom, ctx->d_groups, ngroups, ctx->d_blk, (int)nlong, ctx->d_off2, slot, ctx->d_nstep_ray,
-          ctx->d_traj2, ctx->d_status);
-      HIPCHK(ctx, hipGetLastError());
+      /* the long rays whose records past the slot are all in arena chunks
+       * are copied to their regions; the others are continued from their
+       * last slot record */
+      std::vector<int> glist, rlist;
+      std::vector<long long> goff, roff;
+      for (long long q = 0; q < nlong; q++) {
+        const int r = order[q];
+        if (arena && ovf_host[r]) {
+          glist.push_back(r);
+          goff.push_back(off2[q]);
+        } else {
+          rlist.push_back(r);
+          roff.push_back(off2[q]);
+        }
+      }
+      if (!glist.empty()) {
+        const int ng = (int)glist.size();
+        if ((rc = grow(ctx, ctx->d_glist, ctx->cap_glist, (size_t)ng))
+            || (rc = grow(ctx, ctx->d_goff2, ctx->cap_goff2, (size_t)ng)))
+          return rc;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_glist, glist.data(), ng * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_goff2, goff.data(), ng * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
+        k_ovf_gather<<<(unsigned)ng, 256, 0, ctx->stream>>>(ctx->d_glist, ng, ctx->d_goff2, slot, ctx->d_nstep_ray,
+                                                            ctx->d_traj, ctx->d_ovf_arena, ctx->d_ovf_table, maxc,
+                                                            ctx->d_traj2);
+        HIPCHK(ctx, hipGetLastError());
+      }
+      if (!rlist.empty()) {
+        const int nrl = (int)rlist.size();
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_blk, rlist.data(), nrl * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_off2, roff.data(), nrl * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
+        k_slot_copy<<<1024, 256, 0, ctx->stream>>>(ctx->d_blk, nrl, ctx->d_off2, slot, ctx->d_traj, ctx->d_traj2);
+        HIPCHK(ctx, hipGetLastError());
+        k_trace_resume_list<<<(unsigned)((2 * nrl + WAVE - 1) / WAVE), WAVE, 0, ctx->stream>>>(
+            ctx->geom, ctx->d_groups, ngroups, ctx->d_blk, nrl, ctx->d_off2, slot, ctx->d_nstep_ray, ctx->d_traj2,
+            ctx->d_status);
+        HIPCHK(ctx, hipGetLastError());
+      }
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     /* the (job, ray) items, built in pinned host memory (the upload is then

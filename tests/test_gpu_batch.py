@@ -108,14 +108,18 @@ def test_batch_chunks_identical(setup):
                     assert np.array_equal(a[k], b[k]), k
 
 
+@pytest.mark.parametrize("arena_mib", [8192, 1, 0])
 @pytest.mark.parametrize("slot", [64, 301])
-def test_batch_slot_records(setup, slot, monkeypatch):
+def test_batch_slot_records(setup, slot, arena_mib, monkeypatch):
     """The first trace pass stores each ray's first `slot` records in a slot
-    of its own; the rays longer than that are continued on lane pairs from
-    their last slot record in a full-length region.  The images equal those
-    of the count-then-store passes (a small trajectory memory forces them)."""
+    of its own and, while the arena lasts, the rest in chunks of an arena;
+    the rays longer than the slot get full-length regions, filled from the
+    slot and their chunks, or, for the rays that could not get all their
+    chunks (a 1 MiB arena holds 16; none when it is off), by continuing the
+    ray on lane pairs from its last slot record.  The images equal those of
+    the count-then-store passes (a small trajectory memory forces them)."""
     h, prims, c, g, m = setup
-    say("case batch slot records, slot %d" % slot)
+    say("case batch slot records, slot %d, arena %d MiB" % (slot, arena_mib))
     jobs = []
     for (th, nx, ny, f, s, unpol, et) in JOBS:
         p = O.default_params(nx=nx, ny=ny, thetacam=th, freqcgs=f, only_unpolarized=unpol, emission_type=et)
@@ -126,6 +130,7 @@ def test_batch_slot_records(setup, slot, monkeypatch):
     steps_ref = c.step_counts()[0]
     c.set_batch_memory(48 << 30)
     monkeypatch.setenv("IPGPU_BATCH_SLOT", str(slot))
+    monkeypatch.setenv("IPGPU_BATCH_ARENA_MIB", str(arena_mib))
     c2 = G.Context(0)
     try:
         c2.set_geometry(g)
