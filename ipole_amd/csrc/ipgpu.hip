@@ -234,6 +234,39 @@ __device__ __forceinline__ int trace_pair(const ipgpu_geom &G, const ipgpu_image
   }
 }
 
+/* trace_pair from the camera in two loops: to step cap - 1 storing the
+ * slot's records, then, for a ray that goes on, the rest with its records
+ * in arena chunks.  The second loop continues from the first one's X and K
+ * (its start-point metric is the same function of the same X), so the
+ * records are those of one loop; the first loop carries nothing of the
+ * chunks' state. */
+template <int MK>
+__device__ __forceinline__ int trace_pair_slot_t(const ipgpu_geom &G, const ipgpu_image &P, double X[4], double K[4],
+                                                 int cap, double *rec, bool odd, OverflowChunks &ov)
+{
+  int done = 1;
+  const int ns = trace_pair_t<MK>(G, P, X, K, 0, cap, rec, cap - 1, &done, odd);
+  if (done) return ns;
+  return trace_pair_t<MK, OverflowChunks>(G, P, X, K, ns, cap, rec, 0x7fffffff, nullptr, odd, &ov);
+}
+
+__device__ __forceinline__ int trace_pair_slot(const ipgpu_geom &G, const ipgpu_image &P, int pi, int pj, int cap,
+                                               double *rec, bool odd, OverflowChunks &ov)
+{
+  double X[4], K[4], E[4][4];
+  Metric m;
+  for (int k = 0; k < 4; k++) X[k] = P.Xcam[k];
+  camera_tetrad(G, X, P.old_centering, E, m);
+  init_K(P, E, pi, pj, K);
+  if (!odd) store_rec(rec, X, K, 0.);
+  switch (metric_kind(G)) {
+  case MK_MINK: return trace_pair_slot_t<MK_MINK>(G, P, X, K, cap, rec, odd, ov);
+  case MK_MKS: return trace_pair_slot_t<MK_MKS>(G, P, X, K, cap, rec, odd, ov);
+  case MK_FMKS_INT: return trace_pair_slot_t<MK_FMKS_INT>(G, P, X, K, cap, rec, odd, ov);
+  default: return trace_pair_slot_t<MK_FMKS>(G, P, X, K, cap, rec, odd, ov);
+  }
+}
+
 /* the loss's component weights, by value */
 struct AdjW {
   double w[4];
@@ -831,11 +864,10 @@ __global__ __launch_bounds__(WAVE) void k_trace_slot(ipgpu_geom G, const BGroup 
     return;
   }
   int ns;
-  if (arena) {
+  if (arena && slot > 0) {
     /* the records past the slot go to chunks of the arena as the ray runs */
-    OverflowChunks ov{arena, ovf_counter, nchunks, ovf_table + (size_t)r * maxc, maxc, slot, nullptr, slot > 0};
-    ns = trace_pair(G, P, pi, pj, slot, slot > 0 ? traj + (size_t)r * slot * REC : nullptr, 0, 0x7fffffff, nullptr,
-                    odd, &ov);
+    OverflowChunks ov{arena, ovf_counter, nchunks, ovf_table + (size_t)r * maxc, maxc, slot, nullptr, true};
+    ns = trace_pair_slot(G, P, pi, pj, slot, traj + (size_t)r * slot * REC, odd, ov);
     if (!odd) ovf_ok[r] = ov.ok ? 1 : 0;
   } else {
     ns = trace_pair(G, P, pi, pj, slot, slot > 0 ? traj + (size_t)r * slot * REC : nullptr, 0, 0x7fffffff, nullptr,
