@@ -2457,6 +2457,7 @@ const char *ipgpu_last_error(const ipgpu_ctx *ctx) { return ctx ? ctx->err : "nu
 void *ipgpu_stream(ipgpu_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 static void batch_evict(ipgpu_ctx *ctx);
+static void release_overflow(ipgpu_ctx *ctx);
 
 int ipgpu_set_geometry(ipgpu_ctx *ctx, const ipgpu_geom *g)
 {
@@ -3311,6 +3312,7 @@ int ipgpu_render(ipgpu_ctx *ctx, const ipgpu_image *img, double *image, double *
     return set_err(ctx, IPGPU_ERR_ARG, "emission_type must be 4 (Dexter thermal) or 1 (Pandya thermal)");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   batch_evict(ctx);
+  release_overflow(ctx);
   if (P.nx > 32767 || P.ny > 32767) return set_err(ctx, IPGPU_ERR_ARG, "image side above 32767 pixels");
   int tiles = ((P.nx + TILE - 1) / TILE) * ((P.ny + TILE - 1) / TILE);
   int nrays = tiles * WAVE;
@@ -3489,6 +3491,7 @@ int ipgpu_render_adaptive(ipgpu_ctx *ctx, const ipgpu_image *img, const ipgpu_re
 {
   if (!ctx || !img || !rf || !image) return IPGPU_ERR_ARG;
   batch_evict(ctx);
+  release_overflow(ctx);
   const ipgpu_image &P = *img;
   if (rf->nx_min <= 0 || rf->nx_min == P.nx) {
     ctx->ref_levels = 1;
@@ -3910,6 +3913,25 @@ static void batch_evict(ipgpu_ctx *ctx)
 {
   ctx->blast.valid = false;
   drop_batch_graph(ctx);
+}
+
+/* The batch's first-pass overflow arena and its tables are scratch of the
+ * batch's traces: a single-image render or an adjoint gives their memory
+ * back (the next batch allocates them again). */
+static void release_overflow(ipgpu_ctx *ctx)
+{
+  if (!ctx->d_ovf_arena && !ctx->d_ovf_table && !ctx->d_ovf_ok) return;
+  if (hipSetDevice(ctx->device) != hipSuccess) return;
+  if (ctx->d_ovf_arena && hipFree(ctx->d_ovf_arena) == hipSuccess)
+    ctx->dev_bytes -= ctx->ovf_arena_doubles * sizeof(double);
+  if (ctx->d_ovf_table && hipFree(ctx->d_ovf_table) == hipSuccess) ctx->dev_bytes -= ctx->cap_ovf_table * sizeof(int);
+  if (ctx->d_ovf_ok && hipFree(ctx->d_ovf_ok) == hipSuccess) ctx->dev_bytes -= ctx->cap_ovf_ok * sizeof(int);
+  ctx->d_ovf_arena = nullptr;
+  ctx->ovf_arena_doubles = 0;
+  ctx->d_ovf_table = nullptr;
+  ctx->cap_ovf_table = 0;
+  ctx->d_ovf_ok = nullptr;
+  ctx->cap_ovf_ok = 0;
 }
 
 int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
@@ -4746,6 +4768,7 @@ static int run_adjoint(ipgpu_ctx *ctx, const ipgpu_image *img, const double *tar
     return set_err(ctx, IPGPU_ERR_STATE, "the adjoint needs an iharm dump");
   if (!ctx->has_units) return set_err(ctx, IPGPU_ERR_STATE, "the adjoint needs a dump from ipgpu_upload_prims");
   batch_evict(ctx);
+  release_overflow(ctx);
   const ipgpu_image &P = *img;
   if (P.nx <= 0 || P.ny <= 0 || P.maxnstep < 2) return set_err(ctx, IPGPU_ERR_ARG, "bad image size or maxnstep");
   if (P.only_unpolarized || P.transfer_fp32 || P.target_nturns >= 0 || P.counterjet)
