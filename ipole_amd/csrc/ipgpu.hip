@@ -3837,15 +3837,6 @@ static int group_of_host(const std::vector<BGroup> &groups, long long r)
   return lo;
 }
 
-/* host copy of ray_pixel */
-static void host_ray_pixel(int ray, int nx, int ny, int &i, int &j)
-{
-  int tiles_y = (ny + TILE - 1) / TILE;
-  int t = ray / WAVE, l = ray % WAVE;
-  i = (t / tiles_y) * TILE + l / TILE;
-  j = (t % tiles_y) * TILE + l % TILE;
-}
-
 /* The transfer of a batch's nitems work items (d_items, d_jobs, records at
  * d_traj / traj2), on the context's stream: the persistent work-queue
  * launch or one wave per 64 items.  clear_status: first reset the status
@@ -4142,6 +4133,19 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
         ctx->geom, ctx->d_groups, ngroups, rays_total, ctx->d_nstep_ray);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+  /* while the first pass runs: the rays with a pixel (ray_pixel's tiles
+   * of TILE x TILE pixels, WAVE rays each) */
+  std::vector<char> valid(rays_total, 0);
+  for (int g = 0; g < ngroups; g++) {
+    const int nx = groups[g].P.nx, ny = groups[g].P.ny, tiles_y = (ny + TILE - 1) / TILE;
+    const int tiles_x = (int)(groups[g].nrays / WAVE) / tiles_y;
+    for (int tx = 0; tx < tiles_x; tx++)
+      for (int ty = 0; ty < tiles_y; ty++) {
+        const long long r0 = groups[g].ray_base + ((long long)tx * tiles_y + ty) * WAVE;
+        for (int l = 0; l < WAVE; l++)
+          valid[r0 + l] = (tx * TILE + l / TILE < nx && ty * TILE + l % TILE < ny) ? 1 : 0;
+      }
+  }
   std::vector<int> ns_host(rays_total), ovf_host(arena ? rays_total : 0);
   HIPCHK(ctx, hipMemcpyAsync(ns_host.data(), ctx->d_nstep_ray, rays_total * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   if (arena)
@@ -4150,17 +4154,20 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   hipEventElapsedTime(&t, ctx->ev[1], ctx->ev[2]);
   ms_geo += t;
-  /* rays with a pixel, longest first (counting sort) */
-  std::vector<char> valid(rays_total, 0);
+  /* rays with a pixel, longest first (counting sort); the slot totals in
+   * the same sequential pass */
   int mxs = 0;
-  for (int g = 0; g < ngroups; g++)
-    for (int lr = 0; lr < groups[g].nrays; lr++) {
-      int pi, pj;
-      host_ray_pixel(lr, groups[g].P.nx, groups[g].P.ny, pi, pj);
-      if (pi < groups[g].P.nx && pj < groups[g].P.ny) {
-        long long r = groups[g].ray_base + lr;
-        valid[r] = 1;
-        mxs = ns_host[r] > mxs ? ns_host[r] : mxs;
+  long long long_recs = 0, nlong = 0, slot_steps = 0;
+  for (long long r = 0; r < rays_total; r++)
+    if (valid[r]) {
+      const int v = ns_host[r];
+      mxs = v > mxs ? v : mxs;
+      if (try_slot) {
+        slot_steps += v;
+        if (v >= slot) {
+          long_recs += v + 1;
+          nlong++;
+        }
       }
     }
   std::vector<long long> bucket(mxs + 2, 0);
@@ -4174,16 +4181,6 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     if (valid[r]) order[bucket[mxs - ns_host[r]]++] = (int)r;
   std::vector<long long> off_host(rays_total, 0);
   std::vector<int2> items;
-  long long long_recs = 0, nlong = 0, slot_steps = 0;
-  if (try_slot)
-    for (long long q = 0; q < nvalid; q++) {
-      const int v = ns_host[order[q]];
-      slot_steps += v;
-      if (v >= slot) {
-        long_recs += v + 1;
-        nlong++;
-      }
-    }
   bool in_place = try_slot && (long long)rays_total * slot + long_recs <= budget;
   if (in_place && nlong > 0 && (rc = grow(ctx, ctx->d_traj2, ctx->traj2_doubles, (size_t)long_recs * REC))) {
     if (rc != IPGPU_ERR_NOMEM) return rc;
@@ -4195,15 +4192,12 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
      * the front of order) a full-length region of traj2 (off = -1 - offset) */
     std::vector<long long> off2(nlong > 0 ? nlong : 1);
     long long o2 = 0;
-    for (long long q = 0; q < nvalid; q++) {
+    for (long long r = 0; r < rays_total; r++) off_host[r] = (long long)r * slot;
+    for (long long q = 0; q < nlong; q++) {
       const int r = order[q];
-      if (q < nlong) {
-        off2[q] = o2;
-        off_host[r] = -1 - o2;
-        o2 += ns_host[r] + 1;
-      } else {
-        off_host[r] = (long long)r * slot;
-      }
+      off2[q] = o2;
+      off_host[r] = -1 - o2;
+      o2 += ns_host[r] + 1;
     }
     /* the long rays' continuation goes out first: the host builds the
      * transfer's items while it runs */
