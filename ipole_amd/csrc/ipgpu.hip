@@ -4134,16 +4134,24 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
   /* while the first pass runs: the rays with a pixel (ray_pixel's tiles
-   * of TILE x TILE pixels, WAVE rays each) */
+   * of TILE x TILE pixels, WAVE rays each), every ray's group and each
+   * group's count of rays with a pixel */
   std::vector<char> valid(rays_total, 0);
+  std::vector<int> &gid = ctx->h_gid;
+  gid.resize(rays_total);
+  std::vector<long long> gvalid(ngroups, 0);
   for (int g = 0; g < ngroups; g++) {
     const int nx = groups[g].P.nx, ny = groups[g].P.ny, tiles_y = (ny + TILE - 1) / TILE;
     const int tiles_x = (int)(groups[g].nrays / WAVE) / tiles_y;
+    for (long long r = groups[g].ray_base; r < groups[g].ray_base + groups[g].nrays; r++) gid[r] = g;
     for (int tx = 0; tx < tiles_x; tx++)
       for (int ty = 0; ty < tiles_y; ty++) {
         const long long r0 = groups[g].ray_base + ((long long)tx * tiles_y + ty) * WAVE;
-        for (int l = 0; l < WAVE; l++)
-          valid[r0 + l] = (tx * TILE + l / TILE < nx && ty * TILE + l % TILE < ny) ? 1 : 0;
+        for (int l = 0; l < WAVE; l++) {
+          const bool in = tx * TILE + l / TILE < nx && ty * TILE + l % TILE < ny;
+          valid[r0 + l] = in ? 1 : 0;
+          gvalid[g] += in;
+        }
       }
   }
   std::vector<int> ns_host(rays_total), ovf_host(arena ? rays_total : 0);
@@ -4249,13 +4257,9 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     /* the (job, ray) items, built in pinned host memory (the upload is then
      * a direct copy); every ray's group from a table of the groups' ranges */
-    std::vector<int> &gid = ctx->h_gid;
-    gid.resize(rays_total);
-    for (int g = 0; g < ngroups; g++)
-      for (long long r = groups[g].ray_base; r < groups[g].ray_base + groups[g].nrays; r++) gid[r] = g;
     long long nit = 0;
-    for (long long q = 0; q < nvalid; q++)
-      nit += multi ? (long long)gsub[gid[order[q]]].size() : (long long)members[gid[order[q]]].size();
+    for (int g = 0; g < ngroups; g++)
+      nit += gvalid[g] * (long long)(multi ? gsub[g].size() : members[g].size());
     if ((size_t)nit > ctx->h_items_cap) {
       if (ctx->h_items) hipHostFree(ctx->h_items);
       ctx->h_items = nullptr;
@@ -4281,21 +4285,24 @@ int ipgpu_render_batch(ipgpu_ctx *ctx, int njobs, const ipgpu_job *jobs)
       for (long long r = 0; r < rays_total; r++)
         if (valid[r]) border[fill[nband - 1 - (ns_host[r] >> ITEM_BAND)]++] = (int)r;
     }
+    long long st_steps = 0, st_pairs = 0, st_psteps = 0;
     for (int k = 0; k < nband; k++)
       for (int sz = max_sz; sz >= 1; sz--)
         for (long long q = bstart[k]; q < bstart[k + 1]; q++) {
           const int r = border[q];
-          ray_items(r, gid[r], sz, [&](int2 v) { hit[w++] = v; });
+          const long long v = ns_host[r];
+          ray_items(r, gid[r], sz, [&](int2 it) {
+            hit[w++] = it;
+            st_steps += v;
+            st_pairs += sz;
+            st_psteps += v * sz;
+          });
         }
     const int nitems = (int)nit;
-    for (long long k = 0; k < nit; k++) {
-      const int2 v = hit[k];
-      const int r = multi ? v.x : v.y, nj = multi ? subsets[v.y].x : 1;
-      ctx->bstat[0] += 1;
-      ctx->bstat[1] += ns_host[r];
-      ctx->bstat[3] += nj;
-      ctx->bstat[4] += (long long)ns_host[r] * nj;
-    }
+    ctx->bstat[0] += nit;
+    ctx->bstat[1] += st_steps;
+    ctx->bstat[3] += st_pairs;
+    ctx->bstat[4] += st_psteps;
     if ((rc = grow(ctx, ctx->d_items, ctx->cap_items, (size_t)(nitems > 0 ? nitems : 1)))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_items, hit, nitems * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
