@@ -49,6 +49,24 @@ def test_host_iharm(tmp_path):
     assert np.allclose(img["image"], ref["image"], rtol=1e-5, atol=1e-6 * ref["image"].max())
 
 
+def test_host_iharm_mixed_precision(tmp_path):
+    """The host with the library's mixed-precision transfer (--transfer_fp32=1:
+    FP32 fits, Stokes step and cell records) against the oracle's FP64
+    image: Ftot within 1e-5 and the Stokes images within the mode's 1e-4."""
+    h, prims = small_dump()
+    dump = str(tmp_path / "dump.h5")
+    h5.write_dump(dump, h, prims)
+    rc, err = run(["--dump=%s" % dump, "--M_unit=3e26", "--freqcgs=230e9", "--thetacam=150", "--nx=24", "--ny=24",
+                   "--transfer_fp32=1", "--outfile=%s" % (tmp_path / "mixed.h5")], cwd=str(tmp_path))
+    assert rc == 0, err[-3000:]
+    img = h5.read_image(str(tmp_path / "mixed.h5"), 24, 24)
+    orc, _ = O.iharm_oracle(h, prims, M_unit=3e26)
+    ref = orc.render(O.default_params(nx=24, ny=24, thetacam=150., freqcgs=230e9))
+    F_ref = ref["imageS"][..., 0].sum() * ref["cam"].scale
+    assert abs(img["Ftot"] / F_ref - 1) < 1e-5
+    assert np.allclose(img["imageS"], ref["imageS"], rtol=1e-4, atol=1e-4 * np.abs(ref["imageS"][..., 0]).max())
+
+
 def test_host_unpol_quench_and_maxnstep(tmp_path):
     h, prims = small_dump()
     dump = str(tmp_path / "dump.h5")
